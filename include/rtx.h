@@ -1,0 +1,291 @@
+/*
+ * rtx.h — C ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int32 status codes
+ * (0 = ok, negative = error; rtx_last_error() gives the text, thread-local).
+ * Nothing here ever calls exit().  Every entry point that launches GPU work
+ * takes a `stream` (a hipStream_t passed as void*; NULL = the context's own
+ * stream) and never synchronizes unless its comment says "synchronizes".
+ *
+ * Each declaration cites the reference interface it replaces; paths are
+ * relative to the Tesina-RT source tree.
+ */
+#ifndef RTX_H
+#define RTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status ---- */
+enum rtx_status {
+    RTX_OK = 0,
+    RTX_EINVAL = -1,     /* bad argument or handle */
+    RTX_EDEVICE = -2,    /* HIP runtime error */
+    RTX_ENOMEM = -3,     /* device allocation failed or budget exceeded */
+    RTX_ESTATE = -4,     /* call out of order (e.g. trace before build) */
+    RTX_EUNSUPPORTED = -5
+};
+
+/* Pass as a ray count to use the count held on the device by the bundle
+ * (set by rtx_raygen_primary / rtx_raygen_secondary); lets a frame run with
+ * no host read-back, e.g. inside a captured hipGraph. */
+#define RTX_DEVICE_COUNT ((uint64_t)-1)
+
+typedef struct rtx_ctx rtx_ctx;
+typedef struct rtx_scene rtx_scene;
+typedef struct rtx_rays rtx_rays;
+typedef struct rtx_hits rtx_hits;
+typedef struct rtx_fb rtx_fb;
+typedef struct rtx_renderer rtx_renderer;
+
+/* ---------------------------------------------------- plain data types -- */
+
+/* material_cl, 32 bytes (include/rt/material.hpp:57-73). texture < 0 = none. */
+typedef struct rtx_material {
+    float diffuse[4];
+    float shininess;
+    float reflectiveness;
+    float refractive_index;
+    int32_t texture;
+} rtx_material;
+
+/* Light types: SPOT and DIR as include/rt/light.hpp:69-72; POINT is this
+ * build's extension (shadow ray toward pos with tMax = distance). */
+enum { RTX_LIGHT_SPOT = 0, RTX_LIGHT_DIR = 1, RTX_LIGHT_POINT = 2 };
+#define RTX_MAX_LIGHTS 8
+
+typedef struct rtx_light {
+    int32_t type;
+    float pos[3];
+    float dir[3];   /* DIR: direction the light travels (normalized by the caller
+                       as Scene::set_dir_light does, src/rt/scene.cpp:987-1003) */
+    float color[3];
+    float angle;    /* SPOT: cosine cut-off */
+    float radius;
+} rtx_light;
+
+/* lights_cl generalised to several lights (include/rt/light.hpp:86-92). */
+typedef struct rtx_lights {
+    float ambient[3];
+    int32_t count;
+    rtx_light l[RTX_MAX_LIGHTS];
+} rtx_lights;
+
+/* Camera basis as Camera::set produces it (src/rt/camera.cpp:31-59): dir is
+ * unit length, right and up are scaled by tan(FOV/2)*aspect and tan(FOV/2). */
+typedef struct rtx_camera {
+    float pos[3], dir[3], right[3], up[3];
+} rtx_camera;
+
+/* RendererConfig knobs on the LBVH path (include/rt/renderer-config.hpp:7-36). */
+typedef struct rtx_lbvh_params {
+    int32_t depth;     /* bvh_depth: Morton bits, clamped to [1,32] (reference [1,64]) */
+    int32_t min_leaf;  /* bvh_min_leaf_size, clamped to [1,256] */
+} rtx_lbvh_params;
+
+/* Reference BVHNode layout, 48 bytes (include/rt/bvh.hpp:15-74), produced by
+ * rtx_read_nodes_ref for parity checks. */
+typedef struct rtx_node_ref {
+    float hi[4];
+    float lo[4];
+    uint32_t a;        /* l_child or start_index */
+    uint32_t b;        /* r_child or end_index */
+    uint32_t parent;
+    int8_t split_axis; /* never written by the LBVH (0 here) */
+    int8_t leaf;       /* 0 = internal, nonzero = leaf */
+    uint8_t pad[2];
+} rtx_node_ref;
+
+/* FrameStats (include/rt/frame-stats.hpp:5-64) of one frame: ray counts as
+ * Renderer counts them (src/rt/renderer.cpp:137, 184-185) plus shadow rays,
+ * and per-stage device times in ms (filled when profiling is on). */
+enum {
+    RTX_ST_BVH_BUILD = 0, RTX_ST_PRIM_RAY_GEN, RTX_ST_SEC_RAY_GEN, RTX_ST_PRIM_TRACE, RTX_ST_PRIM_SHADOW_TRACE,
+    RTX_ST_SEC_TRACE, RTX_ST_SEC_SHADOW_TRACE, RTX_ST_SHADE, RTX_ST_FB_CLEAR, RTX_ST_FB_COPY, RTX_ST_COUNT
+};
+typedef struct rtx_frame_stats {
+    uint64_t primary_rays;
+    uint64_t secondary_rays;
+    uint64_t shadow_rays;
+    uint32_t bounces_run;
+    uint32_t node_count;
+    double stage_ms[RTX_ST_COUNT];
+    double frame_ms;
+} rtx_frame_stats;
+
+/* Renderer settings (Renderer::configure_from_*, src/rt/renderer.cpp:279-348). */
+typedef struct rtx_render_params {
+    int32_t width, height;
+    int32_t quad_size;      /* prim_ray_quad_size, default 32 */
+    int32_t max_bounces;
+    int32_t depth;          /* bvh_depth */
+    int32_t min_leaf;       /* bvh_min_leaf_size */
+    uint64_t ray_capacity;  /* secondary-ray capacity per bundle; 0 = 2^max_bounces * pixels */
+} rtx_render_params;
+
+/* ------------------------------------------------------------- context -- */
+/* Replaces CLInfo::initialize + DeviceInterface::initialize
+ * (src/cl-gl/opencl-init.cpp:61-145, src/gpu/interface.cpp:27-38). */
+int32_t rtx_init(int32_t device, rtx_ctx** out);
+int32_t rtx_shutdown(rtx_ctx* ctx);
+const char* rtx_last_error(void);
+int32_t rtx_abi_version(void);
+/* hipStream_t of the context (as void*). */
+void* rtx_stream(rtx_ctx* ctx);
+/* DeviceInterface::finish_commands (src/gpu/interface.cpp:273-293); synchronizes. */
+int32_t rtx_sync(rtx_ctx* ctx, void* stream);
+/* Per-kernel timing with hipEvents around every launch (FrameStats timing,
+ * src/rt/renderer.cpp:441-446, without forced per-stage finishes). */
+int32_t rtx_profile_enable(rtx_ctx* ctx, int32_t on);
+/* Synchronizes; writes "name count total_ms\n" lines; returns bytes needed. */
+int64_t rtx_profile_report(rtx_ctx* ctx, char* buf, uint64_t cap);
+int32_t rtx_profile_reset(rtx_ctx* ctx);
+/* Device memory held by the library for this context, bytes. */
+uint64_t rtx_mem_in_use(rtx_ctx* ctx);
+
+/* --------------------------------------------------------------- scene -- */
+/* Scene::initialize / destroy (src/rt/scene.cpp:163-189). */
+int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out);
+int32_t rtx_scene_destroy(rtx_scene* s);
+/* Vertices SoA: pos V*3 floats, nrm V*3 (may be NULL: normals = 0), uv V*2
+ * (may be NULL). Scene::transfer_aggregate_mesh_to_device
+ * (src/rt/scene.cpp:237-321); blocking upload. */
+int32_t rtx_scene_set_vertices(rtx_scene* s, const float* pos, const float* nrm, const float* uv, uint32_t V);
+/* Same from the reference's 80-byte Vertex array (include/rt/mesh.hpp:23-30). */
+int32_t rtx_scene_set_vertices_ref(rtx_scene* s, const void* vertex80, uint32_t V);
+/* Positions only, asynchronous on `stream` (the per-frame vertex update of
+ * Scene::update_aggregate_mesh_vertices, src/rt/scene.cpp:955-985).
+ * host_pos: V*3 floats; must stay valid until the stream reaches the copy. */
+int32_t rtx_scene_update_positions(rtx_scene* s, const float* host_pos, void* stream);
+/* Triangles: idx 3T vertex ids, matmap T material ids (NULL = 0). */
+int32_t rtx_scene_set_triangles(rtx_scene* s, const uint32_t* idx, const int32_t* matmap, uint32_t T);
+int32_t rtx_scene_set_materials(rtx_scene* s, const rtx_material* mats, uint32_t M);
+/* Scene::set_dir_light / set_spot_light / set_ambient_light (src/rt/scene.cpp:987-1040). */
+int32_t rtx_scene_set_lights(rtx_scene* s, const rtx_lights* lights);
+uint32_t rtx_scene_triangle_count(rtx_scene* s);
+
+/* Travelling-wave displacement of a heightfield, on the device: for every
+ * vertex y = sum_k a_k sin(kappa_k (d_k . (x,z)) - omega_k t + phi_k), with
+ * analytic normals.  waves: n*6 floats (a, kappa, dx, dz, omega, phi).
+ * The dynamic-scene pattern of src/rt/rt-wave.cpp:139-270 (mangle kernel,
+ * src/kernel/mangler.cl:13-32). */
+int32_t rtx_scene_displace_waves(rtx_scene* s, const float* waves, int32_t n, float t, void* stream);
+
+/* ----------------------------------------------------------------- LBVH -- */
+/* BVHBuilder::build_lbvh (src/rt/bvh-builder.cpp:201-741): prim AABB ->
+ * global AABB -> Morton -> radix sort -> permute indices/material map in
+ * place -> level-ordered hierarchy -> AABB refit.  Asynchronous. */
+int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream);
+/* BVHBuilder::refit_lbvh (src/rt/bvh-builder.cpp:743-850): AABBs from the
+ * current (already permuted) vertices, topology kept.  Asynchronous. */
+int32_t rtx_lbvh_refit(rtx_scene* s, void* stream);
+/* Node count and per-iteration output counts OC[0..63] (scene.bvh_node_count,
+ * bvh_level_node_count; src/rt/bvh-builder.cpp:614-625, 733-736).  Synchronizes. */
+int32_t rtx_lbvh_info(rtx_scene* s, uint32_t* node_count, uint32_t* level_counts64);
+
+/* Parity read-backs (synchronize).  Arrays are in the current sorted order. */
+int32_t rtx_read_codes(rtx_scene* s, uint32_t* codes, uint32_t T);      /* sorted Morton codes */
+int32_t rtx_read_perm(rtx_scene* s, uint32_t* perm, uint32_t T);        /* sorted pos -> input pos of the last build */
+int32_t rtx_read_indices(rtx_scene* s, uint32_t* idx3, uint32_t T);     /* permuted index buffer */
+int32_t rtx_read_matmap(rtx_scene* s, int32_t* matmap, uint32_t T);     /* permuted material map */
+int32_t rtx_read_orig_ids(rtx_scene* s, uint32_t* orig, uint32_t T);    /* sorted pos -> triangle id at upload */
+int32_t rtx_read_global_bbox(rtx_scene* s, float* lo3, float* hi3);
+/* Nodes in the reference's 48-byte layout, canonical level order. */
+int32_t rtx_read_nodes_ref(rtx_scene* s, rtx_node_ref* out, uint32_t cap, uint32_t* count);
+
+/* ------------------------------------------------------- ray bundles ---- */
+/* RayBundle::initialize / HitBundle::initialize (src/rt/ray.cpp:48-66,
+ * 120-140).  Storage is SoA on the device: ori4 = (origin, tMin),
+ * dir4 = (direction, tMax), pixel, contribution. */
+int32_t rtx_rays_create(rtx_ctx* ctx, uint64_t capacity, rtx_rays** out);
+int32_t rtx_rays_destroy(rtx_rays* r);
+uint64_t rtx_rays_capacity(rtx_rays* r);
+/* Device-held ray count; synchronizes. */
+int32_t rtx_rays_count(rtx_rays* r, uint64_t* count);
+int32_t rtx_rays_read(rtx_rays* r, float* ori4, float* dir4, int32_t* pixel, float* contrib, uint64_t n);
+int32_t rtx_rays_write(rtx_rays* r, const float* ori4, const float* dir4, const int32_t* pixel,
+                       const float* contrib, uint64_t n);
+
+int32_t rtx_hits_create(rtx_ctx* ctx, uint64_t capacity, rtx_hits** out);
+int32_t rtx_hits_destroy(rtx_hits* h);
+/* SampleTraceInfo (src/kernel/trace-bvh.cl:62-74) SoA: flags bit0 hit, bit1
+ * inverse_n, bit 8+l shadowed by light l; t; id (sorted triangle position,
+ * -1 = miss); bary (u, v); n4; p4 (hit point); uv2.  Any pointer may be NULL. */
+int32_t rtx_hits_read(rtx_hits* h, uint32_t* flags, float* t, int32_t* id, float* bary2, float* n4, float* p4,
+                      float* uv2, uint64_t n);
+int32_t rtx_hits_write(rtx_hits* h, const uint32_t* flags, const float* t, const int32_t* id, const float* n4,
+                       const float* p4, uint64_t n);
+
+/* PrimaryRayGenerator::generate (src/rt/primary-ray-generator.cpp:58-165;
+ * kernel src/kernel/primary-ray-generator.cl:28-103): rays for samples
+ * [offset, offset+count) in quad order; psamples = spp triples (ox, oy,
+ * contribution), NULL = one sample {0,0,1}.  Sets the bundle's device count. */
+int32_t rtx_raygen_primary(rtx_rays* rays, const rtx_camera* cam, int32_t width, int32_t height, int32_t quad_size,
+                           int32_t spp, const float* psamples, uint64_t offset, uint64_t count, void* stream);
+
+/* Tracer::trace (src/rt/tracer.cpp:140-269; src/kernel/trace-bvh.cl:190-481):
+ * closest hit + hit completion for `count` rays (or RTX_DEVICE_COUNT). */
+int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+/* Tracer::shadow_trace (src/rt/tracer.cpp:273-352;
+ * src/kernel/shadow-trace-bvh.cl:160-418): one shadow ray per hit and light. */
+int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+
+/* SecondaryRayGenerator::generate, scan variant (src/rt/secondary-ray-generator.cpp:57-145,
+ * 248-340; src/kernel/secondary-ray-generator.cl:43-181): Whitted children in
+ * parent order, refraction before reflection; a parent whose children would
+ * reach rays_out's capacity emits none.  The output count stays on the
+ * device; if count_out is not NULL it is also read back (synchronizes), as
+ * the reference does. */
+int32_t rtx_raygen_secondary(rtx_scene* s, rtx_rays* rays_in, rtx_hits* hits, rtx_rays* rays_out, uint64_t count_in,
+                             uint64_t* count_out, void* stream);
+
+/* ---------------------------------------------------------- framebuffer -- */
+/* FrameBuffer::initialize / clear / copy (src/rt/framebuffer.cpp:118-165;
+ * src/kernel/framebuffer.cl:21-54): uint32 r,g,b accumulators x1e6 and an
+ * RGBA8 image. */
+int32_t rtx_fb_create(rtx_ctx* ctx, int32_t width, int32_t height, rtx_fb** out);
+int32_t rtx_fb_destroy(rtx_fb* fb);
+int32_t rtx_fb_clear(rtx_fb* fb, void* stream);
+int32_t rtx_fb_resolve(rtx_fb* fb, void* stream);
+int32_t rtx_fb_read_accum(rtx_fb* fb, uint32_t* rgb3);
+int32_t rtx_fb_read_rgba8(rtx_fb* fb, uint8_t* rgba);
+
+/* RayShader::shade (src/rt/ray-shader.cpp:33-154; src/kernel/ray-shader.cl:116-459):
+ * ambient + Lambert cos^2 + Phong^8 per unshadowed light, times
+ * (1 - reflectiveness), floor(clamp(c*val)*1e6) added to the pixel with
+ * integer atomics (order-free).  `primary` is accepted for API parity. */
+int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint64_t count, int32_t primary,
+                  void* stream);
+
+/* ------------------------------------------------------------- renderer -- */
+/* Renderer (src/rt/renderer.cpp:12-272): owns two ray bundles, a hit bundle
+ * and a framebuffer sized from params. */
+int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_renderer** out);
+int32_t rtx_renderer_destroy(rtx_renderer* r);
+rtx_fb* rtx_renderer_fb(rtx_renderer* r);
+/* One frame, plain form: set_up_frame (clear + build if `rebuild`) +
+ * render_to_framebuffer + copy_framebuffer, one launch per stage and the
+ * reference's per-bounce count read-back.  Synchronizes at the end; fills
+ * stats if not NULL. */
+int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild,
+                         rtx_frame_stats* stats, void* stream);
+/* One frame with no host read-back (device-held counts), asynchronous; the
+ * form that rtx_render_graph captures. */
+int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild,
+                               void* stream);
+/* Ray counts of the last frame (synchronizes). */
+int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats);
+
+/* --------------------------------------------------------- host helpers -- */
+/* Camera::set (src/rt/camera.cpp:31-59); host only. */
+int32_t rtx_camera_set(rtx_camera* cam, const float pos[3], const float dir[3], const float up[3], float fov,
+                       float aspect, int32_t right_handed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTX_H */

@@ -1,0 +1,245 @@
+"""Python wrapper of the rt-seq oracle (liboracle.so, built from rtseq.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+NODE_DTYPE = np.dtype([("hi", "<f4", 4), ("lo", "<f4", 4), ("a", "<u4"), ("b", "<u4"), ("parent", "<u4"),
+                       ("split_axis", "i1"), ("leaf", "i1"), ("pad", "u1", 2)])
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3), ("color", "<f4", 3),
+                        ("angle", "<f4"), ("radius", "<f4")])
+LIGHTS_DTYPE = np.dtype([("ambient", "<f4", 3), ("count", "<i4"), ("l", LIGHT_DTYPE, 8)])
+CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("right", "<f4", 3), ("up", "<f4", 3)])
+STATS_DTYPE = np.dtype([("primary", "<u8"), ("secondary", "<u8"), ("shadow", "<u8"), ("bounces_run", "<u8"),
+                        ("per_bounce", "<u8", 16)])
+
+SPOT, DIR, POINT = 0, 1, 2
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.orc_morton_one.restype = C.c_uint64
+        _lib.orc_raygen_secondary.restype = C.c_uint64
+        _lib.orc_render.restype = C.c_uint64
+        _lib.orc_hit_t_f64.restype = C.c_double
+    return _lib
+
+
+def P(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def pos4(pos3: np.ndarray) -> np.ndarray:
+    p = np.zeros((pos3.shape[0], 4), np.float32)
+    p[:, :3] = pos3
+    return p
+
+
+def lights_from(rtx_lights) -> np.ndarray:
+    """Convert an rtx.Lights ctypes struct (same layout) to the oracle's array."""
+    a = np.frombuffer(bytes(rtx_lights), dtype=LIGHTS_DTYPE).copy()
+    return a
+
+
+def camera_from(rtx_cam) -> np.ndarray:
+    return np.frombuffer(bytes(rtx_cam), dtype=CAMERA_DTYPE).copy()
+
+
+def global_bbox(pos3, idx):
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    p4 = pos4(pos3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    lib().orc_global_bbox(P(p4), P(i), C.c_uint32(i.size // 3), P(lo), P(hi))
+    return lo, hi
+
+
+def morton_one(lo, hi, glo, ghi, depth: int) -> int:
+    f = [np.ascontiguousarray(x, np.float32) for x in (lo, hi, glo, ghi)]
+    return int(lib().orc_morton_one(P(f[0]), P(f[1]), P(f[2]), P(f[3]), C.c_int(depth)))
+
+
+def morton(pos3, idx, depth: int) -> np.ndarray:
+    p4 = pos4(pos3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    T = i.size // 3
+    out = np.zeros(T, np.uint64)
+    lib().orc_morton(P(p4), P(i), C.c_uint32(T), C.c_int(depth), P(out))
+    return out
+
+
+def sort_perm(codes: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(codes, np.uint64)
+    out = np.zeros(c.size, np.uint32)
+    lib().orc_sort(P(c), C.c_uint32(c.size), P(out))
+    return out
+
+
+def bitonic_perm(codes: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(codes, np.uint64)
+    out = np.zeros(c.size, np.uint32)
+    lib().orc_bitonic_emulate(P(c), C.c_uint32(c.size), P(out))
+    return out
+
+
+class Build:
+    """Result of orc_lbvh_build; idx/matmap are the permuted copies."""
+
+    def __init__(self, pos3, idx, matmap=None, depth=30, min_leaf=1):
+        self.pos4 = pos4(np.asarray(pos3, np.float32))
+        self.idx = np.ascontiguousarray(np.asarray(idx, np.uint32).reshape(-1)).copy()
+        T = self.idx.size // 3
+        self.T = T
+        self.matmap = (np.zeros(T, np.int32) if matmap is None else np.ascontiguousarray(matmap, np.int32).copy())
+        self.codes = np.zeros(T, np.uint64)
+        self.perm = np.zeros(T, np.uint32)
+        self.nodes = np.zeros(2 * T + 1, NODE_DTYPE)
+        n = C.c_uint32()
+        self.level_counts = np.zeros(64, np.uint32)
+        self.gbox = np.zeros(6, np.float32)
+        rc = lib().orc_lbvh_build(P(self.pos4), P(self.idx), P(self.matmap), C.c_uint32(T), C.c_int(depth),
+                                  C.c_int(min_leaf), P(self.codes), P(self.perm), P(self.nodes),
+                                  C.c_uint32(self.nodes.size), C.byref(n), P(self.level_counts), P(self.gbox))
+        if rc != 0:
+            raise RuntimeError(f"orc_lbvh_build failed ({rc})")
+        self.nodes = self.nodes[: n.value].copy()
+        self.node_count = int(n.value)
+        self.depth, self.min_leaf = depth, min_leaf
+
+    def refit(self, pos3):
+        self.pos4 = pos4(np.asarray(pos3, np.float32))
+        rc = lib().orc_lbvh_refit(P(self.pos4), P(self.idx), C.c_uint32(self.T), P(self.nodes),
+                                  C.c_uint32(self.node_count), C.c_int(self.depth), P(self.level_counts))
+        if rc != 0:
+            raise RuntimeError("orc_lbvh_refit failed")
+
+
+def raygen_primary(cam: np.ndarray, w: int, h: int, quad: int = 32, psamples=None, offset: int = 0, count=None):
+    ps = np.ascontiguousarray(np.array([0.0, 0.0, 1.0], np.float32) if psamples is None else psamples, np.float32)
+    count = w * h - offset if count is None else count
+    o = np.zeros((count, 4), np.float32)
+    d = np.zeros((count, 4), np.float32)
+    px = np.zeros(count, np.int32)
+    c = np.zeros(count, np.float32)
+    lib().orc_raygen_primary(P(cam), C.c_int(w), C.c_int(h), C.c_int(quad), P(ps), C.c_uint64(offset),
+                             C.c_uint64(count), P(o), P(d), P(px), P(c))
+    return o, d, px, c
+
+
+def empty_hits(n):
+    return dict(flags=np.zeros(n, np.uint32), t=np.zeros(n, np.float32), id=np.zeros(n, np.int32),
+                bary=np.zeros((n, 2), np.float32), n4=np.zeros((n, 4), np.float32), p4=np.zeros((n, 4), np.float32),
+                uv2=np.zeros((n, 2), np.float32))
+
+
+def trace(b: Build, nrm3, uv2, ori4, dir4, count_visits=False):
+    n = ori4.shape[0]
+    h = empty_hits(n)
+    nrm4 = pos4(np.asarray(nrm3, np.float32))
+    uv = np.ascontiguousarray(uv2, np.float32)
+    visits = np.zeros(1, np.uint64)
+    lib().orc_trace(P(b.pos4), P(nrm4), P(uv), P(b.idx), P(b.nodes), P(np.ascontiguousarray(ori4, np.float32)),
+                    P(np.ascontiguousarray(dir4, np.float32)), C.c_uint64(n), P(h["flags"]), P(h["t"]), P(h["id"]),
+                    P(h["bary"]), P(h["n4"]), P(h["p4"]), P(h["uv2"]), P(visits) if count_visits else None)
+    if count_visits:
+        h["visits"] = int(visits[0])
+    return h
+
+
+def trace_brute(b: Build, ori4, dir4):
+    n = ori4.shape[0]
+    ids = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    lib().orc_trace_brute(P(b.pos4), P(b.idx), C.c_uint32(b.T), P(np.ascontiguousarray(ori4, np.float32)),
+                          P(np.ascontiguousarray(dir4, np.float32)), C.c_uint64(n), P(ids), P(t))
+    return ids, t
+
+
+def trace_shadow(b: Build, lights: np.ndarray, flags, p4):
+    f = np.ascontiguousarray(flags, np.uint32).copy()
+    lib().orc_trace_shadow(P(b.pos4), P(b.idx), P(b.nodes), P(lights), C.c_uint64(f.size), P(f),
+                           P(np.ascontiguousarray(p4, np.float32)))
+    return f
+
+
+def trace_shadow_brute(b: Build, lights: np.ndarray, flags, p4):
+    f = np.ascontiguousarray(flags, np.uint32).copy()
+    lib().orc_trace_shadow_brute(P(b.pos4), P(b.idx), C.c_uint32(b.T), P(lights), C.c_uint64(f.size), P(f),
+                                 P(np.ascontiguousarray(p4, np.float32)))
+    return f
+
+
+def raygen_secondary(mats: np.ndarray, matmap, ori4, dir4, pixel, contrib, hits: dict, cap: int):
+    n = ori4.shape[0]
+    o = np.zeros((max(cap, 1), 4), np.float32)
+    d = np.zeros((max(cap, 1), 4), np.float32)
+    px = np.zeros(max(cap, 1), np.int32)
+    c = np.zeros(max(cap, 1), np.float32)
+    total = np.zeros(1, np.uint64)
+    m = lib().orc_raygen_secondary(P(mats), P(np.ascontiguousarray(matmap, np.int32)),
+                                   P(np.ascontiguousarray(ori4, np.float32)), P(np.ascontiguousarray(dir4, np.float32)),
+                                   P(np.ascontiguousarray(pixel, np.int32)), P(np.ascontiguousarray(contrib, np.float32)),
+                                   P(hits["flags"]), P(hits["id"]), P(hits["n4"]), P(hits["p4"]), C.c_uint64(n),
+                                   C.c_uint64(cap), P(o), P(d), P(px), P(c), P(total))
+    m = int(m)
+    return o[:m], d[:m], px[:m], c[:m], int(total[0])
+
+
+def shade(mats, matmap, lights, dir4, pixel, contrib, hits, accum):
+    n = dir4.shape[0]
+    lib().orc_shade(P(mats), P(np.ascontiguousarray(matmap, np.int32)), P(lights),
+                    P(np.ascontiguousarray(dir4, np.float32)), P(np.ascontiguousarray(pixel, np.int32)),
+                    P(np.ascontiguousarray(contrib, np.float32)), P(hits["flags"]), P(hits["id"]), P(hits["n4"]),
+                    P(hits["p4"]), C.c_uint64(n), P(accum))
+    return accum
+
+
+def resolve(accum: np.ndarray) -> np.ndarray:
+    out = np.zeros((accum.size // 3, 4), np.uint8)
+    lib().orc_resolve(P(np.ascontiguousarray(accum, np.uint32)), C.c_uint64(accum.size // 3), P(out))
+    return out
+
+
+def render(b: Build, nrm3, uv2, mats, lights, cam, w, h, max_bounces, quad=32, cap=0):
+    accum = np.zeros((w * h, 3), np.uint32)
+    st = np.zeros(1, STATS_DTYPE)
+    nrm4 = pos4(np.asarray(nrm3, np.float32))
+    uv = np.ascontiguousarray(uv2, np.float32)
+    cap = cap or (w * h) << max_bounces
+    rays = lib().orc_render(P(b.pos4), P(nrm4), P(uv), P(b.idx), P(b.matmap), P(b.nodes), P(mats), P(lights), P(cam),
+                            C.c_int(w), C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap), P(accum),
+                            P(st))
+    return accum, st[0], int(rays)
+
+
+def hit_t_f64(b: Build, ori4, dir4, ids, t32):
+    n = ori4.shape[0]
+    t64 = np.zeros(n, np.float64)
+    worst = lib().orc_hit_t_f64(P(b.pos4), P(b.idx), P(np.ascontiguousarray(ori4, np.float32)),
+                                P(np.ascontiguousarray(dir4, np.float32)), P(np.ascontiguousarray(ids, np.int32)),
+                                P(np.ascontiguousarray(t32, np.float32)), C.c_uint64(n), P(t64))
+    return float(worst), t64

@@ -1,0 +1,441 @@
+// Context, errors, device buffers, profiling, scene uploads and read-backs.
+//
+// Replaces CLInfo / DeviceInterface / DeviceMemory (src/cl-gl/opencl-init.cpp,
+// src/gpu/interface.cpp, src/gpu/memory.cpp) and the Scene device-buffer
+// management of src/rt/scene.cpp.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "devmath.hpp"
+#include "rtx_impl.hpp"
+
+// ------------------------------------------------------------- errors ----
+static thread_local std::string g_err;
+void rtx_set_error(const std::string& msg) { g_err = msg; }
+const char* rtx_last_error(void) { return g_err.c_str(); }
+int32_t rtx_abi_version(void) { return 1; }
+
+// ------------------------------------------------------------- buffers ---
+int32_t DevBuf::reserve(rtx_ctx* c, size_t n) {
+    if (n <= bytes && p) return RTX_OK;
+    release();
+    ctx = c;
+    size_t want = n ? n : 16;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        rtx_set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+        (void)hipGetLastError();
+        return RTX_ENOMEM;
+    }
+    bytes = want;
+    if (ctx) ctx->bytes_in_use += want;
+    return RTX_OK;
+}
+
+void DevBuf::release() {
+    if (p) {
+        (void)hipFree(p);
+        if (ctx) ctx->bytes_in_use -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+}
+
+namespace {
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Vertex upload: positions to float4, normals normalized once
+// (complete_trace_info normalizes each vertex normal at every hit,
+// src/kernel/trace-bvh.cl:222-224; same operation, done here once).
+__global__ void k_vertices(const float* __restrict__ pos, const float* __restrict__ nrm, const float* __restrict__ uv,
+                           float4* __restrict__ pos4, float4* __restrict__ nrm4, float2* __restrict__ uv2, uint32_t V,
+                           uint32_t stride_f) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float* p = pos + (size_t)i * stride_f;
+    pos4[i] = make_float4(p[0], p[1], p[2], 0.f);
+    if (nrm4) {
+        v3 n = mk3(0.f, 0.f, 0.f);
+        if (nrm) {
+            const float* q = nrm + (size_t)i * stride_f;
+            n = normalize(mk3(q[0], q[1], q[2]));
+        }
+        nrm4[i] = make_float4(n.x, n.y, n.z, 0.f);
+    }
+    if (uv2) {
+        if (uv) {
+            const float* t = uv + (size_t)i * (stride_f == 3 ? 2 : stride_f);
+            uv2[i] = make_float2(t[0], t[1]);
+        } else {
+            uv2[i] = make_float2(0.f, 0.f);
+        }
+    }
+}
+
+// Heightfield displacement: y = sum_k a sin(kappa (d . (x,z)) - omega t + phi),
+// normal = normalize(-dy/dx, 1, -dy/dz).  Reads x,z from pos4, writes y and
+// the unit normal.
+__global__ void k_waves(float4* __restrict__ pos4, float4* __restrict__ nrm4, const float* __restrict__ waves, int nw,
+                        float t, uint32_t V) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    float4 p = pos4[i];
+    float y = 0.f, gx = 0.f, gz = 0.f;
+    for (int k = 0; k < nw; ++k) {
+        const float* w = waves + 6 * k;
+        float ph = w[1] * (w[2] * p.x + w[3] * p.z) - w[4] * t + w[5];
+        float s, c;
+        sincosf(ph, &s, &c);
+        y += w[0] * s;
+        gx += w[0] * c * w[1] * w[2];
+        gz += w[0] * c * w[1] * w[3];
+    }
+    pos4[i] = make_float4(p.x, y, p.z, 0.f);
+    v3 n = normalize(mk3(-gx, 1.f, -gz));
+    nrm4[i] = make_float4(n.x, n.y, n.z, 0.f);
+}
+}  // namespace
+
+int32_t rtx_fill_u32(rtx_ctx* ctx, void* p, uint32_t value, uint64_t n, hipStream_t st) {
+    if (!n) return RTX_OK;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    ProfScope ps(ctx, st, "fill");
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, (uint32_t*)p, value, n);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+// ------------------------------------------------------------ profiling --
+ProfScope::ProfScope(rtx_ctx* c, hipStream_t s, const char* name) : ctx(c), st(s) {
+    if (!ctx || !ctx->profiling) return;
+    hipEvent_t a, b;
+    if (ctx->event_pool.size() >= 2) {
+        a = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        b = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        if (hipEventCreate(&a) != hipSuccess) return;
+        if (hipEventCreate(&b) != hipSuccess) return;
+    }
+    (void)hipEventRecord(a, st);
+    ctx->pending.push_back(KernelTiming{name, a, b});
+    slot = (int)ctx->pending.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+    if (slot >= 0) (void)hipEventRecord(ctx->pending[slot].stop, st);
+}
+
+static void drain_profile(rtx_ctx* c) {
+    for (auto& k : c->pending) {
+        (void)hipEventSynchronize(k.stop);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, k.start, k.stop);
+        bool found = false;
+        for (auto& a : c->acc)
+            if (a.name == k.name) {
+                a.count++;
+                a.ms += ms;
+                found = true;
+                break;
+            }
+        if (!found) c->acc.push_back(rtx_ctx::Acc{k.name, 1, (double)ms});
+        c->event_pool.push_back(k.start);
+        c->event_pool.push_back(k.stop);
+    }
+    c->pending.clear();
+}
+
+int32_t rtx_profile_enable(rtx_ctx* ctx, int32_t on) {
+    RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_profile_enable: null context");
+    ctx->profiling = on != 0;
+    return RTX_OK;
+}
+
+int32_t rtx_profile_reset(rtx_ctx* ctx) {
+    RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_profile_reset: null context");
+    drain_profile(ctx);
+    ctx->acc.clear();
+    return RTX_OK;
+}
+
+int64_t rtx_profile_report(rtx_ctx* ctx, char* buf, uint64_t cap) {
+    RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_profile_report: null context");
+    drain_profile(ctx);
+    std::string out;
+    char line[256];
+    for (auto& a : ctx->acc) {
+        snprintf(line, sizeof line, "%s %llu %.6f\n", a.name.c_str(), (unsigned long long)a.count, a.ms);
+        out += line;
+    }
+    if (buf && cap) {
+        size_t n = out.size() < cap - 1 ? out.size() : (size_t)cap - 1;
+        memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
+    return (int64_t)out.size() + 1;
+}
+
+// -------------------------------------------------------------- context --
+int32_t rtx_init(int32_t device, rtx_ctx** out) {
+    RTX_REQUIRE(out, RTX_EINVAL, "rtx_init: null out");
+    *out = nullptr;
+    int n = 0;
+    RTX_CHECK_HIP(hipGetDeviceCount(&n));
+    RTX_REQUIRE(device >= 0 && device < n, RTX_EINVAL, "rtx_init: no such device");
+    RTX_CHECK_HIP(hipSetDevice(device));
+    rtx_ctx* c = new rtx_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        rtx_set_error("rtx_init: hipStreamCreate failed");
+        return RTX_EDEVICE;
+    }
+    *out = c;
+    return RTX_OK;
+}
+
+int32_t rtx_shutdown(rtx_ctx* ctx) {
+    RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_shutdown: null context");
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_profile(ctx);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RTX_OK;
+}
+
+void* rtx_stream(rtx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int32_t rtx_sync(rtx_ctx* ctx, void* stream) {
+    RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_sync: null context");
+    RTX_CHECK_HIP(hipStreamSynchronize(ctx->pick(stream)));
+    return RTX_OK;
+}
+
+uint64_t rtx_mem_in_use(rtx_ctx* ctx) { return ctx ? ctx->bytes_in_use : 0; }
+
+// ---------------------------------------------------------------- scene --
+int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out) {
+    RTX_REQUIRE(ctx && out, RTX_EINVAL, "rtx_scene_create: null argument");
+    rtx_scene* s = new rtx_scene();
+    s->ctx = ctx;
+    *out = s;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_destroy(rtx_scene* s) {
+    RTX_REQUIRE(s, RTX_EINVAL, "rtx_scene_destroy: null scene");
+    (void)hipStreamSynchronize(s->ctx->stream);
+    DevBuf* all[] = {&s->pos,      &s->nrm,        &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
+                     &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
+                     &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
+                     &s->levels,   &s->refit_cnt,  &s->bstart,    &s->bend,        &s->blevel,    &s->keys_alt,
+                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets,
+                     &s->hier_status, &s->waves,      &s->stage};
+    for (DevBuf* b : all) b->release();
+    delete s;
+    return RTX_OK;
+}
+
+static int32_t upload_vertices(rtx_scene* s, const float* pos, const float* nrm, const float* uv, uint32_t V,
+                               uint32_t stride_f) {
+    rtx_ctx* c = s->ctx;
+    int32_t e;
+    if ((e = s->pos.reserve(c, (size_t)V * 16))) return e;
+    if ((e = s->nrm.reserve(c, (size_t)V * 16))) return e;
+    if ((e = s->uv.reserve(c, (size_t)V * 8))) return e;
+    // staging: raw host arrays copied as-is, converted on the device
+    size_t fpv = stride_f;
+    DevBuf sp, sn, su;
+    if ((e = sp.reserve(nullptr, (size_t)V * fpv * 4))) return e;
+    RTX_CHECK_HIP(hipMemcpy(sp.p, pos, (size_t)V * fpv * 4, hipMemcpyHostToDevice));
+    if (nrm) {
+        if ((e = sn.reserve(nullptr, (size_t)V * fpv * 4))) return e;
+        RTX_CHECK_HIP(hipMemcpy(sn.p, nrm, (size_t)V * fpv * 4, hipMemcpyHostToDevice));
+    }
+    size_t uvf = stride_f == 3 ? 2 : stride_f;
+    if (uv) {
+        if ((e = su.reserve(nullptr, (size_t)V * uvf * 4))) return e;
+        RTX_CHECK_HIP(hipMemcpy(su.p, uv, (size_t)V * uvf * 4, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_vertices, dim3(rtx_div_up(V, 256)), dim3(256), 0, c->stream, sp.as<float>(),
+                       nrm ? sn.as<float>() : nullptr, uv ? su.as<float>() : nullptr, s->pos.as<float4>(),
+                       s->nrm.as<float4>(), s->uv.as<float2>(), V, stride_f);
+    RTX_CHECK_HIP(hipGetLastError());
+    RTX_CHECK_HIP(hipStreamSynchronize(c->stream));
+    sp.release();
+    sn.release();
+    su.release();
+    s->V = V;
+    s->built = false;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_vertices(rtx_scene* s, const float* pos, const float* nrm, const float* uv, uint32_t V) {
+    RTX_REQUIRE(s && pos && V, RTX_EINVAL, "rtx_scene_set_vertices: null argument or V == 0");
+    return upload_vertices(s, pos, nrm, uv, V, 3);
+}
+
+int32_t rtx_scene_set_vertices_ref(rtx_scene* s, const void* vertex80, uint32_t V) {
+    RTX_REQUIRE(s && vertex80 && V, RTX_EINVAL, "rtx_scene_set_vertices_ref: null argument or V == 0");
+    // Vertex: position@0, normal@16, tangent@32, bitangent@48, texCoord@64 (80 B).
+    const float* f = static_cast<const float*>(vertex80);
+    std::vector<float> p(3 * (size_t)V), n(3 * (size_t)V), t(2 * (size_t)V);
+    for (uint32_t i = 0; i < V; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            p[3 * (size_t)i + k] = f[20 * (size_t)i + k];
+            n[3 * (size_t)i + k] = f[20 * (size_t)i + 4 + k];
+        }
+        t[2 * (size_t)i] = f[20 * (size_t)i + 16];
+        t[2 * (size_t)i + 1] = f[20 * (size_t)i + 17];
+    }
+    return upload_vertices(s, p.data(), n.data(), t.data(), V, 3);
+}
+
+int32_t rtx_scene_update_positions(rtx_scene* s, const float* host_pos, void* stream) {
+    RTX_REQUIRE(s && host_pos && s->V, RTX_EINVAL, "rtx_scene_update_positions: no vertices");
+    hipStream_t st = s->ctx->pick(stream);
+    int32_t e = s->stage.reserve(s->ctx, (size_t)s->V * 12);
+    if (e) return e;
+    RTX_CHECK_HIP(hipMemcpyAsync(s->stage.p, host_pos, (size_t)s->V * 12, hipMemcpyHostToDevice, st));
+    ProfScope ps(s->ctx, st, "vertex_update");
+    hipLaunchKernelGGL(k_vertices, dim3(rtx_div_up(s->V, 256)), dim3(256), 0, st, s->stage.as<float>(),
+                       (const float*)nullptr, (const float*)nullptr, s->pos.as<float4>(), (float4*)nullptr,
+                       (float2*)nullptr, s->V, 3u);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_scene_displace_waves(rtx_scene* s, const float* waves, int32_t n, float t, void* stream) {
+    RTX_REQUIRE(s && s->V && waves && n > 0 && n <= 64, RTX_EINVAL, "rtx_scene_displace_waves: bad argument");
+    hipStream_t st = s->ctx->pick(stream);
+    int32_t e = s->waves.reserve(s->ctx, 64 * 6 * 4);
+    if (e) return e;
+    float* dev_w = s->waves.as<float>();
+    RTX_CHECK_HIP(hipMemcpyAsync(dev_w, waves, (size_t)n * 6 * 4, hipMemcpyHostToDevice, st));
+    ProfScope ps(s->ctx, st, "waves");
+    hipLaunchKernelGGL(k_waves, dim3(rtx_div_up(s->V, 256)), dim3(256), 0, st, s->pos.as<float4>(),
+                       s->nrm.as<float4>(), dev_w, n, t, s->V);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_triangles(rtx_scene* s, const uint32_t* idx, const int32_t* matmap, uint32_t T) {
+    RTX_REQUIRE(s && idx && T, RTX_EINVAL, "rtx_scene_set_triangles: null argument or T == 0");
+    RTX_REQUIRE(s->V, RTX_ESTATE, "rtx_scene_set_triangles: set vertices first");
+    for (uint64_t i = 0; i < 3ull * T; ++i)
+        RTX_REQUIRE(idx[i] < s->V, RTX_EINVAL, "rtx_scene_set_triangles: vertex index out of range");
+    rtx_ctx* c = s->ctx;
+    int32_t e;
+    if ((e = s->idx.reserve(c, (size_t)T * 12))) return e;
+    if ((e = s->idx_alt.reserve(c, (size_t)T * 12))) return e;
+    if ((e = s->matmap.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->matmap_alt.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->orig.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->orig_alt.reserve(c, (size_t)T * 4))) return e;
+    RTX_CHECK_HIP(hipMemcpy(s->idx.p, idx, (size_t)T * 12, hipMemcpyHostToDevice));
+    std::vector<int32_t> mm(T, 0);
+    if (matmap) memcpy(mm.data(), matmap, (size_t)T * 4);
+    RTX_CHECK_HIP(hipMemcpy(s->matmap.p, mm.data(), (size_t)T * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> o(T);
+    for (uint32_t i = 0; i < T; ++i) o[i] = i;
+    RTX_CHECK_HIP(hipMemcpy(s->orig.p, o.data(), (size_t)T * 4, hipMemcpyHostToDevice));
+    s->T = T;
+    s->built = false;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_materials(rtx_scene* s, const rtx_material* mats, uint32_t M) {
+    RTX_REQUIRE(s && mats && M, RTX_EINVAL, "rtx_scene_set_materials: null argument or M == 0");
+    int32_t e = s->mats.reserve(s->ctx, (size_t)M * sizeof(rtx_material));
+    if (e) return e;
+    RTX_CHECK_HIP(hipMemcpy(s->mats.p, mats, (size_t)M * sizeof(rtx_material), hipMemcpyHostToDevice));
+    s->M = M;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_lights(rtx_scene* s, const rtx_lights* lights) {
+    RTX_REQUIRE(s && lights, RTX_EINVAL, "rtx_scene_set_lights: null argument");
+    RTX_REQUIRE(lights->count >= 0 && lights->count <= RTX_MAX_LIGHTS, RTX_EINVAL,
+                "rtx_scene_set_lights: count out of range");
+    int32_t e = s->lights.reserve(s->ctx, sizeof(rtx_lights));
+    if (e) return e;
+    RTX_CHECK_HIP(hipMemcpy(s->lights.p, lights, sizeof(rtx_lights), hipMemcpyHostToDevice));
+    s->host_lights = *lights;
+    return RTX_OK;
+}
+
+uint32_t rtx_scene_triangle_count(rtx_scene* s) { return s ? s->T : 0; }
+
+// ----------------------------------------------------------- read-backs --
+static int32_t readback(rtx_scene* s, const DevBuf& b, void* dst, size_t bytes, const char* what) {
+    RTX_REQUIRE(s && dst, RTX_EINVAL, std::string(what) + ": null argument");
+    RTX_REQUIRE(b.p && b.bytes >= bytes, RTX_ESTATE, std::string(what) + ": buffer not available");
+    RTX_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(dst, b.p, bytes, hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+int32_t rtx_read_codes(rtx_scene* s, uint32_t* codes, uint32_t T) {
+    RTX_REQUIRE(s && s->built && T == s->T, RTX_EINVAL, "rtx_read_codes: no BVH or wrong T");
+    return readback(s, s->codes, codes, (size_t)T * 4, "rtx_read_codes");
+}
+int32_t rtx_read_perm(rtx_scene* s, uint32_t* perm, uint32_t T) {
+    RTX_REQUIRE(s && s->built && T == s->T, RTX_EINVAL, "rtx_read_perm: no BVH or wrong T");
+    return readback(s, s->perm, perm, (size_t)T * 4, "rtx_read_perm");
+}
+int32_t rtx_read_indices(rtx_scene* s, uint32_t* idx3, uint32_t T) {
+    RTX_REQUIRE(s && T == s->T, RTX_EINVAL, "rtx_read_indices: wrong T");
+    return readback(s, s->idx, idx3, (size_t)T * 12, "rtx_read_indices");
+}
+int32_t rtx_read_matmap(rtx_scene* s, int32_t* matmap, uint32_t T) {
+    RTX_REQUIRE(s && T == s->T, RTX_EINVAL, "rtx_read_matmap: wrong T");
+    return readback(s, s->matmap, matmap, (size_t)T * 4, "rtx_read_matmap");
+}
+int32_t rtx_read_orig_ids(rtx_scene* s, uint32_t* orig, uint32_t T) {
+    RTX_REQUIRE(s && T == s->T, RTX_EINVAL, "rtx_read_orig_ids: wrong T");
+    return readback(s, s->orig, orig, (size_t)T * 4, "rtx_read_orig_ids");
+}
+int32_t rtx_read_global_bbox(rtx_scene* s, float* lo3, float* hi3) {
+    RTX_REQUIRE(s && s->built && lo3 && hi3, RTX_EINVAL, "rtx_read_global_bbox: no BVH");
+    float g[6];
+    int32_t e = readback(s, s->gbox, g, sizeof g, "rtx_read_global_bbox");
+    if (e) return e;
+    memcpy(lo3, g, 12);
+    memcpy(hi3, g + 3, 12);
+    return RTX_OK;
+}
+
+// ------------------------------------------------------------ host math --
+int32_t rtx_camera_set(rtx_camera* cam, const float pos[3], const float dir[3], const float up[3], float fov,
+                       float aspect, int32_t right_handed) {
+    RTX_REQUIRE(cam && pos && dir && up, RTX_EINVAL, "rtx_camera_set: null argument");
+    v3 d = mk3(dir[0], dir[1], dir[2]);
+    float dn = std::sqrt(dot(d, d));
+    d = d * (1.0f / dn);
+    v3 U = mk3(up[0], up[1], up[2]);
+    v3 right = right_handed ? cross(d, U) : cross(U, d);
+    v3 upv = right_handed ? cross(right, d) : cross(d, right);
+    float plane_h = (float)std::tan((double)(fov * 0.5f));
+    float rn = std::sqrt(dot(right, right));
+    float un = std::sqrt(dot(upv, upv));
+    right = right * ((plane_h * aspect) / rn);
+    upv = upv * (plane_h / un);
+    for (int k = 0; k < 3; ++k) cam->pos[k] = pos[k];
+    cam->dir[0] = d.x; cam->dir[1] = d.y; cam->dir[2] = d.z;
+    cam->right[0] = right.x; cam->right[1] = right.y; cam->right[2] = right.z;
+    cam->up[0] = upv.x; cam->up[1] = upv.y; cam->up[2] = upv.z;
+    return RTX_OK;
+}

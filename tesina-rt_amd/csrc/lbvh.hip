@@ -1,0 +1,708 @@
+// LBVH build on gfx950: prim AABB + global AABB, Morton codes, onesweep
+// radix sort, permutation of the scene buffers, level-ordered hierarchy
+// emission, bottom-up AABB refit.
+//
+// Reference: BVHBuilder::build_lbvh / refit_lbvh (src/rt/bvh-builder.cpp:201-850)
+// with the kernels of src/kernel/bvh-builder.cl and bvh-builder-sort.cl.
+//
+// Result contract (checked against the oracle): identical Morton codes,
+// the permutation of a stable sort by (code, input position), nodes in
+// canonical level order (root, then each iteration's children block ordered
+// by parent position, left before right — what process_task produces when
+// its work-groups run in order), identical node AABBs.
+#include <cstring>
+
+#include "devmath.hpp"
+#include "rtx_impl.hpp"
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = BS * SORT_ITEMS;
+constexpr int BOUNDS_GRID = 512;
+
+// Counter slots in scene->tickets.
+constexpr int TK_BOUNDS = 0;
+constexpr int TK_SORT = 1;    // 4 passes
+constexpr int TK_LEVEL = 8;   // up to 64 levels
+constexpr int TK_COUNT = 80;
+
+// Layout of scene->levels (uint32): [d] = first node of level d,
+// [64 + d] = node count of level d, [128] = total node count.
+constexpr int LV_OFF = 0, LV_CNT = 64, LV_TOTAL = 128, LV_WORDS = 192;
+
+// ------------------------------------------------------------- helpers --
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+    for (int off = 1; off < RTX_WAVE; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, RTX_WAVE);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, uint32_t* total) {
+    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(x, lane);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BS / RTX_WAVE; ++i) {
+        uint32_t v = s_w[i];
+        base += (i < (int)w) ? v : 0u;
+        tot += v;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - x;
+}
+
+// Decoupled look-back over tiles acquired in order: publishes this tile's
+// aggregate, accumulates the predecessors, publishes the inclusive prefix.
+// Returns the exclusive prefix of `tile`.
+// Word of tile t lives at status[t * stride].
+__device__ uint32_t lookback(uint32_t* status, uint32_t tile, uint32_t agg, uint32_t stride) {
+    if (tile == 0) {
+        st_release(&status[0], RTX_LB_INC | agg);
+        return 0;
+    }
+    st_release(&status[(size_t)tile * stride], RTX_LB_AGG | agg);
+    uint32_t acc = 0;
+    int k = (int)tile - 1;
+    for (;;) {
+        uint32_t s = ld_acquire(&status[(size_t)k * stride]);
+        if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        acc += s & RTX_LB_VAL;
+        if (s & RTX_LB_INC) break;
+        --k;
+    }
+    st_release(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
+    return acc;
+}
+
+// -------------------------------------------------------------- kernels --
+__global__ __launch_bounds__(BS) void k_build_init(uint32_t* zero_a, uint32_t na, uint32_t* zero_b, uint32_t nb,
+                                                   uint32_t* zero_c, uint32_t nc, uint32_t* levels, uint32_t* bstart,
+                                                   uint32_t* bend, uint32_t* blevel, uint32_t* parent, uint32_t T,
+                                                   int set_root) {
+    uint32_t stride = gridDim.x * BS;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < na; i += stride) zero_a[i] = 0;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < nb; i += stride) zero_b[i] = 0;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < nc; i += stride) zero_c[i] = 0;
+    if (set_root && blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < LV_WORDS; i += BS) levels[i] = 0;
+        if (threadIdx.x == 0) {
+            levels[LV_CNT + 0] = 1;
+            levels[LV_TOTAL] = 1;
+            bstart[0] = 0;
+            bend[0] = T;
+            blevel[0] = 0;
+            parent[0] = 0;
+        }
+    }
+}
+
+// build_primitive_bbox + max_local_bbox (src/kernel/bvh-builder.cl:100-130,
+// 318-391): per-triangle AABB, its centroid (hi+lo)*0.5 for the Morton
+// stage, and the union of all AABBs (last block to finish reduces the
+// per-block partials).
+__global__ __launch_bounds__(BS) void k_prim_bounds(const uint32_t* __restrict__ idx, const float4* __restrict__ pos,
+                                                    float4* __restrict__ cen, float* __restrict__ partials,
+                                                    uint32_t* done, float* gbox6, uint32_t T) {
+    __shared__ float s_r[6][BS / RTX_WAVE];
+    __shared__ int s_last;
+    v3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < T; i += gridDim.x * BS) {
+        uint32_t i0 = idx[3 * (size_t)i], i1 = idx[3 * (size_t)i + 1], i2 = idx[3 * (size_t)i + 2];
+        v3 v0 = xyz(pos[i0]), v1 = xyz(pos[i1]), v2 = xyz(pos[i2]);
+        v3 bhi = v0, blo = v0;
+        bhi = vmax(bhi, v1);
+        bhi = vmax(bhi, v2);
+        blo = vmin(blo, v1);
+        blo = vmin(blo, v2);
+        v3 c = (bhi + blo) * 0.5f;
+        cen[i] = make_float4(c.x, c.y, c.z, 0.f);
+        lo = vmin(lo, blo);
+        hi = vmax(hi, bhi);
+    }
+    float r[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], __shfl_xor(r[k], off, RTX_WAVE));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], __shfl_xor(r[k], off, RTX_WAVE));
+    }
+    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) s_r[k][w] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < BS / RTX_WAVE; ++ww) {
+            for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], s_r[k][ww]);
+            for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], s_r[k][ww]);
+        }
+        for (int k = 0; k < 6; ++k) partials[6 * blockIdx.x + k] = r[k];
+        __threadfence();
+        uint32_t old = fetch_add_acqrel(done, 1u);
+        s_last = (old == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // last block: reduce all partials
+    float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += BS) {
+        for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], partials[6 * b + k]);
+        for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], partials[6 * b + k]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+    }
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) s_r[k][w] = q[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < BS / RTX_WAVE; ++ww) {
+            for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], s_r[k][ww]);
+            for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], s_r[k][ww]);
+        }
+        for (int k = 0; k < 6; ++k) gbox6[k] = q[k];
+    }
+}
+
+// morton_encode (src/kernel/bvh-builder.cl:181-252): centroid normalized to
+// the global AABB with a reciprocal then a multiply, clamped with
+// fmax(0, fmin(1, s)), scaled by 2^ceil(depth/3), truncated; bit i of the
+// code takes axis (off+1+i)%3, source bit (i+off)/3.  Also accumulates the
+// 8-bit digit histograms of every radix pass.
+__global__ __launch_bounds__(BS) void k_morton(const float4* __restrict__ cen, const float* __restrict__ gbox6,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                               uint32_t* __restrict__ hist, uint32_t T, int depth, int passes) {
+    __shared__ uint32_t s_h[4][256];
+    for (int p = 0; p < 4; ++p) s_h[p][threadIdx.x] = 0;
+    __syncthreads();
+    v3 glo = mk3(gbox6[0], gbox6[1], gbox6[2]);
+    v3 ghi = mk3(gbox6[3], gbox6[4], gbox6[5]);
+    v3 inv = recip(ghi - glo);
+    float scale = (float)(1u << ((depth + 2) / 3));
+    int offset = depth % 3 ? 3 - (depth % 3) : 0;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < T; i += gridDim.x * BS) {
+        v3 c = xyz(cen[i]);
+        v3 s = (c - glo) * inv;
+        s = vmax(mk3(0.f, 0.f, 0.f), vmin(mk3(1.f, 1.f, 1.f), s));
+        s = s * scale;
+        uint32_t m[3] = {(uint32_t)s.x, (uint32_t)s.y, (uint32_t)s.z};
+        int axis = (offset + 1) % 3;
+        uint32_t code = 0;
+        for (int b = 0; b < depth; ++b) {
+            uint32_t src = (uint32_t)((b + offset) / 3);
+            uint32_t mv = axis == 0 ? m[0] : (axis == 1 ? m[1] : m[2]);
+            code |= ((mv >> src) & 1u) << b;
+            axis = axis == 2 ? 0 : axis + 1;
+        }
+        keys[i] = code;
+        vals[i] = i;
+        for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p][(code >> (8 * p)) & 0xffu], 1u);
+    }
+    __syncthreads();
+    for (int p = 0; p < passes; ++p) {
+        uint32_t c = s_h[p][threadIdx.x];
+        if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+    }
+}
+
+// One stable LSD pass over `bits` bits at `shift` (onesweep: in-tile rank by
+// wave64 ballot matching, per-digit decoupled look-back across tiles,
+// LDS-staged scatter).  Replaces the bitonic network of
+// src/kernel/bvh-builder-sort.cl:26-232.
+__global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                  uint32_t n, int shift, int bits, const uint32_t* __restrict__ hist,
+                                                  uint32_t* status, uint32_t* ticket) {
+    __shared__ uint32_t s_keys[SORT_TILE];
+    __shared__ uint32_t s_vals[SORT_TILE];
+    __shared__ uint32_t s_wcnt[BS / RTX_WAVE][256];
+    __shared__ uint32_t s_tofs[256];
+    __shared__ uint32_t s_gofs[256];
+    __shared__ uint32_t s_w[BS / RTX_WAVE];
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+#pragma unroll
+    for (int i = 0; i < BS / RTX_WAVE; ++i) s_wcnt[i][tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * SORT_TILE;
+    const uint32_t dmask = (1u << bits) - 1u;
+
+    uint32_t k[SORT_ITEMS], v[SORT_ITEMS], r[SORT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+        bool ok = e < n;
+        k[j] = ok ? kin[e] : 0xffffffffu;
+        v[j] = ok ? vin[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+        bool ok = e < n;
+        uint32_t d = (k[j] >> shift) & dmask;
+        uint64_t peers = ballot(ok);
+        for (int b = 0; b < bits; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t below = lanes_below(peers);
+        uint32_t cnt = (uint32_t)__builtin_popcountll(peers);
+        uint32_t basec = ok ? s_wcnt[w][d] : 0u;
+        r[j] = basec + below;
+        if (ok && below == 0) s_wcnt[w][d] = basec + cnt;
+    }
+    __syncthreads();
+    // per-digit tile totals, wave offsets (exclusive, in wave order)
+    uint32_t tot = 0;
+    {
+        uint32_t c[BS / RTX_WAVE];
+#pragma unroll
+        for (int i = 0; i < BS / RTX_WAVE; ++i) c[i] = s_wcnt[i][tid];
+#pragma unroll
+        for (int i = 0; i < BS / RTX_WAVE; ++i) {
+            s_wcnt[i][tid] = tot;
+            tot += c[i];
+        }
+    }
+    uint32_t dummy;
+    uint32_t tofs = block_excl_scan(tot, s_w, &dummy);
+    s_tofs[tid] = tofs;
+    // global start of each digit = exclusive scan of the pass histogram
+    uint32_t hv = tid <= dmask ? hist[tid] : 0u;
+    uint32_t dbase = block_excl_scan(hv, s_w, &dummy);
+    if (tid <= dmask) {
+        uint32_t prefix = lookback(status + tid, tile, tot, 256u);
+        s_gofs[tid] = dbase + prefix - tofs;  // dest = s_gofs[d] + local position
+    }
+    __syncthreads();
+    // stage the tile in LDS in sorted order
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+        if (e < n) {
+            uint32_t d = (k[j] >> shift) & dmask;
+            uint32_t lp = s_tofs[d] + s_wcnt[w][d] + r[j];
+            s_keys[lp] = k[j];
+            s_vals[lp] = v[j];
+        }
+    }
+    __syncthreads();
+    uint32_t nvalid = n - base < (uint32_t)SORT_TILE ? n - base : (uint32_t)SORT_TILE;
+    for (uint32_t i = tid; i < nvalid; i += BS) {
+        uint32_t key = s_keys[i];
+        uint32_t dst = s_gofs[(key >> shift) & dmask] + i;
+        kout[dst] = key;
+        vout[dst] = s_vals[i];
+    }
+}
+
+// rearrange_indices / rearrange_map (src/kernel/bvh-builder.cl:49-72) plus
+// the sorted triangle records used by traversal and refit, and the
+// original-id map (composition of every build's permutation).
+__global__ __launch_bounds__(BS) void k_rearrange(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ idx,
+                                                  uint32_t* __restrict__ idx_out, const int32_t* __restrict__ mm,
+                                                  int32_t* __restrict__ mm_out, const uint32_t* __restrict__ orig,
+                                                  uint32_t* __restrict__ orig_out, const float4* __restrict__ pos,
+                                                  float4* __restrict__ tri, uint32_t T) {
+    uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i >= T) return;
+    uint32_t p = perm[i];
+    uint32_t a = idx[3 * (size_t)p], b = idx[3 * (size_t)p + 1], c = idx[3 * (size_t)p + 2];
+    idx_out[3 * (size_t)i] = a;
+    idx_out[3 * (size_t)i + 1] = b;
+    idx_out[3 * (size_t)i + 2] = c;
+    mm_out[i] = mm[p];
+    orig_out[i] = orig[p];
+    float4 pa = pos[a], pb = pos[b], pc = pos[c];
+    tri[3 * (size_t)i] = make_float4(pa.x, pa.y, pa.z, __uint_as_float(a));
+    tri[3 * (size_t)i + 1] = make_float4(pb.x, pb.y, pb.z, __uint_as_float(b));
+    tri[3 * (size_t)i + 2] = make_float4(pc.x, pc.y, pc.z, __uint_as_float(c));
+}
+
+// Triangle records from the current index buffer, identity order (refit
+// path, Q14 of src/rt/bvh-builder.cpp:779-817).
+__global__ __launch_bounds__(BS) void k_tri_records(const uint32_t* __restrict__ idx, const float4* __restrict__ pos,
+                                                    float4* __restrict__ tri, uint32_t T) {
+    uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i >= T) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t a = idx[3 * (size_t)i + k];
+        float4 p = pos[a];
+        tri[3 * (size_t)i + k] = make_float4(p.x, p.y, p.z, __uint_as_float(a));
+    }
+}
+
+// One iteration of process_task (src/kernel/bvh-builder.cl:582-671) over the
+// nodes of level d, in order.  find_middle (:484-552) reduces to: leaf if
+// count <= min_leaf+2 (unsigned test), or no bit <= the node's level bit
+// differs between its first and last code; else the split level is the
+// highest differing bit b and the split is the first index with bit b set.
+// Children of the k-th splitting node of the level go to
+// off_next + 2k (ordered compaction by decoupled look-back).
+__global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ codes, uint32_t* __restrict__ bstart,
+                                                   uint32_t* __restrict__ bend, uint32_t* __restrict__ blevel,
+                                                   uint32_t* __restrict__ node_u32, uint32_t* __restrict__ parent,
+                                                   uint32_t* __restrict__ refit_cnt, uint32_t* levels,
+                                                   uint32_t* status, uint32_t* ticket, int d, int depth, int min_leaf,
+                                                   int force_leaf) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[BS / RTX_WAVE];
+    const uint32_t off = levels[LV_OFF + d];
+    const uint32_t cnt = levels[LV_CNT + d];
+    if (cnt == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            levels[LV_OFF + d + 1] = off;
+            levels[LV_CNT + d + 1] = 0;
+        }
+        return;
+    }
+    const uint32_t ntiles = (cnt + BS - 1) / BS;
+    if (blockIdx.x >= ntiles) return;  // only ntiles blocks take tickets
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t k = tile * BS + tid;
+    const bool active = k < cnt;
+    const uint32_t n = off + k;
+    uint32_t start = 0, end = 0, split = 0, lvl_child = 0;
+    if (active) {
+        start = bstart[n];
+        end = bend[n];
+        uint32_t lvl = blevel[n];
+        uint32_t count = end - start;
+        if (!force_leaf && count - 2u > (uint32_t)min_leaf) {
+            int bit = depth - 1 - (int)lvl;
+            if (bit >= 0) {
+                uint32_t mask = bit >= 31 ? 0xffffffffu : ((2u << bit) - 1u);
+                uint32_t x = (codes[start] ^ codes[end - 1]) & mask;
+                if (x) {
+                    int b = 31 - __builtin_clz(x);
+                    uint32_t lo = start, hi = end - 1;
+                    while (lo + 1 < hi) {
+                        uint32_t mid = (lo + hi) / 2;
+                        if (!((codes[mid] >> b) & 1u))
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    split = hi;
+                    lvl_child = (uint32_t)(depth - 1 - b) + 1u;
+                }
+            }
+        }
+    }
+    const bool is_split = split != 0;
+    uint64_t bal = ballot(is_split);
+    uint32_t excl = lanes_below(bal);
+    if (lane == 0) s_w[w] = (uint32_t)__builtin_popcountll(bal);
+    __syncthreads();
+    uint32_t wofs = 0, agg = 0;
+#pragma unroll
+    for (int i = 0; i < BS / RTX_WAVE; ++i) {
+        wofs += (i < (int)w) ? s_w[i] : 0u;
+        agg += s_w[i];
+    }
+    if (tid == 0) {
+        uint32_t prefix = lookback(status, tile, agg, 1u);
+        s_prefix = prefix;
+        if (tile == ntiles - 1) {
+            uint32_t total = prefix + agg;
+            levels[LV_OFF + d + 1] = off + cnt;
+            levels[LV_CNT + d + 1] = 2u * total;
+            levels[LV_TOTAL] = off + cnt + 2u * total;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    if (is_split) {
+        uint32_t rank = s_prefix + wofs + excl;
+        uint32_t c = off + cnt + 2u * rank;
+        bstart[c] = start;
+        bend[c] = split;
+        blevel[c] = lvl_child;
+        bstart[c + 1] = split;
+        bend[c + 1] = end;
+        blevel[c + 1] = lvl_child;
+        parent[c] = n;
+        parent[c + 1] = n;
+        node_u32[8 * (size_t)n + 3] = c;
+        node_u32[8 * (size_t)n + 7] = 0u;
+        refit_cnt[n] = 0u;
+    } else {
+        node_u32[8 * (size_t)n + 3] = start;
+        node_u32[8 * (size_t)n + 7] = end - start;
+    }
+}
+
+// build_leaf_bbox + build_node_bbox (src/kernel/bvh-builder.cl:262-311) as
+// one bottom-up pass: each leaf unions its triangles' vertices (= union of
+// the primitive AABBs), then climbs; at every internal node the second
+// arriving child computes the union of both children and the near-child
+// delta (right.lo - left.lo).  Arrival counters count modulo 2, so a refit
+// needs no reset.
+__global__ __launch_bounds__(BS) void k_refit(const float4* __restrict__ tri, float4* node, float4* __restrict__ delta,
+                                              const uint32_t* __restrict__ parent, uint32_t* refit_cnt,
+                                              const uint32_t* __restrict__ levels, uint32_t ncap) {
+    uint32_t N = levels[LV_TOTAL];
+    uint32_t n = blockIdx.x * BS + threadIdx.x;
+    if (n >= N || n >= ncap) return;
+    float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
+    uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+    if (cnt == 0) return;
+    v3 lo = xyz(tri[3 * (size_t)a]), hi = lo;
+    for (uint32_t t = a; t < a + cnt; ++t) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v3 p = xyz(tri[3 * (size_t)t + q]);
+            lo = vmin(lo, p);
+            hi = vmax(hi, p);
+        }
+    }
+    node[2 * (size_t)n] = make_float4(lo.x, lo.y, lo.z, f0.w);
+    node[2 * (size_t)n + 1] = make_float4(hi.x, hi.y, hi.z, f1.w);
+    uint32_t cur = n;
+    while (cur != 0) {
+        uint32_t p = parent[cur];
+        __threadfence();
+        uint32_t old = fetch_add_acqrel(&refit_cnt[p], 1u);
+        if ((old & 1u) == 0u) return;  // first child to arrive
+        __threadfence();
+        float4 pw = node[2 * (size_t)p];
+        uint32_t l = __float_as_uint(pw.w);
+        float4 L0 = node[2 * (size_t)l], L1 = node[2 * (size_t)l + 1];
+        float4 R0 = node[2 * (size_t)l + 2], R1 = node[2 * (size_t)l + 3];
+        v3 plo = vmin(xyz(L0), xyz(R0));
+        v3 phi = vmax(xyz(L1), xyz(R1));
+        node[2 * (size_t)p] = make_float4(plo.x, plo.y, plo.z, pw.w);
+        node[2 * (size_t)p + 1] = make_float4(phi.x, phi.y, phi.z, __uint_as_float(0u));
+        v3 dl = xyz(R0) - xyz(L0);
+        delta[p] = make_float4(dl.x, dl.y, dl.z, 0.f);
+        cur = p;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host --
+static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
+    rtx_ctx* c = s->ctx;
+    size_t ncap = 2 * (size_t)T;
+    uint32_t passes = (uint32_t)(depth + 7) / 8;
+    uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
+    uint32_t hier_tiles = rtx_div_up(T, BS);
+    int32_t e = 0;
+    if ((e = s->node.reserve(c, ncap * 32))) return e;
+    if ((e = s->delta.reserve(c, ncap * 16))) return e;
+    if ((e = s->parent.reserve(c, ncap * 4))) return e;
+    if ((e = s->refit_cnt.reserve(c, ncap * 4))) return e;
+    if ((e = s->bstart.reserve(c, ncap * 4))) return e;
+    if ((e = s->bend.reserve(c, ncap * 4))) return e;
+    if ((e = s->blevel.reserve(c, ncap * 4))) return e;
+    if ((e = s->tri.reserve(c, (size_t)T * 48))) return e;
+    if ((e = s->codes.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->perm.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->keys_alt.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->vals_alt.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->centroid.reserve(c, (size_t)T * 16))) return e;
+    if ((e = s->partials.reserve(c, BOUNDS_GRID * 6 * 4))) return e;
+    if ((e = s->gbox.reserve(c, 8 * 4))) return e;
+    if ((e = s->levels.reserve(c, LV_WORDS * 4))) return e;
+    if ((e = s->tickets.reserve(c, TK_COUNT * 4))) return e;
+    if ((e = s->sort_hist.reserve(c, 4 * 256 * 4))) return e;
+    if ((e = s->sort_status.reserve(c, (size_t)passes * sort_tiles * 256 * 4))) return e;
+    if ((e = s->hier_status.reserve(c, (size_t)(depth + 1) * hier_tiles * 4))) return e;
+    return 0;
+}
+
+int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
+    RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_build: null scene");
+    RTX_REQUIRE(s->T > 0 && s->V > 0, RTX_ESTATE, "rtx_lbvh_build: scene has no triangles");
+    RTX_REQUIRE(s->T < (1u << 29), RTX_EUNSUPPORTED, "rtx_lbvh_build: more than 2^29 triangles");
+    int depth = p ? p->depth : 30;
+    int min_leaf = p ? p->min_leaf : 1;
+    // RendererConfig clamps (src/rt/bvh-builder.cpp:194-199); codes are 32-bit here.
+    RTX_REQUIRE(depth <= 32, RTX_EUNSUPPORTED, "rtx_lbvh_build: bvh_depth > 32 needs 64-bit Morton keys");
+    depth = depth < 1 ? 1 : depth;
+    min_leaf = min_leaf < 1 ? 1 : (min_leaf > 256 ? 256 : min_leaf);
+    const uint32_t T = s->T;
+    hipStream_t st = s->ctx->pick(stream);
+    int32_t e = lbvh_alloc(s, T, depth);
+    if (e) return e;
+    const int passes = (depth + 7) / 8;
+    const uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
+    const uint32_t hier_tiles = rtx_div_up(T, BS);
+    uint32_t* tickets = s->tickets.as<uint32_t>();
+    uint32_t* levels = s->levels.as<uint32_t>();
+    {
+        ProfScope ps(s->ctx, st, "lbvh_init");
+        uint32_t nb = (uint32_t)(passes * (size_t)sort_tiles * 256);
+        uint32_t nc = (uint32_t)((size_t)(depth + 1) * hier_tiles);
+        uint32_t grid = rtx_div_up((uint64_t)(nb > nc ? nb : nc), BS);
+        grid = grid < 1 ? 1 : (grid > 2048 ? 2048 : grid);
+        // zero: tickets + histograms (contiguous? no: separate buffers)
+        hipLaunchKernelGGL(k_build_init, dim3(grid), dim3(BS), 0, st, tickets, (uint32_t)TK_COUNT,
+                           s->sort_status.as<uint32_t>(), nb, s->hier_status.as<uint32_t>(), nc, levels,
+                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->blevel.as<uint32_t>(),
+                           s->parent.as<uint32_t>(), T, 1);
+        hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, s->sort_hist.as<uint32_t>(), 4u * 256u,
+                           (uint32_t*)nullptr, 0u, (uint32_t*)nullptr, 0u, levels, (uint32_t*)nullptr,
+                           (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, T, 0);
+    }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_prim_bounds");
+        uint32_t grid = rtx_div_up(T, BS);
+        grid = grid > BOUNDS_GRID ? BOUNDS_GRID : grid;
+        hipLaunchKernelGGL(k_prim_bounds, dim3(grid), dim3(BS), 0, st, s->idx.as<uint32_t>(), s->pos.as<float4>(),
+                           s->centroid.as<float4>(), s->partials.as<float>(), tickets + TK_BOUNDS,
+                           s->gbox.as<float>(), T);
+    }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_morton");
+        uint32_t grid = rtx_div_up(T, BS);
+        grid = grid > 1024 ? 1024 : grid;
+        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, s->centroid.as<float4>(), s->gbox.as<float>(),
+                           s->codes.as<uint32_t>(), s->perm.as<uint32_t>(), s->sort_hist.as<uint32_t>(), T, depth,
+                           passes);
+    }
+    uint32_t *kin = s->codes.as<uint32_t>(), *vin = s->perm.as<uint32_t>();
+    uint32_t *kout = s->keys_alt.as<uint32_t>(), *vout = s->vals_alt.as<uint32_t>();
+    for (int pass = 0; pass < passes; ++pass) {
+        ProfScope ps(s->ctx, st, "lbvh_sort_pass");
+        int shift = 8 * pass;
+        int bits = depth - shift < 8 ? depth - shift : 8;
+        hipLaunchKernelGGL(k_sort_pass, dim3(sort_tiles), dim3(BS), 0, st, kin, vin, kout, vout, T, shift, bits,
+                           s->sort_hist.as<uint32_t>() + 256 * pass,
+                           s->sort_status.as<uint32_t>() + (size_t)pass * sort_tiles * 256, tickets + TK_SORT + pass);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != s->codes.as<uint32_t>()) {  // odd pass count: sorted data sits in the alternates
+        std::swap(s->codes, s->keys_alt);
+        std::swap(s->perm, s->vals_alt);
+    }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_rearrange");
+        hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->perm.as<uint32_t>(),
+                           s->idx.as<uint32_t>(), s->idx_alt.as<uint32_t>(), s->matmap.as<int32_t>(),
+                           s->matmap_alt.as<int32_t>(), s->orig.as<uint32_t>(), s->orig_alt.as<uint32_t>(),
+                           s->pos.as<float4>(), s->tri.as<float4>(), T);
+        std::swap(s->idx, s->idx_alt);
+        std::swap(s->matmap, s->matmap_alt);
+        std::swap(s->orig, s->orig_alt);
+    }
+    for (int d = 0; d <= depth; ++d) {
+        ProfScope ps(s->ctx, st, "lbvh_hier_level");
+        uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;
+        if (maxn > 2ull * T) maxn = 2ull * T;
+        uint32_t grid = rtx_div_up(maxn, BS);
+        hipLaunchKernelGGL(k_hier_level, dim3(grid), dim3(BS), 0, st, s->codes.as<uint32_t>(),
+                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->blevel.as<uint32_t>(),
+                           s->node.as<uint32_t>(), s->parent.as<uint32_t>(), s->refit_cnt.as<uint32_t>(), levels,
+                           s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d, depth,
+                           min_leaf, d == depth ? 1 : 0);
+    }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_refit");
+        uint32_t ncap = (uint32_t)(2 * (size_t)T);
+        hipLaunchKernelGGL(k_refit, dim3(rtx_div_up(ncap, BS)), dim3(BS), 0, st, s->tri.as<float4>(),
+                           s->node.as<float4>(), s->delta.as<float4>(), s->parent.as<uint32_t>(),
+                           s->refit_cnt.as<uint32_t>(), levels, ncap);
+    }
+    RTX_CHECK_HIP(hipGetLastError());
+    s->built = true;
+    s->depth = depth;
+    s->min_leaf = min_leaf;
+    return RTX_OK;
+}
+
+int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
+    RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_refit: null scene");
+    RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_lbvh_refit: no BVH built");
+    hipStream_t st = s->ctx->pick(stream);
+    const uint32_t T = s->T;
+    {
+        ProfScope ps(s->ctx, st, "lbvh_tri_records");
+        hipLaunchKernelGGL(k_tri_records, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->idx.as<uint32_t>(),
+                           s->pos.as<float4>(), s->tri.as<float4>(), T);
+    }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_refit");
+        uint32_t ncap = (uint32_t)(2 * (size_t)T);
+        hipLaunchKernelGGL(k_refit, dim3(rtx_div_up(ncap, BS)), dim3(BS), 0, st, s->tri.as<float4>(),
+                           s->node.as<float4>(), s->delta.as<float4>(), s->parent.as<uint32_t>(),
+                           s->refit_cnt.as<uint32_t>(), s->levels.as<uint32_t>(), ncap);
+    }
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_lbvh_info(rtx_scene* s, uint32_t* node_count, uint32_t* level_counts64) {
+    RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_info: null scene");
+    RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_lbvh_info: no BVH built");
+    uint32_t lv[LV_WORDS];
+    RTX_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(lv, s->levels.p, sizeof lv, hipMemcpyDeviceToHost));
+    if (node_count) *node_count = lv[LV_TOTAL];
+    if (level_counts64)
+        for (int i = 0; i < 64; ++i) level_counts64[i] = (i < 63 && i < s->depth) ? lv[LV_CNT + i + 1] : 0u;
+    return RTX_OK;
+}
+
+int32_t rtx_read_nodes_ref(rtx_scene* s, rtx_node_ref* out, uint32_t cap, uint32_t* count) {
+    uint32_t N = 0;
+    int32_t e = rtx_lbvh_info(s, &N, nullptr);
+    if (e) return e;
+    if (count) *count = N;
+    if (!out) return RTX_OK;
+    RTX_REQUIRE(cap >= N, RTX_EINVAL, "rtx_read_nodes_ref: capacity below node count");
+    std::vector<float4> nd(2 * (size_t)N);
+    std::vector<uint32_t> par(N);
+    RTX_CHECK_HIP(hipMemcpy(nd.data(), s->node.p, nd.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(par.data(), s->parent.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t n = 0; n < N; ++n) {
+        const float4 a = nd[2 * (size_t)n], b = nd[2 * (size_t)n + 1];
+        rtx_node_ref& o = out[n];
+        o.lo[0] = a.x; o.lo[1] = a.y; o.lo[2] = a.z; o.lo[3] = 0.f;
+        o.hi[0] = b.x; o.hi[1] = b.y; o.hi[2] = b.z; o.hi[3] = 0.f;
+        uint32_t w0, w1;
+        std::memcpy(&w0, &a.w, 4);
+        std::memcpy(&w1, &b.w, 4);
+        if (w1 == 0) {  // internal
+            o.a = w0;
+            o.b = w0 + 1;
+            o.leaf = 0;
+        } else {
+            o.a = w0;
+            o.b = w0 + w1;
+            o.leaf = 1;
+        }
+        o.parent = n == 0 ? 0u : par[n];
+        o.split_axis = 0;
+        o.pad[0] = o.pad[1] = 0;
+    }
+    return RTX_OK;
+}

@@ -1,0 +1,165 @@
+// Frame orchestration: the plain one-launch-per-stage frame and the
+// read-back-free frame.
+//
+// Reference: Renderer::set_up_frame / update_accelerator /
+// render_to_framebuffer / copy_framebuffer / conclude_frame
+// (src/rt/renderer.cpp:12-272).  The whole frame is one tile: the reference
+// splits it into tiles of CUs * max_work_group * 128 rays, which on MI355X
+// already exceeds any configured resolution, and HBM holds worst-case
+// secondary bundles (2^bounces * pixels rays).
+#include <chrono>
+
+#include "rtx_impl.hpp"
+
+namespace {
+// Device counters of a frame (uint64): [0] secondary rays, [1] shadow rays.
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if (threadIdx.x < (unsigned)n) p[threadIdx.x] = 0ull;
+}
+}  // namespace
+
+int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_renderer** out) {
+    RTX_REQUIRE(ctx && p && out, RTX_EINVAL, "rtx_renderer_create: null argument");
+    RTX_REQUIRE(p->width > 0 && p->height > 0 && p->max_bounces >= 0 && p->max_bounces <= 16, RTX_EINVAL,
+                "rtx_renderer_create: bad size or bounce count");
+    rtx_renderer* r = new rtx_renderer();
+    r->ctx = ctx;
+    r->p = *p;
+    if (r->p.quad_size <= 0) r->p.quad_size = 32;
+    if (r->p.depth <= 0) r->p.depth = 30;
+    if (r->p.min_leaf <= 0) r->p.min_leaf = 1;
+    uint64_t P = (uint64_t)p->width * p->height;
+    uint64_t cap = p->ray_capacity;
+    if (!cap) {
+        cap = P << (p->max_bounces > 0 ? p->max_bounces : 0);
+        if (cap > (1ull << 29)) cap = 1ull << 29;
+    }
+    if (cap < P) cap = P;
+    r->p.ray_capacity = cap;
+    int32_t e = 0;
+    if ((e = rtx_rays_create(ctx, cap, &r->a)) || (e = rtx_rays_create(ctx, cap, &r->b)) ||
+        (e = rtx_hits_create(ctx, cap, &r->hits)) || (e = rtx_fb_create(ctx, p->width, p->height, &r->fb)) ||
+        (e = r->counts.reserve(ctx, 64))) {
+        rtx_renderer_destroy(r);
+        return e;
+    }
+    *out = r;
+    return RTX_OK;
+}
+
+int32_t rtx_renderer_destroy(rtx_renderer* r) {
+    RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_destroy: null");
+    if (r->a) rtx_rays_destroy(r->a);
+    if (r->b) rtx_rays_destroy(r->b);
+    if (r->hits) rtx_hits_destroy(r->hits);
+    if (r->fb) rtx_fb_destroy(r->fb);
+    r->counts.release();
+    delete r;
+    return RTX_OK;
+}
+
+rtx_fb* rtx_renderer_fb(rtx_renderer* r) { return r ? r->fb : nullptr; }
+
+int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild,
+                         rtx_frame_stats* stats, void* stream) {
+    RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame: null argument");
+    hipStream_t st = r->ctx->pick(stream);
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t P = (uint64_t)r->p.width * r->p.height;
+    unsigned long long* cnt = r->counts.as<unsigned long long>();
+    int32_t e;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 8);
+    // set_up_frame: clear + update_accelerator
+    if ((e = rtx_fb_clear(r->fb, st))) return e;
+    if (rebuild || !s->built) {
+        rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
+        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
+    }
+    rtx_frame_stats fs{};
+    fs.primary_rays = P;
+    rtx_rays* in = r->a;
+    rtx_rays* out = r->b;
+    if ((e = rtx_raygen_primary(in, cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st))) return e;
+    if ((e = rtx_trace(s, in, r->hits, P, st))) return e;
+    if ((e = rtx_trace_shadow_impl(s, in, r->hits, P, (uint64_t*)(cnt + 1), st))) return e;
+    if ((e = rtx_shade(s, in, r->hits, r->fb, P, 1, st))) return e;
+    uint64_t n = P;
+    for (int b = 0; b < r->p.max_bounces; ++b) {
+        uint64_t m = 0;
+        if ((e = rtx_raygen_secondary_impl(s, in, r->hits, out, n, (uint64_t*)cnt, st))) return e;
+        // the reference reads the new count back every bounce
+        // (src/rt/secondary-ray-generator.cpp:296-299)
+        uint32_t c32 = 0;
+        RTX_CHECK_HIP(hipMemcpyAsync(&c32, out->count.p, 4, hipMemcpyDeviceToHost, st));
+        RTX_CHECK_HIP(hipStreamSynchronize(st));
+        m = c32;
+        std::swap(in, out);
+        if (!m) break;
+        fs.secondary_rays += m;
+        fs.bounces_run++;
+        if ((e = rtx_trace(s, in, r->hits, m, st))) return e;
+        if ((e = rtx_trace_shadow_impl(s, in, r->hits, m, (uint64_t*)(cnt + 1), st))) return e;
+        if ((e = rtx_shade(s, in, r->hits, r->fb, m, 0, st))) return e;
+        n = m;
+    }
+    if ((e = rtx_fb_resolve(r->fb, st))) return e;
+    unsigned long long hc[2] = {0, 0};
+    RTX_CHECK_HIP(hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, st));
+    RTX_CHECK_HIP(hipStreamSynchronize(st));
+    fs.shadow_rays = hc[1];
+    auto t1 = std::chrono::steady_clock::now();
+    fs.frame_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    r->last = fs;
+    r->last_async = false;
+    if (stats) *stats = fs;
+    return RTX_OK;
+}
+
+int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
+    RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame_async: null argument");
+    hipStream_t st = r->ctx->pick(stream);
+    const uint64_t P = (uint64_t)r->p.width * r->p.height;
+    unsigned long long* cnt = r->counts.as<unsigned long long>();
+    int32_t e;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 8);
+    if ((e = rtx_fb_clear(r->fb, st))) return e;
+    if (rebuild || !s->built) {
+        rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
+        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
+    }
+    rtx_rays* in = r->a;
+    rtx_rays* out = r->b;
+    if ((e = rtx_raygen_primary(in, cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st))) return e;
+    if ((e = rtx_trace(s, in, r->hits, P, st))) return e;
+    if ((e = rtx_trace_shadow_impl(s, in, r->hits, P, (uint64_t*)(cnt + 1), st))) return e;
+    if ((e = rtx_shade(s, in, r->hits, r->fb, P, 1, st))) return e;
+    for (int b = 0; b < r->p.max_bounces; ++b) {
+        uint64_t n_in = b == 0 ? P : RTX_DEVICE_COUNT;
+        if ((e = rtx_raygen_secondary_impl(s, in, r->hits, out, n_in, (uint64_t*)cnt, st))) return e;
+        std::swap(in, out);
+        // zero-count bounces run as empty launches (no host decision)
+        if ((e = rtx_trace(s, in, r->hits, RTX_DEVICE_COUNT, st))) return e;
+        if ((e = rtx_trace_shadow_impl(s, in, r->hits, RTX_DEVICE_COUNT, (uint64_t*)(cnt + 1), st))) return e;
+        if ((e = rtx_shade(s, in, r->hits, r->fb, RTX_DEVICE_COUNT, 0, st))) return e;
+    }
+    if ((e = rtx_fb_resolve(r->fb, st))) return e;
+    r->last_async = true;
+    r->last = rtx_frame_stats{};
+    r->last.primary_rays = P;
+    return RTX_OK;
+}
+
+int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats) {
+    RTX_REQUIRE(r && stats, RTX_EINVAL, "rtx_renderer_last_stats: null argument");
+    if (r->last_async) {
+        unsigned long long hc[2] = {0, 0};
+        RTX_CHECK_HIP(hipStreamSynchronize(r->ctx->stream));
+        RTX_CHECK_HIP(hipDeviceSynchronize());
+        RTX_CHECK_HIP(hipMemcpy(hc, r->counts.p, 16, hipMemcpyDeviceToHost));
+        r->last.secondary_rays = hc[0];
+        r->last.shadow_rays = hc[1];
+        r->last_async = false;
+    }
+    *stats = r->last;
+    return RTX_OK;
+}

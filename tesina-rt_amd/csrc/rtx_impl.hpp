@@ -1,0 +1,162 @@
+// Internal declarations of the rtx library (gfx950).
+//
+// Device data layout (HBM):
+//   scene : pos4[V] (x,y,z,0)   nrm4[V] (unit normal, 0)   uv2[V]
+//           idx[3T] (permuted in place by each build)   matmap[T]   orig[T]
+//   bvh   : node[N]  32 B = {lo.xyz, a | hi.xyz, b}; internal: a = left child
+//                    (right = a+1, children are adjacent), b = 0;
+//                    leaf: a = first triangle, b = triangle count (>= 1)
+//           delta[N] 16 B = right.lo - left.lo (near-child test of
+//                    src/kernel/trace-bvh.cl:392-409, precomputed at refit)
+//           parent[N], tri[3T] 48 B per sorted triangle {v0,i0 | v1,i1 | v2,i2}
+//   rays  : ori4 (origin, tMin)  dir4 (dir, tMax)  pixel  contrib   (SoA)
+//   hits  : flags t id bary2 n4 p4 uv2                               (SoA)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rtx.h"
+
+#define RTX_WAVE 64
+
+// ------------------------------------------------------------ error path --
+void rtx_set_error(const std::string& msg);
+#define RTX_CHECK_HIP(expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            rtx_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return RTX_EDEVICE;                                                          \
+        }                                                                                \
+    } while (0)
+#define RTX_REQUIRE(cond, code, msg)   \
+    do {                               \
+        if (!(cond)) {                 \
+            rtx_set_error(msg);        \
+            return (code);             \
+        }                              \
+    } while (0)
+
+// ---------------------------------------------------------- device buffer --
+struct rtx_ctx;
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    rtx_ctx* ctx = nullptr;
+    // Grows to at least `n` bytes; contents are not kept.
+    int32_t reserve(rtx_ctx* c, size_t n);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// --------------------------------------------------------------- objects --
+struct KernelTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+struct rtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    uint64_t bytes_in_use = 0;
+    bool profiling = false;
+    std::vector<KernelTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    struct Acc {
+        std::string name;
+        uint64_t count;
+        double ms;
+    };
+    std::vector<Acc> acc;
+    hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+};
+
+// Brackets one launch with events when profiling is on.
+struct ProfScope {
+    rtx_ctx* ctx;
+    hipStream_t st;
+    int slot = -1;
+    ProfScope(rtx_ctx* c, hipStream_t s, const char* name);
+    ~ProfScope();
+};
+
+struct rtx_scene {
+    rtx_ctx* ctx = nullptr;
+    uint32_t V = 0, T = 0, M = 0;
+    DevBuf pos, nrm, uv;
+    DevBuf idx, idx_alt, matmap, matmap_alt, orig, orig_alt;
+    DevBuf mats, lights, waves, stage;
+    rtx_lights host_lights{};
+    // BVH
+    bool built = false;
+    int depth = 30, min_leaf = 1;
+    DevBuf node, delta, parent, tri, codes, perm;
+    DevBuf gbox;        // 6 x int32 ordered-float keys
+    DevBuf levels;      // uint32: [0..64) level offsets, [64..128) level counts, [128] node count
+    DevBuf refit_cnt;   // uint32[N] arrival counters
+    // build scratch
+    DevBuf bstart, bend, blevel;  // per-node build state
+    DevBuf keys_alt, vals_alt;    // radix ping-pong
+    DevBuf centroid, partials;    // per-triangle centroids, per-block AABB partials
+    DevBuf sort_hist;             // [4][256] digit histograms -> exclusive offsets
+    DevBuf sort_status;           // look-back words, per pass x tile x digit
+    DevBuf tickets;               // tile tickets for every ordered kernel
+    DevBuf hier_status;           // look-back words, per level x tile
+};
+
+struct rtx_rays {
+    rtx_ctx* ctx = nullptr;
+    uint64_t cap = 0;
+    DevBuf ori4, dir4, pixel, contrib;
+    DevBuf count;  // uint32[4]: [0] ray count, [1] scan total, [2] first dropped offset, [3] ticket
+    DevBuf scan_status;  // look-back words of the generation that writes this bundle
+    DevBuf psamp;        // pixel-sample table of primary generation (spp x 3 floats)
+};
+
+struct rtx_hits {
+    rtx_ctx* ctx = nullptr;
+    uint64_t cap = 0;
+    DevBuf flags, t, id, bary, n4, p4, uv2;
+};
+
+struct rtx_fb {
+    rtx_ctx* ctx = nullptr;
+    int w = 0, h = 0;
+    DevBuf accum, rgba;
+};
+
+struct rtx_renderer {
+    rtx_ctx* ctx = nullptr;
+    rtx_render_params p{};
+    rtx_rays* a = nullptr;
+    rtx_rays* b = nullptr;
+    rtx_hits* hits = nullptr;
+    rtx_fb* fb = nullptr;
+    rtx_frame_stats last{};
+    DevBuf counts;  // uint64 device ray counters of the current frame (see render.hip)
+    bool last_async = false;
+};
+
+// ------------------------------------------------ ordered (look-back) scan --
+// Status word: bits 31..30 flag (0 empty, 1 aggregate, 2 inclusive prefix),
+// bits 29..0 value.
+#define RTX_LB_AGG 0x40000000u
+#define RTX_LB_INC 0x80000000u
+#define RTX_LB_VAL 0x3fffffffu
+
+// Fill kernel (used instead of hipMemsetAsync so that every entry point can
+// be captured into a hipGraph).
+int32_t rtx_fill_u32(rtx_ctx* ctx, void* p, uint32_t value, uint64_t n, hipStream_t st);
+// Internal variants used by the renderer: device-count stats accumulation.
+int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, uint64_t* dev_shadow_count,
+                              hipStream_t st);
+int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
+                                  uint64_t* dev_total_acc, hipStream_t st);
+
+static inline uint32_t rtx_div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }

@@ -1,0 +1,204 @@
+// Whitted secondary rays: one fused mark + scan + generate pass.
+//
+// Reference: SecondaryRayGenerator::gen_scan (src/rt/secondary-ray-generator.cpp:248-340)
+// with mark_secondary_rays / generate_secondary_rays
+// (src/kernel/secondary-ray-generator.cl:43-181) and gpu_scan_uint
+// (src/gpu/scan.cpp:17-157).  The reference marks, scans in several
+// launches, reads the total back to the host and generates; here each
+// 256-ray tile computes its children, takes its output offset from a
+// decoupled look-back over the tiles before it (tiles are taken in order),
+// and writes.  Output order equals the reference's: parents in order,
+// refraction before reflection.  The count stays on the device.
+#include "devmath.hpp"
+#include "rtx_impl.hpp"
+
+namespace {
+
+constexpr int SB = 256;
+
+__global__ void k_sec_init(uint32_t* status, uint32_t nwords, uint32_t* in_cnt, uint32_t* out_cnt) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x) status[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        in_cnt[3] = 0;             // tile ticket
+        out_cnt[1] = 0;            // scan total
+        out_cnt[2] = 0xffffffffu;  // first dropped offset
+    }
+}
+
+__global__ __launch_bounds__(SB) void k_secondary(
+    const float4* __restrict__ dir_in, const int32_t* __restrict__ pix_in, const float* __restrict__ con_in,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ hid, const float4* __restrict__ hn,
+    const float4* __restrict__ hp, const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap,
+    uint32_t* in_cnt, uint64_t hcount, float4* __restrict__ ori_out, float4* __restrict__ dir_out,
+    int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt, uint32_t* status, uint32_t cap) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[SB / RTX_WAVE];
+    const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)in_cnt[0] : hcount;
+    const uint32_t ntiles = (uint32_t)((n + SB - 1) / SB);
+    // blocks past the last tile leave before taking a ticket, so the first
+    // ntiles blocks to arrive take exactly the tickets [0, ntiles)
+    if (blockIdx.x >= ntiles) return;  // (n == 0: out_cnt[1] stays 0 from k_sec_init)
+    if (threadIdx.x == 0) s_tile = atomicAdd(&in_cnt[3], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t k = (uint64_t)tile * SB + tid;
+    uint32_t c = 0;
+    float R = 1.f;
+    v3 refr = mk3(0.f, 0.f, 0.f), refl = mk3(0.f, 0.f, 0.f);
+    float reflectiveness = 0.f;
+    if (k < n && (flags[k] & 1u)) {
+        const rtx_material m = mats[matmap[hid[k]]];
+        reflectiveness = m.reflectiveness;
+        v3 d = -normalize(xyz(dir_in[k]));
+        v3 nn = normalize(xyz(hn[k]));
+        if (m.refractive_index > 0.f) {
+            float etai, etat;
+            if (flags[k] & 2u) {
+                etat = 1.f;
+                etai = m.refractive_index;
+            } else {
+                etai = 1.f;
+                etat = m.refractive_index;
+            }
+            float cosi = dot(d, nn);
+            float rel = etai / etat;
+            float wv = rel * cosi;
+            float kk = 1.f + (wv - rel) * (wv + rel);
+            if (kk > 0.f) {
+                float R0 = (etat - 1.f) / (etat + 1.f);
+                R0 *= R0;
+                float x = 1.f - cosi;
+                float x2 = x * x;
+                float cos5 = x2 * x2 * x;
+                R = R0 + (1.f - R0) * cos5;
+                kk = __builtin_sqrtf(kk);
+                refr = normalize((wv - kk) * nn - etai * d);
+                c |= 1u;
+            }
+        }
+        if (m.reflectiveness > 0.f) {
+            v3 dd = -d;
+            refl = normalize(dd - 2.f * nn * dot(dd, nn));
+            c |= 2u;
+        }
+    }
+    const uint32_t cnt = (c & 1u) + (c >> 1);
+    uint64_t b0 = ballot(cnt & 1u), b1 = ballot(cnt >> 1);
+    uint32_t excl = lanes_below(b0) + 2u * lanes_below(b1);
+    if (lane == 0) s_w[w] = (uint32_t)__builtin_popcountll(b0) + 2u * (uint32_t)__builtin_popcountll(b1);
+    __syncthreads();
+    uint32_t wofs = 0, agg = 0;
+#pragma unroll
+    for (int i = 0; i < SB / RTX_WAVE; ++i) {
+        wofs += (i < (int)w) ? s_w[i] : 0u;
+        agg += s_w[i];
+    }
+    if (tid == 0) {
+        // lookback (same protocol as the LBVH kernels)
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            st_release(&status[0], RTX_LB_INC | agg);
+        } else {
+            st_release(&status[tile], RTX_LB_AGG | agg);
+            int q = (int)tile - 1;
+            for (;;) {
+                uint32_t s = ld_acquire(&status[q]);
+                if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += s & RTX_LB_VAL;
+                if (s & RTX_LB_INC) break;
+                --q;
+            }
+            st_release(&status[tile], RTX_LB_INC | (prefix + agg));
+        }
+        s_prefix = prefix;
+        if (tile == ntiles - 1) out_cnt[1] = prefix + agg;
+    }
+    __syncthreads();
+    if (!cnt) return;
+    uint32_t nid = s_prefix + wofs + excl;
+    if ((uint64_t)nid + 2u >= (uint64_t)cap) {  // src/kernel/secondary-ray-generator.cl:61-63
+        atomicMin(&out_cnt[2], nid);
+        return;
+    }
+    float4 hpk = hp[k];
+    const int32_t px = pix_in[k];
+    const float ck = con_in[k];
+    if (c & 1u) {
+        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
+        dir_out[nid] = make_float4(refr.x, refr.y, refr.z, 1e37f);
+        pix_out[nid] = px;
+        con_out[nid] = ck * reflectiveness * (1.f - R);
+        ++nid;
+    }
+    if (c & 2u) {
+        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
+        dir_out[nid] = make_float4(refl.x, refl.y, refl.z, 1e37f);
+        pix_out[nid] = px;
+        con_out[nid] = ck * reflectiveness * R;
+    }
+}
+
+// Output count = rays actually written (a prefix): min(scan total, first
+// dropped offset).  Optionally accumulates the frame's secondary-ray count.
+__global__ void k_sec_finalize(uint32_t* out_cnt, unsigned long long* acc) {
+    uint32_t total = out_cnt[1], drop = out_cnt[2];
+    uint32_t n = total < drop ? total : drop;
+    out_cnt[0] = n;
+    if (acc) *acc += n;
+}
+
+}  // namespace
+
+int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
+                                  uint64_t* dev_total_acc, hipStream_t st) {
+    RTX_REQUIRE(s && in && hits && out, RTX_EINVAL, "rtx_raygen_secondary: null argument");
+    RTX_REQUIRE(in != out, RTX_EINVAL, "rtx_raygen_secondary: input and output bundles must differ");
+    RTX_REQUIRE(s->mats.p && s->matmap.p, RTX_ESTATE, "rtx_raygen_secondary: scene has no materials");
+    RTX_REQUIRE(count_in == RTX_DEVICE_COUNT || (count_in <= in->cap && count_in <= hits->cap), RTX_EINVAL,
+                "rtx_raygen_secondary: count exceeds bundle capacity");
+    uint32_t max_tiles = rtx_div_up(in->cap, SB) + 1;
+    {
+        ProfScope ps(s->ctx, st, "sec_init");
+        uint32_t g = rtx_div_up(max_tiles, 256);
+        hipLaunchKernelGGL(k_sec_init, dim3(g > 256 ? 256 : g), dim3(256), 0, st, in->scan_status.as<uint32_t>(),
+                           max_tiles, in->count.as<uint32_t>(), out->count.as<uint32_t>());
+    }
+    {
+        ProfScope ps(s->ctx, st, "sec_generate");
+        uint64_t n = count_in == RTX_DEVICE_COUNT ? in->cap : count_in;
+        uint32_t grid = rtx_div_up(n ? n : 1, SB);
+        hipLaunchKernelGGL(k_secondary, dim3(grid), dim3(SB), 0, st, in->dir4.as<float4>(), in->pixel.as<int32_t>(),
+                           in->contrib.as<float>(), hits->flags.as<uint32_t>(), hits->id.as<int32_t>(),
+                           hits->n4.as<float4>(), hits->p4.as<float4>(), s->mats.as<rtx_material>(),
+                           s->matmap.as<int32_t>(), in->count.as<uint32_t>(), count_in, out->ori4.as<float4>(),
+                           out->dir4.as<float4>(), out->pixel.as<int32_t>(), out->contrib.as<float>(),
+                           out->count.as<uint32_t>(), in->scan_status.as<uint32_t>(), (uint32_t)out->cap);
+    }
+    {
+        ProfScope ps(s->ctx, st, "sec_finalize");
+        hipLaunchKernelGGL(k_sec_finalize, dim3(1), dim3(1), 0, st, out->count.as<uint32_t>(),
+                           (unsigned long long*)dev_total_acc);
+    }
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_raygen_secondary(rtx_scene* s, rtx_rays* rays_in, rtx_hits* hits, rtx_rays* rays_out, uint64_t count_in,
+                             uint64_t* count_out, void* stream) {
+    RTX_REQUIRE(s, RTX_EINVAL, "rtx_raygen_secondary: null scene");
+    hipStream_t st = s->ctx->pick(stream);
+    int32_t e = rtx_raygen_secondary_impl(s, rays_in, hits, rays_out, count_in, nullptr, st);
+    if (e) return e;
+    if (count_out) {
+        uint32_t c = 0;
+        RTX_CHECK_HIP(hipMemcpyAsync(&c, rays_out->count.p, 4, hipMemcpyDeviceToHost, st));
+        RTX_CHECK_HIP(hipStreamSynchronize(st));
+        *count_out = c;
+    }
+    return RTX_OK;
+}

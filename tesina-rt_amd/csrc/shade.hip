@@ -1,0 +1,164 @@
+// Ray shading and framebuffer.
+//
+// Reference: RayShader::shade (src/rt/ray-shader.cpp:33-154) with shade_sample
+// (src/kernel/ray-shader.cl:116-333), FrameBuffer clear/copy
+// (src/rt/framebuffer.cpp:118-165; src/kernel/framebuffer.cl:21-54).
+// Accumulation is integer (floor(c*1e6) per channel), so atomics make it
+// exact and order-free; shade_secondary's run-head serial loop
+// (src/kernel/ray-shader.cl:415-457) is not needed for that.
+#include "devmath.hpp"
+#include "rtx_impl.hpp"
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, const int32_t* __restrict__ pixel,
+                                               const float* __restrict__ contrib, const uint32_t* __restrict__ flags,
+                                               const int32_t* __restrict__ hid, const float4* __restrict__ hn,
+                                               const float4* __restrict__ hp, const rtx_material* __restrict__ mats,
+                                               const int32_t* __restrict__ matmap,
+                                               const rtx_lights* __restrict__ lights, const uint32_t* dcount,
+                                               uint64_t hcount, uint32_t* __restrict__ accum) {
+    const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
+    const int nl = lights->count;
+    const v3 amb = mk3(lights->ambient[0], lights->ambient[1], lights->ambient[2]);
+    for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
+        uint32_t fl = flags[r];
+        if (!(fl & 1u)) continue;  // a miss adds nothing without a cubemap
+        const rtx_material m = mats[matmap[hid[r]]];
+        v3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        v3 d = xyz(dir4[r]);
+        v3 nn = xyz(hn[r]);
+        v3 p = xyz(hp[r]);
+        v3 val = amb * diffuse;
+        for (int l = 0; l < nl; ++l) {
+            const rtx_light& L = lights->l[l];
+            v3 Ld;
+            float LD;
+            if (L.type == RTX_LIGHT_DIR) {
+                Ld = mk3(L.dir[0], L.dir[1], L.dir[2]);
+                LD = 1.f;
+            } else if (L.type == RTX_LIGHT_SPOT) {
+                Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
+                LD = dot(Ld, mk3(L.dir[0], L.dir[1], L.dir[2])) - L.angle;
+                LD /= 1.f - L.angle;
+                LD = clamp01(LD);
+            } else {
+                Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
+                LD = 1.f;
+            }
+            v3 rgb = mk3(L.color[0], L.color[1], L.color[2]) * LD;
+            if (fl & (1u << (8 + l))) continue;
+            v3 rr = d - 2.f * nn * dot(d, nn);
+            float cosL = clamp01(dot(nn, -Ld));
+            cosL = cosL * cosL;
+            float spec = clamp01(dot(rr, -Ld));
+            float s2 = spec * spec, s4 = s2 * s2;
+            spec = s4 * s4;
+            spec *= cosL * m.shininess;
+            val = val + rgb * diffuse * cosL;
+            val = val + rgb * spec;
+        }
+        val = val * (1.f - m.reflectiveness);
+        const float c = contrib[r];
+        const uint32_t px = (uint32_t)pixel[r];
+        float ch[3] = {val.x * c, val.y * c, val.z * c};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            uint32_t u = (uint32_t)floorf(clamp01(ch[q]) * 1e6f);
+            if (u) atomicAdd(&accum[3 * (size_t)px + q], u);
+        }
+    }
+}
+
+__global__ void k_resolve(const uint32_t* __restrict__ accum, uchar4* __restrict__ rgba, uint32_t P) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float inv = 1.f / 1e6f;
+    uint8_t c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float f = (float)accum[3 * (size_t)i + q] * inv;
+        f = clamp01(f) * 255.f;
+        c[q] = (uint8_t)rintf(f);
+    }
+    rgba[i] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
+}  // namespace
+
+int32_t rtx_fb_create(rtx_ctx* ctx, int32_t width, int32_t height, rtx_fb** out) {
+    RTX_REQUIRE(ctx && out && width > 0 && height > 0, RTX_EINVAL, "rtx_fb_create: bad argument");
+    rtx_fb* f = new rtx_fb();
+    f->ctx = ctx;
+    f->w = width;
+    f->h = height;
+    size_t P = (size_t)width * height;
+    int32_t e = 0;
+    if ((e = f->accum.reserve(ctx, P * 12)) || (e = f->rgba.reserve(ctx, P * 4))) {
+        rtx_fb_destroy(f);
+        return e;
+    }
+    *out = f;
+    return rtx_fb_clear(f, nullptr);
+}
+
+int32_t rtx_fb_destroy(rtx_fb* fb) {
+    RTX_REQUIRE(fb, RTX_EINVAL, "rtx_fb_destroy: null");
+    (void)hipStreamSynchronize(fb->ctx->stream);
+    fb->accum.release();
+    fb->rgba.release();
+    delete fb;
+    return RTX_OK;
+}
+
+int32_t rtx_fb_clear(rtx_fb* fb, void* stream) {
+    RTX_REQUIRE(fb, RTX_EINVAL, "rtx_fb_clear: null");
+    return rtx_fill_u32(fb->ctx, fb->accum.p, 0u, 3ull * fb->w * fb->h, fb->ctx->pick(stream));
+}
+
+int32_t rtx_fb_resolve(rtx_fb* fb, void* stream) {
+    RTX_REQUIRE(fb, RTX_EINVAL, "rtx_fb_resolve: null");
+    hipStream_t st = fb->ctx->pick(stream);
+    uint32_t P = (uint32_t)fb->w * fb->h;
+    ProfScope ps(fb->ctx, st, "fb_resolve");
+    hipLaunchKernelGGL(k_resolve, dim3(rtx_div_up(P, 256)), dim3(256), 0, st, fb->accum.as<uint32_t>(),
+                       fb->rgba.as<uchar4>(), P);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_fb_read_accum(rtx_fb* fb, uint32_t* rgb3) {
+    RTX_REQUIRE(fb && rgb3, RTX_EINVAL, "rtx_fb_read_accum: null");
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(rgb3, fb->accum.p, (size_t)fb->w * fb->h * 12, hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+int32_t rtx_fb_read_rgba8(rtx_fb* fb, uint8_t* rgba) {
+    RTX_REQUIRE(fb && rgba, RTX_EINVAL, "rtx_fb_read_rgba8: null");
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(rgba, fb->rgba.p, (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint64_t count, int32_t primary,
+                  void* stream) {
+    (void)primary;
+    RTX_REQUIRE(s && rays && hits && fb, RTX_EINVAL, "rtx_shade: null argument");
+    RTX_REQUIRE(s->mats.p && s->lights.p, RTX_ESTATE, "rtx_shade: scene has no materials or lights");
+    RTX_REQUIRE(count == RTX_DEVICE_COUNT || (count <= rays->cap && count <= hits->cap), RTX_EINVAL,
+                "rtx_shade: count exceeds bundle capacity");
+    hipStream_t st = s->ctx->pick(stream);
+    if (count == 0) return RTX_OK;
+    uint64_t n = count == RTX_DEVICE_COUNT ? rays->cap : count;
+    uint64_t g = (n + 255) / 256;
+    if (g > (uint64_t)s->ctx->cus * 32) g = (uint64_t)s->ctx->cus * 32;
+    ProfScope ps(s->ctx, st, "shade");
+    hipLaunchKernelGGL(k_shade, dim3((uint32_t)g), dim3(256), 0, st, rays->dir4.as<float4>(), rays->pixel.as<int32_t>(),
+                       rays->contrib.as<float>(), hits->flags.as<uint32_t>(), hits->id.as<int32_t>(),
+                       hits->n4.as<float4>(), hits->p4.as<float4>(), s->mats.as<rtx_material>(),
+                       s->matmap.as<int32_t>(), s->lights.as<rtx_lights>(), rays->count.as<uint32_t>(), count,
+                       fb->accum.as<uint32_t>());
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}

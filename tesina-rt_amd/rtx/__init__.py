@@ -1,0 +1,516 @@
+"""Python binding of librtx.so (the gfx950 ray-tracing hot path) over its C ABI.
+
+Thin ctypes layer: every call goes to the HIP library; there is no CPU
+fallback.  Loading fails loudly if librtx.so has not been built
+(python tesina-rt_amd/build.py).
+
+Object model mirrors the reference's stage classes (include/rt/*.hpp):
+Scene (geometry, materials, lights, LBVH), Rays (RayBundle), Hits
+(HitBundle), FrameBuffer, Renderer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("RTX_LIB", PKG / "librtx.so"))
+
+DEVICE_COUNT = (1 << 64) - 1  # RTX_DEVICE_COUNT
+
+LIGHT_SPOT, LIGHT_DIR, LIGHT_POINT = 0, 1, 2
+MAX_LIGHTS = 8
+
+# FrameStats stage ids (include/rt/frame-stats.hpp:5-17)
+STAGES = ["BVH_BUILD", "PRIM_RAY_GEN", "SEC_RAY_GEN", "PRIM_TRACE", "PRIM_SHADOW_TRACE", "SEC_TRACE",
+          "SEC_SHADOW_TRACE", "SHADE", "FB_CLEAR", "FB_COPY"]
+
+
+class RtxError(RuntimeError):
+    pass
+
+
+class Material(C.Structure):
+    _fields_ = [("diffuse", C.c_float * 4), ("shininess", C.c_float), ("reflectiveness", C.c_float),
+                ("refractive_index", C.c_float), ("texture", C.c_int32)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pos", C.c_float * 3), ("dir", C.c_float * 3), ("color", C.c_float * 3),
+                ("angle", C.c_float), ("radius", C.c_float)]
+
+
+class Lights(C.Structure):
+    _fields_ = [("ambient", C.c_float * 3), ("count", C.c_int32), ("l", Light * MAX_LIGHTS)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3)]
+
+
+class LbvhParams(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("min_leaf", C.c_int32)]
+
+
+class NodeRef(C.Structure):
+    _fields_ = [("hi", C.c_float * 4), ("lo", C.c_float * 4), ("a", C.c_uint32), ("b", C.c_uint32),
+                ("parent", C.c_uint32), ("split_axis", C.c_int8), ("leaf", C.c_int8), ("pad", C.c_uint8 * 2)]
+
+
+NODE_DTYPE = np.dtype([("hi", "<f4", 4), ("lo", "<f4", 4), ("a", "<u4"), ("b", "<u4"), ("parent", "<u4"),
+                       ("split_axis", "i1"), ("leaf", "i1"), ("pad", "u1", 2)])
+assert NODE_DTYPE.itemsize == 48
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("secondary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("bounces_run", C.c_uint32), ("node_count", C.c_uint32), ("stage_ms", C.c_double * 10),
+                ("frame_ms", C.c_double)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("quad_size", C.c_int32), ("max_bounces", C.c_int32),
+                ("depth", C.c_int32), ("min_leaf", C.c_int32), ("ray_capacity", C.c_uint64)]
+
+
+# Every symbol include/rtx.h declares (checked by tests/test_abi.py).
+SYMBOLS = [
+    "rtx_init", "rtx_shutdown", "rtx_last_error", "rtx_abi_version", "rtx_stream", "rtx_sync",
+    "rtx_profile_enable", "rtx_profile_report", "rtx_profile_reset", "rtx_mem_in_use",
+    "rtx_scene_create", "rtx_scene_destroy", "rtx_scene_set_vertices", "rtx_scene_set_vertices_ref",
+    "rtx_scene_update_positions", "rtx_scene_set_triangles", "rtx_scene_set_materials", "rtx_scene_set_lights",
+    "rtx_scene_triangle_count", "rtx_scene_displace_waves",
+    "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info",
+    "rtx_read_codes", "rtx_read_perm", "rtx_read_indices", "rtx_read_matmap", "rtx_read_orig_ids",
+    "rtx_read_global_bbox", "rtx_read_nodes_ref",
+    "rtx_rays_create", "rtx_rays_destroy", "rtx_rays_capacity", "rtx_rays_count", "rtx_rays_read", "rtx_rays_write",
+    "rtx_hits_create", "rtx_hits_destroy", "rtx_hits_read", "rtx_hits_write",
+    "rtx_raygen_primary", "rtx_trace", "rtx_trace_shadow", "rtx_raygen_secondary",
+    "rtx_fb_create", "rtx_fb_destroy", "rtx_fb_clear", "rtx_fb_resolve", "rtx_fb_read_accum", "rtx_fb_read_rgba8",
+    "rtx_shade",
+    "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
+    "rtx_renderer_last_stats",
+    "rtx_camera_set",
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load librtx.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RtxError(f"{LIB_PATH} not built; run python tesina-rt_amd/build.py")
+    L = C.CDLL(str(LIB_PATH))
+    P, I32, U32, U64, F = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64, C.c_float
+    sig = {
+        "rtx_init": (I32, [I32, C.POINTER(P)]),
+        "rtx_shutdown": (I32, [P]),
+        "rtx_last_error": (C.c_char_p, []),
+        "rtx_abi_version": (I32, []),
+        "rtx_stream": (P, [P]),
+        "rtx_sync": (I32, [P, P]),
+        "rtx_profile_enable": (I32, [P, I32]),
+        "rtx_profile_report": (C.c_int64, [P, C.c_char_p, U64]),
+        "rtx_profile_reset": (I32, [P]),
+        "rtx_mem_in_use": (U64, [P]),
+        "rtx_scene_create": (I32, [P, C.POINTER(P)]),
+        "rtx_scene_destroy": (I32, [P]),
+        "rtx_scene_set_vertices": (I32, [P, P, P, P, U32]),
+        "rtx_scene_set_vertices_ref": (I32, [P, P, U32]),
+        "rtx_scene_update_positions": (I32, [P, P, P]),
+        "rtx_scene_set_triangles": (I32, [P, P, P, U32]),
+        "rtx_scene_set_materials": (I32, [P, P, U32]),
+        "rtx_scene_set_lights": (I32, [P, P]),
+        "rtx_scene_triangle_count": (U32, [P]),
+        "rtx_scene_displace_waves": (I32, [P, P, I32, F, P]),
+        "rtx_lbvh_build": (I32, [P, P, P]),
+        "rtx_lbvh_refit": (I32, [P, P]),
+        "rtx_lbvh_info": (I32, [P, P, P]),
+        "rtx_read_codes": (I32, [P, P, U32]),
+        "rtx_read_perm": (I32, [P, P, U32]),
+        "rtx_read_indices": (I32, [P, P, U32]),
+        "rtx_read_matmap": (I32, [P, P, U32]),
+        "rtx_read_orig_ids": (I32, [P, P, U32]),
+        "rtx_read_global_bbox": (I32, [P, P, P]),
+        "rtx_read_nodes_ref": (I32, [P, P, U32, P]),
+        "rtx_rays_create": (I32, [P, U64, C.POINTER(P)]),
+        "rtx_rays_destroy": (I32, [P]),
+        "rtx_rays_capacity": (U64, [P]),
+        "rtx_rays_count": (I32, [P, C.POINTER(U64)]),
+        "rtx_rays_read": (I32, [P, P, P, P, P, U64]),
+        "rtx_rays_write": (I32, [P, P, P, P, P, U64]),
+        "rtx_hits_create": (I32, [P, U64, C.POINTER(P)]),
+        "rtx_hits_destroy": (I32, [P]),
+        "rtx_hits_read": (I32, [P, P, P, P, P, P, P, P, U64]),
+        "rtx_hits_write": (I32, [P, P, P, P, P, P, U64]),
+        "rtx_raygen_primary": (I32, [P, P, I32, I32, I32, I32, P, U64, U64, P]),
+        "rtx_trace": (I32, [P, P, P, U64, P]),
+        "rtx_trace_shadow": (I32, [P, P, P, U64, P]),
+        "rtx_raygen_secondary": (I32, [P, P, P, P, U64, C.POINTER(U64), P]),
+        "rtx_fb_create": (I32, [P, I32, I32, C.POINTER(P)]),
+        "rtx_fb_destroy": (I32, [P]),
+        "rtx_fb_clear": (I32, [P, P]),
+        "rtx_fb_resolve": (I32, [P, P]),
+        "rtx_fb_read_accum": (I32, [P, P]),
+        "rtx_fb_read_rgba8": (I32, [P, P]),
+        "rtx_shade": (I32, [P, P, P, P, U64, I32, P]),
+        "rtx_renderer_create": (I32, [P, P, C.POINTER(P)]),
+        "rtx_renderer_destroy": (I32, [P]),
+        "rtx_renderer_fb": (P, [P]),
+        "rtx_render_frame": (I32, [P, P, P, I32, P, P]),
+        "rtx_render_frame_async": (I32, [P, P, P, I32, P]),
+        "rtx_renderer_last_stats": (I32, [P, P]),
+        "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().rtx_last_error().decode(errors="replace")
+        raise RtxError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to librtx must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def camera_set(pos, look_at=None, up=(0.0, 1.0, 0.0), fov=np.pi / 4, aspect=1.0, right_handed=False,
+               direction=None) -> Camera:
+    """Camera::set (src/rt/camera.cpp:31-59) evaluated by the library (host code)."""
+    cam = Camera()
+    p = np.asarray(pos, np.float32)
+    d = np.asarray(direction if direction is not None else np.asarray(look_at, np.float32) - p, np.float32)
+    u = np.asarray(up, np.float32)
+    _check(lib().rtx_camera_set(C.byref(cam), _ptr(p), _ptr(d), _ptr(u), float(np.float32(fov)),
+                                float(np.float32(aspect)), int(right_handed)), "rtx_camera_set")
+    return cam
+
+
+def make_lights(ambient, lights) -> Lights:
+    """lights: iterable of dicts {type, pos, dir, color, angle, radius}."""
+    L = Lights()
+    L.ambient[:] = [float(x) for x in ambient]
+    lights = list(lights)
+    assert len(lights) <= MAX_LIGHTS
+    L.count = len(lights)
+    for i, d in enumerate(lights):
+        L.l[i].type = int(d.get("type", LIGHT_POINT))
+        L.l[i].pos[:] = [float(x) for x in d.get("pos", (0, 0, 0))]
+        L.l[i].dir[:] = [float(x) for x in d.get("dir", (0, -1, 0))]
+        L.l[i].color[:] = [float(x) for x in d.get("color", (1, 1, 1))]
+        L.l[i].angle = float(d.get("angle", 0.0))
+        L.l[i].radius = float(d.get("radius", 0.0))
+    return L
+
+
+def make_materials(rows) -> np.ndarray:
+    """rows: iterable of (r, g, b, shininess, reflectiveness, ior). Returns the 32-B material array."""
+    rows = list(rows)
+    arr = (Material * len(rows))()
+    for i, (r, g, b, sh, rf, ior) in enumerate(rows):
+        arr[i].diffuse[:] = [r, g, b, 0.0]
+        arr[i].shininess, arr[i].reflectiveness, arr[i].refractive_index = sh, rf, ior
+        arr[i].texture = -1
+    return np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        _check(lib().rtx_init(device, C.byref(self.h)), "rtx_init")
+
+    def close(self):
+        if self.h:
+            lib().rtx_shutdown(self.h)
+            self.h = C.c_void_p()
+
+    def sync(self):
+        _check(lib().rtx_sync(self.h, None), "rtx_sync")
+
+    @property
+    def stream(self):
+        return lib().rtx_stream(self.h)
+
+    def profile(self, on: bool = True):
+        _check(lib().rtx_profile_enable(self.h, int(on)), "rtx_profile_enable")
+
+    def profile_reset(self):
+        _check(lib().rtx_profile_reset(self.h), "rtx_profile_reset")
+
+    def profile_report(self) -> dict:
+        n = lib().rtx_profile_report(self.h, None, 0)
+        buf = C.create_string_buffer(int(n) + 16)
+        lib().rtx_profile_report(self.h, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    def mem_in_use(self) -> int:
+        return int(lib().rtx_mem_in_use(self.h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Scene:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().rtx_scene_create(ctx.h, C.byref(self.h)), "rtx_scene_create")
+        self.T = 0
+        self.V = 0
+
+    def destroy(self):
+        if self.h:
+            lib().rtx_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_geometry(self, pos, idx, nrm=None, uv=None, matmap=None):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        _check(lib().rtx_scene_set_vertices(self.h, _ptr(pos), _ptr(nrm), _ptr(uv), pos.shape[0]),
+               "rtx_scene_set_vertices")
+        mm = None if matmap is None else np.ascontiguousarray(matmap, np.int32)
+        _check(lib().rtx_scene_set_triangles(self.h, _ptr(idx), _ptr(mm), idx.shape[0]), "rtx_scene_set_triangles")
+        self.V, self.T = pos.shape[0], idx.shape[0]
+
+    def set_vertices_ref(self, vertex80: np.ndarray):
+        v = np.ascontiguousarray(vertex80, np.float32).reshape(-1, 20)
+        _check(lib().rtx_scene_set_vertices_ref(self.h, _ptr(v), v.shape[0]), "rtx_scene_set_vertices_ref")
+        self.V = v.shape[0]
+
+    def set_triangles(self, idx, matmap=None):
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        mm = None if matmap is None else np.ascontiguousarray(matmap, np.int32)
+        _check(lib().rtx_scene_set_triangles(self.h, _ptr(idx), _ptr(mm), idx.shape[0]), "rtx_scene_set_triangles")
+        self.T = idx.shape[0]
+
+    def update_positions(self, pos, stream=None):
+        self._staged = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        _check(lib().rtx_scene_update_positions(self.h, _ptr(self._staged), stream), "rtx_scene_update_positions")
+
+    def displace_waves(self, waves: np.ndarray, t: float, stream=None):
+        w = np.ascontiguousarray(waves, np.float32).reshape(-1, 6)
+        _check(lib().rtx_scene_displace_waves(self.h, _ptr(w), w.shape[0], float(t), stream),
+               "rtx_scene_displace_waves")
+
+    def set_materials(self, mats: np.ndarray):
+        mats = np.ascontiguousarray(mats, np.uint8)
+        _check(lib().rtx_scene_set_materials(self.h, _ptr(mats), mats.size // 32), "rtx_scene_set_materials")
+
+    def set_lights(self, lights: Lights):
+        _check(lib().rtx_scene_set_lights(self.h, C.byref(lights)), "rtx_scene_set_lights")
+
+    def build(self, depth: int = 30, min_leaf: int = 1, stream=None):
+        p = LbvhParams(depth, min_leaf)
+        _check(lib().rtx_lbvh_build(self.h, C.byref(p), stream), "rtx_lbvh_build")
+
+    def refit(self, stream=None):
+        _check(lib().rtx_lbvh_refit(self.h, stream), "rtx_lbvh_refit")
+
+    def info(self):
+        n = C.c_uint32()
+        lc = np.zeros(64, np.uint32)
+        _check(lib().rtx_lbvh_info(self.h, C.byref(n), _ptr(lc)), "rtx_lbvh_info")
+        return int(n.value), lc
+
+    def _read(self, fn, dtype, count):
+        out = np.empty(count, dtype)
+        _check(getattr(lib(), fn)(self.h, _ptr(out), self.T), fn)
+        return out
+
+    def codes(self):
+        return self._read("rtx_read_codes", np.uint32, self.T)
+
+    def perm(self):
+        return self._read("rtx_read_perm", np.uint32, self.T)
+
+    def indices(self):
+        return self._read("rtx_read_indices", np.uint32, 3 * self.T).reshape(-1, 3)
+
+    def matmap(self):
+        return self._read("rtx_read_matmap", np.int32, self.T)
+
+    def orig_ids(self):
+        return self._read("rtx_read_orig_ids", np.uint32, self.T)
+
+    def global_bbox(self):
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        _check(lib().rtx_read_global_bbox(self.h, _ptr(lo), _ptr(hi)), "rtx_read_global_bbox")
+        return lo, hi
+
+    def nodes(self) -> np.ndarray:
+        n = C.c_uint32()
+        _check(lib().rtx_read_nodes_ref(self.h, None, 0, C.byref(n)), "rtx_read_nodes_ref")
+        out = np.zeros(n.value, NODE_DTYPE)
+        _check(lib().rtx_read_nodes_ref(self.h, _ptr(out), n.value, C.byref(n)), "rtx_read_nodes_ref")
+        return out
+
+
+class Rays:
+    def __init__(self, ctx: Context, capacity: int):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().rtx_rays_create(ctx.h, int(capacity), C.byref(self.h)), "rtx_rays_create")
+        self.cap = int(capacity)
+
+    def destroy(self):
+        if self.h:
+            lib().rtx_rays_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def count(self) -> int:
+        c = C.c_uint64()
+        _check(lib().rtx_rays_count(self.h, C.byref(c)), "rtx_rays_count")
+        return int(c.value)
+
+    def read(self, n: int):
+        o = np.empty((n, 4), np.float32)
+        d = np.empty((n, 4), np.float32)
+        p = np.empty(n, np.int32)
+        c = np.empty(n, np.float32)
+        _check(lib().rtx_rays_read(self.h, _ptr(o), _ptr(d), _ptr(p), _ptr(c), n), "rtx_rays_read")
+        return o, d, p, c
+
+    def write(self, ori4, dir4, pixel=None, contrib=None):
+        o = np.ascontiguousarray(ori4, np.float32).reshape(-1, 4)
+        d = np.ascontiguousarray(dir4, np.float32).reshape(-1, 4)
+        p = None if pixel is None else np.ascontiguousarray(pixel, np.int32)
+        c = None if contrib is None else np.ascontiguousarray(contrib, np.float32)
+        _check(lib().rtx_rays_write(self.h, _ptr(o), _ptr(d), _ptr(p), _ptr(c), o.shape[0]), "rtx_rays_write")
+
+    def primary(self, cam: Camera, w: int, h: int, quad: int = 32, spp: int = 1, psamples=None, offset: int = 0,
+                count: int | None = None, stream=None):
+        ps = None if psamples is None else np.ascontiguousarray(psamples, np.float32)
+        count = w * h * spp - offset if count is None else count
+        _check(lib().rtx_raygen_primary(self.h, C.byref(cam), w, h, quad, spp, _ptr(ps), offset, count, stream),
+               "rtx_raygen_primary")
+
+
+class Hits:
+    def __init__(self, ctx: Context, capacity: int):
+        self.h = C.c_void_p()
+        _check(lib().rtx_hits_create(ctx.h, int(capacity), C.byref(self.h)), "rtx_hits_create")
+        self.cap = int(capacity)
+
+    def destroy(self):
+        if self.h:
+            lib().rtx_hits_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def read(self, n: int) -> dict:
+        out = dict(flags=np.empty(n, np.uint32), t=np.empty(n, np.float32), id=np.empty(n, np.int32),
+                   bary=np.empty((n, 2), np.float32), n4=np.empty((n, 4), np.float32),
+                   p4=np.empty((n, 4), np.float32), uv2=np.empty((n, 2), np.float32))
+        _check(lib().rtx_hits_read(self.h, *(_ptr(out[k]) for k in ("flags", "t", "id", "bary", "n4", "p4", "uv2")),
+                                   n), "rtx_hits_read")
+        return out
+
+    def write(self, flags, t, id, n4, p4):
+        arrs = [np.ascontiguousarray(flags, np.uint32), np.ascontiguousarray(t, np.float32),
+                np.ascontiguousarray(id, np.int32), np.ascontiguousarray(n4, np.float32).reshape(-1, 4),
+                np.ascontiguousarray(p4, np.float32).reshape(-1, 4)]
+        _check(lib().rtx_hits_write(self.h, *(_ptr(a) for a in arrs), arrs[0].shape[0]), "rtx_hits_write")
+
+
+class FrameBuffer:
+    def __init__(self, ctx: Context, w: int, h: int, handle=None):
+        self.w, self.h_ = w, h
+        self.owned = handle is None
+        self.h = C.c_void_p(handle) if handle is not None else C.c_void_p()
+        if self.owned:
+            _check(lib().rtx_fb_create(ctx.h, w, h, C.byref(self.h)), "rtx_fb_create")
+
+    def destroy(self):
+        if self.h and self.owned:
+            lib().rtx_fb_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def clear(self, stream=None):
+        _check(lib().rtx_fb_clear(self.h, stream), "rtx_fb_clear")
+
+    def resolve(self, stream=None):
+        _check(lib().rtx_fb_resolve(self.h, stream), "rtx_fb_resolve")
+
+    def accum(self) -> np.ndarray:
+        out = np.empty((self.w * self.h_, 3), np.uint32)
+        _check(lib().rtx_fb_read_accum(self.h, _ptr(out)), "rtx_fb_read_accum")
+        return out
+
+    def rgba8(self) -> np.ndarray:
+        out = np.empty((self.w * self.h_, 4), np.uint8)
+        _check(lib().rtx_fb_read_rgba8(self.h, _ptr(out)), "rtx_fb_read_rgba8")
+        return out
+
+
+def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+    _check(lib().rtx_trace(scene.h, rays.h, hits.h, count, stream), "rtx_trace")
+
+
+def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+    _check(lib().rtx_trace_shadow(scene.h, rays.h, hits.h, count, stream), "rtx_trace_shadow")
+
+
+def raygen_secondary(scene: Scene, rays_in: Rays, hits: Hits, rays_out: Rays, count_in=DEVICE_COUNT,
+                     read_count: bool = True, stream=None):
+    c = C.c_uint64()
+    _check(lib().rtx_raygen_secondary(scene.h, rays_in.h, hits.h, rays_out.h, count_in,
+                                      C.byref(c) if read_count else None, stream), "rtx_raygen_secondary")
+    return int(c.value) if read_count else None
+
+
+def shade(scene: Scene, rays: Rays, hits: Hits, fb: FrameBuffer, count=DEVICE_COUNT, primary=True, stream=None):
+    _check(lib().rtx_shade(scene.h, rays.h, hits.h, fb.h, count, int(primary), stream), "rtx_shade")
+
+
+class Renderer:
+    def __init__(self, ctx: Context, width: int, height: int, max_bounces: int, depth: int = 30, min_leaf: int = 1,
+                 quad: int = 32, ray_capacity: int = 0):
+        self.ctx = ctx
+        self.p = RenderParams(width, height, quad, max_bounces, depth, min_leaf, ray_capacity)
+        self.h = C.c_void_p()
+        _check(lib().rtx_renderer_create(ctx.h, C.byref(self.p), C.byref(self.h)), "rtx_renderer_create")
+        self.fb = FrameBuffer(ctx, width, height, handle=lib().rtx_renderer_fb(self.h))
+
+    def destroy(self):
+        if self.h:
+            lib().rtx_renderer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def frame(self, scene: Scene, cam: Camera, rebuild: bool = True, stream=None) -> FrameStats:
+        st = FrameStats()
+        _check(lib().rtx_render_frame(self.h, scene.h, C.byref(cam), int(rebuild), C.byref(st), stream),
+               "rtx_render_frame")
+        return st
+
+    def frame_async(self, scene: Scene, cam: Camera, rebuild: bool = True, stream=None):
+        _check(lib().rtx_render_frame_async(self.h, scene.h, C.byref(cam), int(rebuild), stream),
+               "rtx_render_frame_async")
+
+    def last_stats(self) -> FrameStats:
+        st = FrameStats()
+        _check(lib().rtx_renderer_last_stats(self.h, C.byref(st)), "rtx_renderer_last_stats")
+        return st
