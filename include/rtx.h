@@ -146,6 +146,15 @@ int64_t rtx_profile_report(rtx_ctx* ctx, char* buf, uint64_t cap);
 int32_t rtx_profile_reset(rtx_ctx* ctx);
 /* Device memory held by the library for this context, bytes. */
 uint64_t rtx_mem_in_use(rtx_ctx* ctx);
+/* Tuning knobs, set by the autotuner (no reference counterpart: the
+ * reference fixes its work-group sizes per device, src/gpu/function.cpp:322-385):
+ *   "trace_variant": 0 single-node traversal, 1 single node with the near-child
+ *                    delta loaded early, 2 sibling-pair traversal;
+ *   "trace_block":   64, 128 or 256 threads per traversal work-group.
+ * All variants give identical results. */
+int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
+/* Current value, or -1 for an unknown key. */
+int64_t rtx_tune_get(rtx_ctx* ctx, const char* key);
 
 /* --------------------------------------------------------------- scene -- */
 /* Scene::initialize / destroy (src/rt/scene.cpp:163-189). */

@@ -161,12 +161,13 @@ def trace(b: Build, nrm3, uv2, ori4, dir4, count_visits=False):
     h = empty_hits(n)
     nrm4 = pos4(np.asarray(nrm3, np.float32))
     uv = np.ascontiguousarray(uv2, np.float32)
-    visits = np.zeros(1, np.uint64)
+    visits = np.zeros(2, np.uint64)
     lib().orc_trace(P(b.pos4), P(nrm4), P(uv), P(b.idx), P(b.nodes), P(np.ascontiguousarray(ori4, np.float32)),
                     P(np.ascontiguousarray(dir4, np.float32)), C.c_uint64(n), P(h["flags"]), P(h["t"]), P(h["id"]),
                     P(h["bary"]), P(h["n4"]), P(h["p4"]), P(h["uv2"]), P(visits) if count_visits else None)
     if count_visits:
         h["visits"] = int(visits[0])
+        h["tri_tests"] = int(visits[1])
     return h
 
 
@@ -179,10 +180,11 @@ def trace_brute(b: Build, ori4, dir4):
     return ids, t
 
 
-def trace_shadow(b: Build, lights: np.ndarray, flags, p4):
+def trace_shadow(b: Build, lights: np.ndarray, flags, p4, visits=None):
+    """visits: optional uint64[2] array accumulating (nodes, triangle tests)."""
     f = np.ascontiguousarray(flags, np.uint32).copy()
     lib().orc_trace_shadow(P(b.pos4), P(b.idx), P(b.nodes), P(lights), C.c_uint64(f.size), P(f),
-                           P(np.ascontiguousarray(p4, np.float32)))
+                           P(np.ascontiguousarray(p4, np.float32)), P(visits))
     return f
 
 

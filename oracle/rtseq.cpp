@@ -540,6 +540,7 @@ struct Hit {
 // trace_ray: src/kernel/trace-bvh.cl:330-417.  Stack of 32, near child first
 // by dot(dir, r.lo - l.lo) > 0 (the adir branches all test that one scalar).
 Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit) {
+    // nvisit (optional): [0] nodes visited, [1] triangles tested
     Hit best{-1, r.tmax, 0.f, 0.f};
     float tmax = r.tmax;
     uint32_t stack[32];
@@ -551,6 +552,7 @@ Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_nod
         if (!bbox_hit(n, r, tmax)) {
             pop = true;
         } else if (n.leaf) {
+            if (nvisit) nvisit[1] += n.b - n.a;
             for (uint32_t i = n.a; i < n.b; ++i) {
                 float t, u, v;
                 if (!tri_hit(pos, idx, i, r, &t, &u, &v)) continue;
@@ -582,17 +584,19 @@ Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_nod
 
 // trace_shadow_ray: src/kernel/shadow-trace-bvh.cl:238-315 with leaf_hit_any
 // (:188-234): any hit with tMin < t < tMax.
-bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes) {
+bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit) {
     uint32_t stack[32];
     uint32_t level = 0, curr = 0;
     for (;;) {
         const orc_node& n = nodes[curr];
         bool pop;
+        if (nvisit) ++nvisit[0];
         if (!bbox_hit(n, r, r.tmax)) {
             pop = true;
         } else if (n.leaf) {
             for (uint32_t i = n.a; i < n.b; ++i) {
                 float t, u, v;
+                if (nvisit) ++nvisit[1];
                 if (!tri_hit(pos, idx, i, r, &t, &u, &v)) continue;
                 if (t < r.tmax && t > r.tmin) return true;
             }
@@ -727,7 +731,7 @@ static int shadow_ray_for(const orc_light& L, f3 p, Ray* out) {
 }
 
 void orc_trace_shadow(const float* pos, const uint32_t* idx, const orc_node* nodes, const orc_lights* lights,
-                      uint64_t count, uint32_t* flags, const float* p4) {
+                      uint64_t count, uint32_t* flags, const float* p4, uint64_t* visits) {
     for (uint64_t k = 0; k < count; ++k) {
         uint32_t fl = flags[k] & 0xffu;
         if (fl & ORC_HIT) {
@@ -735,7 +739,7 @@ void orc_trace_shadow(const float* pos, const uint32_t* idx, const orc_node* nod
             for (int l = 0; l < lights->count; ++l) {
                 Ray r;
                 int s = shadow_ray_for(lights->l[l], p, &r);
-                if (s < 0) s = trace_shadow(r, pos, idx, nodes) ? 1 : 0;
+                if (s < 0) s = trace_shadow(r, pos, idx, nodes, visits) ? 1 : 0;
                 if (s) fl |= 1u << (8 + l);
             }
         }
@@ -975,7 +979,7 @@ uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const 
     };
     orc_trace(pos, nrm, uvs, idx, nodes, o_in, d_in, n, fl.data(), t.data(), id.data(), nullptr, n4.data(), p4.data(),
               uv2.data(), nullptr);
-    orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data());
+    orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data(), nullptr);
     count_shadow(n);
     orc_shade(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), n, accum);
     for (int b = 0; b < max_bounces; ++b) {
@@ -992,7 +996,7 @@ uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const 
         s.bounces_run++;
         orc_trace(pos, nrm, uvs, idx, nodes, o_in, d_in, n, fl.data(), t.data(), id.data(), nullptr, n4.data(),
                   p4.data(), uv2.data(), nullptr);
-        orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data());
+        orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data(), nullptr);
         count_shadow(n);
         orc_shade(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), n, accum);
     }

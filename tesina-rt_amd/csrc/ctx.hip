@@ -225,6 +225,31 @@ int32_t rtx_sync(rtx_ctx* ctx, void* stream) {
 
 uint64_t rtx_mem_in_use(rtx_ctx* ctx) { return ctx ? ctx->bytes_in_use : 0; }
 
+int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
+    RTX_REQUIRE(ctx && key, RTX_EINVAL, "rtx_tune_set: null argument");
+    std::string k(key);
+    if (k == "trace_variant") {
+        RTX_REQUIRE(value >= 0 && value <= 2, RTX_EINVAL, "rtx_tune_set: trace_variant must be 0, 1 or 2");
+        ctx->tune.trace_variant = value;
+    } else if (k == "trace_block") {
+        RTX_REQUIRE(value == 64 || value == 128 || value == 256, RTX_EINVAL,
+                    "rtx_tune_set: trace_block must be 64, 128 or 256");
+        ctx->tune.trace_block = value;
+    } else {
+        rtx_set_error("rtx_tune_set: unknown key " + k);
+        return RTX_EINVAL;
+    }
+    return RTX_OK;
+}
+
+int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
+    if (!ctx || !key) return -1;
+    std::string k(key);
+    if (k == "trace_variant") return ctx->tune.trace_variant;
+    if (k == "trace_block") return ctx->tune.trace_block;
+    return -1;
+}
+
 // ---------------------------------------------------------------- scene --
 int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out) {
     RTX_REQUIRE(ctx && out, RTX_EINVAL, "rtx_scene_create: null argument");
@@ -240,7 +265,7 @@ int32_t rtx_scene_destroy(rtx_scene* s) {
     DevBuf* all[] = {&s->pos,      &s->nrm,        &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
                      &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
                      &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
-                     &s->levels,   &s->refit_cnt,  &s->bstart,    &s->bend,        &s->blevel,    &s->keys_alt,
+                     &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
                      &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets,
                      &s->hier_status, &s->waves,      &s->stage};
     for (DevBuf* b : all) b->release();

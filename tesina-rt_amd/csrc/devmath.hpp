@@ -51,13 +51,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-// Device-scope atomics for cross-workgroup protocols (look-back, refit).
-__device__ __forceinline__ uint32_t ld_acquire(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Device-coherent atomics for the look-back protocol.  A status word carries
+// its value, so relaxed agent-scope accesses suffice: no release/acquire
+// fence (on gfx950 those write back / invalidate the XCD's whole L2).
+// Results written next to the protocol are consumed only by later launches.
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t fetch_add_acqrel(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -21,6 +21,7 @@ constexpr int BS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = BS * SORT_ITEMS;
 constexpr int BOUNDS_GRID = 512;
+constexpr uint32_t FAT_LEAF = 16;
 
 // Counter slots in scene->tickets.
 constexpr int TK_BOUNDS = 0;
@@ -65,14 +66,14 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, u
 // Word of tile t lives at status[t * stride].
 __device__ uint32_t lookback(uint32_t* status, uint32_t tile, uint32_t agg, uint32_t stride) {
     if (tile == 0) {
-        st_release(&status[0], RTX_LB_INC | agg);
+        st_relaxed(&status[0], RTX_LB_INC | agg);
         return 0;
     }
-    st_release(&status[(size_t)tile * stride], RTX_LB_AGG | agg);
+    st_relaxed(&status[(size_t)tile * stride], RTX_LB_AGG | agg);
     uint32_t acc = 0;
     int k = (int)tile - 1;
     for (;;) {
-        uint32_t s = ld_acquire(&status[(size_t)k * stride]);
+        uint32_t s = ld_relaxed(&status[(size_t)k * stride]);
         if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
             __builtin_amdgcn_s_sleep(1);
             continue;
@@ -81,15 +82,14 @@ __device__ uint32_t lookback(uint32_t* status, uint32_t tile, uint32_t agg, uint
         if (s & RTX_LB_INC) break;
         --k;
     }
-    st_release(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
+    st_relaxed(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
     return acc;
 }
 
 // -------------------------------------------------------------- kernels --
 __global__ __launch_bounds__(BS) void k_build_init(uint32_t* zero_a, uint32_t na, uint32_t* zero_b, uint32_t nb,
                                                    uint32_t* zero_c, uint32_t nc, uint32_t* levels, uint32_t* bstart,
-                                                   uint32_t* bend, uint32_t* blevel, uint32_t* parent, uint32_t T,
-                                                   int set_root) {
+                                                   uint32_t* bend, uint32_t* parent, uint32_t T, int set_root) {
     uint32_t stride = gridDim.x * BS;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < na; i += stride) zero_a[i] = 0;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < nb; i += stride) zero_b[i] = 0;
@@ -101,7 +101,6 @@ __global__ __launch_bounds__(BS) void k_build_init(uint32_t* zero_a, uint32_t na
             levels[LV_TOTAL] = 1;
             bstart[0] = 0;
             bend[0] = T;
-            blevel[0] = 0;
             parent[0] = 0;
         }
     }
@@ -113,9 +112,8 @@ __global__ __launch_bounds__(BS) void k_build_init(uint32_t* zero_a, uint32_t na
 // per-block partials).
 __global__ __launch_bounds__(BS) void k_prim_bounds(const uint32_t* __restrict__ idx, const float4* __restrict__ pos,
                                                     float4* __restrict__ cen, float* __restrict__ partials,
-                                                    uint32_t* done, float* gbox6, uint32_t T) {
+                                                    uint32_t T) {
     __shared__ float s_r[6][BS / RTX_WAVE];
-    __shared__ int s_last;
     v3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < T; i += gridDim.x * BS) {
         uint32_t i0 = idx[3 * (size_t)i], i1 = idx[3 * (size_t)i + 1], i2 = idx[3 * (size_t)i + 2];
@@ -148,16 +146,15 @@ __global__ __launch_bounds__(BS) void k_prim_bounds(const uint32_t* __restrict__
             for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], s_r[k][ww]);
         }
         for (int k = 0; k < 6; ++k) partials[6 * blockIdx.x + k] = r[k];
-        __threadfence();
-        uint32_t old = fetch_add_acqrel(done, 1u);
-        s_last = (old == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // last block: reduce all partials
+}
+
+// Union of the per-block partial AABBs (one block).
+__global__ __launch_bounds__(BS) void k_gbox_reduce(const float* __restrict__ partials, uint32_t nparts,
+                                                    float* __restrict__ gbox6) {
+    __shared__ float s_r[6][BS / RTX_WAVE];
     float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += BS) {
+    for (uint32_t b = threadIdx.x; b < nparts; b += BS) {
         for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], partials[6 * b + k]);
         for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], partials[6 * b + k]);
     }
@@ -168,7 +165,7 @@ __global__ __launch_bounds__(BS) void k_prim_bounds(const uint32_t* __restrict__
 #pragma unroll
         for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
     }
-    __syncthreads();
+    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < 6; ++k) s_r[k][w] = q[k];
     __syncthreads();
@@ -354,18 +351,68 @@ __global__ __launch_bounds__(BS) void k_tri_records(const uint32_t* __restrict__
 }
 
 // One iteration of process_task (src/kernel/bvh-builder.cl:582-671) over the
-// nodes of level d, in order.  find_middle (:484-552) reduces to: leaf if
-// count <= min_leaf+2 (unsigned test), or no bit <= the node's level bit
-// differs between its first and last code; else the split level is the
-// highest differing bit b and the split is the first index with bit b set.
-// Children of the k-th splitting node of the level go to
-// off_next + 2k (ordered compaction by decoupled look-back).
-__global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ codes, uint32_t* __restrict__ bstart,
-                                                   uint32_t* __restrict__ bend, uint32_t* __restrict__ blevel,
-                                                   uint32_t* __restrict__ node_u32, uint32_t* __restrict__ parent,
-                                                   uint32_t* __restrict__ refit_cnt, uint32_t* levels,
-                                                   uint32_t* status, uint32_t* ticket, int d, int depth, int min_leaf,
-                                                   int force_leaf) {
+// nodes of level d, in order.  Children of the k-th splitting node of the
+// level go to off_next + 2k (ordered compaction by decoupled look-back),
+// which is the order in-order work-groups of the reference produce.
+// Binary radix tree over the sorted codes (Karras 2012, "Maximizing
+// parallelism in the construction of BVHs, octrees and k-d trees"), keys
+// made unique by their position.  For internal node i it stores the other
+// end j of its range and the split gamma (left = [min, gamma], right =
+// [gamma+1, max]).  The reference splits a range [s, e) with differing end
+// codes at the highest differing bit, at the first index with that bit set
+// (find_middle, src/kernel/bvh-builder.cl:484-552): the same split as the
+// radix-tree node whose range is [s, e-1], and that node's index is s or e-1.
+__device__ __forceinline__ int rt_delta(const uint32_t* __restrict__ codes, int T, int i, int j) {
+    if (j < 0 || j >= T) return -1;
+    const uint32_t x = codes[i] ^ codes[j];
+    return x ? __builtin_clz(x) : 32 + __builtin_clz((uint32_t)(i ^ j));
+}
+
+__global__ __launch_bounds__(BS) void k_radix_tree(const uint32_t* __restrict__ codes, uint32_t* __restrict__ kj,
+                                                   uint32_t* __restrict__ kg, int T) {
+    const int i = (int)(blockIdx.x * BS + threadIdx.x);
+    if (i >= T - 1) return;
+    const int d = rt_delta(codes, T, i, i + 1) - rt_delta(codes, T, i, i - 1) > 0 ? 1 : -1;
+    const int dmin = rt_delta(codes, T, i, i - d);
+    int lmax = 2;
+    while (rt_delta(codes, T, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (rt_delta(codes, T, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = rt_delta(codes, T, i, j);
+    int sp = 0;
+    for (int t = l;;) {
+        t = (t + 1) >> 1;
+        if (rt_delta(codes, T, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t <= 1) break;
+    }
+    const int gamma = i + sp * d + (d < 0 ? -1 : 0);
+    kj[i] = (uint32_t)j;
+    kg[i] = (uint32_t)gamma;
+}
+
+__device__ __forceinline__ void leaf_bounds(const float4* __restrict__ tri, float4* __restrict__ node, uint32_t n,
+                                            uint32_t a, uint32_t cnt) {
+    v3 lo = xyz(tri[3 * (size_t)a]), hi = lo;
+    for (uint32_t t = a; t < a + cnt; ++t) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v3 p = xyz(tri[3 * (size_t)t + q]);
+            lo = vmin(lo, p);
+            hi = vmax(hi, p);
+        }
+    }
+    node[2 * (size_t)n] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(a));
+    node[2 * (size_t)n + 1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(cnt));
+}
+
+__global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ kj,
+                                                   const uint32_t* __restrict__ kg, uint32_t* __restrict__ bstart,
+                                                   uint32_t* __restrict__ bend, float4* __restrict__ node,
+                                                   uint32_t* __restrict__ parent, const float4* __restrict__ tri,
+                                                   uint32_t* levels, uint32_t* status, uint32_t* ticket, int d,
+                                                   uint32_t T, int min_leaf, int force_leaf) {
     __shared__ uint32_t s_tile, s_prefix;
     __shared__ uint32_t s_w[BS / RTX_WAVE];
     const uint32_t off = levels[LV_OFF + d];
@@ -387,31 +434,20 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
     const uint32_t k = tile * BS + tid;
     const bool active = k < cnt;
     const uint32_t n = off + k;
-    uint32_t start = 0, end = 0, split = 0, lvl_child = 0;
+    uint32_t start = 0, end = 0, split = 0;
     if (active) {
         start = bstart[n];
         end = bend[n];
-        uint32_t lvl = blevel[n];
-        uint32_t count = end - start;
-        if (!force_leaf && count - 2u > (uint32_t)min_leaf) {
-            int bit = depth - 1 - (int)lvl;
-            if (bit >= 0) {
-                uint32_t mask = bit >= 31 ? 0xffffffffu : ((2u << bit) - 1u);
-                uint32_t x = (codes[start] ^ codes[end - 1]) & mask;
-                if (x) {
-                    int b = 31 - __builtin_clz(x);
-                    uint32_t lo = start, hi = end - 1;
-                    while (lo + 1 < hi) {
-                        uint32_t mid = (lo + hi) / 2;
-                        if (!((codes[mid] >> b) & 1u))
-                            lo = mid;
-                        else
-                            hi = mid;
-                    }
-                    split = hi;
-                    lvl_child = (uint32_t)(depth - 1 - b) + 1u;
-                }
-            }
+        // Leaf iff count <= min_leaf + 2 (unsigned test, so a single triangle
+        // falls through) or the first and last codes are equal: find_middle
+        // advances its level while first and last share the level's bit, so it
+        // ends at their highest differing bit, and runs out of bits only when
+        // the codes are equal (every node's level bit is above all bits in
+        // which its codes differ).
+        if (!force_leaf && end - start - 2u > (uint32_t)min_leaf && codes[start] != codes[end - 1]) {
+            const uint32_t last = end - 1;
+            const uint32_t other = (last <= T - 2u) ? last : start;
+            split = (kj[start] == last ? kg[start] : kg[other]) + 1u;
         }
     }
     const bool is_split = split != 0;
@@ -436,71 +472,84 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
         }
     }
     __syncthreads();
+    // Leaves of more than FAT_LEAF triangles (long equal-code runs) are bounded
+    // by the whole wave, one leaf at a time; min/max is exact in any order.
+    const bool fat = active && !is_split && end - start > FAT_LEAF;
+    uint64_t fm = ballot(fat);
+    while (fm) {
+        const int src = __builtin_ctzll(fm);
+        fm &= fm - 1;
+        const uint32_t fs = __shfl(start, src, RTX_WAVE), fe = __shfl(end, src, RTX_WAVE);
+        const uint32_t fn = __shfl(n, src, RTX_WAVE);
+        float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t vtx = 3 * fs + lane; vtx < 3 * fe; vtx += RTX_WAVE) {
+            const float4 p = tri[vtx];
+            q[0] = fminf(q[0], p.x); q[1] = fminf(q[1], p.y); q[2] = fminf(q[2], p.z);
+            q[3] = fmaxf(q[3], p.x); q[4] = fmaxf(q[4], p.y); q[5] = fmaxf(q[5], p.z);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+#pragma unroll
+            for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+        }
+        if (lane == 0) {
+            node[2 * (size_t)fn] = make_float4(q[0], q[1], q[2], __uint_as_float(fs));
+            node[2 * (size_t)fn + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(fe - fs));
+        }
+    }
     if (!active) return;
     if (is_split) {
         uint32_t rank = s_prefix + wofs + excl;
         uint32_t c = off + cnt + 2u * rank;
         bstart[c] = start;
         bend[c] = split;
-        blevel[c] = lvl_child;
         bstart[c + 1] = split;
         bend[c + 1] = end;
-        blevel[c + 1] = lvl_child;
         parent[c] = n;
         parent[c + 1] = n;
+        uint32_t* node_u32 = reinterpret_cast<uint32_t*>(node);
         node_u32[8 * (size_t)n + 3] = c;
         node_u32[8 * (size_t)n + 7] = 0u;
-        refit_cnt[n] = 0u;
-    } else {
-        node_u32[8 * (size_t)n + 3] = start;
-        node_u32[8 * (size_t)n + 7] = end - start;
+    } else if (!fat) {
+        // build_leaf_bbox (src/kernel/bvh-builder.cl:262-288) for this leaf
+        leaf_bounds(tri, node, n, start, end - start);
     }
 }
 
-// build_leaf_bbox + build_node_bbox (src/kernel/bvh-builder.cl:262-311) as
-// one bottom-up pass: each leaf unions its triangles' vertices (= union of
-// the primitive AABBs), then climbs; at every internal node the second
-// arriving child computes the union of both children and the near-child
-// delta (right.lo - left.lo).  Arrival counters count modulo 2, so a refit
-// needs no reset.
-__global__ __launch_bounds__(BS) void k_refit(const float4* __restrict__ tri, float4* node, float4* __restrict__ delta,
-                                              const uint32_t* __restrict__ parent, uint32_t* refit_cnt,
-                                              const uint32_t* __restrict__ levels, uint32_t ncap) {
-    uint32_t N = levels[LV_TOTAL];
-    uint32_t n = blockIdx.x * BS + threadIdx.x;
-    if (n >= N || n >= ncap) return;
-    float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
-    uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-    if (cnt == 0) return;
-    v3 lo = xyz(tri[3 * (size_t)a]), hi = lo;
-    for (uint32_t t = a; t < a + cnt; ++t) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            v3 p = xyz(tri[3 * (size_t)t + q]);
-            lo = vmin(lo, p);
-            hi = vmax(hi, p);
-        }
-    }
-    node[2 * (size_t)n] = make_float4(lo.x, lo.y, lo.z, f0.w);
-    node[2 * (size_t)n + 1] = make_float4(hi.x, hi.y, hi.z, f1.w);
-    uint32_t cur = n;
-    while (cur != 0) {
-        uint32_t p = parent[cur];
-        __threadfence();
-        uint32_t old = fetch_add_acqrel(&refit_cnt[p], 1u);
-        if ((old & 1u) == 0u) return;  // first child to arrive
-        __threadfence();
-        float4 pw = node[2 * (size_t)p];
-        uint32_t l = __float_as_uint(pw.w);
+// build_node_bbox (src/kernel/bvh-builder.cl:290-311) for the internal nodes
+// of level d: union of the two children (computed by the launch for level
+// d+1, or as leaves), plus the near-child delta right.lo - left.lo used by
+// traversal.  One launch per level, deepest first, as the reference's host
+// loop (src/rt/bvh-builder.cpp:683-695).
+__global__ __launch_bounds__(BS) void k_refit_level(float4* __restrict__ node, float4* __restrict__ delta,
+                                                    const uint32_t* __restrict__ levels, int d) {
+    const uint32_t off = levels[LV_OFF + d], cnt = levels[LV_CNT + d];
+    for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < cnt; k += gridDim.x * BS) {
+        uint32_t n = off + k;
+        float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
+        if (__float_as_uint(f1.w) != 0u) continue;  // leaf: bounded at emission
+        uint32_t l = __float_as_uint(f0.w);
         float4 L0 = node[2 * (size_t)l], L1 = node[2 * (size_t)l + 1];
         float4 R0 = node[2 * (size_t)l + 2], R1 = node[2 * (size_t)l + 3];
         v3 plo = vmin(xyz(L0), xyz(R0));
         v3 phi = vmax(xyz(L1), xyz(R1));
-        node[2 * (size_t)p] = make_float4(plo.x, plo.y, plo.z, pw.w);
-        node[2 * (size_t)p + 1] = make_float4(phi.x, phi.y, phi.z, __uint_as_float(0u));
+        node[2 * (size_t)n] = make_float4(plo.x, plo.y, plo.z, f0.w);
+        node[2 * (size_t)n + 1] = make_float4(phi.x, phi.y, phi.z, f1.w);
         v3 dl = xyz(R0) - xyz(L0);
-        delta[p] = make_float4(dl.x, dl.y, dl.z, 0.f);
-        cur = p;
+        delta[n] = make_float4(dl.x, dl.y, dl.z, 0.f);
+    }
+}
+
+// Leaf AABBs of an existing topology (refit path).
+__global__ __launch_bounds__(BS) void k_leaf_bounds(const float4* __restrict__ tri, float4* __restrict__ node,
+                                                    const uint32_t* __restrict__ levels) {
+    const uint32_t N = levels[LV_TOTAL];
+    for (uint32_t n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS) {
+        float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
+        uint32_t cnt = __float_as_uint(f1.w);
+        if (cnt) leaf_bounds(tri, node, n, __float_as_uint(f0.w), cnt);
     }
 }
 
@@ -517,10 +566,8 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     if ((e = s->node.reserve(c, ncap * 32))) return e;
     if ((e = s->delta.reserve(c, ncap * 16))) return e;
     if ((e = s->parent.reserve(c, ncap * 4))) return e;
-    if ((e = s->refit_cnt.reserve(c, ncap * 4))) return e;
     if ((e = s->bstart.reserve(c, ncap * 4))) return e;
     if ((e = s->bend.reserve(c, ncap * 4))) return e;
-    if ((e = s->blevel.reserve(c, ncap * 4))) return e;
     if ((e = s->tri.reserve(c, (size_t)T * 48))) return e;
     if ((e = s->codes.reserve(c, (size_t)T * 4))) return e;
     if ((e = s->perm.reserve(c, (size_t)T * 4))) return e;
@@ -537,6 +584,19 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     return 0;
 }
 
+// Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
+// `depth` holds only leaves).
+static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
+    for (int d = depth - 1; d >= 0; --d) {
+        ProfScope ps(s->ctx, st, "lbvh_refit_level");
+        uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)s->T;
+        if (maxn > 2ull * s->T) maxn = 2ull * s->T;
+        uint32_t g = rtx_div_up(maxn, BS);
+        hipLaunchKernelGGL(k_refit_level, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->node.as<float4>(),
+                           s->delta.as<float4>(), s->levels.as<uint32_t>(), d);
+    }
+}
+
 int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
     RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_build: null scene");
     RTX_REQUIRE(s->T > 0 && s->V > 0, RTX_ESTATE, "rtx_lbvh_build: scene has no triangles");
@@ -551,6 +611,7 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
     hipStream_t st = s->ctx->pick(stream);
     int32_t e = lbvh_alloc(s, T, depth);
     if (e) return e;
+    ProfScope whole(s->ctx, st, "LBVH_BUILD");
     const int passes = (depth + 7) / 8;
     const uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
     const uint32_t hier_tiles = rtx_div_up(T, BS);
@@ -565,19 +626,19 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         // zero: tickets + histograms (contiguous? no: separate buffers)
         hipLaunchKernelGGL(k_build_init, dim3(grid), dim3(BS), 0, st, tickets, (uint32_t)TK_COUNT,
                            s->sort_status.as<uint32_t>(), nb, s->hier_status.as<uint32_t>(), nc, levels,
-                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->blevel.as<uint32_t>(),
-                           s->parent.as<uint32_t>(), T, 1);
+                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->parent.as<uint32_t>(), T, 1);
         hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, s->sort_hist.as<uint32_t>(), 4u * 256u,
                            (uint32_t*)nullptr, 0u, (uint32_t*)nullptr, 0u, levels, (uint32_t*)nullptr,
-                           (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, T, 0);
+                           (uint32_t*)nullptr, (uint32_t*)nullptr, T, 0);
     }
     {
         ProfScope ps(s->ctx, st, "lbvh_prim_bounds");
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > BOUNDS_GRID ? BOUNDS_GRID : grid;
         hipLaunchKernelGGL(k_prim_bounds, dim3(grid), dim3(BS), 0, st, s->idx.as<uint32_t>(), s->pos.as<float4>(),
-                           s->centroid.as<float4>(), s->partials.as<float>(), tickets + TK_BOUNDS,
-                           s->gbox.as<float>(), T);
+                           s->centroid.as<float4>(), s->partials.as<float>(), T);
+        hipLaunchKernelGGL(k_gbox_reduce, dim3(1), dim3(BS), 0, st, s->partials.as<float>(), grid,
+                           s->gbox.as<float>());
     }
     {
         ProfScope ps(s->ctx, st, "lbvh_morton");
@@ -613,24 +674,24 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         std::swap(s->matmap, s->matmap_alt);
         std::swap(s->orig, s->orig_alt);
     }
+    {
+        ProfScope ps(s->ctx, st, "lbvh_radix_tree");
+        // radix-tree arrays reuse the sort's alternate buffers
+        hipLaunchKernelGGL(k_radix_tree, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->codes.as<uint32_t>(),
+                           s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), (int)T);
+    }
     for (int d = 0; d <= depth; ++d) {
         ProfScope ps(s->ctx, st, "lbvh_hier_level");
         uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;
         if (maxn > 2ull * T) maxn = 2ull * T;
         uint32_t grid = rtx_div_up(maxn, BS);
         hipLaunchKernelGGL(k_hier_level, dim3(grid), dim3(BS), 0, st, s->codes.as<uint32_t>(),
-                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->blevel.as<uint32_t>(),
-                           s->node.as<uint32_t>(), s->parent.as<uint32_t>(), s->refit_cnt.as<uint32_t>(), levels,
-                           s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d, depth,
-                           min_leaf, d == depth ? 1 : 0);
+                           s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), s->bstart.as<uint32_t>(),
+                           s->bend.as<uint32_t>(), s->node.as<float4>(), s->parent.as<uint32_t>(),
+                           s->tri.as<float4>(), levels, s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles,
+                           tickets + TK_LEVEL + d, d, T, min_leaf, d == depth ? 1 : 0);
     }
-    {
-        ProfScope ps(s->ctx, st, "lbvh_refit");
-        uint32_t ncap = (uint32_t)(2 * (size_t)T);
-        hipLaunchKernelGGL(k_refit, dim3(rtx_div_up(ncap, BS)), dim3(BS), 0, st, s->tri.as<float4>(),
-                           s->node.as<float4>(), s->delta.as<float4>(), s->parent.as<uint32_t>(),
-                           s->refit_cnt.as<uint32_t>(), levels, ncap);
-    }
+    refit_internal(s, st, depth);
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
     s->depth = depth;
@@ -649,12 +710,13 @@ int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
                            s->pos.as<float4>(), s->tri.as<float4>(), T);
     }
     {
-        ProfScope ps(s->ctx, st, "lbvh_refit");
+        ProfScope ps(s->ctx, st, "lbvh_leaf_bounds");
         uint32_t ncap = (uint32_t)(2 * (size_t)T);
-        hipLaunchKernelGGL(k_refit, dim3(rtx_div_up(ncap, BS)), dim3(BS), 0, st, s->tri.as<float4>(),
-                           s->node.as<float4>(), s->delta.as<float4>(), s->parent.as<uint32_t>(),
-                           s->refit_cnt.as<uint32_t>(), s->levels.as<uint32_t>(), ncap);
+        uint32_t g = rtx_div_up(ncap, BS);
+        hipLaunchKernelGGL(k_leaf_bounds, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->tri.as<float4>(),
+                           s->node.as<float4>(), s->levels.as<uint32_t>());
     }
+    refit_internal(s, st, s->depth);
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }

@@ -28,18 +28,25 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
     if (r->p.quad_size <= 0) r->p.quad_size = 32;
     if (r->p.depth <= 0) r->p.depth = 30;
     if (r->p.min_leaf <= 0) r->p.min_leaf = 1;
-    uint64_t P = (uint64_t)p->width * p->height;
-    uint64_t cap = p->ray_capacity;
-    if (!cap) {
-        cap = P << (p->max_bounces > 0 ? p->max_bounces : 0);
-        if (cap > (1ull << 29)) cap = 1ull << 29;
-    }
-    if (cap < P) cap = P;
-    r->p.ray_capacity = cap;
+    const uint64_t P = (uint64_t)p->width * p->height;
     int32_t e = 0;
-    if ((e = rtx_rays_create(ctx, cap, &r->a)) || (e = rtx_rays_create(ctx, cap, &r->b)) ||
-        (e = rtx_hits_create(ctx, cap, &r->hits)) || (e = rtx_fb_create(ctx, p->width, p->height, &r->fb)) ||
-        (e = r->counts.reserve(ctx, 64))) {
+    // A bounce holds at most twice the rays of the one before, so (P << b) + 2
+    // slots never trigger the capacity drop (new_id + 2 >= cap).  A smaller
+    // user capacity applies to every secondary bundle, as in the reference.
+    for (int b = 0; b <= r->p.max_bounces && !e; ++b) {
+        uint64_t cap = b == 0 ? P : (P << b) + 2;
+        if (b > 0 && p->ray_capacity && p->ray_capacity < cap) cap = p->ray_capacity;
+        if (cap >= (1ull << 30)) cap = (1ull << 30) - 1;
+        rtx_rays* rr = nullptr;
+        rtx_hits* hh = nullptr;
+        if ((e = rtx_rays_create(ctx, cap, &rr))) break;
+        r->rays.push_back(rr);
+        if ((e = rtx_hits_create(ctx, cap, &hh))) break;
+        r->hits.push_back(hh);
+    }
+    if (!e) e = rtx_fb_create(ctx, p->width, p->height, &r->fb);
+    if (!e) e = r->counts.reserve(ctx, 64);
+    if (e) {
         rtx_renderer_destroy(r);
         return e;
     }
@@ -49,9 +56,8 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
 
 int32_t rtx_renderer_destroy(rtx_renderer* r) {
     RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_destroy: null");
-    if (r->a) rtx_rays_destroy(r->a);
-    if (r->b) rtx_rays_destroy(r->b);
-    if (r->hits) rtx_hits_destroy(r->hits);
+    for (auto x : r->rays) rtx_rays_destroy(x);
+    for (auto x : r->hits) rtx_hits_destroy(x);
     if (r->fb) rtx_fb_destroy(r->fb);
     r->counts.release();
     delete r;
@@ -60,6 +66,20 @@ int32_t rtx_renderer_destroy(rtx_renderer* r) {
 
 rtx_fb* rtx_renderer_fb(rtx_renderer* r) { return r ? r->fb : nullptr; }
 
+// set_up_frame: counters, framebuffer clear, accelerator update.
+static int32_t set_up_frame(rtx_renderer* r, rtx_scene* s, int32_t rebuild, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, r->counts.as<unsigned long long>(), 8);
+    int32_t e;
+    if ((e = rtx_fb_clear(r->fb, st))) return e;
+    if (rebuild || !s->built) {
+        rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
+        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
+    }
+    return RTX_OK;
+}
+
+// The reference's sequence (src/rt/renderer.cpp:113-214): per bounce trace,
+// shadow, shade, then generate and read the new count back.
 int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild,
                          rtx_frame_stats* stats, void* stream) {
     RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame: null argument");
@@ -68,38 +88,32 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
     const uint64_t P = (uint64_t)r->p.width * r->p.height;
     unsigned long long* cnt = r->counts.as<unsigned long long>();
     int32_t e;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 8);
-    // set_up_frame: clear + update_accelerator
-    if ((e = rtx_fb_clear(r->fb, st))) return e;
-    if (rebuild || !s->built) {
-        rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
-        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
-    }
+    if ((e = set_up_frame(r, s, rebuild, st))) return e;
     rtx_frame_stats fs{};
     fs.primary_rays = P;
-    rtx_rays* in = r->a;
-    rtx_rays* out = r->b;
-    if ((e = rtx_raygen_primary(in, cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st))) return e;
-    if ((e = rtx_trace(s, in, r->hits, P, st))) return e;
-    if ((e = rtx_trace_shadow_impl(s, in, r->hits, P, (uint64_t*)(cnt + 1), st))) return e;
-    if ((e = rtx_shade(s, in, r->hits, r->fb, P, 1, st))) return e;
+    if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
+        return e;
+    if ((e = rtx_trace(s, r->rays[0], r->hits[0], P, st))) return e;
+    if ((e = rtx_trace_shadow_impl(s, r->rays[0], r->hits[0], P, (uint64_t*)(cnt + 1), st))) return e;
+    if ((e = rtx_shade(s, r->rays[0], r->hits[0], r->fb, P, 1, st))) return e;
     uint64_t n = P;
     for (int b = 0; b < r->p.max_bounces; ++b) {
-        uint64_t m = 0;
-        if ((e = rtx_raygen_secondary_impl(s, in, r->hits, out, n, (uint64_t*)cnt, st))) return e;
+        rtx_rays *in = r->rays[b], *out = r->rays[b + 1];
+        rtx_hits* hin = r->hits[b];
+        if ((e = rtx_raygen_secondary_impl(s, in, hin, out, n, (uint64_t*)cnt, st))) return e;
         // the reference reads the new count back every bounce
         // (src/rt/secondary-ray-generator.cpp:296-299)
         uint32_t c32 = 0;
         RTX_CHECK_HIP(hipMemcpyAsync(&c32, out->count.p, 4, hipMemcpyDeviceToHost, st));
         RTX_CHECK_HIP(hipStreamSynchronize(st));
-        m = c32;
-        std::swap(in, out);
+        const uint64_t m = c32;
         if (!m) break;
         fs.secondary_rays += m;
         fs.bounces_run++;
-        if ((e = rtx_trace(s, in, r->hits, m, st))) return e;
-        if ((e = rtx_trace_shadow_impl(s, in, r->hits, m, (uint64_t*)(cnt + 1), st))) return e;
-        if ((e = rtx_shade(s, in, r->hits, r->fb, m, 0, st))) return e;
+        rtx_hits* hout = r->hits[b + 1];
+        if ((e = rtx_trace(s, out, hout, m, st))) return e;
+        if ((e = rtx_trace_shadow_impl(s, out, hout, m, (uint64_t*)(cnt + 1), st))) return e;
+        if ((e = rtx_shade(s, out, hout, r->fb, m, 0, st))) return e;
         n = m;
     }
     if ((e = rtx_fb_resolve(r->fb, st))) return e;
@@ -115,33 +129,37 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
     return RTX_OK;
 }
 
+// Same frame with no host read-back: counts stay on the device; the closest
+// hits of every bounce are traced first (each bounce's rays depend only on
+// the previous bounce's hits), then the shadow rays of all bounces in one
+// launch (they are independent of each other and of later bounces), then
+// shading.  Results are identical: shading is an order-free integer sum.
 int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
     RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame_async: null argument");
     hipStream_t st = r->ctx->pick(stream);
     const uint64_t P = (uint64_t)r->p.width * r->p.height;
     unsigned long long* cnt = r->counts.as<unsigned long long>();
     int32_t e;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 8);
-    if ((e = rtx_fb_clear(r->fb, st))) return e;
-    if (rebuild || !s->built) {
-        rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
-        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
-    }
-    rtx_rays* in = r->a;
-    rtx_rays* out = r->b;
-    if ((e = rtx_raygen_primary(in, cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st))) return e;
-    if ((e = rtx_trace(s, in, r->hits, P, st))) return e;
-    if ((e = rtx_trace_shadow_impl(s, in, r->hits, P, (uint64_t*)(cnt + 1), st))) return e;
-    if ((e = rtx_shade(s, in, r->hits, r->fb, P, 1, st))) return e;
-    for (int b = 0; b < r->p.max_bounces; ++b) {
-        uint64_t n_in = b == 0 ? P : RTX_DEVICE_COUNT;
-        if ((e = rtx_raygen_secondary_impl(s, in, r->hits, out, n_in, (uint64_t*)cnt, st))) return e;
-        std::swap(in, out);
+    if ((e = set_up_frame(r, s, rebuild, st))) return e;
+    ProfScope whole(r->ctx, st, "FRAME_RENDER");
+    const int B = r->p.max_bounces;
+    if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
+        return e;
+    if ((e = rtx_trace(s, r->rays[0], r->hits[0], P, st))) return e;
+    for (int b = 0; b < B; ++b) {
+        const uint64_t n_in = b == 0 ? P : RTX_DEVICE_COUNT;
+        if ((e = rtx_raygen_secondary_impl(s, r->rays[b], r->hits[b], r->rays[b + 1], n_in, (uint64_t*)cnt, st)))
+            return e;
         // zero-count bounces run as empty launches (no host decision)
-        if ((e = rtx_trace(s, in, r->hits, RTX_DEVICE_COUNT, st))) return e;
-        if ((e = rtx_trace_shadow_impl(s, in, r->hits, RTX_DEVICE_COUNT, (uint64_t*)(cnt + 1), st))) return e;
-        if ((e = rtx_shade(s, in, r->hits, r->fb, RTX_DEVICE_COUNT, 0, st))) return e;
+        if ((e = rtx_trace(s, r->rays[b + 1], r->hits[b + 1], RTX_DEVICE_COUNT, st))) return e;
     }
+    std::vector<uint64_t> counts(B + 1, RTX_DEVICE_COUNT);
+    counts[0] = P;
+    if ((e = rtx_trace_shadow_multi(s, r->rays.data(), r->hits.data(), counts.data(), B + 1, (uint64_t*)(cnt + 1),
+                                    st)))
+        return e;
+    for (int b = 0; b <= B; ++b)
+        if ((e = rtx_shade(s, r->rays[b], r->hits[b], r->fb, counts[b], b == 0, st))) return e;
     if ((e = rtx_fb_resolve(r->fb, st))) return e;
     r->last_async = true;
     r->last = rtx_frame_stats{};

@@ -60,8 +60,15 @@ struct KernelTiming {
     hipEvent_t start, stop;
 };
 
+// Tuning knobs (set through rtx_tune_set; picked by the autotuner).
+struct rtx_tuning {
+    int64_t trace_variant = 1;  // 0 single node, 1 single node + early delta load, 2 sibling pairs
+    int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
+};
+
 struct rtx_ctx {
     int device = 0;
+    rtx_tuning tune;
     hipStream_t stream = nullptr;
     int cus = 256;
     uint64_t bytes_in_use = 0;
@@ -99,9 +106,8 @@ struct rtx_scene {
     DevBuf node, delta, parent, tri, codes, perm;
     DevBuf gbox;        // 6 x int32 ordered-float keys
     DevBuf levels;      // uint32: [0..64) level offsets, [64..128) level counts, [128] node count
-    DevBuf refit_cnt;   // uint32[N] arrival counters
     // build scratch
-    DevBuf bstart, bend, blevel;  // per-node build state
+    DevBuf bstart, bend;          // per-node build state: triangle range
     DevBuf keys_alt, vals_alt;    // radix ping-pong
     DevBuf centroid, partials;    // per-triangle centroids, per-block AABB partials
     DevBuf sort_hist;             // [4][256] digit histograms -> exclusive offsets
@@ -123,6 +129,9 @@ struct rtx_hits {
     rtx_ctx* ctx = nullptr;
     uint64_t cap = 0;
     DevBuf flags, t, id, bary, n4, p4, uv2;
+    DevBuf list, list_count;     // indices of rays with a hit (filled by the trace kernel)
+    bool list_valid = false;     // list matches the current hit records...
+    uint64_t list_for = 0;       // ...of this many rays (or RTX_DEVICE_COUNT)
 };
 
 struct rtx_fb {
@@ -134,9 +143,8 @@ struct rtx_fb {
 struct rtx_renderer {
     rtx_ctx* ctx = nullptr;
     rtx_render_params p{};
-    rtx_rays* a = nullptr;
-    rtx_rays* b = nullptr;
-    rtx_hits* hits = nullptr;
+    std::vector<rtx_rays*> rays;  // bounce b's rays: [0] primary, capacity (pixels << b) + 2 (or the user cap)
+    std::vector<rtx_hits*> hits;  // bounce b's hit records
     rtx_fb* fb = nullptr;
     rtx_frame_stats last{};
     DevBuf counts;  // uint64 device ray counters of the current frame (see render.hip)
@@ -156,6 +164,9 @@ int32_t rtx_fill_u32(rtx_ctx* ctx, void* p, uint32_t value, uint64_t n, hipStrea
 // Internal variants used by the renderer: device-count stats accumulation.
 int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, uint64_t* dev_shadow_count,
                               hipStream_t st);
+// Shadow rays of several (rays, hits) bundles in one launch.
+int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* const* hits, const uint64_t* counts,
+                               int nseg, uint64_t* dev_shadow_count, hipStream_t st);
 int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
                                   uint64_t* dev_total_acc, hipStream_t st);
 

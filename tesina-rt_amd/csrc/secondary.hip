@@ -99,12 +99,12 @@ __global__ __launch_bounds__(SB) void k_secondary(
         // lookback (same protocol as the LBVH kernels)
         uint32_t prefix = 0;
         if (tile == 0) {
-            st_release(&status[0], RTX_LB_INC | agg);
+            st_relaxed(&status[0], RTX_LB_INC | agg);
         } else {
-            st_release(&status[tile], RTX_LB_AGG | agg);
+            st_relaxed(&status[tile], RTX_LB_AGG | agg);
             int q = (int)tile - 1;
             for (;;) {
-                uint32_t s = ld_acquire(&status[q]);
+                uint32_t s = ld_relaxed(&status[q]);
                 if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(SB) void k_secondary(
                 if (s & RTX_LB_INC) break;
                 --q;
             }
-            st_release(&status[tile], RTX_LB_INC | (prefix + agg));
+            st_relaxed(&status[tile], RTX_LB_INC | (prefix + agg));
         }
         s_prefix = prefix;
         if (tile == ntiles - 1) out_cnt[1] = prefix + agg;

@@ -17,52 +17,10 @@
 
 namespace {
 
-constexpr int TB = 128;        // traversal block: 2 waves, 16 KB LDS stack
 constexpr int STACK = 32;      // MAX_LEVELS of src/kernel/trace-bvh.cl:330
-
-struct RaySoA {
-    const float4* ori4;
-    const float4* dir4;
-};
 
 __device__ __forceinline__ uint64_t ray_count(const uint32_t* dcount, uint64_t hcount) {
     return hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
-}
-
-// bbox_hit (src/kernel/trace-bvh.cl:247-273).
-__device__ __forceinline__ bool slab(v3 lo, v3 hi, v3 ori, v3 inv, float tmin, float tmax) {
-    v3 tl = (lo - ori) * inv;
-    v3 th = (hi - ori) * inv;
-    v3 tM = vmax(tl, th);
-    v3 tm = vmin(tl, th);
-    float t0 = tmin, t1 = tmax;
-    if (fabsf(inv.x) > 1e-6f) { t0 = fmaxf(t0, tm.x); t1 = fminf(t1, tM.x); }
-    if (fabsf(inv.y) > 1e-6f) { t0 = fmaxf(t0, tm.y); t1 = fminf(t1, tM.y); }
-    if (fabsf(inv.z) > 1e-6f) { t0 = fmaxf(t0, tm.z); t1 = fminf(t1, tM.z); }
-    return t0 <= t1;
-}
-
-// Möller–Trumbore of leaf_hit (src/kernel/trace-bvh.cl:282-327).
-__device__ __forceinline__ bool tri_hit(const float4* __restrict__ tri, uint32_t t, v3 o, v3 d, float* tt, float* uu,
-                                        float* vv) {
-    v3 v0 = xyz(tri[3 * (size_t)t]);
-    v3 v1 = xyz(tri[3 * (size_t)t + 1]);
-    v3 v2 = xyz(tri[3 * (size_t)t + 2]);
-    v3 e1 = v1 - v0, e2 = v2 - v0;
-    v3 h = cross(d, e2);
-    float a = dot(e1, h);
-    if (a > -1e-26f && a < 1e-26f) return false;
-    float f = 1.f / a;
-    v3 s = o - v0;
-    float u = f * dot(s, h);
-    if (u < 0.f || u > 1.f) return false;
-    v3 q = cross(s, e1);
-    float v = f * dot(d, q);
-    if (v < 0.f || u + v > 1.f) return false;
-    *tt = f * dot(e2, q);
-    *uu = u;
-    *vv = v;
-    return true;
 }
 
 // ------------------------------------------------------------- raygen ----
@@ -110,189 +68,505 @@ __global__ __launch_bounds__(256) void k_raygen_primary(float4* __restrict__ ori
     contrib[k] = smp[2];
 }
 
-// -------------------------------------------------------------- trace ----
-__global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, const float4* __restrict__ delta,
-                                              const float4* __restrict__ tri, const float4* __restrict__ nrm,
-                                              const float2* __restrict__ uvs, const float4* __restrict__ ori4,
-                                              const float4* __restrict__ dir4, const uint32_t* __restrict__ dcount,
-                                              uint64_t hcount, uint32_t* __restrict__ flags, float* __restrict__ tout,
-                                              int32_t* __restrict__ idout, float2* __restrict__ bary,
-                                              float4* __restrict__ n4, float4* __restrict__ p4,
-                                              float2* __restrict__ uv2) {
-    __shared__ uint32_t stack[STACK][TB];
-    const uint64_t n = ray_count(dcount, hcount);
-    const uint32_t tid = threadIdx.x;
-    for (uint64_t r = blockIdx.x * (uint64_t)TB + tid; r < n; r += (uint64_t)gridDim.x * TB) {
-        float4 o = ori4[r], dd = dir4[r];
-        v3 ori = xyz(o), dir = xyz(dd);
-        v3 inv = recip(dir);
-        const float tmin = o.w;
-        float best_t = dd.w, bu = 0.f, bv = 0.f;
-        int32_t best = -1;
-        uint32_t level = 0, curr = 0;
-        for (;;) {
-            float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-            uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-            bool pop = true;
-            if (slab(xyz(f0), xyz(f1), ori, inv, tmin, best_t)) {
-                if (cnt) {
-                    for (uint32_t t = a; t < a + cnt; ++t) {
-                        float tt, u, v;
-                        if (!tri_hit(tri, t, ori, dir, &tt, &u, &v)) continue;
-                        if (tt <= best_t && tt >= tmin) {
-                            best_t = tt;
-                            best = (int32_t)t;
-                            bu = u;
-                            bv = v;
-                        }
-                    }
-                } else {
-                    v3 dl = xyz(delta[curr]);
-                    bool near_left = dot(dir, dl) > 0.f;
-                    stack[level & (STACK - 1)][tid] = near_left ? a + 1 : a;
-                    level++;
-                    curr = near_left ? a : a + 1;
-                    pop = false;
-                }
-            }
-            if (pop) {
-                if (level > 0 && level < STACK) {
-                    level--;
-                    curr = stack[level][tid];
-                } else {
-                    break;
-                }
-            }
+// ---------------------------------------------------------- traversal --
+// Per-ray constants of the slab test.  The reference skips an axis when
+// |invDir| <= 1e-6 (src/kernel/trace-bvh.cl:262-270); that is fixed per ray.
+struct RayC {
+    v3 o, d, inv;
+    float tmin;
+    bool ax, ay, az;
+};
+
+__device__ __forceinline__ RayC make_ray(v3 o, v3 d, float tmin) {
+    RayC r;
+    r.o = o;
+    r.d = d;
+    r.inv = recip(d);
+    r.tmin = tmin;
+    r.ax = fabsf(r.inv.x) > 1e-6f;
+    r.ay = fabsf(r.inv.y) > 1e-6f;
+    r.az = fabsf(r.inv.z) > 1e-6f;
+    return r;
+}
+
+// bbox_hit (src/kernel/trace-bvh.cl:247-273).  The exit distance is
+// fmin(tMax, m) with m the box's own exit, so a box that fails with some tMax
+// fails with every smaller one: testing both children when the parent is
+// visited (with the current tMax) and again when a pushed child is popped
+// (with the then-current tMax) gives exactly the reference's decisions.
+__device__ __forceinline__ bool slab(float4 lo4, float4 hi4, const RayC& r, float tmax) {
+    v3 tl = (xyz(lo4) - r.o) * r.inv;
+    v3 th = (xyz(hi4) - r.o) * r.inv;
+    v3 tM = vmax(tl, th);
+    v3 tm = vmin(tl, th);
+    float t0 = r.tmin, t1 = tmax;
+    if (r.ax) { t0 = fmaxf(t0, tm.x); t1 = fminf(t1, tM.x); }
+    if (r.ay) { t0 = fmaxf(t0, tm.y); t1 = fminf(t1, tM.y); }
+    if (r.az) { t0 = fmaxf(t0, tm.z); t1 = fminf(t1, tM.z); }
+    return t0 <= t1;
+}
+
+// Traversal stack of node ids, [entry][lane] in LDS (bank-conflict free).
+template <int BLOCK>
+struct LdsStack {
+    uint32_t* id;
+    uint32_t lane;
+    __device__ __forceinline__ void put(uint32_t lv, uint32_t i) { id[lv * BLOCK + lane] = i; }
+    __device__ __forceinline__ uint32_t get(uint32_t lv) const { return id[lv * BLOCK + lane]; }
+};
+
+struct Closest {
+    float t, u, v;
+    int32_t id;
+};
+
+// Möller–Trumbore of leaf_hit (src/kernel/trace-bvh.cl:282-327), on vertices
+// already in registers.
+__device__ __forceinline__ bool tri_hit_v(float4 a, float4 b, float4 c, v3 o, v3 d, float* tt, float* uu,
+                                          float* vv) {
+    v3 v0 = xyz(a), v1 = xyz(b), v2 = xyz(c);
+    v3 e1 = v1 - v0, e2 = v2 - v0;
+    v3 h = cross(d, e2);
+    float aa = dot(e1, h);
+    if (aa > -1e-26f && aa < 1e-26f) return false;
+    float f = 1.f / aa;
+    v3 s = o - v0;
+    float u = f * dot(s, h);
+    if (u < 0.f || u > 1.f) return false;
+    v3 q = cross(s, e1);
+    float v = f * dot(d, q);
+    if (v < 0.f || u + v > 1.f) return false;
+    *tt = f * dot(e2, q);
+    *uu = u;
+    *vv = v;
+    return true;
+}
+
+// Leaf triangles are tested in order; the vertices of two triangles are
+// loaded before either is tested so their memory latencies overlap.
+__device__ __forceinline__ void leaf_closest(const float4* __restrict__ tri, uint32_t a, uint32_t cnt, const RayC& r,
+                                             Closest& best) {
+    const uint32_t end = a + cnt;
+    for (uint32_t t = a; t < end; t += 2) {
+        const bool two = t + 1 < end;
+        const float4 p0 = tri[3 * (size_t)t], p1 = tri[3 * (size_t)t + 1], p2 = tri[3 * (size_t)t + 2];
+        float4 q0 = p0, q1 = p1, q2 = p2;
+        if (two) {
+            q0 = tri[3 * (size_t)t + 3];
+            q1 = tri[3 * (size_t)t + 4];
+            q2 = tri[3 * (size_t)t + 5];
         }
-        // complete_trace_info (src/kernel/trace-bvh.cl:190-245)
-        if (best < 0) {
-            flags[r] = 0u;
-            tout[r] = 0.f;
-            idout[r] = -1;
-            bary[r] = make_float2(0.f, 0.f);
-            n4[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            p4[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            uv2[r] = make_float2(0.f, 0.f);
-            continue;
+        float tt, u, v;
+        if (tri_hit_v(p0, p1, p2, r.o, r.d, &tt, &u, &v) && tt <= best.t && tt >= r.tmin) {
+            best.t = tt;  // later triangle wins an exact tie
+            best.id = (int32_t)t;
+            best.u = u;
+            best.v = v;
         }
-        v3 hp = ori + dir * best_t;
-        uint32_t i0 = __float_as_uint(tri[3 * (size_t)best].w);
-        uint32_t i1 = __float_as_uint(tri[3 * (size_t)best + 1].w);
-        uint32_t i2 = __float_as_uint(tri[3 * (size_t)best + 2].w);
-        float w = 1.f - (bu + bv);
-        v3 n0 = xyz(nrm[i0]), n1 = xyz(nrm[i1]), n2 = xyz(nrm[i2]);
-        v3 nn = normalize(w * n0 + bv * n2 + bu * n1);
-        uint32_t fl = 1u;
-        if (dot(nn, dir) > 0.f) {
-            fl |= 2u;
-            nn = -nn;
+        if (two && tri_hit_v(q0, q1, q2, r.o, r.d, &tt, &u, &v) && tt <= best.t && tt >= r.tmin) {
+            best.t = tt;
+            best.id = (int32_t)(t + 1);
+            best.u = u;
+            best.v = v;
         }
-        float2 t0 = uvs[i0], t1 = uvs[i1], t2 = uvs[i2];
-        flags[r] = fl;
-        tout[r] = best_t;
-        idout[r] = best;
-        bary[r] = make_float2(bu, bv);
-        n4[r] = make_float4(nn.x, nn.y, nn.z, 0.f);
-        p4[r] = make_float4(hp.x, hp.y, hp.z, 0.f);
-        uv2[r] = make_float2(w * t0.x + bv * t2.x + bu * t1.x, w * t0.y + bv * t2.y + bu * t1.y);
     }
 }
 
-// ------------------------------------------------------------- shadow ----
-// shadow_trace_single (src/kernel/shadow-trace-bvh.cl:375-418) for every
-// light: DIR toward -dir, SPOT toward pos (outside the cone = shadowed),
-// POINT toward pos with tMax = distance; any hit with tMin < t < tMax.
-__global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
-                                               const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
-                                               const uint32_t* __restrict__ dcount, uint64_t hcount,
-                                               uint32_t* __restrict__ flags, const float4* __restrict__ p4,
-                                               unsigned long long* shadow_count) {
-    __shared__ uint32_t stack[STACK][TB];
-    __shared__ uint32_t s_cnt;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
+__device__ __forceinline__ bool leaf_any(const float4* __restrict__ tri, uint32_t a, uint32_t cnt, const RayC& r,
+                                         float tmax) {
+    const uint32_t end = a + cnt;
+    for (uint32_t t = a; t < end; t += 2) {
+        const bool two = t + 1 < end;
+        const float4 p0 = tri[3 * (size_t)t], p1 = tri[3 * (size_t)t + 1], p2 = tri[3 * (size_t)t + 2];
+        float4 q0 = p0, q1 = p1, q2 = p2;
+        if (two) {
+            q0 = tri[3 * (size_t)t + 3];
+            q1 = tri[3 * (size_t)t + 4];
+            q2 = tri[3 * (size_t)t + 5];
+        }
+        float tt, u, v;
+        if (tri_hit_v(p0, p1, p2, r.o, r.d, &tt, &u, &v) && tt < tmax && tt > r.tmin) return true;
+        if (two && tri_hit_v(q0, q1, q2, r.o, r.d, &tt, &u, &v) && tt < tmax && tt > r.tmin) return true;
+    }
+    return false;
+}
+
+// Closest hit: the reference's depth-first order (near child first by
+// dot(dir, right.lo - left.lo) > 0, src/kernel/trace-bvh.cl:392-409), visiting
+// two sibling boxes per 64-byte load (siblings are adjacent in the node
+// array).  Leaves are reached in the same sequence with the same tMax as in
+// trace_ray (src/kernel/trace-bvh.cl:330-417), so hits and tie winners match.
+template <class Stack>
+__device__ __forceinline__ void traverse_closest(const float4* __restrict__ node, const float4* __restrict__ tri,
+                                                 const RayC& r, Closest& best, Stack& stk) {
+    {
+        float4 r0 = node[0], r1 = node[1];
+        if (!slab(r0, r1, r, best.t)) return;
+        uint32_t cnt = __float_as_uint(r1.w);
+        if (cnt) {
+            leaf_closest(tri, __float_as_uint(r0.w), cnt, r, best);
+            return;
+        }
+    }
+    uint32_t cur = __float_as_uint(node[0].w);
+    uint32_t level = 0;
+    for (;;) {
+        const float4 L0 = node[2 * (size_t)cur], L1 = node[2 * (size_t)cur + 1];
+        const float4 R0 = node[2 * (size_t)cur + 2], R1 = node[2 * (size_t)cur + 3];
+        const bool hl = slab(L0, L1, r, best.t);
+        const bool hr = slab(R0, R1, r, best.t);
+        const bool nl = dot(r.d, xyz(R0) - xyz(L0)) > 0.f;
+        if (nl ? hr : hl) stk.put(level++, nl ? cur + 1 : cur);
+        if (nl ? hl : hr) {
+            const uint32_t na = __float_as_uint(nl ? L0.w : R0.w), nc = __float_as_uint(nl ? L1.w : R1.w);
+            if (!nc) {
+                cur = na;
+                continue;
+            }
+            leaf_closest(tri, na, nc, r, best);
+        }
+        bool next = false;
+        while (level > 0) {
+            const uint32_t id = stk.get(--level);
+            const float4 P0 = node[2 * (size_t)id], P1 = node[2 * (size_t)id + 1];
+            if (!slab(P0, P1, r, best.t)) continue;  // the reference's test at visit time
+            const uint32_t pa = __float_as_uint(P0.w), pc = __float_as_uint(P1.w);
+            if (pc) {
+                leaf_closest(tri, pa, pc, r, best);
+                continue;
+            }
+            cur = pa;
+            next = true;
+            break;
+        }
+        if (!next) return;
+    }
+}
+
+// Any hit with tMin < t < tMax (trace_shadow_ray + leaf_hit_any,
+// src/kernel/shadow-trace-bvh.cl:188-315).  tMax is fixed, so the set of
+// leaves whose boxes pass does not depend on the visiting order.
+template <class Stack>
+__device__ __forceinline__ bool traverse_any(const float4* __restrict__ node, const float4* __restrict__ tri,
+                                             const RayC& r, float tmax, Stack& stk) {
+    {
+        float4 r0 = node[0], r1 = node[1];
+        if (!slab(r0, r1, r, tmax)) return false;
+        uint32_t cnt = __float_as_uint(r1.w);
+        if (cnt) return leaf_any(tri, __float_as_uint(r0.w), cnt, r, tmax);
+    }
+    uint32_t cur = __float_as_uint(node[0].w);
+    uint32_t level = 0;
+    for (;;) {
+        const float4 L0 = node[2 * (size_t)cur], L1 = node[2 * (size_t)cur + 1];
+        const float4 R0 = node[2 * (size_t)cur + 2], R1 = node[2 * (size_t)cur + 3];
+        const bool hl = slab(L0, L1, r, tmax);
+        const bool hr = slab(R0, R1, r, tmax);
+        const bool nl = dot(r.d, xyz(R0) - xyz(L0)) > 0.f;
+        if (nl ? hr : hl) stk.put(level++, nl ? cur + 1 : cur);
+        if (nl ? hl : hr) {
+            const uint32_t na = __float_as_uint(nl ? L0.w : R0.w), nc = __float_as_uint(nl ? L1.w : R1.w);
+            if (!nc) {
+                cur = na;
+                continue;
+            }
+            if (leaf_any(tri, na, nc, r, tmax)) return true;
+        }
+        bool next = false;
+        while (level > 0) {
+            // tMax is fixed: the box passed when pushed, it passes now
+            const uint32_t id = stk.get(--level);
+            const float4 P0 = node[2 * (size_t)id], P1 = node[2 * (size_t)id + 1];
+            const uint32_t pa = __float_as_uint(P0.w), pc = __float_as_uint(P1.w);
+            if (pc) {
+                if (leaf_any(tri, pa, pc, r, tmax)) return true;
+                continue;
+            }
+            cur = pa;
+            next = true;
+            break;
+        }
+        if (!next) return false;
+    }
+}
+
+// Single-node form of the same traversal (one node per step, as
+// trace_ray visits them).  SPEC issues the node's delta load together with
+// the node load instead of after the box test.
+template <bool SPEC, class Stack>
+__device__ __forceinline__ void traverse_closest_node(const float4* __restrict__ node,
+                                                      const float4* __restrict__ delta,
+                                                      const float4* __restrict__ tri, const RayC& r, Closest& best,
+                                                      Stack& stk) {
+    uint32_t level = 0, curr = 0;
+    for (;;) {
+        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+        float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (SPEC) dl4 = delta[curr];
+        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+        bool pop = true;
+        if (slab(f0, f1, r, best.t)) {
+            if (cnt) {
+                leaf_closest(tri, a, cnt, r, best);
+            } else {
+                if (!SPEC) dl4 = delta[curr];
+                const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+                stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                level++;
+                curr = nl ? a : a + 1;
+                pop = false;
+            }
+        }
+        if (pop) {
+            if (level == 0) return;
+            curr = stk.get(--level);
+        }
+    }
+}
+
+template <bool SPEC, class Stack>
+__device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                                  const float4* __restrict__ tri, const RayC& r, float tmax,
+                                                  Stack& stk) {
+    uint32_t level = 0, curr = 0;
+    for (;;) {
+        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+        float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (SPEC) dl4 = delta[curr];
+        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+        bool pop = true;
+        if (slab(f0, f1, r, tmax)) {
+            if (cnt) {
+                if (leaf_any(tri, a, cnt, r, tmax)) return true;
+            } else {
+                if (!SPEC) dl4 = delta[curr];
+                const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+                stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                level++;
+                curr = nl ? a : a + 1;
+                pop = false;
+            }
+        }
+        if (pop) {
+            if (level == 0) return false;
+            curr = stk.get(--level);
+        }
+    }
+}
+
+// Variant dispatch: 0 = single node, 1 = single node + early delta load,
+// 2 = sibling pairs.
+template <int VAR, class Stack>
+__device__ __forceinline__ void closest_var(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                            const float4* __restrict__ tri, const RayC& r, Closest& best,
+                                            Stack& stk) {
+    if (VAR == 2)
+        traverse_closest(node, tri, r, best, stk);
+    else
+        traverse_closest_node<VAR == 1>(node, delta, tri, r, best, stk);
+}
+
+template <int VAR, class Stack>
+__device__ __forceinline__ bool any_var(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                        const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk) {
+    if (VAR == 2) return traverse_any(node, tri, r, tmax, stk);
+    return traverse_any_node<VAR == 1>(node, delta, tri, r, tmax, stk);
+}
+
+// -------------------------------------------------------------- trace ----
+// Tracer::trace -> trace_single + complete_trace_info.  Rays with a hit are
+// also appended (wave-aggregated) to the bundle's hit list, which the shadow
+// stage walks densely.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                              const float4* __restrict__ tri,
+                                              const float4* __restrict__ nrm, const float2* __restrict__ uvs,
+                                              const float4* __restrict__ ori4, const float4* __restrict__ dir4,
+                                              const uint32_t* __restrict__ dcount, uint64_t hcount,
+                                              uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                              int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                              float4* __restrict__ n4, float4* __restrict__ p4,
+                                              float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
+                                              uint32_t* __restrict__ hit_count) {
+    __shared__ uint32_t s_id[STACK * TB];
+    LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
-    const uint32_t tid = threadIdx.x;
-    const int nl = lights->count;
-    uint32_t my_shadow_rays = 0;
-    for (uint64_t r = blockIdx.x * (uint64_t)TB + tid; r < n; r += (uint64_t)gridDim.x * TB) {
-        uint32_t fl = flags[r] & 0xffu;
-        if (!(fl & 1u)) {
-            flags[r] = fl;
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t base = blockIdx.x * (uint64_t)TB; base < n; base += (uint64_t)gridDim.x * TB) {
+        const uint64_t rr = base + threadIdx.x;
+        const bool active = rr < n;
+        Closest best{0.f, 0.f, 0.f, -1};
+        RayC rc{};
+        if (active) {
+            const float4 o = ori4[rr], dd = dir4[rr];
+            rc = make_ray(xyz(o), xyz(dd), o.w);
+            best.t = dd.w;
+            closest_var<VAR>(node, delta, tri, rc, best, stk);
+        }
+        const bool hit = active && best.id >= 0;
+        const uint64_t hb = ballot(hit);
+        uint32_t wbase = 0;
+        if (hb) {
+            if (lane == (uint32_t)__builtin_ctzll(hb)) wbase = atomicAdd(hit_count, (uint32_t)__builtin_popcountll(hb));
+            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+        }
+        if (!active) continue;
+        if (!hit) {
+            flags[rr] = 0u;
+            tout[rr] = 0.f;
+            idout[rr] = -1;
+            bary[rr] = make_float2(0.f, 0.f);
+            n4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
+            p4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
+            uv2[rr] = make_float2(0.f, 0.f);
             continue;
         }
-        v3 p = xyz(p4[r]);
-        for (int l = 0; l < nl; ++l) {
-            const rtx_light& L = lights->l[l];
-            v3 dir;
-            float tmax = 1e37f;
-            bool shadowed = false, trace = true;
-            if (L.type == RTX_LIGHT_DIR) {
-                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
-            } else if (L.type == RTX_LIGHT_SPOT) {
-                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
-                if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
-                    shadowed = true;
-                    trace = false;
-                }
-            } else if (L.type == RTX_LIGHT_POINT) {
-                v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
-                tmax = __builtin_sqrtf(dot(toL, toL));
-                dir = normalize(toL);
-            } else {
+        hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
+        // complete_trace_info (src/kernel/trace-bvh.cl:190-245)
+        const v3 hp = rc.o + rc.d * best.t;
+        const uint32_t i0 = __float_as_uint(tri[3 * (size_t)best.id].w);
+        const uint32_t i1 = __float_as_uint(tri[3 * (size_t)best.id + 1].w);
+        const uint32_t i2 = __float_as_uint(tri[3 * (size_t)best.id + 2].w);
+        const float w = 1.f - (best.u + best.v);
+        const v3 n0 = xyz(nrm[i0]), n1 = xyz(nrm[i1]), n2 = xyz(nrm[i2]);
+        v3 nn = normalize(w * n0 + best.v * n2 + best.u * n1);
+        uint32_t fl = 1u;
+        if (dot(nn, rc.d) > 0.f) {
+            fl |= 2u;
+            nn = -nn;
+        }
+        const float2 t0 = uvs[i0], t1 = uvs[i1], t2 = uvs[i2];
+        flags[rr] = fl;
+        tout[rr] = best.t;
+        idout[rr] = best.id;
+        bary[rr] = make_float2(best.u, best.v);
+        n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
+        p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
+        uv2[rr] = make_float2(w * t0.x + best.v * t2.x + best.u * t1.x, w * t0.y + best.v * t2.y + best.u * t1.y);
+    }
+}
+
+// Hit list from the flags (when hits were written from the host).
+__global__ __launch_bounds__(256) void k_hit_list(const uint32_t* __restrict__ flags, const uint32_t* dcount,
+                                                  uint64_t hcount, uint32_t* __restrict__ hit_list,
+                                                  uint32_t* __restrict__ hit_count) {
+    const uint64_t n = ray_count(dcount, hcount);
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t base = blockIdx.x * 256ull; base < n; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t rr = base + threadIdx.x;
+        const bool hit = rr < n && (flags[rr] & 1u);
+        const uint64_t hb = ballot(hit);
+        uint32_t wbase = 0;
+        if (hb) {
+            if (lane == (uint32_t)__builtin_ctzll(hb)) wbase = atomicAdd(hit_count, (uint32_t)__builtin_popcountll(hb));
+            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+        }
+        if (hit) hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
+    }
+}
+
+__global__ void k_zero1(uint32_t* p) { *p = 0u; }
+
+// ------------------------------------------------------------- shadow ----
+// Hit lists of up to RTX_SEG_MAX bundles traced by one launch (the shadow
+// rays of every bounce of a frame are independent of each other).
+#define RTX_SEG_MAX 17
+struct ShadowSeg {
+    const uint32_t* list;
+    const uint32_t* count;
+    uint32_t* flags;
+    const float4* p4;
+};
+struct ShadowSegs {
+    ShadowSeg s[RTX_SEG_MAX];
+    int n;
+};
+
+// shadow_trace_single (src/kernel/shadow-trace-bvh.cl:375-418) for every
+// (hit, light) pair of the hit lists: DIR toward -dir, SPOT toward pos
+// (outside the cone = shadowed), POINT toward pos with tMax = distance; any
+// hit with tMin 0.01 < t < tMax sets bit 8+l of the ray's flags.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                               const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
+                                               ShadowSegs segs, unsigned long long* shadow_count) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_end[RTX_SEG_MAX];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const int nl = lights->count;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        uint64_t acc = 0;
+        for (int k = 0; k < segs.n; ++k) {
+            acc += (uint64_t)(*segs.s[k].count) * (uint64_t)nl;
+            s_end[k] = acc;
+        }
+    }
+    __syncthreads();
+    const uint64_t items = segs.n ? s_end[segs.n - 1] : 0;
+    uint32_t traced = 0;
+    for (uint64_t it = blockIdx.x * (uint64_t)TB + threadIdx.x; it < items; it += (uint64_t)gridDim.x * TB) {
+        int sg = 0;
+        while (it >= s_end[sg]) ++sg;
+        const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
+        const ShadowSeg& S = segs.s[sg];
+        const uint32_t rr = S.list[local / nl];
+        const int l = (int)(local % nl);
+        const rtx_light& L = lights->l[l];
+        const v3 p = xyz(S.p4[rr]);
+        v3 dir;
+        float tmax = 1e37f;
+        bool shadowed = false, trace = true;
+        if (L.type == RTX_LIGHT_DIR) {
+            dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+        } else if (L.type == RTX_LIGHT_SPOT) {
+            dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+            if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
                 shadowed = true;
                 trace = false;
             }
-            if (trace) {
-                my_shadow_rays++;
-                v3 inv = recip(dir);
-                const float tmin = 0.01f;
-                uint32_t level = 0, curr = 0;
-                for (;;) {
-                    float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-                    uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-                    bool pop = true;
-                    if (slab(xyz(f0), xyz(f1), p, inv, tmin, tmax)) {
-                        if (cnt) {
-                            for (uint32_t t = a; t < a + cnt; ++t) {
-                                float tt, u, v;
-                                if (tri_hit(tri, t, p, dir, &tt, &u, &v) && tt < tmax && tt > tmin) {
-                                    shadowed = true;
-                                    break;
-                                }
-                            }
-                            if (shadowed) break;
-                        } else {
-                            v3 dl = xyz(delta[curr]);
-                            bool near_left = dot(dir, dl) > 0.f;
-                            stack[level & (STACK - 1)][tid] = near_left ? a + 1 : a;
-                            level++;
-                            curr = near_left ? a : a + 1;
-                            pop = false;
-                        }
-                    }
-                    if (pop) {
-                        if (level > 0 && level < STACK) {
-                            level--;
-                            curr = stack[level][tid];
-                        } else {
-                            break;
-                        }
-                    }
-                }
-            }
-            if (shadowed) fl |= 1u << (8 + l);
+        } else if (L.type == RTX_LIGHT_POINT) {
+            const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+            tmax = __builtin_sqrtf(dot(toL, toL));
+            dir = normalize(toL);
+        } else {
+            shadowed = true;
+            trace = false;
         }
-        flags[r] = fl;
+        if (trace) {
+            ++traced;
+            const RayC rc = make_ray(p, dir, 0.01f);
+            shadowed = any_var<VAR>(node, delta, tri, rc, tmax, stk);
+        }
+        if (shadowed) atomicOr(&S.flags[rr], 1u << (8 + l));
     }
     if (shadow_count) {
-        atomicAdd(&s_cnt, my_shadow_rays);
+        if (traced) atomicAdd(&s_cnt, traced);
         __syncthreads();
         if (threadIdx.x == 0 && s_cnt) atomicAdd(shadow_count, (unsigned long long)s_cnt);
     }
 }
 
-uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap) {
+// Launch macro M(variant, block) with the tuned variant and block size.
+#define RTX_DISPATCH_TRAVERSAL(M, var, tb)     \
+    do {                                       \
+        if (tb == 64) {                        \
+            if (var == 0) M(0, 64);            \
+            else if (var == 1) M(1, 64);       \
+            else M(2, 64);                     \
+        } else if (tb == 256) {                \
+            if (var == 0) M(0, 256);           \
+            else if (var == 1) M(1, 256);      \
+            else M(2, 256);                    \
+        } else {                               \
+            if (var == 0) M(0, 128);           \
+            else if (var == 1) M(1, 128);      \
+            else M(2, 128);                    \
+        }                                      \
+    } while (0)
+
+uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
     uint64_t n = count == RTX_DEVICE_COUNT ? cap : count;
     uint64_t g = (n + TB - 1) / TB;
     uint64_t gmax = (uint64_t)c->cus * 64;  // grid-stride beyond 64 blocks per CU
@@ -385,7 +659,8 @@ int32_t rtx_hits_create(rtx_ctx* ctx, uint64_t capacity, rtx_hits** out) {
     if ((e = h->flags.reserve(ctx, capacity * 4)) || (e = h->t.reserve(ctx, capacity * 4)) ||
         (e = h->id.reserve(ctx, capacity * 4)) || (e = h->bary.reserve(ctx, capacity * 8)) ||
         (e = h->n4.reserve(ctx, capacity * 16)) || (e = h->p4.reserve(ctx, capacity * 16)) ||
-        (e = h->uv2.reserve(ctx, capacity * 8))) {
+        (e = h->uv2.reserve(ctx, capacity * 8)) || (e = h->list.reserve(ctx, capacity * 4)) ||
+        (e = h->list_count.reserve(ctx, 16))) {
         rtx_hits_destroy(h);
         return e;
     }
@@ -396,7 +671,7 @@ int32_t rtx_hits_create(rtx_ctx* ctx, uint64_t capacity, rtx_hits** out) {
 int32_t rtx_hits_destroy(rtx_hits* h) {
     RTX_REQUIRE(h, RTX_EINVAL, "rtx_hits_destroy: null");
     (void)hipStreamSynchronize(h->ctx->stream);
-    DevBuf* all[] = {&h->flags, &h->t, &h->id, &h->bary, &h->n4, &h->p4, &h->uv2};
+    DevBuf* all[] = {&h->flags, &h->t, &h->id, &h->bary, &h->n4, &h->p4, &h->uv2, &h->list, &h->list_count};
     for (DevBuf* b : all) b->release();
     delete h;
     return RTX_OK;
@@ -425,6 +700,7 @@ int32_t rtx_hits_write(rtx_hits* h, const uint32_t* flags, const float* t, const
     if (id) RTX_CHECK_HIP(hipMemcpy(h->id.p, id, n * 4, hipMemcpyHostToDevice));
     if (n4) RTX_CHECK_HIP(hipMemcpy(h->n4.p, n4, n * 16, hipMemcpyHostToDevice));
     if (p4) RTX_CHECK_HIP(hipMemcpy(h->p4.p, p4, n * 16, hipMemcpyHostToDevice));
+    h->list_valid = false;
     return RTX_OK;
 }
 
@@ -464,12 +740,66 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
                 "rtx_trace: hit bundle smaller than ray bundle");
     hipStream_t st = s->ctx->pick(stream);
     if (count == 0) return RTX_OK;
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
     ProfScope ps(s->ctx, st, "trace");
-    hipLaunchKernelGGL(k_trace, dim3(trace_grid(s->ctx, count, rays->cap)), dim3(TB), 0, st, s->node.as<float4>(),
-                       s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(), s->uv.as<float2>(),
-                       rays->ori4.as<float4>(), rays->dir4.as<float4>(), rays->count.as<uint32_t>(), count,
-                       hits->flags.as<uint32_t>(), hits->t.as<float>(), hits->id.as<int32_t>(),
-                       hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), hits->uv2.as<float2>());
+    const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+#define RTX_TRACE_LAUNCH(V, B)                                                                                       \
+    hipLaunchKernelGGL((k_trace<V, B>), dim3(trace_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,              \
+                       s->node.as<float4>(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),       \
+                       s->uv.as<float2>(), rays->ori4.as<float4>(), rays->dir4.as<float4>(),                        \
+                       rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
+                       hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
+                       hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>())
+    RTX_DISPATCH_TRAVERSAL(RTX_TRACE_LAUNCH, var, tb);
+#undef RTX_TRACE_LAUNCH
+    RTX_CHECK_HIP(hipGetLastError());
+    hits->list_valid = true;
+    hits->list_for = count;
+    return RTX_OK;
+}
+
+// Makes hits->list describe the first `count` rays of `rays` (when the hit
+// records did not come from rtx_trace over exactly those rays).
+static void ensure_hit_list(rtx_rays* rays, rtx_hits* hits, uint64_t count, hipStream_t st) {
+    if (hits->list_valid && hits->list_for == count) return;
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
+    uint64_t n = count == RTX_DEVICE_COUNT ? rays->cap : count;
+    uint32_t g = rtx_div_up(n ? n : 1, 256);
+    hipLaunchKernelGGL(k_hit_list, dim3(g > 4096 ? 4096 : g), dim3(256), 0, st, hits->flags.as<uint32_t>(),
+                       rays->count.as<uint32_t>(), count, hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>());
+    hits->list_valid = true;
+    hits->list_for = count;
+}
+
+int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* const* hits, const uint64_t* counts,
+                               int nseg, uint64_t* dev_shadow_count, hipStream_t st) {
+    RTX_REQUIRE(s && rays && hits && counts && nseg > 0 && nseg <= RTX_SEG_MAX, RTX_EINVAL,
+                "rtx_trace_shadow: bad segment list");
+    RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_trace_shadow: scene has no BVH");
+    RTX_REQUIRE(s->lights.p, RTX_ESTATE, "rtx_trace_shadow: scene has no lights");
+    ShadowSegs segs{};
+    uint64_t max_items = 0;
+    for (int k = 0; k < nseg; ++k) {
+        RTX_REQUIRE(rays[k] && hits[k], RTX_EINVAL, "rtx_trace_shadow: null bundle");
+        RTX_REQUIRE(counts[k] == RTX_DEVICE_COUNT || counts[k] <= hits[k]->cap, RTX_EINVAL,
+                    "rtx_trace_shadow: count too large");
+        ensure_hit_list(rays[k], hits[k], counts[k], st);
+        segs.s[k] = ShadowSeg{hits[k]->list.as<uint32_t>(), hits[k]->list_count.as<uint32_t>(),
+                              hits[k]->flags.as<uint32_t>(), hits[k]->p4.as<float4>()};
+        max_items += counts[k] == RTX_DEVICE_COUNT ? rays[k]->cap : counts[k];
+    }
+    segs.n = nseg;
+    const int nl = s->host_lights.count;
+    if (nl <= 0 || max_items == 0) return RTX_OK;
+    ProfScope ps(s->ctx, st, "trace_shadow");
+    uint64_t n = max_items * (uint64_t)nl;
+    const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+#define RTX_SHADOW_LAUNCH(V, B)                                                                                 \
+    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, s->node.as<float4>(), \
+                       s->delta.as<float4>(), s->tri.as<float4>(), s->lights.as<rtx_lights>(), segs,           \
+                       (unsigned long long*)dev_shadow_count)
+    RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_LAUNCH, var, tb);
+#undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -477,17 +807,8 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
 int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, uint64_t* dev_shadow_count,
                               hipStream_t st) {
     RTX_REQUIRE(s && rays && hits, RTX_EINVAL, "rtx_trace_shadow: null argument");
-    RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_trace_shadow: scene has no BVH");
-    RTX_REQUIRE(s->lights.p, RTX_ESTATE, "rtx_trace_shadow: scene has no lights");
-    RTX_REQUIRE(count == RTX_DEVICE_COUNT || count <= hits->cap, RTX_EINVAL, "rtx_trace_shadow: count too large");
     if (count == 0) return RTX_OK;
-    ProfScope ps(s->ctx, st, "trace_shadow");
-    hipLaunchKernelGGL(k_shadow, dim3(trace_grid(s->ctx, count, rays->cap)), dim3(TB), 0, st, s->node.as<float4>(),
-                       s->delta.as<float4>(), s->tri.as<float4>(), s->lights.as<rtx_lights>(),
-                       rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->p4.as<float4>(),
-                       (unsigned long long*)dev_shadow_count);
-    RTX_CHECK_HIP(hipGetLastError());
-    return RTX_OK;
+    return rtx_trace_shadow_multi(s, &rays, &hits, &count, 1, dev_shadow_count, st);
 }
 
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {

@@ -79,7 +79,8 @@ class RenderParams(C.Structure):
 # Every symbol include/rtx.h declares (checked by tests/test_abi.py).
 SYMBOLS = [
     "rtx_init", "rtx_shutdown", "rtx_last_error", "rtx_abi_version", "rtx_stream", "rtx_sync",
-    "rtx_profile_enable", "rtx_profile_report", "rtx_profile_reset", "rtx_mem_in_use",
+    "rtx_profile_enable", "rtx_profile_report", "rtx_profile_reset", "rtx_mem_in_use", "rtx_tune_set",
+    "rtx_tune_get",
     "rtx_scene_create", "rtx_scene_destroy", "rtx_scene_set_vertices", "rtx_scene_set_vertices_ref",
     "rtx_scene_update_positions", "rtx_scene_set_triangles", "rtx_scene_set_materials", "rtx_scene_set_lights",
     "rtx_scene_triangle_count", "rtx_scene_displace_waves",
@@ -119,6 +120,8 @@ def lib() -> C.CDLL:
         "rtx_profile_report": (C.c_int64, [P, C.c_char_p, U64]),
         "rtx_profile_reset": (I32, [P]),
         "rtx_mem_in_use": (U64, [P]),
+        "rtx_tune_set": (I32, [P, C.c_char_p, C.c_int64]),
+        "rtx_tune_get": (C.c_int64, [P, C.c_char_p]),
         "rtx_scene_create": (I32, [P, C.POINTER(P)]),
         "rtx_scene_destroy": (I32, [P]),
         "rtx_scene_set_vertices": (I32, [P, P, P, P, U32]),
@@ -264,6 +267,12 @@ class Context:
 
     def mem_in_use(self) -> int:
         return int(lib().rtx_mem_in_use(self.h))
+
+    def tune(self, key: str, value: int):
+        _check(lib().rtx_tune_set(self.h, key.encode(), int(value)), "rtx_tune_set")
+
+    def tuned(self, key: str) -> int:
+        return int(lib().rtx_tune_get(self.h, key.encode()))
 
     def __enter__(self):
         return self

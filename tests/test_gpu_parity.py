@@ -320,3 +320,44 @@ def test_frame_rebuild_c1(ctx, c1):
     assert np.array_equal(acc, oacc)
     R.destroy()
     s.destroy()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_traversal_variants_identical(ctx, c1, variant, block):
+    # every tuned traversal form reproduces the oracle's hits and shadow flags
+    rtx = _rtx()
+    sd, s, b, cam = c1
+    w, h = 200, 150
+    n = w * h
+    r = rtx.Rays(ctx, 2 * n)
+    r2 = rtx.Rays(ctx, 2 * n)
+    hb = rtx.Hits(ctx, 2 * n)
+    old = (ctx.tuned("trace_variant"), ctx.tuned("trace_block"))
+    ctx.tune("trace_variant", variant)
+    ctx.tune("trace_block", block)
+    try:
+        r.primary(cam, w, h)
+        rtx.trace(s, r, hb, n)
+        rtx.trace_shadow(s, r, hb, n)
+        g = hb.read(n)
+        m = rtx.raygen_secondary(s, r, hb, r2, n)
+        rtx.trace(s, r2, hb, m)
+        rtx.trace_shadow(s, r2, hb, m)
+        g2 = hb.read(m)
+    finally:
+        ctx.tune("trace_variant", old[0])
+        ctx.tune("trace_block", old[1])
+    lights = O.lights_from(sd.lights)
+    oo, od, op, oc = O.raygen_primary(O.camera_from(cam), w, h)
+    o = O.trace(b, sd.nrm, sd.uv, oo, od)
+    o["flags"] = O.trace_shadow(b, lights, o["flags"], o["p4"])
+    for k in ("flags", "id", "t", "bary", "n4", "p4"):
+        assert np.array_equal(g[k], o[k]), f"primary {k} differs (variant {variant}, block {block})"
+    so, sdir, sp, sc, _ = O.raygen_secondary(sd.materials, b.matmap, oo, od, op, oc, o, 2 * n)
+    o2 = O.trace(b, sd.nrm, sd.uv, so, sdir)
+    o2["flags"] = O.trace_shadow(b, lights, o2["flags"], o2["p4"])
+    for k in ("flags", "id", "t", "bary", "n4", "p4"):
+        assert np.array_equal(g2[k], o2[k]), f"secondary {k} differs (variant {variant}, block {block})"
+    for x in (r, r2, hb):
+        x.destroy()
