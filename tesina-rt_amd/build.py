@@ -71,14 +71,35 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_tools()
     if verbose:
         print(f"built {LIB}")
     return LIB
 
 
+TOOLS = {"stage_render": ROOT / "tests" / "cpp" / "stage_render.cpp"}
+BIN = PKG / "bin"
+
+
+def build_tools() -> None:
+    """C++ programs that drive the library through include/rt/stages.hpp
+    (host code only: plain g++ against librtx.so)."""
+    BIN.mkdir(exist_ok=True)
+    for name, src in TOOLS.items():
+        if not src.exists():
+            continue
+        cmd = ["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(BIN / name),
+               f"-L{PKG}", "-lrtx", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"tool build failed: {name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
 def clean() -> None:
     if OBJ.exists():
         shutil.rmtree(OBJ)
+    if BIN.exists():
+        shutil.rmtree(BIN)
     if LIB.exists():
         LIB.unlink()
 

@@ -1,0 +1,629 @@
+// C++ stage classes (include/rt/stages.hpp) over the C ABI.  Each method
+// cites the reference method it stands in for; results go through the same
+// gfx950 kernels as the C ABI, so these classes add no device code.
+#include "rt/stages.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+
+namespace {
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); }
+rtx_ctx* dev() { return DeviceInterface::instance()->ctx(); }
+void report(const char* what) { std::cerr << what << ": " << rtx_last_error() << "\n"; }
+}  // namespace
+
+// ------------------------------------------------------ DeviceInterface --
+DeviceInterface* DeviceInterface::instance() {
+    static DeviceInterface di;
+    return &di;
+}
+
+int32_t DeviceInterface::initialize(int device) {
+    if (m_ctx) return 0;
+    if (rtx_init(device, &m_ctx) != RTX_OK) {
+        report("DeviceInterface::initialize");
+        m_ctx = nullptr;
+        return -1;
+    }
+    return 0;
+}
+
+void DeviceInterface::finish_commands() {
+    if (m_ctx) rtx_sync(m_ctx, nullptr);
+}
+
+int32_t DeviceInterface::shutdown() {
+    if (!m_ctx) return 0;
+    int32_t e = rtx_shutdown(m_ctx);
+    m_ctx = nullptr;
+    return e ? -1 : 0;
+}
+
+// --------------------------------------------------------------- Camera --
+Camera::Camera() : pos{0, 0, 0}, dir{0, 0, 1}, right{1, 0, 0}, up{0, 1, 0}, fov(0.785398f), aspect(1.f),
+                   right_handed(false) {}
+
+Camera::Camera(const vec3& position, const vec3& direction, const vec3& u, float f, float a, bool rh) : Camera() {
+    fov = f;
+    aspect = a;
+    set(position, direction, u, f, a, rh);
+}
+
+// Camera::set (src/rt/camera.cpp:31-59).
+void Camera::set(const vec3& position, const vec3& direction, const vec3& u, float f, float a, bool rh) {
+    if (f > 0.f) fov = f;
+    if (a > 0.f) aspect = a;
+    right_handed = rh;
+    rtx_camera c{};
+    const float p[3] = {position.x, position.y, position.z};
+    const float d[3] = {direction.x, direction.y, direction.z};
+    const float uu[3] = {u.x, u.y, u.z};
+    rtx_camera_set(&c, p, d, uu, f, aspect, rh ? 1 : 0);
+    pos = vec3{c.pos[0], c.pos[1], c.pos[2]};
+    dir = vec3{c.dir[0], c.dir[1], c.dir[2]};
+    right = vec3{c.right[0], c.right[1], c.right[2]};
+    up = vec3{c.up[0], c.up[1], c.up[2]};
+}
+
+rtx_camera Camera::basis() const {
+    rtx_camera c{};
+    const vec3* v[4] = {&pos, &dir, &right, &up};
+    float* o[4] = {c.pos, c.dir, c.right, c.up};
+    for (int k = 0; k < 4; ++k) {
+        o[k][0] = v[k]->x;
+        o[k][1] = v[k]->y;
+        o[k][2] = v[k]->z;
+    }
+    return c;
+}
+
+// ------------------------------------------------------- RendererConfig --
+// Defaults of src/rt/renderer-config.cpp:4-16.
+RendererConfig::RendererConfig()
+    : use_lbvh(1), bvh_refit_only(0), bvh_depth(32), bvh_min_leaf_size(1), sec_ray_use_atomics(0),
+      sec_ray_use_disc(1), prim_ray_quad_size(32), prim_ray_use_zcurve(0), tile_to_cores_ratio(128) {}
+
+// ---------------------------------------------------------------- Scene --
+Scene::Scene() { m_lights.count = 0; }
+Scene::~Scene() { destroy(); }
+
+int32_t Scene::initialize() {
+    if (m_h) return 0;
+    if (!DeviceInterface::instance()->good() && DeviceInterface::instance()->initialize()) return -1;
+    if (rtx_scene_create(dev(), &m_h) != RTX_OK) {
+        report("Scene::initialize");
+        return -1;
+    }
+    return 0;
+}
+
+int32_t Scene::destroy() {
+    if (m_h) rtx_scene_destroy(m_h);
+    m_h = nullptr;
+    return 0;
+}
+
+int32_t Scene::set_geometry(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
+                            const int32_t* matmap, uint32_t T) {
+    if (!m_h || rtx_scene_set_vertices(m_h, pos, nrm, uv, V) != RTX_OK) return -1;
+    m_V = V;
+    return set_triangles(idx, matmap, T);
+}
+
+int32_t Scene::set_vertices_ref(const void* vertex80, uint32_t V) {
+    if (!m_h || rtx_scene_set_vertices_ref(m_h, vertex80, V) != RTX_OK) return -1;
+    m_V = V;
+    m_built = false;
+    return 0;
+}
+
+int32_t Scene::set_triangles(const uint32_t* idx, const int32_t* matmap, uint32_t T) {
+    if (!m_h || rtx_scene_set_triangles(m_h, idx, matmap, T) != RTX_OK) return -1;
+    m_T = T;
+    m_built = false;
+    return 0;
+}
+
+int32_t Scene::update_vertex_positions(const float* pos) {
+    if (!m_h || rtx_scene_update_positions(m_h, pos, nullptr) != RTX_OK) return -1;
+    DeviceInterface::instance()->finish_commands();  // the host array may be released on return
+    return 0;
+}
+
+int32_t Scene::set_materials(const std::vector<rtx_material>& mats) {
+    if (!m_h || mats.empty()) return -1;
+    return rtx_scene_set_materials(m_h, mats.data(), (uint32_t)mats.size()) == RTX_OK ? 0 : -1;
+}
+
+int32_t Scene::push_lights() { return rtx_scene_set_lights(m_h, &m_lights) == RTX_OK ? 0 : -1; }
+
+// Scene::set_dir_light normalizes the direction on the host (src/rt/scene.cpp:987-1003).
+int32_t Scene::set_dir_light(const vec3& d, const vec3& color) {
+    if (!m_h) return -1;
+    float n = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    if (!(n > 0.f)) return -1;
+    rtx_light L{};
+    L.type = RTX_LIGHT_DIR;
+    L.dir[0] = d.x / n;
+    L.dir[1] = d.y / n;
+    L.dir[2] = d.z / n;
+    L.color[0] = color.x;
+    L.color[1] = color.y;
+    L.color[2] = color.z;
+    m_lights.l[0] = L;
+    m_lights.count = 1;
+    return push_lights();
+}
+
+int32_t Scene::set_spot_light(const vec3& p, const vec3& d, const vec3& color, float angle, float radius) {
+    if (!m_h) return -1;
+    rtx_light L{};
+    L.type = RTX_LIGHT_SPOT;
+    L.pos[0] = p.x, L.pos[1] = p.y, L.pos[2] = p.z;
+    L.dir[0] = d.x, L.dir[1] = d.y, L.dir[2] = d.z;
+    L.color[0] = color.x, L.color[1] = color.y, L.color[2] = color.z;
+    L.angle = angle;
+    L.radius = radius;
+    m_lights.l[0] = L;
+    m_lights.count = 1;
+    return push_lights();
+}
+
+int32_t Scene::set_ambient_light(const vec3& c) {
+    if (!m_h) return -1;
+    m_lights.ambient[0] = c.x;
+    m_lights.ambient[1] = c.y;
+    m_lights.ambient[2] = c.z;
+    return push_lights();
+}
+
+int32_t Scene::add_point_light(const vec3& p, const vec3& color) {
+    if (!m_h || m_lights.count >= RTX_MAX_LIGHTS) return -1;
+    rtx_light& L = m_lights.l[m_lights.count++];
+    L = rtx_light{};
+    L.type = RTX_LIGHT_POINT;
+    L.pos[0] = p.x, L.pos[1] = p.y, L.pos[2] = p.z;
+    L.color[0] = color.x, L.color[1] = color.y, L.color[2] = color.z;
+    return push_lights();
+}
+
+int32_t Scene::set_lights(const rtx_lights& lights) {
+    if (!m_h) return -1;
+    m_lights = lights;
+    return push_lights();
+}
+
+// -------------------------------------------------------------- bundles --
+RayBundle::~RayBundle() {
+    if (m_h) rtx_rays_destroy(m_h);
+}
+
+// RayBundle::initialize (src/rt/ray.cpp:48-66): zero rays is an error.
+int32_t RayBundle::initialize(size_t rays) {
+    if (!rays || !DeviceInterface::instance()->good()) return -1;
+    if (m_h) rtx_rays_destroy(m_h);
+    m_h = nullptr;
+    if (rtx_rays_create(dev(), rays, &m_h) != RTX_OK) {
+        report("RayBundle::initialize");
+        m_h = nullptr;
+        return -1;
+    }
+    m_count = rays;
+    return 0;
+}
+
+int32_t RayBundle::resize(size_t rays) { return initialize(rays); }
+
+HitBundle::~HitBundle() {
+    if (m_h) rtx_hits_destroy(m_h);
+}
+
+int32_t HitBundle::initialize(size_t sz) {
+    if (!sz || !DeviceInterface::instance()->good()) return -1;
+    if (m_h) rtx_hits_destroy(m_h);
+    m_h = nullptr;
+    if (rtx_hits_create(dev(), sz, &m_h) != RTX_OK) {
+        report("HitBundle::initialize");
+        m_h = nullptr;
+        return -1;
+    }
+    m_count = sz;
+    return 0;
+}
+
+int32_t HitBundle::resize(size_t sz) { return initialize(sz); }
+
+// ----------------------------------------------------------- FrameBuffer --
+FrameBuffer::~FrameBuffer() {
+    if (m_h) rtx_fb_destroy(m_h);
+}
+
+int32_t FrameBuffer::initialize(size_t sz[2]) {
+    if (!DeviceInterface::instance()->good() || !sz[0] || !sz[1]) return -1;
+    if (m_h) rtx_fb_destroy(m_h);
+    m_h = nullptr;
+    if (rtx_fb_create(dev(), (int32_t)sz[0], (int32_t)sz[1], &m_h) != RTX_OK) {
+        report("FrameBuffer::initialize");
+        m_h = nullptr;
+        return -1;
+    }
+    m_size[0] = sz[0];
+    m_size[1] = sz[1];
+    return 0;
+}
+
+int32_t FrameBuffer::resize(size_t sz[2]) { return initialize(sz); }
+
+// FrameBuffer::clear (src/rt/framebuffer.cpp:118-137).
+int32_t FrameBuffer::clear() {
+    if (!m_h) return -1;
+    auto t = clk::now();
+    if (rtx_fb_clear(m_h, nullptr) != RTX_OK) return -1;
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_clear_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// FrameBuffer::copy (src/rt/framebuffer.cpp:141-165), into the device RGBA8 image.
+int32_t FrameBuffer::copy() {
+    if (!m_h) return -1;
+    auto t = clk::now();
+    if (rtx_fb_resolve(m_h, nullptr) != RTX_OK) return -1;
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_copy_ms = ms_since(t);
+    }
+    return 0;
+}
+
+int32_t FrameBuffer::read_rgba8(uint8_t* rgba) const {
+    return m_h && rtx_fb_read_rgba8(m_h, rgba) == RTX_OK ? 0 : -1;
+}
+
+int32_t FrameBuffer::read_accum(uint32_t* rgb) const {
+    return m_h && rtx_fb_read_accum(m_h, rgb) == RTX_OK ? 0 : -1;
+}
+
+// ------------------------------------------------------------ BVHBuilder --
+int32_t BVHBuilder::initialize() {
+    if (!DeviceInterface::instance()->good()) return -1;
+    m_depth = 32;
+    m_min_leaf = 1;
+    return 0;
+}
+
+// BVHBuilder::update_configuration clamps (src/rt/bvh-builder.cpp:194-199);
+// this build's codes are 32-bit, so depth is clamped to 32.
+void BVHBuilder::update_configuration(const RendererConfig& conf) {
+    m_depth = std::min(std::max(conf.bvh_depth, 1), 32);
+    m_min_leaf = std::min(std::max(conf.bvh_min_leaf_size, 1), 256);
+}
+
+// BVHBuilder::build_lbvh (src/rt/bvh-builder.cpp:201-741): permutes the
+// scene's index buffer and material map in place and records the node
+// counts on the scene.
+int32_t BVHBuilder::build_lbvh(Scene& scene, size_t cq_i) {
+    (void)cq_i;
+    if (!scene.valid()) return -1;
+    auto t = clk::now();
+    rtx_lbvh_params p{m_depth, m_min_leaf};
+    if (rtx_lbvh_build(scene.handle(), &p, nullptr) != RTX_OK) {
+        report("BVHBuilder::build_lbvh");
+        return -1;
+    }
+    if (rtx_lbvh_info(scene.handle(), &scene.bvh_node_count, scene.bvh_level_node_count) != RTX_OK) return -1;
+    if (m_timing) m_time_ms = ms_since(t);
+    scene.m_built = true;
+    return 0;
+}
+
+// BVHBuilder::refit_lbvh (src/rt/bvh-builder.cpp:743-850).
+int32_t BVHBuilder::refit_lbvh(Scene& scene, size_t cq_i) {
+    (void)cq_i;
+    if (!scene.valid() || !scene.ready()) return -1;
+    auto t = clk::now();
+    if (rtx_lbvh_refit(scene.handle(), nullptr) != RTX_OK) {
+        report("BVHBuilder::refit_lbvh");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_time_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// --------------------------------------------------- PrimaryRayGenerator --
+int32_t PrimaryRayGenerator::initialize() {
+    if (m_initialized || !DeviceInterface::instance()->good()) return -1;
+    m_samples = {0.f, 0.f, 1.f};
+    m_quad = 32;
+    m_initialized = true;
+    return 0;
+}
+
+int32_t PrimaryRayGenerator::set_spp(size_t spp, pixel_sample_cl const* ps) {
+    if (!spp || !ps) return -1;
+    m_samples.resize(3 * spp);
+    for (size_t i = 0; i < spp; ++i) {
+        m_samples[3 * i] = ps[i].ox;
+        m_samples[3 * i + 1] = ps[i].oy;
+        m_samples[3 * i + 2] = ps[i].contribution;
+    }
+    return 0;
+}
+
+void PrimaryRayGenerator::update_configuration(const RendererConfig& conf) {
+    m_quad = conf.prim_ray_quad_size > 0 ? conf.prim_ray_quad_size : 32;
+}
+
+// PrimaryRayGenerator::generate (src/rt/primary-ray-generator.cpp:58-165).
+int32_t PrimaryRayGenerator::generate(const Camera& cam, RayBundle& bundle, size_t size[2], const size_t ray_count,
+                                      const size_t offset) {
+    if (!m_initialized || !bundle.valid()) return -1;
+    auto t = clk::now();
+    rtx_camera c = cam.basis();
+    const int spp = (int)get_spp();
+    if (rtx_raygen_primary(bundle.handle(), &c, (int32_t)size[0], (int32_t)size[1], m_quad, spp, m_samples.data(),
+                           offset, ray_count, nullptr) != RTX_OK) {
+        report("PrimaryRayGenerator::generate");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_time_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- Tracer --
+int32_t Tracer::initialize() {
+    if (!DeviceInterface::instance()->good()) return -1;
+    m_initialized = true;
+    return 0;
+}
+
+// Tracer::trace (src/rt/tracer.cpp:140-269).
+int32_t Tracer::trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary) {
+    (void)secondary;
+    if (!m_initialized || ray_count < 0 || !scene.ready()) return -1;
+    auto t = clk::now();
+    if (rtx_trace(scene.handle(), rays.handle(), hits.handle(), (uint64_t)ray_count, nullptr) != RTX_OK) {
+        report("Tracer::trace");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_trace_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// Tracer::shadow_trace (src/rt/tracer.cpp:273-352).
+int32_t Tracer::shadow_trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary) {
+    (void)secondary;
+    if (!m_initialized || ray_count < 0 || !scene.ready()) return -1;
+    auto t = clk::now();
+    if (rtx_trace_shadow(scene.handle(), rays.handle(), hits.handle(), (uint64_t)ray_count, nullptr) != RTX_OK) {
+        report("Tracer::shadow_trace");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_shadow_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// ------------------------------------------------- SecondaryRayGenerator --
+int32_t SecondaryRayGenerator::initialize() {
+    if (!DeviceInterface::instance()->good()) return -1;
+    m_initialized = true;
+    return 0;
+}
+
+// SecondaryRayGenerator::generate, scan form (src/rt/secondary-ray-generator.cpp:57-145,
+// 248-340): the new count is read back, as in the reference.
+int32_t SecondaryRayGenerator::generate(Scene& scene, RayBundle& ray_in, size_t rays_in, HitBundle& hits,
+                                        RayBundle& ray_out, size_t* rays_out) {
+    if (!m_initialized || !rays_out) return -1;
+    auto t = clk::now();
+    uint64_t n = 0;
+    if (rtx_raygen_secondary(scene.handle(), ray_in.handle(), hits.handle(), ray_out.handle(), rays_in, &n,
+                             nullptr) != RTX_OK) {
+        report("SecondaryRayGenerator::generate");
+        return -1;
+    }
+    if (n > 2 * rays_in) {
+        std::cerr << "Error: secondary ray generator reported wrong number of new rays.\n";
+        return -1;
+    }
+    *rays_out = (size_t)n;
+    if (m_timing) m_time_ms = ms_since(t);
+    return 0;
+}
+
+// ------------------------------------------------------------- RayShader --
+int32_t RayShader::initialize() {
+    if (!DeviceInterface::instance()->good()) return -1;
+    m_initialized = true;
+    return 0;
+}
+
+// RayShader::shade (src/rt/ray-shader.cpp:33-154).
+int32_t RayShader::shade(RayBundle& rays, HitBundle& hb, Scene& scene, Cubemap& cm, FrameBuffer& fb, size_t size,
+                         bool primary) {
+    (void)cm;
+    if (!m_initialized) return -1;
+    auto t = clk::now();
+    if (rtx_shade(scene.handle(), rays.handle(), hb.handle(), fb.handle(), size, primary ? 1 : 0, nullptr) !=
+        RTX_OK) {
+        report("RayShader::shade");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_time_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// -------------------------------------------------------------- Renderer --
+Renderer::Renderer() {}
+
+// Renderer::configure_from_defaults (src/rt/renderer.cpp:279-295).
+uint32_t Renderer::configure_from_defaults() {
+    fb_w = 512;
+    fb_h = 512;
+    static_bvh = false;
+    max_bounces = 9;
+    config.bvh_depth = 32;
+    config.tile_to_cores_ratio = 128;
+    config.use_lbvh = 1;
+    config.bvh_refit_only = 0;
+    config.sec_ray_use_disc = 0;
+    config.prim_ray_quad_size = 32;
+    config.prim_ray_use_zcurve = 0;
+    return 0;
+}
+
+// Renderer::initialize (src/rt/renderer.cpp:350-453).
+uint32_t Renderer::initialize() {
+    if (initialized) return 0;
+    if (!DeviceInterface::instance()->good() && DeviceInterface::instance()->initialize()) return (uint32_t)-1;
+    size_t sz[2] = {fb_w, fb_h};
+    if (m_framebuffer.initialize(sz) || bvh_builder.initialize() || prim_ray_gen.initialize() ||
+        tracer.initialize() || sec_ray_gen.initialize() || ray_shader.initialize())
+        return (uint32_t)-1;
+    bvh_builder.timing(true);
+    prim_ray_gen.timing(true);
+    tracer.timing(true);
+    sec_ray_gen.timing(true);
+    ray_shader.timing(true);
+    m_framebuffer.timing(true);
+    tile_size = 0;
+    initialized = true;
+    return update_configuration();
+}
+
+uint32_t Renderer::resize_output(size_t width, size_t height) {
+    fb_w = width;
+    fb_h = height;
+    size_t sz[2] = {fb_w, fb_h};
+    if (initialized && m_framebuffer.resize(sz)) return (uint32_t)-1;
+    return initialized ? update_configuration() : 0;
+}
+
+int32_t Renderer::set_samples_per_pixel(size_t spp, pixel_sample_cl const* ps) {
+    return prim_ray_gen.set_spp(spp, ps);
+}
+
+// Renderer::update_configuration (src/rt/renderer.cpp:79-111): one tile of
+// CUs * 1024 * tile_to_cores_ratio rays at most (the whole frame on MI355X),
+// ray bundles of 3 tiles.
+uint32_t Renderer::update_configuration() {
+    const size_t old_tile = tile_size;
+    const size_t pixels = fb_w * fb_h;
+    size_t tile = (size_t)(256.0 * 1024.0 * config.tile_to_cores_ratio);
+    tile_size = std::min(pixels * get_samples_per_pixel(), std::max<size_t>(tile, 1));
+    if (tile_size != old_tile) {
+        const size_t bundle = tile_size * 3;
+        if (ray_bundle_1.resize(bundle) || ray_bundle_2.resize(bundle) || hit_bundle.resize(bundle)) {
+            std::cerr << "Error resizing ray and hit bundles.\n";
+            return (uint32_t)-1;
+        }
+        sec_ray_gen.set_max_rays(bundle);
+    }
+    bvh_builder.update_configuration(config);
+    prim_ray_gen.update_configuration(config);
+    sec_ray_gen.update_configuration(config);
+    tracer.update_configuration(config);
+    ray_shader.update_configuration(config);
+    return 0;
+}
+
+// Renderer::set_up_frame (src/rt/renderer.cpp:12-51).
+uint32_t Renderer::set_up_frame(Scene& scene) {
+    frame_start = clk::now();
+    if (!initialized) return (uint32_t)-1;
+    stats.clear_times();
+    stats.total_ray_count = 0;
+    stats.total_sec_ray_count = 0;
+    if (m_framebuffer.clear()) return (uint32_t)-1;
+    stats.stage_times[FB_CLEAR] = m_framebuffer.get_clear_exec_time();
+    return update_accelerator(scene);
+}
+
+// Renderer::update_accelerator (src/rt/renderer.cpp:53-77).
+uint32_t Renderer::update_accelerator(Scene& scene) {
+    if (config.use_lbvh && !(static_bvh && scene.ready())) {
+        if (config.bvh_refit_only && scene.ready()) {
+            if (bvh_builder.refit_lbvh(scene)) return (uint32_t)-1;
+        } else if (bvh_builder.build_lbvh(scene)) {
+            return (uint32_t)-1;
+        }
+    }
+    stats.stage_times[BVH_BUILD] = bvh_builder.get_exec_time();
+    return 0;
+}
+
+// Renderer::render_to_framebuffer (src/rt/renderer.cpp:113-214).
+uint32_t Renderer::render_to_framebuffer(Scene& scene) {
+    const size_t pixel_count = fb_w * fb_h;
+    const size_t sample_count = pixel_count * prim_ray_gen.get_spp();
+    size_t fb_size[2] = {fb_w, fb_h};
+    for (size_t offset = 0; offset < sample_count; offset += tile_size) {
+        RayBundle* ray_in = &ray_bundle_1;
+        RayBundle* ray_out = &ray_bundle_2;
+        const size_t cur = std::min(tile_size, sample_count - offset);
+        if (prim_ray_gen.generate(scene.camera, *ray_in, fb_size, cur, offset)) return (uint32_t)-1;
+        stats.stage_times[PRIM_RAY_GEN] += prim_ray_gen.get_exec_time();
+        stats.total_ray_count += cur;
+        if (tracer.trace(scene, (int32_t)cur, *ray_in, hit_bundle)) return (uint32_t)-1;
+        stats.stage_times[PRIM_TRACE] += tracer.get_trace_exec_time();
+        if (tracer.shadow_trace(scene, (int32_t)cur, *ray_in, hit_bundle)) return (uint32_t)-1;
+        stats.stage_times[PRIM_SHADOW_TRACE] += tracer.get_shadow_exec_time();
+        if (ray_shader.shade(*ray_in, hit_bundle, scene, cubemap, m_framebuffer, cur, true)) return (uint32_t)-1;
+        stats.stage_times[SHADE] += ray_shader.get_exec_time();
+        size_t sec = cur;
+        for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
+            size_t sec_in = sec;
+            if (sec_ray_gen.generate(scene, *ray_in, sec_in, hit_bundle, *ray_out, &sec)) return (uint32_t)-1;
+            stats.stage_times[SEC_RAY_GEN] += sec_ray_gen.get_exec_time();
+            std::swap(ray_in, ray_out);
+            if (!sec) break;
+            stats.total_ray_count += sec;
+            stats.total_sec_ray_count += sec;
+            if (tracer.trace(scene, (int32_t)sec, *ray_in, hit_bundle, true)) return (uint32_t)-1;
+            stats.stage_times[SEC_TRACE] += tracer.get_trace_exec_time();
+            if (tracer.shadow_trace(scene, (int32_t)sec, *ray_in, hit_bundle, true)) return (uint32_t)-1;
+            stats.stage_times[SEC_SHADOW_TRACE] += tracer.get_shadow_exec_time();
+            if (ray_shader.shade(*ray_in, hit_bundle, scene, cubemap, m_framebuffer, sec)) return (uint32_t)-1;
+            stats.stage_times[SHADE] += ray_shader.get_exec_time();
+        }
+    }
+    return 0;
+}
+
+// Renderer::copy_framebuffer (src/rt/renderer.cpp:216-227).
+uint32_t Renderer::copy_framebuffer() {
+    if (m_framebuffer.copy()) return (uint32_t)-1;
+    stats.stage_times[FB_COPY] = m_framebuffer.get_copy_exec_time();
+    return 0;
+}
+
+// Renderer::conclude_frame (src/rt/renderer.cpp:248-272).
+uint32_t Renderer::conclude_frame(Scene& scene) {
+    (void)scene;
+    DeviceInterface::instance()->finish_commands();
+    stats.frame_time = ms_since(frame_start);
+    stats.frame_acc_time += stats.frame_time;
+    for (int i = 0; i < STAGE_COUNT; ++i) stats.stage_acc_times[i] += stats.stage_times[i];
+    stats.acc_frames++;
+    return 0;
+}

@@ -1,0 +1,415 @@
+"""Pins every oracle stage other than the hierarchy against checks that do
+not restate its code: hand-computed Morton vectors and special cases, numpy's
+stable sort, closed-form ray directions and pixel coverage, brute-force and
+analytic hits, textbook reflection and Schlick values, closed-form shading
+and resolve values.
+
+Semantics the reference leaves to its OpenCL compiler (normalize, pow,
+min/max on NaN, write_imagef rounding) are defined by the oracle as IEEE
+single-precision sequences; where a check below can only confirm such a
+definition (not the reference's vendor result) it says so.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ------------------------------------------------------------- Morton (a3) --
+def morton_expected(m, depth):
+    """Bit i of the code takes axis (off+1+i)%3 and source bit (i+off)/3
+    (src/kernel/bvh-builder.cl:231-249), written as an explicit bit string."""
+    off = (3 - depth % 3) % 3
+    bits = []
+    for i in range(depth):
+        axis = (off + 1 + i) % 3
+        bits.append((m[axis] >> ((i + off) // 3)) & 1)
+    return sum(b << i for i, b in enumerate(bits))
+
+
+def test_morton_depth30_layout_msb_is_x9():
+    unit_lo, unit_hi = np.zeros(3, np.float32), np.ones(3, np.float32)
+    # centroid (0.5, 0, 0): x = 512 -> bit 9 of x -> code MSB (bit 29)
+    c = np.array([0.5, 0.0, 0.0], np.float32)
+    assert O.morton_one(c, c, unit_lo, unit_hi, 30) == 1 << 29
+    # y = 512 -> y9 is bit 27 (x9 z9 y9 ...); z = 512 -> bit 28
+    c = np.array([0.0, 0.5, 0.0], np.float32)
+    assert O.morton_one(c, c, unit_lo, unit_hi, 30) == 1 << 27
+    c = np.array([0.0, 0.0, 0.5], np.float32)
+    assert O.morton_one(c, c, unit_lo, unit_hi, 30) == 1 << 28
+    # smallest cell: x = 1/1024 -> x0 = bit 2; y0 = bit 0; z0 = bit 1
+    c = np.array([1 / 1024, 0.0, 0.0], np.float32)
+    assert O.morton_one(c, c, unit_lo, unit_hi, 30) == 1 << 2
+    c = np.array([0.0, 1 / 1024, 1 / 1024], np.float32)
+    assert O.morton_one(c, c, unit_lo, unit_hi, 30) == (1 << 0) | (1 << 1)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 8, 10, 24, 29, 30, 31, 32])
+def test_morton_random_against_bit_string(depth):
+    rng = np.random.default_rng(depth)
+    k = (depth + 2) // 3
+    glo = np.array([-2.0, 1.0, -8.0], np.float32)
+    ghi = np.array([6.0, 5.0, 8.0], np.float32)  # power-of-two extents: exact scaling
+    for _ in range(200):
+        m = rng.integers(0, 1 << k, 3)
+        c = (glo + (m + 0.25) / (1 << k) * (ghi - glo)).astype(np.float32)
+        assert O.morton_one(c, c, glo, ghi, depth) == morton_expected(m, depth)
+
+
+def test_morton_depth32_drops_y0():
+    # depth 32: MSB->LSB x10 z10 y10 ... x0 z0; y0 never appears
+    unit_lo, unit_hi = np.zeros(3, np.float32), np.ones(3, np.float32)
+    c = np.array([0.0, 1 / 2048, 0.0], np.float32)  # y = 1 -> only y0
+    assert O.morton_one(c, c, unit_lo, unit_hi, 32) == 0
+    c = np.array([1 / 2048, 0.0, 0.0], np.float32)  # x0 -> bit 1
+    assert O.morton_one(c, c, unit_lo, unit_hi, 32) == 1 << 1
+
+
+def test_morton_max_corner_wraps_and_flat_axis():
+    lo, hi = np.zeros(3, np.float32), np.ones(3, np.float32)
+    # slice == 1.0 gives 2^10, whose low 10 bits are zero: the axis wraps to 0
+    c = np.array([1.0, 1.0, 1.0], np.float32)
+    assert O.morton_one(c, c, lo, hi, 30) == 0
+    # a flat global axis: (c - lo) * 1/0 = 0 * inf = NaN, fmin(1, NaN) = 1 -> wraps to 0
+    flo, fhi = np.array([0, 2, 0], np.float32), np.array([1, 2, 1], np.float32)
+    c = np.array([0.5, 2.0, 0.0], np.float32)
+    assert O.morton_one(c, c, flo, fhi, 30) == 1 << 29
+    # below the box clamps to 0
+    c = np.array([-5.0, -5.0, -5.0], np.float32)
+    assert O.morton_one(c, c, lo, hi, 30) == 0
+
+
+def test_global_bbox_brute_force():
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-3, 7, (500, 3)).astype(np.float32)
+    idx = rng.integers(0, 400, (300, 3)).astype(np.uint32)  # vertices >= 400 unreferenced
+    lo, hi = O.global_bbox(pos, idx)
+    used = pos[np.unique(idx)]
+    assert np.array_equal(lo, used.min(0)) and np.array_equal(hi, used.max(0))
+
+
+# --------------------------------------------------------------- sort (a4) --
+def test_sort_is_numpy_stable_argsort():
+    rng = np.random.default_rng(7)
+    codes = rng.integers(0, 64, 5000).astype(np.uint64)  # many ties
+    assert np.array_equal(O.sort_perm(codes), np.argsort(codes, kind="stable"))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 100, 1000, 1025])
+def test_bitonic_emulation_sorts(n):
+    rng = np.random.default_rng(n)
+    codes = rng.integers(0, 1 << 20, n).astype(np.uint64)
+    p = O.bitonic_perm(codes)
+    assert np.array_equal(np.sort(p), np.arange(n)), "a permutation"
+    assert (np.diff(codes[p].astype(np.int64)) >= 0).all(), "keys nondecreasing"
+
+
+# ------------------------------------------------------------ camera, rays --
+def camera(pos, d, up, fov, aspect, rh=0):
+    cam = np.zeros(1, O.CAMERA_DTYPE)
+    p = np.asarray(pos, np.float32)
+    dd = np.asarray(d, np.float32)
+    u = np.asarray(up, np.float32)
+    O.lib().orc_camera_set(O.P(p), O.P(dd), O.P(u), C.c_float(fov), C.c_float(aspect), C.c_int(rh), O.P(cam))
+    return cam
+
+
+def test_camera_basis_closed_form():
+    cam = camera((1, 2, 3), (0, 0, 2), (0, 1, 0), np.pi / 2, 2.0)[0]
+    assert np.allclose(cam["dir"], [0, 0, 1])
+    assert np.allclose(cam["up"], [0, 1, 0], atol=1e-7)            # tan(45 deg) = 1
+    assert np.allclose(np.abs(cam["right"]), [2, 0, 0], atol=1e-7)  # times aspect
+    # left-handed: right = cross(up, dir) = +x for dir = +z
+    assert cam["right"][0] > 0
+
+
+def test_camera_matches_library_host_code():
+    import rtx  # host-only entry point (no GPU needed)
+    for pos, look, fov, asp in [((0, 4, -8), (0, 0.5, 0), np.pi / 4, 1.0), ((3, 1, 2), (-1, 0, 5), 1.1, 16 / 9)]:
+        d = np.asarray(look, np.float32) - np.asarray(pos, np.float32)
+        a = camera(pos, d, (0, 1, 0), np.float32(fov), np.float32(asp))[0]
+        b = rtx.camera_set(pos, look, (0, 1, 0), fov, asp)
+        for k in ("pos", "dir", "right", "up"):
+            assert np.array_equal(a[k], np.array(getattr(b, k), np.float32))
+
+
+def quad_order(w, h, q):
+    """Sample order of generate_primary_rays written as loops over blocks,
+    then the right strip row by row, then the top strip column by column."""
+    out = []
+    xb, yb = w // q, h // q
+    for by in range(yb):
+        for bx in range(xb):
+            for yy in range(q):
+                for xx in range(q):
+                    out.append((bx * q + xx, by * q + yy))
+    for y in range(h):
+        for x in range(xb * q, w):
+            out.append((x, y))
+    for x in range(xb * q):
+        for y in range(yb * q, h):
+            out.append((x, y))
+    return out
+
+
+@pytest.mark.parametrize("w,h,q", [(70, 40, 32), (64, 64, 32), (33, 17, 8), (5, 9, 4), (100, 37, 16)])
+def test_primary_pixel_order_and_directions(w, h, q):
+    cam = camera((0, 0, 0), (0, 0, 1), (0, 1, 0), np.pi / 3, w / h)
+    o, d, px, c = O.raygen_primary(cam, w, h, q)
+    xy = quad_order(w, h, q)
+    assert np.array_equal(px, np.array([y * w + x for x, y in xy])), "sample -> pixel mapping"
+    assert np.array_equal(np.sort(px), np.arange(w * h)), "each pixel exactly once"
+    # closed form in float64: dir ~ (tan*aspect*(2xn-1), tan*(2yn-1), 1), left-handed right = +x
+    t = np.tan(np.pi / 6)
+    x = np.array([p[0] for p in xy], np.float64)
+    y = np.array([p[1] for p in xy], np.float64)
+    v = np.stack([t * (w / h) * (2 * (x + 0.5) / w - 1), t * (2 * (y + 0.5) / h - 1), np.ones_like(x)], 1)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    assert np.abs(d[:, :3] - v).max() < 2e-6
+    assert (o[:, 3] == 0).all() and (d[:, 3] == np.float32(1e36)).all() and (c == 1).all()
+
+
+# -------------------------------------------------------- traversal (a10) --
+def tiny_scene(seed, T):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-4, 4, (3 * T, 3)).astype(np.float32)
+    idx = np.arange(3 * T, dtype=np.uint32).reshape(T, 3)
+    return pos, idx
+
+
+def random_rays(seed, n, spread=6.0):
+    rng = np.random.default_rng(seed)
+    o = np.zeros((n, 4), np.float32)
+    d = np.zeros((n, 4), np.float32)
+    o[:, :3] = rng.uniform(-spread, spread, (n, 3))
+    dv = rng.normal(size=(n, 3))
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    d[:, :3] = dv
+    d[:, 3] = 1e36
+    return o, d
+
+
+@pytest.mark.parametrize("T,seed", [(1, 1), (2, 2), (50, 3), (400, 4), (2000, 5)])
+def test_closest_hit_equals_brute_force(T, seed):
+    pos, idx = tiny_scene(seed, T)
+    b = O.Build(pos, idx, None, 30, 1)
+    o, d = random_rays(seed + 100, 3000)
+    h = O.trace(b, np.ones_like(pos), np.zeros((pos.shape[0], 2), np.float32), o, d)
+    ids, t = O.trace_brute(b, o, d)
+    same = h["id"] == ids
+    # disagreements are allowed only at exact distance ties
+    for k in np.nonzero(~same)[0]:
+        assert h["id"][k] >= 0 and ids[k] >= 0 and h["t"][k] == t[k]
+    assert same.mean() >= 0.999
+    hit = ids >= 0
+    assert np.array_equal(h["t"][hit], t[hit])
+
+
+def test_any_hit_equals_brute_force():
+    pos, idx = tiny_scene(9, 600)
+    b = O.Build(pos, idx, None, 30, 1)
+    o, d = random_rays(10, 2000)
+    h = O.trace(b, np.ones_like(pos), np.zeros((pos.shape[0], 2), np.float32), o, d)
+    lights = np.zeros(1, O.LIGHTS_DTYPE)
+    lights["count"] = 2
+    lights["l"]["type"][0, 0] = O.POINT
+    lights["l"]["pos"][0, 0] = (0.5, 9.0, 0.25)
+    lights["l"]["type"][0, 1] = O.DIR
+    lights["l"]["dir"][0, 1] = np.array([0.3, -0.9, 0.3]) / np.linalg.norm([0.3, -0.9, 0.3])
+    f1 = O.trace_shadow(b, lights, h["flags"], h["p4"])
+    f2 = O.trace_shadow_brute(b, lights, h["flags"], h["p4"])
+    assert np.array_equal(f1, f2)
+    assert ((f1 >> 8) & 3).any()
+
+
+def quad_plane(y=0.0, half=10.0):
+    pos = np.array([[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    return pos, idx
+
+
+def test_analytic_plane_hits():
+    pos, idx = quad_plane(y=-2.0)
+    b = O.Build(pos, idx, None, 30, 1)
+    o = np.array([[0.3, 5.0, 0.2, 0.0], [1.0, 1.0, 1.0, 0.0]], np.float32)
+    d = np.array([[0.0, -1.0, 0.0, 1e36], [0.0, 1.0, 0.0, 1e36]], np.float32)
+    nrm = np.tile(np.array([[0.0, 1.0, 0.0]], np.float32), (4, 1))
+    h = O.trace(b, nrm, np.zeros((4, 2), np.float32), o, d)
+    assert h["id"][0] >= 0 and h["t"][0] == np.float32(7.0)
+    assert np.allclose(h["p4"][0, :3], [0.3, -2.0, 0.2])
+    assert np.allclose(h["n4"][0, :3], [0, 1, 0]) and not (h["flags"][0] & 2)
+    assert h["id"][1] == -1
+    # seen from below, the normal is flipped toward the ray and inverse_n is set
+    o2 = np.array([[0.3, -5.0, 0.2, 0.0]], np.float32)
+    d2 = np.array([[0.0, 1.0, 0.0, 1e36]], np.float32)
+    h2 = O.trace(b, nrm, np.zeros((4, 2), np.float32), o2, d2)
+    assert h2["t"][0] == np.float32(3.0) and (h2["flags"][0] & 2) and np.allclose(h2["n4"][0, :3], [0, -1, 0])
+
+
+def test_analytic_sphere_hits():
+    import sys
+    sys.path.insert(0, str(__import__("pathlib").Path(__file__).resolve().parent.parent / "tesina-rt_amd"))
+    from rtx.scenes import uv_sphere
+    p, n, uv, idx = uv_sphere(200, 200, (1.0, 2.0, 3.0), 1.5)
+    pos = p.astype(np.float32)
+    b = O.Build(pos, idx, None, 30, 1)
+    rng = np.random.default_rng(3)
+    k = 500
+    dv = rng.normal(size=(k, 3))
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    o = np.zeros((k, 4), np.float32)
+    d = np.zeros((k, 4), np.float32)
+    o[:, :3] = np.array([1.0, 2.0, 3.0]) - 5.0 * dv  # aimed at the centre from distance 5
+    d[:, :3] = dv
+    d[:, 3] = 1e36
+    h = O.trace(b, n.astype(np.float32), uv.astype(np.float32), o, d)
+    assert (h["id"] >= 0).all()
+    # exact sphere: t = 5 - 1.5; tessellation error of a 200x200 sphere is < 2e-3
+    assert np.abs(h["t"] - 3.5).max() < 2e-3
+
+
+def test_fp64_hit_distance_error_small():
+    # float32 Moller-Trumbore against the same intersection in float64: the
+    # median error is a few ulp; random sliver triangles hit at grazing angles
+    # are ill-conditioned, so the tail is bounded loosely.
+    pos, idx = tiny_scene(12, 300)
+    b = O.Build(pos, idx, None, 30, 1)
+    o, d = random_rays(13, 2000)
+    h = O.trace(b, np.ones_like(pos), np.zeros((pos.shape[0], 2), np.float32), o, d)
+    worst, t64 = O.hit_t_f64(b, o, d, h["id"], h["t"])
+    m = h["id"] >= 0
+    rel = np.abs(h["t"][m] - t64[m]) / np.abs(t64[m])
+    assert np.median(rel) < 1e-6 and worst < 1e-3
+
+
+# -------------------------------------------------------- secondary (a12) --
+def mats(rows):
+    m = np.zeros(len(rows), [("diffuse", "<f4", 4), ("shininess", "<f4"), ("reflectiveness", "<f4"),
+                             ("refractive_index", "<f4"), ("texture", "<i4")])
+    for i, (r, g, bb, sh, rf, ior) in enumerate(rows):
+        m[i] = ((r, g, bb, 0), sh, rf, ior, -1)
+    return m
+
+
+def one_hit(n, p, flags=1):
+    h = O.empty_hits(1)
+    h["flags"][0] = flags
+    h["id"][0] = 0
+    h["n4"][0, :3] = n
+    h["p4"][0, :3] = p
+    return h
+
+
+def test_secondary_reflection_textbook():
+    m = mats([(1, 1, 1, 1, 0.7, 0.0)])
+    d = np.array([0.6, -0.8, 0.0])
+    o4 = np.zeros((1, 4), np.float32)
+    d4 = np.array([[0.6, -0.8, 0.0, 1e36]], np.float32)
+    so, sd, sp, sc, tot = O.raygen_secondary(m, np.zeros(1, np.int32), o4, d4, np.array([7], np.int32),
+                                             np.array([0.5], np.float32), one_hit((0, 1, 0), (1, 2, 3)), 10)
+    assert tot == 1 and sp[0] == 7
+    assert np.allclose(sd[0, :3], d - 2 * np.dot(d, [0, 1, 0]) * np.array([0, 1, 0]), atol=1e-6)
+    assert sc[0] == np.float32(0.5) * np.float32(0.7)  # R = 1 without refraction
+    assert np.allclose(so[0, :3], [1, 2, 3]) and so[0, 3] == np.float32(1e-4) and sd[0, 3] == np.float32(1e37)
+
+
+def test_secondary_refraction_normal_incidence_and_schlick():
+    # entering glass (ior 1.5) head-on: the refracted ray goes straight through;
+    # Schlick R = R0 = (0.5/2.5)^2 = 0.04; refraction is emitted first.
+    m = mats([(1, 1, 1, 1, 0.9, 1.5)])
+    d4 = np.array([[0.0, -1.0, 0.0, 1e36]], np.float32)
+    so, sd, sp, sc, tot = O.raygen_secondary(m, np.zeros(1, np.int32), np.zeros((1, 4), np.float32), d4,
+                                             np.array([3], np.int32), np.array([1.0], np.float32),
+                                             one_hit((0, 1, 0), (0, 0, 0)), 10)
+    assert tot == 2
+    assert np.allclose(sd[0, :3], [0, -1, 0], atol=1e-6) and np.allclose(sd[1, :3], [0, 1, 0], atol=1e-6)
+    assert np.isclose(sc[0], 0.9 * 0.96, rtol=1e-6) and np.isclose(sc[1], 0.9 * 0.04, rtol=1e-5)
+
+
+def test_secondary_total_internal_reflection():
+    # leaving glass at a grazing angle (inverse_n set): k <= 0, only a reflection with R = 1
+    m = mats([(1, 1, 1, 1, 0.5, 1.5)])
+    d = np.array([0.95, -np.sqrt(1 - 0.95 ** 2), 0.0])
+    d4 = np.array([[*d, 1e36]], np.float32)
+    so, sd, sp, sc, tot = O.raygen_secondary(m, np.zeros(1, np.int32), np.zeros((1, 4), np.float32), d4,
+                                             np.array([0], np.int32), np.array([1.0], np.float32),
+                                             one_hit((0, 1, 0), (0, 0, 0), flags=3), 10)
+    assert tot == 1 and sc[0] == np.float32(0.5)
+
+
+def test_secondary_counts_scan_and_capacity():
+    # parents with 0/1/2 children; capacity drops whole parents at new_id + 2 >= cap
+    m = mats([(1, 1, 1, 1, 0.0, 0.0), (1, 1, 1, 1, 0.5, 0.0), (1, 1, 1, 1, 0.5, 1.5)])
+    n = 6
+    h = O.empty_hits(n)
+    h["flags"][:] = [1, 1, 1, 0, 1, 1]
+    h["id"][:] = np.arange(n)
+    h["n4"][:, 1] = 1.0
+    matmap = np.array([2, 0, 1, 0, 2, 1], np.int32)  # children: 2, 0, 1, -, 2, 1
+    d4 = np.tile(np.array([[0.0, -1.0, 0.0, 1e36]], np.float32), (n, 1))
+    args = (m, matmap, np.zeros((n, 4), np.float32), d4, np.arange(n, dtype=np.int32), np.ones(n, np.float32), h)
+    so, sd, sp, sc, tot = O.raygen_secondary(*args, 100)
+    assert tot == 6 and np.array_equal(sp, [0, 0, 2, 4, 4, 5])
+    so, sd, sp, sc, tot = O.raygen_secondary(*args, 5)  # offsets 0,2,3,3,5: parent at 3 needs 3+2 >= 5
+    assert tot == 6 and np.array_equal(sp, [0, 0, 2])
+
+
+# ---------------------------------------------------------- shading (a13) --
+def lights_dir(dirv, color=(1, 1, 1), ambient=(0.25, 0.25, 0.25)):
+    L = np.zeros(1, O.LIGHTS_DTYPE)
+    L["ambient"] = ambient
+    L["count"] = 1
+    L["l"]["type"][0, 0] = O.DIR
+    L["l"]["dir"][0, 0] = dirv
+    L["l"]["color"][0, 0] = color
+    return L
+
+
+def shade_one(material, lights, n, d, flags=1, contrib=1.0):
+    h = O.empty_hits(1)
+    h["flags"][0] = flags
+    h["id"][0] = 0
+    h["n4"][0, :3] = n
+    acc = np.zeros((1, 3), np.uint32)
+    d4 = np.array([[*d, 1e36]], np.float32)
+    O.shade(material, np.zeros(1, np.int32), lights, d4, np.zeros(1, np.int32), np.array([contrib], np.float32), h,
+            acc)
+    return acc[0]
+
+
+def test_shading_lit_plane_closed_form():
+    # light straight down, normal up, view straight down: cos^2 = 1, the mirror
+    # direction points at the light so spec = 1^8 * 1 * shininess
+    m = mats([(0.5, 0.25, 0.125, 0.5, 0.0, 0.0)])
+    L = lights_dir((0, -1, 0))
+    acc = shade_one(m, L, (0, 1, 0), (0, -1, 0))
+    # val = 0.25*diff + diff*1 + 0.5
+    exp = [0.25 * 0.5 + 0.5 + 0.5, 0.25 * 0.25 + 0.25 + 0.5, 0.25 * 0.125 + 0.125 + 0.5]
+    assert np.array_equal(acc, np.floor(np.minimum(exp, 1.0) * 1e6).astype(np.uint32))
+
+
+def test_shading_shadowed_and_reflective():
+    m = mats([(0.5, 0.5, 0.5, 1.0, 0.0, 0.0), (0.5, 0.5, 0.5, 1.0, 0.5, 0.0), (0.5, 0.5, 0.5, 1.0, 1.0, 0.0)])
+    L = lights_dir((0, -1, 0))
+    shadowed = shade_one(m[:1], L, (0, 1, 0), (0, -1, 0), flags=1 | (1 << 8))
+    assert np.array_equal(shadowed, [125000] * 3)  # ambient only: 0.25 * 0.5
+    half = shade_one(m[1:2], L, (0, 1, 0), (0, -1, 0), contrib=0.5)
+    assert np.array_equal(half, [np.floor(min(1.0, (0.125 + 0.5 + 1.0) * 0.5) * 0.5 * 1e6)] * 3)
+    mirror = shade_one(m[2:3], L, (0, 1, 0), (0, -1, 0))
+    assert np.array_equal(mirror, [0, 0, 0])  # (1 - reflectiveness) = 0
+
+
+def test_shading_grazing_light_zero_diffuse():
+    m = mats([(1, 1, 1, 0.0, 0.0, 0.0)])
+    L = lights_dir((1, 0, 0), ambient=(0, 0, 0))
+    assert np.array_equal(shade_one(m, L, (0, 1, 0), (0, -1, 0)), [0, 0, 0])
+
+
+def test_resolve_closed_form():
+    acc = np.array([[0, 500000, 1000000], [2000000, 1960784, 1]], np.uint32)
+    out = O.resolve(acc)
+    # 0 -> 0; 0.5*255 = 127.5 -> 128 (nearest even); 1 -> 255; > 1 clamps;
+    # 1.960784 clamps to 255; 1e-6 * 255 -> 0
+    assert out.tolist() == [[0, 128, 255, 255], [255, 255, 0, 255]]
