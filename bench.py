@@ -89,25 +89,40 @@ def main():
     cam = scenes.camera_of(sd)
     R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
 
-    # warm-up (also first-touch of every buffer)
-    for _ in range(args.warmup):
+    # warm-up (also first-touch of every buffer), then capture the frame
+    # (LBVH rebuild + render) into a hipGraph
+    for _ in range(max(1, args.warmup)):
         R.frame_async(s, cam, rebuild=True)
     ctx.sync()
     st = R.last_stats()
     rays_per_frame = st.primary_rays + st.secondary_rays
     shadow_per_frame = st.shadow_rays
     log(f"rays/frame: primary {st.primary_rays} secondary {st.secondary_rays} shadow {st.shadow_rays}")
+    R.capture(s, cam, rebuild=True)
+    for _ in range(args.warmup):
+        R.replay()
+    ctx.sync()
 
-    # timed region: K frames, no host read-back inside
+    # timed region: K graph replays, no host work inside
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        R.replay()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = rays_per_frame * args.steps / dt / 1e6
+    st2 = R.last_stats()
+    assert st2.primary_rays + st2.secondary_rays == rays_per_frame, "replayed frame differs in ray count"
+    log(f"timed {args.steps} graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
+
+    # the same frame launched stage by stage (no graph)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         R.frame_async(s, cam, rebuild=True)
     ctx.sync()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt / args.steps * 1e3
-    value = rays_per_frame * args.steps / dt / 1e6
-    log(f"timed {args.steps} frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
+    async_ms = (time.perf_counter() - t0) / args.steps * 1e3
 
     # plain one-launch-per-stage path with the reference's count read-backs
     plain = []
@@ -156,6 +171,8 @@ def main():
         "shadow_rays_per_frame": int(shadow_per_frame),
         "mrays_incl_shadow": round((rays_per_frame + shadow_per_frame) * args.steps / dt / 1e6, 3),
         "lbvh_build_ms": round(build_ms, 4) if build_ms else None,
+        "frame_mode": "hipGraph replay of LBVH rebuild + render",
+        "stage_launch_ms_per_frame": round(async_ms, 4),
         "plain_path_ms_per_frame": round(plain_ms, 4),
         "plain_path_mrays": round(rays_per_frame / plain_ms / 1e3, 3),
         "roofline": {

@@ -288,6 +288,14 @@ int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* 
                                void* stream);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats);
+/* Captures one rtx_render_frame_async into a hipGraph kept by the renderer
+ * (no counterpart in the reference, which launches every stage from the
+ * host).  Render one frame eagerly first so that every buffer exists.  The
+ * camera and scene handles are fixed at capture; per-frame vertex updates or
+ * displacement are enqueued on the stream before each replay. */
+int32_t rtx_renderer_capture(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream);
+/* Launches the captured frame; asynchronous. */
+int32_t rtx_renderer_replay(rtx_renderer* r, void* stream);
 
 /* --------------------------------------------------------- host helpers -- */
 /* Camera::set (src/rt/camera.cpp:31-59); host only. */

@@ -313,6 +313,21 @@ __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ k
     }
 }
 
+// Copy of the buffers the build permutes in place (the reference copies each
+// to an aux buffer first, src/rt/bvh-builder.cpp:481-527).
+__global__ __launch_bounds__(BS) void k_stage_copy(const uint32_t* __restrict__ idx, uint32_t* __restrict__ idx_c,
+                                                   const int32_t* __restrict__ mm, int32_t* __restrict__ mm_c,
+                                                   const uint32_t* __restrict__ orig, uint32_t* __restrict__ orig_c,
+                                                   uint32_t T) {
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < T; i += gridDim.x * BS) {
+        idx_c[3 * (size_t)i] = idx[3 * (size_t)i];
+        idx_c[3 * (size_t)i + 1] = idx[3 * (size_t)i + 1];
+        idx_c[3 * (size_t)i + 2] = idx[3 * (size_t)i + 2];
+        mm_c[i] = mm[i];
+        orig_c[i] = orig[i];
+    }
+}
+
 // rearrange_indices / rearrange_map (src/kernel/bvh-builder.cl:49-72) plus
 // the sorted triangle records used by traversal and refit, and the
 // original-id map (composition of every build's permutation).
@@ -665,14 +680,17 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         std::swap(s->perm, s->vals_alt);
     }
     {
+        // permute in place: copy, then gather back into the scene's buffers
+        // (no handle swap, so a captured frame chains builds as eager ones do)
         ProfScope ps(s->ctx, st, "lbvh_rearrange");
+        uint32_t g = rtx_div_up(T, BS);
+        hipLaunchKernelGGL(k_stage_copy, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->idx.as<uint32_t>(),
+                           s->idx_alt.as<uint32_t>(), s->matmap.as<int32_t>(), s->matmap_alt.as<int32_t>(),
+                           s->orig.as<uint32_t>(), s->orig_alt.as<uint32_t>(), T);
         hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->perm.as<uint32_t>(),
-                           s->idx.as<uint32_t>(), s->idx_alt.as<uint32_t>(), s->matmap.as<int32_t>(),
-                           s->matmap_alt.as<int32_t>(), s->orig.as<uint32_t>(), s->orig_alt.as<uint32_t>(),
+                           s->idx_alt.as<uint32_t>(), s->idx.as<uint32_t>(), s->matmap_alt.as<int32_t>(),
+                           s->matmap.as<int32_t>(), s->orig_alt.as<uint32_t>(), s->orig.as<uint32_t>(),
                            s->pos.as<float4>(), s->tri.as<float4>(), T);
-        std::swap(s->idx, s->idx_alt);
-        std::swap(s->matmap, s->matmap_alt);
-        std::swap(s->orig, s->orig_alt);
     }
     {
         ProfScope ps(s->ctx, st, "lbvh_radix_tree");

@@ -59,6 +59,8 @@ int32_t rtx_renderer_destroy(rtx_renderer* r) {
     for (auto x : r->rays) rtx_rays_destroy(x);
     for (auto x : r->hits) rtx_hits_destroy(x);
     if (r->fb) rtx_fb_destroy(r->fb);
+    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
     r->counts.release();
     delete r;
     return RTX_OK;
@@ -179,5 +181,50 @@ int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats) {
         r->last_async = false;
     }
     *stats = r->last;
+    return RTX_OK;
+}
+
+// Captures rtx_render_frame_async into a hipGraph (one launch per frame on
+// replay).  Every buffer must already exist: render one frame eagerly first.
+// The camera, scene and parameters are fixed at capture; per-frame scene
+// changes (vertex updates, displacement) are enqueued before each replay.
+int32_t rtx_renderer_capture(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
+    RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_renderer_capture: null argument");
+    hipStream_t st = r->ctx->pick(stream);
+    if (r->graph_exec) {
+        (void)hipGraphExecDestroy(r->graph_exec);
+        r->graph_exec = nullptr;
+    }
+    if (r->graph) {
+        (void)hipGraphDestroy(r->graph);
+        r->graph = nullptr;
+    }
+    const bool prof = r->ctx->profiling;
+    r->ctx->profiling = false;
+    RTX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int32_t e = rtx_render_frame_async(r, s, cam, rebuild, st);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    r->ctx->profiling = prof;
+    if (e) {
+        if (g) (void)hipGraphDestroy(g);
+        return e;
+    }
+    if (ce != hipSuccess) {
+        rtx_set_error(std::string("rtx_renderer_capture: ") + hipGetErrorString(ce));
+        return RTX_EDEVICE;
+    }
+    r->graph = g;
+    RTX_CHECK_HIP(hipGraphInstantiate(&r->graph_exec, g, nullptr, nullptr, 0));
+    return RTX_OK;
+}
+
+int32_t rtx_renderer_replay(rtx_renderer* r, void* stream) {
+    RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_replay: null renderer");
+    RTX_REQUIRE(r->graph_exec, RTX_ESTATE, "rtx_renderer_replay: nothing captured");
+    hipStream_t st = r->ctx->pick(stream);
+    ProfScope ps(r->ctx, st, "FRAME_GRAPH");
+    RTX_CHECK_HIP(hipGraphLaunch(r->graph_exec, st));
+    r->last_async = true;
     return RTX_OK;
 }

@@ -123,6 +123,7 @@ struct rtx_rays {
     DevBuf count;  // uint32[4]: [0] ray count, [1] scan total, [2] first dropped offset, [3] ticket
     DevBuf scan_status;  // look-back words of the generation that writes this bundle
     DevBuf psamp;        // pixel-sample table of primary generation (spp x 3 floats)
+    std::vector<float> psamp_host;  // last table uploaded (uploads happen only on change)
 };
 
 struct rtx_hits {
@@ -149,6 +150,8 @@ struct rtx_renderer {
     rtx_frame_stats last{};
     DevBuf counts;  // uint64 device ray counters of the current frame (see render.hip)
     bool last_async = false;
+    hipGraph_t graph = nullptr;          // captured read-back-free frame
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 // ------------------------------------------------ ordered (look-back) scan --

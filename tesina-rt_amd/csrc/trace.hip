@@ -12,6 +12,8 @@
 // is what differs: 32-byte nodes plus a 16-byte delta fetched only at
 // internal nodes, triangles stored contiguously in leaf order, the stack in
 // LDS ([level][thread], conflict-free), SoA rays and hits.
+#include <cstring>
+
 #include "devmath.hpp"
 #include "rtx_impl.hpp"
 
@@ -718,10 +720,15 @@ int32_t rtx_raygen_primary(rtx_rays* rays, const rtx_camera* cam, int32_t width,
     RTX_REQUIRE(psamples || spp == 1, RTX_EINVAL, "rtx_raygen_primary: spp > 1 needs pixel samples");
     const float* hs = psamples ? psamples : one;
     const size_t n_s = psamples ? (size_t)spp : 1;
-    int32_t e = rays->psamp.reserve(rays->ctx, 12 * n_s);
-    if (e) return e;
+    // upload the table only when it changes, so that a frame that reuses it
+    // (and its hipGraph capture) performs no host-to-device copy
+    if (rays->psamp_host.size() != 3 * n_s || memcmp(rays->psamp_host.data(), hs, 12 * n_s) != 0) {
+        int32_t e = rays->psamp.reserve(rays->ctx, 12 * n_s);
+        if (e) return e;
+        RTX_CHECK_HIP(hipMemcpy(rays->psamp.p, hs, 12 * n_s, hipMemcpyHostToDevice));
+        rays->psamp_host.assign(hs, hs + 3 * n_s);
+    }
     float* dtab = rays->psamp.as<float>();
-    RTX_CHECK_HIP(hipMemcpyAsync(dtab, hs, 12 * n_s, hipMemcpyHostToDevice, st));
     ProfScope ps(rays->ctx, st, "raygen_primary");
     uint32_t grid = rtx_div_up(count ? count : 1, 256);
     hipLaunchKernelGGL(k_raygen_primary, dim3(grid), dim3(256), 0, st, rays->ori4.as<float4>(),

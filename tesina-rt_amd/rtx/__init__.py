@@ -93,7 +93,7 @@ SYMBOLS = [
     "rtx_fb_create", "rtx_fb_destroy", "rtx_fb_clear", "rtx_fb_resolve", "rtx_fb_read_accum", "rtx_fb_read_rgba8",
     "rtx_shade",
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
-    "rtx_renderer_last_stats",
+    "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay",
     "rtx_camera_set",
 ]
 
@@ -169,6 +169,8 @@ def lib() -> C.CDLL:
         "rtx_render_frame": (I32, [P, P, P, I32, P, P]),
         "rtx_render_frame_async": (I32, [P, P, P, I32, P]),
         "rtx_renderer_last_stats": (I32, [P, P]),
+        "rtx_renderer_capture": (I32, [P, P, P, I32, P]),
+        "rtx_renderer_replay": (I32, [P, P]),
         "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
     }
     for name, (res, args) in sig.items():
@@ -518,6 +520,14 @@ class Renderer:
     def frame_async(self, scene: Scene, cam: Camera, rebuild: bool = True, stream=None):
         _check(lib().rtx_render_frame_async(self.h, scene.h, C.byref(cam), int(rebuild), stream),
                "rtx_render_frame_async")
+
+    def capture(self, scene: Scene, cam: Camera, rebuild: bool = True, stream=None):
+        """Capture one read-back-free frame into a hipGraph (render once eagerly first)."""
+        _check(lib().rtx_renderer_capture(self.h, scene.h, C.byref(cam), int(rebuild), stream),
+               "rtx_renderer_capture")
+
+    def replay(self, stream=None):
+        _check(lib().rtx_renderer_replay(self.h, stream), "rtx_renderer_replay")
 
     def last_stats(self) -> FrameStats:
         st = FrameStats()

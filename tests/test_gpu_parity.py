@@ -361,3 +361,35 @@ def test_traversal_variants_identical(ctx, c1, variant, block):
         assert np.array_equal(g2[k], o2[k]), f"secondary {k} differs (variant {variant}, block {block})"
     for x in (r, r2, hb):
         x.destroy()
+
+
+def test_graph_replay_matches_oracle(ctx, c1):
+    # the captured frame (LBVH rebuild + render) replayed several times, with
+    # eager work and synchronizations in between; every replay must match
+    rtx = _rtx()
+    from rtx import scenes
+    sd = scenes.config_c1()
+    s = scenes.load(rtx.Scene(ctx), sd)
+    cam = scenes.camera_of(sd)
+    w, h = 192, 144
+    cam = rtx.camera_set(sd.camera["pos"], sd.camera["look_at"], sd.camera["up"], sd.camera["fov"], w / h)
+    R = rtx.Renderer(ctx, w, h, 2)
+    R.frame_async(s, cam, rebuild=True)   # eager frame: allocates everything
+    ctx.sync()
+    R.capture(s, cam, rebuild=True)
+    b = O.Build(sd.pos, sd.idx, sd.matmap, 30, 1)
+    for k in range(3):
+        R.replay()
+        ctx.sync()
+        b = O.Build(sd.pos, b.idx.reshape(-1, 3), b.matmap, 30, 1)  # each replay rebuilds from the permuted order
+        oacc, st, orays = O.render(b, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights), O.camera_from(cam),
+                                   w, h, 2)
+        assert np.array_equal(R.fb.accum(), oacc), f"replay {k} differs"
+        gst = R.last_stats()
+        assert gst.primary_rays + gst.secondary_rays == orays and gst.shadow_rays == st["shadow"]
+        R.frame(s, cam, rebuild=False)      # eager work between replays
+        b2 = b
+        assert np.array_equal(R.fb.accum(), O.render(b2, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights),
+                                                     O.camera_from(cam), w, h, 2)[0])
+    R.destroy()
+    s.destroy()
