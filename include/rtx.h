@@ -150,8 +150,14 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  * reference fixes its work-group sizes per device, src/gpu/function.cpp:322-385):
  *   "trace_variant": 0 single-node traversal, 1 single node with the near-child
  *                    delta loaded early, 2 sibling-pair traversal;
- *   "trace_block":   64, 128 or 256 threads per traversal work-group.
- * All variants give identical results. */
+ *   "trace_block":   64, 128 or 256 threads per traversal work-group;
+ *   "xcds":          number of L2 domains the traversal grid is split over
+ *                    (work-group i runs on XCD i mod xcds; each XCD gets one
+ *                    contiguous, screen-coherent chunk of the rays; 1 = plain
+ *                    grid order);
+ *   "coop_leaf":     leaves of at least this many triangles are tested by
+ *                    the whole wave, 64 triangles per round (0 = never).
+ * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */
 int64_t rtx_tune_get(rtx_ctx* ctx, const char* key);

@@ -235,6 +235,12 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
         RTX_REQUIRE(value == 64 || value == 128 || value == 256, RTX_EINVAL,
                     "rtx_tune_set: trace_block must be 64, 128 or 256");
         ctx->tune.trace_block = value;
+    } else if (k == "xcds") {
+        RTX_REQUIRE(value >= 1 && value <= 64, RTX_EINVAL, "rtx_tune_set: xcds must be in [1, 64]");
+        ctx->tune.xcds = value;
+    } else if (k == "coop_leaf") {
+        RTX_REQUIRE(value >= 0 && value <= (1 << 20), RTX_EINVAL, "rtx_tune_set: coop_leaf must be in [0, 2^20]");
+        ctx->tune.coop_leaf = value;
     } else {
         rtx_set_error("rtx_tune_set: unknown key " + k);
         return RTX_EINVAL;
@@ -247,6 +253,8 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     std::string k(key);
     if (k == "trace_variant") return ctx->tune.trace_variant;
     if (k == "trace_block") return ctx->tune.trace_block;
+    if (k == "xcds") return ctx->tune.xcds;
+    if (k == "coop_leaf") return ctx->tune.coop_leaf;
     return -1;
 }
 
@@ -266,7 +274,7 @@ int32_t rtx_scene_destroy(rtx_scene* s) {
                      &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
                      &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
                      &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
-                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets,
+                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
                      &s->hier_status, &s->waves,      &s->stage};
     for (DevBuf* b : all) b->release();
     delete s;

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "devmath.hpp"
+#include "lookback.hpp"
 #include "rtx_impl.hpp"
 
 namespace {
@@ -31,7 +32,8 @@ constexpr int TK_COUNT = 80;
 
 // Layout of scene->levels (uint32): [d] = first node of level d,
 // [64 + d] = node count of level d, [128] = total node count.
-constexpr int LV_OFF = 0, LV_CNT = 64, LV_TOTAL = 128, LV_WORDS = 192;
+// [129] = number of fat leaves (more than FAT_LEAF triangles) in scene->fat.
+constexpr int LV_OFF = 0, LV_CNT = 64, LV_TOTAL = 128, LV_FAT = 129, LV_WORDS = 192;
 
 // ------------------------------------------------------------- helpers --
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
@@ -58,32 +60,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, u
     __syncthreads();
     *total = tot;
     return base + inc - x;
-}
-
-// Decoupled look-back over tiles acquired in order: publishes this tile's
-// aggregate, accumulates the predecessors, publishes the inclusive prefix.
-// Returns the exclusive prefix of `tile`.
-// Word of tile t lives at status[t * stride].
-__device__ uint32_t lookback(uint32_t* status, uint32_t tile, uint32_t agg, uint32_t stride) {
-    if (tile == 0) {
-        st_relaxed(&status[0], RTX_LB_INC | agg);
-        return 0;
-    }
-    st_relaxed(&status[(size_t)tile * stride], RTX_LB_AGG | agg);
-    uint32_t acc = 0;
-    int k = (int)tile - 1;
-    for (;;) {
-        uint32_t s = ld_relaxed(&status[(size_t)k * stride]);
-        if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        acc += s & RTX_LB_VAL;
-        if (s & RTX_LB_INC) break;
-        --k;
-    }
-    st_relaxed(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
-    return acc;
 }
 
 // -------------------------------------------------------------- kernels --
@@ -288,7 +264,7 @@ __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ k
     uint32_t hv = tid <= dmask ? hist[tid] : 0u;
     uint32_t dbase = block_excl_scan(hv, s_w, &dummy);
     if (tid <= dmask) {
-        uint32_t prefix = lookback(status + tid, tile, tot, 256u);
+        uint32_t prefix = lookback_thread(status + tid, tile, tot, 256u);
         s_gofs[tid] = dbase + prefix - tofs;  // dest = s_gofs[d] + local position
     }
     __syncthreads();
@@ -407,27 +383,75 @@ __global__ __launch_bounds__(BS) void k_radix_tree(const uint32_t* __restrict__ 
     kg[i] = (uint32_t)gamma;
 }
 
+// build_leaf_bbox (src/kernel/bvh-builder.cl:262-288) for a leaf of at most
+// FAT_LEAF triangles, by one thread: the vertices of four triangles are
+// loaded before any is folded in (a short leaf costs a few memory round
+// trips, not one per vertex).  Min/max is exact in any order.
 __device__ __forceinline__ void leaf_bounds(const float4* __restrict__ tri, float4* __restrict__ node, uint32_t n,
                                             uint32_t a, uint32_t cnt) {
     v3 lo = xyz(tri[3 * (size_t)a]), hi = lo;
-    for (uint32_t t = a; t < a + cnt; ++t) {
+    const uint32_t end = a + cnt;
+    for (uint32_t t = a; t < end; t += 4) {
+        float4 p[12];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            v3 p = xyz(tri[3 * (size_t)t + q]);
-            lo = vmin(lo, p);
-            hi = vmax(hi, p);
+        for (int q = 0; q < 12; ++q) {
+            const uint32_t tq = t + (uint32_t)(q / 3) < end ? t + (uint32_t)(q / 3) : a;
+            p[q] = tri[3 * (size_t)tq + (q % 3)];
+        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            lo = vmin(lo, xyz(p[q]));
+            hi = vmax(hi, xyz(p[q]));
         }
     }
     node[2 * (size_t)n] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(a));
     node[2 * (size_t)n + 1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(cnt));
 }
 
+// Leaves of more than FAT_LEAF triangles (long runs of equal codes, up to
+// hundreds of triangles) are listed during emission and bounded here, one
+// wave per leaf, every lane folding four vertices per round.
+__global__ __launch_bounds__(BS) void k_fat_leaves(const float4* __restrict__ tri, float4* __restrict__ node,
+                                                   const uint32_t* __restrict__ fat, const uint32_t* __restrict__ levels) {
+    const uint32_t nfat = levels[LV_FAT];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wpb = BS / RTX_WAVE;
+    for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < nfat; i += gridDim.x * wpb) {
+        const uint32_t n = fat[i];
+        const uint32_t a = __float_as_uint(node[2 * (size_t)n].w), cnt = __float_as_uint(node[2 * (size_t)n + 1].w);
+        const size_t v0 = 3 * (size_t)a, v1 = 3 * (size_t)(a + cnt);
+        float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (size_t v = v0 + lane; v < v1; v += 4 * RTX_WAVE) {
+            float4 p[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) p[j] = tri[v + j * RTX_WAVE < v1 ? v + j * RTX_WAVE : v];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                q[0] = fminf(q[0], p[j].x); q[1] = fminf(q[1], p[j].y); q[2] = fminf(q[2], p[j].z);
+                q[3] = fmaxf(q[3], p[j].x); q[4] = fmaxf(q[4], p[j].y); q[5] = fmaxf(q[5], p[j].z);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+#pragma unroll
+            for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
+        }
+        if (lane == 0) {
+            node[2 * (size_t)n] = make_float4(q[0], q[1], q[2], __uint_as_float(a));
+            node[2 * (size_t)n + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(cnt));
+        }
+    }
+}
+
 __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ kj,
                                                    const uint32_t* __restrict__ kg, uint32_t* __restrict__ bstart,
                                                    uint32_t* __restrict__ bend, float4* __restrict__ node,
                                                    uint32_t* __restrict__ parent, const float4* __restrict__ tri,
-                                                   uint32_t* levels, uint32_t* status, uint32_t* ticket, int d,
-                                                   uint32_t T, int min_leaf, int force_leaf) {
+                                                   uint32_t* levels, uint32_t* status, uint32_t* ticket,
+                                                   uint32_t* __restrict__ fat_list, int d, uint32_t T, int min_leaf,
+                                                   int force_leaf) {
     __shared__ uint32_t s_tile, s_prefix;
     __shared__ uint32_t s_w[BS / RTX_WAVE];
     const uint32_t off = levels[LV_OFF + d];
@@ -476,10 +500,10 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
         wofs += (i < (int)w) ? s_w[i] : 0u;
         agg += s_w[i];
     }
-    if (tid == 0) {
-        uint32_t prefix = lookback(status, tile, agg, 1u);
-        s_prefix = prefix;
-        if (tile == ntiles - 1) {
+    if (w == 0) {
+        const uint32_t prefix = lookback_wave(status, tile, agg, 1u);
+        if (lane == 0) s_prefix = prefix;
+        if (lane == 0 && tile == ntiles - 1) {
             uint32_t total = prefix + agg;
             levels[LV_OFF + d + 1] = off + cnt;
             levels[LV_CNT + d + 1] = 2u * total;
@@ -487,31 +511,18 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
         }
     }
     __syncthreads();
-    // Leaves of more than FAT_LEAF triangles (long equal-code runs) are bounded
-    // by the whole wave, one leaf at a time; min/max is exact in any order.
+    // Leaves of more than FAT_LEAF triangles go to the fat list (bounded by
+    // k_fat_leaves, one wave each); the node gets its range now.
     const bool fat = active && !is_split && end - start > FAT_LEAF;
-    uint64_t fm = ballot(fat);
-    while (fm) {
-        const int src = __builtin_ctzll(fm);
-        fm &= fm - 1;
-        const uint32_t fs = __shfl(start, src, RTX_WAVE), fe = __shfl(end, src, RTX_WAVE);
-        const uint32_t fn = __shfl(n, src, RTX_WAVE);
-        float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t vtx = 3 * fs + lane; vtx < 3 * fe; vtx += RTX_WAVE) {
-            const float4 p = tri[vtx];
-            q[0] = fminf(q[0], p.x); q[1] = fminf(q[1], p.y); q[2] = fminf(q[2], p.z);
-            q[3] = fmaxf(q[3], p.x); q[4] = fmaxf(q[4], p.y); q[5] = fmaxf(q[5], p.z);
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) q[k] = fminf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
-#pragma unroll
-            for (int k = 3; k < 6; ++k) q[k] = fmaxf(q[k], __shfl_xor(q[k], off, RTX_WAVE));
-        }
-        if (lane == 0) {
-            node[2 * (size_t)fn] = make_float4(q[0], q[1], q[2], __uint_as_float(fs));
-            node[2 * (size_t)fn + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(fe - fs));
+    const uint64_t fm = ballot(fat);
+    if (fm) {
+        uint32_t fbase = 0;
+        if (lane == (uint32_t)__builtin_ctzll(fm)) fbase = atomicAdd(&levels[LV_FAT], (uint32_t)__builtin_popcountll(fm));
+        fbase = __shfl(fbase, __builtin_ctzll(fm), RTX_WAVE);
+        if (fat) {
+            fat_list[fbase + lanes_below(fm)] = n;
+            node[2 * (size_t)n] = make_float4(0.f, 0.f, 0.f, __uint_as_float(start));
+            node[2 * (size_t)n + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(end - start));
         }
     }
     if (!active) return;
@@ -564,7 +575,7 @@ __global__ __launch_bounds__(BS) void k_leaf_bounds(const float4* __restrict__ t
     for (uint32_t n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS) {
         float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
         uint32_t cnt = __float_as_uint(f1.w);
-        if (cnt) leaf_bounds(tri, node, n, __float_as_uint(f0.w), cnt);
+        if (cnt && cnt <= FAT_LEAF) leaf_bounds(tri, node, n, __float_as_uint(f0.w), cnt);
     }
 }
 
@@ -578,7 +589,7 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
     uint32_t hier_tiles = rtx_div_up(T, BS);
     int32_t e = 0;
-    if ((e = s->node.reserve(c, ncap * 32))) return e;
+    if ((e = s->node.reserve(c, ncap * 32 + 64))) return e;
     if ((e = s->delta.reserve(c, ncap * 16))) return e;
     if ((e = s->parent.reserve(c, ncap * 4))) return e;
     if ((e = s->bstart.reserve(c, ncap * 4))) return e;
@@ -596,7 +607,15 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     if ((e = s->sort_hist.reserve(c, 4 * 256 * 4))) return e;
     if ((e = s->sort_status.reserve(c, (size_t)passes * sort_tiles * 256 * 4))) return e;
     if ((e = s->hier_status.reserve(c, (size_t)(depth + 1) * hier_tiles * 4))) return e;
+    if ((e = s->fat.reserve(c, ((size_t)T / (FAT_LEAF + 1) + 1) * 4))) return e;
     return 0;
+}
+
+static void fat_leaf_bounds(rtx_scene* s, hipStream_t st) {
+    ProfScope ps(s->ctx, st, "lbvh_fat_leaves");
+    uint32_t g = rtx_div_up((uint64_t)s->T / (FAT_LEAF + 1) + 1, BS / RTX_WAVE);
+    hipLaunchKernelGGL(k_fat_leaves, dim3(g > 1024 ? 1024 : g), dim3(BS), 0, st, s->tri.as<float4>(), s->nodes(),
+                       s->fat.as<uint32_t>(), s->levels.as<uint32_t>());
 }
 
 // Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
@@ -607,7 +626,7 @@ static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
         uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)s->T;
         if (maxn > 2ull * s->T) maxn = 2ull * s->T;
         uint32_t g = rtx_div_up(maxn, BS);
-        hipLaunchKernelGGL(k_refit_level, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->node.as<float4>(),
+        hipLaunchKernelGGL(k_refit_level, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->nodes(),
                            s->delta.as<float4>(), s->levels.as<uint32_t>(), d);
     }
 }
@@ -705,10 +724,11 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         uint32_t grid = rtx_div_up(maxn, BS);
         hipLaunchKernelGGL(k_hier_level, dim3(grid), dim3(BS), 0, st, s->codes.as<uint32_t>(),
                            s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), s->bstart.as<uint32_t>(),
-                           s->bend.as<uint32_t>(), s->node.as<float4>(), s->parent.as<uint32_t>(),
+                           s->bend.as<uint32_t>(), s->nodes(), s->parent.as<uint32_t>(),
                            s->tri.as<float4>(), levels, s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles,
-                           tickets + TK_LEVEL + d, d, T, min_leaf, d == depth ? 1 : 0);
+                           tickets + TK_LEVEL + d, s->fat.as<uint32_t>(), d, T, min_leaf, d == depth ? 1 : 0);
     }
+    fat_leaf_bounds(s, st);
     refit_internal(s, st, depth);
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
@@ -732,8 +752,9 @@ int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
         uint32_t ncap = (uint32_t)(2 * (size_t)T);
         uint32_t g = rtx_div_up(ncap, BS);
         hipLaunchKernelGGL(k_leaf_bounds, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->tri.as<float4>(),
-                           s->node.as<float4>(), s->levels.as<uint32_t>());
+                           s->nodes(), s->levels.as<uint32_t>());
     }
+    fat_leaf_bounds(s, st);
     refit_internal(s, st, s->depth);
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
@@ -761,7 +782,7 @@ int32_t rtx_read_nodes_ref(rtx_scene* s, rtx_node_ref* out, uint32_t cap, uint32
     RTX_REQUIRE(cap >= N, RTX_EINVAL, "rtx_read_nodes_ref: capacity below node count");
     std::vector<float4> nd(2 * (size_t)N);
     std::vector<uint32_t> par(N);
-    RTX_CHECK_HIP(hipMemcpy(nd.data(), s->node.p, nd.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(nd.data(), s->nodes(), nd.size() * sizeof(float4), hipMemcpyDeviceToHost));
     RTX_CHECK_HIP(hipMemcpy(par.data(), s->parent.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t n = 0; n < N; ++n) {
         const float4 a = nd[2 * (size_t)n], b = nd[2 * (size_t)n + 1];

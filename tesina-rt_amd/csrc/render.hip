@@ -46,6 +46,25 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
     }
     if (!e) e = rtx_fb_create(ctx, p->width, p->height, &r->fb);
     if (!e) e = r->counts.reserve(ctx, 64);
+    if (!e && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess) {
+        r->side = nullptr;
+        rtx_set_error("rtx_renderer_create: side stream creation failed");
+        e = RTX_EDEVICE;
+    }
+    for (int b = 0; b <= r->p.max_bounces && !e; ++b) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            rtx_set_error("rtx_renderer_create: event creation failed");
+            e = RTX_EDEVICE;
+            break;
+        }
+        r->ev.push_back(ev);
+    }
+    if (!e && hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess) {
+        r->ev_join = nullptr;
+        rtx_set_error("rtx_renderer_create: event creation failed");
+        e = RTX_EDEVICE;
+    }
     if (e) {
         rtx_renderer_destroy(r);
         return e;
@@ -56,11 +75,15 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
 
 int32_t rtx_renderer_destroy(rtx_renderer* r) {
     RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_destroy: null");
+    if (r->side) (void)hipStreamSynchronize(r->side);
     for (auto x : r->rays) rtx_rays_destroy(x);
     for (auto x : r->hits) rtx_hits_destroy(x);
     if (r->fb) rtx_fb_destroy(r->fb);
     if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
     if (r->graph) (void)hipGraphDestroy(r->graph);
+    for (auto ev : r->ev) (void)hipEventDestroy(ev);
+    if (r->ev_join) (void)hipEventDestroy(r->ev_join);
+    if (r->side) (void)hipStreamDestroy(r->side);
     r->counts.release();
     delete r;
     return RTX_OK;
@@ -131,14 +154,18 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
     return RTX_OK;
 }
 
-// Same frame with no host read-back: counts stay on the device; the closest
-// hits of every bounce are traced first (each bounce's rays depend only on
-// the previous bounce's hits), then the shadow rays of all bounces in one
-// launch (they are independent of each other and of later bounces), then
-// shading.  Results are identical: shading is an order-free integer sum.
+// Same frame with no host read-back: counts stay on the device.  Bounce b's
+// rays depend only on bounce b-1's hits, and its shadow rays and shading on
+// nothing later, so after bounce b is traced the main stream goes on to
+// generate and trace bounce b+1 while the side stream traces bounce b's
+// shadow rays and shades it (a latency-bound small trace then shares the GPU
+// with independent work).  Results are identical to the reference sequence:
+// shadow rays only set flag bits 8+ that ray generation does not read, and
+// shading is an order-free integer sum.
 int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
     RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame_async: null argument");
     hipStream_t st = r->ctx->pick(stream);
+    hipStream_t sd = r->side;
     const uint64_t P = (uint64_t)r->p.width * r->p.height;
     unsigned long long* cnt = r->counts.as<unsigned long long>();
     int32_t e;
@@ -147,21 +174,23 @@ int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* 
     const int B = r->p.max_bounces;
     if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
         return e;
-    if ((e = rtx_trace(s, r->rays[0], r->hits[0], P, st))) return e;
-    for (int b = 0; b < B; ++b) {
-        const uint64_t n_in = b == 0 ? P : RTX_DEVICE_COUNT;
-        if ((e = rtx_raygen_secondary_impl(s, r->rays[b], r->hits[b], r->rays[b + 1], n_in, (uint64_t*)cnt, st)))
-            return e;
+    for (int b = 0; b <= B; ++b) {
+        const uint64_t nb = b == 0 ? P : RTX_DEVICE_COUNT;
+        if (b > 0) {
+            const uint64_t n_in = b == 1 ? P : RTX_DEVICE_COUNT;
+            if ((e = rtx_raygen_secondary_impl(s, r->rays[b - 1], r->hits[b - 1], r->rays[b], n_in, (uint64_t*)cnt,
+                                               st)))
+                return e;
+        }
         // zero-count bounces run as empty launches (no host decision)
-        if ((e = rtx_trace(s, r->rays[b + 1], r->hits[b + 1], RTX_DEVICE_COUNT, st))) return e;
+        if ((e = rtx_trace(s, r->rays[b], r->hits[b], nb, st))) return e;
+        RTX_CHECK_HIP(hipEventRecord(r->ev[b], st));
+        RTX_CHECK_HIP(hipStreamWaitEvent(sd, r->ev[b], 0));
+        if ((e = rtx_trace_shadow_impl(s, r->rays[b], r->hits[b], nb, (uint64_t*)(cnt + 1), sd))) return e;
+        if ((e = rtx_shade(s, r->rays[b], r->hits[b], r->fb, nb, b == 0, sd))) return e;
     }
-    std::vector<uint64_t> counts(B + 1, RTX_DEVICE_COUNT);
-    counts[0] = P;
-    if ((e = rtx_trace_shadow_multi(s, r->rays.data(), r->hits.data(), counts.data(), B + 1, (uint64_t*)(cnt + 1),
-                                    st)))
-        return e;
-    for (int b = 0; b <= B; ++b)
-        if ((e = rtx_shade(s, r->rays[b], r->hits[b], r->fb, counts[b], b == 0, st))) return e;
+    RTX_CHECK_HIP(hipEventRecord(r->ev_join, sd));
+    RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_join, 0));
     if ((e = rtx_fb_resolve(r->fb, st))) return e;
     r->last_async = true;
     r->last = rtx_frame_stats{};

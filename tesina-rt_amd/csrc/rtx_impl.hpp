@@ -64,6 +64,8 @@ struct KernelTiming {
 struct rtx_tuning {
     int64_t trace_variant = 1;  // 0 single node, 1 single node + early delta load, 2 sibling pairs
     int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
+    int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
+    int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
 };
 
 struct rtx_ctx {
@@ -104,6 +106,9 @@ struct rtx_scene {
     bool built = false;
     int depth = 30, min_leaf = 1;
     DevBuf node, delta, parent, tri, codes, perm;
+    // Node i lives at byte 32 + 32 i of `node`: left children have odd ids, so
+    // every sibling pair (i, i + 1) fills one 64-byte aligned chunk.
+    float4* nodes() const { return reinterpret_cast<float4*>(static_cast<char*>(node.p) + 32); }
     DevBuf gbox;        // 6 x int32 ordered-float keys
     DevBuf levels;      // uint32: [0..64) level offsets, [64..128) level counts, [128] node count
     // build scratch
@@ -114,6 +119,7 @@ struct rtx_scene {
     DevBuf sort_status;           // look-back words, per pass x tile x digit
     DevBuf tickets;               // tile tickets for every ordered kernel
     DevBuf hier_status;           // look-back words, per level x tile
+    DevBuf fat;                   // node ids of the leaves bounded by one wave each (kept for refit)
 };
 
 struct rtx_rays {
@@ -150,6 +156,13 @@ struct rtx_renderer {
     rtx_frame_stats last{};
     DevBuf counts;  // uint64 device ray counters of the current frame (see render.hip)
     bool last_async = false;
+    // Side stream of the read-back-free frame: each bounce's shadow rays and
+    // shading run there while the main stream generates and traces the next
+    // bounce.  ev[b] marks "bounce b traced" on the main stream, ev_join the
+    // end of the side stream's work.  Default priority, non-blocking.
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t ev_join = nullptr;
     hipGraph_t graph = nullptr;          // captured read-back-free frame
     hipGraphExec_t graph_exec = nullptr;
 };

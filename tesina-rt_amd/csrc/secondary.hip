@@ -10,6 +10,7 @@
 // and writes.  Output order equals the reference's: parents in order,
 // refraction before reflection.  The count stays on the device.
 #include "devmath.hpp"
+#include "lookback.hpp"
 #include "rtx_impl.hpp"
 
 namespace {
@@ -95,28 +96,12 @@ __global__ __launch_bounds__(SB) void k_secondary(
         wofs += (i < (int)w) ? s_w[i] : 0u;
         agg += s_w[i];
     }
-    if (tid == 0) {
-        // lookback (same protocol as the LBVH kernels)
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            st_relaxed(&status[0], RTX_LB_INC | agg);
-        } else {
-            st_relaxed(&status[tile], RTX_LB_AGG | agg);
-            int q = (int)tile - 1;
-            for (;;) {
-                uint32_t s = ld_relaxed(&status[q]);
-                if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                prefix += s & RTX_LB_VAL;
-                if (s & RTX_LB_INC) break;
-                --q;
-            }
-            st_relaxed(&status[tile], RTX_LB_INC | (prefix + agg));
+    if (w == 0) {
+        const uint32_t prefix = lookback_wave(status, tile, agg, 1u);
+        if (lane == 0) {
+            s_prefix = prefix;
+            if (tile == ntiles - 1) out_cnt[1] = prefix + agg;
         }
-        s_prefix = prefix;
-        if (tile == ntiles - 1) out_cnt[1] = prefix + agg;
     }
     __syncthreads();
     if (!cnt) return;

@@ -25,6 +25,25 @@ __device__ __forceinline__ uint64_t ray_count(const uint32_t* dcount, uint64_t h
     return hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
 }
 
+// Work split over the L2 domains.  Work-groups are dispatched round-robin
+// over the XCDs (group g on XCD g mod X), each with its own L2, so XCD x
+// takes the x-th contiguous chunk of the n items: neighbouring rays (same
+// screen region, same BVH nodes) then share one L2 instead of all eight.
+// The grid is a multiple of X; a group walks its chunk with stride
+// (grid / X) * TB.
+struct XcdSpan {
+    uint64_t begin, end, stride;
+};
+__device__ __forceinline__ XcdSpan xcd_span(uint64_t n, uint32_t X, uint32_t TB) {
+    const uint32_t x = blockIdx.x % X, slot = blockIdx.x / X, per = gridDim.x / X;
+    XcdSpan sp;
+    const uint64_t lo = n * x / X, hi = n * (x + 1) / X;
+    sp.begin = lo + (uint64_t)slot * TB;
+    sp.end = hi;
+    sp.stride = (uint64_t)per * TB;
+    return sp;
+}
+
 // ------------------------------------------------------------- raygen ----
 __global__ __launch_bounds__(256) void k_raygen_primary(float4* __restrict__ ori4, float4* __restrict__ dir4,
                                                         int32_t* __restrict__ pixel, float* __restrict__ contrib,
@@ -292,87 +311,228 @@ __device__ __forceinline__ bool traverse_any(const float4* __restrict__ node, co
     }
 }
 
+// Wave-cooperative leaf test.  Leaves of at least `coop` triangles (long
+// runs of equal Morton codes, e.g. the pole fans of UV spheres, reach
+// hundreds) are tested by every live lane of the wave together, 64
+// triangles per round, instead of by their own lane alone while the rest
+// of the wave waits.  The owner's ray and current tMax are broadcast; the
+// round's hits are reduced to the smallest t and, among equal t, the
+// highest index, which is what the sequential in-order loop with its
+// `t <= best` update keeps (a later triangle wins an exact tie).
+__device__ __forceinline__ float bcast(float x, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t x, int src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, src);
+}
+
+// Closest hit over the fat leaves flagged in this iteration (cnt > 0 on
+// the owning lanes).  Must be called by all 64 lanes of the wave (`live` is
+// then the full mask): the shuffles of the reduction read every lane.
+__device__ __forceinline__ void coop_closest(const float4* __restrict__ tri, uint32_t la, uint32_t lcnt,
+                                             const RayC& r, Closest& best, uint64_t live) {
+    uint64_t owners = ballot(lcnt != 0u);
+    if (!owners) return;
+    const uint32_t nlive = (uint32_t)__builtin_popcountll(live);
+    const uint32_t rank = lanes_below(live);
+    const int lane = (int)lane_id();
+    while (owners) {
+        const int src = __builtin_ctzll(owners);
+        owners &= owners - 1;
+        const uint32_t a = bcast(la, src), cnt = bcast(lcnt, src);
+        const v3 o = mk3(bcast(r.o.x, src), bcast(r.o.y, src), bcast(r.o.z, src));
+        const v3 d = mk3(bcast(r.d.x, src), bcast(r.d.y, src), bcast(r.d.z, src));
+        const float tmin = bcast(r.tmin, src);
+        for (uint32_t base = 0; base < cnt; base += nlive) {
+            const float bt = bcast(best.t, src);
+            const uint32_t k = base + rank;
+            float tt = 0.f, u = 0.f, v = 0.f;
+            bool h = false;
+            if (k < cnt) {
+                const size_t t3 = 3 * (size_t)(a + k);
+                h = tri_hit_v(tri[t3], tri[t3 + 1], tri[t3 + 2], o, d, &tt, &u, &v) && tt <= bt && tt >= tmin;
+            }
+            // key: smallest t first (the +0.f folds -0 into +0, which compare
+            // equal), then the highest index
+            uint64_t key = ~0ull;
+            if (h) key = ((uint64_t)((uint32_t)fkey(tt + 0.f) ^ 0x80000000u) << 32) | (uint32_t)~k;
+            uint64_t m = key;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t y = __shfl_xor(m, off, RTX_WAVE);
+                m = y < m ? y : m;
+            }
+            const uint64_t wm = ballot(h && key == m);
+            if (m != ~0ull && wm) {
+                const int win = __builtin_ctzll(wm);
+                const float wt = bcast(tt, win), wu = bcast(u, win), wv = bcast(v, win);
+                const uint32_t wk = bcast(k, win);
+                if (lane == src) {
+                    best.t = wt;
+                    best.u = wu;
+                    best.v = wv;
+                    best.id = (int32_t)(a + wk);
+                }
+            }
+        }
+    }
+}
+
+// Any hit over the flagged fat leaves; returns true on the owning lanes
+// whose leaf holds a hit with tMin < t < tMax.
+__device__ __forceinline__ bool coop_any(const float4* __restrict__ tri, uint32_t la, uint32_t lcnt, const RayC& r,
+                                         float tmax, uint64_t live) {
+    uint64_t owners = ballot(lcnt != 0u);
+    bool found = false;
+    if (!owners) return false;
+    const uint32_t nlive = (uint32_t)__builtin_popcountll(live);
+    const uint32_t rank = lanes_below(live);
+    const int lane = (int)lane_id();
+    while (owners) {
+        const int src = __builtin_ctzll(owners);
+        owners &= owners - 1;
+        const uint32_t a = bcast(la, src), cnt = bcast(lcnt, src);
+        const v3 o = mk3(bcast(r.o.x, src), bcast(r.o.y, src), bcast(r.o.z, src));
+        const v3 d = mk3(bcast(r.d.x, src), bcast(r.d.y, src), bcast(r.d.z, src));
+        const float tmin = bcast(r.tmin, src), tm = bcast(tmax, src);
+        for (uint32_t base = 0; base < cnt; base += nlive) {
+            const uint32_t k = base + rank;
+            bool h = false;
+            if (k < cnt) {
+                const size_t t3 = 3 * (size_t)(a + k);
+                float tt, u, v;
+                h = tri_hit_v(tri[t3], tri[t3 + 1], tri[t3 + 2], o, d, &tt, &u, &v) && tt < tm && tt > tmin;
+            }
+            if (ballot(h)) {
+                if (lane == src) found = true;
+                break;
+            }
+        }
+    }
+    return found;
+}
+
 // Single-node form of the same traversal (one node per step, as
 // trace_ray visits them).  SPEC issues the node's delta load together with
-// the node load instead of after the box test.
+// the node load instead of after the box test.  Every lane stays in the
+// loop until the whole wave is done, so that finished lanes help test the
+// fat leaves of the others (leaves of at least `coop` triangles).  Called
+// by all lanes of the wave; lanes without a ray pass start = false.
 template <bool SPEC, class Stack>
 __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__ node,
                                                       const float4* __restrict__ delta,
                                                       const float4* __restrict__ tri, const RayC& r, Closest& best,
-                                                      Stack& stk) {
+                                                      Stack& stk, uint32_t coop, bool start) {
+    const uint64_t live = ballot(true);
     uint32_t level = 0, curr = 0;
-    for (;;) {
-        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-        float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (SPEC) dl4 = delta[curr];
-        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-        bool pop = true;
-        if (slab(f0, f1, r, best.t)) {
-            if (cnt) {
-                leaf_closest(tri, a, cnt, r, best);
-            } else {
-                if (!SPEC) dl4 = delta[curr];
-                const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-                stk.put(level & (STACK - 1), nl ? a + 1 : a);
-                level++;
-                curr = nl ? a : a + 1;
-                pop = false;
+    bool going = start;
+    while (ballot(going)) {
+        uint32_t la = 0, lcnt = 0;
+        if (going) {
+            const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+            float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (SPEC) dl4 = delta[curr];
+            const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+            bool pop = true;
+            if (slab(f0, f1, r, best.t)) {
+                if (cnt >= coop) {
+                    la = a;
+                    lcnt = cnt;
+                } else if (cnt) {
+                    leaf_closest(tri, a, cnt, r, best);
+                } else {
+                    if (!SPEC) dl4 = delta[curr];
+                    const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+                    stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                    level++;
+                    curr = nl ? a : a + 1;
+                    pop = false;
+                }
+            }
+            if (pop) {
+                if (level == 0)
+                    going = false;
+                else
+                    curr = stk.get(--level);
             }
         }
-        if (pop) {
-            if (level == 0) return;
-            curr = stk.get(--level);
-        }
+        coop_closest(tri, la, lcnt, r, best, live);
     }
 }
 
 template <bool SPEC, class Stack>
 __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                   const float4* __restrict__ tri, const RayC& r, float tmax,
-                                                  Stack& stk) {
+                                                  Stack& stk, uint32_t coop, bool start) {
+    const uint64_t live = ballot(true);
     uint32_t level = 0, curr = 0;
-    for (;;) {
-        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-        float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (SPEC) dl4 = delta[curr];
-        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-        bool pop = true;
-        if (slab(f0, f1, r, tmax)) {
-            if (cnt) {
-                if (leaf_any(tri, a, cnt, r, tmax)) return true;
-            } else {
-                if (!SPEC) dl4 = delta[curr];
-                const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-                stk.put(level & (STACK - 1), nl ? a + 1 : a);
-                level++;
-                curr = nl ? a : a + 1;
-                pop = false;
+    bool going = start, hit = false;
+    while (ballot(going)) {
+        uint32_t la = 0, lcnt = 0;
+        if (going) {
+            const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+            float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (SPEC) dl4 = delta[curr];
+            const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+            bool pop = true;
+            if (slab(f0, f1, r, tmax)) {
+                if (cnt >= coop) {
+                    la = a;
+                    lcnt = cnt;
+                } else if (cnt) {
+                    if (leaf_any(tri, a, cnt, r, tmax)) {
+                        hit = true;
+                        going = false;
+                        pop = false;
+                    }
+                } else {
+                    if (!SPEC) dl4 = delta[curr];
+                    const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+                    stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                    level++;
+                    curr = nl ? a : a + 1;
+                    pop = false;
+                }
+            }
+            if (pop) {
+                if (level == 0)
+                    going = false;
+                else
+                    curr = stk.get(--level);
             }
         }
-        if (pop) {
-            if (level == 0) return false;
-            curr = stk.get(--level);
+        if (coop_any(tri, la, lcnt, r, tmax, live)) {
+            hit = true;
+            going = false;
         }
     }
+    return hit;
 }
 
 // Variant dispatch: 0 = single node, 1 = single node + early delta load,
 // 2 = sibling pairs.
+// Called by every lane of the wave; `start` says whether the lane has a ray.
 template <int VAR, class Stack>
 __device__ __forceinline__ void closest_var(const float4* __restrict__ node, const float4* __restrict__ delta,
                                             const float4* __restrict__ tri, const RayC& r, Closest& best,
-                                            Stack& stk) {
-    if (VAR == 2)
-        traverse_closest(node, tri, r, best, stk);
-    else
-        traverse_closest_node<VAR == 1>(node, delta, tri, r, best, stk);
+                                            Stack& stk, uint32_t coop, bool start) {
+    if (VAR == 2) {
+        if (start) traverse_closest(node, tri, r, best, stk);
+    } else {
+        traverse_closest_node<VAR == 1>(node, delta, tri, r, best, stk, coop, start);
+    }
 }
 
 template <int VAR, class Stack>
 __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const float4* __restrict__ delta,
-                                        const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk) {
-    if (VAR == 2) return traverse_any(node, tri, r, tmax, stk);
-    return traverse_any_node<VAR == 1>(node, delta, tri, r, tmax, stk);
+                                        const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk,
+                                        uint32_t coop, bool start) {
+    if (VAR == 2) {
+        bool h = false;
+        if (start) h = traverse_any(node, tri, r, tmax, stk);
+        return h;
+    }
+    return traverse_any_node<VAR == 1>(node, delta, tri, r, tmax, stk, coop, start);
 }
 
 // -------------------------------------------------------------- trace ----
@@ -389,22 +549,23 @@ __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, c
                                               int32_t* __restrict__ idout, float2* __restrict__ bary,
                                               float4* __restrict__ n4, float4* __restrict__ p4,
                                               float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
-                                              uint32_t* __restrict__ hit_count) {
+                                              uint32_t* __restrict__ hit_count, uint32_t xcds, uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
     const uint32_t lane = threadIdx.x & 63;
-    for (uint64_t base = blockIdx.x * (uint64_t)TB; base < n; base += (uint64_t)gridDim.x * TB) {
+    const XcdSpan sp = xcd_span(n, xcds, TB);
+    for (uint64_t base = sp.begin; base < sp.end; base += sp.stride) {
         const uint64_t rr = base + threadIdx.x;
-        const bool active = rr < n;
+        const bool active = rr < sp.end;
         Closest best{0.f, 0.f, 0.f, -1};
         RayC rc{};
         if (active) {
             const float4 o = ori4[rr], dd = dir4[rr];
             rc = make_ray(xyz(o), xyz(dd), o.w);
             best.t = dd.w;
-            closest_var<VAR>(node, delta, tri, rc, best, stk);
         }
+        closest_var<VAR>(node, delta, tri, rc, best, stk, coop, active);  // whole wave
         const bool hit = active && best.id >= 0;
         const uint64_t hb = ballot(hit);
         uint32_t wbase = 0;
@@ -491,7 +652,8 @@ struct ShadowSegs {
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
-                                               ShadowSegs segs, unsigned long long* shadow_count) {
+                                               ShadowSegs segs, unsigned long long* shadow_count, uint32_t xcds,
+                                               uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_end[RTX_SEG_MAX];
@@ -508,40 +670,53 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
     __syncthreads();
     const uint64_t items = segs.n ? s_end[segs.n - 1] : 0;
     uint32_t traced = 0;
-    for (uint64_t it = blockIdx.x * (uint64_t)TB + threadIdx.x; it < items; it += (uint64_t)gridDim.x * TB) {
-        int sg = 0;
-        while (it >= s_end[sg]) ++sg;
-        const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
-        const ShadowSeg& S = segs.s[sg];
-        const uint32_t rr = S.list[local / nl];
-        const int l = (int)(local % nl);
-        const rtx_light& L = lights->l[l];
-        const v3 p = xyz(S.p4[rr]);
-        v3 dir;
+    const XcdSpan sp = xcd_span(items, xcds, TB);
+    // wave-uniform loop: every lane reaches the traversal call (lanes past the
+    // end or with nothing to trace pass start = false)
+    for (uint64_t base = sp.begin; base < sp.end; base += sp.stride) {
+        const uint64_t it = base + threadIdx.x;
+        const bool valid = it < sp.end;
+        uint32_t rr = 0;
+        int l = 0;
+        uint32_t* fl = nullptr;
+        RayC rc{};
         float tmax = 1e37f;
-        bool shadowed = false, trace = true;
-        if (L.type == RTX_LIGHT_DIR) {
-            dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
-        } else if (L.type == RTX_LIGHT_SPOT) {
-            dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
-            if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
+        bool shadowed = false, trace = false;
+        if (valid) {
+            int sg = 0;
+            while (it >= s_end[sg]) ++sg;
+            const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
+            const ShadowSeg& S = segs.s[sg];
+            rr = S.list[local / nl];
+            l = (int)(local % nl);
+            fl = S.flags;
+            const rtx_light& L = lights->l[l];
+            const v3 p = xyz(S.p4[rr]);
+            v3 dir = mk3(0.f, 0.f, 0.f);
+            trace = true;
+            if (L.type == RTX_LIGHT_DIR) {
+                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+            } else if (L.type == RTX_LIGHT_SPOT) {
+                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+                if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
+                    shadowed = true;
+                    trace = false;
+                }
+            } else if (L.type == RTX_LIGHT_POINT) {
+                const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+                tmax = __builtin_sqrtf(dot(toL, toL));
+                dir = normalize(toL);
+            } else {
                 shadowed = true;
                 trace = false;
             }
-        } else if (L.type == RTX_LIGHT_POINT) {
-            const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
-            tmax = __builtin_sqrtf(dot(toL, toL));
-            dir = normalize(toL);
-        } else {
-            shadowed = true;
-            trace = false;
+            if (trace) {
+                ++traced;
+                rc = make_ray(p, dir, 0.01f);
+            }
         }
-        if (trace) {
-            ++traced;
-            const RayC rc = make_ray(p, dir, 0.01f);
-            shadowed = any_var<VAR>(node, delta, tri, rc, tmax, stk);
-        }
-        if (shadowed) atomicOr(&S.flags[rr], 1u << (8 + l));
+        if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
+        if (valid && shadowed) atomicOr(&fl[rr], 1u << (8 + l));
     }
     if (shadow_count) {
         if (traced) atomicAdd(&s_cnt, traced);
@@ -568,12 +743,21 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
         }                                      \
     } while (0)
 
+// Leaf size from which leaves are tested wave-cooperatively (0 = never).
+uint32_t coop_threshold(const rtx_ctx* c) {
+    return c->tune.coop_leaf > 0 ? (uint32_t)c->tune.coop_leaf : 0xffffffffu;
+}
+
+// Traversal grid: one thread per item up to 64 groups per CU (then
+// grid-stride), rounded up to a multiple of the XCD count for xcd_span.
 uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
     uint64_t n = count == RTX_DEVICE_COUNT ? cap : count;
     uint64_t g = (n + TB - 1) / TB;
-    uint64_t gmax = (uint64_t)c->cus * 64;  // grid-stride beyond 64 blocks per CU
+    uint64_t gmax = (uint64_t)c->cus * 64;
     if (g > gmax) g = gmax;
-    return (uint32_t)(g ? g : 1);
+    const uint64_t X = (uint64_t)c->tune.xcds;
+    g = (g + X - 1) / X * X;
+    return (uint32_t)(g ? g : X);
 }
 
 }  // namespace
@@ -752,11 +936,12 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
     const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
 #define RTX_TRACE_LAUNCH(V, B)                                                                                       \
     hipLaunchKernelGGL((k_trace<V, B>), dim3(trace_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,              \
-                       s->node.as<float4>(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),       \
+                       s->nodes(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),       \
                        s->uv.as<float2>(), rays->ori4.as<float4>(), rays->dir4.as<float4>(),                        \
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
-                       hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>())
+                       hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>(),           \
+                       (uint32_t)s->ctx->tune.xcds, coop_threshold(s->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_TRACE_LAUNCH, var, tb);
 #undef RTX_TRACE_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
@@ -802,9 +987,10 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     uint64_t n = max_items * (uint64_t)nl;
     const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
 #define RTX_SHADOW_LAUNCH(V, B)                                                                                 \
-    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, s->node.as<float4>(), \
+    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, s->nodes(), \
                        s->delta.as<float4>(), s->tri.as<float4>(), s->lights.as<rtx_lights>(), segs,           \
-                       (unsigned long long*)dev_shadow_count)
+                       (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
+                       coop_threshold(s->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_LAUNCH, var, tb);
 #undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
