@@ -23,6 +23,9 @@ constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = BS * SORT_ITEMS;
 constexpr int BOUNDS_GRID = 512;
 constexpr uint32_t FAT_LEAF = 16;
+constexpr int HIER_TOP = 8;    // levels 0..8 (<= 256 nodes) emitted by one block
+constexpr int REFIT_TOP = 10;  // levels 10..0 (<= 1024 nodes) refit by one block
+constexpr int GRID_PER_CU = 2; // persistent grids: blocks per CU
 
 // Counter slots in scene->tickets.
 constexpr int TK_BOUNDS = 0;
@@ -445,53 +448,50 @@ __global__ __launch_bounds__(BS) void k_fat_leaves(const float4* __restrict__ tr
     }
 }
 
-__global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ kj,
-                                                   const uint32_t* __restrict__ kg, uint32_t* __restrict__ bstart,
-                                                   uint32_t* __restrict__ bend, float4* __restrict__ node,
-                                                   uint32_t* __restrict__ parent, const float4* __restrict__ tri,
-                                                   uint32_t* levels, uint32_t* status, uint32_t* ticket,
-                                                   uint32_t* __restrict__ fat_list, int d, uint32_t T, int min_leaf,
-                                                   int force_leaf) {
-    __shared__ uint32_t s_tile, s_prefix;
-    __shared__ uint32_t s_w[BS / RTX_WAVE];
-    const uint32_t off = levels[LV_OFF + d];
-    const uint32_t cnt = levels[LV_CNT + d];
-    if (cnt == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            levels[LV_OFF + d + 1] = off;
-            levels[LV_CNT + d + 1] = 0;
-        }
-        return;
-    }
-    const uint32_t ntiles = (cnt + BS - 1) / BS;
-    if (blockIdx.x >= ntiles) return;  // only ntiles blocks take tickets
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= ntiles) return;
+struct HierArgs {
+    const uint32_t* __restrict__ codes;
+    const uint32_t* __restrict__ kj;  // radix tree: other end of node i's range
+    const uint32_t* __restrict__ kg;  // radix tree: split of node i
+    uint32_t* __restrict__ bstart;
+    uint32_t* __restrict__ bend;
+    float4* __restrict__ node;
+    uint32_t* __restrict__ parent;
+    const float4* __restrict__ tri;
+    uint32_t* levels;
+    uint32_t* __restrict__ fat_list;
+    uint32_t T;
+    int min_leaf;
+};
+
+// One tile (BS consecutive nodes) of level d, by the whole block; the tile's
+// exclusive prefix of splitting nodes comes from the look-back over the
+// level's tiles (status).  Ends with a block barrier.
+__device__ __forceinline__ void hier_tile(const HierArgs& A, uint32_t* status, int d, bool force_leaf, uint32_t off,
+                                          uint32_t cnt, uint32_t tile, uint32_t ntiles, uint32_t* s_w,
+                                          uint32_t* s_prefix) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t k = tile * BS + tid;
     const bool active = k < cnt;
     const uint32_t n = off + k;
     uint32_t start = 0, end = 0, split = 0;
     if (active) {
-        start = bstart[n];
-        end = bend[n];
+        start = A.bstart[n];
+        end = A.bend[n];
         // Leaf iff count <= min_leaf + 2 (unsigned test, so a single triangle
         // falls through) or the first and last codes are equal: find_middle
         // advances its level while first and last share the level's bit, so it
         // ends at their highest differing bit, and runs out of bits only when
         // the codes are equal (every node's level bit is above all bits in
         // which its codes differ).
-        if (!force_leaf && end - start - 2u > (uint32_t)min_leaf && codes[start] != codes[end - 1]) {
+        if (!force_leaf && end - start - 2u > (uint32_t)A.min_leaf && A.codes[start] != A.codes[end - 1]) {
             const uint32_t last = end - 1;
-            const uint32_t other = (last <= T - 2u) ? last : start;
-            split = (kj[start] == last ? kg[start] : kg[other]) + 1u;
+            const uint32_t other = (last <= A.T - 2u) ? last : start;
+            split = (A.kj[start] == last ? A.kg[start] : A.kg[other]) + 1u;
         }
     }
     const bool is_split = split != 0;
-    uint64_t bal = ballot(is_split);
-    uint32_t excl = lanes_below(bal);
+    const uint64_t bal = ballot(is_split);
+    const uint32_t excl = lanes_below(bal);
     if (lane == 0) s_w[w] = (uint32_t)__builtin_popcountll(bal);
     __syncthreads();
     uint32_t wofs = 0, agg = 0;
@@ -502,12 +502,12 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
     }
     if (w == 0) {
         const uint32_t prefix = lookback_wave(status, tile, agg, 1u);
-        if (lane == 0) s_prefix = prefix;
+        if (lane == 0) *s_prefix = prefix;
         if (lane == 0 && tile == ntiles - 1) {
-            uint32_t total = prefix + agg;
-            levels[LV_OFF + d + 1] = off + cnt;
-            levels[LV_CNT + d + 1] = 2u * total;
-            levels[LV_TOTAL] = off + cnt + 2u * total;
+            const uint32_t total = prefix + agg;
+            A.levels[LV_OFF + d + 1] = off + cnt;
+            A.levels[LV_CNT + d + 1] = 2u * total;
+            A.levels[LV_TOTAL] = off + cnt + 2u * total;
         }
     }
     __syncthreads();
@@ -517,30 +517,80 @@ __global__ __launch_bounds__(BS) void k_hier_level(const uint32_t* __restrict__ 
     const uint64_t fm = ballot(fat);
     if (fm) {
         uint32_t fbase = 0;
-        if (lane == (uint32_t)__builtin_ctzll(fm)) fbase = atomicAdd(&levels[LV_FAT], (uint32_t)__builtin_popcountll(fm));
+        if (lane == (uint32_t)__builtin_ctzll(fm))
+            fbase = atomicAdd(&A.levels[LV_FAT], (uint32_t)__builtin_popcountll(fm));
         fbase = __shfl(fbase, __builtin_ctzll(fm), RTX_WAVE);
         if (fat) {
-            fat_list[fbase + lanes_below(fm)] = n;
-            node[2 * (size_t)n] = make_float4(0.f, 0.f, 0.f, __uint_as_float(start));
-            node[2 * (size_t)n + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(end - start));
+            A.fat_list[fbase + lanes_below(fm)] = n;
+            A.node[2 * (size_t)n] = make_float4(0.f, 0.f, 0.f, __uint_as_float(start));
+            A.node[2 * (size_t)n + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(end - start));
         }
     }
-    if (!active) return;
-    if (is_split) {
-        uint32_t rank = s_prefix + wofs + excl;
-        uint32_t c = off + cnt + 2u * rank;
-        bstart[c] = start;
-        bend[c] = split;
-        bstart[c + 1] = split;
-        bend[c + 1] = end;
-        parent[c] = n;
-        parent[c + 1] = n;
-        uint32_t* node_u32 = reinterpret_cast<uint32_t*>(node);
+    if (active && is_split) {
+        const uint32_t rank = *s_prefix + wofs + excl;
+        const uint32_t c = off + cnt + 2u * rank;
+        A.bstart[c] = start;
+        A.bend[c] = split;
+        A.bstart[c + 1] = split;
+        A.bend[c + 1] = end;
+        A.parent[c] = n;
+        A.parent[c + 1] = n;
+        uint32_t* node_u32 = reinterpret_cast<uint32_t*>(A.node);
         node_u32[8 * (size_t)n + 3] = c;
         node_u32[8 * (size_t)n + 7] = 0u;
-    } else if (!fat) {
+    } else if (active && !fat) {
         // build_leaf_bbox (src/kernel/bvh-builder.cl:262-288) for this leaf
-        leaf_bounds(tri, node, n, start, end - start);
+        leaf_bounds(A.tri, A.node, n, start, end - start);
+    }
+    __syncthreads();
+}
+
+// Levels 0..dtop (at most 2^d <= BS nodes each, one tile) by one block, one
+// level after the other: the top of the tree costs one launch, not dtop+1.
+__global__ __launch_bounds__(BS) void k_hier_top(HierArgs A, uint32_t* status, uint32_t status_stride, int dtop,
+                                                 int depth) {
+    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_w[BS / RTX_WAVE];
+    for (int d = 0; d <= dtop; ++d) {
+        const uint32_t off = A.levels[LV_OFF + d], cnt = A.levels[LV_CNT + d];
+        if (cnt == 0) {
+            if (threadIdx.x == 0) {
+                A.levels[LV_OFF + d + 1] = off;
+                A.levels[LV_CNT + d + 1] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        hier_tile(A, status + (size_t)d * status_stride, d, d == depth, off, cnt, 0u, 1u, s_w, &s_prefix);
+    }
+}
+
+// One iteration of process_task (src/kernel/bvh-builder.cl:582-671) over the
+// nodes of level d, in order.  Children of the k-th splitting node of the
+// level go to off_next + 2k (ordered compaction by decoupled look-back),
+// which is the order in-order work-groups of the reference produce.  Blocks
+// are persistent: each takes tile tickets until the level is done.
+__global__ __launch_bounds__(BS) void k_hier_level(HierArgs A, uint32_t* status, uint32_t* ticket, int d,
+                                                   int force_leaf) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[BS / RTX_WAVE];
+    const uint32_t off = A.levels[LV_OFF + d];
+    const uint32_t cnt = A.levels[LV_CNT + d];
+    if (cnt == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            A.levels[LV_OFF + d + 1] = off;
+            A.levels[LV_CNT + d + 1] = 0;
+        }
+        return;
+    }
+    const uint32_t ntiles = (cnt + BS - 1) / BS;
+    if (blockIdx.x >= ntiles) return;  // only blocks with work take tickets
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= ntiles) return;
+        hier_tile(A, status, d, force_leaf != 0, off, cnt, tile, ntiles, s_w, &s_prefix);
     }
 }
 
@@ -565,6 +615,30 @@ __global__ __launch_bounds__(BS) void k_refit_level(float4* __restrict__ node, f
         node[2 * (size_t)n + 1] = make_float4(phi.x, phi.y, phi.z, f1.w);
         v3 dl = xyz(R0) - xyz(L0);
         delta[n] = make_float4(dl.x, dl.y, dl.z, 0.f);
+    }
+}
+
+// Internal-node AABBs of levels dhi..0 (at most 2^d nodes each) by one
+// block, deepest first.
+__global__ __launch_bounds__(BS) void k_refit_top(float4* __restrict__ node, float4* __restrict__ delta,
+                                                  const uint32_t* __restrict__ levels, int dhi) {
+    for (int d = dhi; d >= 0; --d) {
+        const uint32_t off = levels[LV_OFF + d], cnt = levels[LV_CNT + d];
+        for (uint32_t k = threadIdx.x; k < cnt; k += BS) {
+            const uint32_t n = off + k;
+            const float4 f0 = node[2 * (size_t)n], f1 = node[2 * (size_t)n + 1];
+            if (__float_as_uint(f1.w) != 0u) continue;
+            const uint32_t l = __float_as_uint(f0.w);
+            const float4 L0 = node[2 * (size_t)l], L1 = node[2 * (size_t)l + 1];
+            const float4 R0 = node[2 * (size_t)l + 2], R1 = node[2 * (size_t)l + 3];
+            const v3 plo = vmin(xyz(L0), xyz(R0));
+            const v3 phi = vmax(xyz(L1), xyz(R1));
+            node[2 * (size_t)n] = make_float4(plo.x, plo.y, plo.z, f0.w);
+            node[2 * (size_t)n + 1] = make_float4(phi.x, phi.y, phi.z, f1.w);
+            const v3 dl = xyz(R0) - xyz(L0);
+            delta[n] = make_float4(dl.x, dl.y, dl.z, 0.f);
+        }
+        __syncthreads();
     }
 }
 
@@ -621,13 +695,20 @@ static void fat_leaf_bounds(rtx_scene* s, hipStream_t st) {
 // Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
 // `depth` holds only leaves).
 static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
-    for (int d = depth - 1; d >= 0; --d) {
+    const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
+    for (int d = depth - 1; d > REFIT_TOP; --d) {
         ProfScope ps(s->ctx, st, "lbvh_refit_level");
         uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)s->T;
         if (maxn > 2ull * s->T) maxn = 2ull * s->T;
         uint32_t g = rtx_div_up(maxn, BS);
-        hipLaunchKernelGGL(k_refit_level, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->nodes(),
+        hipLaunchKernelGGL(k_refit_level, dim3(g > gmax ? gmax : g), dim3(BS), 0, st, s->nodes(),
                            s->delta.as<float4>(), s->levels.as<uint32_t>(), d);
+    }
+    const int dhi = depth - 1 < REFIT_TOP ? depth - 1 : REFIT_TOP;
+    if (dhi >= 0) {
+        ProfScope ps(s->ctx, st, "lbvh_refit_top");
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(BS), 0, st, s->nodes(), s->delta.as<float4>(),
+                           s->levels.as<uint32_t>(), dhi);
     }
 }
 
@@ -717,16 +798,26 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         hipLaunchKernelGGL(k_radix_tree, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->codes.as<uint32_t>(),
                            s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), (int)T);
     }
-    for (int d = 0; d <= depth; ++d) {
-        ProfScope ps(s->ctx, st, "lbvh_hier_level");
-        uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;
-        if (maxn > 2ull * T) maxn = 2ull * T;
-        uint32_t grid = rtx_div_up(maxn, BS);
-        hipLaunchKernelGGL(k_hier_level, dim3(grid), dim3(BS), 0, st, s->codes.as<uint32_t>(),
-                           s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), s->bstart.as<uint32_t>(),
-                           s->bend.as<uint32_t>(), s->nodes(), s->parent.as<uint32_t>(),
-                           s->tri.as<float4>(), levels, s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles,
-                           tickets + TK_LEVEL + d, s->fat.as<uint32_t>(), d, T, min_leaf, d == depth ? 1 : 0);
+    {
+        HierArgs A{s->codes.as<uint32_t>(), s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(),
+                   s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->nodes(), s->parent.as<uint32_t>(),
+                   s->tri.as<float4>(), levels, s->fat.as<uint32_t>(), T, min_leaf};
+        const int dtop = depth < HIER_TOP ? depth : HIER_TOP;
+        {
+            ProfScope ps(s->ctx, st, "lbvh_hier_top");
+            hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, s->hier_status.as<uint32_t>(), hier_tiles,
+                               dtop, depth);
+        }
+        const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
+        for (int d = dtop + 1; d <= depth; ++d) {
+            ProfScope ps(s->ctx, st, "lbvh_hier_level");
+            uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;
+            if (maxn > 2ull * T) maxn = 2ull * T;
+            uint32_t grid = rtx_div_up(maxn, BS);
+            hipLaunchKernelGGL(k_hier_level, dim3(grid > gmax ? gmax : grid), dim3(BS), 0, st, A,
+                               s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d,
+                               d == depth ? 1 : 0);
+        }
     }
     fat_leaf_bounds(s, st);
     refit_internal(s, st, depth);

@@ -83,7 +83,8 @@ def test_lbvh_random(ctx, T, seed):
     s.destroy()
 
 
-@pytest.mark.parametrize("depth,min_leaf", [(30, 1), (30, 4), (24, 1), (32, 2), (9, 1), (1, 1), (12, 8)])
+@pytest.mark.parametrize("depth,min_leaf", [(30, 1), (30, 4), (24, 1), (32, 2), (9, 1), (8, 1), (10, 2), (11, 1),
+                                            (2, 1), (1, 1), (12, 8)])
 def test_lbvh_params(ctx, depth, min_leaf):
     pos, idx, nrm, uv, mm = random_scene(100 + depth, 20_000)
     s = load_scene(ctx, pos, idx, nrm, uv, mm)
