@@ -12,11 +12,14 @@ value = (primary + secondary rays) per frame x steps / timed seconds
 (the reference's ray count, src/rt/renderer.cpp:137, 184-185).
 
 Also reported: LBVH build ms, the plain one-launch-per-stage path (with the
-reference's per-bounce count read-back), per-kernel times, the roofline of
-the radix-sort pass (the build's HBM-bound kernel), and the CPU oracle
-timed on this host on the same frame (one core).
+reference's per-bounce count read-back), per-kernel times (HIP events on the
+stream each kernel runs on), the roofline of the dominant kernel (closest-hit
+traversal) and of the build's HBM-bound radix-sort pass, with HBM traffic
+per launch when recorded under profiles/, and the CPU oracle timed on this
+host on the same frame (one core, median of 3 frames).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --copy-calibration 256   # device copy for counter scaling
 """
 from __future__ import annotations
 
@@ -51,17 +54,33 @@ def cpu_baseline(sd, cam_struct):
     return rays / dt / 1e6, dt, rays, tb
 
 
-def traffic_from_profiles():
-    """HBM bytes per sort-pass launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if the
-    summary has been recorded under profiles/."""
-    f = ROOT / "profiles" / "sort_pass_traffic.json"
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for
+    a device copy of known size), as recorded in profiles/hbm_traffic.json;
+    None when not recorded."""
+    f = ROOT / "profiles" / "hbm_traffic.json"
     if f.exists():
         try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
+            return json.loads(f.read_text())["kernels"][kernel]["bytes_per_launch"]
         except Exception:
             return None
     return None
+
+
+def copy_calibration(mib: int, reps: int = 3):
+    """A plain device-to-device copy of `mib` MiB (read mib + write mib), run
+    under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": mib << 20, "copy_calibration_bytes_written": mib << 20,
+                      "reps": reps}))
 
 
 def main():
@@ -71,7 +90,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--profile-frames", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=3, help="oracle frames timed for the CPU baseline (median)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(image-space sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
@@ -142,11 +167,27 @@ def main():
     kernels = {k: {"launches_per_frame": c / nf, "ms_per_frame": ms / nf, "avg_launch_ms": ms / c}
                for k, (c, ms) in sorted(rep.items())}
     build_ms = kernels.get("LBVH_BUILD", {}).get("avg_launch_ms")
-    sp = kernels.get("lbvh_sort_pass", {})
-    sort_avg_s = sp.get("avg_launch_ms", 0.0) / 1e3
     T = sd.T
-    sort_bytes = 16.0 * T  # per pass: key+value read (8 B) + key+value write (8 B)
-    achieved = sort_bytes / sort_avg_s / 1e9 if sort_avg_s > 0 else None
+
+    def roofline(key, kernel, algo_bytes, note):
+        k = kernels.get(key, {})
+        avg_s = k.get("avg_launch_ms", 0.0) / 1e3
+        ach = algo_bytes / avg_s / 1e9 if avg_s > 0 else None
+        return {"kernel": kernel, "bound": "hbm", "achieved": round(ach, 2) if ach else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5) if ach else None,
+                "traffic": traffic_from_profiles(kernel.split()[0]), "algorithmic_bytes_per_launch": algo_bytes,
+                "avg_launch_ms": round(avg_s * 1e3, 5), "launches_per_frame": k.get("launches_per_frame"),
+                "note": note}
+
+    # dominant kernel: closest-hit traversal (4 launches per frame: primary
+    # + 3 bounces); algorithmic bytes = ray in (32 B) + hit out (16 B) per ray
+    trace_launches = kernels.get("trace", {}).get("launches_per_frame", 1.0) or 1.0
+    rl_trace = roofline("trace", "k_trace (closest-hit LBVH traversal)", 48.0 * rays_per_frame / trace_launches,
+                        "48 B/ray (32 B ray in + 16 B hit out), rays of a frame / launches per frame; node and "
+                        "triangle fetches are cache traffic, not algorithmic bytes: the kernel is latency-bound")
+    # the build's HBM-bound sort: 16 B/element/pass ideal (key+value read and write)
+    rl_sort = roofline("lbvh_sort_pass", "k_sort_pass (onesweep 8-bit LSD pass over (code, id) pairs)", 16.0 * T,
+                       "16 B per element per pass (8 B read + 8 B write)")
     trace_ms = kernels.get("trace", {}).get("ms_per_frame", 0.0)
 
     out = {
@@ -175,17 +216,8 @@ def main():
         "stage_launch_ms_per_frame": round(async_ms, 4),
         "plain_path_ms_per_frame": round(plain_ms, 4),
         "plain_path_mrays": round(rays_per_frame / plain_ms / 1e3, 3),
-        "roofline": {
-            "kernel": "lbvh_sort_pass (onesweep 8-bit LSD pass over (code, id) pairs)",
-            "bound": "hbm",
-            "achieved": round(achieved, 2) if achieved else None,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-            "traffic": traffic_from_profiles(),
-            "algorithmic_bytes_per_launch": sort_bytes,
-            "avg_launch_ms": round(sort_avg_s * 1e3, 5),
-        },
+        "roofline": rl_trace,
+        "roofline_sort_pass": rl_sort,
         "traversal": {
             "trace_ms_per_frame": round(trace_ms, 4),
             "shadow_ms_per_frame": round(kernels.get("trace_shadow", {}).get("ms_per_frame", 0.0), 4),
@@ -194,11 +226,13 @@ def main():
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
     }
     if not args.no_cpu:
-        log("CPU oracle baseline (one core, full frame) ...")
-        mr, secs, rays, tb = cpu_baseline(sd, cam)
+        log(f"CPU oracle baseline (one core, {args.cpu_frames} full frames) ...")
+        runs = [cpu_baseline(sd, cam) for _ in range(max(1, args.cpu_frames))]
+        runs.sort(key=lambda r: r[1])
+        mr, secs, rays, tb = runs[len(runs) // 2]
         out["cpu_baseline"] = {"value": round(mr, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
-                               "sample": f"whole C2 frame on one core: LBVH build ({tb:.2f}s) + render of "
-                                         f"{rays} rays, {secs:.1f}s total",
+                               "sample": f"median of {len(runs)} whole C2 frames on one core (sequential oracle): "
+                                         f"LBVH build ({tb:.2f}s) + render of {rays} rays, {secs:.1f}s per frame",
                                "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
     else:
         out["cpu_baseline"] = None
