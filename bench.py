@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--profile-frames", type=int, default=3)
     ap.add_argument("--cpu-frames", type=int, default=3, help="oracle frames timed for the CPU baseline (median)")
+    ap.add_argument("--c3-frames", type=int, default=300, help="frames of the dynamic-scene leg (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -225,6 +226,13 @@ def main():
         },
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
     }
+    if args.c3_frames > 0:
+        log(f"C3 dynamic scene, {args.c3_frames} graph frames ...")
+        R.destroy()
+        s.destroy()
+        R = s = None
+        out["c3_dynamic"] = bench_c3(ctx, args.c3_frames)
+        log(f"C3: {out['c3_dynamic']['ms_per_frame']} ms/frame, {out['c3_dynamic']['mrays']} Mrays/s")
     if not args.no_cpu:
         log(f"CPU oracle baseline (one core, {args.cpu_frames} full frames) ...")
         runs = [cpu_baseline(sd, cam) for _ in range(max(1, args.cpu_frames))]
@@ -236,10 +244,55 @@ def main():
                                "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
     else:
         out["cpu_baseline"] = None
-    R.destroy()
-    s.destroy()
+    if R is not None:
+        R.destroy()
+        s.destroy()
     ctx.close()
     print(json.dumps(out))
+
+
+def bench_c3(ctx, frames: int):
+    """Dynamic scene C3: 724x724 heightfield (1,045,458 triangles) displaced by
+    8 travelling waves on the device every frame, LBVH rebuilt, 1280x720,
+    1 point light, Whitted depth 2; the whole frame is one hipGraph whose
+    device clock advances the animation by 1/60 s per replay."""
+    import rtx
+    from rtx import scenes
+    sd = scenes.config_c3()
+    s = scenes.load(rtx.Scene(ctx), sd)
+    s.set_waves(sd.waves)
+    cam = scenes.camera_of(sd)
+    R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
+    R.set_animation(True, 0.0, 1.0 / 60.0)
+    for _ in range(2):
+        R.frame_async(s, cam, rebuild=True)
+    ctx.sync()
+    R.capture(s, cam, rebuild=True)
+    # ray counts of every frame (frames are a function of the clock), untimed
+    s.set_clock(0)
+    rays = shadow = 0
+    for _ in range(frames):
+        R.replay()
+        st = R.last_stats()
+        rays += st.primary_rays + st.secondary_rays
+        shadow += st.shadow_rays
+    # the timed pass over the same frames
+    s.set_clock(0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        R.replay()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    out = {"workload": "C3 dynamic: 724x724 heightfield = 1,045,458 triangles displaced on the device every frame "
+                       "(8 waves), full LBVH rebuild, 1280x720, 1 point light, Whitted depth 2, one hipGraph "
+                       "per frame",
+           "frames": frames, "ms_per_frame": round(dt / frames * 1e3, 4),
+           "mrays": round(rays / dt / 1e6, 3), "mrays_incl_shadow": round((rays + shadow) / dt / 1e6, 3),
+           "rays_per_frame_mean": round(rays / frames, 1), "triangles": sd.T}
+    R.destroy()
+    s.destroy()
+    return out
 
 
 def _cpu_model() -> str:

@@ -189,6 +189,15 @@ uint32_t rtx_scene_triangle_count(rtx_scene* s);
  * The dynamic-scene pattern of src/rt/rt-wave.cpp:139-270 (mangle kernel,
  * src/kernel/mangler.cl:13-32). */
 int32_t rtx_scene_displace_waves(rtx_scene* s, const float* waves, int32_t n, float t, void* stream);
+/* Device-resident form of the same animation, for captured frames (no
+ * counterpart in the reference, whose rt-wave loop passes t from the host
+ * every frame): set_waves uploads the wave table once (blocking) and resets
+ * the device frame clock to 0; animate displaces at t = t0 + dt * clock and
+ * then advances the clock by one, both on `stream`. */
+int32_t rtx_scene_set_waves(rtx_scene* s, const float* waves, int32_t n);
+int32_t rtx_scene_set_clock(rtx_scene* s, uint32_t frame);
+int32_t rtx_scene_get_clock(rtx_scene* s, uint32_t* frame);
+int32_t rtx_scene_animate(rtx_scene* s, float t0, float dt, void* stream);
 
 /* ----------------------------------------------------------------- LBVH -- */
 /* BVHBuilder::build_lbvh (src/rt/bvh-builder.cpp:201-741): prim AABB ->
@@ -209,6 +218,11 @@ int32_t rtx_read_indices(rtx_scene* s, uint32_t* idx3, uint32_t T);     /* permu
 int32_t rtx_read_matmap(rtx_scene* s, int32_t* matmap, uint32_t T);     /* permuted material map */
 int32_t rtx_read_orig_ids(rtx_scene* s, uint32_t* orig, uint32_t T);    /* sorted pos -> triangle id at upload */
 int32_t rtx_read_global_bbox(rtx_scene* s, float* lo3, float* hi3);
+/* Current vertex positions and vertex normals (xyz, V*3 floats each; either
+ * may be NULL), as a vertex buffer holds them (hits use their normalization,
+ * src/kernel/trace-bvh.cl:222-224): the displaced geometry that the oracle
+ * renders in dynamic-scene parity checks. */
+int32_t rtx_read_vertices(rtx_scene* s, float* pos3, float* nrm3, uint32_t V);
 /* Nodes in the reference's 48-byte layout, canonical level order. */
 int32_t rtx_read_nodes_ref(rtx_scene* s, rtx_node_ref* out, uint32_t cap, uint32_t* count);
 
@@ -302,6 +316,10 @@ int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats);
 int32_t rtx_renderer_capture(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream);
 /* Launches the captured frame; asynchronous. */
 int32_t rtx_renderer_replay(rtx_renderer* r, void* stream);
+/* Dynamic scene (src/rt/rt-wave.cpp:139-270): when enabled, every frame
+ * (plain, read-back-free or captured) starts with rtx_scene_animate(s, t0,
+ * dt) before the LBVH rebuild. */
+int32_t rtx_renderer_set_animation(rtx_renderer* r, int32_t enable, float t0, float dt);
 
 /* --------------------------------------------------------- host helpers -- */
 /* Camera::set (src/rt/camera.cpp:31-59); host only. */

@@ -54,18 +54,20 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
 // (complete_trace_info normalizes each vertex normal at every hit,
 // src/kernel/trace-bvh.cl:222-224; same operation, done here once).
 __global__ void k_vertices(const float* __restrict__ pos, const float* __restrict__ nrm, const float* __restrict__ uv,
-                           float4* __restrict__ pos4, float4* __restrict__ nrm4, float2* __restrict__ uv2, uint32_t V,
-                           uint32_t stride_f) {
+                           float4* __restrict__ pos4, float4* __restrict__ nrm4, float4* __restrict__ vnrm4,
+                           float2* __restrict__ uv2, uint32_t V, uint32_t stride_f) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     const float* p = pos + (size_t)i * stride_f;
     pos4[i] = make_float4(p[0], p[1], p[2], 0.f);
     if (nrm4) {
-        v3 n = mk3(0.f, 0.f, 0.f);
+        v3 raw = mk3(0.f, 0.f, 0.f), n = raw;
         if (nrm) {
             const float* q = nrm + (size_t)i * stride_f;
-            n = normalize(mk3(q[0], q[1], q[2]));
+            raw = mk3(q[0], q[1], q[2]);
+            n = normalize(raw);
         }
+        vnrm4[i] = make_float4(raw.x, raw.y, raw.z, 0.f);
         nrm4[i] = make_float4(n.x, n.y, n.z, 0.f);
     }
     if (uv2) {
@@ -79,12 +81,12 @@ __global__ void k_vertices(const float* __restrict__ pos, const float* __restric
 }
 
 // Heightfield displacement: y = sum_k a sin(kappa (d . (x,z)) - omega t + phi),
-// normal = normalize(-dy/dx, 1, -dy/dz).  Reads x,z from pos4, writes y and
-// the unit normal.
-__global__ void k_waves(float4* __restrict__ pos4, float4* __restrict__ nrm4, const float* __restrict__ waves, int nw,
-                        float t, uint32_t V) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
+// vertex normal = normalize(-dy/dx, 1, -dy/dz).  Reads x,z from pos4, writes
+// y, the vertex normal (vnrm4, what a vertex buffer holds) and its
+// normalization (nrm4, what the hit completion uses).
+__device__ __forceinline__ void displace_vertex(float4* __restrict__ pos4, float4* __restrict__ nrm4,
+                                                float4* __restrict__ vnrm4, const float* __restrict__ waves, int nw,
+                                                float t, uint32_t i) {
     float4 p = pos4[i];
     float y = 0.f, gx = 0.f, gz = 0.f;
     for (int k = 0; k < nw; ++k) {
@@ -97,9 +99,31 @@ __global__ void k_waves(float4* __restrict__ pos4, float4* __restrict__ nrm4, co
         gz += w[0] * c * w[1] * w[3];
     }
     pos4[i] = make_float4(p.x, y, p.z, 0.f);
-    v3 n = normalize(mk3(-gx, 1.f, -gz));
+    const v3 vn = normalize(mk3(-gx, 1.f, -gz));
+    vnrm4[i] = make_float4(vn.x, vn.y, vn.z, 0.f);
+    const v3 n = normalize(vn);
     nrm4[i] = make_float4(n.x, n.y, n.z, 0.f);
 }
+
+__global__ void k_waves(float4* __restrict__ pos4, float4* __restrict__ nrm4, float4* __restrict__ vnrm4,
+                        const float* __restrict__ waves, int nw, float t, uint32_t V) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    displace_vertex(pos4, nrm4, vnrm4, waves, nw, t, i);
+}
+
+// Same at t = t0 + dt * frame, with the frame number read from the device
+// clock (so that a captured frame advances the animation on every replay).
+__global__ void k_waves_clock(float4* __restrict__ pos4, float4* __restrict__ nrm4, float4* __restrict__ vnrm4,
+                              const float* __restrict__ waves, int nw, float t0, float dt,
+                              const uint32_t* __restrict__ clock, uint32_t V) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float t = t0 + dt * (float)(*clock);
+    displace_vertex(pos4, nrm4, vnrm4, waves, nw, t, i);
+}
+
+__global__ void k_tick(uint32_t* clock) { *clock += 1u; }
 }  // namespace
 
 int32_t rtx_fill_u32(rtx_ctx* ctx, void* p, uint32_t value, uint64_t n, hipStream_t st) {
@@ -270,12 +294,12 @@ int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out) {
 int32_t rtx_scene_destroy(rtx_scene* s) {
     RTX_REQUIRE(s, RTX_EINVAL, "rtx_scene_destroy: null scene");
     (void)hipStreamSynchronize(s->ctx->stream);
-    DevBuf* all[] = {&s->pos,      &s->nrm,        &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
+    DevBuf* all[] = {&s->pos,      &s->nrm,        &s->vnrm,     &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
                      &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
                      &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
                      &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
                      &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
-                     &s->hier_status, &s->waves,      &s->stage};
+                     &s->hier_status, &s->waves,      &s->stage,    &s->clock};
     for (DevBuf* b : all) b->release();
     delete s;
     return RTX_OK;
@@ -287,6 +311,7 @@ static int32_t upload_vertices(rtx_scene* s, const float* pos, const float* nrm,
     int32_t e;
     if ((e = s->pos.reserve(c, (size_t)V * 16))) return e;
     if ((e = s->nrm.reserve(c, (size_t)V * 16))) return e;
+    if ((e = s->vnrm.reserve(c, (size_t)V * 16))) return e;
     if ((e = s->uv.reserve(c, (size_t)V * 8))) return e;
     // staging: raw host arrays copied as-is, converted on the device
     size_t fpv = stride_f;
@@ -304,7 +329,7 @@ static int32_t upload_vertices(rtx_scene* s, const float* pos, const float* nrm,
     }
     hipLaunchKernelGGL(k_vertices, dim3(rtx_div_up(V, 256)), dim3(256), 0, c->stream, sp.as<float>(),
                        nrm ? sn.as<float>() : nullptr, uv ? su.as<float>() : nullptr, s->pos.as<float4>(),
-                       s->nrm.as<float4>(), s->uv.as<float2>(), V, stride_f);
+                       s->nrm.as<float4>(), s->vnrm.as<float4>(), s->uv.as<float2>(), V, stride_f);
     RTX_CHECK_HIP(hipGetLastError());
     RTX_CHECK_HIP(hipStreamSynchronize(c->stream));
     sp.release();
@@ -345,7 +370,7 @@ int32_t rtx_scene_update_positions(rtx_scene* s, const float* host_pos, void* st
     ProfScope ps(s->ctx, st, "vertex_update");
     hipLaunchKernelGGL(k_vertices, dim3(rtx_div_up(s->V, 256)), dim3(256), 0, st, s->stage.as<float>(),
                        (const float*)nullptr, (const float*)nullptr, s->pos.as<float4>(), (float4*)nullptr,
-                       (float2*)nullptr, s->V, 3u);
+                       (float4*)nullptr, (float2*)nullptr, s->V, 3u);
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -359,7 +384,47 @@ int32_t rtx_scene_displace_waves(rtx_scene* s, const float* waves, int32_t n, fl
     RTX_CHECK_HIP(hipMemcpyAsync(dev_w, waves, (size_t)n * 6 * 4, hipMemcpyHostToDevice, st));
     ProfScope ps(s->ctx, st, "waves");
     hipLaunchKernelGGL(k_waves, dim3(rtx_div_up(s->V, 256)), dim3(256), 0, st, s->pos.as<float4>(),
-                       s->nrm.as<float4>(), dev_w, n, t, s->V);
+                       s->nrm.as<float4>(), s->vnrm.as<float4>(), dev_w, n, t, s->V);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_waves(rtx_scene* s, const float* waves, int32_t n) {
+    RTX_REQUIRE(s && waves && n > 0 && n <= 64, RTX_EINVAL, "rtx_scene_set_waves: bad argument");
+    int32_t e = s->waves.reserve(s->ctx, 64 * 6 * 4);
+    if (e) return e;
+    if ((e = s->clock.reserve(s->ctx, 4))) return e;
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(s->waves.p, waves, (size_t)n * 6 * 4, hipMemcpyHostToDevice));
+    const uint32_t zero = 0;
+    RTX_CHECK_HIP(hipMemcpy(s->clock.p, &zero, 4, hipMemcpyHostToDevice));
+    s->nwaves = n;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_clock(rtx_scene* s, uint32_t frame) {
+    RTX_REQUIRE(s && s->nwaves > 0, RTX_ESTATE, "rtx_scene_set_clock: no waves set");
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(s->clock.p, &frame, 4, hipMemcpyHostToDevice));
+    return RTX_OK;
+}
+
+int32_t rtx_scene_get_clock(rtx_scene* s, uint32_t* frame) {
+    RTX_REQUIRE(s && frame && s->nwaves > 0, RTX_ESTATE, "rtx_scene_get_clock: no waves set");
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    RTX_CHECK_HIP(hipMemcpy(frame, s->clock.p, 4, hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+int32_t rtx_scene_animate(rtx_scene* s, float t0, float dt, void* stream) {
+    RTX_REQUIRE(s && s->V, RTX_EINVAL, "rtx_scene_animate: no vertices");
+    RTX_REQUIRE(s->nwaves > 0, RTX_ESTATE, "rtx_scene_animate: no waves set (rtx_scene_set_waves)");
+    hipStream_t st = s->ctx->pick(stream);
+    ProfScope ps(s->ctx, st, "waves");
+    hipLaunchKernelGGL(k_waves_clock, dim3(rtx_div_up(s->V, 256)), dim3(256), 0, st, s->pos.as<float4>(),
+                       s->nrm.as<float4>(), s->vnrm.as<float4>(), s->waves.as<float>(), s->nwaves, t0, dt,
+                       s->clock.as<uint32_t>(), s->V);
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, s->clock.as<uint32_t>());
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -441,6 +506,20 @@ int32_t rtx_read_orig_ids(rtx_scene* s, uint32_t* orig, uint32_t T) {
     RTX_REQUIRE(s && T == s->T, RTX_EINVAL, "rtx_read_orig_ids: wrong T");
     return readback(s, s->orig, orig, (size_t)T * 4, "rtx_read_orig_ids");
 }
+int32_t rtx_read_vertices(rtx_scene* s, float* pos3, float* nrm3, uint32_t V) {
+    RTX_REQUIRE(s && V == s->V, RTX_EINVAL, "rtx_read_vertices: wrong vertex count");
+    std::vector<float> tmp((size_t)V * 4);
+    for (int which = 0; which < 2; ++which) {
+        float* dst = which ? nrm3 : pos3;
+        if (!dst) continue;
+        int32_t e = readback(s, which ? s->vnrm : s->pos, tmp.data(), tmp.size() * 4, "rtx_read_vertices");
+        if (e) return e;
+        for (size_t i = 0; i < V; ++i)
+            for (int k = 0; k < 3; ++k) dst[3 * i + k] = tmp[4 * i + k];
+    }
+    return RTX_OK;
+}
+
 int32_t rtx_read_global_bbox(rtx_scene* s, float* lo3, float* hi3) {
     RTX_REQUIRE(s && s->built && lo3 && hi3, RTX_EINVAL, "rtx_read_global_bbox: no BVH");
     float g[6];

@@ -91,10 +91,20 @@ int32_t rtx_renderer_destroy(rtx_renderer* r) {
 
 rtx_fb* rtx_renderer_fb(rtx_renderer* r) { return r ? r->fb : nullptr; }
 
+int32_t rtx_renderer_set_animation(rtx_renderer* r, int32_t enable, float t0, float dt) {
+    RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_set_animation: null renderer");
+    r->animate = enable != 0;
+    r->anim_t0 = t0;
+    r->anim_dt = dt;
+    return RTX_OK;
+}
+
 // set_up_frame: counters, framebuffer clear, accelerator update.
 static int32_t set_up_frame(rtx_renderer* r, rtx_scene* s, int32_t rebuild, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, r->counts.as<unsigned long long>(), 8);
     int32_t e;
+    // the dynamic scene's displacement (src/rt/rt-wave.cpp:154-167), then the rebuild
+    if (r->animate && (e = rtx_scene_animate(s, r->anim_t0, r->anim_dt, st))) return e;
     if ((e = rtx_fb_clear(r->fb, st))) return e;
     if (rebuild || !s->built) {
         rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};

@@ -1,7 +1,8 @@
 // Internal declarations of the rtx library (gfx950).
 //
 // Device data layout (HBM):
-//   scene : pos4[V] (x,y,z,0)   nrm4[V] (unit normal, 0)   uv2[V]
+//   scene : pos4[V] (x,y,z,0)   nrm4[V] (normalized vertex normal, 0)   uv2[V]
+//           vnrm4[V] (vertex normal as given, read back for parity)
 //           idx[3T] (permuted in place by each build)   matmap[T]   orig[T]
 //   bvh   : node[N]  32 B = {lo.xyz, a | hi.xyz, b}; internal: a = left child
 //                    (right = a+1, children are adjacent), b = 0;
@@ -99,8 +100,11 @@ struct rtx_scene {
     rtx_ctx* ctx = nullptr;
     uint32_t V = 0, T = 0, M = 0;
     DevBuf pos, nrm, uv;
+    DevBuf vnrm;        // vertex normals as given (nrm holds their normalization)
     DevBuf idx, idx_alt, matmap, matmap_alt, orig, orig_alt;
     DevBuf mats, lights, waves, stage;
+    DevBuf clock;       // uint32 frame number of the device-side animation
+    int nwaves = 0;     // wave table entries set by rtx_scene_set_waves
     rtx_lights host_lights{};
     // BVH
     bool built = false;
@@ -156,6 +160,9 @@ struct rtx_renderer {
     rtx_frame_stats last{};
     DevBuf counts;  // uint64 device ray counters of the current frame (see render.hip)
     bool last_async = false;
+    // Animation applied at the start of every frame (rtx_renderer_set_animation).
+    bool animate = false;
+    float anim_t0 = 0.f, anim_dt = 0.f;
     // Side stream of the read-back-free frame: each bounce's shadow rays and
     // shading run there while the main stream generates and traces the next
     // bounce.  ev[b] marks "bounce b traced" on the main stream, ev_join the

@@ -163,32 +163,34 @@ __device__ __forceinline__ bool tri_hit_v(float4 a, float4 b, float4 c, v3 o, v3
     return true;
 }
 
-// Leaf triangles are tested in order; the vertices of two triangles are
-// loaded before either is tested so their memory latencies overlap.
+// Leaf triangles are tested in order, four per round: the vertices of all
+// four are requested before any is tested (indices past the leaf are clamped
+// to its first triangle, a valid address whose result is discarded), so a
+// round costs one memory round trip.
+constexpr uint32_t LEAF_ROUND = 4;
+
 __device__ __forceinline__ void leaf_closest(const float4* __restrict__ tri, uint32_t a, uint32_t cnt, const RayC& r,
                                              Closest& best) {
     const uint32_t end = a + cnt;
-    for (uint32_t t = a; t < end; t += 2) {
-        const bool two = t + 1 < end;
-        const float4 p0 = tri[3 * (size_t)t], p1 = tri[3 * (size_t)t + 1], p2 = tri[3 * (size_t)t + 2];
-        float4 q0 = p0, q1 = p1, q2 = p2;
-        if (two) {
-            q0 = tri[3 * (size_t)t + 3];
-            q1 = tri[3 * (size_t)t + 4];
-            q2 = tri[3 * (size_t)t + 5];
+    for (uint32_t t = a; t < end; t += LEAF_ROUND) {
+        float4 v[3 * LEAF_ROUND];
+#pragma unroll
+        for (uint32_t j = 0; j < LEAF_ROUND; ++j) {
+            const size_t tj = 3 * (size_t)(t + j < end ? t + j : a);
+            v[3 * j] = tri[tj];
+            v[3 * j + 1] = tri[tj + 1];
+            v[3 * j + 2] = tri[tj + 2];
         }
-        float tt, u, v;
-        if (tri_hit_v(p0, p1, p2, r.o, r.d, &tt, &u, &v) && tt <= best.t && tt >= r.tmin) {
-            best.t = tt;  // later triangle wins an exact tie
-            best.id = (int32_t)t;
-            best.u = u;
-            best.v = v;
-        }
-        if (two && tri_hit_v(q0, q1, q2, r.o, r.d, &tt, &u, &v) && tt <= best.t && tt >= r.tmin) {
-            best.t = tt;
-            best.id = (int32_t)(t + 1);
-            best.u = u;
-            best.v = v;
+#pragma unroll
+        for (uint32_t j = 0; j < LEAF_ROUND; ++j) {
+            float tt, u, w;
+            if (t + j < end && tri_hit_v(v[3 * j], v[3 * j + 1], v[3 * j + 2], r.o, r.d, &tt, &u, &w) &&
+                tt <= best.t && tt >= r.tmin) {
+                best.t = tt;  // later triangle wins an exact tie
+                best.id = (int32_t)(t + j);
+                best.u = u;
+                best.v = w;
+            }
         }
     }
 }
@@ -196,18 +198,23 @@ __device__ __forceinline__ void leaf_closest(const float4* __restrict__ tri, uin
 __device__ __forceinline__ bool leaf_any(const float4* __restrict__ tri, uint32_t a, uint32_t cnt, const RayC& r,
                                          float tmax) {
     const uint32_t end = a + cnt;
-    for (uint32_t t = a; t < end; t += 2) {
-        const bool two = t + 1 < end;
-        const float4 p0 = tri[3 * (size_t)t], p1 = tri[3 * (size_t)t + 1], p2 = tri[3 * (size_t)t + 2];
-        float4 q0 = p0, q1 = p1, q2 = p2;
-        if (two) {
-            q0 = tri[3 * (size_t)t + 3];
-            q1 = tri[3 * (size_t)t + 4];
-            q2 = tri[3 * (size_t)t + 5];
+    for (uint32_t t = a; t < end; t += LEAF_ROUND) {
+        float4 v[3 * LEAF_ROUND];
+#pragma unroll
+        for (uint32_t j = 0; j < LEAF_ROUND; ++j) {
+            const size_t tj = 3 * (size_t)(t + j < end ? t + j : a);
+            v[3 * j] = tri[tj];
+            v[3 * j + 1] = tri[tj + 1];
+            v[3 * j + 2] = tri[tj + 2];
         }
-        float tt, u, v;
-        if (tri_hit_v(p0, p1, p2, r.o, r.d, &tt, &u, &v) && tt < tmax && tt > r.tmin) return true;
-        if (two && tri_hit_v(q0, q1, q2, r.o, r.d, &tt, &u, &v) && tt < tmax && tt > r.tmin) return true;
+        bool any = false;
+#pragma unroll
+        for (uint32_t j = 0; j < LEAF_ROUND; ++j) {
+            float tt, u, w;
+            any = any || (t + j < end && tri_hit_v(v[3 * j], v[3 * j + 1], v[3 * j + 2], r.o, r.d, &tt, &u, &w) &&
+                          tt < tmax && tt > r.tmin);
+        }
+        if (any) return true;
     }
     return false;
 }
@@ -431,7 +438,13 @@ __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__
         if (going) {
             const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
             float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (SPEC) dl4 = delta[curr];
+            if (SPEC) {
+                dl4 = delta[curr];
+                // keep the delta load next to the node load: without this the
+                // compiler sinks it into the internal-node branch, after the
+                // box test, and each internal visit waits for two round trips
+                __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));
+            }
             const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
             bool pop = true;
             if (slab(f0, f1, r, best.t)) {
@@ -472,7 +485,10 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
         if (going) {
             const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
             float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (SPEC) dl4 = delta[curr];
+            if (SPEC) {
+                dl4 = delta[curr];
+                __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+            }
             const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
             bool pop = true;
             if (slab(f0, f1, r, tmax)) {

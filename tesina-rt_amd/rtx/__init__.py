@@ -83,17 +83,18 @@ SYMBOLS = [
     "rtx_tune_get",
     "rtx_scene_create", "rtx_scene_destroy", "rtx_scene_set_vertices", "rtx_scene_set_vertices_ref",
     "rtx_scene_update_positions", "rtx_scene_set_triangles", "rtx_scene_set_materials", "rtx_scene_set_lights",
-    "rtx_scene_triangle_count", "rtx_scene_displace_waves",
+    "rtx_scene_triangle_count", "rtx_scene_displace_waves", "rtx_scene_set_waves", "rtx_scene_set_clock",
+    "rtx_scene_get_clock", "rtx_scene_animate",
     "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info",
     "rtx_read_codes", "rtx_read_perm", "rtx_read_indices", "rtx_read_matmap", "rtx_read_orig_ids",
-    "rtx_read_global_bbox", "rtx_read_nodes_ref",
+    "rtx_read_global_bbox", "rtx_read_vertices", "rtx_read_nodes_ref",
     "rtx_rays_create", "rtx_rays_destroy", "rtx_rays_capacity", "rtx_rays_count", "rtx_rays_read", "rtx_rays_write",
     "rtx_hits_create", "rtx_hits_destroy", "rtx_hits_read", "rtx_hits_write",
     "rtx_raygen_primary", "rtx_trace", "rtx_trace_shadow", "rtx_raygen_secondary",
     "rtx_fb_create", "rtx_fb_destroy", "rtx_fb_clear", "rtx_fb_resolve", "rtx_fb_read_accum", "rtx_fb_read_rgba8",
     "rtx_shade",
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
-    "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay",
+    "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay", "rtx_renderer_set_animation",
     "rtx_camera_set",
 ]
 
@@ -132,6 +133,12 @@ def lib() -> C.CDLL:
         "rtx_scene_set_lights": (I32, [P, P]),
         "rtx_scene_triangle_count": (U32, [P]),
         "rtx_scene_displace_waves": (I32, [P, P, I32, F, P]),
+        "rtx_scene_set_waves": (I32, [P, P, I32]),
+        "rtx_scene_set_clock": (I32, [P, U32]),
+        "rtx_scene_get_clock": (I32, [P, P]),
+        "rtx_scene_animate": (I32, [P, F, F, P]),
+        "rtx_read_vertices": (I32, [P, P, P, U32]),
+        "rtx_renderer_set_animation": (I32, [P, I32, F, F]),
         "rtx_lbvh_build": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
@@ -326,6 +333,29 @@ class Scene:
         w = np.ascontiguousarray(waves, np.float32).reshape(-1, 6)
         _check(lib().rtx_scene_displace_waves(self.h, _ptr(w), w.shape[0], float(t), stream),
                "rtx_scene_displace_waves")
+
+    def set_waves(self, waves: np.ndarray):
+        w = np.ascontiguousarray(waves, np.float32).reshape(-1, 6)
+        _check(lib().rtx_scene_set_waves(self.h, _ptr(w), w.shape[0]), "rtx_scene_set_waves")
+
+    def set_clock(self, frame: int):
+        _check(lib().rtx_scene_set_clock(self.h, int(frame)), "rtx_scene_set_clock")
+
+    def clock(self) -> int:
+        f = C.c_uint32(0)
+        _check(lib().rtx_scene_get_clock(self.h, C.byref(f)), "rtx_scene_get_clock")
+        return int(f.value)
+
+    def animate(self, t0: float, dt: float, stream=None):
+        _check(lib().rtx_scene_animate(self.h, float(t0), float(dt), stream), "rtx_scene_animate")
+
+    def vertices(self):
+        """(pos V x 3, nrm V x 3) float32 read back from the device."""
+        V = self.V
+        pos = np.zeros((V, 3), np.float32)
+        nrm = np.zeros((V, 3), np.float32)
+        _check(lib().rtx_read_vertices(self.h, _ptr(pos), _ptr(nrm), V), "rtx_read_vertices")
+        return pos, nrm
 
     def set_materials(self, mats: np.ndarray):
         mats = np.ascontiguousarray(mats, np.uint8)
@@ -525,6 +555,10 @@ class Renderer:
         """Capture one read-back-free frame into a hipGraph (render once eagerly first)."""
         _check(lib().rtx_renderer_capture(self.h, scene.h, C.byref(cam), int(rebuild), stream),
                "rtx_renderer_capture")
+
+    def set_animation(self, enable: bool, t0: float = 0.0, dt: float = 0.0):
+        _check(lib().rtx_renderer_set_animation(self.h, int(bool(enable)), float(t0), float(dt)),
+               "rtx_renderer_set_animation")
 
     def replay(self, stream=None):
         _check(lib().rtx_renderer_replay(self.h, stream), "rtx_renderer_replay")
