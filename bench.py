@@ -6,7 +6,9 @@ Workload (one GPU): 200 UV spheres x 5,000 triangles (1,000,000 triangles),
 depth 3 (reflection + refraction, IOR 1.5 on 30% of the spheres), full LBVH
 rebuild every frame.  A step is one frame: LBVH build -> primary rays ->
 trace -> shadow -> shade -> 3 x (secondary -> trace -> shadow -> shade) ->
-RGBA8 resolve, all on the GPU with inputs resident in HBM.
+RGBA8 resolve, all on the GPU with inputs resident in HBM.  Frames are
+pipelined as the reference's RendererT does (the next frame's build runs on a
+second stream while this frame renders); the sequential frame is timed too.
 
 value = (primary + secondary rays) per frame x steps / timed seconds
 (the reference's ray count, src/rt/renderer.cpp:137, 184-185).
@@ -141,6 +143,28 @@ def main():
     st2 = R.last_stats()
     assert st2.primary_rays + st2.secondary_rays == rays_per_frame, "replayed frame differs in ray count"
     log(f"timed {args.steps} graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
+    seq_ms, seq_value = ms_per_step, value
+
+    # pipelined frames (the reference's RendererT): the next frame's LBVH
+    # build on a second stream while this frame renders; captured likewise
+    R.set_pipelined(True)
+    R.frame_async(s, cam, rebuild=True)
+    ctx.sync()
+    R.capture(s, cam, rebuild=True)
+    for _ in range(args.warmup):
+        R.replay()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        R.replay()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    st3 = R.last_stats()
+    assert st3.primary_rays + st3.secondary_rays == rays_per_frame, "pipelined frame differs in ray count"
+    ms_per_step = dt / args.steps * 1e3
+    value = rays_per_frame * args.steps / dt / 1e6
+    log(f"timed {args.steps} pipelined graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
+    R.set_pipelined(False)
 
     # the same frame launched stage by stage (no graph)
     ctx.sync()
@@ -213,7 +237,11 @@ def main():
         "shadow_rays_per_frame": int(shadow_per_frame),
         "mrays_incl_shadow": round((rays_per_frame + shadow_per_frame) * args.steps / dt / 1e6, 3),
         "lbvh_build_ms": round(build_ms, 4) if build_ms else None,
-        "frame_mode": "hipGraph replay of LBVH rebuild + render",
+        "frame_mode": "hipGraph replay of the pipelined frame: LBVH rebuild of the next frame on a second stream "
+                      "while this frame renders (the reference's RendererT); every timed frame does one full "
+                      "rebuild and one full render",
+        "sequential_graph_ms_per_frame": round(seq_ms, 4),
+        "sequential_graph_mrays": round(seq_value, 3),
         "stage_launch_ms_per_frame": round(async_ms, 4),
         "plain_path_ms_per_frame": round(plain_ms, 4),
         "plain_path_mrays": round(rays_per_frame / plain_ms / 1e3, 3),
@@ -264,31 +292,39 @@ def bench_c3(ctx, frames: int):
     cam = scenes.camera_of(sd)
     R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
     R.set_animation(True, 0.0, 1.0 / 60.0)
-    for _ in range(2):
-        R.frame_async(s, cam, rebuild=True)
-    ctx.sync()
-    R.capture(s, cam, rebuild=True)
-    # ray counts of every frame (frames are a function of the clock), untimed
-    s.set_clock(0)
-    rays = shadow = 0
-    for _ in range(frames):
-        R.replay()
-        st = R.last_stats()
-        rays += st.primary_rays + st.secondary_rays
-        shadow += st.shadow_rays
-    # the timed pass over the same frames
-    s.set_clock(0)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        R.replay()
-    ctx.sync()
-    dt = time.perf_counter() - t0
+    timing = {}
+    for mode in ("sequential", "pipelined"):
+        R.set_pipelined(mode == "pipelined")
+        s.set_clock(0)
+        for _ in range(2):
+            R.frame_async(s, cam, rebuild=True)
+        ctx.sync()
+        R.capture(s, cam, rebuild=True)
+        # ray counts of every frame (frames are a function of the clock), untimed
+        s.set_clock(0)
+        rays = shadow = 0
+        for _ in range(frames):
+            R.replay()
+            st = R.last_stats()
+            rays += st.primary_rays + st.secondary_rays
+            shadow += st.shadow_rays
+        # the timed pass over the same frames
+        s.set_clock(0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            R.replay()
+        ctx.sync()
+        timing[mode] = (time.perf_counter() - t0, rays, shadow)
+    dt, rays, shadow = timing["pipelined"]
+    sdt = timing["sequential"][0]
     out = {"workload": "C3 dynamic: 724x724 heightfield = 1,045,458 triangles displaced on the device every frame "
                        "(8 waves), full LBVH rebuild, 1280x720, 1 point light, Whitted depth 2, one hipGraph "
-                       "per frame",
+                       "per frame, pipelined (next frame's displacement and build on a second stream)",
            "frames": frames, "ms_per_frame": round(dt / frames * 1e3, 4),
            "mrays": round(rays / dt / 1e6, 3), "mrays_incl_shadow": round((rays + shadow) / dt / 1e6, 3),
+           "sequential_ms_per_frame": round(sdt / frames * 1e3, 4),
+           "sequential_mrays": round(timing["sequential"][1] / sdt / 1e6, 3),
            "rays_per_frame_mean": round(rays / frames, 1), "triangles": sd.T}
     R.destroy()
     s.destroy()

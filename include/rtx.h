@@ -149,7 +149,10 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
 /* Tuning knobs, set by the autotuner (no reference counterpart: the
  * reference fixes its work-group sizes per device, src/gpu/function.cpp:322-385):
  *   "trace_variant": 0 single-node traversal, 1 single node with the near-child
- *                    delta loaded early, 2 sibling-pair traversal;
+ *                    delta loaded early, 2 sibling-pair traversal, 3 persistent
+ *                    waves fed from a ray queue (single-node steps);
+ *   "refill":        variant 3: idle lanes of a wave that trigger a refill
+ *                    from the queue (1..64);
  *   "trace_block":   64, 128 or 256 threads per traversal work-group;
  *   "xcds":          number of L2 domains the traversal grid is split over
  *                    (work-group i runs on XCD i mod xcds; each XCD gets one
@@ -320,6 +323,12 @@ int32_t rtx_renderer_replay(rtx_renderer* r, void* stream);
  * (plain, read-back-free or captured) starts with rtx_scene_animate(s, t0,
  * dt) before the LBVH rebuild. */
 int32_t rtx_renderer_set_animation(rtx_renderer* r, int32_t enable, float t0, float dt);
+/* Pipelined read-back-free frames (RendererT, src/rt/renderer-threaded.cpp:
+ * 452-506): while a frame renders, the next frame's animation and LBVH build
+ * run on a second stream; the image of each frame shows the scene as built
+ * one frame earlier.  Takes effect at the next rtx_render_frame_async /
+ * rtx_renderer_capture (capture after one eager pipelined frame). */
+int32_t rtx_renderer_set_pipelined(rtx_renderer* r, int32_t enable);
 
 /* --------------------------------------------------------- host helpers -- */
 /* Camera::set (src/rt/camera.cpp:31-59); host only. */

@@ -253,7 +253,7 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     RTX_REQUIRE(ctx && key, RTX_EINVAL, "rtx_tune_set: null argument");
     std::string k(key);
     if (k == "trace_variant") {
-        RTX_REQUIRE(value >= 0 && value <= 2, RTX_EINVAL, "rtx_tune_set: trace_variant must be 0, 1 or 2");
+        RTX_REQUIRE(value >= 0 && value <= 3, RTX_EINVAL, "rtx_tune_set: trace_variant must be 0, 1, 2 or 3");
         ctx->tune.trace_variant = value;
     } else if (k == "trace_block") {
         RTX_REQUIRE(value == 64 || value == 128 || value == 256, RTX_EINVAL,
@@ -262,6 +262,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "xcds") {
         RTX_REQUIRE(value >= 1 && value <= 64, RTX_EINVAL, "rtx_tune_set: xcds must be in [1, 64]");
         ctx->tune.xcds = value;
+    } else if (k == "refill") {
+        RTX_REQUIRE(value >= 1 && value <= 64, RTX_EINVAL, "rtx_tune_set: refill must be in [1, 64]");
+        ctx->tune.refill = value;
     } else if (k == "coop_leaf") {
         RTX_REQUIRE(value >= 0 && value <= (1 << 20), RTX_EINVAL, "rtx_tune_set: coop_leaf must be in [0, 2^20]");
         ctx->tune.coop_leaf = value;
@@ -279,6 +282,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "trace_block") return ctx->tune.trace_block;
     if (k == "xcds") return ctx->tune.xcds;
     if (k == "coop_leaf") return ctx->tune.coop_leaf;
+    if (k == "refill") return ctx->tune.refill;
     return -1;
 }
 

@@ -16,7 +16,50 @@ namespace {
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (threadIdx.x < (unsigned)n) p[threadIdx.x] = 0ull;
 }
+// Device copy of up to 5 u32 arrays (the render copy of a scene), grid-stride.
+struct CopySegs {
+    uint32_t* dst[5];
+    const uint32_t* src[5];
+    uint64_t n[5];
+    int count;
+};
+__global__ __launch_bounds__(256) void k_copy_segs(CopySegs c) {
+    for (int k = 0; k < c.count; ++k) {
+        const uint64_t n = c.n[k];
+        uint32_t* __restrict__ d = c.dst[k];
+        const uint32_t* __restrict__ sp = c.src[k];
+        if ((n & 3) == 0) {  // 16-byte moves (every buffer is 256-byte aligned)
+            uint4* d4 = reinterpret_cast<uint4*>(d);
+            const uint4* s4 = reinterpret_cast<const uint4*>(sp);
+            for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n / 4; i += (uint64_t)gridDim.x * 256) d4[i] = s4[i];
+        } else {
+            for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) d[i] = sp[i];
+        }
+    }
+}
 }  // namespace
+
+// Makes the render copy of everything the render stages read, on `st`.
+static int32_t refresh_render_copy(rtx_scene* s, hipStream_t st) {
+    int32_t e;
+    DevBuf* src[5] = {&s->node, &s->delta, &s->tri, &s->nrm, &s->matmap};
+    DevBuf* dst[5] = {&s->rc_node, &s->rc_delta, &s->rc_tri, &s->rc_nrm, &s->rc_matmap};
+    const uint64_t bytes[5] = {s->node.bytes, s->delta.bytes, (uint64_t)s->T * 48, (uint64_t)s->V * 16,
+                               (uint64_t)s->T * 4};
+    CopySegs c{};
+    for (int k = 0; k < 5; ++k) {
+        RTX_REQUIRE(src[k]->p && src[k]->bytes >= bytes[k], RTX_ESTATE, "render copy: scene not built");
+        if ((e = dst[k]->reserve(s->ctx, bytes[k]))) return e;
+        c.dst[k] = dst[k]->as<uint32_t>();
+        c.src[k] = src[k]->as<uint32_t>();
+        c.n[k] = bytes[k] / 4;
+    }
+    c.count = 5;
+    ProfScope ps(s->ctx, st, "render_copy");
+    hipLaunchKernelGGL(k_copy_segs, dim3(s->ctx->cus * 4), dim3(256), 0, st, c);
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
+}
 
 int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_renderer** out) {
     RTX_REQUIRE(ctx && p && out, RTX_EINVAL, "rtx_renderer_create: null argument");
@@ -65,6 +108,16 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
         rtx_set_error("rtx_renderer_create: event creation failed");
         e = RTX_EDEVICE;
     }
+    if (!e && hipStreamCreateWithFlags(&r->bstream, hipStreamNonBlocking) != hipSuccess) {
+        r->bstream = nullptr;
+        rtx_set_error("rtx_renderer_create: build stream creation failed");
+        e = RTX_EDEVICE;
+    }
+    if (!e && (hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&r->ev_built, hipEventDisableTiming) != hipSuccess)) {
+        rtx_set_error("rtx_renderer_create: event creation failed");
+        e = RTX_EDEVICE;
+    }
     if (e) {
         rtx_renderer_destroy(r);
         return e;
@@ -76,6 +129,7 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
 int32_t rtx_renderer_destroy(rtx_renderer* r) {
     RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_destroy: null");
     if (r->side) (void)hipStreamSynchronize(r->side);
+    if (r->bstream) (void)hipStreamSynchronize(r->bstream);
     for (auto x : r->rays) rtx_rays_destroy(x);
     for (auto x : r->hits) rtx_hits_destroy(x);
     if (r->fb) rtx_fb_destroy(r->fb);
@@ -84,12 +138,22 @@ int32_t rtx_renderer_destroy(rtx_renderer* r) {
     for (auto ev : r->ev) (void)hipEventDestroy(ev);
     if (r->ev_join) (void)hipEventDestroy(r->ev_join);
     if (r->side) (void)hipStreamDestroy(r->side);
+    if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
+    if (r->ev_built) (void)hipEventDestroy(r->ev_built);
+    if (r->bstream) (void)hipStreamDestroy(r->bstream);
     r->counts.release();
     delete r;
     return RTX_OK;
 }
 
 rtx_fb* rtx_renderer_fb(rtx_renderer* r) { return r ? r->fb : nullptr; }
+
+int32_t rtx_renderer_set_pipelined(rtx_renderer* r, int32_t enable) {
+    RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_set_pipelined: null renderer");
+    r->pipelined = enable != 0;
+    r->copy_valid = false;
+    return RTX_OK;
+}
 
 int32_t rtx_renderer_set_animation(rtx_renderer* r, int32_t enable, float t0, float dt) {
     RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_set_animation: null renderer");
@@ -172,14 +236,19 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
 // with independent work).  Results are identical to the reference sequence:
 // shadow rays only set flag bits 8+ that ray generation does not read, and
 // shading is an order-free integer sum.
-int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
-    RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame_async: null argument");
-    hipStream_t st = r->ctx->pick(stream);
+// Render stages of the read-back-free frame (after set-up): counts stay on
+// the device.  Bounce b's rays depend only on bounce b-1's hits, and its
+// shadow rays and shading on nothing later, so after bounce b is traced the
+// main stream goes on to generate and trace bounce b+1 while the side stream
+// traces bounce b's shadow rays and shades it (a latency-bound small trace
+// then shares the GPU with independent work).  Results are identical to the
+// reference sequence: shadow rays only set flag bits 8+ that ray generation
+// does not read, and shading is an order-free integer sum.
+static int32_t render_stages(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, hipStream_t st) {
     hipStream_t sd = r->side;
     const uint64_t P = (uint64_t)r->p.width * r->p.height;
     unsigned long long* cnt = r->counts.as<unsigned long long>();
     int32_t e;
-    if ((e = set_up_frame(r, s, rebuild, st))) return e;
     ProfScope whole(r->ctx, st, "FRAME_RENDER");
     const int B = r->p.max_bounces;
     if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
@@ -201,10 +270,56 @@ int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* 
     }
     RTX_CHECK_HIP(hipEventRecord(r->ev_join, sd));
     RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_join, 0));
-    if ((e = rtx_fb_resolve(r->fb, st))) return e;
+    return rtx_fb_resolve(r->fb, st);
+}
+
+// Pipelined frame (the reference's RendererT, src/rt/renderer-threaded.cpp:
+// 452-506: the next scene's accelerator is built on a second queue while the
+// last one renders).  The build stream animates and rebuilds the scene for
+// the next frame while the main stream renders this frame from the render
+// copy made at the end of the previous frame; after both, the copy is
+// refreshed.  The image of a frame is therefore the scene as built one frame
+// earlier.  The first pipelined frame builds and copies first.
+static int32_t render_frame_pipelined(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild,
+                                      hipStream_t st) {
+    int32_t e;
+    const rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
+    if (!r->copy_valid || !s->built || s != r->copy_of) {
+        if (r->animate && (e = rtx_scene_animate(s, r->anim_t0, r->anim_dt, st))) return e;
+        if ((e = rtx_lbvh_build(s, &bp, st))) return e;
+        if ((e = refresh_render_copy(s, st))) return e;
+        r->copy_valid = true;
+        r->copy_of = s;
+    }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, r->counts.as<unsigned long long>(), 8);
+    if ((e = rtx_fb_clear(r->fb, st))) return e;
+    RTX_CHECK_HIP(hipEventRecord(r->ev_fork, st));
+    RTX_CHECK_HIP(hipStreamWaitEvent(r->bstream, r->ev_fork, 0));
+    if (r->animate && (e = rtx_scene_animate(s, r->anim_t0, r->anim_dt, r->bstream))) return e;
+    if (rebuild && (e = rtx_lbvh_build(s, &bp, r->bstream))) return e;
+    RTX_CHECK_HIP(hipEventRecord(r->ev_built, r->bstream));
+    s->use_copy = true;
+    e = render_stages(r, s, cam, st);
+    s->use_copy = false;
+    if (e) return e;
+    RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_built, 0));
+    return refresh_render_copy(s, st);
+}
+
+int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {
+    RTX_REQUIRE(r && s && cam, RTX_EINVAL, "rtx_render_frame_async: null argument");
+    hipStream_t st = r->ctx->pick(stream);
+    int32_t e;
+    if (r->pipelined) {
+        e = render_frame_pipelined(r, s, cam, rebuild, st);
+    } else {
+        if ((e = set_up_frame(r, s, rebuild, st))) return e;
+        e = render_stages(r, s, cam, st);
+    }
+    if (e) return e;
     r->last_async = true;
     r->last = rtx_frame_stats{};
-    r->last.primary_rays = P;
+    r->last.primary_rays = (uint64_t)r->p.width * r->p.height;
     return RTX_OK;
 }
 

@@ -63,10 +63,11 @@ struct KernelTiming {
 
 // Tuning knobs (set through rtx_tune_set; picked by the autotuner).
 struct rtx_tuning {
-    int64_t trace_variant = 1;  // 0 single node, 1 single node + early delta load, 2 sibling pairs
+    int64_t trace_variant = 1;  // 0 single node, 1 + early delta load, 2 sibling pairs, 3 persistent + ray queue
     int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
+    int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
 };
 
 struct rtx_ctx {
@@ -124,7 +125,30 @@ struct rtx_scene {
     DevBuf tickets;               // tile tickets for every ordered kernel
     DevBuf hier_status;           // look-back words, per level x tile
     DevBuf fat;                   // node ids of the leaves bounded by one wave each (kept for refit)
+    // Render copy of what the render stages read (pipelined frames: the next
+    // frame's build writes the buffers above while this frame renders from
+    // the copy).  use_copy selects it for the launches being enqueued.
+    DevBuf rc_node, rc_delta, rc_tri, rc_nrm, rc_matmap;
+    bool use_copy = false;
 };
+
+// Buffers the render stages (trace, shadow, secondary generation, shading)
+// read from a scene: the build's own, or the render copy.
+struct SceneView {
+    const float4* node;
+    const float4* delta;
+    const float4* tri;
+    const float4* nrm;
+    const int32_t* matmap;
+};
+inline SceneView scene_view(const rtx_scene* s) {
+    if (s->use_copy)
+        return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(s->rc_node.p) + 32),
+                         s->rc_delta.as<float4>(), s->rc_tri.as<float4>(), s->rc_nrm.as<float4>(),
+                         s->rc_matmap.as<int32_t>()};
+    return SceneView{s->nodes(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),
+                     s->matmap.as<int32_t>()};
+}
 
 struct rtx_rays {
     rtx_ctx* ctx = nullptr;
@@ -170,6 +194,13 @@ struct rtx_renderer {
     hipStream_t side = nullptr;
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_join = nullptr;
+    // Pipelined frames: the next frame's animation and build run on bstream
+    // while this frame renders from the scene's render copy.
+    bool pipelined = false;
+    bool copy_valid = false;
+    const rtx_scene* copy_of = nullptr;
+    hipStream_t bstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_built = nullptr;
     hipGraph_t graph = nullptr;          // captured read-back-free frame
     hipGraphExec_t graph_exec = nullptr;
 };

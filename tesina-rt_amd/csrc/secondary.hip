@@ -160,7 +160,7 @@ int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rt
         hipLaunchKernelGGL(k_secondary, dim3(grid), dim3(SB), 0, st, in->dir4.as<float4>(), in->pixel.as<int32_t>(),
                            in->contrib.as<float>(), hits->flags.as<uint32_t>(), hits->id.as<int32_t>(),
                            hits->n4.as<float4>(), hits->p4.as<float4>(), s->mats.as<rtx_material>(),
-                           s->matmap.as<int32_t>(), in->count.as<uint32_t>(), count_in, out->ori4.as<float4>(),
+                           scene_view(s).matmap, in->count.as<uint32_t>(), count_in, out->ori4.as<float4>(),
                            out->dir4.as<float4>(), out->pixel.as<int32_t>(), out->contrib.as<float>(),
                            out->count.as<uint32_t>(), in->scan_status.as<uint32_t>(), (uint32_t)out->cap);
     }

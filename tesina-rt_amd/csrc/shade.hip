@@ -157,7 +157,7 @@ int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint
     hipLaunchKernelGGL(k_shade, dim3((uint32_t)g), dim3(256), 0, st, rays->dir4.as<float4>(), rays->pixel.as<int32_t>(),
                        rays->contrib.as<float>(), hits->flags.as<uint32_t>(), hits->id.as<int32_t>(),
                        hits->n4.as<float4>(), hits->p4.as<float4>(), s->mats.as<rtx_material>(),
-                       s->matmap.as<int32_t>(), s->lights.as<rtx_lights>(), rays->count.as<uint32_t>(), count,
+                       scene_view(s).matmap, s->lights.as<rtx_lights>(), rays->count.as<uint32_t>(), count,
                        fb->accum.as<uint32_t>());
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;

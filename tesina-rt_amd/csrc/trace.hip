@@ -552,6 +552,48 @@ __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const f
 }
 
 // -------------------------------------------------------------- trace ----
+__device__ __forceinline__ void write_miss_record(uint64_t rr, uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                                  int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                                  float4* __restrict__ n4, float4* __restrict__ p4,
+                                                  float2* __restrict__ uv2) {
+    flags[rr] = 0u;
+    tout[rr] = 0.f;
+    idout[rr] = -1;
+    bary[rr] = make_float2(0.f, 0.f);
+    n4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    p4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    uv2[rr] = make_float2(0.f, 0.f);
+}
+
+// complete_trace_info (src/kernel/trace-bvh.cl:190-245) for a ray with a hit.
+__device__ __forceinline__ void write_hit_record(const float4* __restrict__ tri, const float4* __restrict__ nrm,
+                                                 const float2* __restrict__ uvs, const RayC& rc, const Closest& best,
+                                                 uint64_t rr, uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                                 int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                                 float4* __restrict__ n4, float4* __restrict__ p4,
+                                                 float2* __restrict__ uv2) {
+    const v3 hp = rc.o + rc.d * best.t;
+    const uint32_t i0 = __float_as_uint(tri[3 * (size_t)best.id].w);
+    const uint32_t i1 = __float_as_uint(tri[3 * (size_t)best.id + 1].w);
+    const uint32_t i2 = __float_as_uint(tri[3 * (size_t)best.id + 2].w);
+    const float w = 1.f - (best.u + best.v);
+    const v3 n0 = xyz(nrm[i0]), n1 = xyz(nrm[i1]), n2 = xyz(nrm[i2]);
+    v3 nn = normalize(w * n0 + best.v * n2 + best.u * n1);
+    uint32_t fl = 1u;
+    if (dot(nn, rc.d) > 0.f) {
+        fl |= 2u;
+        nn = -nn;
+    }
+    const float2 t0 = uvs[i0], t1 = uvs[i1], t2 = uvs[i2];
+    flags[rr] = fl;
+    tout[rr] = best.t;
+    idout[rr] = best.id;
+    bary[rr] = make_float2(best.u, best.v);
+    n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
+    p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
+    uv2[rr] = make_float2(w * t0.x + best.v * t2.x + best.u * t1.x, w * t0.y + best.v * t2.y + best.u * t1.y);
+}
+
 // Tracer::trace -> trace_single + complete_trace_info.  Rays with a hit are
 // also appended (wave-aggregated) to the bundle's hit list, which the shadow
 // stage walks densely.
@@ -591,37 +633,11 @@ __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, c
         }
         if (!active) continue;
         if (!hit) {
-            flags[rr] = 0u;
-            tout[rr] = 0.f;
-            idout[rr] = -1;
-            bary[rr] = make_float2(0.f, 0.f);
-            n4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
-            p4[rr] = make_float4(0.f, 0.f, 0.f, 0.f);
-            uv2[rr] = make_float2(0.f, 0.f);
+            write_miss_record(rr, flags, tout, idout, bary, n4, p4, uv2);
             continue;
         }
         hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
-        // complete_trace_info (src/kernel/trace-bvh.cl:190-245)
-        const v3 hp = rc.o + rc.d * best.t;
-        const uint32_t i0 = __float_as_uint(tri[3 * (size_t)best.id].w);
-        const uint32_t i1 = __float_as_uint(tri[3 * (size_t)best.id + 1].w);
-        const uint32_t i2 = __float_as_uint(tri[3 * (size_t)best.id + 2].w);
-        const float w = 1.f - (best.u + best.v);
-        const v3 n0 = xyz(nrm[i0]), n1 = xyz(nrm[i1]), n2 = xyz(nrm[i2]);
-        v3 nn = normalize(w * n0 + best.v * n2 + best.u * n1);
-        uint32_t fl = 1u;
-        if (dot(nn, rc.d) > 0.f) {
-            fl |= 2u;
-            nn = -nn;
-        }
-        const float2 t0 = uvs[i0], t1 = uvs[i1], t2 = uvs[i2];
-        flags[rr] = fl;
-        tout[rr] = best.t;
-        idout[rr] = best.id;
-        bary[rr] = make_float2(best.u, best.v);
-        n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
-        p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
-        uv2[rr] = make_float2(w * t0.x + best.v * t2.x + best.u * t1.x, w * t0.y + best.v * t2.y + best.u * t1.y);
+        write_hit_record(tri, nrm, uvs, rc, best, rr, flags, tout, idout, bary, n4, p4, uv2);
     }
 }
 
@@ -645,6 +661,10 @@ __global__ __launch_bounds__(256) void k_hit_list(const uint32_t* __restrict__ f
 }
 
 __global__ void k_zero1(uint32_t* p) { *p = 0u; }
+__global__ void k_zero2(uint32_t* p) {
+    p[0] = 0u;
+    p[1] = 0u;
+}
 
 // ------------------------------------------------------------- shadow ----
 // Hit lists of up to RTX_SEG_MAX bundles traced by one launch (the shadow
@@ -741,6 +761,275 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
     }
 }
 
+// ------------------------------------------- persistent traversal (3) --
+// Variant 3: persistent waves with a ray queue.  A lane whose ray is done
+// does not wait for the rest of its wave: once `refill` lanes of the wave are
+// idle, their results are written and the wave takes that many new rays
+// from the launch's queue counter with one atomic.  The lanes of a wave then
+// stay busy until the queue drains, so a wave costs the sum of its rays'
+// steps rather than 64 times the longest ray.  Each ray is still traversed
+// alone by the same single-node steps (visiting order, box tests, leaf
+// tests, cooperative fat leaves), so its result is the other variants'.
+
+// One step of the single-node traversal for a lane with a ray: returns true
+// when the ray's traversal is complete.  A fat leaf reached by this step is
+// flagged in (la, lcnt) for the wave's cooperative test.
+template <class Stack>
+__device__ __forceinline__ bool step_closest(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                             const float4* __restrict__ tri, const RayC& r, Closest& best,
+                                             Stack& stk, uint32_t& level, uint32_t& curr, uint32_t coop,
+                                             uint32_t& la, uint32_t& lcnt) {
+    const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+    float4 dl4 = delta[curr];
+    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // issued with the node load
+    const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+    if (slab(f0, f1, r, best.t)) {
+        if (cnt >= coop) {
+            la = a;
+            lcnt = cnt;
+        } else if (cnt) {
+            leaf_closest(tri, a, cnt, r, best);
+        } else {
+            const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+            level++;
+            curr = nl ? a : a + 1;
+            return false;
+        }
+    }
+    if (level == 0) return true;
+    curr = stk.get(--level);
+    return false;
+}
+
+// Any-hit step: returns 1 when a hit was found, 2 when the traversal ended
+// without one, 0 to continue.
+template <class Stack>
+__device__ __forceinline__ int step_any(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                        const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk,
+                                        uint32_t& level, uint32_t& curr, uint32_t coop, uint32_t& la,
+                                        uint32_t& lcnt) {
+    const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+    float4 dl4 = delta[curr];
+    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));
+    const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+    if (slab(f0, f1, r, tmax)) {
+        if (cnt >= coop) {
+            la = a;
+            lcnt = cnt;
+        } else if (cnt) {
+            if (leaf_any(tri, a, cnt, r, tmax)) return 1;
+        } else {
+            const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+            level++;
+            curr = nl ? a : a + 1;
+            return 0;
+        }
+    }
+    if (level == 0) return 2;
+    curr = stk.get(--level);
+    return 0;
+}
+
+// Per-wave view of the launch's item queue.  The wave takes QUEUE_CHUNK
+// consecutive items per atomic (consecutive rays stay together, and the
+// single counter sees one atomic per chunk) and hands them to its idle lanes
+// from a wave-uniform cursor.
+constexpr uint32_t QUEUE_CHUNK = 64;
+struct WaveQueue {
+    uint64_t cur = 0, end = 0;
+    bool last = false;  // no further chunk exists
+    __device__ __forceinline__ bool exhausted() const { return last && cur >= end; }
+};
+
+// Item for this lane if it is idle and the queue still has one, else ~0.
+__device__ __forceinline__ uint64_t wq_take(uint32_t* queue, WaveQueue& q, uint64_t idle, uint64_t n) {
+    if (q.cur >= q.end && !q.last) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(queue, QUEUE_CHUNK);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+        q.cur = base;
+        q.end = (uint64_t)base + QUEUE_CHUNK < n ? (uint64_t)base + QUEUE_CHUNK : n;
+        if (q.end < q.cur) q.end = q.cur;
+        if ((uint64_t)base + QUEUE_CHUNK >= n) q.last = true;
+    }
+    const uint64_t avail = q.end - q.cur;
+    const uint64_t k = (uint64_t)__builtin_popcountll(idle);
+    const uint64_t give = k < avail ? k : avail;
+    const uint64_t r = lanes_below(idle);
+    const uint64_t item = ((idle >> lane_id()) & 1ull) && r < give ? q.cur + r : ~0ull;
+    q.cur += give;
+    return item;
+}
+
+// Hit-list appends batched per wave in LDS: one atomic per flush.
+constexpr uint32_t HITBUF = 128;
+__device__ __forceinline__ void hitbuf_flush(uint32_t* buf, uint32_t& cnt, uint32_t* __restrict__ hit_list,
+                                             uint32_t* hit_count) {
+    if (cnt == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(hit_count, cnt);
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+    for (uint32_t i = lane_id(); i < cnt; i += RTX_WAVE) hit_list[base + i] = buf[i];
+    cnt = 0;
+}
+
+template <int TB>
+__global__ __launch_bounds__(TB) void k_trace_p(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                                const float4* __restrict__ tri, const float4* __restrict__ nrm,
+                                                const float2* __restrict__ uvs, const float4* __restrict__ ori4,
+                                                const float4* __restrict__ dir4, const uint32_t* __restrict__ dcount,
+                                                uint64_t hcount, uint32_t* __restrict__ flags,
+                                                float* __restrict__ tout, int32_t* __restrict__ idout,
+                                                float2* __restrict__ bary, float4* __restrict__ n4,
+                                                float4* __restrict__ p4, float2* __restrict__ uv2,
+                                                uint32_t* __restrict__ hit_list, uint32_t* __restrict__ hit_count,
+                                                uint32_t coop, uint32_t refill) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_hits[TB / RTX_WAVE][HITBUF];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    uint32_t* hbuf = s_hits[threadIdx.x / RTX_WAVE];
+    uint32_t hcnt = 0;  // wave-uniform fill of hbuf
+    const uint64_t n = ray_count(dcount, hcount);
+    uint32_t* queue = hit_count + 1;  // zeroed with the hit count before the launch
+    WaveQueue q;
+    bool have = false, done = false;
+    uint64_t rr = 0;
+    uint32_t level = 0, curr = 0;
+    Closest best{0.f, 0.f, 0.f, -1};
+    RayC rc{};
+    for (;;) {
+        const uint64_t idle = ballot(!have);
+        if (idle == ~0ull || (!q.exhausted() && (uint32_t)__builtin_popcountll(idle) >= refill)) {
+            // results of the lanes that finished since the last refill
+            const bool hit = done && best.id >= 0;
+            const uint64_t hb = ballot(hit);
+            if (hb) {
+                if (hcnt + RTX_WAVE > HITBUF) hitbuf_flush(hbuf, hcnt, hit_list, hit_count);
+                if (hit) {
+                    hbuf[hcnt + lanes_below(hb)] = (uint32_t)rr;
+                    write_hit_record(tri, nrm, uvs, rc, best, rr, flags, tout, idout, bary, n4, p4, uv2);
+                }
+                hcnt += (uint32_t)__builtin_popcountll(hb);
+            }
+            if (done && !hit) write_miss_record(rr, flags, tout, idout, bary, n4, p4, uv2);
+            done = false;
+            const uint64_t my = wq_take(queue, q, idle, n);
+            if (!have && my < n) {
+                rr = my;
+                const float4 o = ori4[rr], dd = dir4[rr];
+                rc = make_ray(xyz(o), xyz(dd), o.w);
+                best = Closest{dd.w, 0.f, 0.f, -1};
+                level = 0;
+                curr = 0;
+                have = true;
+            }
+            if (q.exhausted() && !ballot(have)) break;
+        }
+        uint32_t la = 0, lcnt = 0;
+        if (have && step_closest(node, delta, tri, rc, best, stk, level, curr, coop, la, lcnt)) {
+            have = false;
+            done = true;
+        }
+        coop_closest(tri, la, lcnt, rc, best, ~0ull);  // whole wave
+    }
+    hitbuf_flush(hbuf, hcnt, hit_list, hit_count);
+}
+
+template <int TB>
+__global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                                 const float4* __restrict__ tri,
+                                                 const rtx_lights* __restrict__ lights, ShadowSegs segs,
+                                                 unsigned long long* shadow_count, uint32_t* queue, uint32_t coop,
+                                                 uint32_t refill) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_end[RTX_SEG_MAX];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const int nl = lights->count;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        uint64_t acc = 0;
+        for (int k = 0; k < segs.n; ++k) {
+            acc += (uint64_t)(*segs.s[k].count) * (uint64_t)nl;
+            s_end[k] = acc;
+        }
+    }
+    __syncthreads();
+    const uint64_t items = segs.n ? s_end[segs.n - 1] : 0;
+    uint32_t traced = 0;
+    bool have = false;
+    WaveQueue q;
+    uint32_t rr = 0, level = 0, curr = 0;
+    int l = 0;
+    uint32_t* fl = nullptr;
+    RayC rc{};
+    float tmax = 1e37f;
+    for (;;) {
+        const uint64_t idle = ballot(!have);
+        if (idle == ~0ull || (!q.exhausted() && (uint32_t)__builtin_popcountll(idle) >= refill)) {
+            {
+                const uint64_t it = wq_take(queue, q, idle, items);
+                if (!have && it < items) {
+                    int sg = 0;
+                    while (it >= s_end[sg]) ++sg;
+                    const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
+                    const ShadowSeg& S = segs.s[sg];
+                    rr = S.list[local / nl];
+                    l = (int)(local % nl);
+                    fl = S.flags;
+                    const rtx_light& L = lights->l[l];
+                    const v3 p = xyz(S.p4[rr]);
+                    v3 dir = mk3(0.f, 0.f, 0.f);
+                    bool trace = true, shadowed = false;
+                    tmax = 1e37f;
+                    if (L.type == RTX_LIGHT_DIR) {
+                        dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+                    } else if (L.type == RTX_LIGHT_SPOT) {
+                        dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+                        if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
+                            shadowed = true;
+                            trace = false;
+                        }
+                    } else if (L.type == RTX_LIGHT_POINT) {
+                        const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+                        tmax = __builtin_sqrtf(dot(toL, toL));
+                        dir = normalize(toL);
+                    } else {
+                        shadowed = true;
+                        trace = false;
+                    }
+                    if (shadowed) atomicOr(&fl[rr], 1u << (8 + l));
+                    if (trace) {
+                        ++traced;
+                        rc = make_ray(p, dir, 0.01f);
+                        level = 0;
+                        curr = 0;
+                        have = true;
+                    }
+                }
+            }
+            if (q.exhausted() && !ballot(have)) break;
+        }
+        uint32_t la = 0, lcnt = 0;
+        if (have) {
+            const int st = step_any(node, delta, tri, rc, tmax, stk, level, curr, coop, la, lcnt);
+            if (st == 1) atomicOr(&fl[rr], 1u << (8 + l));
+            if (st != 0) have = false;
+        }
+        if (coop_any(tri, la, lcnt, rc, tmax, ~0ull)) {  // whole wave; true on owners with a hit
+            atomicOr(&fl[rr], 1u << (8 + l));
+            have = false;
+        }
+    }
+    if (shadow_count) {
+        if (traced) atomicAdd(&s_cnt, traced);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_cnt) atomicAdd(shadow_count, (unsigned long long)s_cnt);
+    }
+}
+
 // Launch macro M(variant, block) with the tuned variant and block size.
 #define RTX_DISPATCH_TRAVERSAL(M, var, tb)     \
     do {                                       \
@@ -758,6 +1047,16 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
             else M(2, 128);                    \
         }                                      \
     } while (0)
+
+// Persistent traversal grid: enough groups to fill every CU at 20 waves
+// (the LDS stack allows 5 per SIMD), fewer when the items need fewer.
+uint32_t persist_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
+    uint64_t n = count == RTX_DEVICE_COUNT ? cap : count;
+    uint64_t g = (n + TB - 1) / TB;
+    const uint64_t gmax = (uint64_t)c->cus * (1280 / TB);
+    if (g > gmax) g = gmax;
+    return (uint32_t)(g ? g : 1);
+}
 
 // Leaf size from which leaves are tested wave-cooperatively (0 = never).
 uint32_t coop_threshold(const rtx_ctx* c) {
@@ -947,12 +1246,34 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
                 "rtx_trace: hit bundle smaller than ray bundle");
     hipStream_t st = s->ctx->pick(stream);
     if (count == 0) return RTX_OK;
-    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());  // hit count, queue
     ProfScope ps(s->ctx, st, "trace");
+    const SceneView sv = scene_view(s);
     const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+    if (var == 3) {
+#define RTX_TRACE_P(B)                                                                                              \
+    hipLaunchKernelGGL((k_trace_p<B>), dim3(persist_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,            \
+                       sv.node, sv.delta, sv.tri, sv.nrm,                                                          \
+                       s->uv.as<float2>(), rays->ori4.as<float4>(), rays->dir4.as<float4>(),                        \
+                       rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
+                       hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
+                       hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>(),           \
+                       coop_threshold(s->ctx), (uint32_t)s->ctx->tune.refill)
+        if (tb == 64)
+            RTX_TRACE_P(64);
+        else if (tb == 256)
+            RTX_TRACE_P(256);
+        else
+            RTX_TRACE_P(128);
+#undef RTX_TRACE_P
+        RTX_CHECK_HIP(hipGetLastError());
+        hits->list_valid = true;
+        hits->list_for = count;
+        return RTX_OK;
+    }
 #define RTX_TRACE_LAUNCH(V, B)                                                                                       \
     hipLaunchKernelGGL((k_trace<V, B>), dim3(trace_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,              \
-                       s->nodes(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),       \
+                       sv.node, sv.delta, sv.tri, sv.nrm,                                                 \
                        s->uv.as<float2>(), rays->ori4.as<float4>(), rays->dir4.as<float4>(),                        \
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
@@ -1000,11 +1321,31 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     const int nl = s->host_lights.count;
     if (nl <= 0 || max_items == 0) return RTX_OK;
     ProfScope ps(s->ctx, st, "trace_shadow");
+    const SceneView sv = scene_view(s);
     uint64_t n = max_items * (uint64_t)nl;
     const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+    if (var == 3) {
+        // queue counter: word 2 of the first segment's hit-list counters
+        uint32_t* queue = hits[0]->list_count.as<uint32_t>() + 2;
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, queue);
+#define RTX_SHADOW_P(B)                                                                                          \
+    hipLaunchKernelGGL((k_shadow_p<B>), dim3(persist_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,           \
+                       sv.delta, sv.tri, s->lights.as<rtx_lights>(), segs,                                     \
+                       (unsigned long long*)dev_shadow_count, queue, coop_threshold(s->ctx),                   \
+                       (uint32_t)s->ctx->tune.refill)
+        if (tb == 64)
+            RTX_SHADOW_P(64);
+        else if (tb == 256)
+            RTX_SHADOW_P(256);
+        else
+            RTX_SHADOW_P(128);
+#undef RTX_SHADOW_P
+        RTX_CHECK_HIP(hipGetLastError());
+        return RTX_OK;
+    }
 #define RTX_SHADOW_LAUNCH(V, B)                                                                                 \
-    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, s->nodes(), \
-                       s->delta.as<float4>(), s->tri.as<float4>(), s->lights.as<rtx_lights>(), segs,           \
+    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,    \
+                       sv.delta, sv.tri, s->lights.as<rtx_lights>(), segs,                                     \
                        (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
                        coop_threshold(s->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_LAUNCH, var, tb);

@@ -95,6 +95,7 @@ SYMBOLS = [
     "rtx_shade",
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
     "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay", "rtx_renderer_set_animation",
+    "rtx_renderer_set_pipelined",
     "rtx_camera_set",
 ]
 
@@ -139,6 +140,7 @@ def lib() -> C.CDLL:
         "rtx_scene_animate": (I32, [P, F, F, P]),
         "rtx_read_vertices": (I32, [P, P, P, U32]),
         "rtx_renderer_set_animation": (I32, [P, I32, F, F]),
+        "rtx_renderer_set_pipelined": (I32, [P, I32]),
         "rtx_lbvh_build": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
@@ -555,6 +557,9 @@ class Renderer:
         """Capture one read-back-free frame into a hipGraph (render once eagerly first)."""
         _check(lib().rtx_renderer_capture(self.h, scene.h, C.byref(cam), int(rebuild), stream),
                "rtx_renderer_capture")
+
+    def set_pipelined(self, enable: bool):
+        _check(lib().rtx_renderer_set_pipelined(self.h, int(bool(enable))), "rtx_renderer_set_pipelined")
 
     def set_animation(self, enable: bool, t0: float = 0.0, dt: float = 0.0):
         _check(lib().rtx_renderer_set_animation(self.h, int(bool(enable)), float(t0), float(dt)),

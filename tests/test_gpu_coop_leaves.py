@@ -10,9 +10,9 @@ triangle.  The scene has
     every hit on it is an exact tie between two triangles of one leaf (equal
     centroids give equal codes, and equal codes are never split apart);
   * a refractive sphere and a floor, for secondary and shadow rays.
-Every threshold (0 = never, 1 = every leaf) and both single-node traversal
-forms must give the oracle's hits, shadow flags and secondary rays bit for
-bit.
+Every threshold (0 = never, 1 = every leaf) and the single-node traversal
+forms (plain and persistent) must give the oracle's hits, shadow flags and
+secondary rays bit for bit.
 """
 import numpy as np
 import pytest
@@ -103,7 +103,7 @@ CAMS = {
 
 @pytest.mark.parametrize("cam_name", list(CAMS))
 @pytest.mark.parametrize("coop", [0, 1, 2, 8, 64])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 3])
 def test_coop_leaf_thresholds_identical(ctx, poles, cam_name, coop, variant):
     rtx = _rtx()
     s, b = poles["s"], poles["b"]

@@ -323,7 +323,7 @@ def test_frame_rebuild_c1(ctx, c1):
     s.destroy()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("block", [64, 128, 256])
 def test_traversal_variants_identical(ctx, c1, variant, block):
     # every tuned traversal form reproduces the oracle's hits and shadow flags
@@ -394,3 +394,41 @@ def test_graph_replay_matches_oracle(ctx, c1):
                                                      O.camera_from(cam), w, h, 2)[0])
     R.destroy()
     s.destroy()
+
+
+@pytest.mark.parametrize("variant,refill", [(3, 1), (3, 16), (3, 64), (1, 16)])
+def test_frame_and_graph_per_variant(ctx, variant, refill):
+    # whole frames (async and captured) under each traversal form, including
+    # the persistent ray-queue form at its extreme refill thresholds
+    rtx = _rtx()
+    from rtx import scenes
+    sd = scenes.config_c1()
+    s = scenes.load(rtx.Scene(ctx), sd)
+    w, h = 160, 120
+    cam = rtx.camera_set(sd.camera["pos"], sd.camera["look_at"], sd.camera["up"], sd.camera["fov"], w / h)
+    old = (ctx.tuned("trace_variant"), ctx.tuned("refill"))
+    ctx.tune("trace_variant", variant)
+    ctx.tune("refill", refill)
+    try:
+        R = rtx.Renderer(ctx, w, h, 2)
+        R.frame_async(s, cam, rebuild=True)
+        ctx.sync()
+        b = O.Build(sd.pos, sd.idx, sd.matmap, 30, 1)
+        oacc, st, orays = O.render(b, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights), O.camera_from(cam),
+                                   w, h, 2)
+        assert np.array_equal(R.fb.accum(), oacc), "async frame differs"
+        gst = R.last_stats()
+        assert gst.primary_rays + gst.secondary_rays == orays and gst.shadow_rays == st["shadow"]
+        R.capture(s, cam, rebuild=True)
+        for k in range(2):
+            R.replay()
+            ctx.sync()
+            b = O.Build(sd.pos, b.idx.reshape(-1, 3), b.matmap, 30, 1)
+            oacc = O.render(b, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights), O.camera_from(cam), w, h,
+                            2)[0]
+            assert np.array_equal(R.fb.accum(), oacc), f"replay {k} differs"
+        R.destroy()
+    finally:
+        ctx.tune("trace_variant", old[0])
+        ctx.tune("refill", old[1])
+        s.destroy()
