@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=3)
     ap.add_argument("--cpu-frames", type=int, default=3, help="oracle frames timed for the CPU baseline (median)")
     ap.add_argument("--c3-frames", type=int, default=300, help="frames of the dynamic-scene leg (0 = skip)")
+    ap.add_argument("--c5-rates", type=str, default="50,100,200,500,1000,2000",
+                    help="arrival rates of the serving leg, per second ('' = skip)")
+    ap.add_argument("--c5-max-requests", type=int, default=600)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -261,6 +264,9 @@ def main():
         R = s = None
         out["c3_dynamic"] = bench_c3(ctx, args.c3_frames)
         log(f"C3: {out['c3_dynamic']['ms_per_frame']} ms/frame, {out['c3_dynamic']['mrays']} Mrays/s")
+    if args.c5_rates:
+        rates = [int(x) for x in args.c5_rates.split(",") if x]
+        out["c5_serving"] = bench_c5(ctx, rates, args.c5_max_requests)
     if not args.no_cpu:
         log(f"CPU oracle baseline (one core, {args.cpu_frames} full frames) ...")
         runs = [cpu_baseline(sd, cam) for _ in range(max(1, args.cpu_frames))]
@@ -329,6 +335,99 @@ def bench_c3(ctx, frames: int):
     R.destroy()
     s.destroy()
     return out
+
+
+def c5_scenes(seed: int = 0x5EED0005, n_scenes: int = 12, total_tris: int = 20_000_000):
+    """C5 scene set: T ~ logU[10k, 4M] per scene, rescaled so that the sum is
+    about 20M triangles; sphere sets (5000-triangle UV spheres) and
+    heightfields alternate."""
+    import numpy as np
+    from rtx import scenes
+    rng = scenes.Rng(seed)
+    t = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(4e6)))
+    t = t * (total_tris / t.sum())
+    out = []
+    for i, ti in enumerate(t):
+        if i % 2 == 0:
+            n_sph = max(2, int(round(ti / 5000)))
+            sd = scenes.config_c2(seed=seed + 16 * i, n_spheres=n_sph, res=(256, 256))
+        else:
+            n = max(8, int(round(np.sqrt(ti / 2.0))) + 1)
+            sd = scenes.config_c3(seed=seed + 16 * i, n=n, res=(256, 256))
+            sd.pos[:, 1] = scenes.displace_host(sd, 0.5)[:, 1]  # a static wave field
+        out.append(sd)
+    return out
+
+
+def bench_c5(ctx, rates, max_requests: int, workers: int = 4):
+    """C5 serving: 12 resident scenes (~20M triangles), Poisson arrivals per
+    rate, Zipf(1.1) scene popularity, resolution uniform over 256^2, 512^2 and
+    1280x720, depth uniform over 1..3, camera on a sphere of 2.5x the scene
+    radius looking at its centre.  Latency = arrival -> image complete on the
+    device.  The concurrent server and the plain per-request path see the
+    same request stream."""
+    import numpy as np
+    import rtx
+    from rtx import scenes
+    t0 = time.perf_counter()
+    sds = c5_scenes()
+    log(f"C5: generated {len(sds)} scenes, {sum(sd.T for sd in sds)} triangles in {time.perf_counter() - t0:.1f}s")
+    scs = []
+    for sd in sds:
+        sc = scenes.load(rtx.Scene(ctx), sd)
+        sc.build(30, 1)
+        scs.append(sc)
+    ctx.sync()
+    centres = [0.5 * (sd.pos.min(0) + sd.pos.max(0)) for sd in sds]
+    radii = [0.5 * float(np.linalg.norm(sd.pos.max(0) - sd.pos.min(0))) for sd in sds]
+    classes = [(256, 256), (512, 512), (1280, 720)]
+    zipf = 1.0 / np.arange(1, len(sds) + 1) ** 1.1
+    zipf /= zipf.sum()
+    res = {"workload": "C5 serving: 12 resident scenes, T ~ logU[10k, 4M] rescaled to ~20M triangles "
+                       "(sphere sets and heightfields), Poisson arrivals, Zipf(1.1) scenes, resolution "
+                       "U{256^2, 512^2, 1280x720}, depth U{1..3}, camera on a sphere of 2.5x the scene radius",
+           "triangles": [int(sd.T) for sd in sds], "workers": workers, "rates": list(rates), "server": [],
+           "plain": []}
+    for mode in ("server", "plain"):
+        srv = rtx.Server(ctx, classes, workers=workers, max_bounces=3, plain=(mode == "plain"))
+        ids = [srv.add_scene(sc) for sc in scs]
+        wb, sb = srv.memory()
+        res["worker_bytes" if mode == "server" else "plain_worker_bytes"] = wb
+        res["resident_scene_bytes"] = sb
+        for lam in rates:
+            rng = np.random.default_rng(int(lam) * 7 + 1)
+            n = int(min(max_requests, max(40, lam * 1.5)))
+            gaps = rng.exponential(1.0 / lam, n)
+            start = rtx.now_ns() + 5_000_000
+            arrivals = start + (np.cumsum(gaps) * 1e9).astype(np.int64)
+            reqs = []
+            for i in range(n):
+                k = int(rng.choice(len(sds), p=zipf))
+                w, h = classes[int(rng.integers(0, 3))]
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                pos = centres[k] + 2.5 * radii[k] * d
+                up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
+                cam = rtx.camera_set(pos, centres[k], up, np.pi / 4, w / h)
+                reqs.append(dict(scene=ids[k], width=w, height=h, bounces=int(rng.integers(1, 4)), camera=cam,
+                                 arrival_ns=int(arrivals[i]), tag=i))
+            srv.submit(reqs)
+            srv.drain(600_000)
+            done = srv.poll(len(reqs) + 16)
+            assert len(done) == n and all(c["status"] == 0 for c in done), "C5 request failed"
+            lat = np.array([(c["done_ns"] - c["arrival_ns"]) / 1e6 for c in done])
+            rays = sum(c["rays"] for c in done)
+            span = (max(c["done_ns"] for c in done) - min(c["arrival_ns"] for c in done)) / 1e9
+            row = {"rate_per_s": lam, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(lat.mean()), 3),
+                   "mrays": round(rays / span / 1e6, 2)}
+            res[mode].append(row)
+            log(f"C5 {mode:6s} rate {lam:5d}/s: p50 {row['p50_ms']:8.2f} ms  p99 {row['p99_ms']:8.2f} ms  "
+                f"{row['mrays']:8.1f} Mrays/s  ({n} requests)")
+        srv.destroy()
+    for sc in scs:
+        sc.destroy()
+    return res
 
 
 def _cpu_model() -> str:

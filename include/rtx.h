@@ -41,6 +41,7 @@ typedef struct rtx_rays rtx_rays;
 typedef struct rtx_hits rtx_hits;
 typedef struct rtx_fb rtx_fb;
 typedef struct rtx_renderer rtx_renderer;
+typedef struct rtx_server rtx_server;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -329,6 +330,68 @@ int32_t rtx_renderer_set_animation(rtx_renderer* r, int32_t enable, float t0, fl
  * one frame earlier.  Takes effect at the next rtx_render_frame_async /
  * rtx_renderer_capture (capture after one eager pipelined frame). */
 int32_t rtx_renderer_set_pipelined(rtx_renderer* r, int32_t enable);
+
+/* Bounces of the next frames, at most the max_bounces the renderer was
+ * created with (the reference's RendererConfig::max_bounces, per frame). */
+int32_t rtx_renderer_set_bounces(rtx_renderer* r, int32_t bounces);
+
+/* ------------------------------------------------------------- serving -- */
+/* Request server over resident scenes (no counterpart in the reference,
+ * whose apps render one scene from one thread).  A dispatcher thread of the
+ * library starts each request at its arrival time on one of `workers`
+ * streams (requests run concurrently; a freed worker takes the next ready
+ * request at once), renders it with the read-back-free frame (no rebuild:
+ * scenes are resident and built) and timestamps its completion on the
+ * device.  Scenes are admitted against an HBM budget.  With `plain` set the
+ * server is the per-request baseline: one request at a time, the plain
+ * frame (rtx_render_frame). */
+#define RTX_SERVER_MAX_CLASSES 4
+typedef struct rtx_server_params {
+    int32_t workers;          /* concurrent requests (1..16) */
+    int32_t max_bounces;      /* largest request depth (1..16) */
+    int32_t classes;          /* resolution classes (1..RTX_SERVER_MAX_CLASSES) */
+    int32_t class_w[RTX_SERVER_MAX_CLASSES], class_h[RTX_SERVER_MAX_CLASSES];
+    uint64_t hbm_budget;      /* bytes for workers + resident scenes; 0 = 90% of free device memory */
+    int32_t plain;            /* 1: baseline, one request at a time with the plain frame */
+    int32_t queue_cap;        /* submissions held at most (0 = 65536) */
+} rtx_server_params;
+
+typedef struct rtx_request {
+    int32_t scene;            /* id from rtx_server_add_scene */
+    int32_t width, height;    /* one of the resolution classes */
+    int32_t bounces;          /* 0..max_bounces */
+    rtx_camera camera;
+    int64_t arrival_ns;       /* rtx_now_ns() clock; not started before it */
+    uint64_t tag;             /* returned with the completion */
+    void* out_accum;          /* optional host copy of the accumulators (w*h*3 uint32), may be NULL */
+} rtx_request;
+
+typedef struct rtx_completion {
+    uint64_t tag;
+    int64_t arrival_ns, start_ns, done_ns;  /* done: the frame's completion observed on the device */
+    uint64_t rays;                          /* primary + secondary rays */
+    uint64_t shadow_rays;
+    int32_t status;                         /* RTX_OK or the error of this request */
+    int32_t worker;
+} rtx_completion;
+
+/* Monotonic clock of the server, nanoseconds. */
+int64_t rtx_now_ns(void);
+int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server** out);
+/* Stops the dispatcher after the queued requests finish, then frees. */
+int32_t rtx_server_destroy(rtx_server* srv);
+/* Makes a built scene resident: RTX_ENOMEM when it does not fit the budget
+ * next to the workers and the scenes already resident. */
+int32_t rtx_server_add_scene(rtx_server* srv, rtx_scene* s, int32_t* scene_id);
+/* Device bytes held by the workers and by the resident scenes. */
+int32_t rtx_server_memory(rtx_server* srv, uint64_t* workers_bytes, uint64_t* scenes_bytes);
+/* Queues n requests (thread-safe); RTX_ENOMEM when the queue is full. */
+int32_t rtx_server_submit(rtx_server* srv, const rtx_request* reqs, uint32_t n);
+/* Moves up to cap finished requests into out; *n = how many. */
+int32_t rtx_server_poll(rtx_server* srv, rtx_completion* out, uint32_t cap, uint32_t* n);
+/* Waits until every submitted request has finished (or timeout_ms passed:
+ * RTX_ESTATE). */
+int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
 
 /* --------------------------------------------------------- host helpers -- */
 /* Camera::set (src/rt/camera.cpp:31-59); host only. */

@@ -295,6 +295,21 @@ int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out) {
     return RTX_OK;
 }
 
+// Device bytes held by a scene (buffers and BVH, build scratch included).
+uint64_t rtx_scene_device_bytes(const rtx_scene* sc) {
+    rtx_scene* s = const_cast<rtx_scene*>(sc);
+    const DevBuf* all[] = {&s->pos,      &s->nrm,        &s->vnrm,     &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
+                     &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
+                     &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
+                     &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
+                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
+                     &s->hier_status, &s->waves,      &s->stage,    &s->clock};
+    uint64_t b = 0;
+    for (const DevBuf* d : all) b += d->p ? d->bytes : 0;
+    b += s->rc_node.bytes + s->rc_delta.bytes + s->rc_tri.bytes + s->rc_nrm.bytes + s->rc_matmap.bytes;
+    return b;
+}
+
 int32_t rtx_scene_destroy(rtx_scene* s) {
     RTX_REQUIRE(s, RTX_EINVAL, "rtx_scene_destroy: null scene");
     (void)hipStreamSynchronize(s->ctx->stream);

@@ -148,6 +148,14 @@ int32_t rtx_renderer_destroy(rtx_renderer* r) {
 
 rtx_fb* rtx_renderer_fb(rtx_renderer* r) { return r ? r->fb : nullptr; }
 
+int32_t rtx_renderer_set_bounces(rtx_renderer* r, int32_t bounces) {
+    RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_set_bounces: null renderer");
+    RTX_REQUIRE(bounces >= 0 && bounces + 1 <= (int32_t)r->rays.size(), RTX_EINVAL,
+                "rtx_renderer_set_bounces: more bounces than the renderer was created for");
+    r->p.max_bounces = bounces;
+    return RTX_OK;
+}
+
 int32_t rtx_renderer_set_pipelined(rtx_renderer* r, int32_t enable) {
     RTX_REQUIRE(r, RTX_EINVAL, "rtx_renderer_set_pipelined: null renderer");
     r->pipelined = enable != 0;

@@ -51,6 +51,26 @@ class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3)]
 
 
+SERVER_MAX_CLASSES = 4
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("max_bounces", C.c_int32), ("classes", C.c_int32),
+                ("class_w", C.c_int32 * SERVER_MAX_CLASSES), ("class_h", C.c_int32 * SERVER_MAX_CLASSES),
+                ("hbm_budget", C.c_uint64), ("plain", C.c_int32), ("queue_cap", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("bounces", C.c_int32),
+                ("camera", C.c_float * 12), ("arrival_ns", C.c_int64), ("tag", C.c_uint64),
+                ("out_accum", C.c_void_p)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("arrival_ns", C.c_int64), ("start_ns", C.c_int64), ("done_ns", C.c_int64),
+                ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("status", C.c_int32), ("worker", C.c_int32)]
+
+
 class LbvhParams(C.Structure):
     _fields_ = [("depth", C.c_int32), ("min_leaf", C.c_int32)]
 
@@ -95,7 +115,9 @@ SYMBOLS = [
     "rtx_shade",
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
     "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay", "rtx_renderer_set_animation",
-    "rtx_renderer_set_pipelined",
+    "rtx_renderer_set_pipelined", "rtx_renderer_set_bounces",
+    "rtx_now_ns", "rtx_server_create", "rtx_server_destroy", "rtx_server_add_scene", "rtx_server_memory",
+    "rtx_server_submit", "rtx_server_poll", "rtx_server_drain",
     "rtx_camera_set",
 ]
 
@@ -141,6 +163,15 @@ def lib() -> C.CDLL:
         "rtx_read_vertices": (I32, [P, P, P, U32]),
         "rtx_renderer_set_animation": (I32, [P, I32, F, F]),
         "rtx_renderer_set_pipelined": (I32, [P, I32]),
+        "rtx_renderer_set_bounces": (I32, [P, I32]),
+        "rtx_now_ns": (C.c_int64, []),
+        "rtx_server_create": (I32, [P, P, C.POINTER(P)]),
+        "rtx_server_destroy": (I32, [P]),
+        "rtx_server_add_scene": (I32, [P, P, P]),
+        "rtx_server_memory": (I32, [P, P, P]),
+        "rtx_server_submit": (I32, [P, P, U32]),
+        "rtx_server_poll": (I32, [P, P, U32, P]),
+        "rtx_server_drain": (I32, [P, C.c_int64]),
         "rtx_lbvh_build": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
@@ -572,3 +603,70 @@ class Renderer:
         st = FrameStats()
         _check(lib().rtx_renderer_last_stats(self.h, C.byref(st)), "rtx_renderer_last_stats")
         return st
+
+
+def now_ns() -> int:
+    """The server's monotonic clock (ns)."""
+    return int(lib().rtx_now_ns())
+
+
+class Server:
+    """Request server over resident scenes (rtx_server_*): a native
+    dispatcher thread starts each request at its arrival time on one of
+    `workers` streams and records its device completion time."""
+
+    def __init__(self, ctx: Context, classes, workers: int = 4, max_bounces: int = 3, hbm_budget: int = 0,
+                 plain: bool = False, queue_cap: int = 0):
+        assert 1 <= len(classes) <= SERVER_MAX_CLASSES
+        p = ServerParams()
+        p.workers, p.max_bounces, p.classes = int(workers), int(max_bounces), len(classes)
+        for i, (w, h) in enumerate(classes):
+            p.class_w[i], p.class_h[i] = int(w), int(h)
+        p.hbm_budget, p.plain, p.queue_cap = int(hbm_budget), int(bool(plain)), int(queue_cap)
+        self.h = C.c_void_p()
+        _check(lib().rtx_server_create(ctx.h, C.byref(p), C.byref(self.h)), "rtx_server_create")
+
+    def add_scene(self, scene: Scene) -> int:
+        sid = C.c_int32(-1)
+        _check(lib().rtx_server_add_scene(self.h, scene.h, C.byref(sid)), "rtx_server_add_scene")
+        return int(sid.value)
+
+    def memory(self):
+        w, sc = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().rtx_server_memory(self.h, C.byref(w), C.byref(sc)), "rtx_server_memory")
+        return int(w.value), int(sc.value)
+
+    def submit(self, reqs):
+        """reqs: iterable of dicts {scene, width, height, bounces, camera (Camera), arrival_ns, tag[, out]}
+        where out is an optional C-contiguous uint32 array of w*h*3 for the accumulators."""
+        reqs = list(reqs)
+        arr = (Request * len(reqs))()
+        self._keep = getattr(self, "_keep", [])
+        for i, r in enumerate(reqs):
+            a = arr[i]
+            a.scene, a.width, a.height, a.bounces = int(r["scene"]), int(r["width"]), int(r["height"]), \
+                int(r["bounces"])
+            C.memmove(C.addressof(a.camera), C.addressof(r["camera"]), 48)
+            a.arrival_ns, a.tag = int(r["arrival_ns"]), int(r["tag"])
+            out = r.get("out")
+            if out is not None:
+                assert out.flags["C_CONTIGUOUS"] and out.dtype == np.uint32 and out.size == a.width * a.height * 3
+                self._keep.append(out)
+                a.out_accum = out.ctypes.data
+        _check(lib().rtx_server_submit(self.h, arr, len(reqs)), "rtx_server_submit")
+
+    def poll(self, cap: int = 4096):
+        arr = (Completion * cap)()
+        n = C.c_uint32(0)
+        _check(lib().rtx_server_poll(self.h, arr, cap, C.byref(n)), "rtx_server_poll")
+        return [dict(tag=c.tag, arrival_ns=c.arrival_ns, start_ns=c.start_ns, done_ns=c.done_ns, rays=c.rays,
+                     shadow_rays=c.shadow_rays, status=c.status, worker=c.worker) for c in arr[: n.value]]
+
+    def drain(self, timeout_ms: int = 60000):
+        _check(lib().rtx_server_drain(self.h, int(timeout_ms)), "rtx_server_drain")
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rtx_server_destroy(self.h), "rtx_server_destroy")
+            self.h = C.c_void_p()
+            self._keep = []
