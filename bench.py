@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--c5-rates", type=str, default="50,100,200,500,1000,2000",
                     help="arrival rates of the serving leg, per second ('' = skip)")
     ap.add_argument("--c5-max-requests", type=int, default=600)
+    ap.add_argument("--c4-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -167,6 +168,14 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     value = rays_per_frame * args.steps / dt / 1e6
     log(f"timed {args.steps} pipelined graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
+    # the same pipelined frames launched stage by stage (no graph)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        R.frame_async(s, cam, rebuild=True)
+    ctx.sync()
+    pip_eager_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"pipelined stage-launched frames: {pip_eager_ms:.3f} ms/frame")
     R.set_pipelined(False)
 
     # the same frame launched stage by stage (no graph)
@@ -243,6 +252,7 @@ def main():
         "frame_mode": "hipGraph replay of the pipelined frame: LBVH rebuild of the next frame on a second stream "
                       "while this frame renders (the reference's RendererT); every timed frame does one full "
                       "rebuild and one full render",
+        "pipelined_stage_launch_ms_per_frame": round(pip_eager_ms, 4),
         "sequential_graph_ms_per_frame": round(seq_ms, 4),
         "sequential_graph_mrays": round(seq_value, 3),
         "stage_launch_ms_per_frame": round(async_ms, 4),
@@ -264,6 +274,11 @@ def main():
         R = s = None
         out["c3_dynamic"] = bench_c3(ctx, args.c3_frames)
         log(f"C3: {out['c3_dynamic']['ms_per_frame']} ms/frame, {out['c3_dynamic']['mrays']} Mrays/s")
+    if args.c4_scenes > 0:
+        out["c4_batched"] = bench_c4(ctx, args.c4_scenes)
+        c4 = out["c4_batched"]
+        log(f"C4: {c4['ms_per_batch']} ms per batch, {c4['mrays']} Mrays/s (one by one: "
+            f"{c4['one_by_one_mrays']} Mrays/s); chosen {c4['chosen']}")
     if args.c5_rates:
         rates = [int(x) for x in args.c5_rates.split(",") if x]
         out["c5_serving"] = bench_c5(ctx, rates, args.c5_max_requests)
@@ -428,6 +443,103 @@ def bench_c5(ctx, rates, max_requests: int, workers: int = 4):
     for sc in scs:
         sc.destroy()
     return res
+
+
+def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 4):
+    """C4 batched: 64 scenes, T ~ logU[10k, 2M] (sphere sets and
+    heightfields), resolution U{256^2, 512^2, 1280x720, 1920x1080}, depth
+    U{1..4}; one frame of every scene per batch, rendered side by side on
+    `streams` streams and captured into one hipGraph.  The traversal knobs
+    (variant, work-group size, cooperative-leaf size) are autotuned on the
+    batch first.  Compared with the same 64 frames rendered one after the
+    other."""
+    import numpy as np
+    import rtx
+    from rtx import scenes
+    t0 = time.perf_counter()
+    rng = scenes.Rng(0x5EED0004)
+    tris = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(2e6)))
+    pick = np.random.default_rng(4)
+    res_set = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    scs, rs, cams, rays_cap = [], [], [], []
+    for i, ti in enumerate(tris):
+        if i % 2 == 0:
+            sd = scenes.config_c2(seed=0x5EED0004 + 16 * i, n_spheres=max(2, int(round(ti / 5000))), res=(256, 256))
+        else:
+            sd = scenes.config_c3(seed=0x5EED0004 + 16 * i, n=max(8, int(round(np.sqrt(ti / 2.0))) + 1),
+                                  res=(256, 256))
+            sd.pos[:, 1] = scenes.displace_host(sd, 0.5)[:, 1]
+        w, h = res_set[int(pick.integers(0, 4))]
+        depth = int(pick.integers(1, 5))
+        sc = scenes.load(rtx.Scene(ctx), sd)
+        sc.build(30, 1)
+        c = 0.5 * (sd.pos.min(0) + sd.pos.max(0))
+        r = 0.5 * float(np.linalg.norm(sd.pos.max(0) - sd.pos.min(0)))
+        d = pick.normal(size=3)
+        d /= np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
+        cams.append(rtx.camera_set(c + 2.5 * r * d, c, up, np.pi / 4, w / h))
+        # bundles of 2 x pixels rays per bounce (the reference's capacity
+        # rule drops rays beyond it), to keep 64 renderers within memory
+        rs.append(rtx.Renderer(ctx, w, h, depth, ray_capacity=2 * w * h + 2))
+        scs.append(sc)
+    ctx.sync()
+    log(f"C4: {n_scenes} scenes, {int(tris.sum())} triangles (target), set up in {time.perf_counter() - t0:.1f}s")
+
+    def one_by_one():
+        for R, sc, cam in zip(rs, scs, cams):
+            R.frame_async(sc, cam, rebuild=False)
+        ctx.sync()
+
+    def timed(fn, n):
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t) / n
+
+    # autotune the traversal knobs on the batch (every setting gives the same images)
+    B = rtx.Batch(ctx, rs, scs, cams, streams=streams)
+    tune = []
+    for var, tb, coop in [(1, 128, 8), (1, 64, 8), (1, 256, 8), (0, 128, 8), (1, 128, 4), (1, 128, 16), (2, 128, 8),
+                          (3, 128, 8)]:
+        ctx.tune("trace_variant", var)
+        ctx.tune("trace_block", tb)
+        ctx.tune("coop_leaf", coop)
+        B.render()
+        dt = timed(B.render, 2)
+        tune.append({"trace_variant": var, "trace_block": tb, "coop_leaf": coop, "ms": round(dt * 1e3, 3)})
+    best = min(tune, key=lambda x: x["ms"])
+    ctx.tune("trace_variant", best["trace_variant"])
+    ctx.tune("trace_block", best["trace_block"])
+    ctx.tune("coop_leaf", best["coop_leaf"])
+    B.render()
+    ctx.sync()
+    rays, shadow = B.stats()
+    dt_eager = timed(B.render, reps)
+    B.capture()
+    B.replay()
+    dt_graph = timed(B.replay, reps)
+    dt_seq = timed(one_by_one, reps)
+    dt_best = min(dt_eager, dt_graph)
+    B.destroy()
+    for R in rs:
+        R.destroy()
+    for sc in scs:
+        sc.destroy()
+    ctx.tune("trace_variant", 1)
+    ctx.tune("trace_block", 128)
+    ctx.tune("coop_leaf", 8)
+    return {"workload": "C4 batched: 64 resident scenes, T ~ logU[10k, 2M] (sphere sets and heightfields), "
+                        "resolution U{256^2, 512^2, 1280x720, 1920x1080}, depth U{1..4}, one frame of each per "
+                        "batch, 4 streams, one hipGraph",
+            "scenes": n_scenes, "rays_per_batch": rays, "shadow_rays_per_batch": shadow,
+            "ms_per_batch": round(dt_best * 1e3, 3), "mrays": round(rays / dt_best / 1e6, 2),
+            "mode": "stream launches" if dt_eager <= dt_graph else "hipGraph replay",
+            "stream_launch_ms_per_batch": round(dt_eager * 1e3, 3), "graph_ms_per_batch": round(dt_graph * 1e3, 3),
+            "one_by_one_ms_per_batch": round(dt_seq * 1e3, 3), "one_by_one_mrays": round(rays / dt_seq / 1e6, 2),
+            "autotune": tune, "chosen": best}
 
 
 def _cpu_model() -> str:

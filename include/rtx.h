@@ -42,6 +42,7 @@ typedef struct rtx_hits rtx_hits;
 typedef struct rtx_fb rtx_fb;
 typedef struct rtx_renderer rtx_renderer;
 typedef struct rtx_server rtx_server;
+typedef struct rtx_batch rtx_batch;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -392,6 +393,26 @@ int32_t rtx_server_poll(rtx_server* srv, rtx_completion* out, uint32_t cap, uint
 /* Waits until every submitted request has finished (or timeout_ms passed:
  * RTX_ESTATE). */
 int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
+
+/* ------------------------------------------------------------ batches -- */
+/* A batch of frames of different scenes rendered together (no counterpart
+ * in the reference, which renders one scene per process): frame i renders
+ * scenes[i] from cams[i] with renderers[i] (resident scenes, no rebuild).
+ * The frames are spread round-robin over `streams` internal streams forked
+ * from and joined back to the caller's stream, so that small frames run side
+ * by side; the whole batch can be captured into one hipGraph.  Handles must
+ * outlive the batch. */
+int32_t rtx_batch_create(rtx_ctx* ctx, rtx_renderer* const* renderers, rtx_scene* const* scenes,
+                         const rtx_camera* cams, int32_t n, int32_t streams, rtx_batch** out);
+int32_t rtx_batch_destroy(rtx_batch* b);
+/* Enqueues every frame of the batch on `stream` (asynchronous). */
+int32_t rtx_batch_render(rtx_batch* b, void* stream);
+/* Captures rtx_batch_render into a hipGraph (render the batch eagerly once
+ * first), and launches it. */
+int32_t rtx_batch_capture(rtx_batch* b, void* stream);
+int32_t rtx_batch_replay(rtx_batch* b, void* stream);
+/* Sum of the frames' ray counts of the last batch (synchronizes). */
+int32_t rtx_batch_stats(rtx_batch* b, uint64_t* rays, uint64_t* shadow_rays);
 
 /* --------------------------------------------------------- host helpers -- */
 /* Camera::set (src/rt/camera.cpp:31-59); host only. */

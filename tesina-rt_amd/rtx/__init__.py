@@ -118,6 +118,8 @@ SYMBOLS = [
     "rtx_renderer_set_pipelined", "rtx_renderer_set_bounces",
     "rtx_now_ns", "rtx_server_create", "rtx_server_destroy", "rtx_server_add_scene", "rtx_server_memory",
     "rtx_server_submit", "rtx_server_poll", "rtx_server_drain",
+    "rtx_batch_create", "rtx_batch_destroy", "rtx_batch_render", "rtx_batch_capture", "rtx_batch_replay",
+    "rtx_batch_stats",
     "rtx_camera_set",
 ]
 
@@ -172,6 +174,12 @@ def lib() -> C.CDLL:
         "rtx_server_submit": (I32, [P, P, U32]),
         "rtx_server_poll": (I32, [P, P, U32, P]),
         "rtx_server_drain": (I32, [P, C.c_int64]),
+        "rtx_batch_create": (I32, [P, P, P, P, I32, I32, C.POINTER(P)]),
+        "rtx_batch_destroy": (I32, [P]),
+        "rtx_batch_render": (I32, [P, P]),
+        "rtx_batch_capture": (I32, [P, P]),
+        "rtx_batch_replay": (I32, [P, P]),
+        "rtx_batch_stats": (I32, [P, P, P]),
         "rtx_lbvh_build": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
@@ -670,3 +678,35 @@ class Server:
             _check(lib().rtx_server_destroy(self.h), "rtx_server_destroy")
             self.h = C.c_void_p()
             self._keep = []
+
+
+class Batch:
+    """Frames of several resident scenes rendered together (rtx_batch_*)."""
+
+    def __init__(self, ctx: Context, renderers, scenes, cams, streams: int = 4):
+        n = len(renderers)
+        assert n == len(scenes) == len(cams) and n > 0
+        rs = (C.c_void_p * n)(*[r.h for r in renderers])
+        ss = (C.c_void_p * n)(*[x.h for x in scenes])
+        cs = (Camera * n)(*cams)
+        self.h = C.c_void_p()
+        _check(lib().rtx_batch_create(ctx.h, rs, ss, cs, n, int(streams), C.byref(self.h)), "rtx_batch_create")
+
+    def render(self, stream=None):
+        _check(lib().rtx_batch_render(self.h, stream), "rtx_batch_render")
+
+    def capture(self, stream=None):
+        _check(lib().rtx_batch_capture(self.h, stream), "rtx_batch_capture")
+
+    def replay(self, stream=None):
+        _check(lib().rtx_batch_replay(self.h, stream), "rtx_batch_replay")
+
+    def stats(self):
+        r, sh = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().rtx_batch_stats(self.h, C.byref(r), C.byref(sh)), "rtx_batch_stats")
+        return int(r.value), int(sh.value)
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rtx_batch_destroy(self.h), "rtx_batch_destroy")
+            self.h = C.c_void_p()
