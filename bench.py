@@ -98,6 +98,8 @@ def main():
                     help="arrival rates of the serving leg, per second ('' = skip)")
     ap.add_argument("--c5-max-requests", type=int, default=600)
     ap.add_argument("--c4-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip)")
+    ap.add_argument("--sort-sizes", type=str, default="16777216,67108864",
+                    help="element counts of the sort-scaling leg ('' = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -274,6 +276,8 @@ def main():
         R = s = None
         out["c3_dynamic"] = bench_c3(ctx, args.c3_frames)
         log(f"C3: {out['c3_dynamic']['ms_per_frame']} ms/frame, {out['c3_dynamic']['mrays']} Mrays/s")
+    if args.sort_sizes:
+        out["sort_scaling"] = bench_sort_scaling(ctx, [int(x) for x in args.sort_sizes.split(",") if x])
     if args.c4_scenes > 0:
         out["c4_batched"] = bench_c4(ctx, args.c4_scenes)
         c4 = out["c4_batched"]
@@ -540,6 +544,38 @@ def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 4):
             "stream_launch_ms_per_batch": round(dt_eager * 1e3, 3), "graph_ms_per_batch": round(dt_graph * 1e3, 3),
             "one_by_one_ms_per_batch": round(dt_seq * 1e3, 3), "one_by_one_mrays": round(rays / dt_seq / 1e6, 2),
             "autotune": tune, "chosen": best}
+
+
+def bench_sort_scaling(ctx, sizes):
+    """The build's radix-sort pass at sizes beyond the L2/MALL (random
+    triangle soups): average pass time and GB/s at 16 B/element/pass."""
+    import numpy as np
+    import rtx
+    rows = []
+    for T in sizes:
+        rng = np.random.default_rng(1)
+        V = T // 2
+        pos = rng.uniform(-100, 100, (V, 3)).astype(np.float32)
+        idx = rng.integers(0, V, (T, 3), dtype=np.uint32)
+        s = rtx.Scene(ctx)
+        s.set_geometry(pos, idx, None, None, np.zeros(T, np.int32))
+        del pos, idx
+        s.build(30, 1)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(3):
+            s.build(30, 1)
+        rep = ctx.profile_report()
+        ctx.profile(False)
+        n, ms = rep["lbvh_sort_pass"]
+        avg_s = ms / n / 1e3
+        gbs = 16.0 * T / avg_s / 1e9
+        rows.append({"elements": T, "avg_pass_ms": round(avg_s * 1e3, 4), "achieved_GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "lbvh_build_ms": round(rep["LBVH_BUILD"][1] / 3, 3)})
+        log(f"sort pass at {T} elements: {avg_s * 1e3:.3f} ms, {gbs:.0f} GB/s; build {rows[-1]['lbvh_build_ms']} ms")
+        s.destroy()
+    return rows
 
 
 def _cpu_model() -> str:
