@@ -43,6 +43,7 @@ typedef struct rtx_fb rtx_fb;
 typedef struct rtx_renderer rtx_renderer;
 typedef struct rtx_server rtx_server;
 typedef struct rtx_batch rtx_batch;
+typedef struct rtx_pipeline rtx_pipeline;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -393,6 +394,32 @@ int32_t rtx_server_poll(rtx_server* srv, rtx_completion* out, uint32_t cap, uint
 /* Waits until every submitted request has finished (or timeout_ms passed:
  * RTX_ESTATE). */
 int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
+
+/* ------------------------------------------------------- frame pipeline -- */
+/* Frames of one scene with `depth` (1..4) frames in flight: the reference's
+ * RendererT (src/rt/renderer-threaded.cpp:452-506) overlaps the next build
+ * with the last render; this goes further.  Frame N is rendered by slot
+ * N mod depth (its own renderer, stream and render copy of the scene).  A
+ * build stream animates (if enabled) and rebuilds the scene for frame N, then
+ * copies it into slot N's render copy once frame N - depth has finished with
+ * that copy; slot N's stream renders frame N once its copy is ready.  So
+ * frame N+1's build overlaps frames N and N-1, and renders of consecutive
+ * frames overlap each other.  Each frame's image equals the sequential run's.
+ * The scene must not be driven by another renderer while the pipeline runs.
+ * Each slot renders on one stream; concurrency needs a hardware queue per
+ * active stream (GPU_MAX_HW_QUEUES >= depth + 2). */
+int32_t rtx_pipeline_create(rtx_ctx* ctx, rtx_scene* s, const rtx_render_params* p, int32_t depth,
+                            rtx_pipeline** out);
+int32_t rtx_pipeline_destroy(rtx_pipeline* pl);
+/* Animation at the start of every frame (rtx_scene_animate on the build stream). */
+int32_t rtx_pipeline_set_animation(rtx_pipeline* pl, int32_t enable, float t0, float dt);
+/* Enqueues the next frame (asynchronous); *frame = its index (0, 1, ...). */
+int32_t rtx_pipeline_frame(rtx_pipeline* pl, const rtx_camera* cam, int32_t rebuild, uint64_t* frame);
+/* Waits for `frame`; its framebuffer and stats stay valid until frame+depth
+ * is enqueued. */
+int32_t rtx_pipeline_wait(rtx_pipeline* pl, uint64_t frame);
+rtx_fb* rtx_pipeline_fb(rtx_pipeline* pl, uint64_t frame);
+int32_t rtx_pipeline_stats(rtx_pipeline* pl, uint64_t frame, rtx_frame_stats* stats);
 
 /* ------------------------------------------------------------ batches -- */
 /* A batch of frames of different scenes rendered together (no counterpart

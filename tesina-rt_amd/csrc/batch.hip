@@ -26,6 +26,8 @@ int32_t rtx_batch_create(rtx_ctx* ctx, rtx_renderer* const* renderers, rtx_scene
             RTX_REQUIRE(renderers[j] != renderers[i], RTX_EINVAL,
                         "rtx_batch_create: a renderer may render one frame of the batch");
     }
+    // the batch's own streams give the concurrency; each frame renders on one
+    for (int i = 0; i < n; ++i) renderers[i]->single_stream = true;
     rtx_batch* b = new rtx_batch();
     b->ctx = ctx;
     b->r.assign(renderers, renderers + n);

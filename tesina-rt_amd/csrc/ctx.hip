@@ -306,7 +306,7 @@ uint64_t rtx_scene_device_bytes(const rtx_scene* sc) {
                      &s->hier_status, &s->waves,      &s->stage,    &s->clock};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->p ? d->bytes : 0;
-    b += s->rc_node.bytes + s->rc_delta.bytes + s->rc_tri.bytes + s->rc_nrm.bytes + s->rc_matmap.bytes;
+    for (const auto& c : s->rc) b += c.node.bytes + c.delta.bytes + c.tri.bytes + c.nrm.bytes + c.matmap.bytes;
     return b;
 }
 
@@ -319,6 +319,13 @@ int32_t rtx_scene_destroy(rtx_scene* s) {
                      &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
                      &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
                      &s->hier_status, &s->waves,      &s->stage,    &s->clock};
+    for (auto& c : s->rc) {
+        c.node.release();
+        c.delta.release();
+        c.tri.release();
+        c.nrm.release();
+        c.matmap.release();
+    }
     for (DevBuf* b : all) b->release();
     delete s;
     return RTX_OK;

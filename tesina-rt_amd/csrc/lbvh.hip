@@ -74,10 +74,13 @@ __global__ __launch_bounds__(BS) void k_build_init(uint32_t* zero_a, uint32_t na
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < nb; i += stride) zero_b[i] = 0;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < nc; i += stride) zero_c[i] = 0;
     if (set_root && blockIdx.x == 0) {
-        for (uint32_t i = threadIdx.x; i < LV_WORDS; i += BS) levels[i] = 0;
+        // every word is written once, by one thread, with its final value:
+        // the root's count and the node total are 1, the rest 0 (a zeroing
+        // pass followed by a store of the root words from another wave would
+        // race with it)
+        for (uint32_t i = threadIdx.x; i < LV_WORDS; i += BS)
+            levels[i] = (i == (uint32_t)(LV_CNT + 0) || i == (uint32_t)LV_TOTAL) ? 1u : 0u;
         if (threadIdx.x == 0) {
-            levels[LV_CNT + 0] = 1;
-            levels[LV_TOTAL] = 1;
             bstart[0] = 0;
             bend[0] = T;
             parent[0] = 0;

@@ -39,11 +39,25 @@ __global__ __launch_bounds__(256) void k_copy_segs(CopySegs c) {
 }
 }  // namespace
 
-// Makes the render copy of everything the render stages read, on `st`.
-static int32_t refresh_render_copy(rtx_scene* s, hipStream_t st) {
+// Allocates render copy k (host-side; do it before frames run concurrently).
+int32_t rtx_reserve_render_copy(rtx_scene* s, int k) {
+    RTX_REQUIRE(k >= 0 && k < rtx_scene::COPIES, RTX_EINVAL, "render copy: bad copy index");
+    rtx_scene::RenderCopy& rc = s->rc[k];
     int32_t e;
+    if ((e = rc.node.reserve(s->ctx, s->node.bytes)) || (e = rc.delta.reserve(s->ctx, s->delta.bytes)) ||
+        (e = rc.tri.reserve(s->ctx, (uint64_t)s->T * 48)) || (e = rc.nrm.reserve(s->ctx, (uint64_t)s->V * 16)) ||
+        (e = rc.matmap.reserve(s->ctx, (uint64_t)s->T * 4)))
+        return e;
+    return RTX_OK;
+}
+
+// Makes render copy k of everything the render stages read, on `st`.
+int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st) {
+    int32_t e;
+    RTX_REQUIRE(k >= 0 && k < rtx_scene::COPIES, RTX_EINVAL, "render copy: bad copy index");
+    rtx_scene::RenderCopy& rc = s->rc[k];
     DevBuf* src[5] = {&s->node, &s->delta, &s->tri, &s->nrm, &s->matmap};
-    DevBuf* dst[5] = {&s->rc_node, &s->rc_delta, &s->rc_tri, &s->rc_nrm, &s->rc_matmap};
+    DevBuf* dst[5] = {&rc.node, &rc.delta, &rc.tri, &rc.nrm, &rc.matmap};
     const uint64_t bytes[5] = {s->node.bytes, s->delta.bytes, (uint64_t)s->T * 48, (uint64_t)s->V * 16,
                                (uint64_t)s->T * 4};
     CopySegs c{};
@@ -89,11 +103,6 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
     }
     if (!e) e = rtx_fb_create(ctx, p->width, p->height, &r->fb);
     if (!e) e = r->counts.reserve(ctx, 64);
-    if (!e && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess) {
-        r->side = nullptr;
-        rtx_set_error("rtx_renderer_create: side stream creation failed");
-        e = RTX_EDEVICE;
-    }
     for (int b = 0; b <= r->p.max_bounces && !e; ++b) {
         hipEvent_t ev = nullptr;
         if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
@@ -108,15 +117,22 @@ int32_t rtx_renderer_create(rtx_ctx* ctx, const rtx_render_params* p, rtx_render
         rtx_set_error("rtx_renderer_create: event creation failed");
         e = RTX_EDEVICE;
     }
-    if (!e && hipStreamCreateWithFlags(&r->bstream, hipStreamNonBlocking) != hipSuccess) {
-        r->bstream = nullptr;
-        rtx_set_error("rtx_renderer_create: build stream creation failed");
-        e = RTX_EDEVICE;
-    }
     if (!e && (hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
                hipEventCreateWithFlags(&r->ev_built, hipEventDisableTiming) != hipSuccess)) {
         rtx_set_error("rtx_renderer_create: event creation failed");
         e = RTX_EDEVICE;
+    }
+    if (!e) {
+        // the first primary generation uploads the pixel-sample table (a
+        // blocking copy on the null stream, unordered with the non-blocking
+        // streams frames run on): do it now, synchronously, not in a frame
+        rtx_camera cam{};
+        e = rtx_raygen_primary(r->rays[0], &cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, 0,
+                               ctx->stream);
+        if (!e && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rtx_set_error("rtx_renderer_create: synchronization failed");
+            e = RTX_EDEVICE;
+        }
     }
     if (e) {
         rtx_renderer_destroy(r);
@@ -244,6 +260,19 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
 // with independent work).  Results are identical to the reference sequence:
 // shadow rays only set flag bits 8+ that ray generation does not read, and
 // shading is an order-free integer sum.
+// Streams are created on first use: every stream takes one of the process's
+// hardware queues (round-robin at creation), and streams that share a queue
+// lose their concurrency.
+static int32_t ensure_stream(hipStream_t* st, const char* what) {
+    if (*st) return RTX_OK;
+    if (hipStreamCreateWithFlags(st, hipStreamNonBlocking) != hipSuccess) {
+        *st = nullptr;
+        rtx_set_error(std::string("renderer: ") + what + " stream creation failed");
+        return RTX_EDEVICE;
+    }
+    return RTX_OK;
+}
+
 // Render stages of the read-back-free frame (after set-up): counts stay on
 // the device.  Bounce b's rays depend only on bounce b-1's hits, and its
 // shadow rays and shading on nothing later, so after bounce b is traced the
@@ -253,10 +282,12 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
 // reference sequence: shadow rays only set flag bits 8+ that ray generation
 // does not read, and shading is an order-free integer sum.
 static int32_t render_stages(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, hipStream_t st) {
-    hipStream_t sd = r->side;
+    const bool two = !r->single_stream;
+    int32_t e;
+    if (two && (e = ensure_stream(&r->side, "side"))) return e;
+    hipStream_t sd = two ? r->side : st;
     const uint64_t P = (uint64_t)r->p.width * r->p.height;
     unsigned long long* cnt = r->counts.as<unsigned long long>();
-    int32_t e;
     ProfScope whole(r->ctx, st, "FRAME_RENDER");
     const int B = r->p.max_bounces;
     if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
@@ -271,13 +302,17 @@ static int32_t render_stages(rtx_renderer* r, rtx_scene* s, const rtx_camera* ca
         }
         // zero-count bounces run as empty launches (no host decision)
         if ((e = rtx_trace(s, r->rays[b], r->hits[b], nb, st))) return e;
-        RTX_CHECK_HIP(hipEventRecord(r->ev[b], st));
-        RTX_CHECK_HIP(hipStreamWaitEvent(sd, r->ev[b], 0));
+        if (two) {
+            RTX_CHECK_HIP(hipEventRecord(r->ev[b], st));
+            RTX_CHECK_HIP(hipStreamWaitEvent(sd, r->ev[b], 0));
+        }
         if ((e = rtx_trace_shadow_impl(s, r->rays[b], r->hits[b], nb, (uint64_t*)(cnt + 1), sd))) return e;
         if ((e = rtx_shade(s, r->rays[b], r->hits[b], r->fb, nb, b == 0, sd))) return e;
     }
-    RTX_CHECK_HIP(hipEventRecord(r->ev_join, sd));
-    RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_join, 0));
+    if (two) {
+        RTX_CHECK_HIP(hipEventRecord(r->ev_join, sd));
+        RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_join, 0));
+    }
     return rtx_fb_resolve(r->fb, st);
 }
 
@@ -292,10 +327,11 @@ static int32_t render_frame_pipelined(rtx_renderer* r, rtx_scene* s, const rtx_c
                                       hipStream_t st) {
     int32_t e;
     const rtx_lbvh_params bp{r->p.depth, r->p.min_leaf};
+    if ((e = ensure_stream(&r->bstream, "build"))) return e;
     if (!r->copy_valid || !s->built || s != r->copy_of) {
         if (r->animate && (e = rtx_scene_animate(s, r->anim_t0, r->anim_dt, st))) return e;
         if ((e = rtx_lbvh_build(s, &bp, st))) return e;
-        if ((e = refresh_render_copy(s, st))) return e;
+        if ((e = rtx_refresh_render_copy(s, 0, st))) return e;
         r->copy_valid = true;
         r->copy_of = s;
     }
@@ -306,12 +342,12 @@ static int32_t render_frame_pipelined(rtx_renderer* r, rtx_scene* s, const rtx_c
     if (r->animate && (e = rtx_scene_animate(s, r->anim_t0, r->anim_dt, r->bstream))) return e;
     if (rebuild && (e = rtx_lbvh_build(s, &bp, r->bstream))) return e;
     RTX_CHECK_HIP(hipEventRecord(r->ev_built, r->bstream));
-    s->use_copy = true;
+    s->use_copy = 0;
     e = render_stages(r, s, cam, st);
-    s->use_copy = false;
+    s->use_copy = -1;
     if (e) return e;
     RTX_CHECK_HIP(hipStreamWaitEvent(st, r->ev_built, 0));
-    return refresh_render_copy(s, st);
+    return rtx_refresh_render_copy(s, 0, st);
 }
 
 int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, int32_t rebuild, void* stream) {

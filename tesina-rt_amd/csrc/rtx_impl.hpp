@@ -125,11 +125,16 @@ struct rtx_scene {
     DevBuf tickets;               // tile tickets for every ordered kernel
     DevBuf hier_status;           // look-back words, per level x tile
     DevBuf fat;                   // node ids of the leaves bounded by one wave each (kept for refit)
-    // Render copy of what the render stages read (pipelined frames: the next
-    // frame's build writes the buffers above while this frame renders from
-    // the copy).  use_copy selects it for the launches being enqueued.
-    DevBuf rc_node, rc_delta, rc_tri, rc_nrm, rc_matmap;
-    bool use_copy = false;
+    // Render copies of what the render stages read (pipelined frames: the
+    // next frame's build writes the buffers above while frames in flight
+    // render from copies).  use_copy selects one for the launches being
+    // enqueued (-1: the build's own buffers).
+    struct RenderCopy {
+        DevBuf node, delta, tri, nrm, matmap;
+    };
+    static constexpr int COPIES = 4;
+    RenderCopy rc[COPIES];
+    int use_copy = -1;
 };
 
 // Buffers the render stages (trace, shadow, secondary generation, shading)
@@ -142,10 +147,11 @@ struct SceneView {
     const int32_t* matmap;
 };
 inline SceneView scene_view(const rtx_scene* s) {
-    if (s->use_copy)
-        return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(s->rc_node.p) + 32),
-                         s->rc_delta.as<float4>(), s->rc_tri.as<float4>(), s->rc_nrm.as<float4>(),
-                         s->rc_matmap.as<int32_t>()};
+    if (s->use_copy >= 0) {
+        const rtx_scene::RenderCopy& c = s->rc[s->use_copy];
+        return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(c.node.p) + 32),
+                         c.delta.as<float4>(), c.tri.as<float4>(), c.nrm.as<float4>(), c.matmap.as<int32_t>()};
+    }
     return SceneView{s->nodes(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),
                      s->matmap.as<int32_t>()};
 }
@@ -192,6 +198,7 @@ struct rtx_renderer {
     // bounce.  ev[b] marks "bounce b traced" on the main stream, ev_join the
     // end of the side stream's work.  Default priority, non-blocking.
     hipStream_t side = nullptr;
+    bool single_stream = false;  // render every stage on the caller's stream (frame pipeline slots)
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_join = nullptr;
     // Pipelined frames: the next frame's animation and build run on bstream
@@ -221,6 +228,9 @@ int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint
 // Shadow rays of several (rays, hits) bundles in one launch.
 int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* const* hits, const uint64_t* counts,
                                int nseg, uint64_t* dev_shadow_count, hipStream_t st);
+// Makes render copy k of a scene (what the render stages read), on `st`.
+int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st);
+int32_t rtx_reserve_render_copy(rtx_scene* s, int k);
 int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
                                   uint64_t* dev_total_acc, hipStream_t st);
 

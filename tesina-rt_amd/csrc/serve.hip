@@ -226,6 +226,7 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
             rp.height = p->class_h[c];
             rp.max_bounces = p->max_bounces;
             if ((e = rtx_renderer_create(ctx, &rp, &w.r[c]))) break;
+            w.r[c]->single_stream = true;  // one stream per worker; workers give the concurrency
             // first raygen uploads the pixel-sample table; do it now, not in a request
             rtx_camera cam{};
             e = rtx_raygen_primary(w.r[c]->rays[0], &cam, rp.width, rp.height, 32, 1, nullptr, 0, 0, w.stream);

@@ -1222,10 +1222,14 @@ int32_t rtx_raygen_primary(rtx_rays* rays, const rtx_camera* cam, int32_t width,
     // upload the table only when it changes, so that a frame that reuses it
     // (and its hipGraph capture) performs no host-to-device copy
     if (rays->psamp_host.size() != 3 * n_s || memcmp(rays->psamp_host.data(), hs, 12 * n_s) != 0) {
+        // the old table may still be read by queued work; the new one must
+        // have landed before any kernel on any stream reads it
+        RTX_CHECK_HIP(hipDeviceSynchronize());
         int32_t e = rays->psamp.reserve(rays->ctx, 12 * n_s);
         if (e) return e;
-        RTX_CHECK_HIP(hipMemcpy(rays->psamp.p, hs, 12 * n_s, hipMemcpyHostToDevice));
         rays->psamp_host.assign(hs, hs + 3 * n_s);
+        RTX_CHECK_HIP(hipMemcpyAsync(rays->psamp.p, rays->psamp_host.data(), 12 * n_s, hipMemcpyHostToDevice, st));
+        RTX_CHECK_HIP(hipStreamSynchronize(st));
     }
     float* dtab = rays->psamp.as<float>();
     ProfScope ps(rays->ctx, st, "raygen_primary");

@@ -120,6 +120,8 @@ SYMBOLS = [
     "rtx_server_submit", "rtx_server_poll", "rtx_server_drain",
     "rtx_batch_create", "rtx_batch_destroy", "rtx_batch_render", "rtx_batch_capture", "rtx_batch_replay",
     "rtx_batch_stats",
+    "rtx_pipeline_create", "rtx_pipeline_destroy", "rtx_pipeline_set_animation", "rtx_pipeline_frame",
+    "rtx_pipeline_wait", "rtx_pipeline_fb", "rtx_pipeline_stats",
     "rtx_camera_set",
 ]
 
@@ -180,6 +182,13 @@ def lib() -> C.CDLL:
         "rtx_batch_capture": (I32, [P, P]),
         "rtx_batch_replay": (I32, [P, P]),
         "rtx_batch_stats": (I32, [P, P, P]),
+        "rtx_pipeline_create": (I32, [P, P, P, I32, C.POINTER(P)]),
+        "rtx_pipeline_destroy": (I32, [P]),
+        "rtx_pipeline_set_animation": (I32, [P, I32, F, F]),
+        "rtx_pipeline_frame": (I32, [P, P, I32, P]),
+        "rtx_pipeline_wait": (I32, [P, U64]),
+        "rtx_pipeline_fb": (P, [P, U64]),
+        "rtx_pipeline_stats": (I32, [P, U64, P]),
         "rtx_lbvh_build": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
@@ -299,6 +308,11 @@ class Context:
     def stream(self):
         return lib().rtx_stream(self.h)
 
+    def device_sync(self):
+        """Waits for every stream of the device (not only the context's)."""
+        _hip_check(_hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
     def profile(self, on: bool = True):
         _check(lib().rtx_profile_enable(self.h, int(on)), "rtx_profile_enable")
 
@@ -329,6 +343,46 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+
+def _hip() -> C.CDLL:
+    """The HIP runtime librtx is linked against (the instance it already
+    loaded): streams given to the entry points must come from it."""
+    lib()
+    for name in ("libamdhip64.so.7", "libamdhip64.so"):
+        try:
+            return C.CDLL(name)
+        except OSError:
+            continue
+    raise RtxError("HIP runtime library not found")
+
+
+def _hip_check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RtxError(f"{what} failed (hipError_t {rc})")
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority, to pass as
+    `stream=` to the entry points (e.g. a build beside a rendering frame)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        _hip_check(_hip().hipStreamCreateWithFlags(C.byref(self.h), 1), "hipStreamCreateWithFlags")
+
+    @property
+    def handle(self) -> int:
+        return self.h.value
+
+    def synchronize(self):
+        _hip_check(_hip().hipStreamSynchronize(self.h), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.h:
+            _hip().hipStreamSynchronize(self.h)
+            _hip().hipStreamDestroy(self.h)
+            self.h = C.c_void_p()
 
 
 class Scene:
@@ -709,4 +763,46 @@ class Batch:
     def destroy(self):
         if self.h:
             _check(lib().rtx_batch_destroy(self.h), "rtx_batch_destroy")
+            self.h = C.c_void_p()
+
+
+class Pipeline:
+    """Frames of one scene with `depth` frames in flight (rtx_pipeline_*)."""
+
+    def __init__(self, ctx: Context, scene: Scene, width: int, height: int, max_bounces: int, depth: int = 2,
+                 bvh_depth: int = 30, min_leaf: int = 1, quad: int = 32, ray_capacity: int = 0):
+        self.ctx = ctx
+        self.w, self.hgt = width, height
+        p = RenderParams(width, height, quad, max_bounces, bvh_depth, min_leaf, ray_capacity)
+        self.h = C.c_void_p()
+        _check(lib().rtx_pipeline_create(ctx.h, scene.h, C.byref(p), int(depth), C.byref(self.h)),
+               "rtx_pipeline_create")
+
+    def set_animation(self, enable: bool, t0: float = 0.0, dt: float = 0.0):
+        _check(lib().rtx_pipeline_set_animation(self.h, int(bool(enable)), float(t0), float(dt)),
+               "rtx_pipeline_set_animation")
+
+    def frame(self, cam: Camera, rebuild: bool = True) -> int:
+        f = C.c_uint64(0)
+        _check(lib().rtx_pipeline_frame(self.h, C.byref(cam), int(bool(rebuild)), C.byref(f)),
+               "rtx_pipeline_frame")
+        return int(f.value)
+
+    def wait(self, frame: int):
+        _check(lib().rtx_pipeline_wait(self.h, int(frame)), "rtx_pipeline_wait")
+
+    def fb(self, frame: int) -> "FrameBuffer":
+        h = lib().rtx_pipeline_fb(self.h, int(frame))
+        if not h:
+            raise RtxError("rtx_pipeline_fb: frame not available")
+        return FrameBuffer(self.ctx, self.w, self.hgt, handle=h)
+
+    def stats(self, frame: int) -> FrameStats:
+        st = FrameStats()
+        _check(lib().rtx_pipeline_stats(self.h, int(frame), C.byref(st)), "rtx_pipeline_stats")
+        return st
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rtx_pipeline_destroy(self.h), "rtx_pipeline_destroy")
             self.h = C.c_void_p()
