@@ -162,7 +162,11 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    contiguous, screen-coherent chunk of the rays; 1 = plain
  *                    grid order);
  *   "coop_leaf":     leaves of at least this many triangles are tested by
- *                    the whole wave, 64 triangles per round (0 = never).
+ *                    the whole wave, 64 triangles per round (0 = never);
+ *   "side_stream":   1: a renderer traces each bounce's shadow rays and shades
+ *                    it on a second stream while the next bounce is generated
+ *                    and traced (frames then capture as forked graphs); 0: all
+ *                    stages on the caller's stream (linear graphs).
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */

@@ -268,6 +268,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "coop_leaf") {
         RTX_REQUIRE(value >= 0 && value <= (1 << 20), RTX_EINVAL, "rtx_tune_set: coop_leaf must be in [0, 2^20]");
         ctx->tune.coop_leaf = value;
+    } else if (k == "side_stream") {
+        RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: side_stream must be 0 or 1");
+        ctx->tune.side_stream = value;
     } else {
         rtx_set_error("rtx_tune_set: unknown key " + k);
         return RTX_EINVAL;
@@ -283,6 +286,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "xcds") return ctx->tune.xcds;
     if (k == "coop_leaf") return ctx->tune.coop_leaf;
     if (k == "refill") return ctx->tune.refill;
+    if (k == "side_stream") return ctx->tune.side_stream;
     return -1;
 }
 

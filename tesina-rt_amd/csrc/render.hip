@@ -282,7 +282,7 @@ static int32_t ensure_stream(hipStream_t* st, const char* what) {
 // reference sequence: shadow rays only set flag bits 8+ that ray generation
 // does not read, and shading is an order-free integer sum.
 static int32_t render_stages(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, hipStream_t st) {
-    const bool two = !r->single_stream;
+    const bool two = !r->single_stream && r->ctx->tune.side_stream != 0;
     int32_t e;
     if (two && (e = ensure_stream(&r->side, "side"))) return e;
     hipStream_t sd = two ? r->side : st;

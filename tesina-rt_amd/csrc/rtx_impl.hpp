@@ -68,6 +68,7 @@ struct rtx_tuning {
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
+    int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
 };
 
 struct rtx_ctx {

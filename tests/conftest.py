@@ -9,7 +9,7 @@ ROOT = Path(__file__).resolve().parent.parent
 # pipelines, batches and the server run several streams at once; each needs a
 # hardware queue of its own (HIP's default is 4 per process).  Set before any
 # HIP initialisation.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, str(ROOT / "tesina-rt_amd"))
 sys.path.insert(0, str(ROOT))
 
