@@ -402,16 +402,19 @@ int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
 /* ------------------------------------------------------- frame pipeline -- */
 /* Frames of one scene with `depth` (1..4) frames in flight: the reference's
  * RendererT (src/rt/renderer-threaded.cpp:452-506) overlaps the next build
- * with the last render; this goes further.  Frame N is rendered by slot
- * N mod depth (its own renderer, stream and render copy of the scene).  A
- * build stream animates (if enabled) and rebuilds the scene for frame N, then
- * copies it into slot N's render copy once frame N - depth has finished with
- * that copy; slot N's stream renders frame N once its copy is ready.  So
- * frame N+1's build overlaps frames N and N-1, and renders of consecutive
- * frames overlap each other.  Each frame's image equals the sequential run's.
- * The scene must not be driven by another renderer while the pipeline runs.
- * Each slot renders on one stream; concurrency needs a hardware queue per
- * active stream (GPU_MAX_HW_QUEUES >= depth + 2). */
+ * with the last render; this goes further.  Frame N uses slot N mod depth:
+ * its own renderer, stream, LBVH buffer set and copy of the normals and
+ * material ids.  A build stream runs, frame after frame, the animation (if
+ * enabled) and the sort phase of each build (codes, sort, the in-place
+ * permutation that chains one build to the next), then copies the normals
+ * and material ids; the slot's stream builds the tree into the slot's set and
+ * renders the frame from it.  A slot's set and copy are rewritten only once
+ * frame N - depth has finished with them.  So builds and renders of
+ * consecutive frames overlap.  Each frame's image equals the sequential
+ * run's.  The scene must not be driven by another renderer while the
+ * pipeline runs; afterwards its current buffer set holds the last frame's
+ * build.  Each slot renders on one stream; concurrency needs a hardware queue
+ * per stream (GPU_MAX_HW_QUEUES >= depth + 2; 16 is a good setting). */
 int32_t rtx_pipeline_create(rtx_ctx* ctx, rtx_scene* s, const rtx_render_params* p, int32_t depth,
                             rtx_pipeline** out);
 int32_t rtx_pipeline_destroy(rtx_pipeline* pl);

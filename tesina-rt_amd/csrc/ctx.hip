@@ -302,14 +302,12 @@ int32_t rtx_scene_create(rtx_ctx* ctx, rtx_scene** out) {
 // Device bytes held by a scene (buffers and BVH, build scratch included).
 uint64_t rtx_scene_device_bytes(const rtx_scene* sc) {
     rtx_scene* s = const_cast<rtx_scene*>(sc);
-    const DevBuf* all[] = {&s->pos,      &s->nrm,        &s->vnrm,     &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
-                     &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
-                     &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
-                     &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
-                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
-                     &s->hier_status, &s->waves,      &s->stage,    &s->clock};
+    const DevBuf* all[] = {&s->pos,        &s->nrm,  &s->vnrm,     &s->uv,   &s->idx,    &s->idx_alt, &s->matmap,
+                           &s->matmap_alt, &s->orig, &s->orig_alt, &s->mats, &s->lights, &s->waves,   &s->stage,
+                           &s->clock};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->p ? d->bytes : 0;
+    for (auto& set : s->sets) set.each([&](DevBuf& d) { b += d.p ? d.bytes : 0; });
     for (const auto& c : s->rc) b += c.node.bytes + c.delta.bytes + c.tri.bytes + c.nrm.bytes + c.matmap.bytes;
     return b;
 }
@@ -317,12 +315,10 @@ uint64_t rtx_scene_device_bytes(const rtx_scene* sc) {
 int32_t rtx_scene_destroy(rtx_scene* s) {
     RTX_REQUIRE(s, RTX_EINVAL, "rtx_scene_destroy: null scene");
     (void)hipStreamSynchronize(s->ctx->stream);
-    DevBuf* all[] = {&s->pos,      &s->nrm,        &s->vnrm,     &s->uv,        &s->idx,         &s->idx_alt,   &s->matmap,
-                     &s->matmap_alt, &s->orig,     &s->orig_alt,  &s->mats,        &s->lights,    &s->node,
-                     &s->delta,    &s->parent,     &s->tri,       &s->codes,       &s->perm,      &s->gbox,
-                     &s->levels,   &s->bstart,    &s->bend,        &s->keys_alt,
-                     &s->vals_alt, &s->centroid,   &s->partials,  &s->sort_hist,   &s->sort_status, &s->tickets, &s->fat,
-                     &s->hier_status, &s->waves,      &s->stage,    &s->clock};
+    DevBuf* all[] = {&s->pos,        &s->nrm,  &s->vnrm,     &s->uv,   &s->idx,    &s->idx_alt, &s->matmap,
+                     &s->matmap_alt, &s->orig, &s->orig_alt, &s->mats, &s->lights, &s->waves,   &s->stage,
+                     &s->clock};
+    for (auto& set : s->sets) set.each([](DevBuf& d) { d.release(); });
     for (auto& c : s->rc) {
         c.node.release();
         c.delta.release();
@@ -518,11 +514,11 @@ static int32_t readback(rtx_scene* s, const DevBuf& b, void* dst, size_t bytes, 
 
 int32_t rtx_read_codes(rtx_scene* s, uint32_t* codes, uint32_t T) {
     RTX_REQUIRE(s && s->built && T == s->T, RTX_EINVAL, "rtx_read_codes: no BVH or wrong T");
-    return readback(s, s->codes, codes, (size_t)T * 4, "rtx_read_codes");
+    return readback(s, s->b().codes, codes, (size_t)T * 4, "rtx_read_codes");
 }
 int32_t rtx_read_perm(rtx_scene* s, uint32_t* perm, uint32_t T) {
     RTX_REQUIRE(s && s->built && T == s->T, RTX_EINVAL, "rtx_read_perm: no BVH or wrong T");
-    return readback(s, s->perm, perm, (size_t)T * 4, "rtx_read_perm");
+    return readback(s, s->b().perm, perm, (size_t)T * 4, "rtx_read_perm");
 }
 int32_t rtx_read_indices(rtx_scene* s, uint32_t* idx3, uint32_t T) {
     RTX_REQUIRE(s && T == s->T, RTX_EINVAL, "rtx_read_indices: wrong T");
@@ -553,7 +549,7 @@ int32_t rtx_read_vertices(rtx_scene* s, float* pos3, float* nrm3, uint32_t V) {
 int32_t rtx_read_global_bbox(rtx_scene* s, float* lo3, float* hi3) {
     RTX_REQUIRE(s && s->built && lo3 && hi3, RTX_EINVAL, "rtx_read_global_bbox: no BVH");
     float g[6];
-    int32_t e = readback(s, s->gbox, g, sizeof g, "rtx_read_global_bbox");
+    int32_t e = readback(s, s->b().gbox, g, sizeof g, "rtx_read_global_bbox");
     if (e) return e;
     memcpy(lo3, g, 12);
     memcpy(hi3, g + 3, 12);

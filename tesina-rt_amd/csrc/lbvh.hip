@@ -666,33 +666,33 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
     uint32_t hier_tiles = rtx_div_up(T, BS);
     int32_t e = 0;
-    if ((e = s->node.reserve(c, ncap * 32 + 64))) return e;
-    if ((e = s->delta.reserve(c, ncap * 16))) return e;
-    if ((e = s->parent.reserve(c, ncap * 4))) return e;
-    if ((e = s->bstart.reserve(c, ncap * 4))) return e;
-    if ((e = s->bend.reserve(c, ncap * 4))) return e;
-    if ((e = s->tri.reserve(c, (size_t)T * 48))) return e;
-    if ((e = s->codes.reserve(c, (size_t)T * 4))) return e;
-    if ((e = s->perm.reserve(c, (size_t)T * 4))) return e;
-    if ((e = s->keys_alt.reserve(c, (size_t)T * 4))) return e;
-    if ((e = s->vals_alt.reserve(c, (size_t)T * 4))) return e;
-    if ((e = s->centroid.reserve(c, (size_t)T * 16))) return e;
-    if ((e = s->partials.reserve(c, BOUNDS_GRID * 6 * 4))) return e;
-    if ((e = s->gbox.reserve(c, 8 * 4))) return e;
-    if ((e = s->levels.reserve(c, LV_WORDS * 4))) return e;
-    if ((e = s->tickets.reserve(c, TK_COUNT * 4))) return e;
-    if ((e = s->sort_hist.reserve(c, 4 * 256 * 4))) return e;
-    if ((e = s->sort_status.reserve(c, (size_t)passes * sort_tiles * 256 * 4))) return e;
-    if ((e = s->hier_status.reserve(c, (size_t)(depth + 1) * hier_tiles * 4))) return e;
-    if ((e = s->fat.reserve(c, ((size_t)T / (FAT_LEAF + 1) + 1) * 4))) return e;
+    if ((e = s->b().node.reserve(c, ncap * 32 + 64))) return e;
+    if ((e = s->b().delta.reserve(c, ncap * 16))) return e;
+    if ((e = s->b().parent.reserve(c, ncap * 4))) return e;
+    if ((e = s->b().bstart.reserve(c, ncap * 4))) return e;
+    if ((e = s->b().bend.reserve(c, ncap * 4))) return e;
+    if ((e = s->b().tri.reserve(c, (size_t)T * 48))) return e;
+    if ((e = s->b().codes.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->b().perm.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->b().keys_alt.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->b().vals_alt.reserve(c, (size_t)T * 4))) return e;
+    if ((e = s->b().centroid.reserve(c, (size_t)T * 16))) return e;
+    if ((e = s->b().partials.reserve(c, BOUNDS_GRID * 6 * 4))) return e;
+    if ((e = s->b().gbox.reserve(c, 8 * 4))) return e;
+    if ((e = s->b().levels.reserve(c, LV_WORDS * 4))) return e;
+    if ((e = s->b().tickets.reserve(c, TK_COUNT * 4))) return e;
+    if ((e = s->b().sort_hist.reserve(c, 4 * 256 * 4))) return e;
+    if ((e = s->b().sort_status.reserve(c, (size_t)passes * sort_tiles * 256 * 4))) return e;
+    if ((e = s->b().hier_status.reserve(c, (size_t)(depth + 1) * hier_tiles * 4))) return e;
+    if ((e = s->b().fat.reserve(c, ((size_t)T / (FAT_LEAF + 1) + 1) * 4))) return e;
     return 0;
 }
 
 static void fat_leaf_bounds(rtx_scene* s, hipStream_t st) {
     ProfScope ps(s->ctx, st, "lbvh_fat_leaves");
     uint32_t g = rtx_div_up((uint64_t)s->T / (FAT_LEAF + 1) + 1, BS / RTX_WAVE);
-    hipLaunchKernelGGL(k_fat_leaves, dim3(g > 1024 ? 1024 : g), dim3(BS), 0, st, s->tri.as<float4>(), s->nodes(),
-                       s->fat.as<uint32_t>(), s->levels.as<uint32_t>());
+    hipLaunchKernelGGL(k_fat_leaves, dim3(g > 1024 ? 1024 : g), dim3(BS), 0, st, s->b().tri.as<float4>(), s->nodes(),
+                       s->b().fat.as<uint32_t>(), s->b().levels.as<uint32_t>());
 }
 
 // Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
@@ -705,36 +705,41 @@ static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
         if (maxn > 2ull * s->T) maxn = 2ull * s->T;
         uint32_t g = rtx_div_up(maxn, BS);
         hipLaunchKernelGGL(k_refit_level, dim3(g > gmax ? gmax : g), dim3(BS), 0, st, s->nodes(),
-                           s->delta.as<float4>(), s->levels.as<uint32_t>(), d);
+                           s->b().delta.as<float4>(), s->b().levels.as<uint32_t>(), d);
     }
     const int dhi = depth - 1 < REFIT_TOP ? depth - 1 : REFIT_TOP;
     if (dhi >= 0) {
         ProfScope ps(s->ctx, st, "lbvh_refit_top");
-        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(BS), 0, st, s->nodes(), s->delta.as<float4>(),
-                           s->levels.as<uint32_t>(), dhi);
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(BS), 0, st, s->nodes(), s->b().delta.as<float4>(),
+                           s->b().levels.as<uint32_t>(), dhi);
     }
 }
 
-int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
+// Depth and leaf size of a build, clamped as the reference's RendererConfig
+// does (src/rt/bvh-builder.cpp:194-199); codes are 32-bit here.
+static int32_t build_params(rtx_scene* s, const rtx_lbvh_params* p, int* depth_out, int* min_leaf_out) {
     RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_build: null scene");
     RTX_REQUIRE(s->T > 0 && s->V > 0, RTX_ESTATE, "rtx_lbvh_build: scene has no triangles");
     RTX_REQUIRE(s->T < (1u << 29), RTX_EUNSUPPORTED, "rtx_lbvh_build: more than 2^29 triangles");
     int depth = p ? p->depth : 30;
     int min_leaf = p ? p->min_leaf : 1;
-    // RendererConfig clamps (src/rt/bvh-builder.cpp:194-199); codes are 32-bit here.
     RTX_REQUIRE(depth <= 32, RTX_EUNSUPPORTED, "rtx_lbvh_build: bvh_depth > 32 needs 64-bit Morton keys");
-    depth = depth < 1 ? 1 : depth;
-    min_leaf = min_leaf < 1 ? 1 : (min_leaf > 256 ? 256 : min_leaf);
+    *depth_out = depth < 1 ? 1 : depth;
+    *min_leaf_out = min_leaf < 1 ? 1 : (min_leaf > 256 ? 256 : min_leaf);
+    return RTX_OK;
+}
+
+// Sort phase: build state reset, primitive and global bounds, Morton codes,
+// radix sort, and the in-place permutation of the scene's index, material
+// and id buffers with the triangle records gathered from the positions.  The
+// permutation is what chains one build to the next.
+static void build_sort_phase(rtx_scene* s, int depth, hipStream_t st) {
     const uint32_t T = s->T;
-    hipStream_t st = s->ctx->pick(stream);
-    int32_t e = lbvh_alloc(s, T, depth);
-    if (e) return e;
-    ProfScope whole(s->ctx, st, "LBVH_BUILD");
     const int passes = (depth + 7) / 8;
     const uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
     const uint32_t hier_tiles = rtx_div_up(T, BS);
-    uint32_t* tickets = s->tickets.as<uint32_t>();
-    uint32_t* levels = s->levels.as<uint32_t>();
+    uint32_t* tickets = s->b().tickets.as<uint32_t>();
+    uint32_t* levels = s->b().levels.as<uint32_t>();
     {
         ProfScope ps(s->ctx, st, "lbvh_init");
         uint32_t nb = (uint32_t)(passes * (size_t)sort_tiles * 256);
@@ -743,9 +748,9 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         grid = grid < 1 ? 1 : (grid > 2048 ? 2048 : grid);
         // zero: tickets + histograms (contiguous? no: separate buffers)
         hipLaunchKernelGGL(k_build_init, dim3(grid), dim3(BS), 0, st, tickets, (uint32_t)TK_COUNT,
-                           s->sort_status.as<uint32_t>(), nb, s->hier_status.as<uint32_t>(), nc, levels,
-                           s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->parent.as<uint32_t>(), T, 1);
-        hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, s->sort_hist.as<uint32_t>(), 4u * 256u,
+                           s->b().sort_status.as<uint32_t>(), nb, s->b().hier_status.as<uint32_t>(), nc, levels,
+                           s->b().bstart.as<uint32_t>(), s->b().bend.as<uint32_t>(), s->b().parent.as<uint32_t>(), T, 1);
+        hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, s->b().sort_hist.as<uint32_t>(), 4u * 256u,
                            (uint32_t*)nullptr, 0u, (uint32_t*)nullptr, 0u, levels, (uint32_t*)nullptr,
                            (uint32_t*)nullptr, (uint32_t*)nullptr, T, 0);
     }
@@ -754,33 +759,33 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > BOUNDS_GRID ? BOUNDS_GRID : grid;
         hipLaunchKernelGGL(k_prim_bounds, dim3(grid), dim3(BS), 0, st, s->idx.as<uint32_t>(), s->pos.as<float4>(),
-                           s->centroid.as<float4>(), s->partials.as<float>(), T);
-        hipLaunchKernelGGL(k_gbox_reduce, dim3(1), dim3(BS), 0, st, s->partials.as<float>(), grid,
-                           s->gbox.as<float>());
+                           s->b().centroid.as<float4>(), s->b().partials.as<float>(), T);
+        hipLaunchKernelGGL(k_gbox_reduce, dim3(1), dim3(BS), 0, st, s->b().partials.as<float>(), grid,
+                           s->b().gbox.as<float>());
     }
     {
         ProfScope ps(s->ctx, st, "lbvh_morton");
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > 1024 ? 1024 : grid;
-        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, s->centroid.as<float4>(), s->gbox.as<float>(),
-                           s->codes.as<uint32_t>(), s->perm.as<uint32_t>(), s->sort_hist.as<uint32_t>(), T, depth,
+        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, s->b().centroid.as<float4>(), s->b().gbox.as<float>(),
+                           s->b().codes.as<uint32_t>(), s->b().perm.as<uint32_t>(), s->b().sort_hist.as<uint32_t>(), T, depth,
                            passes);
     }
-    uint32_t *kin = s->codes.as<uint32_t>(), *vin = s->perm.as<uint32_t>();
-    uint32_t *kout = s->keys_alt.as<uint32_t>(), *vout = s->vals_alt.as<uint32_t>();
+    uint32_t *kin = s->b().codes.as<uint32_t>(), *vin = s->b().perm.as<uint32_t>();
+    uint32_t *kout = s->b().keys_alt.as<uint32_t>(), *vout = s->b().vals_alt.as<uint32_t>();
     for (int pass = 0; pass < passes; ++pass) {
         ProfScope ps(s->ctx, st, "lbvh_sort_pass");
         int shift = 8 * pass;
         int bits = depth - shift < 8 ? depth - shift : 8;
         hipLaunchKernelGGL(k_sort_pass, dim3(sort_tiles), dim3(BS), 0, st, kin, vin, kout, vout, T, shift, bits,
-                           s->sort_hist.as<uint32_t>() + 256 * pass,
-                           s->sort_status.as<uint32_t>() + (size_t)pass * sort_tiles * 256, tickets + TK_SORT + pass);
+                           s->b().sort_hist.as<uint32_t>() + 256 * pass,
+                           s->b().sort_status.as<uint32_t>() + (size_t)pass * sort_tiles * 256, tickets + TK_SORT + pass);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (kin != s->codes.as<uint32_t>()) {  // odd pass count: sorted data sits in the alternates
-        std::swap(s->codes, s->keys_alt);
-        std::swap(s->perm, s->vals_alt);
+    if (kin != s->b().codes.as<uint32_t>()) {  // odd pass count: sorted data sits in the alternates
+        std::swap(s->b().codes, s->b().keys_alt);
+        std::swap(s->b().perm, s->b().vals_alt);
     }
     {
         // permute in place: copy, then gather back into the scene's buffers
@@ -790,25 +795,36 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
         hipLaunchKernelGGL(k_stage_copy, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->idx.as<uint32_t>(),
                            s->idx_alt.as<uint32_t>(), s->matmap.as<int32_t>(), s->matmap_alt.as<int32_t>(),
                            s->orig.as<uint32_t>(), s->orig_alt.as<uint32_t>(), T);
-        hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->perm.as<uint32_t>(),
+        hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->b().perm.as<uint32_t>(),
                            s->idx_alt.as<uint32_t>(), s->idx.as<uint32_t>(), s->matmap_alt.as<int32_t>(),
                            s->matmap.as<int32_t>(), s->orig_alt.as<uint32_t>(), s->orig.as<uint32_t>(),
-                           s->pos.as<float4>(), s->tri.as<float4>(), T);
+                           s->pos.as<float4>(), s->b().tri.as<float4>(), T);
     }
+}
+
+// Tree phase: radix tree, level-ordered hierarchy, fat-leaf bounds and the
+// bottom-up refit.  It reads only the buffer set's own sorted codes and
+// triangle records, so it can run beside the next build's sort phase when
+// that one writes another set.
+static void build_tree_phase(rtx_scene* s, int depth, int min_leaf, hipStream_t st) {
+    const uint32_t T = s->T;
+    const uint32_t hier_tiles = rtx_div_up(T, BS);
+    uint32_t* tickets = s->b().tickets.as<uint32_t>();
+    uint32_t* levels = s->b().levels.as<uint32_t>();
     {
         ProfScope ps(s->ctx, st, "lbvh_radix_tree");
         // radix-tree arrays reuse the sort's alternate buffers
-        hipLaunchKernelGGL(k_radix_tree, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->codes.as<uint32_t>(),
-                           s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(), (int)T);
+        hipLaunchKernelGGL(k_radix_tree, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->b().codes.as<uint32_t>(),
+                           s->b().keys_alt.as<uint32_t>(), s->b().vals_alt.as<uint32_t>(), (int)T);
     }
     {
-        HierArgs A{s->codes.as<uint32_t>(), s->keys_alt.as<uint32_t>(), s->vals_alt.as<uint32_t>(),
-                   s->bstart.as<uint32_t>(), s->bend.as<uint32_t>(), s->nodes(), s->parent.as<uint32_t>(),
-                   s->tri.as<float4>(), levels, s->fat.as<uint32_t>(), T, min_leaf};
+        HierArgs A{s->b().codes.as<uint32_t>(), s->b().keys_alt.as<uint32_t>(), s->b().vals_alt.as<uint32_t>(),
+                   s->b().bstart.as<uint32_t>(), s->b().bend.as<uint32_t>(), s->nodes(), s->b().parent.as<uint32_t>(),
+                   s->b().tri.as<float4>(), levels, s->b().fat.as<uint32_t>(), T, min_leaf};
         const int dtop = depth < HIER_TOP ? depth : HIER_TOP;
         {
             ProfScope ps(s->ctx, st, "lbvh_hier_top");
-            hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, s->hier_status.as<uint32_t>(), hier_tiles,
+            hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, s->b().hier_status.as<uint32_t>(), hier_tiles,
                                dtop, depth);
         }
         const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
@@ -818,18 +834,50 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
             if (maxn > 2ull * T) maxn = 2ull * T;
             uint32_t grid = rtx_div_up(maxn, BS);
             hipLaunchKernelGGL(k_hier_level, dim3(grid > gmax ? gmax : grid), dim3(BS), 0, st, A,
-                               s->hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d,
+                               s->b().hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d,
                                d == depth ? 1 : 0);
         }
     }
     fat_leaf_bounds(s, st);
     refit_internal(s, st, depth);
+}
+
+int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
+    int depth = 30, min_leaf = 1;
+    int32_t e = build_params(s, p, &depth, &min_leaf);
+    if (e) return e;
+    hipStream_t st = s->ctx->pick(stream);
+    if ((e = lbvh_alloc(s, s->T, depth))) return e;
+    {
+        ProfScope whole(s->ctx, st, "LBVH_BUILD");
+        build_sort_phase(s, depth, st);
+        build_tree_phase(s, depth, min_leaf, st);
+    }
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
     s->depth = depth;
     s->min_leaf = min_leaf;
     return RTX_OK;
 }
+
+int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t sort_stream,
+                             hipEvent_t sorted, hipStream_t tree_stream) {
+    int depth = 30, min_leaf = 1;
+    int32_t e = build_params(s, p, &depth, &min_leaf);
+    if (e) return e;
+    if ((e = lbvh_alloc(s, s->T, depth))) return e;
+    build_sort_phase(s, depth, sort_stream);
+    RTX_CHECK_HIP(hipEventRecord(sorted, sort_stream));
+    RTX_CHECK_HIP(hipStreamWaitEvent(tree_stream, sorted, 0));
+    build_tree_phase(s, depth, min_leaf, tree_stream);
+    RTX_CHECK_HIP(hipGetLastError());
+    s->built = true;
+    s->depth = depth;
+    s->min_leaf = min_leaf;
+    return RTX_OK;
+}
+
+int32_t rtx_lbvh_reserve(rtx_scene* s, int depth) { return lbvh_alloc(s, s->T, depth); }
 
 int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
     RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_refit: null scene");
@@ -839,14 +887,14 @@ int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
     {
         ProfScope ps(s->ctx, st, "lbvh_tri_records");
         hipLaunchKernelGGL(k_tri_records, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->idx.as<uint32_t>(),
-                           s->pos.as<float4>(), s->tri.as<float4>(), T);
+                           s->pos.as<float4>(), s->b().tri.as<float4>(), T);
     }
     {
         ProfScope ps(s->ctx, st, "lbvh_leaf_bounds");
         uint32_t ncap = (uint32_t)(2 * (size_t)T);
         uint32_t g = rtx_div_up(ncap, BS);
-        hipLaunchKernelGGL(k_leaf_bounds, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->tri.as<float4>(),
-                           s->nodes(), s->levels.as<uint32_t>());
+        hipLaunchKernelGGL(k_leaf_bounds, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->b().tri.as<float4>(),
+                           s->nodes(), s->b().levels.as<uint32_t>());
     }
     fat_leaf_bounds(s, st);
     refit_internal(s, st, s->depth);
@@ -860,7 +908,7 @@ int32_t rtx_lbvh_info(rtx_scene* s, uint32_t* node_count, uint32_t* level_counts
     uint32_t lv[LV_WORDS];
     RTX_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
     RTX_CHECK_HIP(hipDeviceSynchronize());
-    RTX_CHECK_HIP(hipMemcpy(lv, s->levels.p, sizeof lv, hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(lv, s->b().levels.p, sizeof lv, hipMemcpyDeviceToHost));
     if (node_count) *node_count = lv[LV_TOTAL];
     if (level_counts64)
         for (int i = 0; i < 64; ++i) level_counts64[i] = (i < 63 && i < s->depth) ? lv[LV_CNT + i + 1] : 0u;
@@ -877,7 +925,7 @@ int32_t rtx_read_nodes_ref(rtx_scene* s, rtx_node_ref* out, uint32_t cap, uint32
     std::vector<float4> nd(2 * (size_t)N);
     std::vector<uint32_t> par(N);
     RTX_CHECK_HIP(hipMemcpy(nd.data(), s->nodes(), nd.size() * sizeof(float4), hipMemcpyDeviceToHost));
-    RTX_CHECK_HIP(hipMemcpy(par.data(), s->parent.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(par.data(), s->b().parent.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t n = 0; n < N; ++n) {
         const float4 a = nd[2 * (size_t)n], b = nd[2 * (size_t)n + 1];
         rtx_node_ref& o = out[n];

@@ -8,8 +8,10 @@ struct rtx_pipeline {
     rtx_lbvh_params bp{30, 1};
     std::vector<rtx_renderer*> r;     // slot renderers
     std::vector<hipStream_t> st;      // slot streams
-    std::vector<hipEvent_t> copied;   // slot's render copy holds its frame's build
-    std::vector<hipEvent_t> done;     // slot's frame finished (its copy is free again)
+    std::vector<hipEvent_t> sorted;   // slot's frame: sort phase done (its tree phase may start)
+    std::vector<hipEvent_t> copied;   // slot's shading copy holds its frame's normals and material ids
+    std::vector<hipEvent_t> done;     // slot's frame finished (its buffer set and copy are free again)
+    std::vector<int> reads;           // buffer set each slot's last frame renders from (-1: none yet)
     hipStream_t bst = nullptr;        // build stream
     bool animate = false;
     float t0 = 0.f, dt = 0.f;
@@ -38,18 +40,22 @@ int32_t rtx_pipeline_create(rtx_ctx* ctx, rtx_scene* s, const rtx_render_params*
         r->single_stream = true;
         pl->r.push_back(r);
         hipStream_t q = nullptr;
-        hipEvent_t a = nullptr, b = nullptr;
+        hipEvent_t a = nullptr, b = nullptr, c = nullptr;
         if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c, hipEventDisableTiming) != hipSuccess) {
             e = RTX_EDEVICE;
             if (q) (void)hipStreamDestroy(q);
             if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
             break;
         }
         pl->st.push_back(q);
         pl->copied.push_back(a);
         pl->done.push_back(b);
+        pl->sorted.push_back(c);
+        pl->reads.push_back(-1);
     }
     if (e) {
         if (e == RTX_EDEVICE) rtx_set_error("rtx_pipeline_create: stream or event creation failed");
@@ -66,6 +72,7 @@ int32_t rtx_pipeline_destroy(rtx_pipeline* pl) {
     for (auto q : pl->st) (void)hipStreamSynchronize(q);
     for (auto* r : pl->r) rtx_renderer_destroy(r);
     for (auto ev : pl->copied) (void)hipEventDestroy(ev);
+    for (auto ev : pl->sorted) (void)hipEventDestroy(ev);
     for (auto ev : pl->done) (void)hipEventDestroy(ev);
     for (auto q : pl->st) (void)hipStreamDestroy(q);
     if (pl->bst) (void)hipStreamDestroy(pl->bst);
@@ -81,29 +88,52 @@ int32_t rtx_pipeline_set_animation(rtx_pipeline* pl, int32_t enable, float t0, f
     return RTX_OK;
 }
 
+// Frame n uses slot k = n mod depth: buffer set k, shading copy k and stream
+// k.  The build stream runs, in frame order, each frame's animation and the
+// sort phase of its build (which permutes the scene's index buffers in place
+// and so chains the builds), then copies the normals and material ids the
+// frame shades with.  Slot k's stream runs the tree phase of the build into
+// set k and renders the frame from it.  Set k and copy k are rewritten only
+// after frame n - depth, their last user, is done.
 int32_t rtx_pipeline_frame(rtx_pipeline* pl, const rtx_camera* cam, int32_t rebuild, uint64_t* frame) {
     RTX_REQUIRE(pl && cam, RTX_EINVAL, "rtx_pipeline_frame: null argument");
     rtx_scene* s = pl->s;
     const uint64_t n = pl->next;
     const int k = (int)(n % (uint64_t)pl->depth);
-    int32_t e;
-    // build stream: this frame's scene, then its render copy once the frame
-    // that used the copy last (n - depth) is done
+    const bool build = rebuild || !s->built;
+    int32_t e = RTX_OK;
+    if (n == 0) {  // every slot's buffers, before frames run side by side (no allocation after this)
+        const int keep = s->cur;
+        for (int j = 0; j < pl->depth && !e; ++j) {
+            s->cur = j;
+            e = rtx_lbvh_reserve(s, pl->bp.depth);
+            if (!e) e = rtx_reserve_render_copy(s, j, true);
+        }
+        s->cur = keep;
+        if (e) return e;
+    }
     if (pl->animate && (e = rtx_scene_animate(s, pl->t0, pl->dt, pl->bst))) return e;
-    if ((rebuild || !s->built) && (e = rtx_lbvh_build(s, &pl->bp, pl->bst))) return e;
-    if (n == 0)  // every slot's copy, before frames run side by side (no allocation after this)
-        for (int j = 0; j < pl->depth; ++j)
-            if ((e = rtx_reserve_render_copy(s, j))) return e;
     if (n >= (uint64_t)pl->depth) RTX_CHECK_HIP(hipStreamWaitEvent(pl->bst, pl->done[k], 0));
-    if ((e = rtx_refresh_render_copy(s, k, pl->bst))) return e;
+    int set = s->cur;
+    if (build) {
+        // frames without a rebuild read the current set from their own slots
+        for (int j = 0; j < pl->depth; ++j)
+            if (j != k && pl->reads[j] == k) RTX_CHECK_HIP(hipStreamWaitEvent(pl->bst, pl->done[j], 0));
+        s->cur = k;
+        if ((e = rtx_lbvh_build_split(s, &pl->bp, pl->bst, pl->sorted[k], pl->st[k]))) return e;
+        set = k;
+    }
+    if ((e = rtx_refresh_render_copy(s, k, pl->bst, true))) return e;
     RTX_CHECK_HIP(hipEventRecord(pl->copied[k], pl->bst));
-    // slot stream: the frame, from copy k
     RTX_CHECK_HIP(hipStreamWaitEvent(pl->st[k], pl->copied[k], 0));
+    s->use_set = set;
     s->use_copy = k;
     e = rtx_render_frame_async(pl->r[k], s, cam, 0, pl->st[k]);
+    s->use_set = -1;
     s->use_copy = -1;
     if (e) return e;
     RTX_CHECK_HIP(hipEventRecord(pl->done[k], pl->st[k]));
+    pl->reads[k] = set;
     pl->next = n + 1;
     if (frame) *frame = n;
     return RTX_OK;

@@ -40,35 +40,39 @@ __global__ __launch_bounds__(256) void k_copy_segs(CopySegs c) {
 }  // namespace
 
 // Allocates render copy k (host-side; do it before frames run concurrently).
-int32_t rtx_reserve_render_copy(rtx_scene* s, int k) {
+int32_t rtx_reserve_render_copy(rtx_scene* s, int k, bool shading_only) {
     RTX_REQUIRE(k >= 0 && k < rtx_scene::COPIES, RTX_EINVAL, "render copy: bad copy index");
     rtx_scene::RenderCopy& rc = s->rc[k];
     int32_t e;
-    if ((e = rc.node.reserve(s->ctx, s->node.bytes)) || (e = rc.delta.reserve(s->ctx, s->delta.bytes)) ||
-        (e = rc.tri.reserve(s->ctx, (uint64_t)s->T * 48)) || (e = rc.nrm.reserve(s->ctx, (uint64_t)s->V * 16)) ||
-        (e = rc.matmap.reserve(s->ctx, (uint64_t)s->T * 4)))
+    if ((e = rc.nrm.reserve(s->ctx, (uint64_t)s->V * 16)) || (e = rc.matmap.reserve(s->ctx, (uint64_t)s->T * 4)))
+        return e;
+    if (shading_only) return RTX_OK;
+    if ((e = rc.node.reserve(s->ctx, s->b().node.bytes)) || (e = rc.delta.reserve(s->ctx, s->b().delta.bytes)) ||
+        (e = rc.tri.reserve(s->ctx, (uint64_t)s->T * 48)))
         return e;
     return RTX_OK;
 }
 
-// Makes render copy k of everything the render stages read, on `st`.
-int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st) {
+// Makes render copy k of what the render stages read, on `st` (shading only:
+// the normals and material ids).
+int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st, bool shading_only) {
     int32_t e;
     RTX_REQUIRE(k >= 0 && k < rtx_scene::COPIES, RTX_EINVAL, "render copy: bad copy index");
     rtx_scene::RenderCopy& rc = s->rc[k];
-    DevBuf* src[5] = {&s->node, &s->delta, &s->tri, &s->nrm, &s->matmap};
-    DevBuf* dst[5] = {&rc.node, &rc.delta, &rc.tri, &rc.nrm, &rc.matmap};
-    const uint64_t bytes[5] = {s->node.bytes, s->delta.bytes, (uint64_t)s->T * 48, (uint64_t)s->V * 16,
-                               (uint64_t)s->T * 4};
+    DevBuf* src[5] = {&s->nrm, &s->matmap, &s->b().node, &s->b().delta, &s->b().tri};
+    DevBuf* dst[5] = {&rc.nrm, &rc.matmap, &rc.node, &rc.delta, &rc.tri};
+    const uint64_t bytes[5] = {(uint64_t)s->V * 16, (uint64_t)s->T * 4, s->b().node.bytes, s->b().delta.bytes,
+                               (uint64_t)s->T * 48};
+    const int n = shading_only ? 2 : 5;
     CopySegs c{};
-    for (int k = 0; k < 5; ++k) {
-        RTX_REQUIRE(src[k]->p && src[k]->bytes >= bytes[k], RTX_ESTATE, "render copy: scene not built");
-        if ((e = dst[k]->reserve(s->ctx, bytes[k]))) return e;
-        c.dst[k] = dst[k]->as<uint32_t>();
-        c.src[k] = src[k]->as<uint32_t>();
-        c.n[k] = bytes[k] / 4;
+    for (int j = 0; j < n; ++j) {
+        RTX_REQUIRE(src[j]->p && src[j]->bytes >= bytes[j], RTX_ESTATE, "render copy: scene not built");
+        if ((e = dst[j]->reserve(s->ctx, bytes[j]))) return e;
+        c.dst[j] = dst[j]->as<uint32_t>();
+        c.src[j] = src[j]->as<uint32_t>();
+        c.n[j] = bytes[j] / 4;
     }
-    c.count = 5;
+    c.count = n;
     ProfScope ps(s->ctx, st, "render_copy");
     hipLaunchKernelGGL(k_copy_segs, dim3(s->ctx->cus * 4), dim3(256), 0, st, c);
     RTX_CHECK_HIP(hipGetLastError());

@@ -98,6 +98,33 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// One set of LBVH buffers: the build's outputs (nodes, near-child deltas,
+// triangle records, sorted codes) and its scratch.
+struct BuildSet {
+    DevBuf node, delta, parent, tri, codes, perm;
+    DevBuf gbox;        // 6 x int32 ordered-float keys
+    DevBuf levels;      // uint32: [0..64) level offsets, [64..128) level counts, [128] node count
+    // build scratch
+    DevBuf bstart, bend;          // per-node build state: triangle range
+    DevBuf keys_alt, vals_alt;    // radix ping-pong
+    DevBuf centroid, partials;    // per-triangle centroids, per-block AABB partials
+    DevBuf sort_hist;             // [4][256] digit histograms -> exclusive offsets
+    DevBuf sort_status;           // look-back words, per pass x tile x digit
+    DevBuf tickets;               // tile tickets for every ordered kernel
+    DevBuf hier_status;           // look-back words, per level x tile
+    DevBuf fat;                   // node ids of the leaves bounded by one wave each (kept for refit)
+    // Node i lives at byte 32 + 32 i of `node`: left children have odd ids, so
+    // every sibling pair (i, i + 1) fills one 64-byte aligned chunk.
+    float4* nodes() const { return reinterpret_cast<float4*>(static_cast<char*>(node.p) + 32); }
+    template <class F>
+    void each(F f) {
+        DevBuf* all[] = {&node,     &delta,    &parent,   &tri,       &codes,       &perm,    &gbox,
+                         &levels,   &bstart,   &bend,     &keys_alt,  &vals_alt,    &centroid, &partials,
+                         &sort_hist, &sort_status, &tickets, &hier_status, &fat};
+        for (DevBuf* d : all) f(*d);
+    }
+};
+
 struct rtx_scene {
     rtx_ctx* ctx = nullptr;
     uint32_t V = 0, T = 0, M = 0;
@@ -111,23 +138,8 @@ struct rtx_scene {
     // BVH
     bool built = false;
     int depth = 30, min_leaf = 1;
-    DevBuf node, delta, parent, tri, codes, perm;
-    // Node i lives at byte 32 + 32 i of `node`: left children have odd ids, so
-    // every sibling pair (i, i + 1) fills one 64-byte aligned chunk.
-    float4* nodes() const { return reinterpret_cast<float4*>(static_cast<char*>(node.p) + 32); }
-    DevBuf gbox;        // 6 x int32 ordered-float keys
-    DevBuf levels;      // uint32: [0..64) level offsets, [64..128) level counts, [128] node count
-    // build scratch
-    DevBuf bstart, bend;          // per-node build state: triangle range
-    DevBuf keys_alt, vals_alt;    // radix ping-pong
-    DevBuf centroid, partials;    // per-triangle centroids, per-block AABB partials
-    DevBuf sort_hist;             // [4][256] digit histograms -> exclusive offsets
-    DevBuf sort_status;           // look-back words, per pass x tile x digit
-    DevBuf tickets;               // tile tickets for every ordered kernel
-    DevBuf hier_status;           // look-back words, per level x tile
-    DevBuf fat;                   // node ids of the leaves bounded by one wave each (kept for refit)
     // Render copies of what the render stages read (pipelined frames: the
-    // next frame's build writes the buffers above while frames in flight
+    // next frame's build writes the build buffers while frames in flight
     // render from copies).  use_copy selects one for the launches being
     // enqueued (-1: the build's own buffers).
     struct RenderCopy {
@@ -136,6 +148,19 @@ struct rtx_scene {
     static constexpr int COPIES = 4;
     RenderCopy rc[COPIES];
     int use_copy = -1;
+    // Frames in flight: the launches being enqueued read the nodes, deltas
+    // and triangle records of set use_set, and the normals and material ids
+    // of copy use_copy (-1: none).
+    int use_set = -1;
+    // LBVH buffer sets: builds write, and frames read, set `cur`.  Frames in
+    // flight give every slot a set of its own, so that the tree stages of one
+    // frame's build run while the next frame sorts (only the permutation of
+    // the index buffer chains builds).
+    BuildSet sets[COPIES];
+    int cur = 0;
+    BuildSet& b() { return sets[cur]; }
+    const BuildSet& b() const { return sets[cur]; }
+    float4* nodes() const { return sets[cur].nodes(); }
 };
 
 // Buffers the render stages (trace, shadow, secondary generation, shading)
@@ -148,12 +173,19 @@ struct SceneView {
     const int32_t* matmap;
 };
 inline SceneView scene_view(const rtx_scene* s) {
+    if (s->use_set >= 0) {
+        const BuildSet& b = s->sets[s->use_set];
+        const bool c = s->use_copy >= 0;
+        return SceneView{b.nodes(), b.delta.as<float4>(), b.tri.as<float4>(),
+                         c ? s->rc[s->use_copy].nrm.as<float4>() : s->nrm.as<float4>(),
+                         c ? s->rc[s->use_copy].matmap.as<int32_t>() : s->matmap.as<int32_t>()};
+    }
     if (s->use_copy >= 0) {
         const rtx_scene::RenderCopy& c = s->rc[s->use_copy];
         return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(c.node.p) + 32),
                          c.delta.as<float4>(), c.tri.as<float4>(), c.nrm.as<float4>(), c.matmap.as<int32_t>()};
     }
-    return SceneView{s->nodes(), s->delta.as<float4>(), s->tri.as<float4>(), s->nrm.as<float4>(),
+    return SceneView{s->nodes(), s->b().delta.as<float4>(), s->b().tri.as<float4>(), s->nrm.as<float4>(),
                      s->matmap.as<int32_t>()};
 }
 
@@ -230,8 +262,17 @@ int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint
 int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* const* hits, const uint64_t* counts,
                                int nseg, uint64_t* dev_shadow_count, hipStream_t st);
 // Makes render copy k of a scene (what the render stages read), on `st`.
-int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st);
-int32_t rtx_reserve_render_copy(rtx_scene* s, int k);
+// Render copy k of what the render stages read: all of it, or (shading
+// only) the normals and material ids, whose buffers the next frame's
+// animation and permutation overwrite in place.
+int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st, bool shading_only = false);
+int32_t rtx_reserve_render_copy(rtx_scene* s, int k, bool shading_only = false);
+// Build into set s->cur with the sort phase on sort_stream and the tree phase
+// on tree_stream (which waits for `sorted`, recorded after the sort phase).
+int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t sort_stream, hipEvent_t sorted,
+                             hipStream_t tree_stream);
+// Allocates set s->cur for builds of `depth` levels (host side).
+int32_t rtx_lbvh_reserve(rtx_scene* s, int depth);
 int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
                                   uint64_t* dev_total_acc, hipStream_t st);
 

@@ -463,7 +463,9 @@ __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__
                 }
             }
             if (pop) {
-                if (level == 0)
+                // the reference pops only below MAX_LEVELS and ends the ray
+                // otherwise (src/kernel/trace-bvh.cl:361-367)
+                if (level == 0 || level >= STACK)
                     going = false;
                 else
                     curr = stk.get(--level);
@@ -511,7 +513,9 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
                 }
             }
             if (pop) {
-                if (level == 0)
+                // the reference pops only below MAX_LEVELS and ends the ray
+                // otherwise (src/kernel/trace-bvh.cl:361-367)
+                if (level == 0 || level >= STACK)
                     going = false;
                 else
                     curr = stk.get(--level);
@@ -797,7 +801,7 @@ __device__ __forceinline__ bool step_closest(const float4* __restrict__ node, co
             return false;
         }
     }
-    if (level == 0) return true;
+    if (level == 0 || level >= STACK) return true;  // no pop at MAX_LEVELS (src/kernel/trace-bvh.cl:361-367)
     curr = stk.get(--level);
     return false;
 }
@@ -827,7 +831,7 @@ __device__ __forceinline__ int step_any(const float4* __restrict__ node, const f
             return 0;
         }
     }
-    if (level == 0) return 2;
+    if (level == 0 || level >= STACK) return 2;  // no pop at MAX_LEVELS
     curr = stk.get(--level);
     return 0;
 }
@@ -1241,6 +1245,15 @@ int32_t rtx_raygen_primary(rtx_rays* rays, const rtx_camera* cam, int32_t width,
     return RTX_OK;
 }
 
+// The tuned traversal form.  The sibling-pair form pushes only children
+// whose boxes pass, so it cannot follow the reference's stack count, which
+// ends a ray at a pop with MAX_LEVELS entries; that count can only reach
+// MAX_LEVELS in trees of 32 levels, which therefore use the single-node form.
+static int traversal_variant(const rtx_scene* s) {
+    const int var = (int)s->ctx->tune.trace_variant;
+    return (var == 2 && s->depth >= STACK) ? 1 : var;
+}
+
 int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
     RTX_REQUIRE(s && rays && hits, RTX_EINVAL, "rtx_trace: null argument");
     RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_trace: scene has no BVH (call rtx_lbvh_build)");
@@ -1253,7 +1266,7 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());  // hit count, queue
     ProfScope ps(s->ctx, st, "trace");
     const SceneView sv = scene_view(s);
-    const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+    const int var = traversal_variant(s), tb = (int)s->ctx->tune.trace_block;
     if (var == 3) {
 #define RTX_TRACE_P(B)                                                                                              \
     hipLaunchKernelGGL((k_trace_p<B>), dim3(persist_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,            \
@@ -1327,7 +1340,7 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     ProfScope ps(s->ctx, st, "trace_shadow");
     const SceneView sv = scene_view(s);
     uint64_t n = max_items * (uint64_t)nl;
-    const int var = (int)s->ctx->tune.trace_variant, tb = (int)s->ctx->tune.trace_block;
+    const int var = traversal_variant(s), tb = (int)s->ctx->tune.trace_block;
     if (var == 3) {
         // queue counter: word 2 of the first segment's hit-list counters
         uint32_t* queue = hits[0]->list_count.as<uint32_t>() + 2;

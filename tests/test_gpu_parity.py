@@ -396,19 +396,21 @@ def test_graph_replay_matches_oracle(ctx, c1):
     s.destroy()
 
 
-@pytest.mark.parametrize("variant,refill", [(3, 1), (3, 16), (3, 64), (1, 16)])
-def test_frame_and_graph_per_variant(ctx, variant, refill):
+@pytest.mark.parametrize("variant,refill,side", [(3, 1, 1), (3, 16, 1), (3, 64, 1), (1, 16, 1), (1, 16, 0)])
+def test_frame_and_graph_per_variant(ctx, variant, refill, side):
     # whole frames (async and captured) under each traversal form, including
-    # the persistent ray-queue form at its extreme refill thresholds
+    # the persistent ray-queue form at its extreme refill thresholds, and
+    # with every stage on one stream (side = 0: linear graphs)
     rtx = _rtx()
     from rtx import scenes
     sd = scenes.config_c1()
     s = scenes.load(rtx.Scene(ctx), sd)
     w, h = 160, 120
     cam = rtx.camera_set(sd.camera["pos"], sd.camera["look_at"], sd.camera["up"], sd.camera["fov"], w / h)
-    old = (ctx.tuned("trace_variant"), ctx.tuned("refill"))
+    old = (ctx.tuned("trace_variant"), ctx.tuned("refill"), ctx.tuned("side_stream"))
     ctx.tune("trace_variant", variant)
     ctx.tune("refill", refill)
+    ctx.tune("side_stream", side)
     try:
         R = rtx.Renderer(ctx, w, h, 2)
         R.frame_async(s, cam, rebuild=True)
@@ -431,4 +433,5 @@ def test_frame_and_graph_per_variant(ctx, variant, refill):
     finally:
         ctx.tune("trace_variant", old[0])
         ctx.tune("refill", old[1])
+        ctx.tune("side_stream", old[2])
         s.destroy()
