@@ -1,0 +1,83 @@
+"""GPU parity of every traversal form on trees of every shape the build makes.
+
+The C1 tests trace one scene; here the trees are one leaf (T <= 3), trees
+of 1, 2, 9, 12, 30 and 32 levels with leaves at odd and even depths, larger
+leaves (min_leaf 4 and 8), duplicated triangles (runs of equal codes: fat
+leaves, tested by the whole wave) and a flat axis.  Random rays from inside
+and around the scene give long, incoherent traversals.  Hits and shadow
+flags of each form must equal the oracle's bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_gpu_parity import load_scene, random_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _rays(seed, n, extent=12.0):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-extent, extent, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ori4 = np.zeros((n, 4), np.float32)
+    dir4 = np.zeros((n, 4), np.float32)
+    ori4[:, :3] = o
+    ori4[:, 3] = 0.0001
+    dir4[:, :3] = d.astype(np.float32)
+    dir4[:, 3] = 1e36
+    return ori4, dir4
+
+
+def _lights():
+    from rtx import make_lights, LIGHT_POINT, LIGHT_DIR
+    return make_lights((0.1, 0.1, 0.1), [dict(type=LIGHT_POINT, pos=(3.0, 25.0, -4.0), color=(1.0, 1.0, 1.0)),
+                                         dict(type=LIGHT_DIR, dir=(0.3, -1.0, 0.2), color=(0.5, 0.5, 0.5))])
+
+
+CASES = [  # T, depth, min_leaf, seed, kind
+    (1, 30, 1, 1, "random"), (2, 30, 1, 2, "random"), (3, 30, 1, 3, "random"), (7, 30, 1, 4, "random"),
+    (300, 30, 1, 5, "random"), (5000, 30, 1, 6, "random"), (5000, 32, 1, 7, "random"), (5000, 9, 1, 8, "random"),
+    (5000, 2, 1, 9, "random"), (5000, 1, 1, 10, "random"), (20000, 30, 4, 11, "random"),
+    (20000, 12, 8, 12, "random"), (6000, 30, 1, 13, "duplicates"), (6000, 30, 1, 14, "flat"),
+]
+
+
+@pytest.mark.parametrize("T,depth,min_leaf,seed,kind", CASES)
+def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
+    import rtx
+    if kind == "flat":
+        pos, idx, nrm, uv, mm = random_scene(seed, T, flat_axis=1)
+    else:
+        pos, idx, nrm, uv, mm = random_scene(seed, T)
+        if kind == "duplicates":  # every triangle three times: runs of equal codes, fat leaves
+            idx = np.repeat(idx[: T // 3], 3, axis=0)
+            mm = np.repeat(mm[: T // 3], 3)
+    lights = _lights()
+    s = load_scene(ctx, pos, idx, nrm, uv, mm, lights=lights)
+    s.build(depth, min_leaf)
+    b = O.Build(pos, idx, mm, depth, min_leaf)
+    n = 4096
+    ori4, dir4 = _rays(100 + seed, n)
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    o = O.trace(b, nrm, uv, ori4, dir4)
+    o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
+    if T >= 300:
+        assert (o["id"] >= 0).sum() > n // 8, "the rays should hit the scene"
+    old = ctx.tuned("trace_variant")
+    try:
+        for variant in (0, 1, 2):
+            ctx.tune("trace_variant", variant)
+            rtx.trace(s, r, hb, n)
+            rtx.trace_shadow(s, r, hb, n)
+            g = hb.read(n)
+            for k in ("flags", "id", "t", "bary", "n4", "p4"):
+                assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant}, T {T}, depth {depth})"
+    finally:
+        ctx.tune("trace_variant", old)
+    r.destroy()
+    hb.destroy()
+    s.destroy()
