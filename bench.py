@@ -245,13 +245,16 @@ def main():
         plain.append(fs.frame_ms)
     plain_ms = float(np.median(plain))
 
-    # per-kernel times with HIP events on the stream the kernels run on
+    # per-kernel times with HIP events on the stream the kernels run on, over
+    # frames launched on one stream (no kernel of the frame overlaps another)
+    ctx.tune("side_stream", 0)
     ctx.profile(True)
     ctx.profile_reset()
     for _ in range(args.profile_frames):
         R.frame_async(s, cam, rebuild=True)
     rep = ctx.profile_report()
     ctx.profile(False)
+    ctx.tune("side_stream", 1)
     nf = args.profile_frames
     kernels = {k: {"launches_per_frame": c / nf, "ms_per_frame": ms / nf, "avg_launch_ms": ms / c}
                for k, (c, ms) in sorted(rep.items())}
@@ -312,6 +315,7 @@ def main():
         "plain_path_mrays": round(rays_per_frame / plain_ms / 1e3, 3),
         "roofline": rl_trace,
         "roofline_sort_pass": rl_sort,
+        "build_stages": build_stage_rooflines(kernels, T, s.info()[0]),
         "traversal": {
             "trace_ms_per_frame": round(trace_ms, 4),
             "shadow_ms_per_frame": round(kernels.get("trace_shadow", {}).get("ms_per_frame", 0.0), 4),
@@ -352,6 +356,34 @@ def main():
         s.destroy()
     ctx.close()
     print(json.dumps(out))
+
+
+def build_stage_rooflines(kernels, T, nodes):
+    """HBM roofline of the build's streaming stages, with the algorithmic
+    bytes of the reference's stage definitions: prim AABB + Morton 76 B per
+    triangle (12 B indices, 36 B positions, 24 B box, 4 B code); the
+    permutation of the index buffer and material map 2 x (12 + 4) + 4 = 36 B
+    per triangle; the refit 28 T + 32 N_L + 80 N_I bytes (per triangle 4 B id
+    + 24 B box, per leaf 8 B range + 24 B box, per inner node two 24 B child
+    boxes, 8 B ids and a 24 B box).  This build's stages move more in places
+    (centroids, triangle records gathered with the permutation); the bytes
+    are the reference's, the time is this build's."""
+    ni = (nodes - 1) // 2
+    nl = nodes - ni
+    stages = {
+        "prim_bounds_morton": (("lbvh_prim_bounds", "lbvh_morton"), 76.0 * T),
+        "permutation": (("lbvh_rearrange",), 36.0 * T),
+        "refit": (("lbvh_refit_level", "lbvh_refit_top", "lbvh_fat_leaves"), 28.0 * T + 32.0 * nl + 80.0 * ni),
+        "sort_all_passes": (("lbvh_sort_pass",), 16.0 * T * 4),
+    }
+    out = {}
+    for name, (keys, algo) in stages.items():
+        ms = sum(kernels.get(k, {}).get("ms_per_frame", 0.0) for k in keys)
+        gbs = algo / (ms / 1e3) / 1e9 if ms > 0 else None
+        out[name] = {"ms_per_build": round(ms, 4), "algorithmic_bytes": int(algo),
+                     "achieved_GBs": round(gbs, 1) if gbs else None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "kernels": list(keys)}
+    return out
 
 
 def bench_c3(ctx, frames: int):
