@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from chain import chain_scene
 from test_gpu_parity import load_scene, random_scene
 
 pytestmark = pytest.mark.gpu
@@ -78,6 +79,59 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
                 assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant}, T {T}, depth {depth})"
     finally:
         ctx.tune("trace_variant", old)
+    r.destroy()
+    hb.destroy()
+    s.destroy()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_max_levels_rule(ctx, variant):
+    # A ray whose near children all lie on the chain has 32 entries on its
+    # stack at the bottom leaf; the reference then ends the ray instead of
+    # popping (src/kernel/trace-bvh.cl:361-367, shadow-trace-bvh.cl alike).
+    # Rays aimed at the slivers with positive directions take that path, so
+    # for some of them the reference (and the oracle) report no hit where a
+    # brute-force search finds one.  Every traversal form must agree.
+    import rtx
+    from rtx import make_lights, LIGHT_POINT
+    pos, idx = chain_scene()
+    T = len(idx)
+    b = O.Build(pos, idx, None, 32, 1)
+    assert (b.level_counts[:32] > 0).all(), "the chain should have 32 inner levels"
+    rng = np.random.default_rng(5)
+    cent = pos[idx].mean(axis=1)
+    n = 4096
+    tgt = cent[4 + rng.integers(0, T - 4, n)]
+    d = np.abs(rng.normal(size=(n, 3))) + 0.2
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ori4 = np.zeros((n, 4), np.float32)
+    dir4 = np.zeros((n, 4), np.float32)
+    ori4[:, :3] = tgt - 2.0 * d
+    ori4[:, 3] = 0.0001
+    dir4[:, :3] = d
+    dir4[:, 3] = 1e36
+    nrm = np.tile(np.float32([0.0, 0.0, 1.0]), (len(pos), 1))
+    uv = np.zeros((len(pos), 2), np.float32)
+    lights = make_lights((0.1, 0.1, 0.1), [dict(type=LIGHT_POINT, pos=(3.0, 3.0, 3.0), color=(1.0, 1.0, 1.0))])
+    o = O.trace(b, nrm, uv, ori4, dir4)
+    bid, _ = O.trace_brute(b, ori4, dir4)
+    assert ((o["id"] >= 0) != (bid >= 0)).sum() > 100, "the rule should decide some rays"
+    o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
+    s = load_scene(ctx, pos, idx, nrm, uv, np.zeros(T, np.int32), lights=lights)
+    s.build(32, 1)
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    old = ctx.tuned("trace_variant")
+    try:
+        ctx.tune("trace_variant", variant)
+        rtx.trace(s, r, hb, n)
+        rtx.trace_shadow(s, r, hb, n)
+        g = hb.read(n)
+    finally:
+        ctx.tune("trace_variant", old)
+    for k in ("flags", "id", "t", "bary", "n4", "p4"):
+        assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant})"
     r.destroy()
     hb.destroy()
     s.destroy()

@@ -413,3 +413,37 @@ def test_resolve_closed_form():
     # 0 -> 0; 0.5*255 = 127.5 -> 128 (nearest even); 1 -> 255; > 1 clamps;
     # 1.960784 clamps to 255; 1e-6 * 255 -> 0
     assert out.tolist() == [[0, 128, 255, 255], [255, 255, 0, 255]]
+
+
+def test_traversal_ends_at_max_levels():
+    # trace_ray pops only while fewer than MAX_LEVELS = 32 entries are on its
+    # stack and otherwise returns what it has (src/kernel/trace-bvh.cl:361-367).
+    # On a chain of 32 inner levels, rays whose near children all lie on the
+    # chain reach the bottom leaf with 32 entries: the oracle must then report
+    # no hit although a brute-force search finds one (a pop there would find
+    # it), and a hit it reports is never closer than the brute-force one (a ray
+    # that ends early keeps what it found so far).
+    from chain import chain_scene
+    pos, idx = chain_scene()
+    b = O.Build(pos, idx, None, 32, 1)
+    assert (b.level_counts[:32] > 0).all()
+    rng = np.random.default_rng(5)
+    cent = pos[idx].mean(axis=1)
+    n = 2048
+    tgt = cent[4 + rng.integers(0, len(idx) - 4, n)]
+    d = np.abs(rng.normal(size=(n, 3))) + 0.2
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ori4 = np.zeros((n, 4), np.float32)
+    dir4 = np.zeros((n, 4), np.float32)
+    ori4[:, :3] = tgt - 2.0 * d
+    ori4[:, 3] = 0.0001
+    dir4[:, :3] = d
+    dir4[:, 3] = 1e36
+    o = O.trace(b, np.zeros_like(pos), np.zeros((len(pos), 2), np.float32), ori4, dir4)
+    bid, bt = O.trace_brute(b, ori4, dir4)
+    ended = (o["id"] < 0) & (bid >= 0)
+    assert ended.sum() > 50
+    both = o["id"] >= 0
+    assert (o["t"][both] >= bt[both]).all()
+    assert (o["t"][both] > bt[both]).any(), "some rays should end with a farther hit"
+
