@@ -193,26 +193,41 @@ def main():
     R.set_pipelined(False)
 
     # frames in flight (frame pipeline): the next frames' builds overlap the
-    # renders of the frames before, which overlap each other
-    flights = {}
-    for depth in (2, 3, 4):
+    # renders of the frames before, which overlap each other.  The traversal
+    # knobs are tuned on this mode first (every setting gives the same
+    # images): work-group size and cooperative-leaf size, untimed short runs.
+    def flight_ms(depth, steps):
         pl = rtx.Pipeline(ctx, s, sd.width, sd.height, sd.max_bounces, depth=depth)
-        for _ in range(args.warmup + depth):
+        for _ in range(depth + 2):
             pl.frame(cam, rebuild=True)
         ctx.device_sync()
         t0 = time.perf_counter()
         last = 0
-        for _ in range(args.steps):
+        for _ in range(steps):
             last = pl.frame(cam, rebuild=True)
         pl.wait(last)
         ctx.device_sync()
         dt = time.perf_counter() - t0
         stf = pl.stats(last)
+        pl.destroy()
         assert stf.primary_rays + stf.secondary_rays == rays_per_frame, "pipeline frame differs in ray count"
-        flights[depth] = dt / args.steps * 1e3
+        return dt / steps * 1e3
+
+    tune_runs = {}
+    for tb, coop in ((128, 8), (64, 8), (64, 4), (128, 4)):
+        ctx.tune("trace_block", tb)
+        ctx.tune("coop_leaf", coop)
+        tune_runs[(tb, coop)] = flight_ms(3, 10)
+    tb, coop = min(tune_runs, key=tune_runs.get)
+    ctx.tune("trace_block", tb)
+    ctx.tune("coop_leaf", coop)
+    log(f"frame pipeline tuning: {', '.join(f'block {k[0]} coop {k[1]}: {v:.3f} ms' for k, v in tune_runs.items())}"
+        f"; chosen block {tb}, coop {coop}")
+    flights = {}
+    for depth in (2, 3, 4):
+        flights[depth] = flight_ms(depth, args.steps)
         log(f"frame pipeline, {depth} frames in flight: {flights[depth]:.3f} ms/frame, "
             f"{rays_per_frame / flights[depth] / 1e3:.1f} Mrays/s")
-        pl.destroy()
     # the headline is the fastest way of running the same frames (each timed
     # frame does one full LBVH rebuild and one full render in every mode)
     best_depth = min(flights, key=flights.get)
@@ -223,7 +238,8 @@ def main():
                                                  "stage, shadows and shading of each bounce on a side stream"),
         "frames_in_flight": (flights[best_depth], f"frame pipeline with {best_depth} frames in flight (stream "
                                                   "launches): the next frames' LBVH rebuilds and the renders of "
-                                                  "consecutive frames overlap"),
+                                                  f"consecutive frames overlap; traversal work-group {tb}, "
+                                                  f"cooperative leaves from {coop} triangles"),
     }
     best_mode = min(modes, key=lambda k: modes[k][0])
     ms_per_step = modes[best_mode][0]

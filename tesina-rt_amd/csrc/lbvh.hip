@@ -165,6 +165,16 @@ __global__ __launch_bounds__(BS) void k_gbox_reduce(const float* __restrict__ pa
 // fmax(0, fmin(1, s)), scaled by 2^ceil(depth/3), truncated; bit i of the
 // code takes axis (off+1+i)%3, source bit (i+off)/3.  Also accumulates the
 // 8-bit digit histograms of every radix pass.
+// Bit k of v (k < 11) to bit 3k.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x7ffu;
+    x = (x | (x << 16)) & 0x00000000ff0000ffull;
+    x = (x | (x << 8)) & 0x000000f00f00f00full;
+    x = (x | (x << 4)) & 0x00000c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x0000249249249249ull;
+    return x;
+}
+
 __global__ __launch_bounds__(BS) void k_morton(const float4* __restrict__ cen, const float* __restrict__ gbox6,
                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                uint32_t* __restrict__ hist, uint32_t T, int depth, int passes) {
@@ -175,24 +185,39 @@ __global__ __launch_bounds__(BS) void k_morton(const float4* __restrict__ cen, c
     v3 ghi = mk3(gbox6[3], gbox6[4], gbox6[5]);
     v3 inv = recip(ghi - glo);
     float scale = (float)(1u << ((depth + 2) / 3));
-    int offset = depth % 3 ? 3 - (depth % 3) : 0;
-    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < T; i += gridDim.x * BS) {
-        v3 c = xyz(cen[i]);
-        v3 s = (c - glo) * inv;
-        s = vmax(mk3(0.f, 0.f, 0.f), vmin(mk3(1.f, 1.f, 1.f), s));
-        s = s * scale;
-        uint32_t m[3] = {(uint32_t)s.x, (uint32_t)s.y, (uint32_t)s.z};
-        int axis = (offset + 1) % 3;
+    const int offset = depth % 3 ? 3 - (depth % 3) : 0;
+    const uint64_t dmask = depth >= 64 ? ~0ull : ((1ull << depth) - 1ull);
+    const uint32_t lane = threadIdx.x & 63u;
+    // all lanes of a wave step together (the histogram is aggregated per wave)
+    for (uint32_t base = blockIdx.x * BS; base < T; base += gridDim.x * BS) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < T;
         uint32_t code = 0;
-        for (int b = 0; b < depth; ++b) {
-            uint32_t src = (uint32_t)((b + offset) / 3);
-            uint32_t mv = axis == 0 ? m[0] : (axis == 1 ? m[1] : m[2]);
-            code |= ((mv >> src) & 1u) << b;
-            axis = axis == 2 ? 0 : axis + 1;
+        if (active) {
+            v3 c = xyz(cen[i]);
+            v3 sl = (c - glo) * inv;
+            sl = vmax(mk3(0.f, 0.f, 0.f), vmin(mk3(1.f, 1.f, 1.f), sl));
+            sl = sl * scale;
+            // bit b of the code is bit (b + offset) / 3 of axis (offset + 1 + b) % 3:
+            // interleave y, z, x at bits 3k, 3k + 1, 3k + 2, shift by offset, keep depth bits
+            const uint64_t f = spread3((uint32_t)sl.y) | (spread3((uint32_t)sl.z) << 1) |
+                               (spread3((uint32_t)sl.x) << 2);
+            code = (uint32_t)((f >> offset) & dmask);
+            keys[i] = code;
+            vals[i] = i;
         }
-        keys[i] = code;
-        vals[i] = i;
-        for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p][(code >> (8 * p)) & 0xffu], 1u);
+        const uint64_t act = ballot(active);
+        const uint32_t lead = (uint32_t)__builtin_ctzll(act);
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t d = (code >> (8 * p)) & 0xffu;
+            const uint32_t d0 = __shfl(d, (int)lead, RTX_WAVE);
+            // input in the last build's order is nearly sorted: whole waves share a digit
+            if (ballot(active && d == d0) == act) {
+                if (lane == lead) atomicAdd(&s_h[p][d0], (uint32_t)__builtin_popcountll(act));
+            } else if (active) {
+                atomicAdd(&s_h[p][d], 1u);
+            }
+        }
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p) {
