@@ -500,7 +500,7 @@ def c5_scenes(seed: int = 0x5EED0005, n_scenes: int = 12, total_tris: int = 20_0
     return out
 
 
-def bench_c5(ctx, rates, max_requests: int, workers: int = 4):
+def bench_c5(ctx, rates, max_requests: int, workers: int = 12):
     """C5 serving: 12 resident scenes (~20M triangles), Poisson arrivals per
     rate, Zipf(1.1) scene popularity, resolution uniform over 256^2, 512^2 and
     1280x720, depth uniform over 1..3, camera on a sphere of 2.5x the scene
@@ -571,7 +571,7 @@ def bench_c5(ctx, rates, max_requests: int, workers: int = 4):
     return res
 
 
-def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 4):
+def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 8):
     """C4 batched: 64 scenes, T ~ logU[10k, 2M] (sphere sets and
     heightfields), resolution U{256^2, 512^2, 1280x720, 1920x1080}, depth
     U{1..4}; one frame of every scene per batch, rendered side by side on
@@ -659,7 +659,7 @@ def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 4):
     ctx.tune("coop_leaf", 8)
     return {"workload": "C4 batched: 64 resident scenes, T ~ logU[10k, 2M] (sphere sets and heightfields), "
                         "resolution U{256^2, 512^2, 1280x720, 1920x1080}, depth U{1..4}, one frame of each per "
-                        "batch, 4 streams, one hipGraph",
+                        f"batch, {streams} streams (stream launches or one hipGraph, the faster)",
             "scenes": n_scenes, "rays_per_batch": rays, "shadow_rays_per_batch": shadow,
             "ms_per_batch": round(dt_best * 1e3, 3), "mrays": round(rays / dt_best / 1e6, 2),
             "mode": "stream launches" if dt_eager <= dt_graph else "hipGraph replay",
