@@ -245,3 +245,70 @@ def hit_t_f64(b: Build, ori4, dir4, ids, t32):
                                 P(np.ascontiguousarray(dir4, np.float32)), P(np.ascontiguousarray(ids, np.int32)),
                                 P(np.ascontiguousarray(t32, np.float32)), C.c_uint64(n), P(t64))
     return float(worst), t64
+
+
+# ---------------------------------------------------------------- multi-root
+ROOT_DTYPE = np.dtype([("node", "<i4"), ("tr", "<f4", 16), ("tr_inv", "<f4", 16)])
+assert ROOT_DTYPE.itemsize == 132
+
+
+class MultiScene:
+    """Meshes in one shared node array, index buffer and vertex buffer, as
+    Scene::create_bvhs lays them out (src/rt/scene.cpp:644-700): every new
+    mesh, in the order the instances first use it, gets the next node, triangle
+    and vertex offsets; roots give a mesh's first node and the instance's
+    transforms (object to world `tr`, row-major 4x4, and its inverse)."""
+
+    def __init__(self, builds, nrms, uvs, instances):
+        # builds: per mesh O.Build; instances: [(mesh, tr 4x4, tr_inv 4x4)]
+        order = []
+        for m, _, _ in instances:
+            if m not in order:
+                order.append(m)
+        node_off, tri_off, vtx_off = {}, {}, {}
+        nodes, idx, pos, nrm, uv = [], [], [], [], []
+        no = to = vo = 0
+        for m in order:
+            b = builds[m]
+            node_off[m], tri_off[m], vtx_off[m] = no, to, vo
+            nd = b.nodes.copy()
+            leaf = nd["leaf"] != 0
+            nd["a"] = np.where(leaf, nd["a"] + to, nd["a"] + no)
+            nd["b"] = np.where(leaf, nd["b"] + to, nd["b"] + no)
+            nd["parent"] = nd["parent"] + no
+            nodes.append(nd)
+            idx.append(b.idx + vo)
+            pos.append(b.pos4)
+            nrm.append(pos4(np.asarray(nrms[m], np.float32)))
+            uv.append(np.ascontiguousarray(uvs[m], np.float32))
+            no += len(nd)
+            to += b.T
+            vo += b.pos4.shape[0]
+        self.nodes = np.ascontiguousarray(np.concatenate(nodes))
+        self.idx = np.ascontiguousarray(np.concatenate(idx), np.uint32)
+        self.pos4 = np.ascontiguousarray(np.concatenate(pos), np.float32)
+        self.nrm4 = np.ascontiguousarray(np.concatenate(nrm), np.float32)
+        self.uv = np.ascontiguousarray(np.concatenate(uv), np.float32)
+        self.tri_base = tri_off
+        self.roots = np.zeros(len(instances), ROOT_DTYPE)
+        for i, (m, tr, tri) in enumerate(instances):
+            self.roots[i]["node"] = node_off[m]
+            self.roots[i]["tr"] = np.asarray(tr, np.float32).reshape(16)
+            self.roots[i]["tr_inv"] = np.asarray(tri, np.float32).reshape(16)
+
+
+def trace_multi(ms: MultiScene, ori4, dir4):
+    n = ori4.shape[0]
+    h = empty_hits(n)
+    lib().orc_trace_multi(P(ms.pos4), P(ms.nrm4), P(ms.uv), P(ms.idx), P(ms.nodes), P(ms.roots),
+                          C.c_int(len(ms.roots)), P(np.ascontiguousarray(ori4, np.float32)),
+                          P(np.ascontiguousarray(dir4, np.float32)), C.c_uint64(n), P(h["flags"]), P(h["t"]),
+                          P(h["id"]), P(h["bary"]), P(h["n4"]), P(h["p4"]), P(h["uv2"]))
+    return h
+
+
+def trace_shadow_multi(ms: MultiScene, lights: np.ndarray, flags, p4):
+    f = np.ascontiguousarray(flags, np.uint32).copy()
+    lib().orc_trace_shadow_multi(P(ms.pos4), P(ms.idx), P(ms.nodes), P(ms.roots), C.c_int(len(ms.roots)),
+                                 P(lights), C.c_uint64(f.size), P(f), P(np.ascontiguousarray(p4, np.float32)))
+    return f

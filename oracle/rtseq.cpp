@@ -539,12 +539,13 @@ struct Hit {
 
 // trace_ray: src/kernel/trace-bvh.cl:330-417.  Stack of 32, near child first
 // by dot(dir, r.lo - l.lo) > 0 (the adir branches all test that one scalar).
-Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit) {
-    // nvisit (optional): [0] nodes visited, [1] triangles tested
+Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit,
+              uint32_t root = 0) {
+    // nvisit (optional): [0] nodes visited, [1] triangles tested; root: bvh_root
     Hit best{-1, r.tmax, 0.f, 0.f};
     float tmax = r.tmax;
     uint32_t stack[32];
-    uint32_t level = 0, curr = 0;
+    uint32_t level = 0, curr = root;
     for (;;) {
         const orc_node& n = nodes[curr];
         if (nvisit) ++*nvisit;
@@ -584,9 +585,10 @@ Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_nod
 
 // trace_shadow_ray: src/kernel/shadow-trace-bvh.cl:238-315 with leaf_hit_any
 // (:188-234): any hit with tMin < t < tMax.
-bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit) {
+bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit,
+                  uint32_t root = 0) {
     uint32_t stack[32];
-    uint32_t level = 0, curr = 0;
+    uint32_t level = 0, curr = root;
     for (;;) {
         const orc_node& n = nodes[curr];
         bool pop;
@@ -1038,6 +1040,149 @@ double orc_hit_t_f64(const float* pos, const uint32_t* idx, const float* ori4, c
         worst = std::max(worst, err);
     }
     return worst;
+}
+
+// ------------------------------------------------------- multi-root -----
+// BVHRoot (include/rt/multi-bvh.hpp:9-14): a root node of the shared node
+// array and the object's transform tr (object to world) and its inverse,
+// row-major 4x4 (sqmat4: four float4 rows).
+struct orc_root {
+    int32_t node;
+    float tr[16];
+    float tr_inv[16];
+};
+
+// OpenCL dot(float4, float4), defined as the sequential sum.
+static inline float dot4(const float* a, const float* b) {
+    return ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3];
+}
+// multiply_vector / multiply_point: the rows of M dotted with (v, 0) or
+// (v, 1), then a division by w when |w| exceeds `wmin`: 1e-26 in
+// src/kernel/trace-bvh.cl:83-113, 0.00001 in src/kernel/shadow-trace-bvh.cl:107-143.
+static f3 mul_h(const float* M, f3 v, float vw, float wmin) {
+    const float x[4] = {v.x, v.y, v.z, vw};
+    float r[4];
+    for (int i = 0; i < 4; ++i) r[i] = dot4(M + 4 * i, x);
+    if (std::fabs(r[3]) > wmin) {
+        r[0] = r[0] / r[3];
+        r[1] = r[1] / r[3];
+        r[2] = r[2] / r[3];
+    }
+    return mk(r[0], r[1], r[2]);
+}
+// transform_ray (src/kernel/trace-bvh.cl:115-121): direction and origin into
+// the object's frame, tMin / tMax unchanged (an affine map keeps the ray
+// parameter of every point).
+static Ray transform_ray(const Ray& r, const float* M, float wmin) {
+    Ray o = r;
+    o.dir = mul_h(M, r.dir, 0.f, wmin);
+    o.ori = mul_h(M, r.ori, 1.f, wmin);
+    o.inv = recip(o.dir);
+    return o;
+}
+
+// trace_multi + transform_hit_info + complete_transformed_hit_info
+// (src/kernel/trace-bvh.cl:123-183, 419-463): every root traced from the
+// ray's own tMax; a root's hit point is taken back to the world and its t is
+// the world distance from the origin; the first root with the smallest t
+// wins.  The normal is the interpolated object normal times tr (not its
+// inverse transpose), flipped when it faces the ray.  Hit ids index the
+// shared index buffer.
+void orc_trace_multi(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx,
+                     const orc_node* nodes, const orc_root* roots, int nroots, const float* ori4,
+                     const float* dir4, uint64_t count, uint32_t* flags, float* t, int32_t* id, float* bary2,
+                     float* n4, float* p4, float* uv2) {
+    for (uint64_t k = 0; k < count; ++k) {
+        const Ray r = load_ray(ori4, dir4, k);
+        Hit best{-1, 0.f, 0.f, 0.f};
+        int best_root = -1;
+        for (int i = 0; i < nroots; ++i) {
+            const Ray tr = transform_ray(r, roots[i].tr_inv, 1e-26f);
+            Hit h = trace_ray(tr, pos, idx, nodes, nullptr, (uint32_t)roots[i].node);
+            if (h.id < 0) continue;
+            const f3 hp = mul_h(roots[i].tr, tr.ori + tr.dir * h.t, 1.f, 1e-26f);
+            const f3 dv = hp - r.ori;
+            h.t = std::sqrt(dot(dv, dv));
+            if (best.id < 0 || h.t < best.t) {
+                best = h;
+                best_root = i;
+            }
+        }
+        if (best.id < 0) {
+            flags[k] = 0;
+            t[k] = 0.f;
+            id[k] = -1;
+            if (bary2) { bary2[2 * k] = 0.f; bary2[2 * k + 1] = 0.f; }
+            for (int c = 0; c < 4; ++c) { n4[4 * k + c] = 0.f; p4[4 * k + c] = 0.f; }
+            uv2[2 * k] = uv2[2 * k + 1] = 0.f;
+            continue;
+        }
+        const f3 hp = r.ori + r.dir * best.t;
+        uint32_t i0 = idx[3 * (size_t)best.id], i1 = idx[3 * (size_t)best.id + 1], i2 = idx[3 * (size_t)best.id + 2];
+        float u = best.u, v = best.v, w = 1.f - (u + v);
+        f3 n0 = normalize(ld3(nrm + 4 * (size_t)i0));
+        f3 n1 = normalize(ld3(nrm + 4 * (size_t)i1));
+        f3 n2 = normalize(ld3(nrm + 4 * (size_t)i2));
+        f3 n = mul_h(roots[best_root].tr, normalize(w * n0 + v * n2 + u * n1), 0.f, 1e-26f);
+        uint32_t fl = ORC_HIT;
+        if (dot(n, r.dir) > 0.f) {
+            fl |= ORC_INV_N;
+            n = -n;
+        }
+        float tu0 = uvs[2 * (size_t)i0], tv0 = uvs[2 * (size_t)i0 + 1];
+        float tu1 = uvs[2 * (size_t)i1], tv1 = uvs[2 * (size_t)i1 + 1];
+        float tu2 = uvs[2 * (size_t)i2], tv2 = uvs[2 * (size_t)i2 + 1];
+        flags[k] = fl;
+        t[k] = best.t;
+        id[k] = best.id;
+        if (bary2) { bary2[2 * k] = u; bary2[2 * k + 1] = v; }
+        n4[4 * k] = n.x; n4[4 * k + 1] = n.y; n4[4 * k + 2] = n.z; n4[4 * k + 3] = 0.f;
+        p4[4 * k] = hp.x; p4[4 * k + 1] = hp.y; p4[4 * k + 2] = hp.z; p4[4 * k + 3] = 0.f;
+        uv2[2 * k] = w * tu0 + v * tu2 + u * tu1;
+        uv2[2 * k + 1] = w * tv0 + v * tv2 + u * tv1;
+    }
+}
+
+// shadow_trace_multi (src/kernel/shadow-trace-bvh.cl:317-373): from the hit
+// point toward each light, tMin 0.01, tMax 1e37; shadowed when any root has
+// a hit.  A spot light is tested as that kernel does: a point with
+// dot(toward the light, spot dir) < angle is lit, not traced.  Point lights
+// (the lights extension) stop at the light, as in the single-root kernel.
+void orc_trace_shadow_multi(const float* pos, const uint32_t* idx, const orc_node* nodes,
+                            const orc_root* roots, int nroots, const orc_lights* lights, uint64_t count,
+                            uint32_t* flags, const float* p4) {
+    for (uint64_t k = 0; k < count; ++k) {
+        uint32_t fl = flags[k] & 0xffu;
+        if (fl & ORC_HIT) {
+            const f3 p = ld3(p4 + 4 * k);
+            for (int l = 0; l < lights->count; ++l) {
+                const orc_light& L = lights->l[l];
+                Ray r;
+                r.ori = p;
+                r.tmin = 0.01f;
+                r.tmax = 1e37f;
+                if (L.type == ORC_DIR) {
+                    r.dir = -ld3(L.dir);
+                } else if (L.type == ORC_SPOT) {
+                    r.dir = normalize(ld3(L.pos) - p);
+                    if (dot(r.dir, ld3(L.dir)) < L.angle) continue;
+                } else if (L.type == ORC_POINT) {
+                    const f3 toL = ld3(L.pos) - p;
+                    r.tmax = std::sqrt(dot(toL, toL));
+                    r.dir = normalize(toL);
+                } else {
+                    continue;
+                }
+                r.inv = recip(r.dir);
+                bool hit = false;
+                for (int i = 0; i < nroots && !hit; ++i)
+                    hit = trace_shadow(transform_ray(r, roots[i].tr_inv, 0.00001f), pos, idx, nodes, nullptr,
+                                       (uint32_t)roots[i].node);
+                if (hit) fl |= 1u << (8 + l);
+            }
+        }
+        flags[k] = fl;
+    }
 }
 
 int orc_abi_version(void) { return 1; }
