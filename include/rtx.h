@@ -44,6 +44,7 @@ typedef struct rtx_renderer rtx_renderer;
 typedef struct rtx_server rtx_server;
 typedef struct rtx_batch rtx_batch;
 typedef struct rtx_pipeline rtx_pipeline;
+typedef struct rtx_instances rtx_instances;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -272,6 +273,38 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
 /* Tracer::shadow_trace (src/rt/tracer.cpp:273-352;
  * src/kernel/shadow-trace-bvh.cl:160-418): one shadow ray per hit and light. */
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+
+/* ------------------------------------------------------------ instances -- */
+/* Objects of a scene over per-mesh BVHs (Scene::create_bvhs and
+ * Scene::update_bvh_roots, src/rt/scene.cpp:596-700; BVHRoot / MultiBVH,
+ * include/rt/multi-bvh.hpp:9-19): instance i is built scene scenes[i] under
+ * the row-major 4x4 object-to-world transform tr[16i..16i+15] and its
+ * inverse tr_inv.  A scene may back several instances.  Triangle ids are
+ * shared: the distinct scenes in order of first use take consecutive blocks
+ * (rtx_instances_tri_base gives each instance's first id).  The scenes'
+ * current BVHs are used at each trace call, so a scene may be rebuilt (its
+ * geometry animated) between calls; one stream at a time uses a set. */
+#define RTX_MAX_INSTANCES 4096
+int32_t rtx_instances_create(rtx_ctx* ctx, rtx_scene* const* scenes, const float* tr, const float* tr_inv, int32_t n,
+                             rtx_instances** out);
+int32_t rtx_instances_destroy(rtx_instances* in);
+/* New transforms for every instance (Object moves + update_bvh_roots); the
+ * next trace call on a stream ships them, in that stream's order. */
+int32_t rtx_instances_set_transforms(rtx_instances* in, const float* tr, const float* tr_inv);
+/* Lights of the shadow rays (Scene::set_dir_light / set_spot_light). */
+int32_t rtx_instances_set_lights(rtx_instances* in, const rtx_lights* lights);
+int32_t rtx_instances_tri_base(rtx_instances* in, uint32_t* base);
+/* Tracer::trace with the multi-root kernel (src/rt/tracer.cpp:97, 223;
+ * trace_multi + complete_transformed_hit_info, src/kernel/trace-bvh.cl:123-183,
+ * 419-463): every root traced from the ray's tMax in its own frame; t is the
+ * world distance to the nearest hit, the first root wins ties; the normal is
+ * tr times the interpolated object normal, facing the ray. */
+int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+/* Tracer::shadow_trace with shadow_trace_multi (src/rt/tracer.cpp:301, 363;
+ * src/kernel/shadow-trace-bvh.cl:317-373): one ray per hit and light, tMin
+ * 0.01, tested against every root; a spot light lights, untraced, a point
+ * with dot(toward the light, spot dir) < angle, as that kernel does. */
+int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
 
 /* SecondaryRayGenerator::generate, scan variant (src/rt/secondary-ray-generator.cpp:57-145,
  * 248-340; src/kernel/secondary-ray-generator.cl:43-181): Whitted children in

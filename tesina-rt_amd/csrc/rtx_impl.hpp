@@ -208,6 +208,36 @@ struct rtx_hits {
     uint64_t list_for = 0;       // ...of this many rays (or RTX_DEVICE_COUNT)
 };
 
+// One root of a multi-root traversal: a built scene's buffers as the
+// traversal reads them, where its triangles start in the shared numbering,
+// and the object-to-world transform with its inverse (row-major 4x4, the
+// BVHRoot of include/rt/multi-bvh.hpp:9-14).  176 bytes.
+struct InstRoot {
+    const float4* node;
+    const float4* delta;
+    const float4* tri;
+    const float4* nrm;
+    const float2* uv;
+    uint32_t base;
+    uint32_t ntri;
+    float tr[16];
+    float tr_inv[16];
+};
+
+// Objects of a scene (Scene::create_bvhs / update_bvh_roots,
+// src/rt/scene.cpp:596-700): instances of built scenes under transforms.
+// `table` is the host image of the device root table; `shipped` is what the
+// device holds (rewritten chunk by chunk, in stream order, when they differ).
+struct rtx_instances {
+    rtx_ctx* ctx = nullptr;
+    int n = 0;
+    std::vector<rtx_scene*> scene;
+    std::vector<float> tr, tr_inv;  // 16 floats per instance
+    std::vector<InstRoot> table, shipped;
+    DevBuf roots;
+    rtx_lights lights{};
+};
+
 struct rtx_fb {
     rtx_ctx* ctx = nullptr;
     int w = 0, h = 0;

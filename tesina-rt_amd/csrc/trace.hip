@@ -12,6 +12,7 @@
 // is what differs: 32-byte nodes plus a 16-byte delta fetched only at
 // internal nodes, triangles stored contiguously in leaf order, the stack in
 // LDS ([level][thread], conflict-free), SoA rays and hits.
+#include <algorithm>
 #include <cstring>
 
 #include "devmath.hpp"
@@ -1067,6 +1068,194 @@ uint32_t coop_threshold(const rtx_ctx* c) {
     return c->tune.coop_leaf > 0 ? (uint32_t)c->tune.coop_leaf : 0xffffffffu;
 }
 
+// ------------------------------------------------------------ instances --
+// Multi-root traversal (the reference's trace_multi / shadow_trace_multi,
+// src/kernel/trace-bvh.cl:419-463, src/kernel/shadow-trace-bvh.cl:317-373):
+// every instance is a built scene with an object-to-world transform `tr` and
+// its inverse, held in an InstRoot.
+
+// multiply_point / multiply_vector: rows dotted with (v, 1) or (v, 0), then
+// a division by w when |w| > wmin (1e-26 in trace-bvh.cl:83-113, 0.00001 in
+// shadow-trace-bvh.cl:107-143).  The dot is the sequential sum.
+__device__ __forceinline__ v3 mul_h(const float* __restrict__ M, v3 v, float vw, float wmin) {
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = ((M[4 * i] * v.x + M[4 * i + 1] * v.y) + M[4 * i + 2] * v.z) + M[4 * i + 3] * vw;
+    if (fabsf(r[3]) > wmin) {
+        r[0] = r[0] / r[3];
+        r[1] = r[1] / r[3];
+        r[2] = r[2] / r[3];
+    }
+    return mk3(r[0], r[1], r[2]);
+}
+
+// Closest hit over the roots: each root traced from the ray's own tMax in
+// its frame; the root's hit point taken back to the world gives t (its
+// distance from the origin); the first root with the smallest t wins.  The
+// record is completed as complete_transformed_hit_info does: hit point
+// along the world ray, normal = tr x the interpolated object normal, turned
+// to face the ray.  Ids are in the shared numbering.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ roots, int nroots,
+                                                    const float4* __restrict__ ori4, const float4* __restrict__ dir4,
+                                                    const uint32_t* __restrict__ dcount, uint64_t hcount,
+                                                    uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                                    int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                                    float4* __restrict__ n4, float4* __restrict__ p4,
+                                                    float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
+                                                    uint32_t* __restrict__ hit_count, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const uint64_t n = ray_count(dcount, hcount);
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
+        const uint64_t rr = base + threadIdx.x;
+        const bool active = rr < n;
+        v3 o = mk3(0.f, 0.f, 0.f), d = o;
+        float tmin = 0.f, tmax = 0.f;
+        if (active) {
+            const float4 o4 = ori4[rr], d4 = dir4[rr];
+            o = xyz(o4);
+            d = xyz(d4);
+            tmin = o4.w;
+            tmax = d4.w;
+        }
+        float bt = 0.f, bu = 0.f, bv = 0.f;
+        int32_t bid = -1, broot = 0;
+        for (int i = 0; i < nroots; ++i) {  // wave-uniform
+            const InstRoot& R = roots[i];
+            const v3 to = mul_h(R.tr_inv, o, 1.f, 1e-26f), td = mul_h(R.tr_inv, d, 0.f, 1e-26f);
+            const RayC rc = make_ray(to, td, tmin);
+            Closest c{tmax, 0.f, 0.f, -1};
+            closest_var<VAR>(R.node, R.delta, R.tri, rc, c, stk, coop, active);  // whole wave
+            if (active && c.id >= 0) {
+                const v3 hw = mul_h(R.tr, to + td * c.t, 1.f, 1e-26f);
+                const v3 dv = hw - o;
+                const float tw = __builtin_sqrtf(dot(dv, dv));
+                if (bid < 0 || tw < bt) {
+                    bt = tw;
+                    bid = c.id;
+                    bu = c.u;
+                    bv = c.v;
+                    broot = i;
+                }
+            }
+        }
+        const bool hit = active && bid >= 0;
+        const uint64_t hb = ballot(hit);
+        uint32_t wbase = 0;
+        if (hb) {
+            if (lane == (uint32_t)__builtin_ctzll(hb)) wbase = atomicAdd(hit_count, (uint32_t)__builtin_popcountll(hb));
+            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+        }
+        if (!active) continue;
+        if (!hit) {
+            write_miss_record(rr, flags, tout, idout, bary, n4, p4, uv2);
+            continue;
+        }
+        hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
+        const InstRoot& R = roots[broot];
+        const v3 hp = o + d * bt;
+        const uint32_t i0 = __float_as_uint(R.tri[3 * (size_t)bid].w);
+        const uint32_t i1 = __float_as_uint(R.tri[3 * (size_t)bid + 1].w);
+        const uint32_t i2 = __float_as_uint(R.tri[3 * (size_t)bid + 2].w);
+        const float w = 1.f - (bu + bv);
+        const v3 n0 = xyz(R.nrm[i0]), n1 = xyz(R.nrm[i1]), n2 = xyz(R.nrm[i2]);
+        v3 nn = mul_h(R.tr, normalize(w * n0 + bv * n2 + bu * n1), 0.f, 1e-26f);
+        uint32_t fl = 1u;
+        if (dot(nn, d) > 0.f) {
+            fl |= 2u;
+            nn = -nn;
+        }
+        const float2 t0 = R.uv[i0], t1 = R.uv[i1], t2 = R.uv[i2];
+        flags[rr] = fl;
+        tout[rr] = bt;
+        idout[rr] = bid + (int32_t)R.base;
+        bary[rr] = make_float2(bu, bv);
+        n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
+        p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
+        uv2[rr] = make_float2(w * t0.x + bv * t2.x + bu * t1.x, w * t0.y + bv * t2.y + bu * t1.y);
+    }
+}
+
+// Shadow rays of every (hit, light) pair of a hit list against every root:
+// DIR toward -dir; SPOT toward pos, lit without a trace when dot(toward the
+// light, spot dir) < angle (that kernel's test); POINT toward pos with tMax =
+// distance (the lights extension); any hit in any root sets bit 8 + l.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__ roots, int nroots,
+                                                     const rtx_lights lights,
+                                                     const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ list_count,
+                                                     uint32_t* __restrict__ flags, const float4* __restrict__ p4,
+                                                     unsigned long long* shadow_count, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_cnt;
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const int nl = lights.count;
+    const uint64_t items = (uint64_t)(*list_count) * (uint64_t)nl;
+    uint32_t traced = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * TB; base < items; base += (uint64_t)gridDim.x * TB) {
+        const uint64_t it = base + threadIdx.x;
+        const bool valid = it < items;
+        uint32_t rr = 0;
+        int l = 0;
+        v3 p = mk3(0.f, 0.f, 0.f), dir = p;
+        float tmax = 1e37f;
+        bool trace = false;
+        if (valid) {
+            rr = list[it / (uint64_t)nl];
+            l = (int)(it % (uint64_t)nl);
+            const rtx_light& L = lights.l[l];
+            p = xyz(p4[rr]);
+            trace = true;
+            if (L.type == RTX_LIGHT_DIR) {
+                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+            } else if (L.type == RTX_LIGHT_SPOT) {
+                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+                if (dot(dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) trace = false;
+            } else if (L.type == RTX_LIGHT_POINT) {
+                const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+                tmax = __builtin_sqrtf(dot(toL, toL));
+                dir = normalize(toL);
+            } else {
+                trace = false;
+            }
+            if (trace) ++traced;
+        }
+        bool shadowed = false;
+        for (int i = 0; i < nroots; ++i) {  // wave-uniform
+            const InstRoot& R = roots[i];
+            const bool go = trace && !shadowed;
+            RayC rc{};
+            if (go) rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
+            if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go)) shadowed = true;  // whole wave
+        }
+        if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
+    }
+    if (shadow_count) {
+        if (traced) atomicAdd(&s_cnt, traced);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_cnt) atomicAdd(shadow_count, (unsigned long long)s_cnt);
+    }
+}
+
+// Rewrites roots [first, first + n) of the device table from kernel
+// arguments: stream-ordered, and the host image may change right after the
+// launch.
+constexpr int ROOT_CHUNK = 16;
+struct RootChunk {
+    InstRoot r[ROOT_CHUNK];
+};
+__global__ void k_put_roots(InstRoot* __restrict__ table, uint32_t first, uint32_t n, const RootChunk c) {
+    const uint32_t words = n * (uint32_t)(sizeof(InstRoot) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(c.r);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(table + first);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
 // Traversal grid: one thread per item up to 64 groups per CU (then
 // grid-stride), rounded up to a multiple of the XCD count for xcd_span.
 uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
@@ -1381,4 +1570,184 @@ int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
     RTX_REQUIRE(s, RTX_EINVAL, "rtx_trace_shadow: null scene");
     return rtx_trace_shadow_impl(s, rays, hits, count, nullptr, s->ctx->pick(stream));
+}
+
+// ---------------------------------------------------------- instances --
+static_assert(sizeof(InstRoot) == 176, "InstRoot layout");
+static_assert(sizeof(RootChunk) + 16 <= 4096, "root chunk exceeds the kernel argument space");
+
+int32_t rtx_instances_create(rtx_ctx* ctx, rtx_scene* const* scenes, const float* tr, const float* tr_inv, int32_t n,
+                             rtx_instances** out) {
+    RTX_REQUIRE(ctx && scenes && tr && tr_inv && out, RTX_EINVAL, "rtx_instances_create: null argument");
+    RTX_REQUIRE(n > 0 && n <= RTX_MAX_INSTANCES, RTX_EINVAL, "rtx_instances_create: instance count out of range");
+    for (int i = 0; i < n; ++i)
+        RTX_REQUIRE(scenes[i] && scenes[i]->ctx == ctx, RTX_EINVAL,
+                    "rtx_instances_create: null scene or scene of another context");
+    rtx_instances* in = new rtx_instances();
+    in->ctx = ctx;
+    in->n = n;
+    in->scene.assign(scenes, scenes + n);
+    in->tr.assign(tr, tr + 16 * (size_t)n);
+    in->tr_inv.assign(tr_inv, tr_inv + 16 * (size_t)n);
+    int32_t e = in->roots.reserve(ctx, sizeof(InstRoot) * (size_t)n);
+    if (e) {
+        delete in;
+        return e;
+    }
+    *out = in;
+    return RTX_OK;
+}
+
+int32_t rtx_instances_destroy(rtx_instances* in) {
+    RTX_REQUIRE(in, RTX_EINVAL, "rtx_instances_destroy: null");
+    (void)hipDeviceSynchronize();
+    in->roots.release();
+    delete in;
+    return RTX_OK;
+}
+
+int32_t rtx_instances_set_transforms(rtx_instances* in, const float* tr, const float* tr_inv) {
+    RTX_REQUIRE(in && tr && tr_inv, RTX_EINVAL, "rtx_instances_set_transforms: null argument");
+    std::copy(tr, tr + 16 * (size_t)in->n, in->tr.begin());
+    std::copy(tr_inv, tr_inv + 16 * (size_t)in->n, in->tr_inv.begin());
+    return RTX_OK;
+}
+
+int32_t rtx_instances_set_lights(rtx_instances* in, const rtx_lights* lights) {
+    RTX_REQUIRE(in && lights, RTX_EINVAL, "rtx_instances_set_lights: null argument");
+    RTX_REQUIRE(lights->count >= 0 && lights->count <= RTX_MAX_LIGHTS, RTX_EINVAL,
+                "rtx_instances_set_lights: count out of range");
+    in->lights = *lights;
+    return RTX_OK;
+}
+
+// Shared triangle numbering: the distinct scenes in order of first use, each
+// taking the next block of ids (the offsets Scene::create_bvhs gives the
+// meshes' index ranges, src/rt/scene.cpp:639-700).
+static int32_t instance_bases(const rtx_instances* in, std::vector<uint32_t>& base) {
+    base.assign(in->n, 0);
+    std::vector<const rtx_scene*> seen;
+    std::vector<uint32_t> first;
+    uint64_t next = 0;
+    for (int i = 0; i < in->n; ++i) {
+        size_t k = 0;
+        while (k < seen.size() && seen[k] != in->scene[i]) ++k;
+        if (k == seen.size()) {
+            seen.push_back(in->scene[i]);
+            first.push_back((uint32_t)next);
+            next += in->scene[i]->T;
+        }
+        base[i] = first[k];
+    }
+    RTX_REQUIRE(next < (1ull << 31), RTX_EUNSUPPORTED, "rtx_instances: more than 2^31 triangles in all");
+    return RTX_OK;
+}
+
+int32_t rtx_instances_tri_base(rtx_instances* in, uint32_t* base) {
+    RTX_REQUIRE(in && base, RTX_EINVAL, "rtx_instances_tri_base: null argument");
+    std::vector<uint32_t> b;
+    int32_t e = instance_bases(in, b);
+    if (e) return e;
+    std::copy(b.begin(), b.end(), base);
+    return RTX_OK;
+}
+
+// Brings the device root table up to date for work queued on `st`: the
+// scenes' current buffers (a rebuild or a new buffer set may move them) and
+// the transforms.  Only chunks that differ from what the device holds are
+// rewritten.
+static int32_t refresh_roots(rtx_instances* in, hipStream_t st, int* var) {
+    std::vector<uint32_t> base;
+    int32_t e = instance_bases(in, base);
+    if (e) return e;
+    in->table.resize(in->n);
+    bool deep = false;
+    for (int i = 0; i < in->n; ++i) {
+        rtx_scene* s = in->scene[i];
+        RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_trace_instances: an instance's scene has no BVH");
+        const SceneView sv = scene_view(s);
+        InstRoot& R = in->table[i];
+        R.node = sv.node;
+        R.delta = sv.delta;
+        R.tri = sv.tri;
+        R.nrm = sv.nrm;
+        R.uv = s->uv.as<float2>();
+        R.base = base[i];
+        R.ntri = s->T;
+        std::copy(&in->tr[16 * (size_t)i], &in->tr[16 * (size_t)i] + 16, R.tr);
+        std::copy(&in->tr_inv[16 * (size_t)i], &in->tr_inv[16 * (size_t)i] + 16, R.tr_inv);
+        deep = deep || s->depth >= STACK;
+    }
+    // the sibling-pair form cannot follow the MAX_LEVELS rule (32-level
+    // trees) and the persistent form has no multi-root kernel: both use the
+    // single-node form with its early delta load
+    const int v = (int)in->ctx->tune.trace_variant;
+    *var = (v == 2 && !deep) ? 2 : (v == 0 ? 0 : 1);
+    if (in->shipped.size() != in->table.size()) in->shipped.assign(in->table.size(), InstRoot{});
+    const bool fresh = in->shipped.empty() || in->shipped[0].node == nullptr;
+    for (int c0 = 0; c0 < in->n; c0 += ROOT_CHUNK) {
+        const int cn = std::min(ROOT_CHUNK, in->n - c0);
+        const size_t bytes = sizeof(InstRoot) * (size_t)cn;
+        if (!fresh && memcmp(&in->shipped[c0], &in->table[c0], bytes) == 0) continue;
+        RootChunk ch{};
+        memcpy(ch.r, &in->table[c0], bytes);
+        hipLaunchKernelGGL(k_put_roots, dim3(1), dim3(256), 0, st, in->roots.as<InstRoot>(), (uint32_t)c0,
+                           (uint32_t)cn, ch);
+        RTX_CHECK_HIP(hipGetLastError());
+        memcpy(&in->shipped[c0], &in->table[c0], bytes);
+    }
+    return RTX_OK;
+}
+
+int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
+    RTX_REQUIRE(in && rays && hits, RTX_EINVAL, "rtx_trace_instances: null argument");
+    RTX_REQUIRE(count == RTX_DEVICE_COUNT || (count <= rays->cap && count <= hits->cap), RTX_EINVAL,
+                "rtx_trace_instances: count exceeds bundle capacity");
+    RTX_REQUIRE(hits->cap >= rays->cap || count != RTX_DEVICE_COUNT, RTX_EINVAL,
+                "rtx_trace_instances: hit bundle smaller than ray bundle");
+    hipStream_t st = in->ctx->pick(stream);
+    if (count == 0) return RTX_OK;
+    int var = 1;
+    int32_t e = refresh_roots(in, st, &var);
+    if (e) return e;
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
+    ProfScope ps(in->ctx, st, "trace_instances");
+    const int tb = (int)in->ctx->tune.trace_block;
+#define RTX_TRACE_INST(V, B)                                                                                        \
+    hipLaunchKernelGGL((k_trace_inst<V, B>), dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st,       \
+                       in->roots.as<InstRoot>(), in->n, rays->ori4.as<float4>(), rays->dir4.as<float4>(),          \
+                       rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),         \
+                       hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(),                     \
+                       hits->p4.as<float4>(), hits->uv2.as<float2>(), hits->list.as<uint32_t>(),                   \
+                       hits->list_count.as<uint32_t>(), coop_threshold(in->ctx))
+    RTX_DISPATCH_TRAVERSAL(RTX_TRACE_INST, var, tb);
+#undef RTX_TRACE_INST
+    RTX_CHECK_HIP(hipGetLastError());
+    hits->list_valid = true;
+    hits->list_for = count;
+    return RTX_OK;
+}
+
+int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
+    RTX_REQUIRE(in && rays && hits, RTX_EINVAL, "rtx_trace_shadow_instances: null argument");
+    RTX_REQUIRE(count == RTX_DEVICE_COUNT || count <= hits->cap, RTX_EINVAL,
+                "rtx_trace_shadow_instances: count too large");
+    hipStream_t st = in->ctx->pick(stream);
+    if (count == 0 || in->lights.count <= 0) return RTX_OK;
+    int var = 1;
+    int32_t e = refresh_roots(in, st, &var);
+    if (e) return e;
+    ensure_hit_list(rays, hits, count, st);
+    ProfScope ps(in->ctx, st, "trace_shadow_instances");
+    const uint64_t items = (count == RTX_DEVICE_COUNT ? rays->cap : count) * (uint64_t)in->lights.count;
+    const int tb = (int)in->ctx->tune.trace_block;
+#define RTX_SHADOW_INST(V, B)                                                                                       \
+    hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,          \
+                       in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(),                     \
+                       hits->list_count.as<uint32_t>(), hits->flags.as<uint32_t>(), hits->p4.as<float4>(),         \
+                       (unsigned long long*)nullptr, coop_threshold(in->ctx))
+    RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
+#undef RTX_SHADOW_INST
+    RTX_CHECK_HIP(hipGetLastError());
+    return RTX_OK;
 }

@@ -122,6 +122,8 @@ SYMBOLS = [
     "rtx_batch_stats",
     "rtx_pipeline_create", "rtx_pipeline_destroy", "rtx_pipeline_set_animation", "rtx_pipeline_frame",
     "rtx_pipeline_wait", "rtx_pipeline_fb", "rtx_pipeline_stats",
+    "rtx_instances_create", "rtx_instances_destroy", "rtx_instances_set_transforms", "rtx_instances_set_lights",
+    "rtx_instances_tri_base", "rtx_trace_instances", "rtx_trace_shadow_instances",
     "rtx_camera_set",
 ]
 
@@ -229,6 +231,13 @@ def lib() -> C.CDLL:
         "rtx_renderer_capture": (I32, [P, P, P, I32, P]),
         "rtx_renderer_replay": (I32, [P, P]),
         "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
+        "rtx_instances_create": (I32, [P, P, P, P, I32, C.POINTER(P)]),
+        "rtx_instances_destroy": (I32, [P]),
+        "rtx_instances_set_transforms": (I32, [P, P, P]),
+        "rtx_instances_set_lights": (I32, [P, P]),
+        "rtx_instances_tri_base": (I32, [P, P]),
+        "rtx_trace_instances": (I32, [P, P, P, U64, P]),
+        "rtx_trace_shadow_instances": (I32, [P, P, P, U64, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -608,6 +617,51 @@ def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None)
 
 def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
     _check(lib().rtx_trace_shadow(scene.h, rays.h, hits.h, count, stream), "rtx_trace_shadow")
+
+
+class Instances:
+    """Objects of a scene over per-mesh BVHs (Scene::create_bvhs /
+    update_bvh_roots, src/rt/scene.cpp:596-700): instance i is built scene
+    scenes[i] under the 4x4 object-to-world transform tr[i] (row-major) with
+    inverse tr_inv[i].  Triangle ids are shared: the distinct scenes in order
+    of first use take consecutive blocks."""
+
+    def __init__(self, ctx: Context, scenes, tr, tr_inv):
+        self.ctx = ctx
+        self.scenes = list(scenes)  # keeps the scenes alive with the set
+        n = len(self.scenes)
+        arr = (C.c_void_p * n)(*[sc.h.value for sc in self.scenes])
+        tr = np.ascontiguousarray(tr, np.float32).reshape(n, 16)
+        tri = np.ascontiguousarray(tr_inv, np.float32).reshape(n, 16)
+        self.h = C.c_void_p()
+        _check(lib().rtx_instances_create(ctx.h, arr, _ptr(tr), _ptr(tri), n, C.byref(self.h)),
+               "rtx_instances_create")
+        self.n = n
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rtx_instances_destroy(self.h), "rtx_instances_destroy")
+            self.h = C.c_void_p()
+
+    def set_transforms(self, tr, tr_inv):
+        tr = np.ascontiguousarray(tr, np.float32).reshape(self.n, 16)
+        tri = np.ascontiguousarray(tr_inv, np.float32).reshape(self.n, 16)
+        _check(lib().rtx_instances_set_transforms(self.h, _ptr(tr), _ptr(tri)), "rtx_instances_set_transforms")
+
+    def set_lights(self, lights: Lights):
+        _check(lib().rtx_instances_set_lights(self.h, C.byref(lights)), "rtx_instances_set_lights")
+
+    def tri_base(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().rtx_instances_tri_base(self.h, _ptr(out)), "rtx_instances_tri_base")
+        return out
+
+    def trace(self, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+        _check(lib().rtx_trace_instances(self.h, rays.h, hits.h, count, stream), "rtx_trace_instances")
+
+    def trace_shadow(self, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+        _check(lib().rtx_trace_shadow_instances(self.h, rays.h, hits.h, count, stream),
+               "rtx_trace_shadow_instances")
 
 
 def raygen_secondary(scene: Scene, rays_in: Rays, hits: Hits, rays_out: Rays, count_in=DEVICE_COUNT,

@@ -1,0 +1,203 @@
+"""GPU parity of the multi-root (instanced) traversal: trace_multi and
+shadow_trace_multi (src/kernel/trace-bvh.cl:419-463,
+src/kernel/shadow-trace-bvh.cl:317-373) against the oracle, bit for bit, for
+every traversal form.
+
+Cases: rigid instances with scales sharing one mesh; two scenes of the same
+triangles under the same transform (every hit an exact tie, which the first
+root wins); interpenetrating instances; a projective transform (w != 1, the
+division branch of multiply_point); new transforms without a rebuild; and a
+scene rebuilt under its instances.  Lights: a directional light, a point
+light, and spot lights on both sides of that kernel's cone test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from inst_scene import rot, xform, two_meshes, sphere_mesh, rays_at
+from test_gpu_parity import load_scene
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("flags", "t", "id", "bary", "n4", "p4", "uv2")
+
+
+def _lights():
+    from rtx import make_lights, LIGHT_DIR, LIGHT_SPOT, LIGHT_POINT
+    return make_lights((0.1, 0.1, 0.1), [
+        dict(type=LIGHT_DIR, dir=(0.2, -1.0, 0.1), color=(1.0, 1.0, 1.0)),
+        dict(type=LIGHT_POINT, pos=(1.0, 6.0, -5.0), color=(1.0, 1.0, 1.0)),
+        # toward this light from the scene is about (0, +1, 0): below the cut-off, lit untraced
+        dict(type=LIGHT_SPOT, pos=(0.0, 12.0, 0.0), dir=(0.0, -1.0, 0.0), angle=0.8, color=(1.0, 1.0, 1.0)),
+        # toward this light is about (0, -1, 0): above the cut-off, traced
+        dict(type=LIGHT_SPOT, pos=(0.5, -12.0, 0.0), dir=(0.0, -1.0, 0.0), angle=0.8, color=(1.0, 1.0, 1.0)),
+    ])
+
+
+def _projective():
+    # the inverse has the bottom row (0.01, 0, 0.03, 1): the rays (+z) keep
+    # w > 0 in the object frame, both the points and the directions are
+    # divided by it, and the instance is still hit
+    R = rot((1, 1, 0), 0.5)
+    t = np.array((0.5, -1.0, 0.5))
+    Mi = np.eye(4)
+    Mi[:3, :3] = R.T
+    Mi[:3, 3] = -R.T @ t
+    Mi[3, :3] = (0.01, 0.0, 0.03)
+    return np.linalg.inv(Mi).astype(np.float32), Mi.astype(np.float32)
+
+
+CASES = {
+    "rigid": (two_meshes, [(0, *xform(rot((0, 1, 0), 0.7), 1.0, (-2.0, 0.3, 0.0))),
+                           (1, *xform(rot((1, 0, 1), 1.9), 2.0, (2.0, -0.5, 1.0))),
+                           (0, *xform(rot((0, 0, 1), -0.4), 0.5, (0.0, 2.0, -1.0)))]),
+    "tie": (lambda: [sphere_mesh(), sphere_mesh()], [(1, *xform(rot((0, 1, 0), 0.9), 1.5)),
+                                                     (0, *xform(rot((0, 1, 0), 0.9), 1.5))]),
+    "overlap": (two_meshes, [(0, *xform(rot((0, 1, 0), 0.3), 1.2, (0.4, 0.0, 0.0))),
+                             (1, *xform(rot((1, 0, 0), 0.8), 1.6, (-0.5, 0.2, 0.3))),
+                             (0, *xform(rot((0, 0, 1), 1.1), 0.9, (0.0, -0.3, -0.6))),
+                             (1, *xform(np.eye(3), 1.0, (0.0, 2.6, 0.0)))]),
+    "projective": (two_meshes, [(0, *_projective()), (1, *xform(np.eye(3), 1.0, (0.0, 2.5, 0.0)))]),
+}
+
+
+def _load(ctx, meshes, lights):
+    out = []
+    for p, n, uv, idx in meshes:
+        s = load_scene(ctx, p, idx, n, uv, np.zeros(len(idx), np.int32), lights=lights)
+        s.build(30, 1)
+        out.append(s)
+    return out
+
+
+def _oracle(meshes, builds, insts, lights, ori4, dir4):
+    ms = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], insts)
+    o = O.trace_multi(ms, ori4, dir4)
+    o["flags"] = O.trace_shadow_multi(ms, O.lights_from(lights), o["flags"], o["p4"])
+    return o, ms
+
+
+def _check_all_forms(ctx, inst, r, hb, n, o, what):
+    old = ctx.tuned("trace_variant")
+    try:
+        for variant in (0, 1, 2, 3):
+            ctx.tune("trace_variant", variant)
+            inst.trace(r, hb, n)
+            inst.trace_shadow(r, hb, n)
+            g = hb.read(n)
+            for k in FIELDS:
+                assert np.array_equal(g[k], o[k]), f"{what}: {k} differs (variant {variant})"
+    finally:
+        ctx.tune("trace_variant", old)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_instances_match_oracle(ctx, case):
+    import rtx
+    make, insts = CASES[case]
+    meshes = make()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    n = 6000
+    ori4, dir4 = rays_at(11, n)
+    o, ms = _oracle(meshes, builds, insts, lights, ori4, dir4)
+    hit = (o["flags"] & 1) != 0
+    assert hit.sum() > 300, "the rays should hit the instances"
+    for i, (m, _, _) in enumerate(insts):
+        mine = (o["id"][hit] >= ms.tri_base[m]) & (o["id"][hit] < ms.tri_base[m] + builds[m].T)
+        if case == "tie":  # every hit is a tie, won by the first root (mesh 1, laid out first)
+            assert mine.all() if i == 0 else not mine.any()
+        else:
+            assert mine.sum() > 50, f"instance {i}'s mesh should be hit"
+    for l in (0, 1, 3):
+        assert ((o["flags"] >> (8 + l)) & 1)[hit].any(), f"light {l} should shadow some hits"
+    assert not ((o["flags"] >> 10) & 1).any(), "the first spot light lights every point untraced"
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    base = inst.tri_base()
+    assert [int(b) for b in base] == [ms.tri_base[m] for m, _, _ in insts]
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    _check_all_forms(ctx, inst, r, hb, n, o, case)
+    # the ray count kept on the device
+    inst.trace(r, hb)
+    inst.trace_shadow(r, hb)
+    g = hb.read(n)
+    assert all(np.array_equal(g[k], o[k]) for k in FIELDS), f"{case}: device-count trace differs"
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    for s in scs:
+        s.destroy()
+
+
+def test_new_transforms_and_rebuilt_scene(ctx):
+    """Rigid animation ships new roots without a rebuild; a scene rebuilt
+    under its instances (moved vertices) is traced with its new tree."""
+    import rtx
+    meshes = two_meshes()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    n = 4096
+    ori4, dir4 = rays_at(12, n)
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    insts = [(0, *xform(rot((0, 1, 0), 0.2), 1.0, (-1.5, 0.0, 0.0))), (1, *xform(np.eye(3), 1.3, (1.5, 0.0, 0.0))),
+             (0, *xform(np.eye(3), 0.6, (0.0, 1.8, 0.0)))]
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    for step in range(4):
+        ang = 0.35 * step
+        cur = [(0, *xform(rot((0, 1, 0), 0.2 + ang), 1.0, (-1.5, 0.2 * step, 0.0))),
+               (1, *xform(rot((1, 0, 0), ang), 1.3, (1.5, 0.0, 0.1 * step))),
+               (0, *xform(rot((0, 0, 1), -ang), 0.6, (0.0, 1.8 - 0.1 * step, 0.0)))]
+        inst.set_transforms([t for _, t, _ in cur], [ti for _, _, ti in cur])
+        o, _ = _oracle(meshes, builds, cur, lights, ori4, dir4)
+        inst.trace(r, hb, n)
+        inst.trace_shadow(r, hb, n)
+        g = hb.read(n)
+        for k in FIELDS:
+            assert np.array_equal(g[k], o[k]), f"step {step}: {k} differs"
+    # move the vertices of mesh 0 and rebuild its scene: the instances follow
+    p0, n0, uv0, idx0 = meshes[0]
+    p_new = (p0 * np.float32(1.25) + np.float32([0.0, 0.1, 0.0])).astype(np.float32)
+    scs[0].update_positions(p_new)
+    scs[0].build(30, 1)
+    ctx.sync()
+    b0 = O.Build(p_new, builds[0].idx.reshape(-1, 3), None, 30, 1)  # input in the last build's order
+    meshes2 = [(p_new, n0, uv0, idx0), meshes[1]]
+    o, _ = _oracle(meshes2, [b0, builds[1]], cur, lights, ori4, dir4)
+    inst.trace(r, hb, n)
+    inst.trace_shadow(r, hb, n)
+    g = hb.read(n)
+    for k in FIELDS:
+        assert np.array_equal(g[k], o[k]), f"rebuilt scene: {k} differs"
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    for s in scs:
+        s.destroy()
+
+
+def test_instance_errors(ctx):
+    import rtx
+    p, nrm, uv, idx = sphere_mesh(8, 6)
+    s = load_scene(ctx, p, idx, nrm, uv, np.zeros(len(idx), np.int32))
+    I4 = np.eye(4, dtype=np.float32)
+    with pytest.raises(rtx.RtxError):
+        rtx.Instances(ctx, [], np.zeros((0, 16), np.float32), np.zeros((0, 16), np.float32))
+    inst = rtx.Instances(ctx, [s], I4, I4)
+    r = rtx.Rays(ctx, 16)
+    hb = rtx.Hits(ctx, 16)
+    with pytest.raises(rtx.RtxError, match="no BVH"):
+        inst.trace(r, hb, 16)
+    s.build(30, 1)
+    inst.trace(r, hb, 0)
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    s.destroy()
