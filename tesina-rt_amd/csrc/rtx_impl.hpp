@@ -235,6 +235,7 @@ struct rtx_instances {
     std::vector<float> tr, tr_inv;  // 16 floats per instance
     std::vector<InstRoot> table, shipped;
     DevBuf roots;
+    DevBuf hit_root;  // uint32 per ray: the root of its closest hit (an order hint for shadow rays)
     rtx_lights lights{};
 };
 

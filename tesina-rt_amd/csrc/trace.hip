@@ -1076,12 +1076,13 @@ uint32_t coop_threshold(const rtx_ctx* c) {
 
 // multiply_point / multiply_vector: rows dotted with (v, 1) or (v, 0), then
 // a division by w when |w| > wmin (1e-26 in trace-bvh.cl:83-113, 0.00001 in
-// shadow-trace-bvh.cl:107-143).  The dot is the sequential sum.
+// shadow-trace-bvh.cl:107-143).  The dot is the sequential sum.  A division
+// by w == 1 returns its dividend exactly, so it is skipped (affine rows).
 __device__ __forceinline__ v3 mul_h(const float* __restrict__ M, v3 v, float vw, float wmin) {
     float r[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) r[i] = ((M[4 * i] * v.x + M[4 * i + 1] * v.y) + M[4 * i + 2] * v.z) + M[4 * i + 3] * vw;
-    if (fabsf(r[3]) > wmin) {
+    if (fabsf(r[3]) > wmin && r[3] != 1.f) {
         r[0] = r[0] / r[3];
         r[1] = r[1] / r[3];
         r[2] = r[2] / r[3];
@@ -1103,7 +1104,8 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
                                                     int32_t* __restrict__ idout, float2* __restrict__ bary,
                                                     float4* __restrict__ n4, float4* __restrict__ p4,
                                                     float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
-                                                    uint32_t* __restrict__ hit_count, uint32_t coop) {
+                                                    uint32_t* __restrict__ hit_count, uint32_t* __restrict__ hit_root,
+                                                    uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
@@ -1154,6 +1156,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
             continue;
         }
         hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
+        hit_root[rr] = (uint32_t)broot;
         const InstRoot& R = roots[broot];
         const v3 hp = o + d * bt;
         const uint32_t i0 = __float_as_uint(R.tri[3 * (size_t)bid].w);
@@ -1182,12 +1185,17 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
 // DIR toward -dir; SPOT toward pos, lit without a trace when dot(toward the
 // light, spot dir) < angle (that kernel's test); POINT toward pos with tMax =
 // distance (the lights extension); any hit in any root sets bit 8 + l.
+// The result is an OR over the roots, so their order is free: a wave starts
+// at the root its first item's hit came from (`hit_root`, the closest-hit
+// kernel's record; any value only reorders), where an occluded point is
+// usually blocked by its own object, and stops once every ray is blocked.
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__ roots, int nroots,
                                                      const rtx_lights lights,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ list_count,
                                                      uint32_t* __restrict__ flags, const float4* __restrict__ p4,
+                                                     const uint32_t* __restrict__ hit_root,
                                                      unsigned long long* shadow_count, uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
@@ -1226,9 +1234,13 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
             if (trace) ++traced;
         }
         bool shadowed = false;
-        for (int i = 0; i < nroots; ++i) {  // wave-uniform
+        uint32_t mine = 0;
+        if (valid && hit_root) mine = hit_root[rr] % (uint32_t)nroots;
+        const int first = __builtin_amdgcn_readfirstlane((int)mine);
+        for (int k = 0, i = first; k < nroots; ++k, i = (i + 1 == nroots) ? 0 : i + 1) {  // wave-uniform
             const InstRoot& R = roots[i];
             const bool go = trace && !shadowed;
+            if (!ballot(go)) break;
             RayC rc{};
             if (go) rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
             if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go)) shadowed = true;  // whole wave
@@ -1602,6 +1614,7 @@ int32_t rtx_instances_destroy(rtx_instances* in) {
     RTX_REQUIRE(in, RTX_EINVAL, "rtx_instances_destroy: null");
     (void)hipDeviceSynchronize();
     in->roots.release();
+    in->hit_root.release();
     delete in;
     return RTX_OK;
 }
@@ -1710,6 +1723,11 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
     int var = 1;
     int32_t e = refresh_roots(in, st, &var);
     if (e) return e;
+    if (in->hit_root.bytes < hits->cap * 4) {
+        // the buffer may still be read by queued shadow work
+        RTX_CHECK_HIP(hipDeviceSynchronize());
+        if ((e = in->hit_root.reserve(in->ctx, hits->cap * 4))) return e;
+    }
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
     ProfScope ps(in->ctx, st, "trace_instances");
     const int tb = (int)in->ctx->tune.trace_block;
@@ -1719,7 +1737,7 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),         \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(),                     \
                        hits->p4.as<float4>(), hits->uv2.as<float2>(), hits->list.as<uint32_t>(),                   \
-                       hits->list_count.as<uint32_t>(), coop_threshold(in->ctx))
+                       hits->list_count.as<uint32_t>(), in->hit_root.as<uint32_t>(), coop_threshold(in->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_TRACE_INST, var, tb);
 #undef RTX_TRACE_INST
     RTX_CHECK_HIP(hipGetLastError());
@@ -1740,12 +1758,15 @@ int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* 
     ensure_hit_list(rays, hits, count, st);
     ProfScope ps(in->ctx, st, "trace_shadow_instances");
     const uint64_t items = (count == RTX_DEVICE_COUNT ? rays->cap : count) * (uint64_t)in->lights.count;
+    // root order hint: the winning roots of the last closest-hit call, when
+    // the buffer covers every ray of this bundle (stale values only reorder)
+    const uint32_t* order = in->hit_root.bytes >= hits->cap * 4 ? in->hit_root.as<uint32_t>() : nullptr;
     const int tb = (int)in->ctx->tune.trace_block;
 #define RTX_SHADOW_INST(V, B)                                                                                       \
     hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,          \
                        in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(),                     \
                        hits->list_count.as<uint32_t>(), hits->flags.as<uint32_t>(), hits->p4.as<float4>(),         \
-                       (unsigned long long*)nullptr, coop_threshold(in->ctx))
+                       order, (unsigned long long*)nullptr, coop_threshold(in->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
 #undef RTX_SHADOW_INST
     RTX_CHECK_HIP(hipGetLastError());
