@@ -130,7 +130,7 @@ public:
     int32_t initialize();
     int32_t destroy();
     bool valid() const { return m_h != nullptr; }
-    bool ready() const { return m_built; }
+    bool ready() const { return m_built || m_bvhs_built; }
     // Geometry (the aggregate mesh of Scene::transfer_aggregate_mesh_to_device,
     // src/rt/scene.cpp:237-321): pos/nrm V x 3, uv V x 2, idx T x 3.
     int32_t set_geometry(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
@@ -150,6 +150,28 @@ public:
     AcceleratorType get_accelerator_type() const { return LBVH_ACCELERATOR; }
     rtx_scene* handle() const { return m_h; }
 
+    // Objects over per-mesh BVHs (Scene::add_object, create_bvhs,
+    // update_bvh_roots, root_count: src/rt/scene.cpp:596-700).  Meshes come
+    // from add_mesh (in place of load_obj_file); an object is a mesh under
+    // Object::geom's transform, given as its matrices (row-major 4x4,
+    // GeometricProperties::getTransformMatrix / getTransformMatrixInv).  The
+    // per-mesh trees are LBVHs.  With objects and no aggregate mesh,
+    // root_count() is the object count, and the Tracer takes the multi-root
+    // kernels when it exceeds 1 (src/rt/tracer.cpp:97, 223); one object is
+    // traced as the single-root kernel traces it: its mesh, untransformed.
+    int32_t add_mesh(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
+                     uint32_t T);  // mesh id, or -1
+    int32_t add_object(int32_t mesh);  // object id, or -1
+    int32_t set_object_transform(int32_t object, const float* tr16, const float* tr_inv16);
+    int32_t create_bvhs(int depth = 30, int min_leaf = 1);
+    int32_t update_bvh_roots();
+    size_t object_count() const { return m_obj_mesh.size(); }
+    size_t root_count() const { return m_built ? 1 : m_obj_mesh.size(); }
+    bool aggregate_built() const { return m_built; }
+    rtx_instances* instances() const { return m_inst; }
+    rtx_scene* mesh_handle(int32_t mesh) const { return m_meshes[(size_t)mesh]; }
+    int32_t object_mesh(int32_t object) const { return m_obj_mesh[(size_t)object]; }
+
     Camera camera;
     uint32_t bvh_node_count = 0;
     uint32_t bvh_level_node_count[64] = {};
@@ -161,6 +183,11 @@ private:
     rtx_lights m_lights{};
     size_t m_T = 0, m_V = 0;
     bool m_built = false;
+    std::vector<rtx_scene*> m_meshes;
+    std::vector<int32_t> m_obj_mesh;
+    std::vector<float> m_tr, m_tr_inv;
+    rtx_instances* m_inst = nullptr;
+    bool m_bvhs_built = false;
 };
 
 class RayBundle {

@@ -77,7 +77,8 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-TOOLS = {"stage_render": ROOT / "tests" / "cpp" / "stage_render.cpp"}
+TOOLS = {"stage_render": ROOT / "tests" / "cpp" / "stage_render.cpp",
+         "stage_objects": ROOT / "tests" / "cpp" / "stage_objects.cpp"}
 BIN = PKG / "bin"
 
 

@@ -103,7 +103,90 @@ int32_t Scene::initialize() {
 int32_t Scene::destroy() {
     if (m_h) rtx_scene_destroy(m_h);
     m_h = nullptr;
+    if (m_inst) rtx_instances_destroy(m_inst);
+    m_inst = nullptr;
+    for (rtx_scene* m : m_meshes) rtx_scene_destroy(m);
+    m_meshes.clear();
+    m_obj_mesh.clear();
+    m_tr.clear();
+    m_tr_inv.clear();
+    m_bvhs_built = false;
     return 0;
+}
+
+// ------------------------------------------------------------- objects --
+int32_t Scene::add_mesh(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
+                        uint32_t T) {
+    if (!DeviceInterface::instance()->good() && DeviceInterface::instance()->initialize()) return -1;
+    rtx_scene* m = nullptr;
+    if (rtx_scene_create(dev(), &m) != RTX_OK) {
+        report("Scene::add_mesh");
+        return -1;
+    }
+    std::vector<int32_t> mm(T, 0);
+    if (rtx_scene_set_vertices(m, pos, nrm, uv, V) != RTX_OK ||
+        rtx_scene_set_triangles(m, idx, mm.data(), T) != RTX_OK) {
+        report("Scene::add_mesh");
+        rtx_scene_destroy(m);
+        return -1;
+    }
+    m_meshes.push_back(m);
+    return (int32_t)m_meshes.size() - 1;
+}
+
+// Scene::add_object: identity transform until set_object_transform.
+int32_t Scene::add_object(int32_t mesh) {
+    if (mesh < 0 || (size_t)mesh >= m_meshes.size()) return -1;
+    static const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    m_obj_mesh.push_back(mesh);
+    m_tr.insert(m_tr.end(), I, I + 16);
+    m_tr_inv.insert(m_tr_inv.end(), I, I + 16);
+    return (int32_t)m_obj_mesh.size() - 1;
+}
+
+int32_t Scene::set_object_transform(int32_t object, const float* tr16, const float* tr_inv16) {
+    if (object < 0 || (size_t)object >= m_obj_mesh.size() || !tr16 || !tr_inv16) return -1;
+    std::copy(tr16, tr16 + 16, m_tr.begin() + 16 * (size_t)object);
+    std::copy(tr_inv16, tr_inv16 + 16, m_tr_inv.begin() + 16 * (size_t)object);
+    return 0;
+}
+
+// Scene::create_bvhs (src/rt/scene.cpp:639-700): one tree per mesh that an
+// object uses, then the roots.
+int32_t Scene::create_bvhs(int depth, int min_leaf) {
+    if (m_obj_mesh.empty()) return -1;
+    std::vector<bool> used(m_meshes.size(), false);
+    rtx_lbvh_params p{std::min(std::max(depth, 1), 32), std::min(std::max(min_leaf, 1), 256)};
+    for (int32_t m : m_obj_mesh) {
+        if (used[(size_t)m]) continue;
+        used[(size_t)m] = true;
+        if (rtx_lbvh_build(m_meshes[(size_t)m], &p, nullptr) != RTX_OK) {
+            report("Scene::create_bvhs");
+            return -1;
+        }
+    }
+    if (m_inst) rtx_instances_destroy(m_inst);
+    m_inst = nullptr;
+    std::vector<rtx_scene*> sc;
+    for (int32_t m : m_obj_mesh) sc.push_back(m_meshes[(size_t)m]);
+    if (rtx_instances_create(dev(), sc.data(), m_tr.data(), m_tr_inv.data(), (int32_t)sc.size(), &m_inst) !=
+            RTX_OK ||
+        rtx_instances_set_lights(m_inst, &m_lights) != RTX_OK) {
+        report("Scene::create_bvhs");
+        return -1;
+    }
+    m_T = 0;
+    for (size_t m = 0; m < m_meshes.size(); ++m)
+        if (used[m]) m_T += rtx_scene_triangle_count(m_meshes[m]);
+    m_bvhs_built = true;
+    return 0;
+}
+
+// Scene::update_bvh_roots (src/rt/scene.cpp:596-629): the objects' current
+// transforms go to the roots.
+int32_t Scene::update_bvh_roots() {
+    if (!m_bvhs_built) return -1;
+    return rtx_instances_set_transforms(m_inst, m_tr.data(), m_tr_inv.data()) == RTX_OK ? 0 : -1;
 }
 
 int32_t Scene::set_geometry(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
@@ -138,11 +221,17 @@ int32_t Scene::set_materials(const std::vector<rtx_material>& mats) {
     return rtx_scene_set_materials(m_h, mats.data(), (uint32_t)mats.size()) == RTX_OK ? 0 : -1;
 }
 
-int32_t Scene::push_lights() { return rtx_scene_set_lights(m_h, &m_lights) == RTX_OK ? 0 : -1; }
+int32_t Scene::push_lights() {
+    if (m_h && rtx_scene_set_lights(m_h, &m_lights) != RTX_OK) return -1;
+    if (m_inst && rtx_instances_set_lights(m_inst, &m_lights) != RTX_OK) return -1;
+    for (rtx_scene* m : m_meshes)
+        if (rtx_scene_set_lights(m, &m_lights) != RTX_OK) return -1;
+    return 0;
+}
 
 // Scene::set_dir_light normalizes the direction on the host (src/rt/scene.cpp:987-1003).
 int32_t Scene::set_dir_light(const vec3& d, const vec3& color) {
-    if (!m_h) return -1;
+    if ((!m_h && m_meshes.empty())) return -1;
     float n = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
     if (!(n > 0.f)) return -1;
     rtx_light L{};
@@ -159,7 +248,7 @@ int32_t Scene::set_dir_light(const vec3& d, const vec3& color) {
 }
 
 int32_t Scene::set_spot_light(const vec3& p, const vec3& d, const vec3& color, float angle, float radius) {
-    if (!m_h) return -1;
+    if ((!m_h && m_meshes.empty())) return -1;
     rtx_light L{};
     L.type = RTX_LIGHT_SPOT;
     L.pos[0] = p.x, L.pos[1] = p.y, L.pos[2] = p.z;
@@ -173,7 +262,7 @@ int32_t Scene::set_spot_light(const vec3& p, const vec3& d, const vec3& color, f
 }
 
 int32_t Scene::set_ambient_light(const vec3& c) {
-    if (!m_h) return -1;
+    if ((!m_h && m_meshes.empty())) return -1;
     m_lights.ambient[0] = c.x;
     m_lights.ambient[1] = c.y;
     m_lights.ambient[2] = c.z;
@@ -181,7 +270,7 @@ int32_t Scene::set_ambient_light(const vec3& c) {
 }
 
 int32_t Scene::add_point_light(const vec3& p, const vec3& color) {
-    if (!m_h || m_lights.count >= RTX_MAX_LIGHTS) return -1;
+    if ((!m_h && m_meshes.empty()) || m_lights.count >= RTX_MAX_LIGHTS) return -1;
     rtx_light& L = m_lights.l[m_lights.count++];
     L = rtx_light{};
     L.type = RTX_LIGHT_POINT;
@@ -191,7 +280,7 @@ int32_t Scene::add_point_light(const vec3& p, const vec3& color) {
 }
 
 int32_t Scene::set_lights(const rtx_lights& lights) {
-    if (!m_h) return -1;
+    if ((!m_h && m_meshes.empty())) return -1;
     m_lights = lights;
     return push_lights();
 }
@@ -388,12 +477,22 @@ int32_t Tracer::initialize() {
     return 0;
 }
 
+// The scene the single-root kernels trace: the aggregate mesh, or the mesh
+// of a scene's only object (node 0 of the shared array, no transform).
+static rtx_scene* single_root(const Scene& scene) {
+    return scene.aggregate_built() ? scene.handle() : scene.mesh_handle(scene.object_mesh(0));
+}
+
 // Tracer::trace (src/rt/tracer.cpp:140-269).
 int32_t Tracer::trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary) {
     (void)secondary;
     if (!m_initialized || ray_count < 0 || !scene.ready()) return -1;
     auto t = clk::now();
-    if (rtx_trace(scene.handle(), rays.handle(), hits.handle(), (uint64_t)ray_count, nullptr) != RTX_OK) {
+    const int32_t rc = scene.root_count() > 1
+                           ? rtx_trace_instances(scene.instances(), rays.handle(), hits.handle(), (uint64_t)ray_count,
+                                                 nullptr)
+                           : rtx_trace(single_root(scene), rays.handle(), hits.handle(), (uint64_t)ray_count, nullptr);
+    if (rc != RTX_OK) {
         report("Tracer::trace");
         return -1;
     }
@@ -409,7 +508,12 @@ int32_t Tracer::shadow_trace(Scene& scene, int32_t ray_count, RayBundle& rays, H
     (void)secondary;
     if (!m_initialized || ray_count < 0 || !scene.ready()) return -1;
     auto t = clk::now();
-    if (rtx_trace_shadow(scene.handle(), rays.handle(), hits.handle(), (uint64_t)ray_count, nullptr) != RTX_OK) {
+    const int32_t rc = scene.root_count() > 1
+                           ? rtx_trace_shadow_instances(scene.instances(), rays.handle(), hits.handle(),
+                                                        (uint64_t)ray_count, nullptr)
+                           : rtx_trace_shadow(single_root(scene), rays.handle(), hits.handle(), (uint64_t)ray_count,
+                                              nullptr);
+    if (rc != RTX_OK) {
         report("Tracer::shadow_trace");
         return -1;
     }

@@ -53,3 +53,90 @@ def test_cpp_renderer_matches_oracle(tmp_path, frames):
     assert np.array_equal(acc, oacc)
     rgba = np.fromfile(out / "rgba.u8", np.uint8).reshape(-1, 4)
     assert np.array_equal(rgba, O.resolve(oacc))
+
+
+OBJ_BIN = ROOT / "tesina-rt_amd" / "bin" / "stage_objects"
+
+
+def _write_objects(d: Path, meshes, objects, moved, ori4, dir4, lights):
+    (d / "meshes.txt").write_text(f"{len(meshes)}\n" + "".join(f"{len(m[0])} {len(m[3])}\n" for m in meshes))
+    for k, (p, n, uv, idx) in enumerate(meshes):
+        p.astype(np.float32).tofile(d / f"mesh{k}.pos.f32")
+        n.astype(np.float32).tofile(d / f"mesh{k}.nrm.f32")
+        uv.astype(np.float32).tofile(d / f"mesh{k}.uv.f32")
+        idx.astype(np.uint32).tofile(d / f"mesh{k}.idx.u32")
+
+    def mats(tr, tri):
+        return " ".join(repr(float(x)) for x in (*np.asarray(tr, np.float32).ravel(),
+                                                  *np.asarray(tri, np.float32).ravel()))
+    (d / "objects.txt").write_text(f"{len(objects)}\n" + "".join(f"{m} {mats(tr, tri)}\n" for m, tr, tri in objects))
+    if moved is not None:
+        (d / "moved.txt").write_text("".join(f"{mats(tr, tri)}\n" for _, tr, tri in moved))
+    np.concatenate([ori4, dir4], axis=1).astype(np.float32).tofile(d / "rays.f32")
+    (d / "lights.bin").write_bytes(bytes(lights))
+
+
+def _read_pass(out: Path, p: int, n: int):
+    b = out / f"pass{p}"
+    return {"flags": np.fromfile(f"{b}.flags.u32", np.uint32), "t": np.fromfile(f"{b}.t.f32", np.float32),
+            "id": np.fromfile(f"{b}.id.i32", np.int32), "bary": np.fromfile(f"{b}.bary.f32", np.float32).reshape(n, 2),
+            "n4": np.fromfile(f"{b}.n4.f32", np.float32).reshape(n, 4),
+            "p4": np.fromfile(f"{b}.p4.f32", np.float32).reshape(n, 4),
+            "uv2": np.fromfile(f"{b}.uv2.f32", np.float32).reshape(n, 2)}
+
+
+def test_cpp_objects_match_oracle(tmp_path):
+    """Objects through the C++ Scene / Tracer: create_bvhs, trace, shadow,
+    then new transforms through update_bvh_roots; both passes equal the
+    oracle's multi-root traversal."""
+    from inst_scene import rot, xform, two_meshes, rays_at
+    from test_gpu_instances import _lights
+    assert OBJ_BIN.exists(), "stage_objects not built (run __graft_entry__.build())"
+    meshes = two_meshes()
+    objects = [(0, *xform(rot((0, 1, 0), 0.7), 1.0, (-2.0, 0.3, 0.0))),
+               (1, *xform(rot((1, 0, 1), 1.9), 2.0, (2.0, -0.5, 1.0))),
+               (0, *xform(rot((0, 0, 1), -0.4), 0.5, (0.0, 2.0, -1.0)))]
+    moved = [(m, *xform(rot((0, 1, 0), 0.3 * i + 0.2), 1.1, (1.5 * i - 1.5, 0.2, 0.0))) for i, (m, _, _) in
+             enumerate(objects)]
+    n = 3000
+    ori4, dir4 = rays_at(21, n)
+    lights = _lights()
+    _write_objects(tmp_path, meshes, objects, moved, ori4, dir4, lights)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(OBJ_BIN), str(tmp_path), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    for p, objs in enumerate((objects, moved)):
+        ms = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], objs)
+        o = O.trace_multi(ms, ori4, dir4)
+        o["flags"] = O.trace_shadow_multi(ms, O.lights_from(lights), o["flags"], o["p4"])
+        assert ((o["flags"] & 1) != 0).sum() > 300
+        g = _read_pass(out, p, n)
+        for k in ("flags", "t", "id", "bary", "n4", "p4", "uv2"):
+            assert np.array_equal(g[k], o[k]), f"pass {p}: {k} differs"
+
+
+def test_cpp_single_object_is_traced_untransformed(tmp_path):
+    """With one object and no aggregate mesh, root_count() is 1 and the
+    Tracer takes the single-root kernels, which trace the mesh as stored
+    (src/rt/tracer.cpp:97: the transform of a lone object is not applied)."""
+    from inst_scene import rot, xform, sphere_mesh, rays_at
+    from test_gpu_instances import _lights
+    mesh = sphere_mesh()
+    objects = [(0, *xform(rot((0, 1, 0), 0.7), 2.0, (3.0, 0.0, 0.0)))]
+    n = 2000
+    ori4, dir4 = rays_at(22, n)
+    lights = _lights()
+    _write_objects(tmp_path, [mesh], objects, None, ori4, dir4, lights)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(OBJ_BIN), str(tmp_path), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    b = O.Build(mesh[0], mesh[3], None, 30, 1)
+    o = O.trace(b, mesh[1], mesh[2], ori4, dir4)
+    o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
+    assert ((o["flags"] & 1) != 0).sum() > 100
+    g = _read_pass(out, 0, n)
+    for k in ("flags", "t", "id", "bary", "n4", "p4", "uv2"):
+        assert np.array_equal(g[k], o[k]), f"{k} differs"
