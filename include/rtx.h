@@ -192,6 +192,22 @@ int32_t rtx_scene_set_triangles(rtx_scene* s, const uint32_t* idx, const int32_t
 int32_t rtx_scene_set_materials(rtx_scene* s, const rtx_material* mats, uint32_t M);
 /* Scene::set_dir_light / set_spot_light / set_ambient_light (src/rt/scene.cpp:987-1040). */
 int32_t rtx_scene_set_lights(rtx_scene* s, const rtx_lights* lights);
+/* TextureAtlas::load_texture (src/rt/texture-atlas.cpp:25-60) with the images
+ * decoded by the caller: `count` RGBA8 images concatenated (texel (x, y) of an
+ * image at byte 4*(x + y*width), as create_tex_gl_from_file lays it out,
+ * src/cl-gl/opengl-init.cpp:143-200), dims = width, height per image.  A
+ * material with texture k >= 0 takes its diffuse colour from image k sampled
+ * at the hit's uv (linear filter, repeat, normalized coordinates,
+ * src/kernel/ray-shader.cl:145-153, 187-240); an index without an image
+ * samples black.  count 0 removes the images. */
+#define RTX_MAX_TEXTURES 64
+int32_t rtx_scene_set_textures(rtx_scene* s, const uint8_t* rgba, const int32_t* dims, int32_t count);
+/* Cubemap::initialize and its enable flag (src/rt/cubemap.cpp:4-57): six RGBA8
+ * faces +x, -x, +y, -y, +z, -z concatenated, dims = width, height per face.  A
+ * ray without a hit then adds the colour of the cubemap in its direction (linear
+ * filter, clamp to edge) times its contribution (src/kernel/ray-shader.cl:275-307).
+ * rgba NULL disables the cubemap: a miss adds nothing. */
+int32_t rtx_scene_set_cubemap(rtx_scene* s, const uint8_t* rgba, const int32_t* dims);
 uint32_t rtx_scene_triangle_count(rtx_scene* s);
 
 /* Travelling-wave displacement of a heightfield, on the device: for every

@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
         _lib.orc_morton_one.restype = C.c_uint64
         _lib.orc_raygen_secondary.restype = C.c_uint64
         _lib.orc_render.restype = C.c_uint64
+        _lib.orc_render_tex.restype = C.c_uint64
         _lib.orc_hit_t_f64.restype = C.c_double
     return _lib
 
@@ -211,12 +212,58 @@ def raygen_secondary(mats: np.ndarray, matmap, ori4, dir4, pixel, contrib, hits:
     return o[:m], d[:m], px[:m], c[:m], int(total[0])
 
 
-def shade(mats, matmap, lights, dir4, pixel, contrib, hits, accum):
+class _TexSetC(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("tab", C.c_void_p), ("count", C.c_int32), ("cube", C.c_int32)]
+
+
+class TexSet:
+    """Textures of the shading stage: material textures (h x w x 4 uint8
+    arrays, texel (x, y) at [y, x]) and optionally the six cubemap faces
+    (+x, -x, +y, -y, +z, -z)."""
+
+    def __init__(self, textures=(), cubemap=None):
+        imgs = [np.ascontiguousarray(t, np.uint8) for t in textures]
+        self.count = len(imgs)
+        self.cube = 0
+        if cubemap is not None:
+            faces = [np.ascontiguousarray(f, np.uint8) for f in cubemap]
+            assert len(faces) == 6
+            imgs += faces
+            self.cube = 1
+        tab, off = [], 0
+        for im in imgs:
+            h, w = im.shape[:2]
+            tab += [off, w, h]
+            off += w * h
+        self.rgba = np.ascontiguousarray(np.concatenate([im.reshape(-1, 4) for im in imgs]) if imgs
+                                         else np.zeros((1, 4), np.uint8))
+        self.tab = np.asarray(tab or [0, 1, 1], np.int64)
+        self.c = _TexSetC(self.rgba.ctypes.data, self.tab.ctypes.data, self.count, self.cube)
+
+
+def sample_texture(ts: TexSet, entry: int, st, repeat: bool = True) -> np.ndarray:
+    st = np.ascontiguousarray(st, np.float32).reshape(-1, 2)
+    out = np.zeros((st.shape[0], 3), np.float32)
+    lib().orc_sample_texture(C.byref(ts.c), C.c_int32(entry), P(st), C.c_uint64(st.shape[0]), C.c_int32(int(repeat)),
+                             P(out))
+    return out
+
+
+def sample_cubemap(ts: TexSet, dir4) -> np.ndarray:
+    d = np.ascontiguousarray(dir4, np.float32)
+    out = np.zeros((d.shape[0], 3), np.float32)
+    lib().orc_sample_cubemap(C.byref(ts.c), P(d), C.c_uint64(d.shape[0]), P(out))
+    return out
+
+
+def shade(mats, matmap, lights, dir4, pixel, contrib, hits, accum, tex: TexSet | None = None):
     n = dir4.shape[0]
-    lib().orc_shade(P(mats), P(np.ascontiguousarray(matmap, np.int32)), P(lights),
-                    P(np.ascontiguousarray(dir4, np.float32)), P(np.ascontiguousarray(pixel, np.int32)),
-                    P(np.ascontiguousarray(contrib, np.float32)), P(hits["flags"]), P(hits["id"]), P(hits["n4"]),
-                    P(hits["p4"]), C.c_uint64(n), P(accum))
+    uv2 = hits.get("uv2")
+    lib().orc_shade_tex(P(mats), P(np.ascontiguousarray(matmap, np.int32)), P(lights),
+                        P(np.ascontiguousarray(dir4, np.float32)), P(np.ascontiguousarray(pixel, np.int32)),
+                        P(np.ascontiguousarray(contrib, np.float32)), P(hits["flags"]), P(hits["id"]), P(hits["n4"]),
+                        P(hits["p4"]), P(None if uv2 is None else np.ascontiguousarray(uv2, np.float32)),
+                        C.c_uint64(n), C.byref(tex.c) if tex is not None else None, P(accum))
     return accum
 
 
@@ -226,15 +273,15 @@ def resolve(accum: np.ndarray) -> np.ndarray:
     return out
 
 
-def render(b: Build, nrm3, uv2, mats, lights, cam, w, h, max_bounces, quad=32, cap=0):
+def render(b: Build, nrm3, uv2, mats, lights, cam, w, h, max_bounces, quad=32, cap=0, tex: TexSet | None = None):
     accum = np.zeros((w * h, 3), np.uint32)
     st = np.zeros(1, STATS_DTYPE)
     nrm4 = pos4(np.asarray(nrm3, np.float32))
     uv = np.ascontiguousarray(uv2, np.float32)
     cap = cap or (w * h) << max_bounces
-    rays = lib().orc_render(P(b.pos4), P(nrm4), P(uv), P(b.idx), P(b.matmap), P(b.nodes), P(mats), P(lights), P(cam),
-                            C.c_int(w), C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap), P(accum),
-                            P(st))
+    rays = lib().orc_render_tex(P(b.pos4), P(nrm4), P(uv), P(b.idx), P(b.matmap), P(b.nodes), P(mats), P(lights),
+                                P(cam), C.c_int(w), C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap),
+                                C.byref(tex.c) if tex is not None else None, P(accum), P(st))
     return accum, st[0], int(rays)
 
 

@@ -872,13 +872,102 @@ uint64_t orc_raygen_secondary(const orc_material* mats, const int32_t* matmap, c
 }
 
 // ------------------------------------------------------------ a13 / a14 ----
-// shade_sample (src/kernel/ray-shader.cl:116-333) without textures/cubemap:
-// ambient*diffuse + per unshadowed light (diffuse*cos^2 + Phong^8), times
-// (1 - reflectiveness), contribution-scaled, clamped, floor(x * 1e6) added to
-// the pixel.  accum: uint32 r,g,b per pixel.
+// Textures of the shading stage: RGBA8 texels of every image in one array;
+// entry e of `tab` = {first texel, width, height}.  Entries [0, count) are
+// the material textures (TextureAtlas, src/rt/texture-atlas.cpp:25-60; texel
+// (x, y) at (x + y*width), as create_tex_gl_from_file lays it out,
+// src/cl-gl/opengl-init.cpp:143-200); with `cube`, entries count..count+5
+// are the cubemap faces +x, -x, +y, -y, +z, -z (Cubemap::initialize,
+// src/rt/cubemap.cpp:4-57).
+struct orc_texset {
+    const uint8_t* rgba;
+    const int64_t* tab;
+    int32_t count;
+    int32_t cube;
+};
+
+// UNORM_INT8 texel to float: c / 255.
+static f3 texel(const orc_texset* tx, int e, int x, int y) {
+    const int64_t off = tx->tab[3 * e], w = tx->tab[3 * e + 1];
+    const uint8_t* p = tx->rgba + 4 * (off + x + (int64_t)y * w);
+    return mk(p[0] / 255.f, p[1] / 255.f, p[2] / 255.f);
+}
+
+// read_imagef with CLK_NORMALIZED_COORDS_TRUE | CLK_FILTER_LINEAR and
+// CLK_ADDRESS_REPEAT (material textures) or CLK_ADDRESS_CLAMP_TO_EDGE (the
+// cubemap), ray-shader.cl:145-153: the OpenCL 1.2 formulas (section 8.2),
+//   repeat: u = (s - floor(s)) * w; i0 = floor(u - 0.5), i1 = i0 + 1, each
+//           wrapped into [0, w); clamp: u = s * w, i0, i1 clamped to [0, w);
+//   a = frac(u - 0.5) (likewise v, j0, j1, b);
+//   T = (1-a)(1-b) T00 + a(1-b) T10 + (1-a)b T01 + ab T11,
+// evaluated in float32 in that order.  OpenCL leaves the weights' precision
+// to the implementation; this is the definition both the oracle and the
+// kernel follow.  A non-finite coordinate samples as 0.
+static f3 sample_linear(const orc_texset* tx, int e, float s, float t, bool repeat) {
+    const int w = (int)tx->tab[3 * e + 1], h = (int)tx->tab[3 * e + 2];
+    if (!std::isfinite(s)) s = 0.f;
+    if (!std::isfinite(t)) t = 0.f;
+    float u, v;
+    if (repeat) {
+        u = (s - std::floor(s)) * (float)w;
+        v = (t - std::floor(t)) * (float)h;
+    } else {
+        u = s * (float)w;
+        v = t * (float)h;
+    }
+    const float fu = std::floor(u - 0.5f), fv = std::floor(v - 0.5f);
+    const float a = (u - 0.5f) - fu, b = (v - 0.5f) - fv;
+    int i0, i1, j0, j1;
+    if (repeat) {
+        i0 = (int)fu;
+        j0 = (int)fv;
+        i1 = i0 + 1;
+        j1 = j0 + 1;
+        if (i0 < 0) i0 += w;
+        if (j0 < 0) j0 += h;
+        if (i1 > w - 1) i1 -= w;
+        if (j1 > h - 1) j1 -= h;
+    } else {
+        const int iu = (int)std::fmin(std::fmax(fu, -1.f), (float)w);
+        const int iv = (int)std::fmin(std::fmax(fv, -1.f), (float)h);
+        i0 = std::min(std::max(iu, 0), w - 1);
+        i1 = std::min(std::max(iu + 1, 0), w - 1);
+        j0 = std::min(std::max(iv, 0), h - 1);
+        j1 = std::min(std::max(iv + 1, 0), h - 1);
+    }
+    const float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    return ((w00 * texel(tx, e, i0, j0) + w10 * texel(tx, e, i1, j0)) + w01 * texel(tx, e, i0, j1)) +
+           w11 * texel(tx, e, i1, j1);
+}
+
+static bool in_range(float f1, float f2) { return f1 <= 1.f && f1 >= -1.f && f2 <= 1.f && f2 >= -1.f; }
+
+// The miss branch of shade_sample (ray-shader.cl:275-307): the first face
+// whose projection of the direction lies in [-1, 1]^2, in the order
+// +x, -x, +y, -y, +z, -z; no face (a NaN direction) gives black.
+static f3 cubemap_rgb(const orc_texset* tx, f3 d) {
+    const f3 inv = recip(d);
+    const f3 xp = d * inv.x, yp = d * inv.y, zp = d * inv.z;
+    const int c = tx->count;
+    if (d.x > 0.f && in_range(xp.y, xp.z)) return sample_linear(tx, c + 0, (-xp.z + 1.f) * 0.5f, (xp.y + 1.f) * 0.5f, false);
+    if (d.x < 0.f && in_range(xp.y, xp.z)) return sample_linear(tx, c + 1, (-xp.z + 1.f) * 0.5f, (-xp.y + 1.f) * 0.5f, false);
+    if (d.y > 0.f && in_range(yp.x, yp.z)) return sample_linear(tx, c + 2, (yp.x + 1.f) * 0.5f, (-yp.z + 1.f) * 0.5f, false);
+    if (d.y < 0.f && in_range(yp.x, yp.z)) return sample_linear(tx, c + 3, (-yp.x + 1.f) * 0.5f, (-yp.z + 1.f) * 0.5f, false);
+    if (d.z > 0.f && in_range(zp.x, zp.y)) return sample_linear(tx, c + 4, (zp.x + 1.f) * 0.5f, (zp.y + 1.f) * 0.5f, false);
+    if (d.z < 0.f && in_range(zp.x, zp.y)) return sample_linear(tx, c + 5, (zp.x + 1.f) * 0.5f, (-zp.y + 1.f) * 0.5f, false);
+    return mk(0.f, 0.f, 0.f);
+}
+
+// shade_sample (src/kernel/ray-shader.cl:116-333): ambient*diffuse + per
+// unshadowed light (diffuse*cos^2 + Phong^8), times (1 - reflectiveness),
+// contribution-scaled, clamped, floor(x * 1e6) added to the pixel.  The
+// diffuse colour is the material's, or its texture sampled at the hit's uv
+// (an index without a texture samples black).  accum: uint32 r,g,b per pixel.
 static void shade_one(const orc_material& m, const orc_lights* lights, uint32_t fl, f3 d, f3 n, f3 p,
-                      float contribution, float out[3]) {
+                      float contribution, float out[3], const orc_texset* tx, float tu, float tv) {
     f3 diffuse = mk(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    if (m.texture >= 0)
+        diffuse = (tx && m.texture < tx->count) ? sample_linear(tx, m.texture, tu, tv, true) : mk(0.f, 0.f, 0.f);
     f3 val = ld3(lights->ambient) * diffuse;
     for (int l = 0; l < lights->count; ++l) {
         const orc_light& L = lights->l[l];
@@ -914,18 +1003,55 @@ static void shade_one(const orc_material& m, const orc_lights* lights, uint32_t 
     out[2] = val.z * contribution;
 }
 
-void orc_shade(const orc_material* mats, const int32_t* matmap, const orc_lights* lights, const float* dir4,
-               const int32_t* pixel, const float* contrib, const uint32_t* flags, const int32_t* id, const float* n4,
-               const float* p4, uint64_t count, uint32_t* accum) {
+// tx may be null (no textures, no cubemap); uv2 is read only for textured
+// materials.  Without a cubemap a miss adds nothing (the `return` of the
+// miss branch).
+void orc_shade_tex(const orc_material* mats, const int32_t* matmap, const orc_lights* lights, const float* dir4,
+                   const int32_t* pixel, const float* contrib, const uint32_t* flags, const int32_t* id,
+                   const float* n4, const float* p4, const float* uv2, uint64_t count, const orc_texset* tx,
+                   uint32_t* accum) {
     for (uint64_t k = 0; k < count; ++k) {
-        if (!(flags[k] & ORC_HIT)) continue;  // no cubemap: a miss adds nothing
         float c[3];
-        shade_one(mats[matmap[id[k]]], lights, flags[k], ld3(dir4 + 4 * k), ld3(n4 + 4 * k), ld3(p4 + 4 * k),
-                  contrib[k], c);
+        if (!(flags[k] & ORC_HIT)) {
+            if (!tx || !tx->cube) continue;
+            const f3 rgb = cubemap_rgb(tx, ld3(dir4 + 4 * k));
+            c[0] = contrib[k] * rgb.x;
+            c[1] = contrib[k] * rgb.y;
+            c[2] = contrib[k] * rgb.z;
+        } else {
+            shade_one(mats[matmap[id[k]]], lights, flags[k], ld3(dir4 + 4 * k), ld3(n4 + 4 * k), ld3(p4 + 4 * k),
+                      contrib[k], c, tx, uv2 ? uv2[2 * k] : 0.f, uv2 ? uv2[2 * k + 1] : 0.f);
+        }
         for (int ch = 0; ch < 3; ++ch) {
             float f = clampf(c[ch], 0.f, 1.f);
             accum[3 * (size_t)pixel[k] + ch] += (uint32_t)std::floor(f * 1e6f);
         }
+    }
+}
+
+void orc_shade(const orc_material* mats, const int32_t* matmap, const orc_lights* lights, const float* dir4,
+               const int32_t* pixel, const float* contrib, const uint32_t* flags, const int32_t* id, const float* n4,
+               const float* p4, uint64_t count, uint32_t* accum) {
+    orc_shade_tex(mats, matmap, lights, dir4, pixel, contrib, flags, id, n4, p4, nullptr, count, nullptr, accum);
+}
+
+// One texture sample as the shading stage takes it (for the sampler's pins).
+void orc_sample_texture(const orc_texset* tx, int32_t entry, const float* st, uint64_t n, int32_t repeat,
+                        float* rgb) {
+    for (uint64_t k = 0; k < n; ++k) {
+        const f3 c = sample_linear(tx, entry, st[2 * k], st[2 * k + 1], repeat != 0);
+        rgb[3 * k] = c.x;
+        rgb[3 * k + 1] = c.y;
+        rgb[3 * k + 2] = c.z;
+    }
+}
+
+void orc_sample_cubemap(const orc_texset* tx, const float* dir4, uint64_t n, float* rgb) {
+    for (uint64_t k = 0; k < n; ++k) {
+        const f3 c = cubemap_rgb(tx, ld3(dir4 + 4 * k));
+        rgb[3 * k] = c.x;
+        rgb[3 * k + 1] = c.y;
+        rgb[3 * k + 2] = c.z;
     }
 }
 
@@ -954,10 +1080,10 @@ struct orc_frame_stats {
     uint64_t per_bounce[16];
 };
 
-uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx, const int32_t* matmap,
-                    const orc_node* nodes, const orc_material* mats, const orc_lights* lights, const orc_camera* cam,
-                    int width, int height, int quad, int max_bounces, uint64_t cap, uint32_t* accum,
-                    orc_frame_stats* st) {
+uint64_t orc_render_tex(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx,
+                        const int32_t* matmap, const orc_node* nodes, const orc_material* mats,
+                        const orc_lights* lights, const orc_camera* cam, int width, int height, int quad,
+                        int max_bounces, uint64_t cap, const orc_texset* tx, uint32_t* accum, orc_frame_stats* st) {
     uint64_t P = (uint64_t)width * height;
     if (cap < P) cap = P;
     std::vector<float> o4a(4 * cap), d4a(4 * cap), o4b(4 * cap), d4b(4 * cap), ca(cap), cb(cap);
@@ -983,7 +1109,8 @@ uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const 
               uv2.data(), nullptr);
     orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data(), nullptr);
     count_shadow(n);
-    orc_shade(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), n, accum);
+    orc_shade_tex(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), uv2.data(), n,
+                  tx, accum);
     for (int b = 0; b < max_bounces; ++b) {
         uint64_t m = orc_raygen_secondary(mats, matmap, o_in, d_in, p_in, c_in, fl.data(), id.data(), n4.data(),
                                           p4.data(), n, cap, o_out, d_out, p_out, c_out, nullptr);
@@ -1000,10 +1127,19 @@ uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const 
                   p4.data(), uv2.data(), nullptr);
         orc_trace_shadow(pos, idx, nodes, lights, n, fl.data(), p4.data(), nullptr);
         count_shadow(n);
-        orc_shade(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), n, accum);
+        orc_shade_tex(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), uv2.data(), n,
+                  tx, accum);
     }
     if (st) *st = s;
     return s.primary + s.secondary;
+}
+
+uint64_t orc_render(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx, const int32_t* matmap,
+                    const orc_node* nodes, const orc_material* mats, const orc_lights* lights, const orc_camera* cam,
+                    int width, int height, int quad, int max_bounces, uint64_t cap, uint32_t* accum,
+                    orc_frame_stats* st) {
+    return orc_render_tex(pos, nrm, uvs, idx, matmap, nodes, mats, lights, cam, width, height, quad, max_bounces, cap,
+                          nullptr, accum, st);
 }
 
 // ------------------------------------------------------------ FP64 shadow --

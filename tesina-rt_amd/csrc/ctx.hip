@@ -304,7 +304,7 @@ uint64_t rtx_scene_device_bytes(const rtx_scene* sc) {
     rtx_scene* s = const_cast<rtx_scene*>(sc);
     const DevBuf* all[] = {&s->pos,        &s->nrm,  &s->vnrm,     &s->uv,   &s->idx,    &s->idx_alt, &s->matmap,
                            &s->matmap_alt, &s->orig, &s->orig_alt, &s->mats, &s->lights, &s->waves,   &s->stage,
-                           &s->clock};
+                           &s->clock,      &s->tex,  &s->tex_tab};
     uint64_t b = 0;
     for (const DevBuf* d : all) b += d->p ? d->bytes : 0;
     for (auto& set : s->sets) set.each([&](DevBuf& d) { b += d.p ? d.bytes : 0; });
@@ -317,7 +317,7 @@ int32_t rtx_scene_destroy(rtx_scene* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf* all[] = {&s->pos,        &s->nrm,  &s->vnrm,     &s->uv,   &s->idx,    &s->idx_alt, &s->matmap,
                      &s->matmap_alt, &s->orig, &s->orig_alt, &s->mats, &s->lights, &s->waves,   &s->stage,
-                     &s->clock};
+                     &s->clock,      &s->tex,  &s->tex_tab};
     for (auto& set : s->sets) set.each([](DevBuf& d) { d.release(); });
     for (auto& c : s->rc) {
         c.node.release();
@@ -501,6 +501,82 @@ int32_t rtx_scene_set_lights(rtx_scene* s, const rtx_lights* lights) {
 }
 
 uint32_t rtx_scene_triangle_count(rtx_scene* s) { return s ? s->T : 0; }
+
+// The atlas holds the material images, then the cubemap's faces; it is
+// rebuilt whenever either changes (uploads synchronize: shading in flight may
+// read the old atlas).
+static int32_t upload_atlas(rtx_scene* s) {
+    std::vector<TexEnt> tab;
+    uint64_t off = 0;
+    auto add = [&](const std::vector<int32_t>& dims) {
+        for (size_t k = 0; k + 1 < dims.size(); k += 2) {
+            tab.push_back(TexEnt{off, dims[k], dims[k + 1]});
+            off += (uint64_t)dims[k] * (uint64_t)dims[k + 1];
+        }
+    };
+    add(s->host_tex_dims);
+    if (s->cube) add(s->host_cube_dims);
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    if (tab.empty()) {
+        s->tex.release();
+        s->tex_tab.release();
+        return RTX_OK;
+    }
+    int32_t e = 0;
+    if ((e = s->tex.reserve(s->ctx, off * 4)) || (e = s->tex_tab.reserve(s->ctx, tab.size() * sizeof(TexEnt))))
+        return e;
+    const size_t nt = s->host_tex.size();
+    if (nt) RTX_CHECK_HIP(hipMemcpy(s->tex.p, s->host_tex.data(), nt, hipMemcpyHostToDevice));
+    if (s->cube)
+        RTX_CHECK_HIP(hipMemcpy(static_cast<uint8_t*>(s->tex.p) + nt, s->host_cube.data(), s->host_cube.size(),
+                                hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(s->tex_tab.p, tab.data(), tab.size() * sizeof(TexEnt), hipMemcpyHostToDevice));
+    return RTX_OK;
+}
+
+static int32_t image_bytes(const int32_t* dims, int32_t n, const char* what, uint64_t* bytes) {
+    uint64_t b = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        RTX_REQUIRE(dims[2 * k] > 0 && dims[2 * k + 1] > 0 && dims[2 * k] <= 65536 && dims[2 * k + 1] <= 65536,
+                    RTX_EINVAL, std::string(what) + ": image size out of range");
+        b += 4ull * (uint64_t)dims[2 * k] * (uint64_t)dims[2 * k + 1];
+    }
+    *bytes = b;
+    return RTX_OK;
+}
+
+int32_t rtx_scene_set_textures(rtx_scene* s, const uint8_t* rgba, const int32_t* dims, int32_t count) {
+    RTX_REQUIRE(s && count >= 0 && count <= RTX_MAX_TEXTURES, RTX_EINVAL,
+                "rtx_scene_set_textures: null scene or count out of range");
+    RTX_REQUIRE(count == 0 || (rgba && dims), RTX_EINVAL, "rtx_scene_set_textures: null images");
+    uint64_t bytes = 0;
+    if (count) {
+        int32_t e = image_bytes(dims, count, "rtx_scene_set_textures", &bytes);
+        if (e) return e;
+    }
+    s->host_tex.assign(rgba, rgba + bytes);
+    s->host_tex_dims.assign(dims, dims + 2 * (size_t)count);
+    s->ntex = count;
+    return upload_atlas(s);
+}
+
+int32_t rtx_scene_set_cubemap(rtx_scene* s, const uint8_t* rgba, const int32_t* dims) {
+    RTX_REQUIRE(s, RTX_EINVAL, "rtx_scene_set_cubemap: null scene");
+    if (!rgba) {
+        s->cube = false;
+        s->host_cube.clear();
+        s->host_cube_dims.clear();
+        return upload_atlas(s);
+    }
+    RTX_REQUIRE(dims, RTX_EINVAL, "rtx_scene_set_cubemap: null dims");
+    uint64_t bytes = 0;
+    int32_t e = image_bytes(dims, 6, "rtx_scene_set_cubemap", &bytes);
+    if (e) return e;
+    s->host_cube.assign(rgba, rgba + bytes);
+    s->host_cube_dims.assign(dims, dims + 12);
+    s->cube = true;
+    return upload_atlas(s);
+}
 
 // ----------------------------------------------------------- read-backs --
 static int32_t readback(rtx_scene* s, const DevBuf& b, void* dst, size_t bytes, const char* what) {

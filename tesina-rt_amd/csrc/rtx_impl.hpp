@@ -133,6 +133,13 @@ struct rtx_scene {
     DevBuf idx, idx_alt, matmap, matmap_alt, orig, orig_alt;
     DevBuf mats, lights, waves, stage;
     DevBuf clock;       // uint32 frame number of the device-side animation
+    // Images of the shading stage: RGBA8 texels and a TexEnt per image; the
+    // material textures, then (when the cubemap is on) its six faces.
+    DevBuf tex, tex_tab;
+    int ntex = 0;
+    bool cube = false;
+    std::vector<uint8_t> host_tex, host_cube;  // the images as given (the atlas is rebuilt from both)
+    std::vector<int32_t> host_tex_dims, host_cube_dims;
     int nwaves = 0;     // wave table entries set by rtx_scene_set_waves
     rtx_lights host_lights{};
     // BVH
@@ -161,6 +168,12 @@ struct rtx_scene {
     BuildSet& b() { return sets[cur]; }
     const BuildSet& b() const { return sets[cur]; }
     float4* nodes() const { return sets[cur].nodes(); }
+};
+
+// One image of the shading stage's atlas: first texel, width, height.
+struct TexEnt {
+    uint64_t off;
+    int32_t w, h;
 };
 
 // Buffers the render stages (trace, shadow, secondary generation, shading)

@@ -11,21 +11,123 @@
 
 namespace {
 
+// Images of the shading stage: the material textures, then (cube != 0) the
+// cubemap's six faces +x, -x, +y, -y, +z, -z.
+struct Images {
+    const uchar4* texel;
+    const TexEnt* tab;
+    int count;
+    int cube;
+};
+
+// UNORM_INT8 texel to float: c / 255.
+__device__ __forceinline__ v3 texel(const Images& im, const TexEnt& e, int x, int y) {
+    const uchar4 c = im.texel[e.off + (uint64_t)x + (uint64_t)y * (uint64_t)e.w];
+    return mk3((float)c.x / 255.f, (float)c.y / 255.f, (float)c.z / 255.f);
+}
+
+// read_imagef with normalized coordinates and CLK_FILTER_LINEAR
+// (src/kernel/ray-shader.cl:145-153), REPEAT or CLAMP_TO_EDGE, by the
+// OpenCL 1.2 filter formulas in float32 (the weights' precision is left to
+// the implementation there; this is this library's definition):
+//   repeat: u = (s - floor(s)) * w, i0 = floor(u - 0.5), i1 = i0 + 1, wrapped;
+//   clamp:  u = s * w, i0 and i1 clamped to [0, w);  a = frac(u - 0.5);
+//   T = (1-a)(1-b) T00 + a(1-b) T10 + (1-a)b T01 + ab T11, in that order.
+// A non-finite coordinate samples as 0.
+__device__ v3 sample_linear(const Images& im, int entry, float s, float t, bool repeat) {
+    const TexEnt e = im.tab[entry];
+    const int w = e.w, h = e.h;
+    if (!__builtin_isfinite(s)) s = 0.f;
+    if (!__builtin_isfinite(t)) t = 0.f;
+    float u, v;
+    if (repeat) {
+        u = (s - floorf(s)) * (float)w;
+        v = (t - floorf(t)) * (float)h;
+    } else {
+        u = s * (float)w;
+        v = t * (float)h;
+    }
+    const float fu = floorf(u - 0.5f), fv = floorf(v - 0.5f);
+    const float a = (u - 0.5f) - fu, b = (v - 0.5f) - fv;
+    int i0, i1, j0, j1;
+    if (repeat) {
+        i0 = (int)fu;
+        j0 = (int)fv;
+        i1 = i0 + 1;
+        j1 = j0 + 1;
+        if (i0 < 0) i0 += w;
+        if (j0 < 0) j0 += h;
+        if (i1 > w - 1) i1 -= w;
+        if (j1 > h - 1) j1 -= h;
+    } else {
+        const int iu = (int)fminf(fmaxf(fu, -1.f), (float)w);
+        const int iv = (int)fminf(fmaxf(fv, -1.f), (float)h);
+        i0 = min(max(iu, 0), w - 1);
+        i1 = min(max(iu + 1, 0), w - 1);
+        j0 = min(max(iv, 0), h - 1);
+        j1 = min(max(iv + 1, 0), h - 1);
+    }
+    const float w00 = (1.f - a) * (1.f - b), w10 = a * (1.f - b), w01 = (1.f - a) * b, w11 = a * b;
+    return ((w00 * texel(im, e, i0, j0) + w10 * texel(im, e, i1, j0)) + w01 * texel(im, e, i0, j1)) +
+           w11 * texel(im, e, i1, j1);
+}
+
+__device__ __forceinline__ bool in_range(float f1, float f2) {
+    return f1 <= 1.f && f1 >= -1.f && f2 <= 1.f && f2 >= -1.f;
+}
+
+// The miss branch of shade_sample (src/kernel/ray-shader.cl:275-307): the
+// first face, in the order +x, -x, +y, -y, +z, -z, whose projection of the
+// direction lies in [-1, 1]^2; none (a NaN direction) gives black.
+__device__ v3 cubemap_rgb(const Images& im, v3 d) {
+    const v3 inv = recip(d);
+    const v3 xp = d * inv.x, yp = d * inv.y, zp = d * inv.z;
+    const int c = im.count;
+    if (d.x > 0.f && in_range(xp.y, xp.z)) return sample_linear(im, c + 0, (-xp.z + 1.f) * 0.5f, (xp.y + 1.f) * 0.5f, false);
+    if (d.x < 0.f && in_range(xp.y, xp.z)) return sample_linear(im, c + 1, (-xp.z + 1.f) * 0.5f, (-xp.y + 1.f) * 0.5f, false);
+    if (d.y > 0.f && in_range(yp.x, yp.z)) return sample_linear(im, c + 2, (yp.x + 1.f) * 0.5f, (-yp.z + 1.f) * 0.5f, false);
+    if (d.y < 0.f && in_range(yp.x, yp.z)) return sample_linear(im, c + 3, (-yp.x + 1.f) * 0.5f, (-yp.z + 1.f) * 0.5f, false);
+    if (d.z > 0.f && in_range(zp.x, zp.y)) return sample_linear(im, c + 4, (zp.x + 1.f) * 0.5f, (zp.y + 1.f) * 0.5f, false);
+    if (d.z < 0.f && in_range(zp.x, zp.y)) return sample_linear(im, c + 5, (zp.x + 1.f) * 0.5f, (-zp.y + 1.f) * 0.5f, false);
+    return mk3(0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ void add_pixel(uint32_t* __restrict__ accum, uint32_t px, const float ch[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint32_t u = (uint32_t)floorf(clamp01(ch[q]) * 1e6f);
+        if (u) atomicAdd(&accum[3 * (size_t)px + q], u);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, const int32_t* __restrict__ pixel,
                                                const float* __restrict__ contrib, const uint32_t* __restrict__ flags,
                                                const int32_t* __restrict__ hid, const float4* __restrict__ hn,
-                                               const float4* __restrict__ hp, const rtx_material* __restrict__ mats,
+                                               const float4* __restrict__ hp, const float2* __restrict__ huv,
+                                               const rtx_material* __restrict__ mats,
                                                const int32_t* __restrict__ matmap,
-                                               const rtx_lights* __restrict__ lights, const uint32_t* dcount,
-                                               uint64_t hcount, uint32_t* __restrict__ accum) {
+                                               const rtx_lights* __restrict__ lights, const Images im,
+                                               const uint32_t* dcount, uint64_t hcount,
+                                               uint32_t* __restrict__ accum) {
     const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
     const int nl = lights->count;
     const v3 amb = mk3(lights->ambient[0], lights->ambient[1], lights->ambient[2]);
     for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
         uint32_t fl = flags[r];
-        if (!(fl & 1u)) continue;  // a miss adds nothing without a cubemap
+        if (!(fl & 1u)) {  // a miss adds the cubemap's colour, or nothing without one
+            if (!im.cube) continue;
+            const v3 rgb = cubemap_rgb(im, xyz(dir4[r]));
+            const float c = contrib[r];
+            const float ch[3] = {c * rgb.x, c * rgb.y, c * rgb.z};
+            add_pixel(accum, (uint32_t)pixel[r], ch);
+            continue;
+        }
         const rtx_material m = mats[matmap[hid[r]]];
         v3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        if (m.texture >= 0) {
+            const float2 uv = huv[r];
+            diffuse = m.texture < im.count ? sample_linear(im, m.texture, uv.x, uv.y, true) : mk3(0.f, 0.f, 0.f);
+        }
         v3 d = xyz(dir4[r]);
         v3 nn = xyz(hn[r]);
         v3 p = xyz(hp[r]);
@@ -60,13 +162,8 @@ __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, 
         }
         val = val * (1.f - m.reflectiveness);
         const float c = contrib[r];
-        const uint32_t px = (uint32_t)pixel[r];
-        float ch[3] = {val.x * c, val.y * c, val.z * c};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            uint32_t u = (uint32_t)floorf(clamp01(ch[q]) * 1e6f);
-            if (u) atomicAdd(&accum[3 * (size_t)px + q], u);
-        }
+        const float ch[3] = {val.x * c, val.y * c, val.z * c};
+        add_pixel(accum, (uint32_t)pixel[r], ch);
     }
 }
 
@@ -156,9 +253,11 @@ int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint
     ProfScope ps(s->ctx, st, "shade");
     hipLaunchKernelGGL(k_shade, dim3((uint32_t)g), dim3(256), 0, st, rays->dir4.as<float4>(), rays->pixel.as<int32_t>(),
                        rays->contrib.as<float>(), hits->flags.as<uint32_t>(), hits->id.as<int32_t>(),
-                       hits->n4.as<float4>(), hits->p4.as<float4>(), s->mats.as<rtx_material>(),
-                       scene_view(s).matmap, s->lights.as<rtx_lights>(), rays->count.as<uint32_t>(), count,
-                       fb->accum.as<uint32_t>());
+                       hits->n4.as<float4>(), hits->p4.as<float4>(), hits->uv2.as<float2>(),
+                       s->mats.as<rtx_material>(), scene_view(s).matmap, s->lights.as<rtx_lights>(),
+                       Images{s->tex.as<uchar4>(), s->tex_tab.as<TexEnt>(), s->tex.p ? s->ntex : 0,
+                              s->tex.p && s->cube ? 1 : 0},
+                       rays->count.as<uint32_t>(), count, fb->accum.as<uint32_t>());
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }

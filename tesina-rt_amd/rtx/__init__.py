@@ -103,6 +103,7 @@ SYMBOLS = [
     "rtx_tune_get",
     "rtx_scene_create", "rtx_scene_destroy", "rtx_scene_set_vertices", "rtx_scene_set_vertices_ref",
     "rtx_scene_update_positions", "rtx_scene_set_triangles", "rtx_scene_set_materials", "rtx_scene_set_lights",
+    "rtx_scene_set_textures", "rtx_scene_set_cubemap",
     "rtx_scene_triangle_count", "rtx_scene_displace_waves", "rtx_scene_set_waves", "rtx_scene_set_clock",
     "rtx_scene_get_clock", "rtx_scene_animate",
     "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info",
@@ -231,6 +232,8 @@ def lib() -> C.CDLL:
         "rtx_renderer_capture": (I32, [P, P, P, I32, P]),
         "rtx_renderer_replay": (I32, [P, P]),
         "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
+        "rtx_scene_set_textures": (I32, [P, P, P, I32]),
+        "rtx_scene_set_cubemap": (I32, [P, P, P]),
         "rtx_instances_create": (I32, [P, P, P, P, I32, C.POINTER(P)]),
         "rtx_instances_destroy": (I32, [P]),
         "rtx_instances_set_transforms": (I32, [P, P, P]),
@@ -468,6 +471,24 @@ class Scene:
     def set_lights(self, lights: Lights):
         _check(lib().rtx_scene_set_lights(self.h, C.byref(lights)), "rtx_scene_set_lights")
 
+    def set_textures(self, images):
+        """Material textures: h x w x 4 uint8 arrays, texel (x, y) at [y, x]."""
+        imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+        dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])] or [0], np.int32)
+        data = np.concatenate([im.reshape(-1) for im in imgs]) if imgs else np.zeros(4, np.uint8)
+        _check(lib().rtx_scene_set_textures(self.h, _ptr(data), _ptr(dims), len(imgs)), "rtx_scene_set_textures")
+
+    def set_cubemap(self, faces):
+        """Six h x w x 4 uint8 faces (+x, -x, +y, -y, +z, -z), or None to disable."""
+        if faces is None:
+            _check(lib().rtx_scene_set_cubemap(self.h, None, None), "rtx_scene_set_cubemap")
+            return
+        imgs = [np.ascontiguousarray(f, np.uint8) for f in faces]
+        assert len(imgs) == 6
+        dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])], np.int32)
+        data = np.concatenate([im.reshape(-1) for im in imgs])
+        _check(lib().rtx_scene_set_cubemap(self.h, _ptr(data), _ptr(dims)), "rtx_scene_set_cubemap")
+
     def build(self, depth: int = 30, min_leaf: int = 1, stream=None):
         p = LbvhParams(depth, min_leaf)
         _check(lib().rtx_lbvh_build(self.h, C.byref(p), stream), "rtx_lbvh_build")
@@ -704,6 +725,9 @@ class Renderer:
         """Capture one read-back-free frame into a hipGraph (render once eagerly first)."""
         _check(lib().rtx_renderer_capture(self.h, scene.h, C.byref(cam), int(rebuild), stream),
                "rtx_renderer_capture")
+
+    def set_bounces(self, bounces: int):
+        _check(lib().rtx_renderer_set_bounces(self.h, int(bounces)), "rtx_renderer_set_bounces")
 
     def set_pipelined(self, enable: bool):
         _check(lib().rtx_renderer_set_pipelined(self.h, int(bool(enable))), "rtx_renderer_set_pipelined")
