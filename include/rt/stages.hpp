@@ -19,9 +19,8 @@
 //
 // Differences by design: device memory lives behind opaque rtx handles
 // instead of DeviceMemory ids; FrameBuffer::copy resolves into a device RGBA8
-// image (read with read_rgba8) instead of a GL texture; no OpenGL, OpenCL,
-// textures or cubemap sampling (Cubemap is accepted for signature parity and
-// has no effect).
+// image (read with read_rgba8) instead of a GL texture; no OpenGL or OpenCL;
+// TextureAtlas and Cubemap take decoded RGBA8 images instead of file names.
 #pragma once
 #ifndef RT_STAGES_HPP
 #define RT_STAGES_HPP
@@ -123,6 +122,43 @@ struct FrameStats {
 
 enum AcceleratorType { LBVH_ACCELERATOR, SAH_BVH_ACCELERATOR, KDTREE_ACCELERATOR };
 
+typedef int32_t texture_id;
+
+// TextureAtlas (include/rt/texture-atlas.hpp:14-40; src/rt/texture-atlas.cpp:25-60)
+// with the images decoded by the caller: RGBA8, texel (x, y) at byte
+// 4*(x + y*width), as create_tex_gl_from_file lays them out.  Materials refer
+// to an image by its id; the images reach the device before the next shading.
+class TextureAtlas {
+public:
+    int32_t initialize() { return 0; }
+    int32_t destroy();
+    texture_id load_texture_rgba(const uint8_t* rgba, uint32_t width, uint32_t height);
+    size_t texture_count() const { return m_dims.size() / 2; }
+    static const texture_id invalid_tex_id = -1;
+
+private:
+    friend class Scene;
+    std::vector<uint8_t> m_rgba;
+    std::vector<int32_t> m_dims;
+    bool m_dirty = false;
+};
+
+// Cubemap (include/rt/cubemap.hpp; src/rt/cubemap.cpp:4-57) from six decoded
+// RGBA8 faces, +x, -x, +y, -y, +z, -z; `enabled` as the reference's flag
+// (set by initialize).  Misses take its colour when enabled.
+class Cubemap {
+public:
+    int32_t initialize_rgba(const uint8_t* const faces[6], const uint32_t dims[12]);
+    int32_t destroy();
+    bool enabled = false;
+
+private:
+    friend class Scene;
+    std::vector<uint8_t> m_rgba;
+    std::vector<int32_t> m_dims;
+    uint64_t m_version = 0;
+};
+
 class Scene {
 public:
     Scene();
@@ -172,13 +208,21 @@ public:
     rtx_scene* mesh_handle(int32_t mesh) const { return m_meshes[(size_t)mesh]; }
     int32_t object_mesh(int32_t object) const { return m_obj_mesh[(size_t)object]; }
 
+    // The atlas's images and the cubemap `cm` (enabled or not) on the device
+    // for the shading stage; uploads only what changed.
+    int32_t sync_images(const Cubemap& cm);
+
     Camera camera;
+    TextureAtlas texture_atlas;
+    Cubemap cubemap;
     uint32_t bvh_node_count = 0;
     uint32_t bvh_level_node_count[64] = {};
 
 private:
     friend class BVHBuilder;
     int32_t push_lights();
+    const Cubemap* m_cube_on = nullptr;  // the cubemap on the device, and its version
+    uint64_t m_cube_version = 0;
     rtx_scene* m_h = nullptr;
     rtx_lights m_lights{};
     size_t m_T = 0, m_V = 0;
@@ -220,7 +264,6 @@ private:
     size_t m_count = 0;
 };
 
-class Cubemap {};  // background sampling is outside this path
 
 class FrameBuffer {
 public:
@@ -363,7 +406,6 @@ private:
     RayBundle ray_bundle_1, ray_bundle_2;
     HitBundle hit_bundle;
     FrameBuffer m_framebuffer;
-    Cubemap cubemap;
     BVHBuilder bvh_builder;
     PrimaryRayGenerator prim_ray_gen;
     SecondaryRayGenerator sec_ray_gen;

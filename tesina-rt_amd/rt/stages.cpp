@@ -114,6 +114,68 @@ int32_t Scene::destroy() {
     return 0;
 }
 
+// ------------------------------------------------------- images --
+int32_t TextureAtlas::destroy() {
+    m_rgba.clear();
+    m_dims.clear();
+    m_dirty = true;
+    return 0;
+}
+
+texture_id TextureAtlas::load_texture_rgba(const uint8_t* rgba, uint32_t width, uint32_t height) {
+    if (!rgba || !width || !height || texture_count() >= RTX_MAX_TEXTURES) return invalid_tex_id;
+    m_rgba.insert(m_rgba.end(), rgba, rgba + 4ull * width * height);
+    m_dims.push_back((int32_t)width);
+    m_dims.push_back((int32_t)height);
+    m_dirty = true;
+    return (texture_id)texture_count() - 1;
+}
+
+int32_t Cubemap::initialize_rgba(const uint8_t* const faces[6], const uint32_t dims[12]) {
+    std::vector<uint8_t> data;
+    for (int f = 0; f < 6; ++f) {
+        if (!faces[f] || !dims[2 * f] || !dims[2 * f + 1]) return -1;
+        data.insert(data.end(), faces[f], faces[f] + 4ull * dims[2 * f] * dims[2 * f + 1]);
+    }
+    m_rgba.swap(data);
+    m_dims.assign(dims, dims + 12);
+    ++m_version;
+    enabled = true;
+    return 0;
+}
+
+int32_t Cubemap::destroy() {
+    m_rgba.clear();
+    m_dims.clear();
+    ++m_version;
+    enabled = false;
+    return 0;
+}
+
+int32_t Scene::sync_images(const Cubemap& cm) {
+    if (!m_h) return -1;
+    if (texture_atlas.m_dirty) {
+        const int32_t n = (int32_t)texture_atlas.texture_count();
+        if (rtx_scene_set_textures(m_h, n ? texture_atlas.m_rgba.data() : nullptr,
+                                   n ? texture_atlas.m_dims.data() : nullptr, n) != RTX_OK) {
+            report("Scene::sync_images");
+            return -1;
+        }
+        texture_atlas.m_dirty = false;
+    }
+    const Cubemap* want = (cm.enabled && cm.m_dims.size() == 12) ? &cm : nullptr;
+    if (want != m_cube_on || (want && want->m_version != m_cube_version)) {
+        if (rtx_scene_set_cubemap(m_h, want ? want->m_rgba.data() : nullptr, want ? want->m_dims.data() : nullptr) !=
+            RTX_OK) {
+            report("Scene::sync_images");
+            return -1;
+        }
+        m_cube_on = want;
+        m_cube_version = want ? want->m_version : 0;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------------- objects --
 int32_t Scene::add_mesh(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
                         uint32_t T) {
@@ -562,8 +624,7 @@ int32_t RayShader::initialize() {
 // RayShader::shade (src/rt/ray-shader.cpp:33-154).
 int32_t RayShader::shade(RayBundle& rays, HitBundle& hb, Scene& scene, Cubemap& cm, FrameBuffer& fb, size_t size,
                          bool primary) {
-    (void)cm;
-    if (!m_initialized) return -1;
+    if (!m_initialized || scene.sync_images(cm)) return -1;
     auto t = clk::now();
     if (rtx_shade(scene.handle(), rays.handle(), hb.handle(), fb.handle(), size, primary ? 1 : 0, nullptr) !=
         RTX_OK) {
@@ -692,7 +753,7 @@ uint32_t Renderer::render_to_framebuffer(Scene& scene) {
         stats.stage_times[PRIM_TRACE] += tracer.get_trace_exec_time();
         if (tracer.shadow_trace(scene, (int32_t)cur, *ray_in, hit_bundle)) return (uint32_t)-1;
         stats.stage_times[PRIM_SHADOW_TRACE] += tracer.get_shadow_exec_time();
-        if (ray_shader.shade(*ray_in, hit_bundle, scene, cubemap, m_framebuffer, cur, true)) return (uint32_t)-1;
+        if (ray_shader.shade(*ray_in, hit_bundle, scene, scene.cubemap, m_framebuffer, cur, true)) return (uint32_t)-1;
         stats.stage_times[SHADE] += ray_shader.get_exec_time();
         size_t sec = cur;
         for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
@@ -707,7 +768,7 @@ uint32_t Renderer::render_to_framebuffer(Scene& scene) {
             stats.stage_times[SEC_TRACE] += tracer.get_trace_exec_time();
             if (tracer.shadow_trace(scene, (int32_t)sec, *ray_in, hit_bundle, true)) return (uint32_t)-1;
             stats.stage_times[SEC_SHADOW_TRACE] += tracer.get_shadow_exec_time();
-            if (ray_shader.shade(*ray_in, hit_bundle, scene, cubemap, m_framebuffer, sec)) return (uint32_t)-1;
+            if (ray_shader.shade(*ray_in, hit_bundle, scene, scene.cubemap, m_framebuffer, sec)) return (uint32_t)-1;
             stats.stage_times[SHADE] += ray_shader.get_exec_time();
         }
     }

@@ -55,6 +55,40 @@ def test_cpp_renderer_matches_oracle(tmp_path, frames):
     assert np.array_equal(rgba, O.resolve(oacc))
 
 
+def test_cpp_renderer_textures_and_cubemap(tmp_path):
+    """TextureAtlas and Cubemap of the C++ Scene, shaded by RayShader through
+    the Renderer: the framebuffer equals the oracle's with the same images."""
+    import rtx
+    from test_gpu_textures import _images
+    from rtx import scenes, make_materials
+    sd = scenes.config_c1()
+    sd.uv = (sd.uv * 3.0 - 1.0).astype(np.float32)
+    mats = make_materials([(0.6, 0.5, 0.4, 0.3, 0.0, 0.0), (1.0, 1.0, 1.0, 1.0, 0.8, 0.0)]).view(np.int32)
+    mats[7] = 0
+    sd.materials = mats.view(np.uint8)
+    textures, faces = _images(3)
+    w, h, bounces = 160, 120, 2
+    sd.camera = dict(pos=(0.0, 2.5, -6.0), look_at=(0.0, 1.2, 0.0), up=(0.0, 1.0, 0.0), fov=sd.camera["fov"])
+    write_scene(tmp_path, sd, w, h, bounces, 30, 1, 1)
+    lines = [str(len(textures))] + [f"{t.shape[1]} {t.shape[0]}" for t in textures] + ["1"] + \
+        [f"{f.shape[1]} {f.shape[0]}" for f in faces]
+    (tmp_path / "images.txt").write_text("\n".join(lines) + "\n")
+    for k, t in enumerate(textures):
+        t.tofile(tmp_path / f"tex{k}.rgba")
+    for k, f in enumerate(faces):
+        f.tofile(tmp_path / f"face{k}.rgba")
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(BIN), str(tmp_path), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    acc = np.fromfile(out / "accum.u32", np.uint32).reshape(-1, 3)
+    cam = rtx.camera_set(sd.camera["pos"], sd.camera["look_at"], sd.camera["up"], np.float32(sd.camera["fov"]), w / h)
+    b = O.Build(sd.pos, sd.idx, sd.matmap, 30, 1)
+    oacc, _, _ = O.render(b, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights), O.camera_from(cam), w, h,
+                          bounces, cap=3 * w * h, tex=O.TexSet(textures, faces))
+    assert np.array_equal(acc, oacc)
+
+
 OBJ_BIN = ROOT / "tesina-rt_amd" / "bin" / "stage_objects"
 
 
