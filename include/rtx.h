@@ -45,6 +45,7 @@ typedef struct rtx_server rtx_server;
 typedef struct rtx_batch rtx_batch;
 typedef struct rtx_pipeline rtx_pipeline;
 typedef struct rtx_instances rtx_instances;
+typedef struct rtx_obj rtx_obj;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -289,6 +290,35 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
 /* Tracer::shadow_trace (src/rt/tracer.cpp:273-352;
  * src/kernel/shadow-trace-bvh.cl:160-418): one shadow ray per hit and light. */
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+
+/* ------------------------------------------------------------ OBJ ingest -- */
+/* ModelOBJ::import (src/rt/obj-loader.cpp:163-227) with its MTL reader
+ * (:1124-1290), host only: positions, normals and uvs of the deduplicated
+ * vertices, triangles in file order (polygons fanned), and material groups.
+ * rebuild_normals = the reference's rebuildNormals.  Errors: unreadable
+ * file, a face with fewer than 3 corners, an index out of range. */
+typedef struct rtx_obj_group {
+    uint32_t first_triangle, triangle_count;  /* ModelMesh startIndex / 3, triangleCount */
+    int32_t material;                         /* index in the model's material list */
+    float ambient[4], diffuse[4], specular[4];
+    float shininess, alpha;
+    char name[128], color_map[256], bump_map[256];
+} rtx_obj_group;
+int32_t rtx_obj_load(const char* path, int32_t rebuild_normals, rtx_obj** out);
+int32_t rtx_obj_destroy(rtx_obj* m);
+int32_t rtx_obj_info(rtx_obj* m, uint32_t* vertices, uint32_t* triangles, uint32_t* groups);
+/* ModelOBJ::get_aggregate_mesh (:1354-1390): every vertex and triangle. */
+int32_t rtx_obj_read(rtx_obj* m, float* pos3, float* nrm3, float* uv2, uint32_t* idx3);
+/* Group g (ModelOBJ::ModelMesh and its Material), in the loader's order:
+ * opaque first (by alpha), equal alpha in file order. */
+int32_t rtx_obj_group_info(rtx_obj* m, uint32_t g, rtx_obj_group* out);
+/* ModelOBJ::get_meshes (:1292-1352) for group g: its triangles with vertices
+ * renumbered in order of first use; *vertices = their count (call with
+ * NULL arrays to size them; idx3 holds triangle_count x 3).  The reference
+ * also leaves triangle_count zero triangles ahead of them (a resize followed
+ * by push_back); they are not produced. */
+int32_t rtx_obj_read_group(rtx_obj* m, uint32_t g, uint32_t* vertices, float* pos3, float* nrm3, float* uv2,
+                           uint32_t* idx3);
 
 /* ------------------------------------------------------------ instances -- */
 /* Objects of a scene over per-mesh BVHs (Scene::create_bvhs and

@@ -28,8 +28,15 @@ SPOT, DIR, POINT = 0, 1, 2
 _lib = None
 
 
+REF_TREE = Path("/root/reference")
+
+
 def build() -> Path:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    # the reference's own OBJ loader, when its tree is here (oracle/_ref)
+    if REF_TREE.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "-f", "Makefile.ref", f"REF={REF_TREE}"], check=True,
+                       capture_output=True)
     return LIB
 
 

@@ -91,6 +91,13 @@ class FrameStats(C.Structure):
                 ("frame_ms", C.c_double)]
 
 
+class ObjGroup(C.Structure):
+    _fields_ = [("first_triangle", C.c_uint32), ("triangle_count", C.c_uint32), ("material", C.c_int32),
+                ("ambient", C.c_float * 4), ("diffuse", C.c_float * 4), ("specular", C.c_float * 4),
+                ("shininess", C.c_float), ("alpha", C.c_float), ("name", C.c_char * 128),
+                ("color_map", C.c_char * 256), ("bump_map", C.c_char * 256)]
+
+
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("quad_size", C.c_int32), ("max_bounces", C.c_int32),
                 ("depth", C.c_int32), ("min_leaf", C.c_int32), ("ray_capacity", C.c_uint64)]
@@ -123,6 +130,7 @@ SYMBOLS = [
     "rtx_batch_stats",
     "rtx_pipeline_create", "rtx_pipeline_destroy", "rtx_pipeline_set_animation", "rtx_pipeline_frame",
     "rtx_pipeline_wait", "rtx_pipeline_fb", "rtx_pipeline_stats",
+    "rtx_obj_load", "rtx_obj_destroy", "rtx_obj_info", "rtx_obj_read", "rtx_obj_group_info", "rtx_obj_read_group",
     "rtx_instances_create", "rtx_instances_destroy", "rtx_instances_set_transforms", "rtx_instances_set_lights",
     "rtx_instances_tri_base", "rtx_trace_instances", "rtx_trace_shadow_instances",
     "rtx_camera_set",
@@ -234,6 +242,12 @@ def lib() -> C.CDLL:
         "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
         "rtx_scene_set_textures": (I32, [P, P, P, I32]),
         "rtx_scene_set_cubemap": (I32, [P, P, P]),
+        "rtx_obj_load": (I32, [C.c_char_p, I32, C.POINTER(P)]),
+        "rtx_obj_destroy": (I32, [P]),
+        "rtx_obj_info": (I32, [P, P, P, P]),
+        "rtx_obj_read": (I32, [P, P, P, P, P]),
+        "rtx_obj_group_info": (I32, [P, U32, P]),
+        "rtx_obj_read_group": (I32, [P, U32, P, P, P, P, P]),
         "rtx_instances_create": (I32, [P, P, P, P, I32, C.POINTER(P)]),
         "rtx_instances_destroy": (I32, [P]),
         "rtx_instances_set_transforms": (I32, [P, P, P]),
@@ -638,6 +652,45 @@ def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None)
 
 def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
     _check(lib().rtx_trace_shadow(scene.h, rays.h, hits.h, count, stream), "rtx_trace_shadow")
+
+
+class ObjModel:
+    """A Wavefront OBJ model read by the library's loader (ModelOBJ::import,
+    src/rt/obj-loader.cpp:163-227): `pos`, `nrm` (V x 3), `uv` (V x 2),
+    `idx` (T x 3) as get_aggregate_mesh gives them, and `groups` (one dict per
+    material run, the loader's order)."""
+
+    def __init__(self, path, rebuild_normals: bool = False):
+        h = C.c_void_p()
+        _check(lib().rtx_obj_load(str(path).encode(), int(rebuild_normals), C.byref(h)), "rtx_obj_load")
+        try:
+            V, T, G = C.c_uint32(), C.c_uint32(), C.c_uint32()
+            _check(lib().rtx_obj_info(h, C.byref(V), C.byref(T), C.byref(G)), "rtx_obj_info")
+            self.pos = np.zeros((V.value, 3), np.float32)
+            self.nrm = np.zeros((V.value, 3), np.float32)
+            self.uv = np.zeros((V.value, 2), np.float32)
+            self.idx = np.zeros((T.value, 3), np.uint32)
+            _check(lib().rtx_obj_read(h, _ptr(self.pos), _ptr(self.nrm), _ptr(self.uv), _ptr(self.idx)),
+                   "rtx_obj_read")
+            self.groups = []
+            for g in range(G.value):
+                gi = ObjGroup()
+                _check(lib().rtx_obj_group_info(h, g, C.byref(gi)), "rtx_obj_group_info")
+                nv = C.c_uint32()
+                _check(lib().rtx_obj_read_group(h, g, C.byref(nv), None, None, None, None), "rtx_obj_read_group")
+                gp = np.zeros((nv.value, 3), np.float32)
+                gn = np.zeros((nv.value, 3), np.float32)
+                gu = np.zeros((nv.value, 2), np.float32)
+                gx = np.zeros((gi.triangle_count, 3), np.uint32)
+                _check(lib().rtx_obj_read_group(h, g, C.byref(nv), _ptr(gp), _ptr(gn), _ptr(gu), _ptr(gx)),
+                       "rtx_obj_read_group")
+                self.groups.append(dict(first=gi.first_triangle, count=gi.triangle_count,
+                                        material=gi.material, ambient=list(gi.ambient), diffuse=list(gi.diffuse),
+                                        specular=list(gi.specular), shininess=gi.shininess, alpha=gi.alpha,
+                                        name=gi.name.decode(), color_map=gi.color_map.decode(),
+                                        bump_map=gi.bump_map.decode(), pos=gp, nrm=gn, uv=gu, idx=gx))
+        finally:
+            lib().rtx_obj_destroy(h)
 
 
 class Instances:

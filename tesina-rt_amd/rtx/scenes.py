@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import LIGHT_POINT, make_lights, make_materials
+from . import LIGHT_DIR, LIGHT_POINT, make_lights, make_materials
 
 SEED_BASE = 0x5EED0000
 _M64 = (1 << 64) - 1
@@ -254,4 +254,125 @@ def load(scene, sd: SceneData):
 def camera_of(sd: SceneData):
     from . import camera_set
     c = sd.camera
+    if "dir" in c:
+        return camera_set(c["pos"], None, c["up"], c["fov"], sd.width / sd.height, direction=c["dir"])
     return camera_set(c["pos"], c["look_at"], c["up"], c["fov"], sd.width / sd.height)
+
+
+# ------------------------------------------------------------------ objects --
+# The reference's scenes of objects: meshes from OBJ files, each object a
+# mesh under GeometricProperties (pos, rpy, scale) with a material, merged by
+# Scene::create_aggregate_mesh into one mesh whose vertices are transformed
+# on the host.  Float32 arithmetic in the reference's order of evaluation.
+
+def _mat4_mul(a, b):
+    """sqmat<4>::operator* (include/rt/matrix.hpp:55-66): each entry summed
+    over k = 0..3 from 0 in float32."""
+    r = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        for j in range(4):
+            acc = np.float32(0.0)
+            for k in range(4):
+                acc = np.float32(acc + np.float32(a[i, k] * b[k, j]))
+            r[i, j] = acc
+    return r
+
+
+def geom_matrix(pos=(0.0, 0.0, 0.0), rpy=(0.0, 0.0, 0.0), scale=1.0) -> np.ndarray:
+    """GeometricProperties::updateTransform (src/rt/geom.cpp:35-42): M =
+    translation * rotation * scale, the rotation MY * MP * MR of
+    rotationMatrix4x4 (src/rt/math.cpp:93-127), cos/sin in double then float."""
+    import math
+    f = np.float32
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = np.asarray(pos, np.float32)
+    cr, sr = f(math.cos(f(rpy[0]))), f(math.sin(f(rpy[0])))
+    cp, sp = f(math.cos(f(rpy[1]))), f(math.sin(f(rpy[1])))
+    cy, sy = f(math.cos(f(rpy[2]))), f(math.sin(f(rpy[2])))
+    MR = np.zeros((4, 4), np.float32)
+    MR[2, 2], MR[0, 0], MR[0, 1], MR[1, 0], MR[1, 1] = 1.0, cr, -sr, sr, cr
+    MP = np.zeros((4, 4), np.float32)
+    MP[0, 0], MP[1, 1], MP[1, 2], MP[2, 1], MP[2, 2] = 1.0, cp, -sp, sp, cp
+    MY = np.zeros((4, 4), np.float32)
+    MY[1, 1], MY[0, 0], MY[0, 2], MY[2, 0], MY[2, 2] = 1.0, cy, sy, -sy, cy
+    R = _mat4_mul(_mat4_mul(MY, MP), MR)
+    R[3, 3] = 1.0
+    sc = np.asarray(scale, np.float32)
+    S = np.zeros((4, 4), np.float32)
+    S[0, 0], S[1, 1], S[2, 2] = (sc, sc, sc) if sc.ndim == 0 else sc
+    S[3, 3] = 1.0
+    return _mat4_mul(_mat4_mul(T, R), S)
+
+
+def transform_vertices(M: np.ndarray, pos: np.ndarray, nrm: np.ndarray):
+    """GeometricProperties::transform (src/rt/geom.cpp:45-64): M times (p, 1)
+    and (n, 0), each row summed in index order, then homogenize with
+    1/w (the normal's w set to 1 first)."""
+    M = np.asarray(M, np.float32)
+
+    def mv(v, w):
+        out = np.zeros((v.shape[0], 4), np.float32)
+        for i in range(4):
+            acc = np.zeros(v.shape[0], np.float32)
+            for j in range(3):
+                acc = acc + M[i, j] * v[:, j]
+            out[:, i] = acc + M[i, 3] * np.float32(w)
+        return out
+    p4 = mv(np.asarray(pos, np.float32), 1.0)
+    n4 = mv(np.asarray(nrm, np.float32), 0.0)
+    n4[:, 3] = 1.0
+    ip = (np.float32(1.0) / p4[:, 3]).astype(np.float32)
+    inn = (np.float32(1.0) / n4[:, 3]).astype(np.float32)
+    return (p4[:, :3] * ip[:, None]).astype(np.float32), (n4[:, :3] * inn[:, None]).astype(np.float32)
+
+
+def aggregate(objects, lights, camera, width, height, max_bounces, name="objects") -> SceneData:
+    """Scene::create_aggregate_mesh (src/rt/scene.cpp:495-539): objects =
+    [(model with pos/nrm/uv/idx, M, material row (r, g, b, shininess,
+    reflectiveness, ior))]; material i belongs to object i."""
+    P, N, U, I, MM, rows = [], [], [], [], [], []
+    base = 0
+    for k, (m, M, row) in enumerate(objects):
+        p, n = transform_vertices(M, m.pos, m.nrm)
+        P.append(p), N.append(n), U.append(np.asarray(m.uv, np.float32))
+        I.append(np.asarray(m.idx, np.uint32) + np.uint32(base))
+        MM.append(np.full(len(m.idx), k, np.int32))
+        rows.append(row)
+        base += len(p)
+    return SceneData(np.concatenate(P), np.concatenate(N), np.concatenate(U), np.concatenate(I),
+                     np.concatenate(MM), make_materials(rows), lights, camera, width, height, max_bounces,
+                     meta=dict(name=name))
+
+
+def _normalized(v):
+    """vec3::normalize (include/rt/vector.hpp:151-175): v / sqrt(sum v_i^2)."""
+    v = np.asarray(v, np.float32)
+    n = np.float32(0.0)
+    for x in v:
+        n = np.float32(n + np.float32(x * x))
+    return (v / np.float32(np.sqrt(n))).astype(np.float32)
+
+
+def preset_boat(models_dir, res=(1280, 720), max_bounces=3, view=0) -> SceneData:
+    """boat_set_scene (include/rt/test-params.hpp:10-50): the water frame
+    (scale 2, at (0, -8, 0), blue, reflectiveness 0.98, IOR 1.333) and the
+    boat (same place and scale, red, shininess 1), each loaded as one mesh
+    (load_obj_file_as_aggregate); a directional light toward (0.05, -1,
+    -0.02), colour 0.7, ambient 0.2; the camera at one of the three stats
+    views, FOV pi/4."""
+    import math
+    from pathlib import Path
+    from . import ObjModel
+    d = Path(models_dir)
+    water = ObjModel(d / "frame_water1.obj")
+    boat = ObjModel(d / "frame_boat1.obj")
+    M = geom_matrix((0.0, -8.0, 0.0), (0.0, 0.0, 0.0), 2.0)
+    objects = [(water, M, (0.0, 0.0, 1.0, 0.0, 0.98, 1.333)), (boat, M, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0))]
+    lights = make_lights((0.2, 0.2, 0.2), [dict(type=LIGHT_DIR, dir=tuple(_normalized((0.05, -1.0, -0.02))),
+                                                color=(0.7, 0.7, 0.7))])
+    cams = [((20.0186, -5.49632, 71.8718), (0.131732, 0.0845093, -0.987677)),
+            ((-56.9387, -2.41959, 29.574), (0.927574, -0.22893, -0.295292)),
+            ((68.6859, 5.18034, 13.6691), (-0.820819, -0.478219, -0.31235))]
+    pos, dr = cams[view]
+    cam = dict(pos=pos, dir=dr, up=(0.0, 1.0, 0.0), fov=math.pi / 4)
+    return aggregate(objects, lights, cam, res[0], res[1], max_bounces, name="boat")
