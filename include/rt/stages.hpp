@@ -197,8 +197,19 @@ public:
     // traced as the single-root kernel traces it: its mesh, untransformed.
     int32_t add_mesh(const float* pos, const float* nrm, const float* uv, uint32_t V, const uint32_t* idx,
                      uint32_t T);  // mesh id, or -1
+    // Scene::load_obj_file_as_aggregate / load_obj_file (src/rt/scene.cpp:395-470)
+    // through the library's OBJ loader: the whole model as one mesh, or one
+    // mesh per material group.
+    int32_t load_obj_file_as_aggregate(const std::string& filename);  // mesh id, or -1
+    std::vector<int32_t> load_obj_file(const std::string& filename);
     int32_t add_object(int32_t mesh);  // object id, or -1
     int32_t set_object_transform(int32_t object, const float* tr16, const float* tr_inv16);
+    int32_t set_object_material(int32_t object, const rtx_material& mat);  // Object::mat
+    // Scene::create_aggregate_mesh (src/rt/scene.cpp:495-539): every object's
+    // mesh with its vertices transformed on the host (M (p, 1) and M (n, 0),
+    // homogenized), one material per object, as this scene's geometry; the
+    // BVH builder then builds it and root_count() is 1.
+    int32_t create_aggregate_mesh();
     int32_t create_bvhs(int depth = 30, int min_leaf = 1);
     int32_t update_bvh_roots();
     size_t object_count() const { return m_obj_mesh.size(); }
@@ -228,7 +239,13 @@ private:
     size_t m_T = 0, m_V = 0;
     bool m_built = false;
     std::vector<rtx_scene*> m_meshes;
+    struct HostMesh {
+        std::vector<float> pos, nrm, uv;
+        std::vector<uint32_t> idx;
+    };
+    std::vector<HostMesh> m_host;  // the meshes as given (create_aggregate_mesh reads them)
     std::vector<int32_t> m_obj_mesh;
+    std::vector<rtx_material> m_obj_mat;
     std::vector<float> m_tr, m_tr_inv;
     rtx_instances* m_inst = nullptr;
     bool m_bvhs_built = false;

@@ -107,7 +107,9 @@ int32_t Scene::destroy() {
     m_inst = nullptr;
     for (rtx_scene* m : m_meshes) rtx_scene_destroy(m);
     m_meshes.clear();
+    m_host.clear();
     m_obj_mesh.clear();
+    m_obj_mat.clear();
     m_tr.clear();
     m_tr_inv.clear();
     m_bvhs_built = false;
@@ -193,7 +195,107 @@ int32_t Scene::add_mesh(const float* pos, const float* nrm, const float* uv, uin
         return -1;
     }
     m_meshes.push_back(m);
+    HostMesh h;
+    h.pos.assign(pos, pos + 3 * (size_t)V);
+    h.nrm = nrm ? std::vector<float>(nrm, nrm + 3 * (size_t)V) : std::vector<float>(3 * (size_t)V, 0.f);
+    h.uv = uv ? std::vector<float>(uv, uv + 2 * (size_t)V) : std::vector<float>(2 * (size_t)V, 0.f);
+    h.idx.assign(idx, idx + 3 * (size_t)T);
+    m_host.push_back(std::move(h));
     return (int32_t)m_meshes.size() - 1;
+}
+
+int32_t Scene::load_obj_file_as_aggregate(const std::string& filename) {
+    rtx_obj* o = nullptr;
+    if (rtx_obj_load(filename.c_str(), 0, &o) != RTX_OK) {
+        report("Scene::load_obj_file_as_aggregate");
+        return -1;
+    }
+    uint32_t V = 0, T = 0;
+    rtx_obj_info(o, &V, &T, nullptr);
+    std::vector<float> pos(3 * (size_t)V), nrm(3 * (size_t)V), uv(2 * (size_t)V);
+    std::vector<uint32_t> idx(3 * (size_t)T);
+    rtx_obj_read(o, pos.data(), nrm.data(), uv.data(), idx.data());
+    rtx_obj_destroy(o);
+    return add_mesh(pos.data(), nrm.data(), uv.data(), V, idx.data(), T);
+}
+
+std::vector<int32_t> Scene::load_obj_file(const std::string& filename) {
+    std::vector<int32_t> ids;
+    rtx_obj* o = nullptr;
+    if (rtx_obj_load(filename.c_str(), 0, &o) != RTX_OK) {
+        report("Scene::load_obj_file");
+        return ids;
+    }
+    uint32_t G = 0;
+    rtx_obj_info(o, nullptr, nullptr, &G);
+    for (uint32_t g = 0; g < G; ++g) {
+        rtx_obj_group gi;
+        uint32_t nv = 0;
+        rtx_obj_group_info(o, g, &gi);
+        rtx_obj_read_group(o, g, &nv, nullptr, nullptr, nullptr, nullptr);
+        std::vector<float> pos(3 * (size_t)nv), nrm(3 * (size_t)nv), uv(2 * (size_t)nv);
+        std::vector<uint32_t> idx(3 * (size_t)gi.triangle_count);
+        rtx_obj_read_group(o, g, &nv, pos.data(), nrm.data(), uv.data(), idx.data());
+        const int32_t id = add_mesh(pos.data(), nrm.data(), uv.data(), nv, idx.data(), gi.triangle_count);
+        if (id < 0) break;
+        ids.push_back(id);
+    }
+    rtx_obj_destroy(o);
+    return ids;
+}
+
+int32_t Scene::set_object_material(int32_t object, const rtx_material& mat) {
+    if (object < 0 || (size_t)object >= m_obj_mesh.size()) return -1;
+    m_obj_mat[(size_t)object] = mat;
+    return 0;
+}
+
+// M (v, w) with each row summed in index order, then homogenized by 1/w
+// (GeometricProperties::transform, src/rt/geom.cpp:45-64; the normal's w is
+// set to 1 before).
+static void transform_h(const float* M, const float* v, float w, bool normal, float* out) {
+    float r[4];
+    for (int i = 0; i < 4; ++i) {
+        float acc = 0.f;
+        acc += M[4 * i] * v[0];
+        acc += M[4 * i + 1] * v[1];
+        acc += M[4 * i + 2] * v[2];
+        acc += M[4 * i + 3] * w;
+        r[i] = acc;
+    }
+    if (normal) r[3] = 1.f;
+    const float inv = 1.f / r[3];
+    out[0] = r[0] * inv;
+    out[1] = r[1] * inv;
+    out[2] = r[2] * inv;
+}
+
+int32_t Scene::create_aggregate_mesh() {
+    if (!m_h || m_obj_mesh.empty()) return -1;
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> idx;
+    std::vector<int32_t> matmap;
+    for (size_t o = 0; o < m_obj_mesh.size(); ++o) {
+        const HostMesh& h = m_host[(size_t)m_obj_mesh[o]];
+        const float* M = &m_tr[16 * o];
+        const uint32_t base = (uint32_t)(pos.size() / 3);
+        const size_t V = h.pos.size() / 3;
+        for (size_t v = 0; v < V; ++v) {
+            float p[3], n[3];
+            transform_h(M, &h.pos[3 * v], 1.f, false, p);
+            transform_h(M, &h.nrm[3 * v], 0.f, true, n);
+            pos.insert(pos.end(), p, p + 3);
+            nrm.insert(nrm.end(), n, n + 3);
+        }
+        uv.insert(uv.end(), h.uv.begin(), h.uv.end());
+        for (uint32_t i : h.idx) idx.push_back(i + base);
+        matmap.insert(matmap.end(), h.idx.size() / 3, (int32_t)o);
+    }
+    if (set_geometry(pos.data(), nrm.data(), uv.data(), (uint32_t)(pos.size() / 3), idx.data(), matmap.data(),
+                     (uint32_t)(idx.size() / 3)) ||
+        set_materials(m_obj_mat))
+        return -1;
+    return 0;
 }
 
 // Scene::add_object: identity transform until set_object_transform.
@@ -201,6 +303,9 @@ int32_t Scene::add_object(int32_t mesh) {
     if (mesh < 0 || (size_t)mesh >= m_meshes.size()) return -1;
     static const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     m_obj_mesh.push_back(mesh);
+    rtx_material black{};  // material_cl(): black, texture -1
+    black.texture = -1;
+    m_obj_mat.push_back(black);
     m_tr.insert(m_tr.end(), I, I + 16);
     m_tr_inv.insert(m_tr_inv.end(), I, I + 16);
     return (int32_t)m_obj_mesh.size() - 1;

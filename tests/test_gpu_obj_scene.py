@@ -31,3 +31,27 @@ def test_boat_preset_frame(ctx, view):
     assert np.array_equal(R.fb.accum(), oacc)
     R.destroy()
     s.destroy()
+
+
+@pytest.mark.parametrize("view", [0, 2])
+def test_boat_through_cpp_stage_classes(tmp_path, view):
+    """The same scene built in C++ as boat_set_scene does (load_obj_file_as_aggregate,
+    add_object, create_aggregate_mesh) and rendered by the C++ Renderer."""
+    import subprocess
+    import rtx
+    from rtx import scenes
+    exe = Path(__file__).resolve().parent.parent / "tesina-rt_amd" / "bin" / "stage_boat"
+    assert exe.exists(), "stage_boat not built (run __graft_entry__.build())"
+    w, h = 192, 108
+    r = subprocess.run([str(exe), str(MODELS), str(tmp_path), str(w), str(h), str(view)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    acc = np.fromfile(tmp_path / "accum.u32", np.uint32).reshape(-1, 3)
+    sd = scenes.preset_boat(MODELS, res=(w, h), max_bounces=3, view=view)
+    cam = scenes.camera_of(sd)
+    b = O.Build(sd.pos, sd.idx, sd.matmap, 30, 1)
+    # the C++ Renderer's bundles hold 3 tiles (src/rt/renderer.cpp:376)
+    oacc, _, _ = O.render(b, sd.nrm, sd.uv, sd.materials, O.lights_from(sd.lights), O.camera_from(cam), w, h, 3,
+                          cap=3 * w * h)
+    assert (oacc.sum(1) > 0).mean() > 0.2
+    assert np.array_equal(acc, oacc)
