@@ -238,6 +238,18 @@ int32_t rtx_lbvh_refit(rtx_scene* s, void* stream);
 /* Node count and per-iteration output counts OC[0..63] (scene.bvh_node_count,
  * bvh_level_node_count; src/rt/bvh-builder.cpp:614-625, 733-736).  Synchronizes. */
 int32_t rtx_lbvh_info(rtx_scene* s, uint32_t* node_count, uint32_t* level_counts64);
+/* The CPU SAH BVH, the reference's alternative accelerator (BVH::construct_and_map,
+ * src/rt/bvh.cpp:7-175, via Scene::create_aggregate_bvh): built on the host
+ * from the scene's current triangle order (synchronizes), then the index
+ * buffer, material map and upload ids are permuted into its order and every
+ * traversal form traces it.  A later rtx_lbvh_build replaces it; refit is
+ * LBVH-only.  rtx_lbvh_info / rtx_read_nodes_ref read it back. */
+int32_t rtx_sah_build(rtx_scene* s, void* stream);
+/* The same builder on host arrays only (pos3 V x 3, idx3 T x 3), for
+ * comparison: nodes in the reference layout (cap >= 2T - 1), order =
+ * sorted position -> input triangle. */
+int32_t rtx_sah_build_host(const float* pos3, uint32_t V, const uint32_t* idx3, uint32_t T, rtx_node_ref* nodes,
+                           uint32_t cap, uint32_t* count, uint32_t* order);
 
 /* Parity read-backs (synchronize).  Arrays are in the current sorted order. */
 int32_t rtx_read_codes(rtx_scene* s, uint32_t* codes, uint32_t T);      /* sorted Morton codes */

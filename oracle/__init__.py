@@ -146,6 +146,30 @@ class Build:
             raise RuntimeError("orc_lbvh_refit failed")
 
 
+class Tree:
+    """A tree given from outside, e.g. the reference's CPU SAH BVH
+    (src/rt/bvh.cpp): node boxes (lo[3], hi[3]), words (a, b, parent, leaf)
+    and the triangle order; the index buffer and material map are permuted
+    into that order (BVH::construct_and_map), as Build's are.  The oracle's
+    traversal functions take it where they take a Build."""
+
+    def __init__(self, pos3, idx, matmap, box, words, order):
+        self.pos4 = pos4(np.asarray(pos3, np.float32))
+        order = np.asarray(order, np.int64)
+        self.idx = np.ascontiguousarray(np.asarray(idx, np.uint32).reshape(-1, 3)[order].reshape(-1))
+        self.T = len(order)
+        mm = np.zeros(self.T, np.int32) if matmap is None else np.asarray(matmap, np.int32)
+        self.matmap = np.ascontiguousarray(mm[order])
+        N = len(box)
+        self.nodes = np.zeros(N, NODE_DTYPE)
+        self.nodes["lo"][:, :3] = box[:, :3]
+        self.nodes["hi"][:, :3] = box[:, 3:6]
+        self.nodes["a"], self.nodes["b"] = words[:, 0], words[:, 1]
+        self.nodes["parent"] = words[:, 2]
+        self.nodes["leaf"] = (words[:, 3] != 0).astype(np.int8)
+        self.node_count = N
+
+
 def raygen_primary(cam: np.ndarray, w: int, h: int, quad: int = 32, psamples=None, offset: int = 0, count=None):
     ps = np.ascontiguousarray(np.array([0.0, 0.0, 1.0], np.float32) if psamples is None else psamples, np.float32)
     count = w * h - offset if count is None else count

@@ -567,7 +567,10 @@ Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_nod
             f3 delta = mk(R.lo[0], R.lo[1], R.lo[2]) - mk(L.lo[0], L.lo[1], L.lo[2]);
             bool order = dot(r.dir, delta) > 0.f;
             uint32_t first = order ? n.a : n.b, second = order ? n.b : n.a;
-            stack[level] = second;
+            // trees deeper than the stack: the reference writes past its
+            // 32 entries there (undefined); an entry at 32 or above is never
+            // read, since the ray ends at its next pop, so it is not stored
+            if (level < 32) stack[level] = second;
             level++;
             curr = first;
             pop = false;
@@ -609,7 +612,10 @@ bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc
             f3 delta = mk(R.lo[0], R.lo[1], R.lo[2]) - mk(L.lo[0], L.lo[1], L.lo[2]);
             bool order = dot(r.dir, delta) > 0.f;
             uint32_t first = order ? n.a : n.b, second = order ? n.b : n.a;
-            stack[level] = second;
+            // trees deeper than the stack: the reference writes past its
+            // 32 entries there (undefined); an entry at 32 or above is never
+            // read, since the ray ends at its next pop, so it is not stored
+            if (level < 32) stack[level] = second;
             level++;
             curr = first;
             pop = false;

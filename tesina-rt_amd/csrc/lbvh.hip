@@ -880,6 +880,7 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
     }
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
+    s->sah = false;
     s->depth = depth;
     s->min_leaf = min_leaf;
     return RTX_OK;
@@ -897,6 +898,7 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
     build_tree_phase(s, depth, min_leaf, tree_stream);
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
+    s->sah = false;
     s->depth = depth;
     s->min_leaf = min_leaf;
     return RTX_OK;
@@ -904,9 +906,90 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
 
 int32_t rtx_lbvh_reserve(rtx_scene* s, int depth) { return lbvh_alloc(s, s->T, depth); }
 
+// Scene::create_aggregate_bvh with the SAH accelerator (BVH::construct_and_map,
+// src/rt/bvh.cpp:126-175): the tree is built on the host from the scene's
+// current triangle order, the index buffer, material map and upload ids are
+// permuted into its order, and the nodes go to the current buffer set in the
+// layout every traversal form reads (children adjacent as the builder
+// allocates them, deltas, parents, triangle records).
+int32_t rtx_sah_build(rtx_scene* s, void* stream) {
+    RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_sah_build: null scene");
+    RTX_REQUIRE(s->T > 0 && s->V > 0, RTX_ESTATE, "rtx_sah_build: scene has no triangles");
+    hipStream_t st = s->ctx->pick(stream);
+    const uint32_t T = s->T, V = s->V;
+    RTX_CHECK_HIP(hipDeviceSynchronize());
+    std::vector<float4> p4(V);
+    std::vector<uint32_t> idx(3 * (size_t)T), orig(T);
+    std::vector<int32_t> mm(T);
+    RTX_CHECK_HIP(hipMemcpy(p4.data(), s->pos.p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(idx.data(), s->idx.p, idx.size() * 4, hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(mm.data(), s->matmap.p, T * 4, hipMemcpyDeviceToHost));
+    RTX_CHECK_HIP(hipMemcpy(orig.data(), s->orig.p, T * 4, hipMemcpyDeviceToHost));
+    std::vector<float> p3(3 * (size_t)V);
+    for (uint32_t v = 0; v < V; ++v) {
+        p3[3 * (size_t)v] = p4[v].x;
+        p3[3 * (size_t)v + 1] = p4[v].y;
+        p3[3 * (size_t)v + 2] = p4[v].z;
+    }
+    std::vector<rtx_node_ref> nodes;
+    std::vector<uint32_t> order;
+    uint32_t height = 0;
+    int32_t e = rtx_sah_construct(p3.data(), V, idx.data(), T, nodes, order, &height);
+    if (e) return e;
+    const uint32_t N = (uint32_t)nodes.size();
+    std::vector<uint32_t> idx2(idx.size()), orig2(T);
+    std::vector<int32_t> mm2(T);
+    for (uint32_t i = 0; i < T; ++i) {
+        for (int k = 0; k < 3; ++k) idx2[3 * (size_t)i + k] = idx[3 * (size_t)order[i] + k];
+        mm2[i] = mm[order[i]];
+        orig2[i] = orig[order[i]];
+    }
+    std::vector<float4> nd(2 * (size_t)N), dl(N, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<uint32_t> par(N), lv(LV_WORDS, 0u), depth_of(N, 0u);
+    for (uint32_t n = 0; n < N; ++n) {
+        const rtx_node_ref& r = nodes[n];
+        const bool leaf = r.leaf != 0;
+        if (!leaf) RTX_REQUIRE(r.b == r.a + 1, RTX_EUNSUPPORTED, "rtx_sah_build: children not adjacent");
+        const uint32_t w0 = r.a, w1 = leaf ? r.b - r.a : 0u;
+        float f0, f1;
+        std::memcpy(&f0, &w0, 4);
+        std::memcpy(&f1, &w1, 4);
+        nd[2 * (size_t)n] = make_float4(r.lo[0], r.lo[1], r.lo[2], f0);
+        nd[2 * (size_t)n + 1] = make_float4(r.hi[0], r.hi[1], r.hi[2], f1);
+        if (!leaf) {
+            const rtx_node_ref &L = nodes[r.a], &R = nodes[r.a + 1];
+            dl[n] = make_float4(R.lo[0] - L.lo[0], R.lo[1] - L.lo[1], R.lo[2] - L.lo[2], 0.f);
+            depth_of[r.a] = depth_of[r.a + 1] = depth_of[n] + 1;  // children follow their parent
+        }
+        par[n] = r.parent;
+        if (depth_of[n] < 64) lv[LV_CNT + depth_of[n]]++;
+    }
+    lv[LV_TOTAL] = N;
+    if ((e = lbvh_alloc(s, T, 30))) return e;
+    BuildSet& b = s->b();
+    RTX_CHECK_HIP(hipMemcpy(s->idx.p, idx2.data(), idx2.size() * 4, hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(s->matmap.p, mm2.data(), T * 4, hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(s->orig.p, orig2.data(), T * 4, hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(s->nodes(), nd.data(), nd.size() * sizeof(float4), hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(b.delta.p, dl.data(), dl.size() * sizeof(float4), hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(b.parent.p, par.data(), N * 4, hipMemcpyHostToDevice));
+    RTX_CHECK_HIP(hipMemcpy(b.levels.p, lv.data(), LV_WORDS * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_tri_records, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->idx.as<uint32_t>(),
+                       s->pos.as<float4>(), b.tri.as<float4>(), T);
+    RTX_CHECK_HIP(hipGetLastError());
+    RTX_CHECK_HIP(hipStreamSynchronize(st));
+    s->built = true;
+    s->sah = true;
+    // the deepest node's depth: trees of 32 or more levels keep to the
+    // traversal forms that follow the MAX_LEVELS rule
+    s->depth = (int)std::min<uint32_t>(height, 63);
+    return RTX_OK;
+}
+
 int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
     RTX_REQUIRE(s && s->ctx, RTX_EINVAL, "rtx_lbvh_refit: null scene");
     RTX_REQUIRE(s->built, RTX_ESTATE, "rtx_lbvh_refit: no BVH built");
+    RTX_REQUIRE(!s->sah, RTX_ESTATE, "rtx_lbvh_refit: the tree is an SAH tree (refit follows LBVH levels)");
     hipStream_t st = s->ctx->pick(stream);
     const uint32_t T = s->T;
     {

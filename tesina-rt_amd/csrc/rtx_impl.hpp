@@ -144,6 +144,7 @@ struct rtx_scene {
     rtx_lights host_lights{};
     // BVH
     bool built = false;
+    bool sah = false;   // the tree came from the CPU SAH builder (no level layout: no refit)
     int depth = 30, min_leaf = 1;
     // Render copies of what the render stages read (pipelined frames: the
     // next frame's build writes the build buffers while frames in flight
@@ -317,6 +318,9 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
                              hipStream_t tree_stream);
 // Allocates set s->cur for builds of `depth` levels (host side).
 int32_t rtx_lbvh_reserve(rtx_scene* s, int depth);
+// The CPU SAH builder (tesina-rt_amd/rt/sah.cpp).
+int32_t rtx_sah_construct(const float* pos3, uint32_t V, const uint32_t* idx3, uint32_t T,
+                          std::vector<rtx_node_ref>& out, std::vector<uint32_t>& order, uint32_t* max_depth);
 int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rtx_rays* out, uint64_t count_in,
                                   uint64_t* dev_total_acc, hipStream_t st);
 

@@ -113,7 +113,7 @@ SYMBOLS = [
     "rtx_scene_set_textures", "rtx_scene_set_cubemap",
     "rtx_scene_triangle_count", "rtx_scene_displace_waves", "rtx_scene_set_waves", "rtx_scene_set_clock",
     "rtx_scene_get_clock", "rtx_scene_animate",
-    "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info",
+    "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info", "rtx_sah_build", "rtx_sah_build_host",
     "rtx_read_codes", "rtx_read_perm", "rtx_read_indices", "rtx_read_matmap", "rtx_read_orig_ids",
     "rtx_read_global_bbox", "rtx_read_vertices", "rtx_read_nodes_ref",
     "rtx_rays_create", "rtx_rays_destroy", "rtx_rays_capacity", "rtx_rays_count", "rtx_rays_read", "rtx_rays_write",
@@ -242,6 +242,8 @@ def lib() -> C.CDLL:
         "rtx_camera_set": (I32, [P, P, P, P, F, F, I32]),
         "rtx_scene_set_textures": (I32, [P, P, P, I32]),
         "rtx_scene_set_cubemap": (I32, [P, P, P]),
+        "rtx_sah_build": (I32, [P, P]),
+        "rtx_sah_build_host": (I32, [P, U32, P, U32, P, U32, P, P]),
         "rtx_obj_load": (I32, [C.c_char_p, I32, C.POINTER(P)]),
         "rtx_obj_destroy": (I32, [P]),
         "rtx_obj_info": (I32, [P, P, P, P]),
@@ -507,6 +509,10 @@ class Scene:
         p = LbvhParams(depth, min_leaf)
         _check(lib().rtx_lbvh_build(self.h, C.byref(p), stream), "rtx_lbvh_build")
 
+    def build_sah(self, stream=None):
+        """The CPU SAH BVH (BVH::construct_and_map, src/rt/bvh.cpp:126-175)."""
+        _check(lib().rtx_sah_build(self.h, stream), "rtx_sah_build")
+
     def refit(self, stream=None):
         _check(lib().rtx_lbvh_refit(self.h, stream), "rtx_lbvh_refit")
 
@@ -652,6 +658,21 @@ def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None)
 
 def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
     _check(lib().rtx_trace_shadow(scene.h, rays.h, hits.h, count, stream), "rtx_trace_shadow")
+
+
+def sah_build_host(pos, idx):
+    """The library's CPU SAH builder on host arrays: (nodes as NodeRef records
+    in the reference layout, triangle order)."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    T = idx.shape[0]
+    nodes = (NodeRef * max(1, 2 * T))()
+    count = C.c_uint32()
+    order = np.zeros(T, np.uint32)
+    _check(lib().rtx_sah_build_host(_ptr(pos), pos.shape[0], _ptr(idx), T, nodes, len(nodes), C.byref(count),
+                                    _ptr(order)), "rtx_sah_build_host")
+    raw = np.frombuffer(bytes(nodes), dtype=NODE_DTYPE, count=count.value).copy()
+    return raw, order
 
 
 class ObjModel:
