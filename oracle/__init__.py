@@ -146,6 +146,18 @@ class Build:
             raise RuntimeError("orc_lbvh_refit failed")
 
 
+class stackless:
+    """Context in which every trace of the oracle walks the tree without a
+    stack (src/kernel/trace-bvh-no-stack.cl, shadow-trace-bvh-no-stack.cl)."""
+
+    def __enter__(self):
+        lib().orc_set_traversal(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_traversal(0)
+
+
 class Tree:
     """A tree given from outside, e.g. the reference's CPU SAH BVH
     (src/rt/bvh.cpp): node boxes (lo[3], hi[3]), words (a, b, parent, leaf)

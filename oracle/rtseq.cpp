@@ -537,11 +537,121 @@ struct Hit {
     float t, u, v;
 };
 
+// The stackless walks of src/kernel/trace-bvh-no-stack.cl:408-520 and
+// src/kernel/shadow-trace-bvh-no-stack.cl:213-311, selected for every trace
+// of the oracle by orc_set_traversal(1): the tree is walked by parent
+// pointers (no stack, so no MAX_LEVELS), the near child first as in the
+// stack walk.  Closest hit: per leaf the hits with t <= the leaf's running
+// best (starting at the ray's tMax), merged into the ray's best only when
+// strictly nearer (merge_hit_info, :210-219), so across leaves the first of
+// equal hits is kept.  Going up from the root ends the walk.
+static int g_stackless = 0;
+
+static Hit trace_ray_ns(const Ray& r0, const float* pos, const uint32_t* idx, const orc_node* nodes,
+                        uint32_t root) {
+    Ray r = r0;
+    Hit best{-1, r.tmax, 0.f, 0.f};
+    bool up = false;
+    uint32_t last = root, curr = root, first = 0, second = 0;
+    for (;;) {
+        const orc_node& n = nodes[curr];
+        if (!n.leaf) {
+            const orc_node& L = nodes[n.a];
+            const orc_node& R = nodes[n.b];
+            f3 delta = mk(R.lo[0], R.lo[1], R.lo[2]) - mk(L.lo[0], L.lo[1], L.lo[2]);
+            bool order = dot(r.dir, delta) > 0.f;
+            first = order ? n.a : n.b;
+            second = order ? n.b : n.a;
+        }
+        const uint32_t parent = curr == root ? root : n.parent;
+        if (up) {
+            if (last == root) break;
+            if (last == first) {
+                last = curr;
+                curr = second;
+                up = false;
+            } else {
+                last = curr;
+                curr = parent;
+            }
+        } else if (bbox_hit(n, r, r.tmax)) {
+            if (n.leaf) {
+                Hit info{-1, r.tmax, 0.f, 0.f};
+                for (uint32_t i = n.a; i < n.b; ++i) {
+                    float t, u, v;
+                    if (!tri_hit(pos, idx, i, r, &t, &u, &v)) continue;
+                    if (t <= info.t && t >= r.tmin) info = Hit{(int)i, t, u, v};
+                }
+                if (best.id < 0 || (info.id >= 0 && info.t < best.t)) best = info;
+                if (best.id >= 0) r.tmax = best.t;
+                last = curr;
+                curr = parent;
+                up = true;
+            } else {
+                last = curr;
+                curr = first;
+            }
+        } else {
+            last = curr;
+            curr = parent;
+            up = true;
+        }
+    }
+    return best;
+}
+
+static bool trace_shadow_ns(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes,
+                            uint32_t root) {
+    bool up = false;
+    uint32_t last = root, curr = root, first = 0, second = 0;
+    for (;;) {
+        const orc_node& n = nodes[curr];
+        if (!n.leaf) {
+            const orc_node& L = nodes[n.a];
+            const orc_node& R = nodes[n.b];
+            f3 delta = mk(R.lo[0], R.lo[1], R.lo[2]) - mk(L.lo[0], L.lo[1], L.lo[2]);
+            bool order = dot(r.dir, delta) > 0.f;
+            first = order ? n.a : n.b;
+            second = order ? n.b : n.a;
+        }
+        const uint32_t parent = curr == root ? root : n.parent;
+        if (up) {
+            if (last == root) return false;
+            if (last == first) {
+                last = curr;
+                curr = second;
+                up = false;
+            } else {
+                last = curr;
+                curr = parent;
+            }
+        } else if (bbox_hit(n, r, r.tmax)) {
+            if (n.leaf) {
+                for (uint32_t i = n.a; i < n.b; ++i) {
+                    float t, u, v;
+                    if (tri_hit(pos, idx, i, r, &t, &u, &v) && t < r.tmax && t > r.tmin) return true;
+                }
+                last = curr;
+                curr = parent;
+                up = true;
+            } else {
+                last = curr;
+                curr = first;
+            }
+        } else {
+            last = curr;
+            curr = parent;
+            up = true;
+        }
+    }
+}
+
 // trace_ray: src/kernel/trace-bvh.cl:330-417.  Stack of 32, near child first
 // by dot(dir, r.lo - l.lo) > 0 (the adir branches all test that one scalar).
 Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit,
               uint32_t root = 0) {
     // nvisit (optional): [0] nodes visited, [1] triangles tested; root: bvh_root
+    if (g_stackless) return trace_ray_ns(r, pos, idx, nodes, root);
     Hit best{-1, r.tmax, 0.f, 0.f};
     float tmax = r.tmax;
     uint32_t stack[32];
@@ -590,6 +700,7 @@ Hit trace_ray(const Ray& r, const float* pos, const uint32_t* idx, const orc_nod
 // (:188-234): any hit with tMin < t < tMax.
 bool trace_shadow(const Ray& r, const float* pos, const uint32_t* idx, const orc_node* nodes, uint64_t* nvisit,
                   uint32_t root = 0) {
+    if (g_stackless) return trace_shadow_ns(r, pos, idx, nodes, root);
     uint32_t stack[32];
     uint32_t level = 0, curr = root;
     for (;;) {
@@ -1326,6 +1437,10 @@ void orc_trace_shadow_multi(const float* pos, const uint32_t* idx, const orc_nod
         flags[k] = fl;
     }
 }
+
+// 1: every trace of the oracle walks the tree without a stack (the no-stack
+// kernels); 0: the stack walk of trace-bvh.cl / shadow-trace-bvh.cl.
+void orc_set_traversal(int stackless) { g_stackless = stackless; }
 
 int orc_abi_version(void) { return 1; }
 

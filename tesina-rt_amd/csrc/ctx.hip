@@ -253,7 +253,7 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     RTX_REQUIRE(ctx && key, RTX_EINVAL, "rtx_tune_set: null argument");
     std::string k(key);
     if (k == "trace_variant") {
-        RTX_REQUIRE(value >= 0 && value <= 3, RTX_EINVAL, "rtx_tune_set: trace_variant must be 0, 1, 2 or 3");
+        RTX_REQUIRE(value >= 0 && value <= 4, RTX_EINVAL, "rtx_tune_set: trace_variant must be 0 to 4");
         ctx->tune.trace_variant = value;
     } else if (k == "trace_block") {
         RTX_REQUIRE(value == 64 || value == 128 || value == 256, RTX_EINVAL,

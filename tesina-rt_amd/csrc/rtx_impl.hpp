@@ -63,7 +63,8 @@ struct KernelTiming {
 
 // Tuning knobs (set through rtx_tune_set; picked by the autotuner).
 struct rtx_tuning {
-    int64_t trace_variant = 1;  // 0 single node, 1 + early delta load, 2 sibling pairs, 3 persistent + ray queue
+    int64_t trace_variant = 1;  // 0 single node, 1 + early delta load, 2 sibling pairs, 3 persistent + ray queue,
+                                // 4 stackless walk by parent pointers
     int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
@@ -185,6 +186,7 @@ struct SceneView {
     const float4* tri;
     const float4* nrm;
     const int32_t* matmap;
+    const uint32_t* parent;  // node parents (the stackless walk); null for a render copy
 };
 inline SceneView scene_view(const rtx_scene* s) {
     if (s->use_set >= 0) {
@@ -192,15 +194,17 @@ inline SceneView scene_view(const rtx_scene* s) {
         const bool c = s->use_copy >= 0;
         return SceneView{b.nodes(), b.delta.as<float4>(), b.tri.as<float4>(),
                          c ? s->rc[s->use_copy].nrm.as<float4>() : s->nrm.as<float4>(),
-                         c ? s->rc[s->use_copy].matmap.as<int32_t>() : s->matmap.as<int32_t>()};
+                         c ? s->rc[s->use_copy].matmap.as<int32_t>() : s->matmap.as<int32_t>(),
+                         b.parent.as<uint32_t>()};
     }
     if (s->use_copy >= 0) {
         const rtx_scene::RenderCopy& c = s->rc[s->use_copy];
         return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(c.node.p) + 32),
-                         c.delta.as<float4>(), c.tri.as<float4>(), c.nrm.as<float4>(), c.matmap.as<int32_t>()};
+                         c.delta.as<float4>(), c.tri.as<float4>(), c.nrm.as<float4>(), c.matmap.as<int32_t>(),
+                         nullptr};
     }
     return SceneView{s->nodes(), s->b().delta.as<float4>(), s->b().tri.as<float4>(), s->nrm.as<float4>(),
-                     s->matmap.as<int32_t>()};
+                     s->matmap.as<int32_t>(), s->b().parent.as<uint32_t>()};
 }
 
 struct rtx_rays {

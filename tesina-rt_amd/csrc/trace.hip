@@ -530,14 +530,113 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
     return hit;
 }
 
+// Variant 4: the stackless walks of src/kernel/trace-bvh-no-stack.cl:408-520
+// and shadow-trace-bvh-no-stack.cl:213-311.  Each lane walks its own ray by
+// parent pointers (no stack, hence no MAX_LEVELS), the near child first:
+// down into a hit box, up from a leaf or a missed box, and from the near
+// child on to the far one.  Closest hit: a leaf's hits with t <= its running
+// best (from the ray's tMax) replace the ray's best only when strictly nearer
+// (merge_hit_info), so across leaves the first of equal hits stays.  Going
+// up from the root ends the walk; `bound` (above the 3 visits per node a
+// tree allows) ends a walk over a malformed parent array instead of looping.
+__device__ void traverse_closest_ns(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                    const uint32_t* __restrict__ parent, const float4* __restrict__ tri,
+                                    const RayC& r, Closest& best, uint32_t bound) {
+    bool up = false;
+    uint32_t last = 0, curr = 0, first = 0, second = 0;
+    for (uint32_t it = 0; it < bound; ++it) {
+        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+        if (!cnt) {
+            const bool nl = dot(r.d, xyz(delta[curr])) > 0.f;
+            first = nl ? a : a + 1;
+            second = nl ? a + 1 : a;
+        }
+        const uint32_t up_to = curr == 0 ? 0u : parent[curr];
+        if (up) {
+            if (last == 0) return;
+            if (last == first) {
+                last = curr;
+                curr = second;
+                up = false;
+            } else {
+                last = curr;
+                curr = up_to;
+            }
+        } else if (slab(f0, f1, r, best.t)) {
+            if (cnt) {
+                Closest info{best.t, 0.f, 0.f, -1};
+                leaf_closest(tri, a, cnt, r, info);
+                if (info.id >= 0 && (best.id < 0 || info.t < best.t)) best = info;
+                last = curr;
+                curr = up_to;
+                up = true;
+            } else {
+                last = curr;
+                curr = first;
+            }
+        } else {
+            last = curr;
+            curr = up_to;
+            up = true;
+        }
+    }
+}
+
+__device__ bool traverse_any_ns(const float4* __restrict__ node, const float4* __restrict__ delta,
+                                const uint32_t* __restrict__ parent, const float4* __restrict__ tri, const RayC& r,
+                                float tmax, uint32_t bound) {
+    bool up = false;
+    uint32_t last = 0, curr = 0, first = 0, second = 0;
+    for (uint32_t it = 0; it < bound; ++it) {
+        const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+        const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+        if (!cnt) {
+            const bool nl = dot(r.d, xyz(delta[curr])) > 0.f;
+            first = nl ? a : a + 1;
+            second = nl ? a + 1 : a;
+        }
+        const uint32_t up_to = curr == 0 ? 0u : parent[curr];
+        if (up) {
+            if (last == 0) return false;
+            if (last == first) {
+                last = curr;
+                curr = second;
+                up = false;
+            } else {
+                last = curr;
+                curr = up_to;
+            }
+        } else if (slab(f0, f1, r, tmax)) {
+            if (cnt) {
+                if (leaf_any(tri, a, cnt, r, tmax)) return true;
+                last = curr;
+                curr = up_to;
+                up = true;
+            } else {
+                last = curr;
+                curr = first;
+            }
+        } else {
+            last = curr;
+            curr = up_to;
+            up = true;
+        }
+    }
+    return false;
+}
+
 // Variant dispatch: 0 = single node, 1 = single node + early delta load,
 // 2 = sibling pairs.
 // Called by every lane of the wave; `start` says whether the lane has a ray.
 template <int VAR, class Stack>
 __device__ __forceinline__ void closest_var(const float4* __restrict__ node, const float4* __restrict__ delta,
                                             const float4* __restrict__ tri, const RayC& r, Closest& best,
-                                            Stack& stk, uint32_t coop, bool start) {
-    if (VAR == 2) {
+                                            Stack& stk, uint32_t coop, bool start,
+                                            const uint32_t* __restrict__ parent = nullptr, uint32_t bound = 0) {
+    if (VAR == 4) {
+        if (start) traverse_closest_ns(node, delta, parent, tri, r, best, bound);
+    } else if (VAR == 2) {
         if (start) traverse_closest(node, tri, r, best, stk);
     } else {
         traverse_closest_node<VAR == 1>(node, delta, tri, r, best, stk, coop, start);
@@ -547,7 +646,13 @@ __device__ __forceinline__ void closest_var(const float4* __restrict__ node, con
 template <int VAR, class Stack>
 __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const float4* __restrict__ delta,
                                         const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk,
-                                        uint32_t coop, bool start) {
+                                        uint32_t coop, bool start, const uint32_t* __restrict__ parent = nullptr,
+                                        uint32_t bound = 0) {
+    if (VAR == 4) {
+        bool h = false;
+        if (start) h = traverse_any_ns(node, delta, parent, tri, r, tmax, bound);
+        return h;
+    }
     if (VAR == 2) {
         bool h = false;
         if (start) h = traverse_any(node, tri, r, tmax, stk);
@@ -612,8 +717,9 @@ __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, c
                                               int32_t* __restrict__ idout, float2* __restrict__ bary,
                                               float4* __restrict__ n4, float4* __restrict__ p4,
                                               float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
-                                              uint32_t* __restrict__ hit_count, uint32_t xcds, uint32_t coop) {
-    __shared__ uint32_t s_id[STACK * TB];
+                                              uint32_t* __restrict__ hit_count, uint32_t xcds, uint32_t coop,
+                                              const uint32_t* __restrict__ parent, uint32_t bound) {
+    __shared__ uint32_t s_id[VAR == 4 ? 1 : STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
     const uint32_t lane = threadIdx.x & 63;
@@ -628,7 +734,7 @@ __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, c
             rc = make_ray(xyz(o), xyz(dd), o.w);
             best.t = dd.w;
         }
-        closest_var<VAR>(node, delta, tri, rc, best, stk, coop, active);  // whole wave
+        closest_var<VAR>(node, delta, tri, rc, best, stk, coop, active, parent, bound);  // whole wave
         const bool hit = active && best.id >= 0;
         const uint64_t hb = ballot(hit);
         uint32_t wbase = 0;
@@ -694,8 +800,8 @@ template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
                                                ShadowSegs segs, unsigned long long* shadow_count, uint32_t xcds,
-                                               uint32_t coop) {
-    __shared__ uint32_t s_id[STACK * TB];
+                                               uint32_t coop, const uint32_t* __restrict__ parent, uint32_t bound) {
+    __shared__ uint32_t s_id[VAR == 4 ? 1 : STACK * TB];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_end[RTX_SEG_MAX];
     LdsStack<TB> stk{s_id, threadIdx.x};
@@ -756,7 +862,7 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
                 rc = make_ray(p, dir, 0.01f);
             }
         }
-        if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
+        if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace, parent, bound)) shadowed = true;  // whole wave
         if (valid && shadowed) atomicOr(&fl[rr], 1u << (8 + l));
     }
     if (shadow_count) {
@@ -1036,6 +1142,18 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
 }
 
 // Launch macro M(variant, block) with the tuned variant and block size.
+// Scene kernels: also the stackless form (4).
+#define RTX_DISPATCH_TRAVERSAL_NS(M, var, tb)                 \
+    do {                                                      \
+        if (var == 4) {                                       \
+            if (tb == 64) M(4, 64);                           \
+            else if (tb == 256) M(4, 256);                    \
+            else M(4, 128);                                   \
+        } else {                                              \
+            RTX_DISPATCH_TRAVERSAL(M, var, tb);               \
+        }                                                     \
+    } while (0)
+
 #define RTX_DISPATCH_TRAVERSAL(M, var, tb)     \
     do {                                       \
         if (tb == 64) {                        \
@@ -1450,10 +1568,17 @@ int32_t rtx_raygen_primary(rtx_rays* rays, const rtx_camera* cam, int32_t width,
 // whose boxes pass, so it cannot follow the reference's stack count, which
 // ends a ray at a pop with MAX_LEVELS entries; that count can only reach
 // MAX_LEVELS in trees of 32 levels, which therefore use the single-node form.
-static int traversal_variant(const rtx_scene* s) {
+static int traversal_variant(const rtx_scene* s, const SceneView& sv) {
     const int var = (int)s->ctx->tune.trace_variant;
+    // the stackless form needs the parents of the nodes it reads (a render
+    // copy of the pipelined frame holds none): the single-node form instead
+    if (var == 4 && !sv.parent) return 1;
     return (var == 2 && s->depth >= STACK) ? 1 : var;
 }
+
+// Iteration bound of the stackless walk: a tree of N <= 2T - 1 nodes allows
+// at most 3 visits per node.
+static uint32_t stackless_bound(const rtx_scene* s) { return 6u * s->T + 8u; }
 
 int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
     RTX_REQUIRE(s && rays && hits, RTX_EINVAL, "rtx_trace: null argument");
@@ -1467,7 +1592,7 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());  // hit count, queue
     ProfScope ps(s->ctx, st, "trace");
     const SceneView sv = scene_view(s);
-    const int var = traversal_variant(s), tb = (int)s->ctx->tune.trace_block;
+    const int var = traversal_variant(s, sv), tb = (int)s->ctx->tune.trace_block;
     if (var == 3) {
 #define RTX_TRACE_P(B)                                                                                              \
     hipLaunchKernelGGL((k_trace_p<B>), dim3(persist_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,            \
@@ -1496,8 +1621,8 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
                        hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>(),           \
-                       (uint32_t)s->ctx->tune.xcds, coop_threshold(s->ctx))
-    RTX_DISPATCH_TRAVERSAL(RTX_TRACE_LAUNCH, var, tb);
+                       (uint32_t)s->ctx->tune.xcds, coop_threshold(s->ctx), sv.parent, stackless_bound(s))
+    RTX_DISPATCH_TRAVERSAL_NS(RTX_TRACE_LAUNCH, var, tb);
 #undef RTX_TRACE_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
     hits->list_valid = true;
@@ -1541,7 +1666,7 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     ProfScope ps(s->ctx, st, "trace_shadow");
     const SceneView sv = scene_view(s);
     uint64_t n = max_items * (uint64_t)nl;
-    const int var = traversal_variant(s), tb = (int)s->ctx->tune.trace_block;
+    const int var = traversal_variant(s, sv), tb = (int)s->ctx->tune.trace_block;
     if (var == 3) {
         // queue counter: word 2 of the first segment's hit-list counters
         uint32_t* queue = hits[0]->list_count.as<uint32_t>() + 2;
@@ -1565,8 +1690,8 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,    \
                        sv.delta, sv.tri, s->lights.as<rtx_lights>(), segs,                                     \
                        (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
-                       coop_threshold(s->ctx))
-    RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_LAUNCH, var, tb);
+                       coop_threshold(s->ctx), sv.parent, stackless_bound(s))
+    RTX_DISPATCH_TRAVERSAL_NS(RTX_SHADOW_LAUNCH, var, tb);
 #undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;

@@ -3,7 +3,8 @@ alternative accelerator, src/rt/bvh.cpp): the scene's tree, index buffer and
 material map after the build equal the reference builder's (the committed
 outputs of its own code), and the hits and shadow flags of every traversal
 form, and whole frames, equal the oracle's traversal over that same
-reference tree, bit for bit.  Cases: a 5000-triangle soup, geometry below
+reference tree, bit for bit, for the stack forms (0-3) and the stackless
+walk (4, trace-bvh-no-stack.cl).  Cases: a 5000-triangle soup, geometry below
 zero, duplicated triangles, a flat axis, a regular grid, the 36-level tree
 (beyond the stack: the MAX_LEVELS rule and the fallback from the
 sibling-pair form) and a shaded terrain scene."""
@@ -100,6 +101,22 @@ def test_sah_tree_traversal(ctx, case):
                 assert np.array_equal(got[k], o[k]), f"{case}: {k} differs (variant {variant})"
     finally:
         ctx.tune("trace_variant", old)
+    # the stackless walk (form 4) against the oracle's stackless walk: on the
+    # 36-level tree it does not end rays at MAX_LEVELS
+    with O.stackless():
+        ons = O.trace(tree, nrm, uv, ori4, dir4)
+        ons["flags"] = O.trace_shadow(tree, O.lights_from(lights), ons["flags"], ons["p4"])
+    if case == "deep":
+        assert ((ons["id"] >= 0) & (o["id"] < 0)).sum() > 100
+    try:
+        ctx.tune("trace_variant", 4)
+        rtx.trace(s, r, hb, n)
+        rtx.trace_shadow(s, r, hb, n)
+        got = hb.read(n)
+        for k in ("flags", "id", "t", "bary", "n4", "p4", "uv2"):
+            assert np.array_equal(got[k], ons[k]), f"{case}: {k} differs (stackless)"
+    finally:
+        ctx.tune("trace_variant", old)
     with pytest.raises(rtx.RtxError, match="SAH"):
         s.refit()
     r.destroy()
@@ -125,6 +142,16 @@ def test_sah_frame(ctx):
     oacc, ost, orays = O.render(tree, nrm, uv, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
     assert st.primary_rays + st.secondary_rays == orays and st.secondary_rays > 0
     assert np.array_equal(R.fb.accum(), oacc)
+    # the stackless walk renders the oracle's stackless frame
+    old = ctx.tuned("trace_variant")
+    try:
+        ctx.tune("trace_variant", 4)
+        R.frame(s, cam, rebuild=False)
+        with O.stackless():
+            oacc_ns, _, _ = O.render(tree, nrm, uv, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
+        assert np.array_equal(R.fb.accum(), oacc_ns)
+    finally:
+        ctx.tune("trace_variant", old)
     # a later LBVH build replaces the SAH tree
     s.build(30, 1)
     b = O.Build(pos, idx[z["order"]], mm[z["order"]], 30, 1)

@@ -77,6 +77,16 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
             g = hb.read(n)
             for k in ("flags", "id", "t", "bary", "n4", "p4"):
                 assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant}, T {T}, depth {depth})"
+        # the stackless walk (parent pointers of the LBVH) against the oracle's
+        with O.stackless():
+            ons = O.trace(b, nrm, uv, ori4, dir4)
+            ons["flags"] = O.trace_shadow(b, O.lights_from(lights), ons["flags"], ons["p4"])
+        ctx.tune("trace_variant", 4)
+        rtx.trace(s, r, hb, n)
+        rtx.trace_shadow(s, r, hb, n)
+        g = hb.read(n)
+        for k in ("flags", "id", "t", "bary", "n4", "p4"):
+            assert np.array_equal(g[k], ons[k]), f"{k} differs (stackless, T {T}, depth {depth})"
     finally:
         ctx.tune("trace_variant", old)
     r.destroy()
@@ -84,7 +94,7 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
     s.destroy()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_max_levels_rule(ctx, variant):
     # A ray whose near children all lie on the chain has 32 entries on its
     # stack at the bottom leaf; the reference then ends the ray instead of
@@ -117,6 +127,11 @@ def test_max_levels_rule(ctx, variant):
     bid, _ = O.trace_brute(b, ori4, dir4)
     assert ((o["id"] >= 0) != (bid >= 0)).sum() > 100, "the rule should decide some rays"
     o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
+    if variant == 4:  # the stackless walk has no MAX_LEVELS: it finds what the brute force finds
+        with O.stackless():
+            o = O.trace(b, nrm, uv, ori4, dir4)
+            o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
+        assert np.array_equal(o["id"] >= 0, bid >= 0)
     s = load_scene(ctx, pos, idx, nrm, uv, np.zeros(T, np.int32), lights=lights)
     s.build(32, 1)
     r = rtx.Rays(ctx, n)

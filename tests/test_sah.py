@@ -90,3 +90,37 @@ def test_oracle_on_a_tree_deeper_than_the_stack():
     assert not ((h["id"] >= 0) & (bid < 0)).any()
     assert (h["t"][both] >= bt[both]).all()
     assert ((h["id"] >= 0) != (bid >= 0)).sum() > 100
+
+
+def test_stackless_walk_pins():
+    """The oracle's stackless walk (trace-bvh-no-stack.cl): with no stack it
+    has no MAX_LEVELS cut, so on the 36-level tree it finds what a brute
+    force finds (the same triangle and t: the rays have no exact ties); on a
+    shallow tree it equals the stack walk; shadow flags equal brute force."""
+    import oracle as O
+    from rtx import make_lights, LIGHT_POINT
+    from test_gpu_sah import axis_rays, _rays
+    z = np.load(GOLD / "deep.npz")
+    tree = O.Tree(z["pos"], z["idx"], None, z["box"], z["words"], z["order"])
+    ori4, dir4 = axis_rays(7, 4096)
+    nrm = np.tile(np.float32([0, 0, 1]), (len(z["pos"]), 1))
+    uv = np.zeros((len(z["pos"]), 2), np.float32)
+    with O.stackless():
+        h = O.trace(tree, nrm, uv, ori4, dir4)
+    bid, bt = O.trace_brute(tree, ori4, dir4)
+    assert np.array_equal(h["id"], bid) and np.array_equal(h["t"][bid >= 0], bt[bid >= 0])
+    z = np.load(GOLD / "soup_5000.npz")
+    tree = O.Tree(z["pos"], z["idx"], None, z["box"], z["words"], z["order"])
+    nrm = np.tile(np.float32([0, 0, 1]), (len(z["pos"]), 1))
+    uv = np.zeros((len(z["pos"]), 2), np.float32)
+    ori4, dir4 = _rays(31, 4096, z["pos"])
+    stack = O.trace(tree, nrm, uv, ori4, dir4)
+    with O.stackless():
+        ns = O.trace(tree, nrm, uv, ori4, dir4)
+    for k in ("id", "t", "bary", "n4", "p4"):
+        assert np.array_equal(stack[k], ns[k]), k
+    lights = O.lights_from(make_lights((0.1, 0.1, 0.1), [dict(type=LIGHT_POINT, pos=(0.0, 80.0, 0.0))]))
+    with O.stackless():
+        f = O.trace_shadow(tree, lights, stack["flags"], stack["p4"])
+    fb = O.trace_shadow_brute(tree, lights, stack["flags"], stack["p4"])
+    assert ((f >> 8) & 1).sum() > 100 and np.array_equal(f, fb)
