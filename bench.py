@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--c5-max-requests", type=int, default=600)
     ap.add_argument("--c4-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip)")
     ap.add_argument("--inst-frames", type=int, default=50, help="frames of the instanced leg (0 = skip)")
+    ap.add_argument("--accel-frames", type=int, default=10,
+                    help="frames of the LBVH / SAH, stack / stackless leg (0 = skip)")
     ap.add_argument("--sort-sizes", type=str, default="16777216,67108864",
                     help="element counts of the sort-scaling leg ('' = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -349,6 +351,8 @@ def main():
         log(f"C3: {out['c3_dynamic']['ms_per_frame']} ms/frame, {out['c3_dynamic']['mrays']} Mrays/s")
     if args.inst_frames > 0:
         out["instances"] = bench_instances(ctx, args.inst_frames)
+    if args.accel_frames > 0:
+        out["accelerators"] = bench_accelerators(ctx, args.accel_frames)
     if args.sort_sizes:
         out["sort_scaling"] = bench_sort_scaling(ctx, [int(x) for x in args.sort_sizes.split(",") if x])
     if args.c4_scenes > 0:
@@ -830,6 +834,64 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
             "aggregate_stage_avg_ms": agg_kern,
             "aggregate_note": "the same rays over one aggregate mesh of the 16 moved copies: vertex upload "
                               "(positions moved on the host beforehand, not timed) + full LBVH rebuild per frame"}
+
+
+def bench_accelerators(ctx, frames: int = 10):
+    """The C2 frame (1M triangles, 1920x1080, Whitted depth 3) rendered
+    without a rebuild over the two trees the reference can build -- the
+    LBVH and the CPU SAH BVH (src/rt/bvh.cpp) -- each with the stack walk
+    (form 1) and the stackless walk (form 4, trace-bvh-no-stack.cl); the SAH
+    tree's host build time beside the LBVH's device build."""
+    import numpy as np
+    import rtx
+    from rtx import scenes
+    sd = scenes.config_c2()
+    s = scenes.load(rtx.Scene(ctx), sd)
+    cam = scenes.camera_of(sd)
+    R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces)
+    out = {"workload": "C2 frame without rebuild (1M triangles, 1920x1080, 2 point lights, Whitted depth 3) over "
+                       "the LBVH and the CPU SAH BVH, stack walk (form 1) and stackless walk (form 4)"}
+    old = ctx.tuned("trace_variant")
+
+    def measure(tag):
+        row = {}
+        for form in (1, 4):
+            ctx.tune("trace_variant", form)
+            R.frame_async(s, cam, rebuild=False)
+            ctx.device_sync()
+            t = time.perf_counter()
+            for _ in range(frames):
+                R.frame_async(s, cam, rebuild=False)
+            ctx.device_sync()
+            ms = (time.perf_counter() - t) / frames * 1e3
+            ctx.tune("side_stream", 0)
+            ctx.profile(True)
+            ctx.profile_reset()
+            R.frame_async(s, cam, rebuild=False)
+            rep = ctx.profile_report()
+            ctx.profile(False)
+            ctx.tune("side_stream", 1)
+            st = R.last_stats()
+            rays = st.primary_rays + st.secondary_rays
+            row[f"form{form}"] = {"ms_per_frame": round(ms, 4), "mrays": round(rays / ms / 1e3, 1),
+                                  "trace_ms": round(rep.get("trace", (0, 0.0))[1], 4),
+                                  "shadow_ms": round(rep.get("trace_shadow", (0, 0.0))[1], 4)}
+        ctx.tune("trace_variant", old)
+        out[tag] = row
+
+    s.build(30, 1)
+    ctx.device_sync()
+    out["lbvh_nodes"] = int(s.info()[0])
+    measure("lbvh")
+    t = time.perf_counter()
+    s.build_sah()
+    out["sah_host_build_s"] = round(time.perf_counter() - t, 3)
+    out["sah_nodes"] = int(s.info()[0])
+    measure("sah")
+    log(f"accelerators: LBVH {out['lbvh']}, SAH {out['sah']} (host build {out['sah_host_build_s']} s)")
+    R.destroy()
+    s.destroy()
+    return out
 
 
 def bench_sort_scaling(ctx, sizes):
