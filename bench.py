@@ -59,6 +59,19 @@ def cpu_baseline(sd, cam_struct):
     return rays / dt / 1e6, dt, rays, tb
 
 
+def cpu_visit_counts(sd, cam_struct, stride: int = 16):
+    """Node visits and triangle tests per primary ray, counted by the oracle's
+    traversal (deterministic) over every `stride`-th primary ray of the frame:
+    the fetch counts behind the traversal's effective bytes."""
+    sys.path.insert(0, str(ROOT))
+    import oracle as O
+    b = O.Build(sd.pos, sd.idx, sd.matmap, 30, 1)
+    o4, d4, _, _ = O.raygen_primary(O.camera_from(cam_struct), sd.width, sd.height)
+    o4, d4 = o4[::stride].copy(), d4[::stride].copy()
+    h = O.trace(b, sd.nrm, sd.uv, o4, d4, count_visits=True)
+    return float(h["visits"]) / len(o4), float(h["tri_tests"]) / len(o4), len(o4)
+
+
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for
@@ -300,6 +313,20 @@ def main():
     rl_sort = roofline("lbvh_sort_pass", "k_sort_pass (onesweep 8-bit LSD pass over (code, id) pairs)", 16.0 * T,
                        "16 B per element per pass (8 B read + 8 B write)")
     trace_ms = kernels.get("trace", {}).get("ms_per_frame", 0.0)
+    # the primary launch alone (2.07M rays), event-timed on its stream
+    prim_rays, prim_hits = rtx.Rays(ctx, sd.width * sd.height), rtx.Hits(ctx, sd.width * sd.height)
+    prim_rays.primary(cam, sd.width, sd.height)
+    rtx.trace(s, prim_rays, prim_hits)
+    ctx.device_sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(5):
+        rtx.trace(s, prim_rays, prim_hits)
+    rep_p = ctx.profile_report()
+    ctx.profile(False)
+    primary_trace_ms = rep_p["trace"][1] / rep_p["trace"][0]
+    prim_rays.destroy()
+    prim_hits.destroy()
 
     out = {
         "metric": "Mrays/s @1M-tri 1080p Whitted",
@@ -339,6 +366,8 @@ def main():
             "trace_ms_per_frame": round(trace_ms, 4),
             "shadow_ms_per_frame": round(kernels.get("trace_shadow", {}).get("ms_per_frame", 0.0), 4),
             "trace_mrays": round(rays_per_frame / trace_ms / 1e3, 2) if trace_ms else None,
+            "primary_trace_ms": round(primary_trace_ms, 4),
+            "primary_trace_mrays": round(sd.width * sd.height / primary_trace_ms / 1e3, 1),
         },
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
     }
@@ -368,6 +397,16 @@ def main():
         runs = [cpu_baseline(sd, cam) for _ in range(max(1, args.cpu_frames))]
         runs.sort(key=lambda r: r[1])
         mr, secs, rays, tb = runs[len(runs) // 2]
+        # fetch counts of the traversal (oracle counters), against the GPU's primary launch
+        nv, nt, ns = cpu_visit_counts(sd, cam)
+        P = sd.width * sd.height
+        eff = P * (nv * 32.0 + nt * 48.0)
+        out["traversal"]["oracle_fetches_per_primary_ray"] = {"node_visits": round(nv, 2),
+                                                              "triangle_tests": round(nt, 2), "sample_rays": ns}
+        out["traversal"]["effective_fetch_bytes_per_primary_launch"] = int(eff)
+        out["traversal"]["effective_fetch_GBs"] = round(eff / (out["traversal"]["primary_trace_ms"] / 1e3) / 1e9, 1)
+        out["traversal"]["effective_note"] = ("32 B per node visit and 48 B per triangle test (the oracle's counts), "
+                                              "over the primary launch's time: cache traffic, mostly L2 hits")
         out["cpu_baseline"] = {"value": round(mr, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
                                "sample": f"median of {len(runs)} whole C2 frames on one core (sequential oracle): "
                                          f"LBVH build ({tb:.2f}s) + render of {rays} rays, {secs:.1f}s per frame",
@@ -398,6 +437,9 @@ def build_stage_rooflines(kernels, T, nodes):
         "permutation": (("lbvh_rearrange",), 36.0 * T),
         "refit": (("lbvh_refit_level", "lbvh_refit_top", "lbvh_fat_leaves"), 28.0 * T + 32.0 * nl + 80.0 * ni),
         "sort_all_passes": (("lbvh_sort_pass",), 16.0 * T * 4),
+        # the same passes against 4 + 16 P_s B per element (one key read for
+        # the digit histograms, then key and value read and written per pass)
+        "sort_with_histogram_read": (("lbvh_sort_pass",), (4.0 + 16.0 * 4) * T),
     }
     out = {}
     for name, (keys, algo) in stages.items():
