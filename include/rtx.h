@@ -363,6 +363,10 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
  * 0.01, tested against every root; a spot light lights, untraced, a point
  * with dot(toward the light, spot dir) < angle, as that kernel does. */
 int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+/* Object::mat for every instance.  As Scene::create_bvhs (src/rt/scene.cpp:652-670)
+ * builds the material map, a scene's triangles take the material of the
+ * first instance that uses it (later instances of a scene share it). */
+int32_t rtx_instances_set_materials(rtx_instances* in, const rtx_material* per_instance);
 
 /* SecondaryRayGenerator::generate, scan variant (src/rt/secondary-ray-generator.cpp:57-145,
  * 248-340; src/kernel/secondary-ray-generator.cl:43-181): Whitted children in
@@ -409,6 +413,11 @@ int32_t rtx_render_frame_async(rtx_renderer* r, rtx_scene* s, const rtx_camera* 
                                void* stream);
 /* Ray counts of the last frame (synchronizes). */
 int32_t rtx_renderer_last_stats(rtx_renderer* r, rtx_frame_stats* stats);
+/* Renderer::render_to_framebuffer over a scene of objects (root_count() > 1):
+ * the reference's per-bounce sequence with the multi-root trace and shadow
+ * rays, secondary rays and shading through the instances' material map. */
+int32_t rtx_render_instances(rtx_renderer* r, rtx_instances* in, const rtx_camera* cam, rtx_frame_stats* stats,
+                             void* stream);
 /* Captures one rtx_render_frame_async into a hipGraph kept by the renderer
  * (no counterpart in the reference, which launches every stage from the
  * host).  Render one frame eagerly first so that every buffer exists.  The

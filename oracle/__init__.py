@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
         _lib.orc_raygen_secondary.restype = C.c_uint64
         _lib.orc_render.restype = C.c_uint64
         _lib.orc_render_tex.restype = C.c_uint64
+        _lib.orc_render_multi.restype = C.c_uint64
         _lib.orc_hit_t_f64.restype = C.c_double
     return _lib
 
@@ -380,6 +381,7 @@ class MultiScene:
         self.nrm4 = np.ascontiguousarray(np.concatenate(nrm), np.float32)
         self.uv = np.ascontiguousarray(np.concatenate(uv), np.float32)
         self.tri_base = tri_off
+        self.inst_mesh = [m for m, _, _ in instances]
         self.roots = np.zeros(len(instances), ROOT_DTYPE)
         for i, (m, tr, tri) in enumerate(instances):
             self.roots[i]["node"] = node_off[m]
@@ -395,6 +397,25 @@ def trace_multi(ms: MultiScene, ori4, dir4):
                           P(np.ascontiguousarray(dir4, np.float32)), C.c_uint64(n), P(h["flags"]), P(h["t"]),
                           P(h["id"]), P(h["bary"]), P(h["n4"]), P(h["p4"]), P(h["uv2"]))
     return h
+
+
+def render_multi(ms: MultiScene, inst_materials, lights, cam, w, h, max_bounces, quad=32, cap=0):
+    """A frame over the instances (orc_render_multi): inst_materials = one
+    32-B material per instance; each mesh takes its first instance's
+    (Scene::create_bvhs).  Returns (accum, stats, rays)."""
+    raw = np.asarray(inst_materials, np.uint8).reshape(-1, 32)
+    order = sorted(ms.tri_base, key=ms.tri_base.get)  # the meshes in order of first use
+    starts = [ms.tri_base[m] for m in order] + [ms.idx.size // 3]
+    mats = np.ascontiguousarray(np.concatenate([raw[ms.inst_mesh.index(m)] for m in order]))
+    matmap = np.ascontiguousarray(np.concatenate([np.full(starts[k + 1] - starts[k], k, np.int32)
+                                                  for k in range(len(order))]))
+    accum = np.zeros((w * h, 3), np.uint32)
+    st = np.zeros(1, STATS_DTYPE)
+    cap = cap or (w * h) << max_bounces
+    rays = lib().orc_render_multi(P(ms.pos4), P(ms.nrm4), P(ms.uv), P(ms.idx), P(ms.nodes), P(ms.roots),
+                                  C.c_int(len(ms.roots)), P(matmap), P(mats), P(lights), P(cam), C.c_int(w),
+                                  C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap), P(accum), P(st))
+    return accum, st[0], int(rays)
 
 
 def trace_shadow_multi(ms: MultiScene, lights: np.ndarray, flags, p4):

@@ -1438,6 +1438,58 @@ void orc_trace_shadow_multi(const float* pos, const uint32_t* idx, const orc_nod
     }
 }
 
+// Renderer::render_to_framebuffer over a scene of objects (root_count() > 1,
+// src/rt/renderer.cpp:113-214 with the multi-root kernels of
+// src/rt/tracer.cpp:97, 223, 301, 363): orc_render's sequence with
+// orc_trace_multi / orc_trace_shadow_multi; secondary rays and shading read
+// the shared material map (one material per mesh, Scene::create_bvhs).
+uint64_t orc_render_multi(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx,
+                          const orc_node* nodes, const orc_root* roots, int nroots, const int32_t* matmap,
+                          const orc_material* mats, const orc_lights* lights, const orc_camera* cam, int width,
+                          int height, int quad, int max_bounces, uint64_t cap, uint32_t* accum, orc_frame_stats* st) {
+    uint64_t P = (uint64_t)width * height;
+    if (cap < P) cap = P;
+    std::vector<float> o4a(4 * cap), d4a(4 * cap), o4b(4 * cap), d4b(4 * cap), ca(cap), cb(cap);
+    std::vector<int32_t> pa(cap), pb(cap), id(cap);
+    std::vector<uint32_t> fl(cap);
+    std::vector<float> t(cap), bary(2 * cap), n4(4 * cap), p4(4 * cap), uv2(2 * cap);
+    const float ps[3] = {0.f, 0.f, 1.f};
+    orc_raygen_primary(cam, width, height, quad, ps, 0, P, o4a.data(), d4a.data(), pa.data(), ca.data());
+    std::memset(accum, 0, sizeof(uint32_t) * 3 * P);
+    orc_frame_stats s;
+    std::memset(&s, 0, sizeof s);
+    s.primary = P;
+    uint64_t n = P;
+    float *o_in = o4a.data(), *d_in = d4a.data(), *c_in = ca.data();
+    int32_t* p_in = pa.data();
+    float *o_out = o4b.data(), *d_out = d4b.data(), *c_out = cb.data();
+    int32_t* p_out = pb.data();
+    for (int b = -1; b < max_bounces; ++b) {
+        if (b >= 0) {
+            uint64_t m = orc_raygen_secondary(mats, matmap, o_in, d_in, p_in, c_in, fl.data(), id.data(), n4.data(),
+                                              p4.data(), n, cap, o_out, d_out, p_out, c_out, nullptr);
+            std::swap(o_in, o_out);
+            std::swap(d_in, d_out);
+            std::swap(p_in, p_out);
+            std::swap(c_in, c_out);
+            n = m;
+            if (!n) break;
+            s.secondary += n;
+            if (b < 16) s.per_bounce[b] = n;
+            s.bounces_run++;
+        }
+        orc_trace_multi(pos, nrm, uvs, idx, nodes, roots, nroots, o_in, d_in, n, fl.data(), t.data(), id.data(),
+                        bary.data(), n4.data(), p4.data(), uv2.data());
+        orc_trace_shadow_multi(pos, idx, nodes, roots, nroots, lights, n, fl.data(), p4.data());
+        for (uint64_t k = 0; k < n; ++k)
+            if (fl[k] & ORC_HIT) s.shadow += (uint64_t)lights->count;
+        orc_shade_tex(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), uv2.data(),
+                      n, nullptr, accum);
+    }
+    if (st) *st = s;
+    return s.primary + s.secondary;
+}
+
 // 1: every trace of the oracle walks the tree without a stack (the no-stack
 // kernels); 0: the stack walk of trace-bvh.cl / shadow-trace-bvh.cl.
 void orc_set_traversal(int stackless) { g_stackless = stackless; }

@@ -256,6 +256,58 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
     return RTX_OK;
 }
 
+// Renderer::render_to_framebuffer over a scene of objects (root_count() > 1):
+// the reference's sequence with the multi-root trace and shadow kernels
+// (src/rt/tracer.cpp:97, 223, 301, 363) and the shared material map of
+// Scene::create_bvhs for secondary rays and shading.
+int32_t rtx_render_instances(rtx_renderer* r, rtx_instances* in, const rtx_camera* cam, rtx_frame_stats* stats,
+                             void* stream) {
+    RTX_REQUIRE(r && in && cam, RTX_EINVAL, "rtx_render_instances: null argument");
+    hipStream_t st = r->ctx->pick(stream);
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t P = (uint64_t)r->p.width * r->p.height;
+    unsigned long long* cnt = r->counts.as<unsigned long long>();
+    rtx_scene* sh = nullptr;
+    int32_t e;
+    if ((e = rtx_instances_shading_scene(in, &sh))) return e;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt, 8);
+    if ((e = rtx_fb_clear(r->fb, st))) return e;
+    rtx_frame_stats fs{};
+    fs.primary_rays = P;
+    if ((e = rtx_raygen_primary(r->rays[0], cam, r->p.width, r->p.height, r->p.quad_size, 1, nullptr, 0, P, st)))
+        return e;
+    if ((e = rtx_trace_instances(in, r->rays[0], r->hits[0], P, st))) return e;
+    if ((e = rtx_trace_shadow_instances_impl(in, r->rays[0], r->hits[0], P, (uint64_t*)(cnt + 1), st))) return e;
+    if ((e = rtx_shade(sh, r->rays[0], r->hits[0], r->fb, P, 1, st))) return e;
+    uint64_t n = P;
+    for (int b = 0; b < r->p.max_bounces; ++b) {
+        rtx_rays *rin = r->rays[b], *rout = r->rays[b + 1];
+        if ((e = rtx_raygen_secondary_impl(sh, rin, r->hits[b], rout, n, (uint64_t*)cnt, st))) return e;
+        uint32_t c32 = 0;
+        RTX_CHECK_HIP(hipMemcpyAsync(&c32, rout->count.p, 4, hipMemcpyDeviceToHost, st));
+        RTX_CHECK_HIP(hipStreamSynchronize(st));
+        const uint64_t m = c32;
+        if (!m) break;
+        fs.secondary_rays += m;
+        fs.bounces_run++;
+        rtx_hits* hout = r->hits[b + 1];
+        if ((e = rtx_trace_instances(in, rout, hout, m, st))) return e;
+        if ((e = rtx_trace_shadow_instances_impl(in, rout, hout, m, (uint64_t*)(cnt + 1), st))) return e;
+        if ((e = rtx_shade(sh, rout, hout, r->fb, m, 0, st))) return e;
+        n = m;
+    }
+    if ((e = rtx_fb_resolve(r->fb, st))) return e;
+    unsigned long long hc[2] = {0, 0};
+    RTX_CHECK_HIP(hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, st));
+    RTX_CHECK_HIP(hipStreamSynchronize(st));
+    fs.shadow_rays = hc[1];
+    fs.frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    r->last = fs;
+    r->last_async = false;
+    if (stats) *stats = fs;
+    return RTX_OK;
+}
+
 // Same frame with no host read-back: counts stay on the device.  Bounce b's
 // rays depend only on bounce b-1's hits, and its shadow rays and shading on
 // nothing later, so after bounce b is traced the main stream goes on to

@@ -1740,7 +1740,52 @@ int32_t rtx_instances_destroy(rtx_instances* in) {
     (void)hipDeviceSynchronize();
     in->roots.release();
     in->hit_root.release();
+    if (in->shading) rtx_scene_destroy(in->shading);
     delete in;
+    return RTX_OK;
+}
+
+static int32_t instance_bases(const rtx_instances* in, std::vector<uint32_t>& base);
+
+int32_t rtx_instances_set_materials(rtx_instances* in, const rtx_material* per_instance) {
+    RTX_REQUIRE(in && per_instance, RTX_EINVAL, "rtx_instances_set_materials: null argument");
+    in->inst_mats.assign(per_instance, per_instance + in->n);
+    in->shading_bases.clear();  // rebuilt by the next frame
+    return RTX_OK;
+}
+
+// The shading scene of frames over the instances (Scene::create_bvhs,
+// src/rt/scene.cpp:652-670): the distinct scenes in order of first use each
+// add the material of their first instance, and their triangles take its id
+// in the shared numbering (later instances of a scene share it).
+int32_t rtx_instances_shading_scene(rtx_instances* in, rtx_scene** out) {
+    RTX_REQUIRE(in && out, RTX_EINVAL, "rtx_instances_shading_scene: null argument");
+    RTX_REQUIRE((int)in->inst_mats.size() == in->n, RTX_ESTATE,
+                "rtx_render_instances: instances have no materials (rtx_instances_set_materials)");
+    std::vector<uint32_t> base;
+    int32_t e = instance_bases(in, base);
+    if (e) return e;
+    if (!in->shading && (e = rtx_scene_create(in->ctx, &in->shading))) return e;
+    rtx_scene* sh = in->shading;
+    if (base != in->shading_bases) {
+        std::vector<rtx_material> mats;
+        std::vector<int32_t> mm;
+        std::vector<const rtx_scene*> seen;
+        for (int i = 0; i < in->n; ++i) {
+            if (std::find(seen.begin(), seen.end(), in->scene[i]) != seen.end()) continue;
+            seen.push_back(in->scene[i]);
+            mats.push_back(in->inst_mats[i]);
+            mm.insert(mm.end(), in->scene[i]->T, (int32_t)mats.size() - 1);
+        }
+        RTX_CHECK_HIP(hipDeviceSynchronize());
+        if ((e = rtx_scene_set_materials(sh, mats.data(), (uint32_t)mats.size()))) return e;
+        if ((e = sh->matmap.reserve(in->ctx, mm.size() * 4))) return e;
+        RTX_CHECK_HIP(hipMemcpy(sh->matmap.p, mm.data(), mm.size() * 4, hipMemcpyHostToDevice));
+        sh->T = (uint32_t)mm.size();
+        in->shading_bases = base;
+    }
+    if ((e = rtx_scene_set_lights(sh, &in->lights))) return e;
+    *out = sh;
     return RTX_OK;
 }
 
@@ -1872,10 +1917,15 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
 }
 
 int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
+    RTX_REQUIRE(in, RTX_EINVAL, "rtx_trace_shadow_instances: null argument");
+    return rtx_trace_shadow_instances_impl(in, rays, hits, count, nullptr, in->ctx->pick(stream));
+}
+
+int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count,
+                                        uint64_t* dev_shadow_count, hipStream_t st) {
     RTX_REQUIRE(in && rays && hits, RTX_EINVAL, "rtx_trace_shadow_instances: null argument");
     RTX_REQUIRE(count == RTX_DEVICE_COUNT || count <= hits->cap, RTX_EINVAL,
                 "rtx_trace_shadow_instances: count too large");
-    hipStream_t st = in->ctx->pick(stream);
     if (count == 0 || in->lights.count <= 0) return RTX_OK;
     int var = 1;
     int32_t e = refresh_roots(in, st, &var);
@@ -1891,7 +1941,7 @@ int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* 
     hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,          \
                        in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(),                     \
                        hits->list_count.as<uint32_t>(), hits->flags.as<uint32_t>(), hits->p4.as<float4>(),         \
-                       order, (unsigned long long*)nullptr, coop_threshold(in->ctx))
+                       order, (unsigned long long*)dev_shadow_count, coop_threshold(in->ctx))
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
 #undef RTX_SHADOW_INST
     RTX_CHECK_HIP(hipGetLastError());

@@ -132,7 +132,8 @@ SYMBOLS = [
     "rtx_pipeline_wait", "rtx_pipeline_fb", "rtx_pipeline_stats",
     "rtx_obj_load", "rtx_obj_destroy", "rtx_obj_info", "rtx_obj_read", "rtx_obj_group_info", "rtx_obj_read_group",
     "rtx_instances_create", "rtx_instances_destroy", "rtx_instances_set_transforms", "rtx_instances_set_lights",
-    "rtx_instances_tri_base", "rtx_trace_instances", "rtx_trace_shadow_instances",
+    "rtx_instances_tri_base", "rtx_trace_instances", "rtx_trace_shadow_instances", "rtx_instances_set_materials",
+    "rtx_render_instances",
     "rtx_camera_set",
 ]
 
@@ -257,6 +258,8 @@ def lib() -> C.CDLL:
         "rtx_instances_tri_base": (I32, [P, P]),
         "rtx_trace_instances": (I32, [P, P, P, U64, P]),
         "rtx_trace_shadow_instances": (I32, [P, P, P, U64, P]),
+        "rtx_instances_set_materials": (I32, [P, P]),
+        "rtx_render_instances": (I32, [P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -746,6 +749,12 @@ class Instances:
     def set_lights(self, lights: Lights):
         _check(lib().rtx_instances_set_lights(self.h, C.byref(lights)), "rtx_instances_set_lights")
 
+    def set_materials(self, mats: np.ndarray):
+        """One 32-B material per instance (make_materials rows)."""
+        m = np.ascontiguousarray(mats, np.uint8)
+        assert m.size == 32 * self.n
+        _check(lib().rtx_instances_set_materials(self.h, _ptr(m)), "rtx_instances_set_materials")
+
     def tri_base(self) -> np.ndarray:
         out = np.zeros(self.n, np.uint32)
         _check(lib().rtx_instances_tri_base(self.h, _ptr(out)), "rtx_instances_tri_base")
@@ -802,6 +811,12 @@ class Renderer:
 
     def set_bounces(self, bounces: int):
         _check(lib().rtx_renderer_set_bounces(self.h, int(bounces)), "rtx_renderer_set_bounces")
+
+    def frame_instances(self, inst: "Instances", cam: Camera, stream=None) -> FrameStats:
+        """A frame over a scene of objects (multi-root trace and shadow rays)."""
+        st = FrameStats()
+        _check(lib().rtx_render_instances(self.h, inst.h, C.byref(cam), C.byref(st), stream), "rtx_render_instances")
+        return st
 
     def set_pipelined(self, enable: bool):
         _check(lib().rtx_renderer_set_pipelined(self.h, int(bool(enable))), "rtx_renderer_set_pipelined")

@@ -201,3 +201,45 @@ def test_instance_errors(ctx):
     r.destroy()
     hb.destroy()
     s.destroy()
+
+
+@pytest.mark.parametrize("case", ["rigid", "overlap"])
+def test_frame_over_instances(ctx, case):
+    """Renderer::render_to_framebuffer over a scene of objects: primary rays,
+    the multi-root trace and shadow rays, secondary rays and shading through
+    the shared material map (a mesh takes its first instance's material:
+    reflective and refractive ones make bounces), bit for bit."""
+    import rtx
+    from rtx import make_materials
+    make, insts = CASES[case]
+    meshes = make()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    rows = [(0.9, 0.2, 0.2, 0.5, 0.6, 1.5), (0.2, 0.8, 0.3, 1.0, 0.7, 0.0), (0.3, 0.3, 0.9, 0.2, 0.0, 0.0),
+            (0.8, 0.8, 0.8, 1.0, 0.9, 1.33)]
+    mats = make_materials(rows[: len(insts)])
+    inst.set_materials(mats)
+    w, h = 160, 120
+    cam = rtx.camera_set((0.0, 0.8, -5.0), (0.0, 0.3, 0.0), (0.0, 1.0, 0.0), np.pi / 3, w / h)
+    R = rtx.Renderer(ctx, w, h, 3)
+    st = R.frame_instances(inst, cam)
+    ms = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], insts)
+    oacc, ost, orays = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
+    assert (oacc.sum(1) > 0).mean() > 0.15 and st.secondary_rays > 0
+    assert st.primary_rays + st.secondary_rays == orays
+    assert np.array_equal(R.fb.accum(), oacc)
+    # new transforms: the next frame follows them, no rebuild
+    moved = [(m, *xform(rot((0, 1, 0), 0.4 * (i + 1)), 1.0, (1.2 * i - 1.2, 0.1 * i, 0.0))) for i, (m, _, _) in
+             enumerate(insts)]
+    inst.set_transforms([t for _, t, _ in moved], [ti for _, _, ti in moved])
+    R.frame_instances(inst, cam)
+    ms2 = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], moved)
+    oacc2, _, _ = O.render_multi(ms2, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
+    assert np.array_equal(R.fb.accum(), oacc2)
+    R.destroy()
+    inst.destroy()
+    for s in scs:
+        s.destroy()
