@@ -219,6 +219,11 @@ public:
     rtx_scene* mesh_handle(int32_t mesh) const { return m_meshes[(size_t)mesh]; }
     int32_t object_mesh(int32_t object) const { return m_obj_mesh[(size_t)object]; }
 
+    // The scene whose materials shade this scene's hits and spawn its
+    // secondary rays: the aggregate mesh's own, or, for objects over per-mesh
+    // BVHs, one material per mesh, its first object's (Scene::create_bvhs,
+    // src/rt/scene.cpp:652-670).  nullptr on error.
+    rtx_scene* shading_handle();
     // The atlas's images and the cubemap `cm` (enabled or not) on the device
     // for the shading stage; uploads only what changed.
     int32_t sync_images(const Cubemap& cm);
@@ -232,6 +237,7 @@ public:
 private:
     friend class BVHBuilder;
     int32_t push_lights();
+    rtx_scene* m_img_target = nullptr;   // the scene that holds the uploaded images
     const Cubemap* m_cube_on = nullptr;  // the cubemap on the device, and its version
     uint64_t m_cube_version = 0;
     rtx_scene* m_h = nullptr;

@@ -367,6 +367,14 @@ int32_t rtx_trace_shadow_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* 
  * builds the material map, a scene's triangles take the material of the
  * first instance that uses it (later instances of a scene share it). */
 int32_t rtx_instances_set_materials(rtx_instances* in, const rtx_material* per_instance);
+/* The scene that shades the instances' hits and spawns their secondary rays
+ * (the shared material map and materials of Scene::create_bvhs,
+ * src/rt/scene.cpp:652-670, with the set's lights).  It holds no geometry;
+ * textures and the cubemap are set on it with rtx_scene_set_textures /
+ * rtx_scene_set_cubemap and stay across calls.  Owned by the instance set
+ * (destroyed with it); rtx_instances_set_materials must have been called.
+ * Refreshed by each call when the scenes' triangle counts changed. */
+int32_t rtx_instances_shading_scene(rtx_instances* in, rtx_scene** out);
 
 /* SecondaryRayGenerator::generate, scan variant (src/rt/secondary-ray-generator.cpp:57-145,
  * 248-340; src/kernel/secondary-ray-generator.cl:43-181): Whitted children in

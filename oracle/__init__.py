@@ -399,7 +399,8 @@ def trace_multi(ms: MultiScene, ori4, dir4):
     return h
 
 
-def render_multi(ms: MultiScene, inst_materials, lights, cam, w, h, max_bounces, quad=32, cap=0):
+def render_multi(ms: MultiScene, inst_materials, lights, cam, w, h, max_bounces, quad=32, cap=0,
+                 tex: TexSet | None = None):
     """A frame over the instances (orc_render_multi): inst_materials = one
     32-B material per instance; each mesh takes its first instance's
     (Scene::create_bvhs).  Returns (accum, stats, rays)."""
@@ -414,7 +415,8 @@ def render_multi(ms: MultiScene, inst_materials, lights, cam, w, h, max_bounces,
     cap = cap or (w * h) << max_bounces
     rays = lib().orc_render_multi(P(ms.pos4), P(ms.nrm4), P(ms.uv), P(ms.idx), P(ms.nodes), P(ms.roots),
                                   C.c_int(len(ms.roots)), P(matmap), P(mats), P(lights), P(cam), C.c_int(w),
-                                  C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap), P(accum), P(st))
+                                  C.c_int(h), C.c_int(quad), C.c_int(max_bounces), C.c_uint64(cap),
+                                  C.byref(tex.c) if tex is not None else None, P(accum), P(st))
     return accum, st[0], int(rays)
 
 

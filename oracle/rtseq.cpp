@@ -1446,7 +1446,8 @@ void orc_trace_shadow_multi(const float* pos, const uint32_t* idx, const orc_nod
 uint64_t orc_render_multi(const float* pos, const float* nrm, const float* uvs, const uint32_t* idx,
                           const orc_node* nodes, const orc_root* roots, int nroots, const int32_t* matmap,
                           const orc_material* mats, const orc_lights* lights, const orc_camera* cam, int width,
-                          int height, int quad, int max_bounces, uint64_t cap, uint32_t* accum, orc_frame_stats* st) {
+                          int height, int quad, int max_bounces, uint64_t cap, const orc_texset* tx,
+                          uint32_t* accum, orc_frame_stats* st) {
     uint64_t P = (uint64_t)width * height;
     if (cap < P) cap = P;
     std::vector<float> o4a(4 * cap), d4a(4 * cap), o4b(4 * cap), d4b(4 * cap), ca(cap), cb(cap);
@@ -1484,7 +1485,7 @@ uint64_t orc_render_multi(const float* pos, const float* nrm, const float* uvs, 
         for (uint64_t k = 0; k < n; ++k)
             if (fl[k] & ORC_HIT) s.shadow += (uint64_t)lights->count;
         orc_shade_tex(mats, matmap, lights, d_in, p_in, c_in, fl.data(), id.data(), n4.data(), p4.data(), uv2.data(),
-                      n, nullptr, accum);
+                      n, tx, accum);
     }
     if (st) *st = s;
     return s.primary + s.secondary;

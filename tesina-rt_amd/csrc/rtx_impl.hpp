@@ -330,7 +330,6 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
 int32_t rtx_lbvh_reserve(rtx_scene* s, int depth);
 int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count,
                                         uint64_t* dev_shadow_count, hipStream_t st);
-int32_t rtx_instances_shading_scene(rtx_instances* in, rtx_scene** out);
 // The CPU SAH builder (tesina-rt_amd/rt/sah.cpp).
 int32_t rtx_sah_construct(const float* pos3, uint32_t V, const uint32_t* idx3, uint32_t T,
                           std::vector<rtx_node_ref>& out, std::vector<uint32_t>& order, uint32_t* max_depth);

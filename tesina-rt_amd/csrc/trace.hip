@@ -1761,7 +1761,7 @@ int32_t rtx_instances_set_materials(rtx_instances* in, const rtx_material* per_i
 int32_t rtx_instances_shading_scene(rtx_instances* in, rtx_scene** out) {
     RTX_REQUIRE(in && out, RTX_EINVAL, "rtx_instances_shading_scene: null argument");
     RTX_REQUIRE((int)in->inst_mats.size() == in->n, RTX_ESTATE,
-                "rtx_render_instances: instances have no materials (rtx_instances_set_materials)");
+                "rtx_instances_shading_scene: instances have no materials (rtx_instances_set_materials)");
     std::vector<uint32_t> base;
     int32_t e = instance_bases(in, base);
     if (e) return e;

@@ -105,6 +105,8 @@ int32_t Scene::destroy() {
     m_h = nullptr;
     if (m_inst) rtx_instances_destroy(m_inst);
     m_inst = nullptr;
+    m_img_target = nullptr;
+    m_cube_on = nullptr;
     for (rtx_scene* m : m_meshes) rtx_scene_destroy(m);
     m_meshes.clear();
     m_host.clear();
@@ -154,11 +156,24 @@ int32_t Cubemap::destroy() {
     return 0;
 }
 
+rtx_scene* Scene::shading_handle() {
+    if (m_built || !m_bvhs_built || !m_inst) return m_h;
+    rtx_scene* sh = nullptr;
+    if (rtx_instances_shading_scene(m_inst, &sh) != RTX_OK) {
+        report("Scene::shading_handle");
+        return nullptr;
+    }
+    return sh;
+}
+
 int32_t Scene::sync_images(const Cubemap& cm) {
-    if (!m_h) return -1;
-    if (texture_atlas.m_dirty) {
+    rtx_scene* target = shading_handle();
+    if (!target) return -1;
+    const bool moved = target != m_img_target;  // a new target starts without images
+    m_img_target = nullptr;
+    if (texture_atlas.m_dirty || moved) {
         const int32_t n = (int32_t)texture_atlas.texture_count();
-        if (rtx_scene_set_textures(m_h, n ? texture_atlas.m_rgba.data() : nullptr,
+        if (rtx_scene_set_textures(target, n ? texture_atlas.m_rgba.data() : nullptr,
                                    n ? texture_atlas.m_dims.data() : nullptr, n) != RTX_OK) {
             report("Scene::sync_images");
             return -1;
@@ -166,15 +181,16 @@ int32_t Scene::sync_images(const Cubemap& cm) {
         texture_atlas.m_dirty = false;
     }
     const Cubemap* want = (cm.enabled && cm.m_dims.size() == 12) ? &cm : nullptr;
-    if (want != m_cube_on || (want && want->m_version != m_cube_version)) {
-        if (rtx_scene_set_cubemap(m_h, want ? want->m_rgba.data() : nullptr, want ? want->m_dims.data() : nullptr) !=
-            RTX_OK) {
+    if (moved || want != m_cube_on || (want && want->m_version != m_cube_version)) {
+        if (rtx_scene_set_cubemap(target, want ? want->m_rgba.data() : nullptr,
+                                  want ? want->m_dims.data() : nullptr) != RTX_OK) {
             report("Scene::sync_images");
             return -1;
         }
         m_cube_on = want;
         m_cube_version = want ? want->m_version : 0;
     }
+    m_img_target = target;
     return 0;
 }
 
@@ -334,11 +350,13 @@ int32_t Scene::create_bvhs(int depth, int min_leaf) {
     }
     if (m_inst) rtx_instances_destroy(m_inst);
     m_inst = nullptr;
+    m_img_target = nullptr;
     std::vector<rtx_scene*> sc;
     for (int32_t m : m_obj_mesh) sc.push_back(m_meshes[(size_t)m]);
     if (rtx_instances_create(dev(), sc.data(), m_tr.data(), m_tr_inv.data(), (int32_t)sc.size(), &m_inst) !=
             RTX_OK ||
-        rtx_instances_set_lights(m_inst, &m_lights) != RTX_OK) {
+        rtx_instances_set_lights(m_inst, &m_lights) != RTX_OK ||
+        rtx_instances_set_materials(m_inst, m_obj_mat.data()) != RTX_OK) {
         report("Scene::create_bvhs");
         return -1;
     }
@@ -705,8 +723,9 @@ int32_t SecondaryRayGenerator::generate(Scene& scene, RayBundle& ray_in, size_t 
     if (!m_initialized || !rays_out) return -1;
     auto t = clk::now();
     uint64_t n = 0;
-    if (rtx_raygen_secondary(scene.handle(), ray_in.handle(), hits.handle(), ray_out.handle(), rays_in, &n,
-                             nullptr) != RTX_OK) {
+    rtx_scene* sh = scene.shading_handle();
+    if (!sh || rtx_raygen_secondary(sh, ray_in.handle(), hits.handle(), ray_out.handle(), rays_in, &n, nullptr) !=
+                   RTX_OK) {
         report("SecondaryRayGenerator::generate");
         return -1;
     }
@@ -731,7 +750,7 @@ int32_t RayShader::shade(RayBundle& rays, HitBundle& hb, Scene& scene, Cubemap& 
                          bool primary) {
     if (!m_initialized || scene.sync_images(cm)) return -1;
     auto t = clk::now();
-    if (rtx_shade(scene.handle(), rays.handle(), hb.handle(), fb.handle(), size, primary ? 1 : 0, nullptr) !=
+    if (rtx_shade(scene.shading_handle(), rays.handle(), hb.handle(), fb.handle(), size, primary ? 1 : 0, nullptr) !=
         RTX_OK) {
         report("RayShader::shade");
         return -1;

@@ -133,7 +133,7 @@ SYMBOLS = [
     "rtx_obj_load", "rtx_obj_destroy", "rtx_obj_info", "rtx_obj_read", "rtx_obj_group_info", "rtx_obj_read_group",
     "rtx_instances_create", "rtx_instances_destroy", "rtx_instances_set_transforms", "rtx_instances_set_lights",
     "rtx_instances_tri_base", "rtx_trace_instances", "rtx_trace_shadow_instances", "rtx_instances_set_materials",
-    "rtx_render_instances",
+    "rtx_instances_shading_scene", "rtx_render_instances",
     "rtx_camera_set",
 ]
 
@@ -259,6 +259,7 @@ def lib() -> C.CDLL:
         "rtx_trace_instances": (I32, [P, P, P, U64, P]),
         "rtx_trace_shadow_instances": (I32, [P, P, P, U64, P]),
         "rtx_instances_set_materials": (I32, [P, P]),
+        "rtx_instances_shading_scene": (I32, [P, P]),
         "rtx_render_instances": (I32, [P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -320,6 +321,24 @@ def make_materials(rows) -> np.ndarray:
         arr[i].shininess, arr[i].reflectiveness, arr[i].refractive_index = sh, rf, ior
         arr[i].texture = -1
     return np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+
+
+def _set_textures(h, images):
+    imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+    dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])] or [0], np.int32)
+    data = np.concatenate([im.reshape(-1) for im in imgs]) if imgs else np.zeros(4, np.uint8)
+    _check(lib().rtx_scene_set_textures(h, _ptr(data), _ptr(dims), len(imgs)), "rtx_scene_set_textures")
+
+
+def _set_cubemap(h, faces):
+    if faces is None:
+        _check(lib().rtx_scene_set_cubemap(h, None, None), "rtx_scene_set_cubemap")
+        return
+    imgs = [np.ascontiguousarray(f, np.uint8) for f in faces]
+    assert len(imgs) == 6
+    dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])], np.int32)
+    data = np.concatenate([im.reshape(-1) for im in imgs])
+    _check(lib().rtx_scene_set_cubemap(h, _ptr(data), _ptr(dims)), "rtx_scene_set_cubemap")
 
 
 class Context:
@@ -492,21 +511,11 @@ class Scene:
 
     def set_textures(self, images):
         """Material textures: h x w x 4 uint8 arrays, texel (x, y) at [y, x]."""
-        imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
-        dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])] or [0], np.int32)
-        data = np.concatenate([im.reshape(-1) for im in imgs]) if imgs else np.zeros(4, np.uint8)
-        _check(lib().rtx_scene_set_textures(self.h, _ptr(data), _ptr(dims), len(imgs)), "rtx_scene_set_textures")
+        _set_textures(self.h, images)
 
     def set_cubemap(self, faces):
         """Six h x w x 4 uint8 faces (+x, -x, +y, -y, +z, -z), or None to disable."""
-        if faces is None:
-            _check(lib().rtx_scene_set_cubemap(self.h, None, None), "rtx_scene_set_cubemap")
-            return
-        imgs = [np.ascontiguousarray(f, np.uint8) for f in faces]
-        assert len(imgs) == 6
-        dims = np.asarray([d for im in imgs for d in (im.shape[1], im.shape[0])], np.int32)
-        data = np.concatenate([im.reshape(-1) for im in imgs])
-        _check(lib().rtx_scene_set_cubemap(self.h, _ptr(data), _ptr(dims)), "rtx_scene_set_cubemap")
+        _set_cubemap(self.h, faces)
 
     def build(self, depth: int = 30, min_leaf: int = 1, stream=None):
         p = LbvhParams(depth, min_leaf)
@@ -754,6 +763,19 @@ class Instances:
         m = np.ascontiguousarray(mats, np.uint8)
         assert m.size == 32 * self.n
         _check(lib().rtx_instances_set_materials(self.h, _ptr(m)), "rtx_instances_set_materials")
+
+    def _shading(self):
+        sh = C.c_void_p()
+        _check(lib().rtx_instances_shading_scene(self.h, C.byref(sh)), "rtx_instances_shading_scene")
+        return sh
+
+    def set_textures(self, images):
+        """Material textures of frames over the instances (after set_materials)."""
+        _set_textures(self._shading(), images)
+
+    def set_cubemap(self, faces):
+        """The background of frames over the instances, or None (after set_materials)."""
+        _set_cubemap(self._shading(), faces)
 
     def tri_base(self) -> np.ndarray:
         out = np.zeros(self.n, np.uint32)

@@ -10,31 +10,22 @@
 //            moved.txt   (optional) N lines "tr[16] tr_inv[16]": second pass
 //                        after Object moves + update_bvh_roots
 //            rays.f32    ori4 | dir4 interleaved per ray (8 floats), lights.bin
-// out_dir:   pass<p>.flags.u32 .t.f32 .id.i32 .bary.f32 .n4.f32 .p4.f32 .uv2.f32
+//            objmats.bin (optional) one rtx_material per object (Object::mat)
+//            render.txt  (optional) "width height max_bounces" and the view
+//                        "px py pz lx ly lz ux uy uz fov": after each pass, a
+//                        frame through Renderer with use_lbvh = 0, the way the
+//                        reference renders a scene of objects; images.txt
+//                        (stage_io.hpp) gives it textures and a cubemap
+// out_dir:   pass<p>.flags.u32 .t.f32 .id.i32 .bary.f32 .n4.f32 .p4.f32 .uv2.f32,
+//            frame<p>.accum.u32 frame<p>.stats.txt "rays secondary_rays"
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
-#include "rt/stages.hpp"
-
-template <class T>
-static std::vector<T> load(const std::string& path) {
-    std::ifstream f(path, std::ios::binary | std::ios::ate);
-    if (!f) return {};
-    size_t n = (size_t)f.tellg();
-    std::vector<T> v(n / sizeof(T));
-    f.seekg(0);
-    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
-    return v;
-}
-
-template <class T>
-static void save(const std::string& path, const std::vector<T>& v) {
-    std::ofstream f(path, std::ios::binary);
-    f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
-}
+#include "stage_io.hpp"
 
 static bool read_mats(std::istream& in, float* tr, float* tri) {
     for (int k = 0; k < 16; ++k) in >> tr[k];
@@ -58,6 +49,40 @@ static int dump(HitBundle& hits, size_t n, const std::string& prefix) {
     save(prefix + ".uv2.f32", uv2);
     return 0;
 }
+
+struct FrameDriver {
+    int w = 0, h = 0, bounces = 0;
+    float view[10] = {};
+    Renderer r;
+
+    int open(const std::string& in, Scene& scene) {
+        std::ifstream f(in + "/render.txt");
+        f >> w >> h >> bounces;
+        for (float& x : view) f >> x;
+        if (!f || w <= 0 || h <= 0) return 2;
+        if (const int e = load_images(in, scene)) return e;
+        vec3 p{view[0], view[1], view[2]}, look{view[3], view[4], view[5]}, up{view[6], view[7], view[8]};
+        scene.camera.set(p, vec3{look.x - p.x, look.y - p.y, look.z - p.z}, up, view[9], (float)w / (float)h);
+        r.configure_from_defaults();
+        r.config.use_lbvh = 0;  // the objects' own trees
+        r.resize_output((size_t)w, (size_t)h);
+        r.set_max_bounces((uint32_t)bounces);
+        return r.initialize() ? 1 : 0;
+    }
+
+    int frame(Scene& scene, const std::string& prefix) {
+        if (r.set_up_frame(scene) || r.render_to_framebuffer(scene) || r.copy_framebuffer() ||
+            r.conclude_frame(scene))
+            return 1;
+        std::vector<uint32_t> acc((size_t)w * h * 3);
+        if (r.framebuffer().read_accum(acc.data())) return 1;
+        save(prefix + ".accum.u32", acc);
+        const FrameStats& st = r.get_frame_stats();
+        std::ofstream s(prefix + ".stats.txt");
+        s << st.get_ray_count() << " " << st.get_secondary_ray_count() << "\n";
+        return 0;
+    }
+};
 
 int main(int argc, char** argv) {
     if (argc < 3) {
@@ -107,7 +132,18 @@ int main(int argc, char** argv) {
             const int32_t o = scene.add_object(mesh);
             if (o != i || scene.set_object_transform(o, tr, tri)) return 1;
         }
+        auto omats = load<rtx_material>(in + "/objmats.bin");
+        if (!omats.empty()) {
+            if (omats.size() != (size_t)N) return 2;
+            for (int i = 0; i < N; ++i)
+                if (scene.set_object_material(i, omats[(size_t)i])) return 1;
+        }
         if (scene.set_lights(lights[0]) || scene.create_bvhs(30, 1)) return 1;
+        std::unique_ptr<FrameDriver> fd;
+        if (std::ifstream(in + "/render.txt")) {
+            fd.reset(new FrameDriver);
+            if (const int e = fd->open(in, scene)) return e;
+        }
         RayBundle rb;
         HitBundle hb;
         Tracer tracer;
@@ -115,6 +151,7 @@ int main(int argc, char** argv) {
         if (rtx_rays_write(rb.handle(), ori4.data(), dir4.data(), nullptr, nullptr, n) != RTX_OK) return 1;
         if (tracer.trace(scene, (int32_t)n, rb, hb) || tracer.shadow_trace(scene, (int32_t)n, rb, hb)) return 1;
         if (dump(hb, n, out + "/pass0")) return 1;
+        if (fd && fd->frame(scene, out + "/frame0")) return 1;
         std::ifstream mv(in + "/moved.txt");
         int passes = 1;
         if (mv) {
@@ -123,7 +160,8 @@ int main(int argc, char** argv) {
                 if (!read_mats(mv, tr, tri) || scene.set_object_transform(i, tr, tri)) return 2;
             }
             if (scene.update_bvh_roots() || tracer.trace(scene, (int32_t)n, rb, hb) ||
-                tracer.shadow_trace(scene, (int32_t)n, rb, hb) || dump(hb, n, out + "/pass1"))
+                tracer.shadow_trace(scene, (int32_t)n, rb, hb) || dump(hb, n, out + "/pass1") ||
+                (fd && fd->frame(scene, out + "/frame1")))
                 return 1;
             passes = 2;
         }

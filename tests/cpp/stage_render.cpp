@@ -6,8 +6,7 @@
 // scene_dir: pos.f32 nrm.f32 uv.f32 idx.u32 matmap.i32 mats.bin lights.bin,
 //            view.txt "px py pz lx ly lz ux uy uz fov",
 //            params.txt "width height max_bounces bvh_depth min_leaf frames"
-//            optional images.txt: "N", then N lines "w h" (textures tex<k>.rgba),
-//            then "C" (1: cubemap faces face0..face5.rgba follow as 6 lines "w h")
+//            optional images.txt (stage_io.hpp)
 // out_dir:   accum.u32 rgba.u8 stats.txt
 #include <cstdio>
 #include <fstream>
@@ -15,24 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "rt/stages.hpp"
-
-template <class T>
-static std::vector<T> load(const std::string& path) {
-    std::ifstream f(path, std::ios::binary | std::ios::ate);
-    if (!f) return {};
-    size_t n = (size_t)f.tellg();
-    std::vector<T> v(n / sizeof(T));
-    f.seekg(0);
-    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
-    return v;
-}
-
-template <class T>
-static void save(const std::string& path, const std::vector<T>& v) {
-    std::ofstream f(path, std::ios::binary);
-    f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
-}
+#include "stage_io.hpp"
 
 int main(int argc, char** argv) {
     if (argc < 3) {
@@ -67,31 +49,7 @@ int main(int argc, char** argv) {
                                (uint32_t)(idx.size() / 3)) ||
             scene.set_materials(mats) || scene.set_lights(lights[0]))
             return 1;
-        std::ifstream imf(in + "/images.txt");
-        if (imf) {
-            int n = 0, cube = 0;
-            imf >> n;
-            for (int k = 0; k < n; ++k) {
-                uint32_t tw = 0, th = 0;
-                imf >> tw >> th;
-                auto px = load<uint8_t>(in + "/tex" + std::to_string(k) + ".rgba");
-                if (px.size() != 4ull * tw * th || scene.texture_atlas.load_texture_rgba(px.data(), tw, th) != k)
-                    return 2;
-            }
-            imf >> cube;
-            if (cube) {
-                std::vector<uint8_t> faces[6];
-                const uint8_t* fp[6];
-                uint32_t dims[12];
-                for (int f = 0; f < 6; ++f) {
-                    imf >> dims[2 * f] >> dims[2 * f + 1];
-                    faces[f] = load<uint8_t>(in + "/face" + std::to_string(f) + ".rgba");
-                    if (faces[f].size() != 4ull * dims[2 * f] * dims[2 * f + 1]) return 2;
-                    fp[f] = faces[f].data();
-                }
-                if (scene.cubemap.initialize_rgba(fp, dims)) return 1;
-            }
-        }
+        if (const int e = load_images(in, scene)) return e;
         vec3 p{view[0], view[1], view[2]}, look{view[3], view[4], view[5]}, up{view[6], view[7], view[8]};
         scene.camera.set(p, vec3{look.x - p.x, look.y - p.y, look.z - p.z}, up, view[9], (float)w / (float)h);
         Renderer r;

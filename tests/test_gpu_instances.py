@@ -243,3 +243,50 @@ def test_frame_over_instances(ctx, case):
     inst.destroy()
     for s in scs:
         s.destroy()
+
+
+def test_textured_frame_over_instances(ctx):
+    """Textures and the cubemap on the instances' shading scene: a mesh's
+    texture index comes from its first instance's material, misses take the
+    sky, bit for bit; with the cubemap off again misses add nothing.  Images
+    before materials is an error."""
+    import rtx
+    from rtx import make_materials
+    from test_gpu_textures import _images
+    make, insts = CASES["rigid"]
+    meshes = make()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    textures, faces = _images(4)
+    with pytest.raises(rtx.RtxError):
+        inst.set_textures(textures)
+    rows = [(0.9, 0.2, 0.2, 0.5, 0.6, 0.0), (0.2, 0.8, 0.3, 1.0, 0.0, 0.0), (0.3, 0.3, 0.9, 0.2, 0.0, 0.0),
+            (0.8, 0.8, 0.8, 1.0, 0.9, 1.33)]
+    mats = make_materials(rows[: len(insts)])
+    mi = mats.view(np.int32).reshape(-1, 8)
+    mi[:, 7] = [k % 3 - 1 for k in range(1, len(insts) + 1)]  # 0, 1, -1, 0, ...: image, image, none
+    inst.set_materials(mats)
+    inst.set_textures(textures)
+    inst.set_cubemap(faces)
+    w, h = 160, 120
+    cam = rtx.camera_set((0.0, 0.8, -5.0), (0.0, 0.3, 0.0), (0.0, 1.0, 0.0), np.pi / 3, w / h)
+    R = rtx.Renderer(ctx, w, h, 3)
+    R.frame_instances(inst, cam)
+    ms = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], insts)
+    oacc, _, _ = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3,
+                                tex=O.TexSet(textures, faces))
+    plain, _, _ = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
+    assert (oacc != plain).any(1).mean() > 0.5, "the sky and the textures should change much of the frame"
+    assert np.array_equal(R.fb.accum(), oacc)
+    inst.set_cubemap(None)
+    R.frame_instances(inst, cam)
+    oacc2, _, _ = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3,
+                                 tex=O.TexSet(textures))
+    assert np.array_equal(R.fb.accum(), oacc2)
+    R.destroy()
+    inst.destroy()
+    for s in scs:
+        s.destroy()

@@ -174,3 +174,68 @@ def test_cpp_single_object_is_traced_untransformed(tmp_path):
     g = _read_pass(out, 0, n)
     for k in ("flags", "t", "id", "bary", "n4", "p4", "uv2"):
         assert np.array_equal(g[k], o[k]), f"{k} differs"
+
+
+@pytest.mark.parametrize("count", [3, 1])
+def test_cpp_renderer_over_objects(tmp_path, count):
+    """Renderer over a scene of objects with use_lbvh = 0 (the reference's
+    Renderer with Scene::create_bvhs): the multi-root trace and shadow rays,
+    secondary rays and shading through the objects' material map (a mesh takes
+    its first object's material), textures and the cubemap; then the objects
+    move (update_bvh_roots) and the next frame follows.  With one object the
+    single-root kernels trace its mesh as stored, shaded with its material."""
+    import rtx
+    from rtx import make_materials
+    from inst_scene import rot, xform, two_meshes, rays_at
+    from test_gpu_instances import _lights
+    from test_gpu_textures import _images
+    assert OBJ_BIN.exists(), "stage_objects not built (run __graft_entry__.build())"
+    meshes = two_meshes()
+    objects = [(0, *xform(rot((0, 1, 0), 0.7), 1.0, (-1.6, 0.3, 0.0))),
+               (1, *xform(rot((1, 0, 1), 1.9), 1.5, (1.6, -0.5, 1.0))),
+               (0, *xform(rot((0, 0, 1), -0.4), 0.5, (0.0, 1.6, -1.0)))][:count]
+    moved = [(m, *xform(rot((0, 1, 0), 0.3 * i + 0.2), 1.1, (1.5 * i - 1.5, 0.2, 0.0))) for i, (m, _, _) in
+             enumerate(objects)]
+    n = 500
+    ori4, dir4 = rays_at(23, n)
+    lights = _lights()
+    _write_objects(tmp_path, meshes, objects, moved, ori4, dir4, lights)
+    rows = [(0.9, 0.2, 0.2, 0.5, 0.6, 1.5), (0.6, 0.8, 0.3, 1.0, 0.7, 0.0), (0.3, 0.3, 0.9, 0.2, 0.0, 0.0)]
+    mats = make_materials(rows[:count])
+    mi = mats.view(np.int32).reshape(-1, 8)
+    mi[0, 7] = 0                      # the first object's mesh is textured
+    if count > 1:
+        mi[1, 7] = 1
+    mats.tofile(tmp_path / "objmats.bin")
+    textures, faces = _images(3)
+    lines = [str(len(textures))] + [f"{t.shape[1]} {t.shape[0]}" for t in textures] + ["1"] + \
+        [f"{f.shape[1]} {f.shape[0]}" for f in faces]
+    (tmp_path / "images.txt").write_text("\n".join(lines) + "\n")
+    for k, t in enumerate(textures):
+        t.tofile(tmp_path / f"tex{k}.rgba")
+    for k, f in enumerate(faces):
+        f.tofile(tmp_path / f"face{k}.rgba")
+    w, h, bounces = 160, 120, 3
+    pos, look, up, fov = (0.0, 1.0, -7.0), (0.0, 0.5, 0.0), (0.0, 1.0, 0.0), np.float32(np.pi / 3)
+    (tmp_path / "render.txt").write_text(f"{w} {h} {bounces}\n" +
+                                         " ".join(repr(float(x)) for x in (*pos, *look, *up, fov)) + "\n")
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(OBJ_BIN), str(tmp_path), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cam = O.camera_from(rtx.camera_set(pos, look, up, fov, w / h))
+    tex = O.TexSet(textures, faces)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    for p, objs in enumerate((objects, moved)):
+        if count > 1:
+            ms = O.MultiScene(builds, [m[1] for m in meshes], [m[2] for m in meshes], objs)
+            oacc, ost, orays = O.render_multi(ms, mats, O.lights_from(lights), cam, w, h, bounces, cap=3 * w * h,
+                                              tex=tex)
+        else:
+            b, m0 = builds[0], meshes[0]
+            oacc, ost, orays = O.render(b, m0[1], m0[2], mats, O.lights_from(lights), cam, w, h, bounces,
+                                        cap=3 * w * h, tex=tex)
+        acc = np.fromfile(out / f"frame{p}.accum.u32", np.uint32).reshape(-1, 3)
+        rays, sec = (int(x) for x in (out / f"frame{p}.stats.txt").read_text().split())
+        assert sec > 0 and rays == orays, (rays, sec, orays)
+        assert np.array_equal(acc, oacc), f"frame {p}: accumulators differ in {(acc != oacc).any(1).sum()} pixels"
