@@ -806,8 +806,50 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
     kern = {k: {"launches": c, "avg_launch_ms": round(ms / c, 4)} for k, (c, ms) in rep.items()}
     trace_ms = kern.get("trace_instances", {}).get("avg_launch_ms")
     shadow_ms = kern.get("trace_shadow_instances", {}).get("avg_launch_ms")
+
+    def wave_roots(fn, nf):
+        """fn timed with the wave-uniform root loop (every lane waits for the
+        wave's slowest walk of each root)."""
+        ctx.tune("inst_lane_roots", 0)
+        try:
+            fn(0)
+            return min(run(fn, nf) for _ in range(reps))
+        finally:
+            ctx.tune("inst_lane_roots", 1)
+
+    dt_inst_wave = wave_roots(inst_frame, frames)
     log(f"instances: {dt_inst * 1e3:.3f} ms/frame, {(n + shadow_rays) / dt_inst / 1e6:.1f} Mrays/s incl. shadow; "
-        f"trace {trace_ms} ms, shadow {shadow_ms} ms")
+        f"trace {trace_ms} ms, shadow {shadow_ms} ms; wave-uniform root loop {dt_inst_wave * 1e3:.3f} ms/frame")
+
+    # whole frames over the instances: Whitted depth 3, one material per mesh
+    # (its first instance's), reflection and refraction in the mix
+    rows = [(0.8, 0.3, 0.3, 0.5, 0.0, 0.0), (0.3, 0.8, 0.3, 1.0, 0.5, 0.0), (0.3, 0.3, 0.8, 0.2, 0.3, 1.5),
+            (0.8, 0.8, 0.8, 1.0, 0.0, 0.0)]
+    inst.set_materials(rtx.make_materials([rows[m] for m in mesh_of]))
+    R = rtx.Renderer(ctx, W, H, 3)
+    fst = {}
+
+    def inst_render(f):
+        inst.set_transforms(*poses[f % frames])
+        fst["st"] = R.frame_instances(inst, cam)
+
+    for f in range(3):
+        inst_render(f)
+    dt_inst_frame = min(run(inst_render, min(frames, 20)) for _ in range(reps))
+    ist = fst["st"]
+    ctx.profile(True)
+    ctx.profile_reset()
+    for f in range(3):
+        inst_render(f)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    frame_kern = {k: {"launches_per_frame": round(c / 3, 2), "ms_per_frame": round(ms / 3, 4)}
+                  for k, (c, ms) in rep.items()}
+    dt_inst_frame_wave = wave_roots(inst_render, min(frames, 20))
+    inst_frame_rays = int(ist.primary_rays + ist.secondary_rays)
+    log(f"instances, whole frames: {dt_inst_frame * 1e3:.3f} ms/frame, {inst_frame_rays} rays "
+        f"({ist.secondary_rays} secondary, {ist.shadow_rays} shadow), {inst_frame_rays / dt_inst_frame / 1e6:.1f} "
+        f"Mrays/s; wave-uniform root loop {dt_inst_frame_wave * 1e3:.3f} ms/frame; stages {frame_kern}")
 
     # the aggregate mesh of the same frames: vertices moved on the host
     # (not timed), uploaded and rebuilt every frame
@@ -819,7 +861,7 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
             sd = meshes[mesh_of[i]]
             M = trs[i].astype(np.float64)
             P.append((sd["pos"].astype(np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32))
-            N.append(sd["nrm"])
+            N.append((sd["nrm"].astype(np.float64) @ M[:3, :3].T).astype(np.float32))
             U.append(sd["uv"])
             I.append(sd["idx"] + base)
             base += len(sd["pos"])
@@ -828,7 +870,9 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
     P0, I0, N0, U0 = aggregate(poses[0])
     agg_poses = [P0, aggregate(poses[1])[0]]
     agg = rtx.Scene(ctx)
-    agg.set_geometry(P0, I0, N0, U0, np.zeros(len(I0), np.int32))
+    agg_matmap = np.concatenate([np.full(meshes[mesh_of[i]]["T"], mesh_of[i], np.int32) for i in range(n_inst)])
+    agg.set_geometry(P0, I0, N0, U0, agg_matmap)
+    agg.set_materials(rtx.make_materials(rows))
     agg.set_lights(lights)
     agg.build(30, 1)
 
@@ -850,6 +894,18 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
     ctx.profile(False)
     agg_kern = {k: round(ms / c, 4) for k, (c, ms) in rep.items()}
     log(f"aggregate mesh with rebuild: {dt_agg * 1e3:.3f} ms/frame; kernels {agg_kern}")
+
+    def agg_render(f):
+        agg.update_positions(agg_poses[f % 2])
+        fst["st"] = R.frame(agg, cam, rebuild=True)
+
+    for f in range(2):
+        agg_render(f)
+    dt_agg_frame = min(run(agg_render, min(frames, 20)) for _ in range(reps))
+    ast = fst["st"]
+    log(f"aggregate mesh, whole frames with rebuild: {dt_agg_frame * 1e3:.3f} ms/frame, "
+        f"{int(ast.primary_rays + ast.secondary_rays)} rays")
+    R.destroy()
     inst.destroy()
     agg.destroy()
     for sc in scs:
@@ -863,6 +919,9 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
             "triangles_world": int(n_inst * meshes[0]['T']), "triangles_unique": int(4 * meshes[0]['T']),
             "primary_rays": n, "hits": n_hit, "shadow_rays": shadow_rays,
             "ms_per_frame": round(dt_inst * 1e3, 4), "mrays": round(n / dt_inst / 1e6, 2),
+            "wave_uniform_roots_ms_per_frame": round(dt_inst_wave * 1e3, 4),
+            "root_schedule": "each lane walks its own roots in one loop (inst_lane_roots 1); the wave-uniform root "
+                             "loop beside it",
             "mrays_incl_shadow": round((n + shadow_rays) / dt_inst / 1e6, 2),
             "trace_instances_ms": trace_ms, "trace_shadow_instances_ms": shadow_ms,
             "roofline": {"kernel": "k_trace_inst (closest hit over 16 roots)", "bound": "hbm",
@@ -872,6 +931,20 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
                          "traffic": None, "algorithmic_bytes_per_launch": 48 * n,
                          "note": "48 B/ray (32 B ray in + 16 B hit out); node, triangle and root fetches are cache "
                                  "traffic: the kernel is latency-bound like the single-root traversal"},
+            "whole_frame": {"what": "Renderer frame over the instances (rtx_render_instances): primary rays, then "
+                                    "per bounce the multi-root trace and shadow kernels, secondary rays and shading "
+                                    "through the shared material map, Whitted depth 3, one material per mesh "
+                                    "(diffuse, reflective, refractive IOR 1.5, diffuse); ray counts read back per "
+                                    "bounce; new transforms every frame, no rebuild",
+                            "ms_per_frame": round(dt_inst_frame * 1e3, 4), "rays": inst_frame_rays,
+                            "wave_uniform_roots_ms_per_frame": round(dt_inst_frame_wave * 1e3, 4),
+                            "stages": frame_kern,
+                            "secondary_rays": int(ist.secondary_rays), "shadow_rays": int(ist.shadow_rays),
+                            "mrays": round(inst_frame_rays / dt_inst_frame / 1e6, 2),
+                            "aggregate_rebuild_ms_per_frame": round(dt_agg_frame * 1e3, 4),
+                            "aggregate_rays": int(ast.primary_rays + ast.secondary_rays),
+                            "aggregate_note": "the same frames over the aggregate mesh (rtx_render_frame with a full "
+                                              "LBVH rebuild after the vertex upload; normals of the first pose)"},
             "aggregate_rebuild_ms_per_frame": round(dt_agg * 1e3, 4),
             "aggregate_stage_avg_ms": agg_kern,
             "aggregate_note": "the same rays over one aggregate mesh of the 16 moved copies: vertex upload "

@@ -271,6 +271,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "side_stream") {
         RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: side_stream must be 0 or 1");
         ctx->tune.side_stream = value;
+    } else if (k == "inst_lane_roots") {
+        RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: inst_lane_roots must be 0 or 1");
+        ctx->tune.inst_lane_roots = value;
     } else {
         rtx_set_error("rtx_tune_set: unknown key " + k);
         return RTX_EINVAL;
@@ -287,6 +290,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "coop_leaf") return ctx->tune.coop_leaf;
     if (k == "refill") return ctx->tune.refill;
     if (k == "side_stream") return ctx->tune.side_stream;
+    if (k == "inst_lane_roots") return ctx->tune.inst_lane_roots;
     return -1;
 }
 

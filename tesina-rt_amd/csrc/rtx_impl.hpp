@@ -70,6 +70,7 @@ struct rtx_tuning {
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
     int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
+    int64_t inst_lane_roots = 1;  // instances, single-node forms: each lane walks its own roots in one loop
 };
 
 struct rtx_ctx {

@@ -337,6 +337,16 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x, int src) {
 // Closest hit over the fat leaves flagged in this iteration (cnt > 0 on
 // the owning lanes).  Must be called by all 64 lanes of the wave (`live` is
 // then the full mask): the shuffles of the reduction read every lane.
+// The owner lane's pointer, for callers whose lanes walk different trees.
+__device__ __forceinline__ const float4* bcast(const float4* p, int src) {
+    const uint64_t v = (uint64_t)p;
+    const uint64_t lo = bcast((uint32_t)v, src), hi = bcast((uint32_t)(v >> 32), src);
+    return reinterpret_cast<const float4*>((hi << 32) | lo);
+}
+
+// LANE_TRI: `tri` differs between lanes (each lane's own tree); the owner's
+// is used for its leaf.
+template <bool LANE_TRI = false>
 __device__ __forceinline__ void coop_closest(const float4* __restrict__ tri, uint32_t la, uint32_t lcnt,
                                              const RayC& r, Closest& best, uint64_t live) {
     uint64_t owners = ballot(lcnt != 0u);
@@ -348,6 +358,7 @@ __device__ __forceinline__ void coop_closest(const float4* __restrict__ tri, uin
         const int src = __builtin_ctzll(owners);
         owners &= owners - 1;
         const uint32_t a = bcast(la, src), cnt = bcast(lcnt, src);
+        const float4* __restrict__ ts = LANE_TRI ? bcast(tri, src) : tri;
         const v3 o = mk3(bcast(r.o.x, src), bcast(r.o.y, src), bcast(r.o.z, src));
         const v3 d = mk3(bcast(r.d.x, src), bcast(r.d.y, src), bcast(r.d.z, src));
         const float tmin = bcast(r.tmin, src);
@@ -358,7 +369,7 @@ __device__ __forceinline__ void coop_closest(const float4* __restrict__ tri, uin
             bool h = false;
             if (k < cnt) {
                 const size_t t3 = 3 * (size_t)(a + k);
-                h = tri_hit_v(tri[t3], tri[t3 + 1], tri[t3 + 2], o, d, &tt, &u, &v) && tt <= bt && tt >= tmin;
+                h = tri_hit_v(ts[t3], ts[t3 + 1], ts[t3 + 2], o, d, &tt, &u, &v) && tt <= bt && tt >= tmin;
             }
             // key: smallest t first (the +0.f folds -0 into +0, which compare
             // equal), then the highest index
@@ -388,6 +399,7 @@ __device__ __forceinline__ void coop_closest(const float4* __restrict__ tri, uin
 
 // Any hit over the flagged fat leaves; returns true on the owning lanes
 // whose leaf holds a hit with tMin < t < tMax.
+template <bool LANE_TRI = false>
 __device__ __forceinline__ bool coop_any(const float4* __restrict__ tri, uint32_t la, uint32_t lcnt, const RayC& r,
                                          float tmax, uint64_t live) {
     uint64_t owners = ballot(lcnt != 0u);
@@ -400,6 +412,7 @@ __device__ __forceinline__ bool coop_any(const float4* __restrict__ tri, uint32_
         const int src = __builtin_ctzll(owners);
         owners &= owners - 1;
         const uint32_t a = bcast(la, src), cnt = bcast(lcnt, src);
+        const float4* __restrict__ ts = LANE_TRI ? bcast(tri, src) : tri;
         const v3 o = mk3(bcast(r.o.x, src), bcast(r.o.y, src), bcast(r.o.z, src));
         const v3 d = mk3(bcast(r.d.x, src), bcast(r.d.y, src), bcast(r.d.z, src));
         const float tmin = bcast(r.tmin, src), tm = bcast(tmax, src);
@@ -409,7 +422,7 @@ __device__ __forceinline__ bool coop_any(const float4* __restrict__ tri, uint32_
             if (k < cnt) {
                 const size_t t3 = 3 * (size_t)(a + k);
                 float tt, u, v;
-                h = tri_hit_v(tri[t3], tri[t3 + 1], tri[t3 + 2], o, d, &tt, &u, &v) && tt < tm && tt > tmin;
+                h = tri_hit_v(ts[t3], ts[t3 + 1], ts[t3 + 2], o, d, &tt, &u, &v) && tt < tm && tt > tmin;
             }
             if (ballot(h)) {
                 if (lane == src) found = true;
@@ -1208,6 +1221,54 @@ __device__ __forceinline__ v3 mul_h(const float* __restrict__ M, v3 v, float vw,
     return mk3(r[0], r[1], r[2]);
 }
 
+// The record of ray rr after the root loop (complete_transformed_hit_info):
+// hit point along the world ray, normal = tr x the interpolated object
+// normal, turned to face the ray; ids in the shared numbering.  Hits join
+// the hit list.  Called by every lane of the wave.
+__device__ __forceinline__ void inst_record(const InstRoot* __restrict__ roots, uint64_t rr, bool active, v3 o, v3 d,
+                                            float bt, float bu, float bv, int32_t bid, int32_t broot,
+                                            uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                            int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                            float4* __restrict__ n4, float4* __restrict__ p4,
+                                            float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
+                                            uint32_t* __restrict__ hit_count, uint32_t* __restrict__ hit_root) {
+    const bool hit = active && bid >= 0;
+    const uint64_t hb = ballot(hit);
+    uint32_t wbase = 0;
+    if (hb) {
+        if (lane_id() == (uint32_t)__builtin_ctzll(hb)) wbase = atomicAdd(hit_count, (uint32_t)__builtin_popcountll(hb));
+        wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+    }
+    if (!active) return;
+    if (!hit) {
+        write_miss_record(rr, flags, tout, idout, bary, n4, p4, uv2);
+        return;
+    }
+    hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
+    hit_root[rr] = (uint32_t)broot;
+    const InstRoot& R = roots[broot];
+    const v3 hp = o + d * bt;
+    const uint32_t i0 = __float_as_uint(R.tri[3 * (size_t)bid].w);
+    const uint32_t i1 = __float_as_uint(R.tri[3 * (size_t)bid + 1].w);
+    const uint32_t i2 = __float_as_uint(R.tri[3 * (size_t)bid + 2].w);
+    const float w = 1.f - (bu + bv);
+    const v3 n0 = xyz(R.nrm[i0]), n1 = xyz(R.nrm[i1]), n2 = xyz(R.nrm[i2]);
+    v3 nn = mul_h(R.tr, normalize(w * n0 + bv * n2 + bu * n1), 0.f, 1e-26f);
+    uint32_t fl = 1u;
+    if (dot(nn, d) > 0.f) {
+        fl |= 2u;
+        nn = -nn;
+    }
+    const float2 t0 = R.uv[i0], t1 = R.uv[i1], t2 = R.uv[i2];
+    flags[rr] = fl;
+    tout[rr] = bt;
+    idout[rr] = bid + (int32_t)R.base;
+    bary[rr] = make_float2(bu, bv);
+    n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
+    p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
+    uv2[rr] = make_float2(w * t0.x + bv * t2.x + bu * t1.x, w * t0.y + bv * t2.y + bu * t1.y);
+}
+
 // Closest hit over the roots: each root traced from the ray's own tMax in
 // its frame; the root's hit point taken back to the world gives t (its
 // distance from the origin); the first root with the smallest t wins.  The
@@ -1227,7 +1288,6 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
     __shared__ uint32_t s_id[STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
-    const uint32_t lane = threadIdx.x & 63;
     for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
         const uint64_t rr = base + threadIdx.x;
         const bool active = rr < n;
@@ -1261,41 +1321,8 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
                 }
             }
         }
-        const bool hit = active && bid >= 0;
-        const uint64_t hb = ballot(hit);
-        uint32_t wbase = 0;
-        if (hb) {
-            if (lane == (uint32_t)__builtin_ctzll(hb)) wbase = atomicAdd(hit_count, (uint32_t)__builtin_popcountll(hb));
-            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
-        }
-        if (!active) continue;
-        if (!hit) {
-            write_miss_record(rr, flags, tout, idout, bary, n4, p4, uv2);
-            continue;
-        }
-        hit_list[wbase + lanes_below(hb)] = (uint32_t)rr;
-        hit_root[rr] = (uint32_t)broot;
-        const InstRoot& R = roots[broot];
-        const v3 hp = o + d * bt;
-        const uint32_t i0 = __float_as_uint(R.tri[3 * (size_t)bid].w);
-        const uint32_t i1 = __float_as_uint(R.tri[3 * (size_t)bid + 1].w);
-        const uint32_t i2 = __float_as_uint(R.tri[3 * (size_t)bid + 2].w);
-        const float w = 1.f - (bu + bv);
-        const v3 n0 = xyz(R.nrm[i0]), n1 = xyz(R.nrm[i1]), n2 = xyz(R.nrm[i2]);
-        v3 nn = mul_h(R.tr, normalize(w * n0 + bv * n2 + bu * n1), 0.f, 1e-26f);
-        uint32_t fl = 1u;
-        if (dot(nn, d) > 0.f) {
-            fl |= 2u;
-            nn = -nn;
-        }
-        const float2 t0 = R.uv[i0], t1 = R.uv[i1], t2 = R.uv[i2];
-        flags[rr] = fl;
-        tout[rr] = bt;
-        idout[rr] = bid + (int32_t)R.base;
-        bary[rr] = make_float2(bu, bv);
-        n4[rr] = make_float4(nn.x, nn.y, nn.z, 0.f);
-        p4[rr] = make_float4(hp.x, hp.y, hp.z, 0.f);
-        uv2[rr] = make_float2(w * t0.x + bv * t2.x + bu * t1.x, w * t0.y + bv * t2.y + bu * t1.y);
+        inst_record(roots, rr, active, o, d, bt, bu, bv, bid, broot, flags, tout, idout, bary, n4, p4, uv2, hit_list,
+                    hit_count, hit_root);
     }
 }
 
@@ -1362,6 +1389,250 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
             RayC rc{};
             if (go) rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
             if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go)) shadowed = true;  // whole wave
+        }
+        if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
+    }
+    if (shadow_count) {
+        if (traced) atomicAdd(&s_cnt, traced);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_cnt) atomicAdd(shadow_count, (unsigned long long)s_cnt);
+    }
+}
+
+// Lane-roots form of the two kernels above for the single-node walks (SPEC
+// as in traverse_closest_node).  The root loop and the node loop are one
+// loop: a lane whose walk of a root ends enters its next root in the next
+// step, instead of waiting for the wave's slowest walk of that root, which
+// serializes incoherent rays (bounces) over the roots.  Each root is walked
+// as traverse_closest_node / traverse_any_node walk it (the same visits,
+// stack and MAX_LEVELS rule) and the closest-hit kernel merges the roots in
+// the same order with the same test, so the records are those of the
+// wave-uniform loop.  A fat leaf flagged in a step is tested by the wave
+// (with the owner's triangle pointer) before the owner leaves its root.
+// Entering a root is one step like any other: a root the ray misses is left
+// at its first node test, in the same step as the other lanes' visits.
+template <bool SPEC, int TB>
+__global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict__ roots, int nroots,
+                                                       const float4* __restrict__ ori4,
+                                                       const float4* __restrict__ dir4,
+                                                       const uint32_t* __restrict__ dcount, uint64_t hcount,
+                                                       uint32_t* __restrict__ flags, float* __restrict__ tout,
+                                                       int32_t* __restrict__ idout, float2* __restrict__ bary,
+                                                       float4* __restrict__ n4, float4* __restrict__ p4,
+                                                       float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
+                                                       uint32_t* __restrict__ hit_count,
+                                                       uint32_t* __restrict__ hit_root, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const uint64_t n = ray_count(dcount, hcount);
+    for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
+        const uint64_t rr = base + threadIdx.x;
+        const bool active = rr < n;
+        v3 o = mk3(0.f, 0.f, 0.f), d = o;
+        float tmin = 0.f, tmax = 0.f;
+        if (active) {
+            const float4 o4 = ori4[rr], d4 = dir4[rr];
+            o = xyz(o4);
+            d = xyz(d4);
+            tmin = o4.w;
+            tmax = d4.w;
+        }
+        float bt = 0.f, bu = 0.f, bv = 0.f;
+        int32_t bid = -1, broot = 0;
+        // the lane's current root i: its tree, the ray in its frame, its best hit
+        int i = 0;
+        const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
+        RayC rc{};
+        Closest c{tmax, 0.f, 0.f, -1};
+        uint32_t level = 0, curr = 0;
+        auto enter = [&](int k) {
+            const InstRoot& R = roots[k];
+            node = R.node;
+            delta = R.delta;
+            tri = R.tri;
+            rc = make_ray(mul_h(R.tr_inv, o, 1.f, 1e-26f), mul_h(R.tr_inv, d, 0.f, 1e-26f), tmin);
+            c = Closest{tmax, 0.f, 0.f, -1};
+            level = 0;
+            curr = 0;
+        };
+        bool going = active && nroots > 0;
+        if (going) enter(0);
+        const uint64_t live = ballot(true);
+        while (ballot(going)) {
+            uint32_t la = 0, lcnt = 0;
+            bool done = false;
+            if (going) {
+                const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+                float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (SPEC) {
+                    dl4 = delta[curr];
+                    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+                }
+                const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+                bool pop = true;
+                if (slab(f0, f1, rc, c.t)) {
+                    if (cnt >= coop) {
+                        la = a;
+                        lcnt = cnt;
+                    } else if (cnt) {
+                        leaf_closest(tri, a, cnt, rc, c);
+                    } else {
+                        if (!SPEC) dl4 = delta[curr];
+                        const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
+                        stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                        level++;
+                        curr = nl ? a : a + 1;
+                        pop = false;
+                    }
+                }
+                if (pop) {
+                    if (level == 0 || level >= STACK)  // the walk of root i ends (MAX_LEVELS as trace_ray)
+                        done = true;
+                    else
+                        curr = stk.get(--level);
+                }
+            }
+            coop_closest<true>(tri, la, lcnt, rc, c, live);
+            if (done) {
+                if (c.id >= 0) {
+                    const v3 hw = mul_h(roots[i].tr, rc.o + rc.d * c.t, 1.f, 1e-26f);
+                    const v3 dv = hw - o;
+                    const float tw = __builtin_sqrtf(dot(dv, dv));
+                    if (bid < 0 || tw < bt) {
+                        bt = tw;
+                        bid = c.id;
+                        bu = c.u;
+                        bv = c.v;
+                        broot = i;
+                    }
+                }
+                if (++i < nroots)
+                    enter(i);
+                else
+                    going = false;
+            }
+        }
+        inst_record(roots, rr, active, o, d, bt, bu, bv, bid, broot, flags, tout, idout, bary, n4, p4, uv2, hit_list,
+                    hit_count, hit_root);
+    }
+}
+
+template <bool SPEC, int TB>
+__global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restrict__ roots, int nroots,
+                                                        const rtx_lights lights,
+                                                        const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ list_count,
+                                                        uint32_t* __restrict__ flags, const float4* __restrict__ p4,
+                                                        const uint32_t* __restrict__ hit_root,
+                                                        unsigned long long* shadow_count, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_cnt;
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const int nl = lights.count;
+    const uint64_t items = (uint64_t)(*list_count) * (uint64_t)nl;
+    uint32_t traced = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * TB; base < items; base += (uint64_t)gridDim.x * TB) {
+        const uint64_t it = base + threadIdx.x;
+        const bool valid = it < items;
+        uint32_t rr = 0;
+        int l = 0;
+        v3 p = mk3(0.f, 0.f, 0.f), dir = p;
+        float tmax = 1e37f;
+        bool trace = false;
+        if (valid) {
+            rr = list[it / (uint64_t)nl];
+            l = (int)(it % (uint64_t)nl);
+            const rtx_light& L = lights.l[l];
+            p = xyz(p4[rr]);
+            trace = true;
+            if (L.type == RTX_LIGHT_DIR) {
+                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+            } else if (L.type == RTX_LIGHT_SPOT) {
+                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+                if (dot(dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) trace = false;
+            } else if (L.type == RTX_LIGHT_POINT) {
+                const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+                tmax = __builtin_sqrtf(dot(toL, toL));
+                dir = normalize(toL);
+            } else {
+                trace = false;
+            }
+            if (trace) ++traced;
+        }
+        // the lane's roots: all of them once (k counts them), from the one its
+        // hit came from (an OR over the roots: the order changes no bit)
+        int i = 0, k = 0;
+        const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
+        RayC rc{};
+        uint32_t level = 0, curr = 0;
+        auto enter = [&](int j) {
+            const InstRoot& R = roots[j];
+            node = R.node;
+            delta = R.delta;
+            tri = R.tri;
+            rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
+            level = 0;
+            curr = 0;
+        };
+        bool going = trace && nroots > 0, shadowed = false;
+        if (going) {
+            i = hit_root ? (int)(hit_root[rr] % (uint32_t)nroots) : 0;
+            enter(i);
+        }
+        const uint64_t live = ballot(true);
+        while (ballot(going)) {
+            uint32_t la = 0, lcnt = 0;
+            bool done = false;
+            if (going) {
+                const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+                float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (SPEC) {
+                    dl4 = delta[curr];
+                    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+                }
+                const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+                bool pop = true;
+                if (slab(f0, f1, rc, tmax)) {
+                    if (cnt >= coop) {
+                        la = a;
+                        lcnt = cnt;
+                    } else if (cnt) {
+                        if (leaf_any(tri, a, cnt, rc, tmax)) {
+                            shadowed = true;
+                            going = false;
+                            pop = false;
+                        }
+                    } else {
+                        if (!SPEC) dl4 = delta[curr];
+                        const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
+                        stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                        level++;
+                        curr = nl ? a : a + 1;
+                        pop = false;
+                    }
+                }
+                if (pop) {
+                    if (level == 0 || level >= STACK)
+                        done = true;
+                    else
+                        curr = stk.get(--level);
+                }
+            }
+            if (coop_any<true>(tri, la, lcnt, rc, tmax, live)) {
+                shadowed = true;
+                going = false;
+                done = false;
+            }
+            if (done) {
+                if (++k < nroots) {
+                    i = (i + 1 == nroots) ? 0 : i + 1;
+                    enter(i);
+                } else {
+                    going = false;
+                }
+            }
         }
         if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
     }
@@ -1901,8 +2172,11 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, st, hits->list_count.as<uint32_t>());
     ProfScope ps(in->ctx, st, "trace_instances");
     const int tb = (int)in->ctx->tune.trace_block;
+    const bool lane_roots = var != 2 && in->ctx->tune.inst_lane_roots != 0;
 #define RTX_TRACE_INST(V, B)                                                                                        \
-    hipLaunchKernelGGL((k_trace_inst<V, B>), dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st,       \
+    hipLaunchKernelGGL((lane_roots ? (V == 1 ? k_trace_inst_lr<true, B> : k_trace_inst_lr<false, B>)              \
+                                   : k_trace_inst<V, B>),                                                          \
+                       dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st,                             \
                        in->roots.as<InstRoot>(), in->n, rays->ori4.as<float4>(), rays->dir4.as<float4>(),          \
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),         \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(),                     \
@@ -1937,8 +2211,11 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
     // the buffer covers every ray of this bundle (stale values only reorder)
     const uint32_t* order = in->hit_root.bytes >= hits->cap * 4 ? in->hit_root.as<uint32_t>() : nullptr;
     const int tb = (int)in->ctx->tune.trace_block;
+    const bool lane_roots = var != 2 && in->ctx->tune.inst_lane_roots != 0;
 #define RTX_SHADOW_INST(V, B)                                                                                       \
-    hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,          \
+    hipLaunchKernelGGL((lane_roots ? (V == 1 ? k_shadow_inst_lr<true, B> : k_shadow_inst_lr<false, B>)            \
+                                   : k_shadow_inst<V, B>),                                                         \
+                       dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,                                 \
                        in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(),                     \
                        hits->list_count.as<uint32_t>(), hits->flags.as<uint32_t>(), hits->p4.as<float4>(),         \
                        order, (unsigned long long*)dev_shadow_count, coop_threshold(in->ctx))

@@ -78,17 +78,23 @@ def _oracle(meshes, builds, insts, lights, ori4, dir4):
 
 
 def _check_all_forms(ctx, inst, r, hb, n, o, what):
-    old = ctx.tuned("trace_variant")
+    """Every traversal form; the single-node forms (0, 1 and 3, which maps to
+    1) under both root schedules: each lane on its own roots in one loop, and
+    the wave-uniform root loop."""
+    old, old_lr = ctx.tuned("trace_variant"), ctx.tuned("inst_lane_roots")
     try:
         for variant in (0, 1, 2, 3):
-            ctx.tune("trace_variant", variant)
-            inst.trace(r, hb, n)
-            inst.trace_shadow(r, hb, n)
-            g = hb.read(n)
-            for k in FIELDS:
-                assert np.array_equal(g[k], o[k]), f"{what}: {k} differs (variant {variant})"
+            for lane_roots in ((1, 0) if variant != 2 else (1,)):
+                ctx.tune("trace_variant", variant)
+                ctx.tune("inst_lane_roots", lane_roots)
+                inst.trace(r, hb, n)
+                inst.trace_shadow(r, hb, n)
+                g = hb.read(n)
+                for k in FIELDS:
+                    assert np.array_equal(g[k], o[k]), f"{what}: {k} differs (variant {variant}, lane roots {lane_roots})"
     finally:
         ctx.tune("trace_variant", old)
+        ctx.tune("inst_lane_roots", old_lr)
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -230,6 +236,13 @@ def test_frame_over_instances(ctx, case):
     oacc, ost, orays = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
     assert (oacc.sum(1) > 0).mean() > 0.15 and st.secondary_rays > 0
     assert st.primary_rays + st.secondary_rays == orays
+    assert np.array_equal(R.fb.accum(), oacc)
+    # the wave-uniform root loop gives the same frame (bounce rays are incoherent)
+    ctx.tune("inst_lane_roots", 0)
+    try:
+        R.frame_instances(inst, cam)
+    finally:
+        ctx.tune("inst_lane_roots", 1)
     assert np.array_equal(R.fb.accum(), oacc)
     # new transforms: the next frame follows them, no rebuild
     moved = [(m, *xform(rot((0, 1, 0), 0.4 * (i + 1)), 1.0, (1.2 * i - 1.2, 0.1 * i, 0.0))) for i, (m, _, _) in
