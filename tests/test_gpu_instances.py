@@ -303,3 +303,43 @@ def test_textured_frame_over_instances(ctx):
     inst.destroy()
     for s in scs:
         s.destroy()
+
+
+def test_frame_after_scene_geometry_changes(ctx):
+    """A frame over instances, then one scene's geometry replaced by a mesh of
+    another triangle count and rebuilt: the next frame renumbers the shared
+    ids and rebuilds the material map (each mesh its first instance's
+    material), bit for bit with the oracle's frame over the new meshes."""
+    import rtx
+    from rtx import make_materials
+    meshes = two_meshes()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    insts = [(0, *xform(rot((0, 1, 0), 0.5), 1.0, (-1.5, 0.2, 0.0))),
+             (1, *xform(rot((1, 0, 0), 0.9), 1.4, (1.4, -0.2, 0.5))),
+             (0, *xform(np.eye(3), 0.6, (0.0, 1.7, -0.5)))]
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    mats = make_materials([(0.9, 0.3, 0.2, 0.5, 0.6, 0.0), (0.2, 0.8, 0.3, 1.0, 0.3, 1.5),
+                           (0.3, 0.3, 0.9, 0.2, 0.0, 0.0)])
+    inst.set_materials(mats)
+    w, h = 160, 120
+    cam = rtx.camera_set((0.0, 0.8, -5.0), (0.0, 0.3, 0.0), (0.0, 1.0, 0.0), np.pi / 3, w / h)
+    R = rtx.Renderer(ctx, w, h, 3)
+    for step, mesh0 in enumerate((meshes[0], sphere_mesh(40, 28, 0.9))):
+        if step:
+            p, nn, uv, idx = mesh0
+            scs[0].set_geometry(p, idx, nn, uv, np.zeros(len(idx), np.int32))
+            scs[0].build(30, 1)
+        cur = [mesh0, meshes[1]]
+        builds = [O.Build(m[0], m[3], None, 30, 1) for m in cur]
+        st = R.frame_instances(inst, cam)
+        ms = O.MultiScene(builds, [m[1] for m in cur], [m[2] for m in cur], insts)
+        oacc, _, orays = O.render_multi(ms, mats, O.lights_from(lights), O.camera_from(cam), w, h, 3)
+        assert st.primary_rays + st.secondary_rays == orays
+        assert np.array_equal(R.fb.accum(), oacc), f"step {step}: accumulators differ"
+        assert np.array_equal(inst.tri_base(), np.asarray([0, len(mesh0[3]), 0], np.uint32))
+    R.destroy()
+    inst.destroy()
+    for s in scs:
+        s.destroy()
