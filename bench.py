@@ -807,19 +807,22 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
     trace_ms = kern.get("trace_instances", {}).get("avg_launch_ms")
     shadow_ms = kern.get("trace_shadow_instances", {}).get("avg_launch_ms")
 
-    def wave_roots(fn, nf):
-        """fn timed with the wave-uniform root loop (every lane waits for the
+    def schedule(fn, nf, mode):
+        """fn timed under another root schedule: 1 each lane walks every root
+        in one loop, 0 the wave-uniform root loop (every lane waits for the
         wave's slowest walk of each root)."""
-        ctx.tune("inst_lane_roots", 0)
+        ctx.tune("inst_lane_roots", mode)
         try:
             fn(0)
             return min(run(fn, nf) for _ in range(reps))
         finally:
-            ctx.tune("inst_lane_roots", 1)
+            ctx.tune("inst_lane_roots", 2)
 
-    dt_inst_wave = wave_roots(inst_frame, frames)
+    dt_inst_all = schedule(inst_frame, frames, 1)
+    dt_inst_wave = schedule(inst_frame, frames, 0)
     log(f"instances: {dt_inst * 1e3:.3f} ms/frame, {(n + shadow_rays) / dt_inst / 1e6:.1f} Mrays/s incl. shadow; "
-        f"trace {trace_ms} ms, shadow {shadow_ms} ms; wave-uniform root loop {dt_inst_wave * 1e3:.3f} ms/frame")
+        f"trace {trace_ms} ms, shadow {shadow_ms} ms; lane roots without culling {dt_inst_all * 1e3:.3f} ms/frame, "
+        f"wave-uniform root loop {dt_inst_wave * 1e3:.3f} ms/frame")
 
     # whole frames over the instances: Whitted depth 3, one material per mesh
     # (its first instance's), reflection and refraction in the mix
@@ -845,11 +848,13 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
     ctx.profile(False)
     frame_kern = {k: {"launches_per_frame": round(c / 3, 2), "ms_per_frame": round(ms / 3, 4)}
                   for k, (c, ms) in rep.items()}
-    dt_inst_frame_wave = wave_roots(inst_render, min(frames, 20))
+    dt_inst_frame_all = schedule(inst_render, min(frames, 20), 1)
+    dt_inst_frame_wave = schedule(inst_render, min(frames, 20), 0)
     inst_frame_rays = int(ist.primary_rays + ist.secondary_rays)
     log(f"instances, whole frames: {dt_inst_frame * 1e3:.3f} ms/frame, {inst_frame_rays} rays "
         f"({ist.secondary_rays} secondary, {ist.shadow_rays} shadow), {inst_frame_rays / dt_inst_frame / 1e6:.1f} "
-        f"Mrays/s; wave-uniform root loop {dt_inst_frame_wave * 1e3:.3f} ms/frame; stages {frame_kern}")
+        f"Mrays/s; lane roots without culling {dt_inst_frame_all * 1e3:.3f} ms/frame, wave-uniform root loop "
+        f"{dt_inst_frame_wave * 1e3:.3f} ms/frame; stages {frame_kern}")
 
     # the aggregate mesh of the same frames: vertices moved on the host
     # (not timed), uploaded and rebuilt every frame
@@ -919,9 +924,11 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
             "triangles_world": int(n_inst * meshes[0]['T']), "triangles_unique": int(4 * meshes[0]['T']),
             "primary_rays": n, "hits": n_hit, "shadow_rays": shadow_rays,
             "ms_per_frame": round(dt_inst * 1e3, 4), "mrays": round(n / dt_inst / 1e6, 2),
+            "lane_roots_no_culling_ms_per_frame": round(dt_inst_all * 1e3, 4),
             "wave_uniform_roots_ms_per_frame": round(dt_inst_wave * 1e3, 4),
-            "root_schedule": "each lane walks its own roots in one loop (inst_lane_roots 1); the wave-uniform root "
-                             "loop beside it",
+            "root_schedule": "each lane walks, in one loop, the roots whose widened world box its ray meets "
+                             "(inst_lane_roots 2); beside it every root in one loop (1) and the wave-uniform root "
+                             "loop (0)",
             "mrays_incl_shadow": round((n + shadow_rays) / dt_inst / 1e6, 2),
             "trace_instances_ms": trace_ms, "trace_shadow_instances_ms": shadow_ms,
             "roofline": {"kernel": "k_trace_inst (closest hit over 16 roots)", "bound": "hbm",
@@ -937,6 +944,7 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
                                     "(diffuse, reflective, refractive IOR 1.5, diffuse); ray counts read back per "
                                     "bounce; new transforms every frame, no rebuild",
                             "ms_per_frame": round(dt_inst_frame * 1e3, 4), "rays": inst_frame_rays,
+                            "lane_roots_no_culling_ms_per_frame": round(dt_inst_frame_all * 1e3, 4),
                             "wave_uniform_roots_ms_per_frame": round(dt_inst_frame_wave * 1e3, 4),
                             "stages": frame_kern,
                             "secondary_rays": int(ist.secondary_rays), "shadow_rays": int(ist.shadow_rays),

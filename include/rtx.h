@@ -171,9 +171,12 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    stages on the caller's stream (linear graphs).
  *   "inst_lane_roots": instances under the single-node forms (0, 1): 1: each
  *                    lane moves on to its next root as soon as its walk of the
- *                    last one ends, in one loop; 0: the wave walks the roots
- *                    one after the other, every lane waiting for the wave's
- *                    slowest walk of each root.
+ *                    last one ends, in one loop; 2 (default): the same over
+ *                    only the roots whose world-space box (the root box under
+ *                    the transform, widened far beyond rounding) the ray
+ *                    meets, for sets of up to 64 affine roots; 0: the wave
+ *                    walks the roots one after the other, every lane waiting
+ *                    for the wave's slowest walk of each root.
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */

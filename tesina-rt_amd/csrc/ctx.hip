@@ -272,7 +272,7 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
         RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: side_stream must be 0 or 1");
         ctx->tune.side_stream = value;
     } else if (k == "inst_lane_roots") {
-        RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: inst_lane_roots must be 0 or 1");
+        RTX_REQUIRE(value >= 0 && value <= 2, RTX_EINVAL, "rtx_tune_set: inst_lane_roots must be 0, 1 or 2");
         ctx->tune.inst_lane_roots = value;
     } else {
         rtx_set_error("rtx_tune_set: unknown key " + k);

@@ -70,7 +70,8 @@ struct rtx_tuning {
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
     int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
-    int64_t inst_lane_roots = 1;  // instances, single-node forms: each lane walks its own roots in one loop
+    int64_t inst_lane_roots = 2;  // instances, single-node forms: 1 each lane walks its own roots in one loop,
+                                  // 2 the same over the roots whose world box its ray meets, 0 wave-uniform
 };
 
 struct rtx_ctx {
@@ -243,6 +244,16 @@ struct InstRoot {
     float tr_inv[16];
 };
 
+// World-space box of a root for the lane-roots kernels' culling: the root
+// node's box under tr, inflated at use by k * (2 |o| + s) (k < 0: never
+// culled).
+struct RootBox {
+    float lo[3];
+    float k;
+    float hi[3];
+    float s;
+};
+
 // Objects of a scene (Scene::create_bvhs / update_bvh_roots,
 // src/rt/scene.cpp:596-700): instances of built scenes under transforms.
 // `table` is the host image of the device root table; `shipped` is what the
@@ -255,6 +266,7 @@ struct rtx_instances {
     std::vector<InstRoot> table, shipped;
     DevBuf roots;
     DevBuf hit_root;  // uint32 per ray: the root of its closest hit (an order hint for shadow rays)
+    DevBuf wbox;      // RootBox per root (culling of the lane-roots kernels)
     rtx_lights lights{};
     // Shading of frames over the instances: a scene holding only the
     // materials (one per distinct scene: its first instance's), the material

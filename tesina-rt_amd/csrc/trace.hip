@@ -1399,6 +1399,86 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
     }
 }
 
+// World-space boxes of the roots for culling (one thread per root): the
+// root node's box under tr (its 8 corners, rows summed in order), the box's
+// magnitude s, and k = 1e-4 |tr| |tr_inv| (infinity norms).  A ray is
+// tested against the box widened by k (2 |o| + s): the rounding of the ray's
+// transform into the object's frame and of the hit point's way back (a few
+// units of 6e-8, times the transforms' condition, times the magnitudes of
+// the origin and of the point, which lies within the box) stays far below
+// it, so a root whose widened box the ray misses holds no hit the walk
+// could find.  Roots under a projective transform (bottom rows other than
+// 0 0 0 1), or with a non-finite box, get k = -1: never culled.
+__global__ void k_root_boxes(const InstRoot* __restrict__ roots, int n, RootBox* __restrict__ out) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    const InstRoot& R = roots[k];
+    const float* M = R.tr;
+    const float* Mi = R.tr_inv;
+    const bool affine = M[12] == 0.f && M[13] == 0.f && M[14] == 0.f && M[15] == 1.f && Mi[12] == 0.f &&
+                        Mi[13] == 0.f && Mi[14] == 0.f && Mi[15] == 1.f;
+    const float4 b0 = R.node[0], b1 = R.node[1];
+    RootBox b;
+    float nm = 1.f, nmi = 1.f;
+    for (int r = 0; r < 3; ++r) {
+        b.lo[r] = INFINITY;
+        b.hi[r] = -INFINITY;
+        nm = fmaxf(nm, fabsf(M[4 * r]) + fabsf(M[4 * r + 1]) + fabsf(M[4 * r + 2]) + fabsf(M[4 * r + 3]));
+        nmi = fmaxf(nmi, fabsf(Mi[4 * r]) + fabsf(Mi[4 * r + 1]) + fabsf(Mi[4 * r + 2]) + fabsf(Mi[4 * r + 3]));
+    }
+    for (int c = 0; c < 8; ++c) {
+        const float x = (c & 1) ? b1.x : b0.x, y = (c & 2) ? b1.y : b0.y, z = (c & 4) ? b1.z : b0.z;
+        for (int r = 0; r < 3; ++r) {
+            const float w = ((M[4 * r] * x + M[4 * r + 1] * y) + M[4 * r + 2] * z) + M[4 * r + 3];
+            b.lo[r] = fminf(b.lo[r], w);
+            b.hi[r] = fmaxf(b.hi[r], w);
+        }
+    }
+    b.s = 0.f;
+    bool finite = true;
+    for (int r = 0; r < 3; ++r) {
+        b.s = fmaxf(b.s, fmaxf(fabsf(b.lo[r]), fabsf(b.hi[r])));
+        finite = finite && isfinite(b.lo[r]) && isfinite(b.hi[r]);
+    }
+    b.k = 1e-4f * nm * nmi;
+    if (!affine || !finite || !isfinite(b.k) || !isfinite(b.s)) b.k = -1.f;
+    out[k] = b;
+}
+
+// Bit k set: the segment [tmin, tmax] of the ray (o, d) meets root k's
+// widened world box, or the root is never culled.  n <= 64.  The loop is
+// wave-uniform (the boxes are read as scalars).
+__device__ __forceinline__ uint64_t root_candidates(const RootBox* __restrict__ wb, int n, v3 o, v3 d, float tmin,
+                                                    float tmax) {
+    const float io[3] = {o.x, o.y, o.z}, id[3] = {d.x, d.y, d.z};
+    float inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = id[a] != 0.f ? 1.f / id[a] : 0.f;
+    const float mo = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
+    uint64_t m = 0;
+    for (int k = 0; k < n; ++k) {
+        const RootBox b = wb[k];
+        bool meet = true;
+        if (b.k >= 0.f) {
+            const float g = b.k * (2.f * mo + b.s);
+            float t0 = tmin, t1 = tmax;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float l = b.lo[a] - g, h = b.hi[a] + g;
+                if (id[a] == 0.f) {
+                    meet = meet && io[a] >= l && io[a] <= h;
+                } else {
+                    const float ta = (l - io[a]) * inv[a], tb = (h - io[a]) * inv[a];
+                    t0 = fmaxf(t0, fminf(ta, tb));
+                    t1 = fminf(t1, fmaxf(ta, tb));
+                }
+            }
+            meet = meet && t0 <= t1;
+        }
+        if (meet) m |= 1ull << k;
+    }
+    return m;
+}
+
 // Lane-roots form of the two kernels above for the single-node walks (SPEC
 // as in traverse_closest_node).  The root loop and the node loop are one
 // loop: a lane whose walk of a root ends enters its next root in the next
@@ -1421,10 +1501,12 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
                                                        float4* __restrict__ n4, float4* __restrict__ p4,
                                                        float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
                                                        uint32_t* __restrict__ hit_count,
-                                                       uint32_t* __restrict__ hit_root, uint32_t coop) {
+                                                       uint32_t* __restrict__ hit_root, uint32_t coop,
+                                                       const RootBox* __restrict__ wbox) {
     __shared__ uint32_t s_id[STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
+    const bool cull = wbox != nullptr && nroots <= 64;
     for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
         const uint64_t rr = base + threadIdx.x;
         const bool active = rr < n;
@@ -1439,8 +1521,19 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
         }
         float bt = 0.f, bu = 0.f, bv = 0.f;
         int32_t bid = -1, broot = 0;
-        // the lane's current root i: its tree, the ray in its frame, its best hit
-        int i = 0;
+        // the lane's current root i: its tree, the ray in its frame, its best
+        // hit; the roots in ascending order (with culling: those in cand)
+        int i = -1;
+        uint64_t cand = cull ? root_candidates(wbox, nroots, o, d, tmin, tmax) : 0;
+        auto next = [&]() -> bool {
+            if (cull) {
+                if (!cand) return false;
+                i = __builtin_ctzll(cand);
+                cand &= cand - 1;
+                return true;
+            }
+            return ++i < nroots;
+        };
         const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
         RayC rc{};
         Closest c{tmax, 0.f, 0.f, -1};
@@ -1455,8 +1548,8 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
             level = 0;
             curr = 0;
         };
-        bool going = active && nroots > 0;
-        if (going) enter(0);
+        bool going = active && next();
+        if (going) enter(i);
         const uint64_t live = ballot(true);
         while (ballot(going)) {
             uint32_t la = 0, lcnt = 0;
@@ -1506,7 +1599,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
                         broot = i;
                     }
                 }
-                if (++i < nroots)
+                if (next())
                     enter(i);
                 else
                     going = false;
@@ -1524,12 +1617,14 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                                                         const uint32_t* __restrict__ list_count,
                                                         uint32_t* __restrict__ flags, const float4* __restrict__ p4,
                                                         const uint32_t* __restrict__ hit_root,
-                                                        unsigned long long* shadow_count, uint32_t coop) {
+                                                        unsigned long long* shadow_count, uint32_t coop,
+                                                        const RootBox* __restrict__ wbox) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
     LdsStack<TB> stk{s_id, threadIdx.x};
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
+    const bool cull = wbox != nullptr && nroots <= 64;
     const int nl = lights.count;
     const uint64_t items = (uint64_t)(*list_count) * (uint64_t)nl;
     uint32_t traced = 0;
@@ -1561,9 +1656,25 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
             }
             if (trace) ++traced;
         }
-        // the lane's roots: all of them once (k counts them), from the one its
-        // hit came from (an OR over the roots: the order changes no bit)
-        int i = 0, k = 0;
+        // the lane's roots: all of them once (k counts them; with culling,
+        // those in cand), from the one its hit came from (an OR over the
+        // roots: the order changes no bit)
+        int i = 0, k = 0, i0 = 0;
+        uint64_t cand = cull ? root_candidates(wbox, nroots, p, dir, 0.01f, tmax) : 0;
+        if (trace && hit_root && nroots > 0) i0 = (int)(hit_root[rr] % (uint32_t)nroots);
+        auto next = [&]() -> bool {
+            if (cull) {
+                if (!cand) return false;
+                i = (k == 0 && ((cand >> i0) & 1ull)) ? i0 : __builtin_ctzll(cand);
+                cand &= ~(1ull << i);
+                ++k;
+                return true;
+            }
+            if (k >= nroots) return false;
+            i = k == 0 ? i0 : (i + 1 == nroots ? 0 : i + 1);
+            ++k;
+            return true;
+        };
         const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
         RayC rc{};
         uint32_t level = 0, curr = 0;
@@ -1576,11 +1687,9 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
             level = 0;
             curr = 0;
         };
-        bool going = trace && nroots > 0, shadowed = false;
-        if (going) {
-            i = hit_root ? (int)(hit_root[rr] % (uint32_t)nroots) : 0;
-            enter(i);
-        }
+        bool shadowed = false;
+        bool going = trace && next();
+        if (going) enter(i);
         const uint64_t live = ballot(true);
         while (ballot(going)) {
             uint32_t la = 0, lcnt = 0;
@@ -1626,12 +1735,10 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                 done = false;
             }
             if (done) {
-                if (++k < nroots) {
-                    i = (i + 1 == nroots) ? 0 : i + 1;
+                if (next())
                     enter(i);
-                } else {
+                else
                     going = false;
-                }
             }
         }
         if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
@@ -1998,7 +2105,9 @@ int32_t rtx_instances_create(rtx_ctx* ctx, rtx_scene* const* scenes, const float
     in->tr.assign(tr, tr + 16 * (size_t)n);
     in->tr_inv.assign(tr_inv, tr_inv + 16 * (size_t)n);
     int32_t e = in->roots.reserve(ctx, sizeof(InstRoot) * (size_t)n);
+    if (!e) e = in->wbox.reserve(ctx, sizeof(RootBox) * (size_t)n);
     if (e) {
+        in->roots.release();
         delete in;
         return e;
     }
@@ -2011,6 +2120,7 @@ int32_t rtx_instances_destroy(rtx_instances* in) {
     (void)hipDeviceSynchronize();
     in->roots.release();
     in->hit_root.release();
+    in->wbox.release();
     if (in->shading) rtx_scene_destroy(in->shading);
     delete in;
     return RTX_OK;
@@ -2153,6 +2263,16 @@ static int32_t refresh_roots(rtx_instances* in, hipStream_t st, int* var) {
     return RTX_OK;
 }
 
+// The culling boxes of the lane-roots kernels (inst_lane_roots 2, sets of
+// up to 64 roots), computed on the stream from the current root table and
+// root boxes; nullptr: no culling.
+static const RootBox* root_boxes(rtx_instances* in, bool lane_roots, hipStream_t st) {
+    if (!lane_roots || in->ctx->tune.inst_lane_roots != 2 || in->n > 64) return nullptr;
+    hipLaunchKernelGGL(k_root_boxes, dim3((in->n + 63) / 64), dim3(64), 0, st, in->roots.as<InstRoot>(), in->n,
+                       in->wbox.as<RootBox>());
+    return in->wbox.as<RootBox>();
+}
+
 int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
     RTX_REQUIRE(in && rays && hits, RTX_EINVAL, "rtx_trace_instances: null argument");
     RTX_REQUIRE(count == RTX_DEVICE_COUNT || (count <= rays->cap && count <= hits->cap), RTX_EINVAL,
@@ -2174,15 +2294,24 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
     const int tb = (int)in->ctx->tune.trace_block;
     const bool lane_roots = var != 2 && in->ctx->tune.inst_lane_roots != 0;
 #define RTX_TRACE_INST(V, B)                                                                                        \
-    hipLaunchKernelGGL((lane_roots ? (V == 1 ? k_trace_inst_lr<true, B> : k_trace_inst_lr<false, B>)              \
-                                   : k_trace_inst<V, B>),                                                          \
-                       dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st,                             \
-                       in->roots.as<InstRoot>(), in->n, rays->ori4.as<float4>(), rays->dir4.as<float4>(),          \
-                       rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),         \
-                       hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(),                     \
-                       hits->p4.as<float4>(), hits->uv2.as<float2>(), hits->list.as<uint32_t>(),                   \
-                       hits->list_count.as<uint32_t>(), in->hit_root.as<uint32_t>(), coop_threshold(in->ctx))
+    do {                                                                                                        \
+        if (lane_roots) {                                                                                           \
+            hipLaunchKernelGGL((V == 1 ? k_trace_inst_lr<true, B> : k_trace_inst_lr<false, B>),                     \
+                               dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st, TRACE_INST_ARGS,     \
+                               boxes);                                                                              \
+        } else {                                                                                                    \
+            hipLaunchKernelGGL((k_trace_inst<V, B>), dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st,\
+                               TRACE_INST_ARGS);                                                                    \
+        }                                                                                                           \
+    } while (0)
+#define TRACE_INST_ARGS                                                                                             \
+    in->roots.as<InstRoot>(), in->n, rays->ori4.as<float4>(), rays->dir4.as<float4>(), rays->count.as<uint32_t>(), \
+        count, hits->flags.as<uint32_t>(), hits->t.as<float>(), hits->id.as<int32_t>(), hits->bary.as<float2>(),    \
+        hits->n4.as<float4>(), hits->p4.as<float4>(), hits->uv2.as<float2>(), hits->list.as<uint32_t>(),            \
+        hits->list_count.as<uint32_t>(), in->hit_root.as<uint32_t>(), coop_threshold(in->ctx)
+    const RootBox* boxes = root_boxes(in, lane_roots, st);
     RTX_DISPATCH_TRAVERSAL(RTX_TRACE_INST, var, tb);
+#undef TRACE_INST_ARGS
 #undef RTX_TRACE_INST
     RTX_CHECK_HIP(hipGetLastError());
     hits->list_valid = true;
@@ -2213,13 +2342,22 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
     const int tb = (int)in->ctx->tune.trace_block;
     const bool lane_roots = var != 2 && in->ctx->tune.inst_lane_roots != 0;
 #define RTX_SHADOW_INST(V, B)                                                                                       \
-    hipLaunchKernelGGL((lane_roots ? (V == 1 ? k_shadow_inst_lr<true, B> : k_shadow_inst_lr<false, B>)            \
-                                   : k_shadow_inst<V, B>),                                                         \
-                       dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,                                 \
-                       in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(),                     \
-                       hits->list_count.as<uint32_t>(), hits->flags.as<uint32_t>(), hits->p4.as<float4>(),         \
-                       order, (unsigned long long*)dev_shadow_count, coop_threshold(in->ctx))
+    do {                                                                                                        \
+        if (lane_roots) {                                                                                           \
+            hipLaunchKernelGGL((V == 1 ? k_shadow_inst_lr<true, B> : k_shadow_inst_lr<false, B>),                   \
+                               dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st, SHADOW_INST_ARGS, boxes);\
+        } else {                                                                                                    \
+            hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,   \
+                               SHADOW_INST_ARGS);                                                                   \
+        }                                                                                                           \
+    } while (0)
+#define SHADOW_INST_ARGS                                                                                            \
+    in->roots.as<InstRoot>(), in->n, in->lights, hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>(),        \
+        hits->flags.as<uint32_t>(), hits->p4.as<float4>(), order, (unsigned long long*)dev_shadow_count,            \
+        coop_threshold(in->ctx)
+    const RootBox* boxes = root_boxes(in, lane_roots, st);
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
+#undef SHADOW_INST_ARGS
 #undef RTX_SHADOW_INST
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;

@@ -79,12 +79,13 @@ def _oracle(meshes, builds, insts, lights, ori4, dir4):
 
 def _check_all_forms(ctx, inst, r, hb, n, o, what):
     """Every traversal form; the single-node forms (0, 1 and 3, which maps to
-    1) under both root schedules: each lane on its own roots in one loop, and
-    the wave-uniform root loop."""
+    1) under the three root schedules: each lane on its own roots in one loop,
+    over the roots whose world box its ray meets (2) or over all of them (1),
+    and the wave-uniform root loop (0)."""
     old, old_lr = ctx.tuned("trace_variant"), ctx.tuned("inst_lane_roots")
     try:
         for variant in (0, 1, 2, 3):
-            for lane_roots in ((1, 0) if variant != 2 else (1,)):
+            for lane_roots in ((2, 1, 0) if variant != 2 else (2,)):
                 ctx.tune("trace_variant", variant)
                 ctx.tune("inst_lane_roots", lane_roots)
                 inst.trace(r, hb, n)
@@ -237,13 +238,14 @@ def test_frame_over_instances(ctx, case):
     assert (oacc.sum(1) > 0).mean() > 0.15 and st.secondary_rays > 0
     assert st.primary_rays + st.secondary_rays == orays
     assert np.array_equal(R.fb.accum(), oacc)
-    # the wave-uniform root loop gives the same frame (bounce rays are incoherent)
-    ctx.tune("inst_lane_roots", 0)
-    try:
-        R.frame_instances(inst, cam)
-    finally:
-        ctx.tune("inst_lane_roots", 1)
-    assert np.array_equal(R.fb.accum(), oacc)
+    # the other root schedules give the same frame (bounce rays are incoherent)
+    for lane_roots in (1, 0):
+        ctx.tune("inst_lane_roots", lane_roots)
+        try:
+            R.frame_instances(inst, cam)
+        finally:
+            ctx.tune("inst_lane_roots", 2)
+        assert np.array_equal(R.fb.accum(), oacc), f"root schedule {lane_roots}"
     # new transforms: the next frame follows them, no rebuild
     moved = [(m, *xform(rot((0, 1, 0), 0.4 * (i + 1)), 1.0, (1.2 * i - 1.2, 0.1 * i, 0.0))) for i, (m, _, _) in
              enumerate(insts)]
