@@ -345,3 +345,86 @@ def test_frame_after_scene_geometry_changes(ctx):
     inst.destroy()
     for s in scs:
         s.destroy()
+
+
+def _world_boxes(meshes, insts):
+    out = []
+    for m, tr, _ in insts:
+        M = tr.astype(np.float64)
+        w = meshes[m][0].astype(np.float64) @ M[:3, :3].T + M[:3, 3]
+        out.append((w.min(0), w.max(0)))
+    return out
+
+
+def test_culling_boundaries(ctx):
+    """Rays at the edges of root culling (inst_lane_roots 2) against the
+    oracle, which walks every root: axis-parallel rays along the faces, edges
+    and middles of the instances' world boxes (zero direction components),
+    rays from 1e4 away, rays whose tMax ends just short of, at or just past
+    their hit, and instances with large and tiny scales; every form and
+    schedule, shadow rays included."""
+    import rtx
+    meshes = two_meshes()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    insts = [(0, *xform(rot((0, 1, 0), 0.3), 1.0, (-2.0, 0.0, 0.0))),
+             (1, *xform(np.eye(3), 1.0, (2.0, 0.5, 0.0))),
+             (0, *xform(rot((1, 1, 0), 1.1), 250.0, (0.0, 0.0, 900.0))),
+             (1, *xform(np.eye(3), 0.004, (0.0, 3.0, 0.0)))]
+    rng = np.random.default_rng(77)
+    O4, D4 = [], []
+
+    def ray(o, d, tmax=1e30):
+        O4.append([*o, 0.0])
+        D4.append([*d, tmax])
+    for lo, hi in _world_boxes(meshes, insts):
+        mid = (lo + hi) / 2
+        ext = float((hi - lo).max())
+        for a in range(3):
+            b, c = [k for k in range(3) if k != a]
+            for vb in (lo[b], mid[b], hi[b]):
+                for vc in (lo[c], mid[c], hi[c]):
+                    for sgn in (1.0, -1.0):
+                        o = np.zeros(3)
+                        o[a] = (lo[a] - ext) if sgn > 0 else (hi[a] + ext)
+                        o[b], o[c] = vb, vc
+                        d = np.zeros(3)
+                        d[a] = sgn
+                        ray(o, d)
+        for _ in range(60):  # from far away toward the box
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            o = mid + 1e4 * u
+            tgt = mid + rng.uniform(-0.5, 0.5, 3) * (hi - lo)
+            d = (tgt - o) / np.linalg.norm(tgt - o)
+            ray(o, d)
+    O4 = np.asarray(O4, np.float32)
+    D4 = np.asarray(D4, np.float32)
+    # tMax at, just short of and just past the hits of a first pass
+    o0, _ = _oracle(meshes, builds, insts, lights, O4, D4)
+    hit = np.nonzero((o0["flags"] & 1) != 0)[0]
+    assert hit.size > 100
+    t = o0["t"][hit]
+    extra_o, extra_d = [], []
+    for tm in (t, np.nextafter(t, np.float32(0)), np.nextafter(t, np.float32(np.inf)),
+               (t * np.float32(1 - 1e-6)).astype(np.float32)):
+        extra_o.append(O4[hit])
+        dd = D4[hit].copy()
+        dd[:, 3] = tm
+        extra_d.append(dd)
+    O4 = np.concatenate([O4, *extra_o])
+    D4 = np.concatenate([D4, *extra_d])
+    n = len(O4)
+    o, _ = _oracle(meshes, builds, insts, lights, O4, D4)
+    r = rtx.Rays(ctx, n)
+    r.write(O4, D4)
+    hb = rtx.Hits(ctx, n)
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    _check_all_forms(ctx, inst, r, hb, n, o, "culling boundaries")
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    for s in scs:
+        s.destroy()
