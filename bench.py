@@ -127,7 +127,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(image-space sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
+              "(image-space sharding of a frame across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
