@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--inst-frames", type=int, default=50, help="frames of the instanced leg (0 = skip)")
     ap.add_argument("--accel-frames", type=int, default=10,
                     help="frames of the LBVH / SAH, stack / stackless leg (0 = skip)")
+    ap.add_argument("--image-frames", type=int, default=10,
+                    help="frames of the textured-frame leg; also times OBJ ingest (0 = skip)")
     ap.add_argument("--sort-sizes", type=str, default="16777216,67108864",
                     help="element counts of the sort-scaling leg ('' = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -382,6 +384,8 @@ def main():
         out["instances"] = bench_instances(ctx, args.inst_frames)
     if args.accel_frames > 0:
         out["accelerators"] = bench_accelerators(ctx, args.accel_frames)
+    if args.image_frames > 0:
+        out["images_and_ingest"] = bench_images_and_ingest(ctx, args.image_frames)
     if args.sort_sizes:
         out["sort_scaling"] = bench_sort_scaling(ctx, [int(x) for x in args.sort_sizes.split(",") if x])
     if args.c4_scenes > 0:
@@ -957,6 +961,111 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
             "aggregate_stage_avg_ms": agg_kern,
             "aggregate_note": "the same rays over one aggregate mesh of the 16 moved copies: vertex upload "
                               "(positions moved on the host beforehand, not timed) + full LBVH rebuild per frame"}
+
+
+def bench_images_and_ingest(ctx, frames: int = 10, reps: int = 3):
+    """Textured diffuse and the cubemap background, and OBJ ingest.
+
+    Textures: the C2 frame without rebuild (1M triangles, 1920x1080, Whitted
+    depth 3), plain, then with every material textured (four 512x512 RGBA8
+    images, index m % 4) under a cubemap of six 512x512 faces; frame time and
+    the shading stage's time (HIP events).  Ingest: a 250k-triangle OBJ with
+    positions, uvs and normals and its MTL file, written at run time, read by
+    the library's loader (host), then uploaded and built (device)."""
+    import os
+    import shutil
+    import tempfile
+    import numpy as np
+    import rtx
+    from rtx import scenes
+    sd = scenes.config_c2()
+    s = scenes.load(rtx.Scene(ctx), sd)
+    cam = scenes.camera_of(sd)
+    R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces)
+    R.frame(s, cam, rebuild=True)
+
+    def timed():
+        ctx.device_sync()
+        t = time.perf_counter()
+        for _ in range(frames):
+            R.frame(s, cam, rebuild=False)
+        ctx.device_sync()
+        return (time.perf_counter() - t) / frames
+
+    def shade_ms():
+        ctx.profile(True)
+        ctx.profile_reset()
+        R.frame(s, cam, rebuild=False)
+        rep = ctx.profile_report()
+        ctx.profile(False)
+        c, ms = rep.get("shade", (1, 0.0))
+        return round(ms, 4), int(c)
+
+    dt_plain = min(timed() for _ in range(reps))
+    shade_plain, launches = shade_ms()
+    rng = np.random.default_rng(0x7E57)
+    texs = [rng.integers(0, 256, (512, 512, 4), dtype=np.uint8) for _ in range(4)]
+    faces = [rng.integers(0, 256, (512, 512, 4), dtype=np.uint8) for _ in range(6)]
+    mats = np.ascontiguousarray(sd.materials).view(np.int32).reshape(-1, 8).copy()
+    mats[:, 7] = np.arange(len(mats)) % 4
+    s.set_materials(mats.view(np.uint8).reshape(-1))
+    s.set_textures(texs)
+    s.set_cubemap(faces)
+    R.frame(s, cam, rebuild=False)
+    dt_tex = min(timed() for _ in range(reps))
+    shade_tex, _ = shade_ms()
+    log(f"textures: C2 frame without rebuild {dt_plain * 1e3:.3f} ms plain, {dt_tex * 1e3:.3f} ms textured with "
+        f"cubemap; shading {shade_plain} -> {shade_tex} ms per frame ({launches} launches)")
+    R.destroy()
+    s.destroy()
+
+    # OBJ ingest
+    tmp = tempfile.mkdtemp(prefix="rtx_obj_", dir="/tmp")
+    try:
+        p, nrm, uv, idx = scenes.uv_sphere(500, 250, (0.0, 0.0, 0.0), 1.0)
+        path = os.path.join(tmp, "sphere.obj")
+        with open(os.path.join(tmp, "sphere.mtl"), "w") as f:
+            f.write("newmtl red\nKd 0.8 0.2 0.2\nKa 0.1 0.1 0.1\nKs 0.5 0.5 0.5\nNs 20\n")
+        with open(path, "w") as f:
+            f.write("mtllib sphere.mtl\n")
+            np.savetxt(f, p, fmt="v %.6f %.6f %.6f")
+            np.savetxt(f, uv, fmt="vt %.6f %.6f")
+            np.savetxt(f, nrm, fmt="vn %.6f %.6f %.6f")
+            f.write("usemtl red\n")
+            i1 = idx.astype(np.int64) + 1
+            np.savetxt(f, np.repeat(i1, 3, axis=1), fmt="f %d/%d/%d %d/%d/%d %d/%d/%d")
+        size = os.path.getsize(path)
+        t = time.perf_counter()
+        m = rtx.ObjModel(path)
+        dt_parse = time.perf_counter() - t
+        sc = rtx.Scene(ctx)
+        ctx.device_sync()
+        t = time.perf_counter()
+        sc.set_geometry(m.pos, m.idx, m.nrm, m.uv, np.zeros(len(m.idx), np.int32))
+        sc.build(30, 1)
+        ctx.device_sync()
+        dt_up = time.perf_counter() - t
+        sc.destroy()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    T = int(len(m.idx))
+    log(f"OBJ ingest: {T} triangles, {size / 1e6:.1f} MB parsed in {dt_parse * 1e3:.1f} ms "
+        f"({size / dt_parse / 1e6:.0f} MB/s), upload + LBVH build {dt_up * 1e3:.1f} ms")
+    return {"textures": {"workload": "C2 frame without rebuild (1M triangles, 1920x1080, 2 point lights, Whitted "
+                                     "depth 3); textured: every material samples one of four 512x512 RGBA8 images "
+                                     "(OpenCL linear filter, repeat), misses sample a cubemap of six 512x512 faces "
+                                     "(clamp to edge)",
+                         "plain_ms_per_frame": round(dt_plain * 1e3, 4),
+                         "textured_ms_per_frame": round(dt_tex * 1e3, 4),
+                         "shade_ms_per_frame_plain": shade_plain, "shade_ms_per_frame_textured": shade_tex,
+                         "shade_launches_per_frame": launches},
+            "obj_ingest": {"workload": "one OBJ file with v, vt, vn and v/vt/vn faces plus its MTL file, written at "
+                                       "run time (a 500x250 UV sphere), read by the library's loader on the host, "
+                                       "then uploaded and built into an LBVH",
+                           "triangles": T, "bytes": int(size), "parse_ms": round(dt_parse * 1e3, 2),
+                           "parse_MBps": round(size / dt_parse / 1e6, 1),
+                           "parse_mtris_per_s": round(T / dt_parse / 1e6, 3),
+                           "upload_and_build_ms": round(dt_up * 1e3, 2)}}
 
 
 def bench_accelerators(ctx, frames: int = 10):
