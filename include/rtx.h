@@ -174,7 +174,8 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    last one ends, in one loop; 2 (default): the same over
  *                    only the roots whose world-space box (the root box under
  *                    the transform, widened far beyond rounding) the ray
- *                    meets, for sets of up to 64 affine roots; 0: the wave
+ *                    meets, affine roots only, 64 roots per wave-uniform
+ *                    chunk; 0: the wave
  *                    walks the roots one after the other, every lane waiting
  *                    for the wave's slowest walk of each root.
  * All settings give identical results. */

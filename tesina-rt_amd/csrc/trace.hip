@@ -1491,7 +1491,7 @@ __device__ __forceinline__ uint64_t root_candidates(const RootBox* __restrict__ 
 // (with the owner's triangle pointer) before the owner leaves its root.
 // Entering a root is one step like any other: a root the ray misses is left
 // at its first node test, in the same step as the other lanes' visits.
-template <bool SPEC, int TB>
+template <bool SPEC, bool CHUNKS, int TB>
 __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict__ roots, int nroots,
                                                        const float4* __restrict__ ori4,
                                                        const float4* __restrict__ dir4,
@@ -1506,7 +1506,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
     __shared__ uint32_t s_id[STACK * TB];
     LdsStack<TB> stk{s_id, threadIdx.x};
     const uint64_t n = ray_count(dcount, hcount);
-    const bool cull = wbox != nullptr && nroots <= 64;
+    const bool cull = wbox != nullptr && (CHUNKS || nroots <= 64);
     for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
         const uint64_t rr = base + threadIdx.x;
         const bool active = rr < n;
@@ -1522,18 +1522,10 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
         float bt = 0.f, bu = 0.f, bv = 0.f;
         int32_t bid = -1, broot = 0;
         // the lane's current root i: its tree, the ray in its frame, its best
-        // hit; the roots in ascending order (with culling: those in cand)
+        // hit.  The roots in ascending order; with culling, those in cand,
+        // over chunks of 64 roots when CHUNKS (the chunk loop is
+        // wave-uniform; sets of up to 64 roots use the kernel without it).
         int i = -1;
-        uint64_t cand = cull ? root_candidates(wbox, nroots, o, d, tmin, tmax) : 0;
-        auto next = [&]() -> bool {
-            if (cull) {
-                if (!cand) return false;
-                i = __builtin_ctzll(cand);
-                cand &= cand - 1;
-                return true;
-            }
-            return ++i < nroots;
-        };
         const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
         RayC rc{};
         Closest c{tmax, 0.f, 0.f, -1};
@@ -1548,69 +1540,87 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
             level = 0;
             curr = 0;
         };
-        bool going = active && next();
-        if (going) enter(i);
         const uint64_t live = ballot(true);
-        while (ballot(going)) {
-            uint32_t la = 0, lcnt = 0;
-            bool done = false;
-            if (going) {
-                const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-                float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (SPEC) {
-                    dl4 = delta[curr];
-                    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+        auto walk = [&](int c0, int cn) {
+            uint64_t cand = cull ? root_candidates(wbox + c0, cn, o, d, tmin, tmax) : 0;
+            i = c0 - 1;
+            auto next = [&]() -> bool {
+                if (cull) {
+                    if (!cand) return false;
+                    i = c0 + __builtin_ctzll(cand);
+                    cand &= cand - 1;
+                    return true;
                 }
-                const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-                bool pop = true;
-                if (slab(f0, f1, rc, c.t)) {
-                    if (cnt >= coop) {
-                        la = a;
-                        lcnt = cnt;
-                    } else if (cnt) {
-                        leaf_closest(tri, a, cnt, rc, c);
-                    } else {
-                        if (!SPEC) dl4 = delta[curr];
-                        const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
-                        stk.put(level & (STACK - 1), nl ? a + 1 : a);
-                        level++;
-                        curr = nl ? a : a + 1;
-                        pop = false;
+                return ++i < nroots;
+            };
+            bool going = active && next();
+            if (going) enter(i);
+            while (ballot(going)) {
+                uint32_t la = 0, lcnt = 0;
+                bool done = false;
+                if (going) {
+                    const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+                    float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (SPEC) {
+                        dl4 = delta[curr];
+                        __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+                    }
+                    const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+                    bool pop = true;
+                    if (slab(f0, f1, rc, c.t)) {
+                        if (cnt >= coop) {
+                            la = a;
+                            lcnt = cnt;
+                        } else if (cnt) {
+                            leaf_closest(tri, a, cnt, rc, c);
+                        } else {
+                            if (!SPEC) dl4 = delta[curr];
+                            const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
+                            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                            level++;
+                            curr = nl ? a : a + 1;
+                            pop = false;
+                        }
+                    }
+                    if (pop) {
+                        if (level == 0 || level >= STACK)  // the walk of root i ends (MAX_LEVELS as trace_ray)
+                            done = true;
+                        else
+                            curr = stk.get(--level);
                     }
                 }
-                if (pop) {
-                    if (level == 0 || level >= STACK)  // the walk of root i ends (MAX_LEVELS as trace_ray)
-                        done = true;
+                coop_closest<true>(tri, la, lcnt, rc, c, live);
+                if (done) {
+                    if (c.id >= 0) {
+                        const v3 hw = mul_h(roots[i].tr, rc.o + rc.d * c.t, 1.f, 1e-26f);
+                        const v3 dv = hw - o;
+                        const float tw = __builtin_sqrtf(dot(dv, dv));
+                        if (bid < 0 || tw < bt) {
+                            bt = tw;
+                            bid = c.id;
+                            bu = c.u;
+                            bv = c.v;
+                            broot = i;
+                        }
+                    }
+                    if (next())
+                        enter(i);
                     else
-                        curr = stk.get(--level);
+                        going = false;
                 }
             }
-            coop_closest<true>(tri, la, lcnt, rc, c, live);
-            if (done) {
-                if (c.id >= 0) {
-                    const v3 hw = mul_h(roots[i].tr, rc.o + rc.d * c.t, 1.f, 1e-26f);
-                    const v3 dv = hw - o;
-                    const float tw = __builtin_sqrtf(dot(dv, dv));
-                    if (bid < 0 || tw < bt) {
-                        bt = tw;
-                        bid = c.id;
-                        bu = c.u;
-                        bv = c.v;
-                        broot = i;
-                    }
-                }
-                if (next())
-                    enter(i);
-                else
-                    going = false;
-            }
+        };
+        if (CHUNKS) {
+            for (int c0 = 0; c0 < nroots; c0 += 64) walk(c0, nroots - c0 < 64 ? nroots - c0 : 64);
+        } else {
+            walk(0, nroots);
         }
         inst_record(roots, rr, active, o, d, bt, bu, bv, bid, broot, flags, tout, idout, bary, n4, p4, uv2, hit_list,
                     hit_count, hit_root);
     }
 }
 
-template <bool SPEC, int TB>
+template <bool SPEC, bool CHUNKS, int TB>
 __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restrict__ roots, int nroots,
                                                         const rtx_lights lights,
                                                         const uint32_t* __restrict__ list,
@@ -1624,7 +1634,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
     LdsStack<TB> stk{s_id, threadIdx.x};
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    const bool cull = wbox != nullptr && nroots <= 64;
+    const bool cull = wbox != nullptr && (CHUNKS || nroots <= 64);
     const int nl = lights.count;
     const uint64_t items = (uint64_t)(*list_count) * (uint64_t)nl;
     uint32_t traced = 0;
@@ -1656,25 +1666,12 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
             }
             if (trace) ++traced;
         }
-        // the lane's roots: all of them once (k counts them; with culling,
-        // those in cand), from the one its hit came from (an OR over the
-        // roots: the order changes no bit)
-        int i = 0, k = 0, i0 = 0;
-        uint64_t cand = cull ? root_candidates(wbox, nroots, p, dir, 0.01f, tmax) : 0;
+        // the lane's roots: all of them once (k counts them), an OR over the
+        // roots, so the order changes no bit: from the one its hit came from,
+        // or, with culling, those in cand, its hit's root first (over chunks
+        // of 64 roots when CHUNKS; the chunk loop is wave-uniform)
+        int i = 0, i0 = 0;
         if (trace && hit_root && nroots > 0) i0 = (int)(hit_root[rr] % (uint32_t)nroots);
-        auto next = [&]() -> bool {
-            if (cull) {
-                if (!cand) return false;
-                i = (k == 0 && ((cand >> i0) & 1ull)) ? i0 : __builtin_ctzll(cand);
-                cand &= ~(1ull << i);
-                ++k;
-                return true;
-            }
-            if (k >= nroots) return false;
-            i = k == 0 ? i0 : (i + 1 == nroots ? 0 : i + 1);
-            ++k;
-            return true;
-        };
         const float4 *node = nullptr, *delta = nullptr, *tri = nullptr;
         RayC rc{};
         uint32_t level = 0, curr = 0;
@@ -1688,58 +1685,86 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
             curr = 0;
         };
         bool shadowed = false;
-        bool going = trace && next();
-        if (going) enter(i);
         const uint64_t live = ballot(true);
-        while (ballot(going)) {
-            uint32_t la = 0, lcnt = 0;
-            bool done = false;
-            if (going) {
-                const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
-                float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (SPEC) {
-                    dl4 = delta[curr];
-                    __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+        auto walk = [&](int c0, int cn) {
+            uint64_t cand = cull ? root_candidates(wbox + c0, cn, p, dir, 0.01f, tmax) : 0;
+            int k = 0;
+            auto next = [&]() -> bool {
+                if (cull) {
+                    if (!cand) return false;
+                    const int j0 = i0 - c0;
+                    const bool own = k == 0 && j0 >= 0 && j0 < cn && ((cand >> j0) & 1ull);
+                    const int j = own ? j0 : __builtin_ctzll(cand);
+                    cand &= ~(1ull << j);
+                    i = c0 + j;
+                    ++k;
+                    return true;
                 }
-                const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
-                bool pop = true;
-                if (slab(f0, f1, rc, tmax)) {
-                    if (cnt >= coop) {
-                        la = a;
-                        lcnt = cnt;
-                    } else if (cnt) {
-                        if (leaf_any(tri, a, cnt, rc, tmax)) {
-                            shadowed = true;
-                            going = false;
+                if (k >= nroots) return false;
+                i = k == 0 ? i0 : (i + 1 == nroots ? 0 : i + 1);
+                ++k;
+                return true;
+            };
+            bool going = trace && !shadowed && next();
+            if (going) enter(i);
+            while (ballot(going)) {
+                uint32_t la = 0, lcnt = 0;
+                bool done = false;
+                if (going) {
+                    const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
+                    float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (SPEC) {
+                        dl4 = delta[curr];
+                        __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
+                    }
+                    const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
+                    bool pop = true;
+                    if (slab(f0, f1, rc, tmax)) {
+                        if (cnt >= coop) {
+                            la = a;
+                            lcnt = cnt;
+                        } else if (cnt) {
+                            if (leaf_any(tri, a, cnt, rc, tmax)) {
+                                shadowed = true;
+                                going = false;
+                                pop = false;
+                            }
+                        } else {
+                            if (!SPEC) dl4 = delta[curr];
+                            const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
+                            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                            level++;
+                            curr = nl ? a : a + 1;
                             pop = false;
                         }
-                    } else {
-                        if (!SPEC) dl4 = delta[curr];
-                        const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
-                        stk.put(level & (STACK - 1), nl ? a + 1 : a);
-                        level++;
-                        curr = nl ? a : a + 1;
-                        pop = false;
+                    }
+                    if (pop) {
+                        if (level == 0 || level >= STACK)
+                            done = true;
+                        else
+                            curr = stk.get(--level);
                     }
                 }
-                if (pop) {
-                    if (level == 0 || level >= STACK)
-                        done = true;
+                if (coop_any<true>(tri, la, lcnt, rc, tmax, live)) {
+                    shadowed = true;
+                    going = false;
+                    done = false;
+                }
+                if (done) {
+                    if (next())
+                        enter(i);
                     else
-                        curr = stk.get(--level);
+                        going = false;
                 }
             }
-            if (coop_any<true>(tri, la, lcnt, rc, tmax, live)) {
-                shadowed = true;
-                going = false;
-                done = false;
+        };
+        if (CHUNKS) {
+            for (int c0 = 0; c0 < nroots; c0 += 64) {
+                if (!ballot(trace && !shadowed)) break;
+                walk(c0, nroots - c0 < 64 ? nroots - c0 : 64);
             }
-            if (done) {
-                if (next())
-                    enter(i);
-                else
-                    going = false;
-            }
+        } else {
+            walk(0, nroots);
         }
         if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
     }
@@ -2263,11 +2288,11 @@ static int32_t refresh_roots(rtx_instances* in, hipStream_t st, int* var) {
     return RTX_OK;
 }
 
-// The culling boxes of the lane-roots kernels (inst_lane_roots 2, sets of
-// up to 64 roots), computed on the stream from the current root table and
-// root boxes; nullptr: no culling.
+// The culling boxes of the lane-roots kernels (inst_lane_roots 2), computed
+// on the stream from the current root table and root boxes; nullptr: no
+// culling.
 static const RootBox* root_boxes(rtx_instances* in, bool lane_roots, hipStream_t st) {
-    if (!lane_roots || in->ctx->tune.inst_lane_roots != 2 || in->n > 64) return nullptr;
+    if (!lane_roots || in->ctx->tune.inst_lane_roots != 2) return nullptr;
     hipLaunchKernelGGL(k_root_boxes, dim3((in->n + 63) / 64), dim3(64), 0, st, in->roots.as<InstRoot>(), in->n,
                        in->wbox.as<RootBox>());
     return in->wbox.as<RootBox>();
@@ -2296,7 +2321,8 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
 #define RTX_TRACE_INST(V, B)                                                                                        \
     do {                                                                                                        \
         if (lane_roots) {                                                                                           \
-            hipLaunchKernelGGL((V == 1 ? k_trace_inst_lr<true, B> : k_trace_inst_lr<false, B>),                     \
+            hipLaunchKernelGGL((chunks ? (V == 1 ? k_trace_inst_lr<true, true, B> : k_trace_inst_lr<false, true, B>)  \
+                                      : (V == 1 ? k_trace_inst_lr<true, false, B> : k_trace_inst_lr<false, false, B>)),\
                                dim3(trace_grid(in->ctx, count, rays->cap, B)), dim3(B), 0, st, TRACE_INST_ARGS,     \
                                boxes);                                                                              \
         } else {                                                                                                    \
@@ -2310,6 +2336,7 @@ int32_t rtx_trace_instances(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, u
         hits->n4.as<float4>(), hits->p4.as<float4>(), hits->uv2.as<float2>(), hits->list.as<uint32_t>(),            \
         hits->list_count.as<uint32_t>(), in->hit_root.as<uint32_t>(), coop_threshold(in->ctx)
     const RootBox* boxes = root_boxes(in, lane_roots, st);
+    const bool chunks = boxes != nullptr && in->n > 64;  // culling over chunks of 64 roots
     RTX_DISPATCH_TRAVERSAL(RTX_TRACE_INST, var, tb);
 #undef TRACE_INST_ARGS
 #undef RTX_TRACE_INST
@@ -2344,7 +2371,8 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
 #define RTX_SHADOW_INST(V, B)                                                                                       \
     do {                                                                                                        \
         if (lane_roots) {                                                                                           \
-            hipLaunchKernelGGL((V == 1 ? k_shadow_inst_lr<true, B> : k_shadow_inst_lr<false, B>),                   \
+            hipLaunchKernelGGL((chunks ? (V == 1 ? k_shadow_inst_lr<true, true, B> : k_shadow_inst_lr<false, true, B>)\
+                                      : (V == 1 ? k_shadow_inst_lr<true, false, B> : k_shadow_inst_lr<false, false, B>)),\
                                dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st, SHADOW_INST_ARGS, boxes);\
         } else {                                                                                                    \
             hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,   \
@@ -2356,6 +2384,7 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
         hits->flags.as<uint32_t>(), hits->p4.as<float4>(), order, (unsigned long long*)dev_shadow_count,            \
         coop_threshold(in->ctx)
     const RootBox* boxes = root_boxes(in, lane_roots, st);
+    const bool chunks = boxes != nullptr && in->n > 64;  // culling over chunks of 64 roots
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
 #undef SHADOW_INST_ARGS
 #undef RTX_SHADOW_INST

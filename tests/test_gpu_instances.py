@@ -428,3 +428,40 @@ def test_culling_boundaries(ctx):
     hb.destroy()
     for s in scs:
         s.destroy()
+
+
+def test_many_instances_culled_in_chunks(ctx):
+    """100 instances (a 10 x 10 grid of the two meshes under rotations and
+    scales): root culling runs over two chunks of roots (64 + 36); closest
+    hits and shadow rays equal the oracle's in every form and schedule."""
+    import rtx
+    meshes = two_meshes()
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    insts = []
+    for k in range(100):
+        gx, gy = k % 10, k // 10
+        insts.append((k % 2, *xform(rot((gx + 1.0, 1.0, gy + 1.0), 0.37 * k), 0.3 + 0.02 * (k % 7),
+                                    (1.1 * (gx - 4.5), 1.1 * (gy - 4.5), 0.2 * (k % 3)))))
+    n = 6000
+    ori4, dir4 = rays_at(31, n, spread=8.0)
+    rng = np.random.default_rng(5)
+    tgt = rng.uniform(-5.5, 5.5, (n, 3))
+    tgt[:, 2] = rng.uniform(-0.5, 0.7, n)
+    d = tgt - ori4[:, :3]
+    dir4[:, :3] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    o, _ = _oracle(meshes, builds, insts, lights, ori4, dir4)
+    tris_hit = np.unique(o["id"][(o["flags"] & 1) != 0])
+    assert ((o["flags"] & 1) != 0).sum() > 1000 and tris_hit.size > 50
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    _check_all_forms(ctx, inst, r, hb, n, o, "100 instances")
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    for s in scs:
+        s.destroy()
