@@ -49,3 +49,23 @@ def test_host_only_entry_points():
     # error convention: negative status, message, no exit
     assert lib.rtx_scene_create(None, None) < 0
     assert b"null" in lib.rtx_last_error()
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="librtx.so not built")
+def test_instance_entry_points_reject_null_handles():
+    """The instance set's entry points check their arguments on the host:
+    a negative status and a message naming the call, with no device work."""
+    import rtx
+    lib = rtx.lib()
+    calls = [
+        ("rtx_instances_create", lambda: lib.rtx_instances_create(None, None, None, None, 0, None)),
+        ("rtx_instances_set_materials", lambda: lib.rtx_instances_set_materials(None, None)),
+        ("rtx_instances_shading_scene", lambda: lib.rtx_instances_shading_scene(None, None)),
+        ("rtx_instances_set_transforms", lambda: lib.rtx_instances_set_transforms(None, None, None)),
+        ("rtx_trace_instances", lambda: lib.rtx_trace_instances(None, None, None, 0, None)),
+        ("rtx_trace_shadow_instances", lambda: lib.rtx_trace_shadow_instances(None, None, None, 0, None)),
+        ("rtx_render_instances", lambda: lib.rtx_render_instances(None, None, None, None, None)),
+    ]
+    for name, call in calls:
+        assert call() < 0, name
+        assert name.encode() in lib.rtx_last_error(), (name, lib.rtx_last_error())
