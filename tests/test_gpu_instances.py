@@ -82,20 +82,26 @@ def _check_all_forms(ctx, inst, r, hb, n, o, what):
     1) under the three root schedules: each lane on its own roots in one loop,
     over the roots whose world box its ray meets (2) or over all of them (1),
     and the wave-uniform root loop (0)."""
-    old, old_lr = ctx.tuned("trace_variant"), ctx.tuned("inst_lane_roots")
+    old, old_lr, old_coop = ctx.tuned("trace_variant"), ctx.tuned("inst_lane_roots"), ctx.tuned("coop_leaf")
     try:
         for variant in (0, 1, 2, 3):
             for lane_roots in ((2, 1, 0) if variant != 2 else (2,)):
-                ctx.tune("trace_variant", variant)
-                ctx.tune("inst_lane_roots", lane_roots)
-                inst.trace(r, hb, n)
-                inst.trace_shadow(r, hb, n)
-                g = hb.read(n)
-                for k in FIELDS:
-                    assert np.array_equal(g[k], o[k]), f"{what}: {k} differs (variant {variant}, lane roots {lane_roots})"
+                # coop_leaf 1: every leaf tested by the whole wave (with the
+                # owner lane's triangle pointer in the lane-roots kernels)
+                for coop in ((old_coop, 1) if variant == 1 else (old_coop,)):
+                    ctx.tune("trace_variant", variant)
+                    ctx.tune("inst_lane_roots", lane_roots)
+                    ctx.tune("coop_leaf", coop)
+                    inst.trace(r, hb, n)
+                    inst.trace_shadow(r, hb, n)
+                    g = hb.read(n)
+                    for k in FIELDS:
+                        assert np.array_equal(g[k], o[k]), \
+                            f"{what}: {k} differs (variant {variant}, lane roots {lane_roots}, coop leaf {coop})"
     finally:
         ctx.tune("trace_variant", old)
         ctx.tune("inst_lane_roots", old_lr)
+        ctx.tune("coop_leaf", old_coop)
 
 
 @pytest.mark.parametrize("case", list(CASES))
