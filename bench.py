@@ -935,7 +935,7 @@ def bench_instances(ctx, frames: int = 50, reps: int = 3):
                              "loop (0)",
             "mrays_incl_shadow": round((n + shadow_rays) / dt_inst / 1e6, 2),
             "trace_instances_ms": trace_ms, "trace_shadow_instances_ms": shadow_ms,
-            "roofline": {"kernel": "k_trace_inst (closest hit over 16 roots)", "bound": "hbm",
+            "roofline": {"kernel": "k_trace_inst_lr (closest hit over 16 roots, culled by world boxes)", "bound": "hbm",
                          "achieved": round(48.0 * n / (trace_ms / 1e3) / 1e9, 2) if trace_ms else None,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(48.0 * n / (trace_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5) if trace_ms else None,
