@@ -471,3 +471,71 @@ def test_many_instances_culled_in_chunks(ctx):
     hb.destroy()
     for s in scs:
         s.destroy()
+
+
+def _random_set(rng):
+    """Instances of three meshes: mostly rigid with scales up to 100x either
+    way, some projective; 2 to 80 of them (around the 64-root chunk)."""
+    insts = []
+    for _ in range(int(rng.choice([2, 16, 63, 64, 65, 80]))):
+        m = int(rng.integers(0, 3))
+        if rng.uniform() < 0.1:
+            M = np.eye(4)
+            M[:3, :3] = rot(rng.normal(size=3), rng.uniform(0, 3)) * rng.uniform(0.5, 2)
+            M[:3, 3] = rng.uniform(-3, 3, 3)
+            Mi = np.linalg.inv(M)
+            Mi[3, :3] = rng.uniform(-0.02, 0.02, 3)
+            insts.append((m, np.linalg.inv(Mi).astype(np.float32), Mi.astype(np.float32)))
+        else:
+            sc = float(np.exp(rng.uniform(np.log(0.01), np.log(100.0)))) if rng.uniform() < 0.2 else rng.uniform(0.3, 1.5)
+            insts.append((m, *xform(rot(rng.normal(size=3), rng.uniform(0, 6)), sc, rng.uniform(-6, 6, 3))))
+    return insts
+
+
+def _random_rays(rng, n):
+    """Rays at the scene, a fifth from 1e3 times farther, some axis-parallel,
+    a fifth with a finite tMax."""
+    o = rng.uniform(-8, 8, (n, 3))
+    far = rng.uniform(size=n) < 0.2
+    o[far] *= 1e3
+    d = rng.uniform(-6, 6, (n, 3)) - o
+    ax = rng.uniform(size=n) < 0.15
+    keep = rng.integers(0, 3, n)
+    d[ax] = 0.0
+    d[ax, keep[ax]] = np.sign(rng.normal(size=ax.sum())) + 0.0
+    o[ax] = rng.uniform(-6, 6, (ax.sum(), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ori4 = np.zeros((n, 4), np.float32)
+    dir4 = np.zeros((n, 4), np.float32)
+    ori4[:, :3] = o
+    dir4[:, :3] = d
+    dir4[:, 3] = np.where(rng.uniform(size=n) < 0.2, rng.uniform(0.5, 15, n), 1e30)
+    return ori4, dir4
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_instance_sets(ctx, seed):
+    """Random instance sets and rays (see _random_set, _random_rays): every
+    form and root schedule equals the oracle bit for bit."""
+    import rtx
+    rng = np.random.default_rng(1000 + seed)
+    meshes = [sphere_mesh(12, 8, 1.0), sphere_mesh(20, 14, 0.6), sphere_mesh(9, 6, 1.3)]
+    lights = _lights()
+    scs = _load(ctx, meshes, lights)
+    builds = [O.Build(m[0], m[3], None, 30, 1) for m in meshes]
+    insts = _random_set(rng)
+    n = 3000
+    ori4, dir4 = _random_rays(rng, n)
+    o, _ = _oracle(meshes, builds, insts, lights, ori4, dir4)
+    assert ((o["flags"] & 1) != 0).sum() > 50
+    r = rtx.Rays(ctx, n)
+    r.write(ori4, dir4)
+    hb = rtx.Hits(ctx, n)
+    inst = rtx.Instances(ctx, [scs[m] for m, _, _ in insts], [t for _, t, _ in insts], [ti for _, _, ti in insts])
+    inst.set_lights(lights)
+    _check_all_forms(ctx, inst, r, hb, n, o, f"random set {seed} ({len(insts)} instances)")
+    inst.destroy()
+    r.destroy()
+    hb.destroy()
+    for s in scs:
+        s.destroy()
