@@ -805,6 +805,18 @@ struct ShadowSegs {
     int n;
 };
 
+// Bit 8+l of a hit's flags = "light l is blocked".  The reference assigns
+// shadow_hit on every call (src/kernel/shadow-trace-bvh.cl:402, 412), so a
+// second shadow pass over the same hits (a moved light) must clear the bits
+// of pairs that became lit, not only set new ones.  Lights of one hit run on
+// different lanes, hence the atomics.
+__device__ __forceinline__ void set_shadow_bit(uint32_t* f, int l, bool shadowed) {
+    if (shadowed)
+        atomicOr(f, 1u << (8 + l));
+    else
+        atomicAnd(f, ~(1u << (8 + l)));
+}
+
 // shadow_trace_single (src/kernel/shadow-trace-bvh.cl:375-418) for every
 // (hit, light) pair of the hit lists: DIR toward -dir, SPOT toward pos
 // (outside the cone = shadowed), POINT toward pos with tMax = distance; any
@@ -876,7 +888,7 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
             }
         }
         if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace, parent, bound)) shadowed = true;  // whole wave
-        if (valid && shadowed) atomicOr(&fl[rr], 1u << (8 + l));
+        if (valid) set_shadow_bit(&fl[rr], l, shadowed);
     }
     if (shadow_count) {
         if (traced) atomicAdd(&s_cnt, traced);
@@ -1124,7 +1136,7 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
                         shadowed = true;
                         trace = false;
                     }
-                    if (shadowed) atomicOr(&fl[rr], 1u << (8 + l));
+                    if (!trace) set_shadow_bit(&fl[rr], l, shadowed);
                     if (trace) {
                         ++traced;
                         rc = make_ray(p, dir, 0.01f);
@@ -1139,11 +1151,13 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
         uint32_t la = 0, lcnt = 0;
         if (have) {
             const int st = step_any(node, delta, tri, rc, tmax, stk, level, curr, coop, la, lcnt);
-            if (st == 1) atomicOr(&fl[rr], 1u << (8 + l));
-            if (st != 0) have = false;
+            if (st != 0) {
+                set_shadow_bit(&fl[rr], l, st == 1);
+                have = false;
+            }
         }
         if (coop_any(tri, la, lcnt, rc, tmax, ~0ull)) {  // whole wave; true on owners with a hit
-            atomicOr(&fl[rr], 1u << (8 + l));
+            set_shadow_bit(&fl[rr], l, true);
             have = false;
         }
     }
@@ -1390,7 +1404,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
             if (go) rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
             if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go)) shadowed = true;  // whole wave
         }
-        if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
+        if (valid) set_shadow_bit(&flags[rr], l, shadowed);
     }
     if (shadow_count) {
         if (traced) atomicAdd(&s_cnt, traced);
@@ -1766,7 +1780,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
         } else {
             walk(0, nroots);
         }
-        if (valid && shadowed) atomicOr(&flags[rr], 1u << (8 + l));
+        if (valid) set_shadow_bit(&flags[rr], l, shadowed);
     }
     if (shadow_count) {
         if (traced) atomicAdd(&s_cnt, traced);
