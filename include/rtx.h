@@ -241,6 +241,19 @@ int32_t rtx_scene_animate(rtx_scene* s, float t0, float dt, void* stream);
  * global AABB -> Morton -> radix sort -> permute indices/material map in
  * place -> level-ordered hierarchy -> AABB refit.  Asynchronous. */
 int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving or animation loops):
+ * rtx_lbvh_build with the same kernels and the same tree, calling no
+ * allocator.  Its scratch is a fixed array of RTX_LBVH_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds until the process ends; scenes of more than
+ * RTX_LBVH_NOALLOC_MAX_TRIS triangles are rejected (RTX_EINVAL).  The outputs
+ * are the scene's tree buffers, which must already be reserved for its
+ * triangle count by an earlier rtx_lbvh_build (else RTX_ESTATE).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race.  Asynchronous. */
+#define RTX_LBVH_NOALLOC_MAX_TRIS 1048576u
+#define RTX_LBVH_NOALLOC_SCRATCH_BYTES 43549184u
+int32_t rtx_lbvh_build_noalloc(rtx_scene* s, const rtx_lbvh_params* p, void* stream);
 /* BVHBuilder::refit_lbvh (src/rt/bvh-builder.cpp:743-850): AABBs from the
  * current (already permuted) vertices, topology kept.  Asynchronous. */
 int32_t rtx_lbvh_refit(rtx_scene* s, void* stream);

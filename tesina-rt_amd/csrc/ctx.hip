@@ -225,6 +225,11 @@ int32_t rtx_init(int32_t device, rtx_ctx** out) {
         rtx_set_error("rtx_init: hipStreamCreate failed");
         return RTX_EDEVICE;
     }
+    if (int32_t e = rtx_lbvh_scratch_init(c)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return e;
+    }
     *out = c;
     return RTX_OK;
 }

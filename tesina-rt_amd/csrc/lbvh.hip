@@ -713,8 +713,26 @@ static int32_t lbvh_alloc(rtx_scene* s, uint32_t T, int depth) {
     return 0;
 }
 
-static void fat_leaf_bounds(rtx_scene* s, hipStream_t st) {
-    ProfScope ps(s->ctx, st, "lbvh_fat_leaves");
+// The build's scratch (device memory beyond its inputs and outputs): the
+// scene's buffer set holds it for rtx_lbvh_build; the allocation-free build
+// carves it from a fixed array instead.
+struct BuildScratch {
+    uint32_t *bstart, *bend;        // per-node triangle range during emission
+    uint32_t *keys_alt, *vals_alt;  // radix ping-pong, then the radix-tree arrays
+    float4* centroid;
+    float* partials;
+    uint32_t *sort_hist, *sort_status, *tickets, *hier_status;
+};
+
+static BuildScratch scratch_of(BuildSet& b) {
+    return BuildScratch{b.bstart.as<uint32_t>(),   b.bend.as<uint32_t>(),        b.keys_alt.as<uint32_t>(),
+                        b.vals_alt.as<uint32_t>(), b.centroid.as<float4>(),      b.partials.as<float>(),
+                        b.sort_hist.as<uint32_t>(), b.sort_status.as<uint32_t>(), b.tickets.as<uint32_t>(),
+                        b.hier_status.as<uint32_t>()};
+}
+
+static void fat_leaf_bounds(rtx_scene* s, hipStream_t st, rtx_ctx* pc) {
+    ProfScope ps(pc, st, "lbvh_fat_leaves");
     uint32_t g = rtx_div_up((uint64_t)s->T / (FAT_LEAF + 1) + 1, BS / RTX_WAVE);
     hipLaunchKernelGGL(k_fat_leaves, dim3(g > 1024 ? 1024 : g), dim3(BS), 0, st, s->b().tri.as<float4>(), s->nodes(),
                        s->b().fat.as<uint32_t>(), s->b().levels.as<uint32_t>());
@@ -722,10 +740,10 @@ static void fat_leaf_bounds(rtx_scene* s, hipStream_t st) {
 
 // Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
 // `depth` holds only leaves).
-static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
+static void refit_internal(rtx_scene* s, hipStream_t st, int depth, rtx_ctx* pc) {
     const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
     for (int d = depth - 1; d > REFIT_TOP; --d) {
-        ProfScope ps(s->ctx, st, "lbvh_refit_level");
+        ProfScope ps(pc, st, "lbvh_refit_level");
         uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)s->T;
         if (maxn > 2ull * s->T) maxn = 2ull * s->T;
         uint32_t g = rtx_div_up(maxn, BS);
@@ -734,7 +752,7 @@ static void refit_internal(rtx_scene* s, hipStream_t st, int depth) {
     }
     const int dhi = depth - 1 < REFIT_TOP ? depth - 1 : REFIT_TOP;
     if (dhi >= 0) {
-        ProfScope ps(s->ctx, st, "lbvh_refit_top");
+        ProfScope ps(pc, st, "lbvh_refit_top");
         hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(BS), 0, st, s->nodes(), s->b().delta.as<float4>(),
                            s->b().levels.as<uint32_t>(), dhi);
     }
@@ -758,113 +776,120 @@ static int32_t build_params(rtx_scene* s, const rtx_lbvh_params* p, int* depth_o
 // radix sort, and the in-place permutation of the scene's index, material
 // and id buffers with the triangle records gathered from the positions.  The
 // permutation is what chains one build to the next.
-static void build_sort_phase(rtx_scene* s, int depth, hipStream_t st) {
+// `W` is the scratch; `pc` the context that profiles the launches (null:
+// none).  Returns true when the sorted codes and permutation were left in
+// W.keys_alt / W.vals_alt (an odd pass count).
+static bool build_sort_phase(rtx_scene* s, int depth, hipStream_t st, const BuildScratch& W, rtx_ctx* pc) {
     const uint32_t T = s->T;
     const int passes = (depth + 7) / 8;
     const uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
     const uint32_t hier_tiles = rtx_div_up(T, BS);
-    uint32_t* tickets = s->b().tickets.as<uint32_t>();
+    uint32_t* tickets = W.tickets;
     uint32_t* levels = s->b().levels.as<uint32_t>();
     {
-        ProfScope ps(s->ctx, st, "lbvh_init");
+        ProfScope ps(pc, st, "lbvh_init");
         uint32_t nb = (uint32_t)(passes * (size_t)sort_tiles * 256);
         uint32_t nc = (uint32_t)((size_t)(depth + 1) * hier_tiles);
         uint32_t grid = rtx_div_up((uint64_t)(nb > nc ? nb : nc), BS);
         grid = grid < 1 ? 1 : (grid > 2048 ? 2048 : grid);
         // zero: tickets + histograms (contiguous? no: separate buffers)
-        hipLaunchKernelGGL(k_build_init, dim3(grid), dim3(BS), 0, st, tickets, (uint32_t)TK_COUNT,
-                           s->b().sort_status.as<uint32_t>(), nb, s->b().hier_status.as<uint32_t>(), nc, levels,
-                           s->b().bstart.as<uint32_t>(), s->b().bend.as<uint32_t>(), s->b().parent.as<uint32_t>(), T, 1);
-        hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, s->b().sort_hist.as<uint32_t>(), 4u * 256u,
+        hipLaunchKernelGGL(k_build_init, dim3(grid), dim3(BS), 0, st, tickets, (uint32_t)TK_COUNT, W.sort_status, nb,
+                           W.hier_status, nc, levels, W.bstart, W.bend, s->b().parent.as<uint32_t>(), T, 1);
+        hipLaunchKernelGGL(k_build_init, dim3(1), dim3(BS), 0, st, W.sort_hist, 4u * 256u,
                            (uint32_t*)nullptr, 0u, (uint32_t*)nullptr, 0u, levels, (uint32_t*)nullptr,
                            (uint32_t*)nullptr, (uint32_t*)nullptr, T, 0);
     }
     {
-        ProfScope ps(s->ctx, st, "lbvh_prim_bounds");
+        ProfScope ps(pc, st, "lbvh_prim_bounds");
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > BOUNDS_GRID ? BOUNDS_GRID : grid;
         hipLaunchKernelGGL(k_prim_bounds, dim3(grid), dim3(BS), 0, st, s->idx.as<uint32_t>(), s->pos.as<float4>(),
-                           s->b().centroid.as<float4>(), s->b().partials.as<float>(), T);
-        hipLaunchKernelGGL(k_gbox_reduce, dim3(1), dim3(BS), 0, st, s->b().partials.as<float>(), grid,
+                           W.centroid, W.partials, T);
+        hipLaunchKernelGGL(k_gbox_reduce, dim3(1), dim3(BS), 0, st, W.partials, grid,
                            s->b().gbox.as<float>());
     }
     {
-        ProfScope ps(s->ctx, st, "lbvh_morton");
+        ProfScope ps(pc, st, "lbvh_morton");
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > 1024 ? 1024 : grid;
-        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, s->b().centroid.as<float4>(), s->b().gbox.as<float>(),
-                           s->b().codes.as<uint32_t>(), s->b().perm.as<uint32_t>(), s->b().sort_hist.as<uint32_t>(), T, depth,
-                           passes);
+        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, W.centroid, s->b().gbox.as<float>(),
+                           s->b().codes.as<uint32_t>(), s->b().perm.as<uint32_t>(), W.sort_hist, T, depth, passes);
     }
     uint32_t *kin = s->b().codes.as<uint32_t>(), *vin = s->b().perm.as<uint32_t>();
-    uint32_t *kout = s->b().keys_alt.as<uint32_t>(), *vout = s->b().vals_alt.as<uint32_t>();
+    uint32_t *kout = W.keys_alt, *vout = W.vals_alt;
     for (int pass = 0; pass < passes; ++pass) {
-        ProfScope ps(s->ctx, st, "lbvh_sort_pass");
+        ProfScope ps(pc, st, "lbvh_sort_pass");
         int shift = 8 * pass;
         int bits = depth - shift < 8 ? depth - shift : 8;
         hipLaunchKernelGGL(k_sort_pass, dim3(sort_tiles), dim3(BS), 0, st, kin, vin, kout, vout, T, shift, bits,
-                           s->b().sort_hist.as<uint32_t>() + 256 * pass,
-                           s->b().sort_status.as<uint32_t>() + (size_t)pass * sort_tiles * 256, tickets + TK_SORT + pass);
+                           W.sort_hist + 256 * pass, W.sort_status + (size_t)pass * sort_tiles * 256,
+                           tickets + TK_SORT + pass);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (kin != s->b().codes.as<uint32_t>()) {  // odd pass count: sorted data sits in the alternates
-        std::swap(s->b().codes, s->b().keys_alt);
-        std::swap(s->b().perm, s->b().vals_alt);
-    }
+    const bool in_alt = kin != s->b().codes.as<uint32_t>();  // odd pass count
     {
         // permute in place: copy, then gather back into the scene's buffers
         // (no handle swap, so a captured frame chains builds as eager ones do)
-        ProfScope ps(s->ctx, st, "lbvh_rearrange");
+        ProfScope ps(pc, st, "lbvh_rearrange");
         uint32_t g = rtx_div_up(T, BS);
         hipLaunchKernelGGL(k_stage_copy, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->idx.as<uint32_t>(),
                            s->idx_alt.as<uint32_t>(), s->matmap.as<int32_t>(), s->matmap_alt.as<int32_t>(),
                            s->orig.as<uint32_t>(), s->orig_alt.as<uint32_t>(), T);
-        hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->b().perm.as<uint32_t>(),
+        hipLaunchKernelGGL(k_rearrange, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, vin,
                            s->idx_alt.as<uint32_t>(), s->idx.as<uint32_t>(), s->matmap_alt.as<int32_t>(),
                            s->matmap.as<int32_t>(), s->orig_alt.as<uint32_t>(), s->orig.as<uint32_t>(),
                            s->pos.as<float4>(), s->b().tri.as<float4>(), T);
     }
+    return in_alt;
 }
 
 // Tree phase: radix tree, level-ordered hierarchy, fat-leaf bounds and the
 // bottom-up refit.  It reads only the buffer set's own sorted codes and
 // triangle records, so it can run beside the next build's sort phase when
 // that one writes another set.
-static void build_tree_phase(rtx_scene* s, int depth, int min_leaf, hipStream_t st) {
+static void build_tree_phase(rtx_scene* s, int depth, int min_leaf, hipStream_t st, const BuildScratch& W,
+                             rtx_ctx* pc) {
     const uint32_t T = s->T;
     const uint32_t hier_tiles = rtx_div_up(T, BS);
-    uint32_t* tickets = s->b().tickets.as<uint32_t>();
+    uint32_t* tickets = W.tickets;
     uint32_t* levels = s->b().levels.as<uint32_t>();
     {
-        ProfScope ps(s->ctx, st, "lbvh_radix_tree");
+        ProfScope ps(pc, st, "lbvh_radix_tree");
         // radix-tree arrays reuse the sort's alternate buffers
         hipLaunchKernelGGL(k_radix_tree, dim3(rtx_div_up(T, BS)), dim3(BS), 0, st, s->b().codes.as<uint32_t>(),
-                           s->b().keys_alt.as<uint32_t>(), s->b().vals_alt.as<uint32_t>(), (int)T);
+                           W.keys_alt, W.vals_alt, (int)T);
     }
     {
-        HierArgs A{s->b().codes.as<uint32_t>(), s->b().keys_alt.as<uint32_t>(), s->b().vals_alt.as<uint32_t>(),
-                   s->b().bstart.as<uint32_t>(), s->b().bend.as<uint32_t>(), s->nodes(), s->b().parent.as<uint32_t>(),
-                   s->b().tri.as<float4>(), levels, s->b().fat.as<uint32_t>(), T, min_leaf};
+        HierArgs A{s->b().codes.as<uint32_t>(), W.keys_alt, W.vals_alt, W.bstart, W.bend, s->nodes(),
+                   s->b().parent.as<uint32_t>(), s->b().tri.as<float4>(), levels, s->b().fat.as<uint32_t>(), T,
+                   min_leaf};
         const int dtop = depth < HIER_TOP ? depth : HIER_TOP;
         {
-            ProfScope ps(s->ctx, st, "lbvh_hier_top");
-            hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, s->b().hier_status.as<uint32_t>(), hier_tiles,
-                               dtop, depth);
+            ProfScope ps(pc, st, "lbvh_hier_top");
+            hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, W.hier_status, hier_tiles, dtop, depth);
         }
         const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
         for (int d = dtop + 1; d <= depth; ++d) {
-            ProfScope ps(s->ctx, st, "lbvh_hier_level");
+            ProfScope ps(pc, st, "lbvh_hier_level");
             uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;
             if (maxn > 2ull * T) maxn = 2ull * T;
             uint32_t grid = rtx_div_up(maxn, BS);
             hipLaunchKernelGGL(k_hier_level, dim3(grid > gmax ? gmax : grid), dim3(BS), 0, st, A,
-                               s->b().hier_status.as<uint32_t>() + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d,
-                               d == depth ? 1 : 0);
+                               W.hier_status + (size_t)d * hier_tiles, tickets + TK_LEVEL + d, d, d == depth ? 1 : 0);
         }
     }
-    fat_leaf_bounds(s, st);
-    refit_internal(s, st, depth);
+    fat_leaf_bounds(s, st, pc);
+    refit_internal(s, st, depth, pc);
+}
+
+// Sorted codes and permutation left in the scratch's alternates by an odd
+// pass count go to the buffer set: by a handle swap when the set owns the
+// alternates, by a copy otherwise.
+static void settle_sorted(rtx_scene* s, bool in_alt) {
+    if (!in_alt) return;
+    std::swap(s->b().codes, s->b().keys_alt);
+    std::swap(s->b().perm, s->b().vals_alt);
 }
 
 int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
@@ -875,8 +900,8 @@ int32_t rtx_lbvh_build(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
     if ((e = lbvh_alloc(s, s->T, depth))) return e;
     {
         ProfScope whole(s->ctx, st, "LBVH_BUILD");
-        build_sort_phase(s, depth, st);
-        build_tree_phase(s, depth, min_leaf, st);
+        settle_sorted(s, build_sort_phase(s, depth, st, scratch_of(s->b()), s->ctx));
+        build_tree_phase(s, depth, min_leaf, st, scratch_of(s->b()), s->ctx);
     }
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
@@ -892,10 +917,10 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
     int32_t e = build_params(s, p, &depth, &min_leaf);
     if (e) return e;
     if ((e = lbvh_alloc(s, s->T, depth))) return e;
-    build_sort_phase(s, depth, sort_stream);
+    settle_sorted(s, build_sort_phase(s, depth, sort_stream, scratch_of(s->b()), s->ctx));
     RTX_CHECK_HIP(hipEventRecord(sorted, sort_stream));
     RTX_CHECK_HIP(hipStreamWaitEvent(tree_stream, sorted, 0));
-    build_tree_phase(s, depth, min_leaf, tree_stream);
+    build_tree_phase(s, depth, min_leaf, tree_stream, scratch_of(s->b()), s->ctx);
     RTX_CHECK_HIP(hipGetLastError());
     s->built = true;
     s->sah = false;
@@ -905,6 +930,109 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
 }
 
 int32_t rtx_lbvh_reserve(rtx_scene* s, int depth) { return lbvh_alloc(s, s->T, depth); }
+
+// ------------------------------------------------ allocation-free build --
+// rtx_lbvh_build_noalloc runs the same kernels as rtx_lbvh_build with the
+// build scratch carved from `scratch`, a fixed array that every GPU holds for
+// the life of the process, so that it calls no allocator.  Sized for
+// RTX_LBVH_NOALLOC_MAX_TRIS triangles at any depth <= 32 (4 sort passes, 33
+// levels).
+namespace {
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_T = RTX_LBVH_NOALLOC_MAX_TRIS;
+constexpr size_t NA_SORT_TILES = (NA_T + SORT_TILE - 1) / SORT_TILE;
+constexpr size_t NA_HIER_TILES = (NA_T + BS - 1) / BS;
+constexpr size_t NA_BYTES = 2 * al256(2 * NA_T * 4)                  // bstart, bend
+                          + 2 * al256(NA_T * 4)                      // keys_alt, vals_alt
+                          + al256(NA_T * 16)                         // centroid
+                          + al256((size_t)BOUNDS_GRID * 6 * 4)       // partials
+                          + al256(4 * 256 * 4)                       // sort_hist
+                          + al256(4 * NA_SORT_TILES * 256 * 4)       // sort_status
+                          + al256((size_t)TK_COUNT * 4)              // tickets
+                          + al256(33 * NA_HIER_TILES * 4);           // hier_status
+}  // namespace
+
+static_assert(NA_BYTES == RTX_LBVH_NOALLOC_SCRATCH_BYTES, "include/rtx.h states the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Once per context (one device, current when rtx_init calls this): the
+// device address of this device's copy of `scratch`.  Host code uses only
+// this address, never the array's name.
+int32_t rtx_lbvh_scratch_init(rtx_ctx* c) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        rtx_set_error(std::string("rtx_init: hipGetSymbolAddress(build scratch): ") + hipGetErrorString(e));
+        return RTX_EDEVICE;
+    }
+    c->build_scratch = ptr;
+    return RTX_OK;
+}
+
+static BuildScratch carve_scratch(void* base, uint32_t T) {
+    char* p = static_cast<char*>(base);
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += al256(bytes);
+        return q;
+    };
+    const size_t sort_tiles = rtx_div_up(T, SORT_TILE), hier_tiles = rtx_div_up(T, BS);
+    BuildScratch W;
+    W.bstart = reinterpret_cast<uint32_t*>(take(2 * (size_t)T * 4));
+    W.bend = reinterpret_cast<uint32_t*>(take(2 * (size_t)T * 4));
+    W.keys_alt = reinterpret_cast<uint32_t*>(take((size_t)T * 4));
+    W.vals_alt = reinterpret_cast<uint32_t*>(take((size_t)T * 4));
+    W.centroid = reinterpret_cast<float4*>(take((size_t)T * 16));
+    W.partials = reinterpret_cast<float*>(take((size_t)BOUNDS_GRID * 6 * 4));
+    W.sort_hist = reinterpret_cast<uint32_t*>(take(4 * 256 * 4));
+    W.sort_status = reinterpret_cast<uint32_t*>(take(4 * sort_tiles * 256 * 4));
+    W.tickets = reinterpret_cast<uint32_t*>(take((size_t)TK_COUNT * 4));
+    W.hier_status = reinterpret_cast<uint32_t*>(take(33 * hier_tiles * 4));
+    return W;
+}
+
+namespace {
+__global__ void k_copy_pairs(const uint32_t* __restrict__ ka, const uint32_t* __restrict__ va, uint32_t* __restrict__ kb,
+                             uint32_t* __restrict__ vb, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        kb[i] = ka[i];
+        vb[i] = va[i];
+    }
+}
+}  // namespace
+
+int32_t rtx_lbvh_build_noalloc(rtx_scene* s, const rtx_lbvh_params* p, void* stream) {
+    int depth = 30, min_leaf = 1;
+    int32_t e = build_params(s, p, &depth, &min_leaf);
+    if (e) return e;
+    RTX_REQUIRE(s->T <= RTX_LBVH_NOALLOC_MAX_TRIS, RTX_EINVAL,
+                "rtx_lbvh_build_noalloc: more than RTX_LBVH_NOALLOC_MAX_TRIS (1048576) triangles");
+    RTX_REQUIRE(s->ctx->build_scratch, RTX_ESTATE, "rtx_lbvh_build_noalloc: context has no build scratch");
+    // outputs: the current buffer set, reserved beforehand (rtx_lbvh_build)
+    const size_t T = s->T, ncap = 2 * T;
+    BuildSet& b = s->b();
+    RTX_REQUIRE(b.node.bytes >= ncap * 32 + 64 && b.delta.bytes >= ncap * 16 && b.parent.bytes >= ncap * 4 &&
+                    b.tri.bytes >= T * 48 && b.codes.bytes >= T * 4 && b.perm.bytes >= T * 4 && b.gbox.bytes >= 32 &&
+                    b.levels.bytes >= LV_WORDS * 4 && b.fat.bytes >= (T / (FAT_LEAF + 1) + 1) * 4,
+                RTX_ESTATE,
+                "rtx_lbvh_build_noalloc: the scene's tree buffers are not reserved for its triangle count "
+                "(run rtx_lbvh_build once first)");
+    hipStream_t st = s->ctx->pick(stream);
+    const BuildScratch W = carve_scratch(s->ctx->build_scratch, s->T);
+    if (build_sort_phase(s, depth, st, W, nullptr)) {
+        uint32_t g = rtx_div_up(T, BS);
+        hipLaunchKernelGGL(k_copy_pairs, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, W.keys_alt, W.vals_alt,
+                           b.codes.as<uint32_t>(), b.perm.as<uint32_t>(), (uint32_t)T);
+    }
+    build_tree_phase(s, depth, min_leaf, st, W, nullptr);
+    RTX_CHECK_HIP(hipGetLastError());
+    s->built = true;
+    s->sah = false;
+    s->depth = depth;
+    s->min_leaf = min_leaf;
+    return RTX_OK;
+}
 
 // Scene::create_aggregate_bvh with the SAH accelerator (BVH::construct_and_map,
 // src/rt/bvh.cpp:126-175): the tree is built on the host from the scene's
@@ -1004,8 +1132,8 @@ int32_t rtx_lbvh_refit(rtx_scene* s, void* stream) {
         hipLaunchKernelGGL(k_leaf_bounds, dim3(g > 4096 ? 4096 : g), dim3(BS), 0, st, s->b().tri.as<float4>(),
                            s->nodes(), s->b().levels.as<uint32_t>());
     }
-    fat_leaf_bounds(s, st);
-    refit_internal(s, st, s->depth);
+    fat_leaf_bounds(s, st, s->ctx);
+    refit_internal(s, st, s->depth, s->ctx);
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }

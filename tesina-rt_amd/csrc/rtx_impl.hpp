@@ -81,6 +81,7 @@ struct rtx_ctx {
     int cus = 256;
     uint64_t bytes_in_use = 0;
     bool profiling = false;
+    void* build_scratch = nullptr;  // this device's copy of the allocation-free build's scratch array
     std::vector<KernelTiming> pending;
     std::vector<hipEvent_t> event_pool;
     struct Acc {
@@ -341,6 +342,9 @@ int32_t rtx_lbvh_build_split(rtx_scene* s, const rtx_lbvh_params* p, hipStream_t
                              hipStream_t tree_stream);
 // Allocates set s->cur for builds of `depth` levels (host side).
 int32_t rtx_lbvh_reserve(rtx_scene* s, int depth);
+// Looks up this device's copy of the allocation-free build's scratch array
+// (lbvh.hip); called once by rtx_init with the device current.
+int32_t rtx_lbvh_scratch_init(rtx_ctx* c);
 int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_hits* hits, uint64_t count,
                                         uint64_t* dev_shadow_count, hipStream_t st);
 // The CPU SAH builder (tesina-rt_amd/rt/sah.cpp).

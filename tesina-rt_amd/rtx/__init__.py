@@ -113,7 +113,7 @@ SYMBOLS = [
     "rtx_scene_set_textures", "rtx_scene_set_cubemap",
     "rtx_scene_triangle_count", "rtx_scene_displace_waves", "rtx_scene_set_waves", "rtx_scene_set_clock",
     "rtx_scene_get_clock", "rtx_scene_animate",
-    "rtx_lbvh_build", "rtx_lbvh_refit", "rtx_lbvh_info", "rtx_sah_build", "rtx_sah_build_host",
+    "rtx_lbvh_build", "rtx_lbvh_build_noalloc", "rtx_lbvh_refit", "rtx_lbvh_info", "rtx_sah_build", "rtx_sah_build_host",
     "rtx_read_codes", "rtx_read_perm", "rtx_read_indices", "rtx_read_matmap", "rtx_read_orig_ids",
     "rtx_read_global_bbox", "rtx_read_vertices", "rtx_read_nodes_ref",
     "rtx_rays_create", "rtx_rays_destroy", "rtx_rays_capacity", "rtx_rays_count", "rtx_rays_read", "rtx_rays_write",
@@ -202,6 +202,7 @@ def lib() -> C.CDLL:
         "rtx_pipeline_fb": (P, [P, U64]),
         "rtx_pipeline_stats": (I32, [P, U64, P]),
         "rtx_lbvh_build": (I32, [P, P, P]),
+        "rtx_lbvh_build_noalloc": (I32, [P, P, P]),
         "rtx_lbvh_refit": (I32, [P, P]),
         "rtx_lbvh_info": (I32, [P, P, P]),
         "rtx_read_codes": (I32, [P, P, U32]),
@@ -520,6 +521,12 @@ class Scene:
     def build(self, depth: int = 30, min_leaf: int = 1, stream=None):
         p = LbvhParams(depth, min_leaf)
         _check(lib().rtx_lbvh_build(self.h, C.byref(p), stream), "rtx_lbvh_build")
+
+    def build_noalloc(self, depth: int = 30, min_leaf: int = 1, stream=None):
+        """The same build with no allocation: scratch in the library's fixed
+        array, outputs in tree buffers reserved by an earlier build."""
+        p = LbvhParams(depth, min_leaf)
+        _check(lib().rtx_lbvh_build_noalloc(self.h, C.byref(p), stream), "rtx_lbvh_build_noalloc")
 
     def build_sah(self, stream=None):
         """The CPU SAH BVH (BVH::construct_and_map, src/rt/bvh.cpp:126-175)."""
