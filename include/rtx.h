@@ -471,6 +471,59 @@ int32_t rtx_renderer_set_pipelined(rtx_renderer* r, int32_t enable);
  * created with (the reference's RendererConfig::max_bounces, per frame). */
 int32_t rtx_renderer_set_bounces(rtx_renderer* r, int32_t bounces);
 
+/* ----------------------------------------------------------- wavefront -- */
+/* Frames of many scenes rendered together (no counterpart in the reference,
+ * which renders one frame of one scene at a time: Renderer::
+ * render_to_framebuffer, src/rt/renderer.cpp:113-214).  Each step advances
+ * every admitted frame by one bounce of that sequence -- primary rays (new
+ * frames), closest hit, shadow rays, shading, secondary rays, resolve (frames
+ * that end) -- with one launch per stage over all frames' rays: persistent
+ * waves take 64-ray chunks of one frame each from a queue, frames grouped
+ * by scene.  Each frame's accumulators, RGBA8 and counts equal its
+ * stand-alone frame's (rtx_render_frame with rebuild = 0) bit for bit.
+ * A frame's rays, hits and framebuffer come from a device arena allocated
+ * once at creation: admission charges rtx_wavefront_frame_bytes() against
+ * it, and frames never allocate.  One thread drives a wavefront. */
+typedef struct rtx_wavefront rtx_wavefront;
+#define RTX_WF_MAX_FRAMES 256
+typedef struct rtx_wavefront_params {
+    uint64_t arena_bytes;  /* device memory for frames in flight; 0 = 8 GiB */
+    int32_t max_frames;    /* frames admitted and not released at once, 1..RTX_WF_MAX_FRAMES */
+    int32_t quad_size;     /* prim_ray_quad_size, 0 = 32 */
+} rtx_wavefront_params;
+int32_t rtx_wavefront_create(rtx_ctx* ctx, const rtx_wavefront_params* p, rtx_wavefront** out);
+/* Waits for enqueued steps, then frees. */
+int32_t rtx_wavefront_destroy(rtx_wavefront* wf);
+/* Arena bytes of a frame of width x height pixels and `bounces` bounces
+ * (two ray buffers sized like the stand-alone renderer's largest bundles,
+ * hit records, framebuffer, counters); 0 for bad sizes. */
+uint64_t rtx_wavefront_frame_bytes(int32_t width, int32_t height, int32_t bounces, uint64_t ray_capacity);
+/* Admits a frame of a built scene (one sample per pixel).  ray_capacity:
+ * secondary rays per bounce at most, as rtx_render_params::ray_capacity (0 =
+ * the full (pixels << bounce) + 2).  RTX_ENOMEM: no room in the arena now,
+ * or max_frames unreleased frames (release frames and retry); RTX_EINVAL:
+ * larger than the whole arena.  No GPU work. */
+int32_t rtx_wavefront_add(rtx_wavefront* wf, rtx_scene* s, const rtx_camera* cam, int32_t width, int32_t height,
+                          int32_t bounces, uint64_t ray_capacity, uint64_t tag, int32_t* frame);
+/* Enqueues one step on `stream` (asynchronous; waits only when the step
+ * table it reuses is still in flight).  *pending = frames that still need
+ * steps. */
+int32_t rtx_wavefront_step(rtx_wavefront* wf, void* stream, int32_t* pending);
+/* Steps until every admitted frame has enqueued its last bounce (asynchronous). */
+int32_t rtx_wavefront_run(rtx_wavefront* wf, void* stream);
+/* Waits for every enqueued step (synchronizes). */
+int32_t rtx_wavefront_sync(rtx_wavefront* wf);
+/* Frames completed on the device since the last poll (non-blocking). */
+int32_t rtx_wavefront_poll(rtx_wavefront* wf, int32_t* frames, uint64_t* tags, uint32_t cap, uint32_t* n);
+/* Counts (primary, secondary, shadow rays, bounces run) and images of a
+ * frame whose last bounce is enqueued; they wait for it. */
+int32_t rtx_wavefront_frame_stats(rtx_wavefront* wf, int32_t frame, rtx_frame_stats* stats);
+int32_t rtx_wavefront_read_accum(rtx_wavefront* wf, int32_t frame, uint32_t* rgb3);
+int32_t rtx_wavefront_read_rgba8(rtx_wavefront* wf, int32_t frame, uint8_t* rgba);
+/* Returns a complete frame's memory to the arena. */
+int32_t rtx_wavefront_release(rtx_wavefront* wf, int32_t frame);
+int32_t rtx_wavefront_memory(rtx_wavefront* wf, uint64_t* used, uint64_t* capacity);
+
 /* ------------------------------------------------------------- serving -- */
 /* Request server over resident scenes (no counterpart in the reference,
  * whose apps render one scene from one thread).  A dispatcher thread of the

@@ -354,3 +354,82 @@ int32_t rtx_raygen_secondary_impl(rtx_scene* s, rtx_rays* in, rtx_hits* hits, rt
                                   uint64_t* dev_total_acc, hipStream_t st);
 
 static inline uint32_t rtx_div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ------------------------------------------------------------ wavefront --
+// One frame's share of a wavefront step: every frame in flight advances one
+// bounce per step, and each stage of the step is one launch over the rays of
+// every frame.  The step's table of segments lives in device memory; a wave
+// (or block) always works inside one segment, so the scene's pointers are
+// uniform across it.
+constexpr int RTX_WF_MAX_SEGS = 256;
+struct WfSeg {
+    // scene (resident, built)
+    const float4* node;
+    const float4* delta;
+    const float4* tri;
+    const float4* nrm;
+    const float2* uv;
+    const rtx_material* mats;
+    const int32_t* matmap;
+    const rtx_lights* lights;
+    const uchar4* tex;
+    const TexEnt* tex_tab;
+    int32_t ntex, cube;
+    // rays of this bounce (count: [0] rays, [1] scan total, [2] first dropped
+    // offset, as rtx_rays::count) and of the next one
+    float4* ori4;
+    float4* dir4;
+    int32_t* pixel;
+    float* contrib;
+    uint32_t* count;
+    float4* o_ori4;
+    float4* o_dir4;
+    int32_t* o_pixel;
+    float* o_contrib;
+    uint32_t* o_count;
+    uint32_t* status;  // look-back words of this step's secondary generation (one per 256 input rays)
+    // hit records of this bounce's rays, and the list of rays with a hit
+    uint32_t* flags;
+    float* t;
+    int32_t* id;
+    float2* bary;
+    float4* n4;
+    float4* p4;
+    float2* uv2;
+    uint32_t* list;
+    uint32_t* list_count;
+    // framebuffer and the frame's device counters ([0] secondary rays,
+    // [1] shadow rays, [2] bounces run)
+    uint32_t* accum;
+    uchar4* rgba;
+    unsigned long long* stats;
+    rtx_camera cam;
+    int32_t w, h, quad;
+    uint32_t o_cap;    // capacity of the next bounce's rays (the renderer's (P << (b + 1)) + 2)
+    uint32_t primary;  // bounce 0: primary rays and the framebuffer clear
+    uint32_t gen;      // secondary rays are generated this step
+    uint32_t resolve;  // last bounce: the framebuffer is resolved to RGBA8
+    uint32_t fin;      // resolving frames: slot of its counters in the step's output
+};
+// Counters of one step in flight (zeroed by the step's first kernel).
+enum { WF_Q_TRACE = 0, WF_Q_SHADOW, WF_Q_SEC, WF_Q_COUNT = 4 };
+
+// Launchers of the segmented stage kernels (trace.hip, shade.hip,
+// secondary.hip).  `any_*` says whether some segment needs the stage.
+void rtx_wf_init(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, hipStream_t st);
+void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, hipStream_t st);
+void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
+                  hipStream_t st);
+void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
+                   hipStream_t st);
+void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st);
+void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st);
+// Also copies the counters of each resolving frame to fin_out[S.fin] (4 x u64).
+void rtx_wf_resolve(const WfSeg* segs, uint32_t nseg, uint64_t pixels, unsigned long long* fin_out, int cus,
+                    hipStream_t st);
+
+// Wavefront internals for the request server (wavefront.hip).
+int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pending, std::vector<int>* finishing);
+uint64_t rtx_wf_arena_size(const rtx_wavefront* wf);
+int64_t rtx_wf_steps_in_flight(const rtx_wavefront* wf);
+const void* rtx_wf_accum_dev(rtx_wavefront* wf, int32_t frame);

@@ -12,6 +12,7 @@
 #include "devmath.hpp"
 #include "lookback.hpp"
 #include "rtx_impl.hpp"
+#include "wavefront.hpp"
 
 namespace {
 
@@ -26,23 +27,18 @@ __global__ void k_sec_init(uint32_t* status, uint32_t nwords, uint32_t* in_cnt, 
     }
 }
 
-__global__ __launch_bounds__(SB) void k_secondary(
-    const float4* __restrict__ dir_in, const int32_t* __restrict__ pix_in, const float* __restrict__ con_in,
-    const uint32_t* __restrict__ flags, const int32_t* __restrict__ hid, const float4* __restrict__ hn,
-    const float4* __restrict__ hp, const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap,
-    uint32_t* in_cnt, uint64_t hcount, float4* __restrict__ ori_out, float4* __restrict__ dir_out,
-    int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt, uint32_t* status, uint32_t cap) {
-    __shared__ uint32_t s_tile, s_prefix;
-    __shared__ uint32_t s_w[SB / RTX_WAVE];
-    const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)in_cnt[0] : hcount;
-    const uint32_t ntiles = (uint32_t)((n + SB - 1) / SB);
-    // blocks past the last tile leave before taking a ticket, so the first
-    // ntiles blocks to arrive take exactly the tickets [0, ntiles)
-    if (blockIdx.x >= ntiles) return;  // (n == 0: out_cnt[1] stays 0 from k_sec_init)
-    if (threadIdx.x == 0) s_tile = atomicAdd(&in_cnt[3], 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= ntiles) return;
+// One 256-ray tile of n input rays (tiles taken in order): children per
+// parent, the tile's output offset by look-back, then the children written.
+// Every thread of the block calls it (two block barriers inside); s_w and
+// s_prefix are the block's shared scratch.
+__device__ __forceinline__ void sec_tile(
+    uint32_t tile, uint32_t ntiles, uint64_t n, const float4* __restrict__ dir_in,
+    const int32_t* __restrict__ pix_in, const float* __restrict__ con_in, const uint32_t* __restrict__ flags,
+    const int32_t* __restrict__ hid, const float4* __restrict__ hn, const float4* __restrict__ hp,
+    const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap, float4* __restrict__ ori_out,
+    float4* __restrict__ dir_out, int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt,
+    uint32_t* status, uint32_t cap, uint32_t* s_w, uint32_t* s_prefix_p) {
+    uint32_t& s_prefix = *s_prefix_p;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t k = (uint64_t)tile * SB + tid;
     uint32_t c = 0;
@@ -128,6 +124,52 @@ __global__ __launch_bounds__(SB) void k_secondary(
     }
 }
 
+__global__ __launch_bounds__(SB) void k_secondary(
+    const float4* __restrict__ dir_in, const int32_t* __restrict__ pix_in, const float* __restrict__ con_in,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ hid, const float4* __restrict__ hn,
+    const float4* __restrict__ hp, const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap,
+    uint32_t* in_cnt, uint64_t hcount, float4* __restrict__ ori_out, float4* __restrict__ dir_out,
+    int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt, uint32_t* status, uint32_t cap) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[SB / RTX_WAVE];
+    const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)in_cnt[0] : hcount;
+    const uint32_t ntiles = (uint32_t)((n + SB - 1) / SB);
+    // blocks past the last tile leave before taking a ticket, so the first
+    // ntiles blocks to arrive take exactly the tickets [0, ntiles)
+    if (blockIdx.x >= ntiles) return;  // (n == 0: out_cnt[1] stays 0 from k_sec_init)
+    if (threadIdx.x == 0) s_tile = atomicAdd(&in_cnt[3], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    sec_tile(tile, ntiles, n, dir_in, pix_in, con_in, flags, hid, hn, hp, mats, matmap, ori_out, dir_out, pix_out,
+             con_out, out_cnt, status, cap, s_w, &s_prefix);
+}
+
+// Secondary rays of every frame that generates this step: persistent blocks
+// take tickets over the concatenation of the frames' tiles, so each frame's
+// tiles are taken in order and a tile's look-back only waits on tiles
+// already taken (by running blocks).  Look-back words were cleared by this
+// step's trace kernel; scan totals and drop offsets by its first kernel.
+__global__ __launch_bounds__(SB) void k_wf_secondary(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                     uint32_t* __restrict__ ticket) {
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[SB / RTX_WAVE];
+    const uint32_t total = wf_chunks<WF_GEN_TILES>(segs, nseg, 1u, s_end);
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t g = s_tile;
+        if (g >= total) break;
+        const uint32_t sg = wf_find(s_end, nseg, g);
+        const WfSeg& S = segs[sg];
+        const uint64_t n = S.count[0];
+        sec_tile(g - wf_first(s_end, sg), (uint32_t)((n + SB - 1) / SB), n, S.dir4, S.pixel, S.contrib, S.flags,
+                 S.id, S.n4, S.p4, S.mats, S.matmap, S.o_ori4, S.o_dir4, S.o_pixel, S.o_contrib, S.o_count, S.status,
+                 S.o_cap, s_w, &s_prefix);
+    }
+}
+
 // Output count = rays actually written (a prefix): min(scan total, first
 // dropped offset).  Optionally accumulates the frame's secondary-ray count.
 __global__ void k_sec_finalize(uint32_t* out_cnt, unsigned long long* acc) {
@@ -186,4 +228,8 @@ int32_t rtx_raygen_secondary(rtx_scene* s, rtx_rays* rays_in, rtx_hits* hits, rt
         *count_out = c;
     }
     return RTX_OK;
+}
+
+void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_secondary, dim3((uint32_t)cus * 8), dim3(SB), 0, st, segs, nseg, ctr + WF_Q_SEC);
 }

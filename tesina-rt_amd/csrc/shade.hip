@@ -8,6 +8,7 @@
 // (src/kernel/ray-shader.cl:415-457) is not needed for that.
 #include "devmath.hpp"
 #include "rtx_impl.hpp"
+#include "wavefront.hpp"
 
 namespace {
 
@@ -100,6 +101,68 @@ __device__ __forceinline__ void add_pixel(uint32_t* __restrict__ accum, uint32_t
     }
 }
 
+// shade_sample for ray r: its colour times its contribution added to its
+// pixel's accumulators.
+__device__ __forceinline__ void shade_ray(uint64_t r, const float4* __restrict__ dir4, const int32_t* __restrict__ pixel,
+                                          const float* __restrict__ contrib, const uint32_t* __restrict__ flags,
+                                          const int32_t* __restrict__ hid, const float4* __restrict__ hn,
+                                          const float4* __restrict__ hp, const float2* __restrict__ huv,
+                                          const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap,
+                                          const rtx_lights* __restrict__ lights, int nl, v3 amb, const Images& im,
+                                          uint32_t* __restrict__ accum) {
+    uint32_t fl = flags[r];
+    if (!(fl & 1u)) {  // a miss adds the cubemap's colour, or nothing without one
+        if (!im.cube) return;
+        const v3 rgb = cubemap_rgb(im, xyz(dir4[r]));
+        const float c = contrib[r];
+        const float ch[3] = {c * rgb.x, c * rgb.y, c * rgb.z};
+        add_pixel(accum, (uint32_t)pixel[r], ch);
+        return;
+    }
+    const rtx_material m = mats[matmap[hid[r]]];
+    v3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    if (m.texture >= 0) {
+        const float2 uv = huv[r];
+        diffuse = m.texture < im.count ? sample_linear(im, m.texture, uv.x, uv.y, true) : mk3(0.f, 0.f, 0.f);
+    }
+    v3 d = xyz(dir4[r]);
+    v3 nn = xyz(hn[r]);
+    v3 p = xyz(hp[r]);
+    v3 val = amb * diffuse;
+    for (int l = 0; l < nl; ++l) {
+        const rtx_light& L = lights->l[l];
+        v3 Ld;
+        float LD;
+        if (L.type == RTX_LIGHT_DIR) {
+            Ld = mk3(L.dir[0], L.dir[1], L.dir[2]);
+            LD = 1.f;
+        } else if (L.type == RTX_LIGHT_SPOT) {
+            Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
+            LD = dot(Ld, mk3(L.dir[0], L.dir[1], L.dir[2])) - L.angle;
+            LD /= 1.f - L.angle;
+            LD = clamp01(LD);
+        } else {
+            Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
+            LD = 1.f;
+        }
+        v3 rgb = mk3(L.color[0], L.color[1], L.color[2]) * LD;
+        if (fl & (1u << (8 + l))) continue;
+        v3 rr = d - 2.f * nn * dot(d, nn);
+        float cosL = clamp01(dot(nn, -Ld));
+        cosL = cosL * cosL;
+        float spec = clamp01(dot(rr, -Ld));
+        float s2 = spec * spec, s4 = s2 * s2;
+        spec = s4 * s4;
+        spec *= cosL * m.shininess;
+        val = val + rgb * diffuse * cosL;
+        val = val + rgb * spec;
+    }
+    val = val * (1.f - m.reflectiveness);
+    const float c = contrib[r];
+    const float ch[3] = {val.x * c, val.y * c, val.z * c};
+    add_pixel(accum, (uint32_t)pixel[r], ch);
+}
+
 __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, const int32_t* __restrict__ pixel,
                                                const float* __restrict__ contrib, const uint32_t* __restrict__ flags,
                                                const int32_t* __restrict__ hid, const float4* __restrict__ hn,
@@ -112,59 +175,8 @@ __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, 
     const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
     const int nl = lights->count;
     const v3 amb = mk3(lights->ambient[0], lights->ambient[1], lights->ambient[2]);
-    for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
-        uint32_t fl = flags[r];
-        if (!(fl & 1u)) {  // a miss adds the cubemap's colour, or nothing without one
-            if (!im.cube) continue;
-            const v3 rgb = cubemap_rgb(im, xyz(dir4[r]));
-            const float c = contrib[r];
-            const float ch[3] = {c * rgb.x, c * rgb.y, c * rgb.z};
-            add_pixel(accum, (uint32_t)pixel[r], ch);
-            continue;
-        }
-        const rtx_material m = mats[matmap[hid[r]]];
-        v3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-        if (m.texture >= 0) {
-            const float2 uv = huv[r];
-            diffuse = m.texture < im.count ? sample_linear(im, m.texture, uv.x, uv.y, true) : mk3(0.f, 0.f, 0.f);
-        }
-        v3 d = xyz(dir4[r]);
-        v3 nn = xyz(hn[r]);
-        v3 p = xyz(hp[r]);
-        v3 val = amb * diffuse;
-        for (int l = 0; l < nl; ++l) {
-            const rtx_light& L = lights->l[l];
-            v3 Ld;
-            float LD;
-            if (L.type == RTX_LIGHT_DIR) {
-                Ld = mk3(L.dir[0], L.dir[1], L.dir[2]);
-                LD = 1.f;
-            } else if (L.type == RTX_LIGHT_SPOT) {
-                Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
-                LD = dot(Ld, mk3(L.dir[0], L.dir[1], L.dir[2])) - L.angle;
-                LD /= 1.f - L.angle;
-                LD = clamp01(LD);
-            } else {
-                Ld = normalize(p - mk3(L.pos[0], L.pos[1], L.pos[2]));
-                LD = 1.f;
-            }
-            v3 rgb = mk3(L.color[0], L.color[1], L.color[2]) * LD;
-            if (fl & (1u << (8 + l))) continue;
-            v3 rr = d - 2.f * nn * dot(d, nn);
-            float cosL = clamp01(dot(nn, -Ld));
-            cosL = cosL * cosL;
-            float spec = clamp01(dot(rr, -Ld));
-            float s2 = spec * spec, s4 = s2 * s2;
-            spec = s4 * s4;
-            spec *= cosL * m.shininess;
-            val = val + rgb * diffuse * cosL;
-            val = val + rgb * spec;
-        }
-        val = val * (1.f - m.reflectiveness);
-        const float c = contrib[r];
-        const float ch[3] = {val.x * c, val.y * c, val.z * c};
-        add_pixel(accum, (uint32_t)pixel[r], ch);
-    }
+    for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256)
+        shade_ray(r, dir4, pixel, contrib, flags, hid, hn, hp, huv, mats, matmap, lights, nl, amb, im, accum);
 }
 
 __global__ void k_resolve(const uint32_t* __restrict__ accum, uchar4* __restrict__ rgba, uint32_t P) {
@@ -179,6 +191,49 @@ __global__ void k_resolve(const uint32_t* __restrict__ accum, uchar4* __restrict
         c[q] = (uint8_t)rintf(f);
     }
     rgba[i] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
+// Shading of every frame's rays of this bounce: blocks take 256-ray chunks
+// in a fixed stride.
+__global__ __launch_bounds__(256) void k_wf_shade(const WfSeg* __restrict__ segs, uint32_t nseg) {
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    const uint32_t total = wf_chunks<WF_RAYS>(segs, nseg, 256u, s_end);
+    for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+        const uint32_t sg = wf_find(s_end, nseg, c);
+        const WfSeg& S = segs[sg];
+        const uint64_t r = (uint64_t)(c - wf_first(s_end, sg)) * 256u + threadIdx.x;
+        if (r >= (uint64_t)S.count[0]) continue;
+        const Images im{S.tex, S.tex_tab, S.tex ? S.ntex : 0, S.tex && S.cube ? 1 : 0};
+        const v3 amb = mk3(S.lights->ambient[0], S.lights->ambient[1], S.lights->ambient[2]);
+        shade_ray(r, S.dir4, S.pixel, S.contrib, S.flags, S.id, S.n4, S.p4, S.uv2, S.mats, S.matmap, S.lights,
+                  S.lights->count, amb, im, S.accum);
+    }
+}
+
+// RGBA8 of the frames whose last bounce this step shaded.
+__global__ __launch_bounds__(256) void k_wf_resolve(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                    unsigned long long* __restrict__ fin_out) {
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nseg; i += 256)
+            if (segs[i].resolve)
+                for (int q = 0; q < 3; ++q) fin_out[4 * (size_t)segs[i].fin + q] = segs[i].stats[q];
+    const uint32_t total = wf_chunks<WF_PIXELS_DONE>(segs, nseg, 256u, s_end);
+    const float inv = 1.f / 1e6f;
+    for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+        const uint32_t sg = wf_find(s_end, nseg, c);
+        const WfSeg& S = segs[sg];
+        const uint32_t i = (c - wf_first(s_end, sg)) * 256u + threadIdx.x;
+        if (i >= (uint32_t)S.w * (uint32_t)S.h) continue;
+        uint8_t ch[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            float f = (float)S.accum[3 * (size_t)i + q] * inv;
+            f = clamp01(f) * 255.f;
+            ch[q] = (uint8_t)rintf(f);
+        }
+        S.rgba[i] = make_uchar4(ch[0], ch[1], ch[2], 255);
+    }
 }
 
 }  // namespace
@@ -260,4 +315,16 @@ int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint
                        rays->count.as<uint32_t>(), count, fb->accum.as<uint32_t>());
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
+}
+
+// ------------------------------------------------- wavefront launchers --
+void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_shade, dim3((uint32_t)cus * 16), dim3(256), 0, st, segs, nseg);
+}
+
+void rtx_wf_resolve(const WfSeg* segs, uint32_t nseg, uint64_t pixels, unsigned long long* fin_out, int cus,
+                    hipStream_t st) {
+    uint64_t g = (pixels + 255) / 256;
+    if (g > (uint64_t)cus * 16) g = (uint64_t)cus * 16;
+    hipLaunchKernelGGL(k_wf_resolve, dim3((uint32_t)(g ? g : 1)), dim3(256), 0, st, segs, nseg, fin_out);
 }

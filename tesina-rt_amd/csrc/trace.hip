@@ -17,6 +17,7 @@
 
 #include "devmath.hpp"
 #include "rtx_impl.hpp"
+#include "wavefront.hpp"
 
 namespace {
 
@@ -46,23 +47,14 @@ __device__ __forceinline__ XcdSpan xcd_span(uint64_t n, uint32_t X, uint32_t TB)
 }
 
 // ------------------------------------------------------------- raygen ----
-__global__ __launch_bounds__(256) void k_raygen_primary(float4* __restrict__ ori4, float4* __restrict__ dir4,
-                                                        int32_t* __restrict__ pixel, float* __restrict__ contrib,
-                                                        uint32_t* dcount, rtx_camera cam, int width, int height,
-                                                        int quad, const float* __restrict__ ps, uint64_t offset,
-                                                        uint64_t count) {
-    uint64_t k = blockIdx.x * 256ull + threadIdx.x;
-    if (k == 0) *dcount = (uint32_t)count;
-    if (k >= count) return;
-    int gid = (int)(offset + k);
-    int pixels = width * height;
-    int i = gid % pixels;
-    const float* smp = ps + 3 * (gid / pixels);
+// Pixel of sample i (generate_primary_rays, src/kernel/primary-ray-generator.cl:
+// 28-103): quad x quad blocks row by row, then the strip right of them row by
+// row, then the strip above them column by column.
+__device__ __forceinline__ void quad_pixel(int i, int width, int height, int quad, int& x, int& y) {
     int bs = quad, ba = quad * quad;
     int xb = width / bs, yb = height / bs;
     int block_rays = xb * yb * ba;
     int right_rays = (width - xb * bs) * height;
-    int x, y;
     if (i < block_rays) {
         int block = i / ba;
         int y_block = block / xb, x_block = block % xb;
@@ -78,12 +70,34 @@ __global__ __launch_bounds__(256) void k_raygen_primary(float4* __restrict__ ori
         x = ti / slab_h;
         y = sy + ti % slab_h;
     }
-    float xn = (0.5f + (float)x + smp[0]) / (float)width;
-    float yn = (0.5f + (float)y + smp[1]) / (float)height;
+}
+
+// Pinhole direction through pixel (x, y) offset by the sample (s0, s1).
+__device__ __forceinline__ v3 primary_dir(const rtx_camera& cam, int x, int y, int width, int height, float s0,
+                                          float s1) {
+    float xn = (0.5f + (float)x + s0) / (float)width;
+    float yn = (0.5f + (float)y + s1) / (float)height;
     v3 dir = mk3(cam.dir[0], cam.dir[1], cam.dir[2]);
     v3 right = mk3(cam.right[0], cam.right[1], cam.right[2]);
     v3 up = mk3(cam.up[0], cam.up[1], cam.up[2]);
-    v3 d = normalize(dir + right * (xn * 2.f - 1.f) + up * (yn * 2.f - 1.f));
+    return normalize(dir + right * (xn * 2.f - 1.f) + up * (yn * 2.f - 1.f));
+}
+
+__global__ __launch_bounds__(256) void k_raygen_primary(float4* __restrict__ ori4, float4* __restrict__ dir4,
+                                                        int32_t* __restrict__ pixel, float* __restrict__ contrib,
+                                                        uint32_t* dcount, rtx_camera cam, int width, int height,
+                                                        int quad, const float* __restrict__ ps, uint64_t offset,
+                                                        uint64_t count) {
+    uint64_t k = blockIdx.x * 256ull + threadIdx.x;
+    if (k == 0) *dcount = (uint32_t)count;
+    if (k >= count) return;
+    int gid = (int)(offset + k);
+    int pixels = width * height;
+    int i = gid % pixels;
+    const float* smp = ps + 3 * (gid / pixels);
+    int x, y;
+    quad_pixel(i, width, height, quad, x, y);
+    const v3 d = primary_dir(cam, x, y, width, height, smp[0], smp[1]);
     ori4[k] = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], 0.f);
     dir4[k] = make_float4(d.x, d.y, d.z, 1e36f);
     pixel[k] = width * y + x;
@@ -1815,7 +1829,203 @@ uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
     return (uint32_t)(g ? g : X);
 }
 
+// ---------------------------------------------------------- wavefront --
+// Stage kernels of a wavefront step (one launch per stage over the frames of
+// every segment).  Per ray they run the code of the single-frame kernels
+// above, so each frame's records are those of its stand-alone frame.
+
+// First kernel of a step: the step's queue counters; per frame, the ray
+// count of this bounce (primary: the pixel count; later bounces: the rays
+// the last step's generation wrote, min(scan total, first dropped offset),
+// added to the frame's secondary count as k_sec_finalize does), an empty hit
+// list and, when this bounce generates, the next bounce's scan total and
+// drop offset.
+__global__ __launch_bounds__(256) void k_wf_init(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                 uint32_t* __restrict__ ctr) {
+    if (threadIdx.x < WF_Q_COUNT) ctr[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < nseg; i += blockDim.x) {
+        const WfSeg& S = segs[i];
+        if (S.primary) {
+            S.count[0] = (uint32_t)S.w * (uint32_t)S.h;
+            S.stats[0] = 0ull;
+            S.stats[1] = 0ull;
+            S.stats[2] = 0ull;
+        } else {
+            const uint32_t total = S.count[1], drop = S.count[2];
+            const uint32_t n = total < drop ? total : drop;
+            S.count[0] = n;
+            S.stats[0] += n;
+            if (n) S.stats[2] += 1ull;
+        }
+        S.list_count[0] = 0u;
+        if (S.gen) {
+            S.o_count[1] = 0u;
+            S.o_count[2] = 0xffffffffu;
+        }
+    }
+}
+
+// Primary rays of the frames that start this step (one sample per pixel, the
+// default sample table {0, 0, 1}), and their framebuffer clear: ray k is
+// the only ray of its pixel, so it zeroes that pixel's accumulators.
+__global__ __launch_bounds__(256) void k_wf_raygen(const WfSeg* __restrict__ segs, uint32_t nseg) {
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    const uint32_t total = wf_chunks<WF_PIXELS_NEW>(segs, nseg, 256u, s_end);
+    for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+        const uint32_t sg = wf_find(s_end, nseg, c);
+        const WfSeg& S = segs[sg];
+        const uint32_t k = (c - wf_first(s_end, sg)) * 256u + threadIdx.x;
+        if (k >= (uint32_t)S.w * (uint32_t)S.h) continue;
+        int x, y;
+        quad_pixel((int)k, S.w, S.h, S.quad, x, y);
+        const v3 d = primary_dir(S.cam, x, y, S.w, S.h, 0.f, 0.f);
+        S.ori4[k] = make_float4(S.cam.pos[0], S.cam.pos[1], S.cam.pos[2], 0.f);
+        S.dir4[k] = make_float4(d.x, d.y, d.z, 1e36f);
+        const uint32_t px = (uint32_t)(S.w * y + x);
+        S.pixel[k] = (int32_t)px;
+        S.contrib[k] = 1.f;
+        S.accum[3 * (size_t)px] = 0u;
+        S.accum[3 * (size_t)px + 1] = 0u;
+        S.accum[3 * (size_t)px + 2] = 0u;
+    }
+}
+
+// Closest hit over every frame's rays: persistent waves take 64-ray chunks
+// from the step's queue.  A chunk that starts a 256-ray tile also clears
+// that tile's look-back word for this step's secondary generation.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                 uint32_t* __restrict__ queue, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const uint32_t total = wf_chunks<WF_RAYS>(segs, nseg, RTX_WAVE, s_end);
+    const uint32_t lane = threadIdx.x & (RTX_WAVE - 1);
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= total) break;
+        const uint32_t sg = __builtin_amdgcn_readfirstlane(wf_find(s_end, nseg, c));
+        const WfSeg& S = segs[sg];
+        const uint32_t cl = c - wf_first(s_end, sg);
+        if (S.gen && (cl & 3u) == 0u && lane == 0) S.status[cl >> 2] = 0u;
+        const uint64_t rr = (uint64_t)cl * RTX_WAVE + lane;
+        const bool active = rr < (uint64_t)S.count[0];
+        Closest best{0.f, 0.f, 0.f, -1};
+        RayC rc{};
+        if (active) {
+            const float4 o = S.ori4[rr], dd = S.dir4[rr];
+            rc = make_ray(xyz(o), xyz(dd), o.w);
+            best.t = dd.w;
+        }
+        closest_var<VAR>(S.node, S.delta, S.tri, rc, best, stk, coop, active);  // whole wave
+        const bool hit = active && best.id >= 0;
+        const uint64_t hb = ballot(hit);
+        uint32_t wbase = 0;
+        if (hb) {
+            if (lane == (uint32_t)__builtin_ctzll(hb))
+                wbase = atomicAdd(S.list_count, (uint32_t)__builtin_popcountll(hb));
+            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+        }
+        if (!active) continue;
+        if (!hit) {
+            write_miss_record(rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+            continue;
+        }
+        S.list[wbase + lanes_below(hb)] = (uint32_t)rr;
+        write_hit_record(S.tri, S.nrm, S.uv, rc, best, rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+    }
+}
+
+// Shadow rays of every frame's (hit, light) pairs, as k_shadow, from the
+// step's queue of 64-pair chunks.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                  uint32_t* __restrict__ queue, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    const uint32_t total = wf_chunks<WF_PAIRS>(segs, nseg, RTX_WAVE, s_end);
+    const uint32_t lane = threadIdx.x & (RTX_WAVE - 1);
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= total) break;
+        const uint32_t sg = __builtin_amdgcn_readfirstlane(wf_find(s_end, nseg, c));
+        const WfSeg& S = segs[sg];
+        const int nl = S.lights->count;
+        const uint64_t it = (uint64_t)(c - wf_first(s_end, sg)) * RTX_WAVE + lane;
+        const bool valid = it < (uint64_t)S.list_count[0] * (uint64_t)nl;
+        uint32_t rr = 0;
+        int l = 0;
+        RayC rc{};
+        float tmax = 1e37f;
+        bool shadowed = false, trace = false;
+        if (valid) {
+            rr = S.list[it / (uint64_t)nl];
+            l = (int)(it % (uint64_t)nl);
+            const rtx_light& L = S.lights->l[l];
+            const v3 p = xyz(S.p4[rr]);
+            v3 dir = mk3(0.f, 0.f, 0.f);
+            trace = true;
+            if (L.type == RTX_LIGHT_DIR) {
+                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+            } else if (L.type == RTX_LIGHT_SPOT) {
+                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+                if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
+                    shadowed = true;
+                    trace = false;
+                }
+            } else if (L.type == RTX_LIGHT_POINT) {
+                const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+                tmax = __builtin_sqrtf(dot(toL, toL));
+                dir = normalize(toL);
+            } else {
+                shadowed = true;
+                trace = false;
+            }
+            if (trace) rc = make_ray(p, dir, 0.01f);
+        }
+        if (any_var<VAR>(S.node, S.delta, S.tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
+        if (valid) set_shadow_bit(&S.flags[rr], l, shadowed);
+        const uint64_t tb = ballot(trace);
+        if (lane == 0 && tb) atomicAdd(&S.stats[1], (unsigned long long)__builtin_popcountll(tb));
+    }
+}
+
 }  // namespace
+
+// ------------------------------------------------- wavefront launchers --
+void rtx_wf_init(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, hipStream_t st) {
+    hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(256), 0, st, segs, nseg, ctr);
+}
+
+void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, hipStream_t st) {
+    uint64_t g = (pixels + 255) / 256;
+    if (g > (uint64_t)cus * 16) g = (uint64_t)cus * 16;
+    hipLaunchKernelGGL(k_wf_raygen, dim3((uint32_t)(g ? g : 1)), dim3(256), 0, st, segs, nseg);
+}
+
+// Persistent grids: 20 waves per CU, the occupancy the LDS stack allows.
+void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
+                  hipStream_t st) {
+#define RTX_WF_TRACE(V, B)                                                                                  \
+    hipLaunchKernelGGL((k_wf_trace<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
+                       ctr + WF_Q_TRACE, coop)
+    RTX_DISPATCH_TRAVERSAL(RTX_WF_TRACE, var, tb);
+#undef RTX_WF_TRACE
+}
+
+void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
+                   hipStream_t st) {
+#define RTX_WF_SHADOW(V, B)                                                                                  \
+    hipLaunchKernelGGL((k_wf_shadow<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
+                       ctr + WF_Q_SHADOW, coop)
+    RTX_DISPATCH_TRAVERSAL(RTX_WF_SHADOW, var, tb);
+#undef RTX_WF_SHADOW
+}
 
 // ----------------------------------------------------------- bundles -----
 int32_t rtx_rays_create(rtx_ctx* ctx, uint64_t capacity, rtx_rays** out) {

@@ -71,6 +71,10 @@ class Completion(C.Structure):
                 ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("status", C.c_int32), ("worker", C.c_int32)]
 
 
+class WavefrontParams(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("max_frames", C.c_int32), ("quad_size", C.c_int32)]
+
+
 class LbvhParams(C.Structure):
     _fields_ = [("depth", C.c_int32), ("min_leaf", C.c_int32)]
 
@@ -124,6 +128,10 @@ SYMBOLS = [
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
     "rtx_renderer_last_stats", "rtx_renderer_capture", "rtx_renderer_replay", "rtx_renderer_set_animation",
     "rtx_renderer_set_pipelined", "rtx_renderer_set_bounces",
+    "rtx_wavefront_create", "rtx_wavefront_destroy", "rtx_wavefront_frame_bytes", "rtx_wavefront_add",
+    "rtx_wavefront_step", "rtx_wavefront_run", "rtx_wavefront_sync", "rtx_wavefront_poll",
+    "rtx_wavefront_frame_stats", "rtx_wavefront_read_accum", "rtx_wavefront_read_rgba8", "rtx_wavefront_release",
+    "rtx_wavefront_memory",
     "rtx_now_ns", "rtx_server_create", "rtx_server_destroy", "rtx_server_add_scene", "rtx_server_memory",
     "rtx_server_submit", "rtx_server_poll", "rtx_server_drain",
     "rtx_batch_create", "rtx_batch_destroy", "rtx_batch_render", "rtx_batch_capture", "rtx_batch_replay",
@@ -262,6 +270,19 @@ def lib() -> C.CDLL:
         "rtx_instances_set_materials": (I32, [P, P]),
         "rtx_instances_shading_scene": (I32, [P, P]),
         "rtx_render_instances": (I32, [P, P, P, P, P]),
+        "rtx_wavefront_create": (I32, [P, P, C.POINTER(P)]),
+        "rtx_wavefront_destroy": (I32, [P]),
+        "rtx_wavefront_frame_bytes": (U64, [I32, I32, I32, U64]),
+        "rtx_wavefront_add": (I32, [P, P, P, I32, I32, I32, U64, U64, C.POINTER(I32)]),
+        "rtx_wavefront_step": (I32, [P, P, C.POINTER(I32)]),
+        "rtx_wavefront_run": (I32, [P, P]),
+        "rtx_wavefront_sync": (I32, [P]),
+        "rtx_wavefront_poll": (I32, [P, P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "rtx_wavefront_frame_stats": (I32, [P, I32, P]),
+        "rtx_wavefront_read_accum": (I32, [P, I32, P]),
+        "rtx_wavefront_read_rgba8": (I32, [P, I32, P]),
+        "rtx_wavefront_release": (I32, [P, I32]),
+        "rtx_wavefront_memory": (I32, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -866,6 +887,92 @@ class Renderer:
 def now_ns() -> int:
     """The server's monotonic clock (ns)."""
     return int(lib().rtx_now_ns())
+
+
+class Wavefront:
+    """Frames of many scenes rendered together (rtx_wavefront_*): each step
+    advances every admitted frame by one bounce, one launch per stage over
+    all of them.  Frame memory comes from a device arena of `arena_bytes`."""
+
+    def __init__(self, ctx: Context, arena_bytes: int = 0, max_frames: int = 64, quad_size: int = 0):
+        p = WavefrontParams(int(arena_bytes), int(max_frames), int(quad_size))
+        self.h = C.c_void_p()
+        _check(lib().rtx_wavefront_create(ctx.h, C.byref(p), C.byref(self.h)), "rtx_wavefront_create")
+        self.sizes = {}
+
+    @staticmethod
+    def frame_bytes(width: int, height: int, bounces: int, ray_capacity: int = 0) -> int:
+        return int(lib().rtx_wavefront_frame_bytes(int(width), int(height), int(bounces), int(ray_capacity)))
+
+    def add(self, scene: Scene, cam: Camera, width: int, height: int, bounces: int, tag: int = 0,
+            ray_capacity: int = 0) -> int:
+        """Frame id; raises RtxError (-3, RTX_ENOMEM) when the arena has no room now."""
+        fid = C.c_int32(-1)
+        _check(lib().rtx_wavefront_add(self.h, scene.h, C.byref(cam), int(width), int(height), int(bounces),
+                                       int(ray_capacity), int(tag), C.byref(fid)), "rtx_wavefront_add")
+        self.sizes[int(fid.value)] = (int(width), int(height))
+        return int(fid.value)
+
+    def try_add(self, scene: Scene, cam: Camera, width: int, height: int, bounces: int, tag: int = 0,
+                ray_capacity: int = 0):
+        """Frame id, or None when the arena (or the frame table) is full now."""
+        fid = C.c_int32(-1)
+        rc = lib().rtx_wavefront_add(self.h, scene.h, C.byref(cam), int(width), int(height), int(bounces),
+                                     int(ray_capacity), int(tag), C.byref(fid))
+        if rc == -3:
+            return None
+        _check(rc, "rtx_wavefront_add")
+        self.sizes[int(fid.value)] = (int(width), int(height))
+        return int(fid.value)
+
+    def step(self, stream=None) -> int:
+        pend = C.c_int32(0)
+        _check(lib().rtx_wavefront_step(self.h, stream, C.byref(pend)), "rtx_wavefront_step")
+        return int(pend.value)
+
+    def run(self, stream=None):
+        _check(lib().rtx_wavefront_run(self.h, stream), "rtx_wavefront_run")
+
+    def sync(self):
+        _check(lib().rtx_wavefront_sync(self.h), "rtx_wavefront_sync")
+
+    def poll(self, cap: int = 256):
+        fr = (C.c_int32 * cap)()
+        tg = (C.c_uint64 * cap)()
+        n = C.c_uint32(0)
+        _check(lib().rtx_wavefront_poll(self.h, fr, tg, cap, C.byref(n)), "rtx_wavefront_poll")
+        return [(int(fr[i]), int(tg[i])) for i in range(n.value)]
+
+    def stats(self, fid: int) -> FrameStats:
+        st = FrameStats()
+        _check(lib().rtx_wavefront_frame_stats(self.h, int(fid), C.byref(st)), "rtx_wavefront_frame_stats")
+        return st
+
+    def accum(self, fid: int) -> np.ndarray:
+        w, h = self.sizes[int(fid)]
+        out = np.zeros((w * h, 3), np.uint32)
+        _check(lib().rtx_wavefront_read_accum(self.h, int(fid), _ptr(out)), "rtx_wavefront_read_accum")
+        return out
+
+    def rgba8(self, fid: int) -> np.ndarray:
+        w, h = self.sizes[int(fid)]
+        out = np.zeros((w * h, 4), np.uint8)
+        _check(lib().rtx_wavefront_read_rgba8(self.h, int(fid), _ptr(out)), "rtx_wavefront_read_rgba8")
+        return out
+
+    def release(self, fid: int):
+        _check(lib().rtx_wavefront_release(self.h, int(fid)), "rtx_wavefront_release")
+        self.sizes.pop(int(fid), None)
+
+    def memory(self):
+        u, c = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().rtx_wavefront_memory(self.h, C.byref(u), C.byref(c)), "rtx_wavefront_memory")
+        return int(u.value), int(c.value)
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rtx_wavefront_destroy(self.h), "rtx_wavefront_destroy")
+            self.h = C.c_void_p()
 
 
 class Server:
