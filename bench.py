@@ -703,6 +703,7 @@ def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 8):
     dt_seq = timed(one_by_one, reps)
     dt_best = min(dt_eager, dt_graph)
     B.destroy()
+    wfr = bench_c4_wavefront(ctx, rs, scs, cams, rays, reps)
     for R in rs:
         R.destroy()
     for sc in scs:
@@ -718,7 +719,70 @@ def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 8):
             "mode": "stream launches" if dt_eager <= dt_graph else "hipGraph replay",
             "stream_launch_ms_per_batch": round(dt_eager * 1e3, 3), "graph_ms_per_batch": round(dt_graph * 1e3, 3),
             "one_by_one_ms_per_batch": round(dt_seq * 1e3, 3), "one_by_one_mrays": round(rays / dt_seq / 1e6, 2),
-            "autotune": tune, "chosen": best}
+            "autotune": tune, "chosen": best, "wavefront": wfr,
+            "wavefront_vs_one_by_one": round(dt_seq / (wfr["ms_per_batch"] / 1e3), 2)}
+
+
+def bench_c4_wavefront(ctx, rs, scs, cams, rays_expected, reps):
+    """The C4 batch through the wavefront renderer: the same 64 frames (same
+    scenes, cameras, sizes, depths and 2 x pixels + 2 ray capacity) admitted
+    together and stepped to completion, one launch per stage per step.  The
+    traversal knobs are autotuned on it first."""
+    import rtx
+    frames = [(sc, cam, R.p.width, R.p.height, R.p.max_bounces, 2 * R.p.width * R.p.height + 2)
+              for R, sc, cam in zip(rs, scs, cams)]
+    need = sum(rtx.Wavefront.frame_bytes(w, h, d, cap) for _, _, w, h, d, cap in frames)
+    wf = rtx.Wavefront(ctx, arena_bytes=need + (256 << 20), max_frames=len(frames))
+    last = {}
+
+    def batch():
+        ids = [wf.add(sc, cam, w, h, d, ray_capacity=cap) for sc, cam, w, h, d, cap in frames]
+        wf.run()
+        wf.sync()
+        last["rays"] = sum(st.primary_rays + st.secondary_rays for st in map(wf.stats, ids))
+        last["shadow"] = sum(wf.stats(i).shadow_rays for i in ids)
+        for i in ids:
+            wf.release(i)
+
+    def timed(n):
+        ctx.device_sync()
+        t = time.perf_counter()
+        for _ in range(n):
+            batch()
+        return (time.perf_counter() - t) / n
+
+    tune = []
+    for var, tb, coop, grain in [(1, 128, 8, 1), (1, 128, 8, 2), (1, 128, 8, 4), (1, 64, 8, 1), (1, 256, 8, 1),
+                                 (0, 128, 8, 1), (2, 128, 8, 1), (1, 128, 4, 1), (1, 128, 16, 1)]:
+        ctx.tune("trace_variant", var)
+        ctx.tune("trace_block", tb)
+        ctx.tune("coop_leaf", coop)
+        ctx.tune("wf_grain", grain)
+        batch()
+        tune.append({"trace_variant": var, "trace_block": tb, "coop_leaf": coop, "wf_grain": grain,
+                     "ms": round(timed(2) * 1e3, 3)})
+    best = min(tune, key=lambda x: x["ms"])
+    for key in ("trace_variant", "trace_block", "coop_leaf", "wf_grain"):
+        ctx.tune(key, best[key])
+    batch()
+    assert last["rays"] == rays_expected, "the wavefront batch must trace the one-by-one frames' rays"
+    ctx.profile(True)
+    ctx.profile_reset()
+    batch()
+    stages = ctx.profile_report()
+    ctx.profile(False)
+    dt = timed(reps)
+    wf.destroy()
+    ctx.tune("trace_variant", 1)
+    ctx.tune("trace_block", 128)
+    ctx.tune("coop_leaf", 8)
+    ctx.tune("wf_grain", 1)
+    return {"what": "the 64 frames admitted together into the wavefront renderer and stepped to completion "
+                    "(admission, steps, device completion and release timed)",
+            "ms_per_batch": round(dt * 1e3, 3), "mrays": round(rays_expected / dt / 1e6, 2),
+            "shadow_rays_per_batch": last["shadow"], "arena_bytes": need, "autotune": tune, "chosen": best,
+            "stage_ms_per_batch": {k: round(v[1], 4) for k, v in stages.items()},
+            "launches_per_batch": {k: v[0] for k, v in stages.items()}}
 
 
 def bench_instances(ctx, frames: int = 50, reps: int = 3):

@@ -178,6 +178,8 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    chunk; 0: the wave
  *                    walks the roots one after the other, every lane waiting
  *                    for the wave's slowest walk of each root.
+ *   "wf_grain":      wavefront renderer: consecutive 64-ray chunks a
+ *                    traversal wave takes per queue atomic (1..64).
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */

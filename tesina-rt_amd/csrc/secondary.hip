@@ -27,27 +27,24 @@ __global__ void k_sec_init(uint32_t* status, uint32_t nwords, uint32_t* in_cnt, 
     }
 }
 
-// One 256-ray tile of n input rays (tiles taken in order): children per
-// parent, the tile's output offset by look-back, then the children written.
-// Every thread of the block calls it (two block barriers inside); s_w and
-// s_prefix are the block's shared scratch.
-__device__ __forceinline__ void sec_tile(
-    uint32_t tile, uint32_t ntiles, uint64_t n, const float4* __restrict__ dir_in,
-    const int32_t* __restrict__ pix_in, const float* __restrict__ con_in, const uint32_t* __restrict__ flags,
-    const int32_t* __restrict__ hid, const float4* __restrict__ hn, const float4* __restrict__ hp,
-    const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap, float4* __restrict__ ori_out,
-    float4* __restrict__ dir_out, int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt,
-    uint32_t* status, uint32_t cap, uint32_t* s_w, uint32_t* s_prefix_p) {
-    uint32_t& s_prefix = *s_prefix_p;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint64_t k = (uint64_t)tile * SB + tid;
-    uint32_t c = 0;
-    float R = 1.f;
-    v3 refr = mk3(0.f, 0.f, 0.f), refl = mk3(0.f, 0.f, 0.f);
-    float reflectiveness = 0.f;
+// Children of parent k (mark_secondary_rays / generate_secondary_rays,
+// src/kernel/secondary-ray-generator.cl:43-181): bit 0 a refracted ray
+// (refractive material and k > 0), bit 1 a reflected ray (reflective
+// material), with the Schlick R of the refraction (Q22).
+struct Kids {
+    uint32_t c;
+    float R, reflectiveness;
+    v3 refr, refl;
+};
+
+__device__ __forceinline__ Kids sec_children(uint64_t k, uint64_t n, const float4* __restrict__ dir_in,
+                                             const uint32_t* __restrict__ flags, const int32_t* __restrict__ hid,
+                                             const float4* __restrict__ hn, const rtx_material* __restrict__ mats,
+                                             const int32_t* __restrict__ matmap) {
+    Kids q{0u, 1.f, 0.f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
     if (k < n && (flags[k] & 1u)) {
         const rtx_material m = mats[matmap[hid[k]]];
-        reflectiveness = m.reflectiveness;
+        q.reflectiveness = m.reflectiveness;
         v3 d = -normalize(xyz(dir_in[k]));
         v3 nn = normalize(xyz(hn[k]));
         if (m.refractive_index > 0.f) {
@@ -69,22 +66,80 @@ __device__ __forceinline__ void sec_tile(
                 float x = 1.f - cosi;
                 float x2 = x * x;
                 float cos5 = x2 * x2 * x;
-                R = R0 + (1.f - R0) * cos5;
+                q.R = R0 + (1.f - R0) * cos5;
                 kk = __builtin_sqrtf(kk);
-                refr = normalize((wv - kk) * nn - etai * d);
-                c |= 1u;
+                q.refr = normalize((wv - kk) * nn - etai * d);
+                q.c |= 1u;
             }
         }
         if (m.reflectiveness > 0.f) {
             v3 dd = -d;
-            refl = normalize(dd - 2.f * nn * dot(dd, nn));
-            c |= 2u;
+            q.refl = normalize(dd - 2.f * nn * dot(dd, nn));
+            q.c |= 2u;
         }
     }
-    const uint32_t cnt = (c & 1u) + (c >> 1);
-    uint64_t b0 = ballot(cnt & 1u), b1 = ballot(cnt >> 1);
-    uint32_t excl = lanes_below(b0) + 2u * lanes_below(b1);
-    if (lane == 0) s_w[w] = (uint32_t)__builtin_popcountll(b0) + 2u * (uint32_t)__builtin_popcountll(b1);
+    return q;
+}
+
+__device__ __forceinline__ uint32_t kid_count(const Kids& q) { return (q.c & 1u) + (q.c >> 1); }
+
+// The children of parent k at output offset nid: refraction, then reflection.
+__device__ __forceinline__ void sec_write(const Kids& q, uint32_t nid, uint64_t k, const float4* __restrict__ hp,
+                                          const int32_t* __restrict__ pix_in, const float* __restrict__ con_in,
+                                          float4* __restrict__ ori_out, float4* __restrict__ dir_out,
+                                          int32_t* __restrict__ pix_out, float* __restrict__ con_out) {
+    float4 hpk = hp[k];
+    const int32_t px = pix_in[k];
+    const float ck = con_in[k];
+    if (q.c & 1u) {
+        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
+        dir_out[nid] = make_float4(q.refr.x, q.refr.y, q.refr.z, 1e37f);
+        pix_out[nid] = px;
+        con_out[nid] = ck * q.reflectiveness * (1.f - q.R);
+        ++nid;
+    }
+    if (q.c & 2u) {
+        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
+        dir_out[nid] = make_float4(q.refl.x, q.refl.y, q.refl.z, 1e37f);
+        pix_out[nid] = px;
+        con_out[nid] = ck * q.reflectiveness * q.R;
+    }
+}
+
+// One tile of SB * IPT input rays (tiles taken in order), thread t holding
+// rays IPT t .. IPT t + IPT - 1: children per parent, the tile's output
+// offset by look-back, then the children written, parents in order.  A
+// parent whose children would reach the capacity (nid + 2 >= cap,
+// src/kernel/secondary-ray-generator.cl:61-63) writes none and records its
+// offset.  Every thread of the block calls it (two block barriers inside);
+// s_w and s_prefix are the block's shared scratch.
+template <int IPT>
+__device__ __forceinline__ void sec_tile(
+    uint32_t tile, uint32_t ntiles, uint64_t n, const float4* __restrict__ dir_in,
+    const int32_t* __restrict__ pix_in, const float* __restrict__ con_in, const uint32_t* __restrict__ flags,
+    const int32_t* __restrict__ hid, const float4* __restrict__ hn, const float4* __restrict__ hp,
+    const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap, float4* __restrict__ ori_out,
+    float4* __restrict__ dir_out, int32_t* __restrict__ pix_out, float* __restrict__ con_out, uint32_t* out_cnt,
+    uint32_t* status, uint32_t cap, uint32_t* s_w, uint32_t* s_prefix_p) {
+    uint32_t& s_prefix = *s_prefix_p;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t k0 = ((uint64_t)tile * SB + tid) * IPT;
+    Kids q[IPT];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        q[j] = sec_children(k0 + j, n, dir_in, flags, hid, hn, mats, matmap);
+        cnt += kid_count(q[j]);
+    }
+    // inclusive scan of the per-thread counts over the wave
+    uint32_t v = cnt;
+#pragma unroll
+    for (int off = 1; off < RTX_WAVE; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, RTX_WAVE);
+        if (lane >= (uint32_t)off) v += u;
+    }
+    const uint32_t excl = v - cnt;
+    if (lane == RTX_WAVE - 1) s_w[w] = v;
     __syncthreads();
     uint32_t wofs = 0, agg = 0;
 #pragma unroll
@@ -102,25 +157,15 @@ __device__ __forceinline__ void sec_tile(
     __syncthreads();
     if (!cnt) return;
     uint32_t nid = s_prefix + wofs + excl;
-    if ((uint64_t)nid + 2u >= (uint64_t)cap) {  // src/kernel/secondary-ray-generator.cl:61-63
-        atomicMin(&out_cnt[2], nid);
-        return;
-    }
-    float4 hpk = hp[k];
-    const int32_t px = pix_in[k];
-    const float ck = con_in[k];
-    if (c & 1u) {
-        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
-        dir_out[nid] = make_float4(refr.x, refr.y, refr.z, 1e37f);
-        pix_out[nid] = px;
-        con_out[nid] = ck * reflectiveness * (1.f - R);
-        ++nid;
-    }
-    if (c & 2u) {
-        ori_out[nid] = make_float4(hpk.x, hpk.y, hpk.z, 0.0001f);
-        dir_out[nid] = make_float4(refl.x, refl.y, refl.z, 1e37f);
-        pix_out[nid] = px;
-        con_out[nid] = ck * reflectiveness * R;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const uint32_t cj = kid_count(q[j]);
+        if (!cj) continue;
+        if ((uint64_t)nid + 2u >= (uint64_t)cap)
+            atomicMin(&out_cnt[2], nid);
+        else
+            sec_write(q[j], nid, k0 + j, hp, pix_in, con_in, ori_out, dir_out, pix_out, con_out);
+        nid += cj;
     }
 }
 
@@ -141,8 +186,8 @@ __global__ __launch_bounds__(SB) void k_secondary(
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= ntiles) return;
-    sec_tile(tile, ntiles, n, dir_in, pix_in, con_in, flags, hid, hn, hp, mats, matmap, ori_out, dir_out, pix_out,
-             con_out, out_cnt, status, cap, s_w, &s_prefix);
+    sec_tile<1>(tile, ntiles, n, dir_in, pix_in, con_in, flags, hid, hn, hp, mats, matmap, ori_out, dir_out, pix_out,
+                con_out, out_cnt, status, cap, s_w, &s_prefix);
 }
 
 // Secondary rays of every frame that generates this step: persistent blocks
@@ -156,6 +201,9 @@ __global__ __launch_bounds__(SB) void k_wf_secondary(const WfSeg* __restrict__ s
     __shared__ uint32_t s_tile, s_prefix;
     __shared__ uint32_t s_w[SB / RTX_WAVE];
     const uint32_t total = wf_chunks<WF_GEN_TILES>(segs, nseg, 1u, s_end);
+    // only the first `total` blocks take tickets (a ticket is a same-address
+    // atomic; idle blocks would only queue behind the working ones)
+    if (blockIdx.x >= total) return;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
         __syncthreads();
@@ -164,9 +212,9 @@ __global__ __launch_bounds__(SB) void k_wf_secondary(const WfSeg* __restrict__ s
         const uint32_t sg = wf_find(s_end, nseg, g);
         const WfSeg& S = segs[sg];
         const uint64_t n = S.count[0];
-        sec_tile(g - wf_first(s_end, sg), (uint32_t)((n + SB - 1) / SB), n, S.dir4, S.pixel, S.contrib, S.flags,
-                 S.id, S.n4, S.p4, S.mats, S.matmap, S.o_ori4, S.o_dir4, S.o_pixel, S.o_contrib, S.o_count, S.status,
-                 S.o_cap, s_w, &s_prefix);
+        sec_tile<WF_GEN_TILE / SB>(g - wf_first(s_end, sg), (uint32_t)((n + WF_GEN_TILE - 1) / WF_GEN_TILE), n,
+                                   S.dir4, S.pixel, S.contrib, S.flags, S.id, S.n4, S.p4, S.mats, S.matmap, S.o_ori4,
+                                   S.o_dir4, S.o_pixel, S.o_contrib, S.o_count, S.status, S.o_cap, s_w, &s_prefix);
     }
 }
 

@@ -873,8 +873,9 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
             while (it >= s_end[sg]) ++sg;
             const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
             const ShadowSeg& S = segs.s[sg];
-            rr = S.list[local / nl];
-            l = (int)(local % nl);
+            const uint64_t nh = (s_end[sg] - (sg ? s_end[sg - 1] : 0)) / (uint64_t)nl;
+            rr = S.list[local % nh];  // light-major: a wave traces one light's rays from neighbouring hits
+            l = (int)(local / nh);
             fl = S.flags;
             const rtx_light& L = lights->l[l];
             const v3 p = xyz(S.p4[rr]);
@@ -1126,8 +1127,9 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
                     while (it >= s_end[sg]) ++sg;
                     const uint64_t local = it - (sg ? s_end[sg - 1] : 0);
                     const ShadowSeg& S = segs.s[sg];
-                    rr = S.list[local / nl];
-                    l = (int)(local % nl);
+                    const uint64_t nh = (s_end[sg] - (sg ? s_end[sg - 1] : 0)) / (uint64_t)nl;
+                    rr = S.list[local % nh];
+                    l = (int)(local / nh);
                     fl = S.flags;
                     const rtx_light& L = lights->l[l];
                     const v3 p = xyz(S.p4[rr]);
@@ -1842,7 +1844,7 @@ uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
 // drop offset.
 __global__ __launch_bounds__(256) void k_wf_init(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                  uint32_t* __restrict__ ctr) {
-    if (threadIdx.x < WF_Q_COUNT) ctr[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < WF_Q_COUNT; i += blockDim.x) ctr[i] = 0u;
     for (uint32_t i = threadIdx.x; i < nseg; i += blockDim.x) {
         const WfSeg& S = segs[i];
         if (S.primary) {
@@ -1891,25 +1893,25 @@ __global__ __launch_bounds__(256) void k_wf_raygen(const WfSeg* __restrict__ seg
 }
 
 // Closest hit over every frame's rays: persistent waves take 64-ray chunks
-// from the step's queue.  A chunk that starts a 256-ray tile also clears
-// that tile's look-back word for this step's secondary generation.
+// (ChunkSched).  A chunk that starts a secondary-generation tile also clears
+// that tile's look-back word for this step's generation.
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs, uint32_t nseg,
-                                                 uint32_t* __restrict__ queue, uint32_t coop) {
+                                                 uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
     __shared__ uint32_t s_id[STACK * TB];
-    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
     LdsStack<TB> stk{s_id, threadIdx.x};
-    const uint32_t total = wf_chunks<WF_RAYS>(segs, nseg, RTX_WAVE, s_end);
+    WaveChunks<WF_RAYS> wc;
+    wc.init(segs, nseg, RTX_WAVE);
+    ChunkSched sch;
+    sch.init(wc.total, queue, grain);
     const uint32_t lane = threadIdx.x & (RTX_WAVE - 1);
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(queue, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= total) break;
-        const uint32_t sg = __builtin_amdgcn_readfirstlane(wf_find(s_end, nseg, c));
+    constexpr uint32_t PER_TILE = WF_GEN_TILE / RTX_WAVE;
+    for (uint32_t c = 0; sch.next(c);) {
+        uint32_t c0 = 0;
+        const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
         const WfSeg& S = segs[sg];
-        const uint32_t cl = c - wf_first(s_end, sg);
-        if (S.gen && (cl & 3u) == 0u && lane == 0) S.status[cl >> 2] = 0u;
+        const uint32_t cl = c - c0;
+        if (S.gen && cl % PER_TILE == 0u && lane == 0) S.status[cl / PER_TILE] = 0u;
         const uint64_t rr = (uint64_t)cl * RTX_WAVE + lane;
         const bool active = rr < (uint64_t)S.count[0];
         Closest best{0.f, 0.f, 0.f, -1};
@@ -1938,34 +1940,35 @@ __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs,
     }
 }
 
-// Shadow rays of every frame's (hit, light) pairs, as k_shadow, from the
-// step's queue of 64-pair chunks.
+// Shadow rays of every frame's (hit, light) pairs, as k_shadow, in 64-pair
+// chunks (ChunkSched).
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
-                                                  uint32_t* __restrict__ queue, uint32_t coop) {
+                                                  uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
     __shared__ uint32_t s_id[STACK * TB];
-    __shared__ uint32_t s_end[RTX_WF_MAX_SEGS];
     LdsStack<TB> stk{s_id, threadIdx.x};
-    const uint32_t total = wf_chunks<WF_PAIRS>(segs, nseg, RTX_WAVE, s_end);
+    WaveChunks<WF_PAIRS> wc;
+    wc.init(segs, nseg, RTX_WAVE);
+    ChunkSched sch;
+    sch.init(wc.total, queue, grain);
     const uint32_t lane = threadIdx.x & (RTX_WAVE - 1);
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(queue, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= total) break;
-        const uint32_t sg = __builtin_amdgcn_readfirstlane(wf_find(s_end, nseg, c));
+    for (uint32_t c = 0; sch.next(c);) {
+        uint32_t c0 = 0;
+        const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
         const WfSeg& S = segs[sg];
         const int nl = S.lights->count;
-        const uint64_t it = (uint64_t)(c - wf_first(s_end, sg)) * RTX_WAVE + lane;
-        const bool valid = it < (uint64_t)S.list_count[0] * (uint64_t)nl;
+        const uint64_t nh = S.list_count[0];
+        // light-major: a wave traces one light's rays from neighbouring hits
+        const uint64_t it = (uint64_t)(c - c0) * RTX_WAVE + lane;
+        const bool valid = it < nh * (uint64_t)nl;
         uint32_t rr = 0;
         int l = 0;
         RayC rc{};
         float tmax = 1e37f;
         bool shadowed = false, trace = false;
         if (valid) {
-            rr = S.list[it / (uint64_t)nl];
-            l = (int)(it % (uint64_t)nl);
+            rr = S.list[it % nh];
+            l = (int)(it / nh);
             const rtx_light& L = S.lights->l[l];
             const v3 p = xyz(S.p4[rr]);
             v3 dir = mk3(0.f, 0.f, 0.f);
@@ -2009,20 +2012,20 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
 }
 
 // Persistent grids: 20 waves per CU, the occupancy the LDS stack allows.
-void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
-                  hipStream_t st) {
+void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
+                  int cus, hipStream_t st) {
 #define RTX_WF_TRACE(V, B)                                                                                  \
     hipLaunchKernelGGL((k_wf_trace<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
-                       ctr + WF_Q_TRACE, coop)
+                       ctr + WF_Q_TRACE, coop, grain)
     RTX_DISPATCH_TRAVERSAL(RTX_WF_TRACE, var, tb);
 #undef RTX_WF_TRACE
 }
 
-void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, int cus,
-                   hipStream_t st) {
+void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
+                   int cus, hipStream_t st) {
 #define RTX_WF_SHADOW(V, B)                                                                                  \
     hipLaunchKernelGGL((k_wf_shadow<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
-                       ctr + WF_Q_SHADOW, coop)
+                       ctr + WF_Q_SHADOW, coop, grain)
     RTX_DISPATCH_TRAVERSAL(RTX_WF_SHADOW, var, tb);
 #undef RTX_WF_SHADOW
 }

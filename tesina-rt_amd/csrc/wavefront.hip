@@ -125,7 +125,7 @@ int32_t rtx_wavefront_create(rtx_ctx* ctx, const rtx_wavefront_params* p, rtx_wa
     wf->f.resize(p->max_frames);
     int32_t e = RTX_OK;
     if ((e = wf->arena_mem.reserve(ctx, bytes)) || (e = wf->dtab.reserve(ctx, sizeof(WfSeg) * RTX_WF_MAX_SEGS * RING)) ||
-        (e = wf->dctr.reserve(ctx, 64 * RING)) || (e = wf->dfin.reserve(ctx, 32 * RTX_WF_MAX_SEGS * RING))) {
+        (e = wf->dctr.reserve(ctx, (size_t)WF_Q_COUNT * 4 * RING)) || (e = wf->dfin.reserve(ctx, 32 * RTX_WF_MAX_SEGS * RING))) {
         rtx_wavefront_destroy(wf);
         return e;
     }
@@ -297,7 +297,7 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     }
     rtx_ctx* c = wf->ctx;
     WfSeg* D = wf->dtab.as<WfSeg>() + (size_t)k * RTX_WF_MAX_SEGS;
-    uint32_t* ctr = wf->dctr.as<uint32_t>() + 16 * k;
+    uint32_t* ctr = wf->dctr.as<uint32_t>() + (size_t)WF_Q_COUNT * k;
     unsigned long long* dfin = wf->dfin.as<unsigned long long>() + (size_t)k * RTX_WF_MAX_SEGS * 4;
     // the traversal form: every form gives the same records; the sibling-pair
     // form cannot follow the MAX_LEVELS rule of trees of 32 levels, and the
@@ -317,11 +317,11 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     }
     {
         ProfScope ps(c, st, "wf_trace");
-        rtx_wf_trace(D, n, ctr, var, tb, coop, c->cus, st);
+        rtx_wf_trace(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shadow");
-        rtx_wf_shadow(D, n, ctr, var, tb, coop, c->cus, st);
+        rtx_wf_shadow(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shade");
