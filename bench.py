@@ -531,40 +531,20 @@ def bench_c3(ctx, frames: int):
     return out
 
 
-def c5_scenes(seed: int = 0x5EED0005, n_scenes: int = 12, total_tris: int = 20_000_000):
-    """C5 scene set: T ~ logU[10k, 4M] per scene, rescaled so that the sum is
-    about 20M triangles; sphere sets (5000-triangle UV spheres) and
-    heightfields alternate."""
-    import numpy as np
-    from rtx import scenes
-    rng = scenes.Rng(seed)
-    t = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(4e6)))
-    t = t * (total_tris / t.sum())
-    out = []
-    for i, ti in enumerate(t):
-        if i % 2 == 0:
-            n_sph = max(2, int(round(ti / 5000)))
-            sd = scenes.config_c2(seed=seed + 16 * i, n_spheres=n_sph, res=(256, 256))
-        else:
-            n = max(8, int(round(np.sqrt(ti / 2.0))) + 1)
-            sd = scenes.config_c3(seed=seed + 16 * i, n=n, res=(256, 256))
-            sd.pos[:, 1] = scenes.displace_host(sd, 0.5)[:, 1]  # a static wave field
-        out.append(sd)
-    return out
-
-
-def bench_c5(ctx, rates, max_requests: int, workers: int = 12):
+def bench_c5(ctx, rates, max_requests: int, workers: int = 64, arena_bytes: int = 16 << 30):
     """C5 serving: 12 resident scenes (~20M triangles), Poisson arrivals per
     rate, Zipf(1.1) scene popularity, resolution uniform over 256^2, 512^2 and
     1280x720, depth uniform over 1..3, camera on a sphere of 2.5x the scene
     radius looking at its centre.  Latency = arrival -> image complete on the
     device.  The concurrent server and the plain per-request path see the
-    same request stream."""
+    same request stream.  The server admits requests by the bytes they take
+    from a request arena of `arena_bytes`, at most `workers` in flight, and
+    renders them with the wavefront renderer."""
     import numpy as np
     import rtx
     from rtx import scenes
     t0 = time.perf_counter()
-    sds = c5_scenes()
+    sds = scenes.config_c5()
     log(f"C5: generated {len(sds)} scenes, {sum(sd.T for sd in sds)} triangles in {time.perf_counter() - t0:.1f}s")
     scs = []
     for sd in sds:
@@ -572,39 +552,26 @@ def bench_c5(ctx, rates, max_requests: int, workers: int = 12):
         sc.build(30, 1)
         scs.append(sc)
     ctx.device_sync()
-    centres = [0.5 * (sd.pos.min(0) + sd.pos.max(0)) for sd in sds]
-    radii = [0.5 * float(np.linalg.norm(sd.pos.max(0) - sd.pos.min(0))) for sd in sds]
     classes = [(256, 256), (512, 512), (1280, 720)]
-    zipf = 1.0 / np.arange(1, len(sds) + 1) ** 1.1
-    zipf /= zipf.sum()
     res = {"workload": "C5 serving: 12 resident scenes, T ~ logU[10k, 4M] rescaled to ~20M triangles "
                        "(sphere sets and heightfields), Poisson arrivals, Zipf(1.1) scenes, resolution "
                        "U{256^2, 512^2, 1280x720}, depth U{1..3}, camera on a sphere of 2.5x the scene radius",
-           "triangles": [int(sd.T) for sd in sds], "workers": workers, "rates": list(rates), "server": [],
-           "plain": []}
+           "triangles": [int(sd.T) for sd in sds], "requests_in_flight_max": workers, "arena_bytes": arena_bytes,
+           "rates": list(rates), "server": [], "plain": []}
     for mode in ("server", "plain"):
-        srv = rtx.Server(ctx, classes, workers=workers, max_bounces=3, plain=(mode == "plain"))
+        srv = rtx.Server(ctx, classes, workers=workers, max_bounces=3, plain=(mode == "plain"),
+                         arena_bytes=arena_bytes)
         ids = [srv.add_scene(sc) for sc in scs]
         wb, sb = srv.memory()
-        res["worker_bytes" if mode == "server" else "plain_worker_bytes"] = wb
+        res["request_memory_bytes" if mode == "server" else "plain_renderer_bytes"] = wb
         res["resident_scene_bytes"] = sb
         for lam in rates:
-            rng = np.random.default_rng(int(lam) * 7 + 1)
             n = int(min(max_requests, max(40, lam * 1.5)))
-            gaps = rng.exponential(1.0 / lam, n)
+            reqs = scenes.c5_requests(sds, lam, n, int(lam) * 7 + 1)
             start = rtx.now_ns() + 5_000_000
-            arrivals = start + (np.cumsum(gaps) * 1e9).astype(np.int64)
-            reqs = []
-            for i in range(n):
-                k = int(rng.choice(len(sds), p=zipf))
-                w, h = classes[int(rng.integers(0, 3))]
-                d = rng.normal(size=3)
-                d /= np.linalg.norm(d)
-                pos = centres[k] + 2.5 * radii[k] * d
-                up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
-                cam = rtx.camera_set(pos, centres[k], up, np.pi / 4, w / h)
-                reqs.append(dict(scene=ids[k], width=w, height=h, bounces=int(rng.integers(1, 4)), camera=cam,
-                                 arrival_ns=int(arrivals[i]), tag=i))
+            for r in reqs:
+                r["scene"] = ids[r["scene"]]
+                r["arrival_ns"] += start
             srv.submit(reqs)
             srv.drain(600_000)
             done = srv.poll(len(reqs) + 16)
@@ -636,32 +603,16 @@ def bench_c4(ctx, n_scenes: int = 64, reps: int = 5, streams: int = 8):
     import rtx
     from rtx import scenes
     t0 = time.perf_counter()
-    rng = scenes.Rng(0x5EED0004)
-    tris = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(2e6)))
-    pick = np.random.default_rng(4)
-    res_set = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
-    scs, rs, cams, rays_cap = [], [], [], []
-    for i, ti in enumerate(tris):
-        if i % 2 == 0:
-            sd = scenes.config_c2(seed=0x5EED0004 + 16 * i, n_spheres=max(2, int(round(ti / 5000))), res=(256, 256))
-        else:
-            sd = scenes.config_c3(seed=0x5EED0004 + 16 * i, n=max(8, int(round(np.sqrt(ti / 2.0))) + 1),
-                                  res=(256, 256))
-            sd.pos[:, 1] = scenes.displace_host(sd, 0.5)[:, 1]
-        w, h = res_set[int(pick.integers(0, 4))]
-        depth = int(pick.integers(1, 5))
-        sc = scenes.load(rtx.Scene(ctx), sd)
+    scs, rs, cams = [], [], []
+    for fr in scenes.config_c4(n_scenes=n_scenes):
+        sc = scenes.load(rtx.Scene(ctx), fr.sd)
         sc.build(30, 1)
-        c = 0.5 * (sd.pos.min(0) + sd.pos.max(0))
-        r = 0.5 * float(np.linalg.norm(sd.pos.max(0) - sd.pos.min(0)))
-        d = pick.normal(size=3)
-        d /= np.linalg.norm(d)
-        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
-        cams.append(rtx.camera_set(c + 2.5 * r * d, c, up, np.pi / 4, w / h))
+        cams.append(scenes.orbit_camera(fr.sd, fr.direction, fr.width, fr.height))
         # bundles of 2 x pixels rays per bounce (the reference's capacity
         # rule drops rays beyond it), to keep 64 renderers within memory
-        rs.append(rtx.Renderer(ctx, w, h, depth, ray_capacity=2 * w * h + 2))
+        rs.append(rtx.Renderer(ctx, fr.width, fr.height, fr.bounces, ray_capacity=fr.ray_capacity))
         scs.append(sc)
+    tris = np.array([sc.T for sc in scs])
     ctx.device_sync()
     log(f"C4: {n_scenes} scenes, {int(tris.sum())} triangles (target), set up in {time.perf_counter() - t0:.1f}s")
 

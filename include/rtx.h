@@ -529,27 +529,33 @@ int32_t rtx_wavefront_memory(rtx_wavefront* wf, uint64_t* used, uint64_t* capaci
 /* ------------------------------------------------------------- serving -- */
 /* Request server over resident scenes (no counterpart in the reference,
  * whose apps render one scene from one thread).  A dispatcher thread of the
- * library starts each request at its arrival time on one of `workers`
- * streams (requests run concurrently; a freed worker takes the next ready
- * request at once), renders it with the read-back-free frame (no rebuild:
- * scenes are resident and built) and timestamps its completion on the
- * device.  Scenes are admitted against an HBM budget.  With `plain` set the
- * server is the per-request baseline: one request at a time, the plain
- * frame (rtx_render_frame). */
+ * library admits requests at their arrival time, first come first served,
+ * when their rays, hits and framebuffer (rtx_wavefront_frame_bytes) fit the
+ * request arena and fewer than `workers` are in flight (a request larger
+ * than the arena is refused with RTX_ENOMEM), and renders them with the
+ * wavefront renderer: every step advances all requests in flight by one
+ * bounce in shared launches, so requests that arrive while others run join
+ * at the next step whatever their scene, resolution or depth (no rebuild:
+ * scenes are resident and built).  Completion is timestamped when the
+ * device has finished the request.  Scenes are admitted against an HBM
+ * budget that also holds the arena.  With `plain` set the server is the
+ * per-request baseline: one request at a time, the plain frame
+ * (rtx_render_frame) on a renderer per resolution class. */
 #define RTX_SERVER_MAX_CLASSES 4
 typedef struct rtx_server_params {
-    int32_t workers;          /* concurrent requests (1..16) */
+    int32_t workers;          /* requests in flight at most (1..RTX_WF_MAX_FRAMES); plain: ignored */
     int32_t max_bounces;      /* largest request depth (1..16) */
-    int32_t classes;          /* resolution classes (1..RTX_SERVER_MAX_CLASSES) */
+    int32_t classes;          /* plain mode: resolution classes (1..RTX_SERVER_MAX_CLASSES); else unused */
     int32_t class_w[RTX_SERVER_MAX_CLASSES], class_h[RTX_SERVER_MAX_CLASSES];
-    uint64_t hbm_budget;      /* bytes for workers + resident scenes; 0 = 90% of free device memory */
+    uint64_t hbm_budget;      /* bytes for the request memory + resident scenes; 0 = 90% of free device memory */
     int32_t plain;            /* 1: baseline, one request at a time with the plain frame */
     int32_t queue_cap;        /* submissions held at most (0 = 65536) */
+    uint64_t arena_bytes;     /* request arena (part of the budget); 0 = half the budget */
 } rtx_server_params;
 
 typedef struct rtx_request {
     int32_t scene;            /* id from rtx_server_add_scene */
-    int32_t width, height;    /* one of the resolution classes */
+    int32_t width, height;    /* any size (plain mode: one of the resolution classes) */
     int32_t bounces;          /* 0..max_bounces */
     rtx_camera camera;
     int64_t arrival_ns;       /* rtx_now_ns() clock; not started before it */
@@ -559,11 +565,11 @@ typedef struct rtx_request {
 
 typedef struct rtx_completion {
     uint64_t tag;
-    int64_t arrival_ns, start_ns, done_ns;  /* done: the frame's completion observed on the device */
+    int64_t arrival_ns, start_ns, done_ns;  /* start: admitted; done: completion observed on the device */
     uint64_t rays;                          /* primary + secondary rays */
     uint64_t shadow_rays;
     int32_t status;                         /* RTX_OK or the error of this request */
-    int32_t worker;
+    int32_t worker;                         /* wavefront frame slot (plain: resolution class) */
 } rtx_completion;
 
 /* Monotonic clock of the server, nanoseconds. */

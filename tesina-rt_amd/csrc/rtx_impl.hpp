@@ -18,6 +18,7 @@
 
 #include <cstdint>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/rtx.h"
@@ -432,7 +433,10 @@ void rtx_wf_resolve(const WfSeg* segs, uint32_t nseg, uint64_t pixels, unsigned 
                     hipStream_t st);
 
 // Wavefront internals for the request server (wavefront.hip).
-int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pending, std::vector<int>* finishing);
+// before_done(stream, finishing frames) runs after the step's kernels and
+// before its completion event is recorded.
+int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pending, std::vector<int>* finishing,
+                                const std::function<int32_t(hipStream_t, const std::vector<int>&)>& before_done);
 uint64_t rtx_wf_arena_size(const rtx_wavefront* wf);
 int64_t rtx_wf_steps_in_flight(const rtx_wavefront* wf);
 const void* rtx_wf_accum_dev(rtx_wavefront* wf, int32_t frame);

@@ -1,49 +1,65 @@
-// Request server over resident scenes: open-loop arrivals, concurrent
-// workers (one stream and one preallocated renderer per resolution class
-// each), HBM-budget admission of scenes, device-side completion times.
+// Request server over resident scenes: open-loop arrivals, continuous
+// batching on the wavefront renderer, HBM-budget admission, device-side
+// completion times.  No counterpart in the reference, whose apps render one
+// frame of one scene from one thread (src/rt/renderer.cpp:113-214).
 //
-// A dispatcher thread owns every GPU call of the server: it moves requests
-// whose arrival time has come into the ready queue, starts ready requests on
-// free workers (a worker freed by a completion takes the next request at
-// once, so requests of different scenes, sizes and depths run side by side),
-// and polls the workers' completion events.  A request's latency is
-// done_ns - arrival_ns, where done_ns is when the dispatcher observed its
-// frame (accumulators and ray counts included) complete on the device.
+// A dispatcher thread owns every GPU call of the server.  Requests whose
+// arrival time has come join a first-come-first-served admission queue
+// (arena.hpp: Admission): a request is admitted when its rays, hits and
+// framebuffer (rtx_wavefront_frame_bytes) fit the request arena and fewer
+// than `workers` requests are in flight; otherwise it waits at the head, and
+// one larger than the whole arena is refused.  While requests are in flight
+// the dispatcher keeps up to two wavefront steps enqueued: each step
+// advances every admitted request by one bounce in shared launches, so
+// requests that arrive while others run join at the next step, whatever
+// their scene, size or depth.  A request's latency is done_ns - arrival_ns,
+// where done_ns is when the dispatcher observed the event of its last step
+// (accumulators copied out included).
+//
+// `plain` mode is the per-request baseline: one request at a time with the
+// reference's frame and its per-bounce read-backs (rtx_render_frame) on a
+// renderer per resolution class.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <mutex>
 #include <queue>
 #include <thread>
+#include <unordered_map>
 
+#include "arena.hpp"
 #include "rtx_impl.hpp"
 
 uint64_t rtx_scene_device_bytes(const rtx_scene* s);
 
 namespace {
 
-struct Worker {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
-    rtx_renderer* r[RTX_SERVER_MAX_CLASSES] = {};
-    unsigned long long* counts_host = nullptr;  // pinned: [0] secondary, [1] shadow
-    bool busy = false;
-    rtx_request req{};
-    int cls = 0;
-    int64_t start_ns = 0;
-};
-
 struct Later {
     bool operator()(const rtx_request& a, const rtx_request& b) const { return a.arrival_ns > b.arrival_ns; }
 };
+
+// A request between its arrival and its completion.
+struct Live {
+    rtx_request req{};
+    int32_t frame = -1;
+    int64_t start_ns = 0;
+    void* staging = nullptr;  // pinned copy of the accumulators (requests with out_accum)
+};
+
+constexpr int64_t STEPS_AHEAD = 2;  // wavefront steps kept enqueued
 
 }  // namespace
 
 struct rtx_server {
     rtx_ctx* ctx = nullptr;
     rtx_server_params p{};
-    std::vector<Worker> workers;
+    // plain mode
+    rtx_renderer* plain_r[RTX_SERVER_MAX_CLASSES] = {};
+    // wavefront mode
+    rtx_wavefront* wf = nullptr;
+    hipStream_t stream = nullptr;
     std::vector<rtx_scene*> scenes;
     uint64_t workers_bytes = 0, scenes_bytes = 0, budget = 0;
     std::mutex mu;
@@ -53,7 +69,8 @@ struct rtx_server {
     uint64_t submitted = 0, finished = 0;
     bool stop = false;
     std::thread thread;
-    void run();
+    void run_plain();
+    void run_wavefront();
 };
 
 int64_t rtx_now_ns(void) {
@@ -68,137 +85,232 @@ static int class_of(const rtx_server* srv, int w, int h) {
     return -1;
 }
 
-void rtx_server::run() {
-    std::deque<rtx_request> ready;
-    std::vector<rtx_completion> done_batch;
+static rtx_completion completion_of(const rtx_request& rq, int32_t status) {
+    rtx_completion c{};
+    c.tag = rq.tag;
+    c.arrival_ns = rq.arrival_ns;
+    c.status = status;
+    c.worker = -1;
+    return c;
+}
+
+// The per-request baseline: the reference's frame, one request at a time.
+void rtx_server::run_plain() {
+    for (;;) {
+        rtx_request rq{};
+        bool have = false;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                if (!pending.empty() && pending.top().arrival_ns <= rtx_now_ns()) {
+                    rq = pending.top();
+                    pending.pop();
+                    have = true;
+                    break;
+                }
+                if (stop && pending.empty()) return;
+                int64_t until = rtx_now_ns() + 2000000;
+                if (!pending.empty()) until = std::min(until, pending.top().arrival_ns);
+                const int64_t wait = until - rtx_now_ns();
+                if (wait > 20000)
+                    cv.wait_for(lk, std::chrono::nanoseconds(wait - 10000));
+                else
+                    std::this_thread::yield();
+            }
+        }
+        if (!have) continue;
+        rtx_completion c = completion_of(rq, RTX_OK);
+        const int cls = class_of(this, rq.width, rq.height);
+        rtx_scene* sc = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (rq.scene >= 0 && rq.scene < (int)scenes.size()) sc = scenes[rq.scene];
+        }
+        c.start_ns = rtx_now_ns();
+        if (cls < 0 || !sc || rq.bounces < 0 || rq.bounces > p.max_bounces) {
+            c.status = RTX_EINVAL;
+        } else {
+            rtx_renderer* R = plain_r[cls];
+            rtx_frame_stats fs{};
+            int32_t e = rtx_renderer_set_bounces(R, rq.bounces);
+            if (!e) e = rtx_render_frame(R, sc, &rq.camera, 0, &fs, stream);
+            if (!e && rq.out_accum)
+                e = hipMemcpy(rq.out_accum, R->fb->accum.p, (size_t)rq.width * rq.height * 12,
+                              hipMemcpyDeviceToHost) == hipSuccess ? RTX_OK : RTX_EDEVICE;
+            c.rays = fs.primary_rays + fs.secondary_rays;
+            c.shadow_rays = fs.shadow_rays;
+            c.status = e;
+            c.worker = cls;
+        }
+        c.done_ns = rtx_now_ns();
+        std::lock_guard<std::mutex> lk(mu);
+        completions.push_back(c);
+        ++finished;
+        cv.notify_all();
+    }
+}
+
+void rtx_server::run_wavefront() {
+    rtxwf::Admission adm;
+    std::unordered_map<uint64_t, Live> live;  // by the server's request sequence number
+    uint64_t seq = 0;
+    size_t inflight = 0;
+    int32_t active = 0;  // admitted requests with steps still to enqueue
+    std::vector<rtx_completion> done;
+    const uint64_t arena = rtx_wf_arena_size(wf);
+    auto finish = [&](const rtx_completion& c) { done.push_back(c); };
     for (;;) {
         const int64_t now = rtx_now_ns();
         bool stopping = false;
         {
             std::lock_guard<std::mutex> lk(mu);
             while (!pending.empty() && pending.top().arrival_ns <= now) {
-                ready.push_back(pending.top());
+                const rtx_request rq = pending.top();
                 pending.pop();
+                const bool ok = rq.scene >= 0 && rq.scene < (int)scenes.size() && rq.bounces >= 0 &&
+                                rq.bounces <= p.max_bounces && rq.width > 0 && rq.height > 0;
+                if (!ok) {
+                    rtx_completion c = completion_of(rq, RTX_EINVAL);
+                    c.start_ns = c.done_ns = now;
+                    finish(c);
+                    continue;
+                }
+                Live L;
+                L.req = rq;
+                live.emplace(seq, L);
+                adm.push(seq, rtx_wavefront_frame_bytes(rq.width, rq.height, rq.bounces, 0));
+                ++seq;
             }
             stopping = stop && pending.empty();
         }
-        // completions
-        int inflight = 0;
-        for (size_t wi = 0; wi < workers.size(); ++wi) {
-            Worker& w = workers[wi];
-            if (!w.busy) continue;
-            const hipError_t q = hipEventQuery(w.done);
-            if (q == hipErrorNotReady) {
-                ++inflight;
-                continue;
-            }
-            rtx_completion c{};
-            c.tag = w.req.tag;
-            c.arrival_ns = w.req.arrival_ns;
-            c.start_ns = w.start_ns;
-            c.done_ns = rtx_now_ns();
-            c.rays = (uint64_t)w.req.width * w.req.height + w.counts_host[0];
-            c.shadow_rays = w.counts_host[1];
-            c.status = q == hipSuccess ? RTX_OK : RTX_EDEVICE;
-            c.worker = (int32_t)wi;
-            done_batch.push_back(c);
-            w.busy = false;
-        }
-        // start ready requests on free workers
-        for (size_t wi = 0; wi < workers.size() && !ready.empty(); ++wi) {
-            Worker& w = workers[wi];
-            if (w.busy) continue;
-            const rtx_request rq = ready.front();
-            ready.pop_front();
-            rtx_completion c{};
-            c.tag = rq.tag;
-            c.arrival_ns = rq.arrival_ns;
-            c.worker = (int32_t)wi;
-            const int cls = class_of(this, rq.width, rq.height);
-            int nscenes = 0;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                nscenes = (int)scenes.size();
-            }
-            if (cls < 0 || rq.scene < 0 || rq.scene >= nscenes || rq.bounces < 0 || rq.bounces > p.max_bounces) {
-                c.status = RTX_EINVAL;
-                c.start_ns = c.done_ns = rtx_now_ns();
-                done_batch.push_back(c);
-                continue;
-            }
-            rtx_renderer* R = w.r[cls];
+        // admission: first come first served, by arena bytes
+        std::vector<uint64_t> admitted, rejected;
+        auto try_add = [&](const rtxwf::Admission::Item& it) {
+            Live& L = live.at(it.id);
             rtx_scene* sc = nullptr;
             {
                 std::lock_guard<std::mutex> lk(mu);
-                sc = scenes[rq.scene];
+                sc = scenes[L.req.scene];
             }
-            int32_t e = rtx_renderer_set_bounces(R, rq.bounces);
-            w.start_ns = rtx_now_ns();
-            if (p.plain) {
-                // the per-request baseline: the reference's frame with its
-                // per-bounce read-backs, one request at a time
-                rtx_frame_stats fs{};
-                if (!e) e = rtx_render_frame(R, sc, &rq.camera, 0, &fs, w.stream);
-                if (!e && rq.out_accum)
-                    e = hipMemcpy(rq.out_accum, R->fb->accum.p, (size_t)rq.width * rq.height * 12,
-                                  hipMemcpyDeviceToHost) == hipSuccess ? RTX_OK : RTX_EDEVICE;
-                c.start_ns = w.start_ns;
+            int32_t fr = -1;
+            const int32_t e = rtx_wavefront_add(wf, sc, &L.req.camera, L.req.width, L.req.height, L.req.bounces, 0,
+                                                it.id, &fr);
+            if (e == RTX_ENOMEM) return false;  // no room now: wait at the head
+            if (e) {
+                rtx_completion c = completion_of(L.req, e);
+                c.start_ns = c.done_ns = rtx_now_ns();
+                finish(c);
+                live.erase(it.id);
+                return true;  // leaves the queue (not in flight)
+            }
+            L.frame = fr;
+            L.start_ns = rtx_now_ns();
+            if (L.req.out_accum &&
+                hipHostMalloc(&L.staging, (size_t)L.req.width * L.req.height * 12, hipHostMallocDefault) != hipSuccess)
+                L.staging = nullptr;
+            return true;
+        };
+        adm.admit(arena, inflight, (size_t)p.workers, try_add, admitted, rejected);
+        for (uint64_t id : admitted)
+            if (live.count(id)) {
+                ++inflight;
+                ++active;
+            }
+        for (uint64_t id : rejected) {
+            Live& L = live.at(id);
+            rtx_completion c = completion_of(L.req, RTX_ENOMEM);
+            c.start_ns = c.done_ns = rtx_now_ns();
+            finish(c);
+            live.erase(id);
+        }
+        // steps: keep up to STEPS_AHEAD enqueued while requests need them
+        if (active > 0 && rtx_wf_steps_in_flight(wf) < STEPS_AHEAD) {
+            auto copy_out = [&](hipStream_t st, const std::vector<int>& fin) -> int32_t {
+                for (auto& kv : live) {
+                    Live& L = kv.second;
+                    if (!L.staging) continue;
+                    if (std::find(fin.begin(), fin.end(), L.frame) == fin.end()) continue;
+                    RTX_CHECK_HIP(hipMemcpyAsync(L.staging, rtx_wf_accum_dev(wf, L.frame),
+                                                 (size_t)L.req.width * L.req.height * 12, hipMemcpyDeviceToHost, st));
+                }
+                return RTX_OK;
+            };
+            int32_t pend = 0;
+            const int32_t e = rtx_wavefront_step_impl(wf, stream, &pend, nullptr, copy_out);
+            if (e) {  // a failed step fails every request in flight
+                for (auto& kv : live)
+                    if (kv.second.frame >= 0) {
+                        rtx_completion c = completion_of(kv.second.req, e);
+                        c.start_ns = kv.second.start_ns;
+                        c.done_ns = rtx_now_ns();
+                        finish(c);
+                    }
+                live.clear();
+                pend = 0;
+                inflight = 0;
+            }
+            active = pend;
+        }
+        // completions
+        int32_t fr[64];
+        uint64_t tags[64];
+        uint32_t nf = 0;
+        if (rtx_wavefront_poll(wf, fr, tags, 64, &nf) == RTX_OK) {
+            for (uint32_t i = 0; i < nf; ++i) {
+                auto it = live.find(tags[i]);
+                if (it == live.end()) continue;
+                Live& L = it->second;
+                rtx_completion c = completion_of(L.req, RTX_OK);
+                c.start_ns = L.start_ns;
                 c.done_ns = rtx_now_ns();
+                rtx_frame_stats fs{};
+                c.status = rtx_wavefront_frame_stats(wf, L.frame, &fs);
                 c.rays = fs.primary_rays + fs.secondary_rays;
                 c.shadow_rays = fs.shadow_rays;
-                c.status = e;
-                done_batch.push_back(c);
-                continue;
+                c.worker = L.frame;
+                if (L.req.out_accum) {
+                    if (L.staging)
+                        memcpy(L.req.out_accum, L.staging, (size_t)L.req.width * L.req.height * 12);
+                    else
+                        c.status = RTX_ENOMEM;
+                }
+                if (L.staging) (void)hipHostFree(L.staging);
+                (void)rtx_wavefront_release(wf, L.frame);
+                --inflight;
+                finish(c);
+                live.erase(it);
             }
-            if (!e) e = rtx_render_frame_async(R, sc, &rq.camera, 0, w.stream);
-            if (!e && hipMemcpyAsync(w.counts_host, R->counts.p, 16, hipMemcpyDeviceToHost, w.stream) != hipSuccess)
-                e = RTX_EDEVICE;
-            if (!e && rq.out_accum &&
-                hipMemcpyAsync(rq.out_accum, R->fb->accum.p, (size_t)rq.width * rq.height * 12,
-                               hipMemcpyDeviceToHost, w.stream) != hipSuccess)
-                e = RTX_EDEVICE;
-            if (!e && hipEventRecord(w.done, w.stream) != hipSuccess) e = RTX_EDEVICE;
-            if (e) {
-                c.status = e;
-                c.start_ns = w.start_ns;
-                c.done_ns = rtx_now_ns();
-                done_batch.push_back(c);
-                continue;
-            }
-            w.req = rq;
-            w.cls = cls;
-            w.busy = true;
-            ++inflight;
         }
-        if (!done_batch.empty()) {
+        if (!done.empty()) {
             std::lock_guard<std::mutex> lk(mu);
-            for (auto& c : done_batch) completions.push_back(c);
-            finished += done_batch.size();
-            done_batch.clear();
+            for (auto& c : done) completions.push_back(c);
+            finished += done.size();
+            done.clear();
             cv.notify_all();
         }
-        if (stopping && ready.empty() && inflight == 0) break;
-        // wait: spin while frames are in flight (completion times matter),
+        if (stopping && live.empty()) break;
+        // wait: spin while steps are in flight (completion times matter),
         // otherwise sleep until the next arrival or a submission
-        if (inflight > 0 && (ready.empty() || inflight == (int)workers.size())) {
+        if (!live.empty()) {
             std::this_thread::yield();
-        } else if (ready.empty()) {
+        } else {
             std::unique_lock<std::mutex> lk(mu);
             int64_t until = rtx_now_ns() + 2000000;  // 2 ms
             if (!pending.empty()) until = std::min(until, pending.top().arrival_ns);
             const int64_t wait = until - rtx_now_ns();
-            if (wait > 20000 && !stop)
-                cv.wait_for(lk, std::chrono::nanoseconds(wait - 10000));
+            if (wait > 20000 && !stop) cv.wait_for(lk, std::chrono::nanoseconds(wait - 10000));
         }
     }
 }
 
 int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server** out) {
     RTX_REQUIRE(ctx && p && out, RTX_EINVAL, "rtx_server_create: null argument");
-    RTX_REQUIRE(p->workers >= 1 && p->workers <= 16, RTX_EINVAL, "rtx_server_create: workers must be in [1, 16]");
     RTX_REQUIRE(p->max_bounces >= 0 && p->max_bounces <= 16, RTX_EINVAL, "rtx_server_create: bad max_bounces");
-    RTX_REQUIRE(p->classes >= 1 && p->classes <= RTX_SERVER_MAX_CLASSES, RTX_EINVAL,
+    RTX_REQUIRE(p->classes >= 0 && p->classes <= RTX_SERVER_MAX_CLASSES, RTX_EINVAL,
                 "rtx_server_create: bad class count");
+    RTX_REQUIRE(!p->plain || p->classes >= 1, RTX_EINVAL, "rtx_server_create: plain mode needs resolution classes");
+    RTX_REQUIRE(p->plain || (p->workers >= 1 && p->workers <= RTX_WF_MAX_FRAMES), RTX_EINVAL,
+                "rtx_server_create: workers (requests in flight) must be in [1, 256]");
     for (int c = 0; c < p->classes; ++c)
         RTX_REQUIRE(p->class_w[c] > 0 && p->class_h[c] > 0, RTX_EINVAL, "rtx_server_create: bad class size");
     rtx_server* srv = new rtx_server();
@@ -210,40 +322,47 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     srv->budget = p->hbm_budget ? p->hbm_budget : (uint64_t)(0.9 * (double)free_b);
     const uint64_t before = ctx->bytes_in_use;
-    int32_t e = 0;
-    srv->workers.resize(srv->p.workers);
-    for (auto& w : srv->workers) {
-        if (hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc((void**)&w.counts_host, 16, hipHostMallocDefault) != hipSuccess) {
-            rtx_set_error("rtx_server_create: stream, event or pinned buffer creation failed");
-            e = RTX_EDEVICE;
-            break;
-        }
+    int32_t e = RTX_OK;
+    if (hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking) != hipSuccess) {
+        srv->stream = nullptr;
+        rtx_set_error("rtx_server_create: stream creation failed");
+        e = RTX_EDEVICE;
+    }
+    if (!e && srv->p.plain) {
         for (int c = 0; c < srv->p.classes && !e; ++c) {
             rtx_render_params rp{};
             rp.width = p->class_w[c];
             rp.height = p->class_h[c];
             rp.max_bounces = p->max_bounces;
-            if ((e = rtx_renderer_create(ctx, &rp, &w.r[c]))) break;
-            w.r[c]->single_stream = true;  // one stream per worker; workers give the concurrency
-            // first raygen uploads the pixel-sample table; do it now, not in a request
-            rtx_camera cam{};
-            e = rtx_raygen_primary(w.r[c]->rays[0], &cam, rp.width, rp.height, 32, 1, nullptr, 0, 0, w.stream);
+            e = rtx_renderer_create(ctx, &rp, &srv->plain_r[c]);
         }
-        if (e) break;
+    } else if (!e) {
+        rtx_wavefront_params wp{};
+        wp.arena_bytes = p->arena_bytes ? p->arena_bytes : srv->budget / 2;
+        wp.max_frames = p->workers;
+        if (wp.arena_bytes >= srv->budget) {
+            rtx_set_error("rtx_server_create: the request arena exceeds the HBM budget");
+            e = RTX_ENOMEM;
+        } else {
+            e = rtx_wavefront_create(ctx, &wp, &srv->wf);
+        }
     }
     if (!e && hipDeviceSynchronize() != hipSuccess) e = RTX_EDEVICE;
     srv->workers_bytes = ctx->bytes_in_use - before;
     if (!e && srv->workers_bytes > srv->budget) {
-        rtx_set_error("rtx_server_create: workers alone exceed the HBM budget");
+        rtx_set_error("rtx_server_create: the request memory alone exceeds the HBM budget");
         e = RTX_ENOMEM;
     }
     if (e) {
         rtx_server_destroy(srv);
         return e;
     }
-    srv->thread = std::thread([srv] { srv->run(); });
+    srv->thread = std::thread([srv] {
+        if (srv->p.plain)
+            srv->run_plain();
+        else
+            srv->run_wavefront();
+    });
     *out = srv;
     return RTX_OK;
 }
@@ -258,14 +377,11 @@ int32_t rtx_server_destroy(rtx_server* srv) {
         srv->cv.notify_all();
         srv->thread.join();
     }
-    for (auto& w : srv->workers) {
-        if (w.stream) (void)hipStreamSynchronize(w.stream);
-        for (auto* r : w.r)
-            if (r) rtx_renderer_destroy(r);
-        if (w.counts_host) (void)hipHostFree(w.counts_host);
-        if (w.done) (void)hipEventDestroy(w.done);
-        if (w.stream) (void)hipStreamDestroy(w.stream);
-    }
+    if (srv->stream) (void)hipStreamSynchronize(srv->stream);
+    for (auto* r : srv->plain_r)
+        if (r) rtx_renderer_destroy(r);
+    if (srv->wf) rtx_wavefront_destroy(srv->wf);
+    if (srv->stream) (void)hipStreamDestroy(srv->stream);
     delete srv;
     return RTX_OK;
 }

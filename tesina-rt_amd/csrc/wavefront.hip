@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cstring>
 #include <deque>
+#include <functional>
 
 #include "arena.hpp"
 #include "rtx_impl.hpp"
@@ -260,7 +261,8 @@ static WfSeg make_seg(rtx_wavefront* wf, const WfFrame& fr, uint32_t fin) {
 
 // One step: every active frame runs its next bounce.  `finishing` (optional)
 // receives the frames this step completes.
-int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pending, std::vector<int>* finishing) {
+int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pending, std::vector<int>* finishing,
+                                const std::function<int32_t(hipStream_t, const std::vector<int>&)>& before_done) {
     std::vector<int> act;
     for (int i = 0; i < wf->max_frames; ++i)
         if (wf->f[i].state == F_ACTIVE) act.push_back(i);
@@ -338,6 +340,10 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
                                      hipMemcpyDeviceToHost, st));
     }
     RTX_CHECK_HIP(hipGetLastError());
+    if (before_done && !fin.empty()) {  // the caller's copies of the finishing frames, covered by the step's event
+        int32_t e = before_done(st, fin);
+        if (e) return e;
+    }
     RTX_CHECK_HIP(hipEventRecord(sl.ev, st));
     sl.step = wf->steps;
     sl.finishing = fin;
@@ -362,14 +368,14 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
 
 int32_t rtx_wavefront_step(rtx_wavefront* wf, void* stream, int32_t* pending) {
     RTX_REQUIRE(wf, RTX_EINVAL, "rtx_wavefront_step: null");
-    return rtx_wavefront_step_impl(wf, wf->ctx->pick(stream), pending, nullptr);
+    return rtx_wavefront_step_impl(wf, wf->ctx->pick(stream), pending, nullptr, nullptr);
 }
 
 int32_t rtx_wavefront_run(rtx_wavefront* wf, void* stream) {
     RTX_REQUIRE(wf, RTX_EINVAL, "rtx_wavefront_run: null");
     hipStream_t st = wf->ctx->pick(stream);
     for (int32_t pending = 1; pending > 0;) {
-        int32_t e = rtx_wavefront_step_impl(wf, st, &pending, nullptr);
+        int32_t e = rtx_wavefront_step_impl(wf, st, &pending, nullptr, nullptr);
         if (e) return e;
     }
     return RTX_OK;

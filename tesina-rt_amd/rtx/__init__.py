@@ -57,7 +57,8 @@ SERVER_MAX_CLASSES = 4
 class ServerParams(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_bounces", C.c_int32), ("classes", C.c_int32),
                 ("class_w", C.c_int32 * SERVER_MAX_CLASSES), ("class_h", C.c_int32 * SERVER_MAX_CLASSES),
-                ("hbm_budget", C.c_uint64), ("plain", C.c_int32), ("queue_cap", C.c_int32)]
+                ("hbm_budget", C.c_uint64), ("plain", C.c_int32), ("queue_cap", C.c_int32),
+                ("arena_bytes", C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -977,17 +978,22 @@ class Wavefront:
 
 class Server:
     """Request server over resident scenes (rtx_server_*): a native
-    dispatcher thread starts each request at its arrival time on one of
-    `workers` streams and records its device completion time."""
+    dispatcher thread admits requests at their arrival time (first come
+    first served, by arena bytes, at most `workers` in flight) and renders
+    them with the wavefront renderer, every step advancing all requests in
+    flight by one bounce; it records each request's device completion time.
+    plain=True: the per-request baseline, one at a time with the plain frame
+    on a renderer per resolution class."""
 
-    def __init__(self, ctx: Context, classes, workers: int = 4, max_bounces: int = 3, hbm_budget: int = 0,
-                 plain: bool = False, queue_cap: int = 0):
-        assert 1 <= len(classes) <= SERVER_MAX_CLASSES
+    def __init__(self, ctx: Context, classes=(), workers: int = 64, max_bounces: int = 3, hbm_budget: int = 0,
+                 plain: bool = False, queue_cap: int = 0, arena_bytes: int = 0):
+        assert len(classes) <= SERVER_MAX_CLASSES
         p = ServerParams()
         p.workers, p.max_bounces, p.classes = int(workers), int(max_bounces), len(classes)
         for i, (w, h) in enumerate(classes):
             p.class_w[i], p.class_h[i] = int(w), int(h)
         p.hbm_budget, p.plain, p.queue_cap = int(hbm_budget), int(bool(plain)), int(queue_cap)
+        p.arena_bytes = int(arena_bytes)
         self.h = C.c_void_p()
         _check(lib().rtx_server_create(ctx.h, C.byref(p), C.byref(self.h)), "rtx_server_create")
 

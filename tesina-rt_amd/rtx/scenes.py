@@ -232,6 +232,93 @@ def config_c3(seed: int = SEED_BASE + 3, n: int = 724, res=(1280, 720)) -> Scene
                      meta=dict(name="C3", seed=seed))
 
 
+def _mixed_scene(seed: int, i: int, tris: float) -> SceneData:
+    """Scene i of a C4/C5 set: sphere sets (5000-triangle UV spheres) at even
+    i, heightfields displaced by a static wave field at odd i."""
+    if i % 2 == 0:
+        return config_c2(seed=seed + 16 * i, n_spheres=max(2, int(round(tris / 5000))), res=(256, 256))
+    sd = config_c3(seed=seed + 16 * i, n=max(8, int(round(np.sqrt(tris / 2.0))) + 1), res=(256, 256))
+    sd.pos[:, 1] = displace_host(sd, 0.5)[:, 1]
+    return sd
+
+
+def bounding_sphere(sd: SceneData):
+    """Centre and radius of the scene's axis-aligned box."""
+    lo, hi = sd.pos.min(0), sd.pos.max(0)
+    return 0.5 * (lo + hi), 0.5 * float(np.linalg.norm(hi - lo))
+
+
+def orbit_camera(sd: SceneData, direction, width: int, height: int, sphere=None):
+    """Camera on a sphere of 2.5x the scene's bounding radius, looking at its
+    centre (C4 frames, C5 requests); `sphere` = bounding_sphere(sd) when
+    already known."""
+    from . import camera_set
+    c, r = sphere if sphere is not None else bounding_sphere(sd)
+    d = np.asarray(direction, np.float64)
+    d = d / np.linalg.norm(d)
+    up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
+    return camera_set(c + 2.5 * r * d, c, up, np.pi / 4, width / height)
+
+
+@dataclass
+class BatchFrame:
+    sd: SceneData
+    width: int
+    height: int
+    bounces: int
+    direction: np.ndarray
+    ray_capacity: int
+
+
+def config_c4(seed: int = SEED_BASE + 4, n_scenes: int = 64) -> list:
+    """C4 batch: n_scenes scenes with T ~ logU[10k, 2M] (sphere sets and
+    heightfields alternating), one frame each at a resolution uniform over
+    256^2, 512^2, 1280x720 and 1920x1080, depth uniform over 1..4, a camera on
+    a random direction; secondary bundles of 2 x pixels + 2 rays."""
+    rng = Rng(seed)
+    tris = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(2e6)))
+    pick = np.random.default_rng(4)
+    res_set = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    out = []
+    for i, ti in enumerate(tris):
+        sd = _mixed_scene(seed, i, ti)
+        w, h = res_set[int(pick.integers(0, 4))]
+        depth = int(pick.integers(1, 5))
+        out.append(BatchFrame(sd, w, h, depth, pick.normal(size=3), 2 * w * h + 2))
+    return out
+
+
+def config_c5(seed: int = SEED_BASE + 5, n_scenes: int = 12, total_tris: int = 20_000_000) -> list:
+    """C5 resident scenes: T ~ logU[10k, 4M] per scene, rescaled so that the
+    sum is about 20M triangles (sphere sets and heightfields alternating)."""
+    rng = Rng(seed)
+    t = np.exp(rng.uniform(n_scenes, np.log(1e4), np.log(4e6)))
+    t = t * (total_tris / t.sum())
+    return [_mixed_scene(seed, i, ti) for i, ti in enumerate(t)]
+
+
+def c5_requests(sds, rate: float, n: int, seed: int, start_ns: int = 0):
+    """A C5 request stream: Poisson arrivals at `rate` per second from
+    start_ns (add the submission time to arrival_ns before submitting),
+    scenes Zipf(1.1) over the resident set, resolution uniform over 256^2,
+    512^2 and 1280x720, depth uniform over 1..3, a camera on a random
+    direction.  Returns request dicts for rtx.Server.submit (scene = index)."""
+    spheres = [bounding_sphere(sd) for sd in sds]
+    rng = np.random.default_rng(seed)
+    classes = [(256, 256), (512, 512), (1280, 720)]
+    zipf = 1.0 / np.arange(1, len(sds) + 1) ** 1.1
+    zipf /= zipf.sum()
+    arrivals = start_ns + (np.cumsum(rng.exponential(1.0 / rate, n)) * 1e9).astype(np.int64)
+    reqs = []
+    for i in range(n):
+        k = int(rng.choice(len(sds), p=zipf))
+        w, h = classes[int(rng.integers(0, 3))]
+        cam = orbit_camera(sds[k], rng.normal(size=3), w, h, spheres[k])
+        reqs.append(dict(scene=k, width=w, height=h, bounces=int(rng.integers(1, 4)), camera=cam,
+                         arrival_ns=int(arrivals[i]), tag=i))
+    return reqs
+
+
 def displace_host(sd: SceneData, t: float) -> np.ndarray:
     """Host copy of the wave displacement (float32, same formula as the
     device kernel) — used only to stage positions for update_positions."""
