@@ -531,7 +531,7 @@ def bench_c3(ctx, frames: int):
     return out
 
 
-def bench_c5(ctx, rates, max_requests: int, workers: int = 64, arena_bytes: int = 16 << 30):
+def bench_c5(ctx, rates, max_requests: int, workers: int = 128, arena_bytes: int = 32 << 30, groups: int = 2):
     """C5 serving: 12 resident scenes (~20M triangles), Poisson arrivals per
     rate, Zipf(1.1) scene popularity, resolution uniform over 256^2, 512^2 and
     1280x720, depth uniform over 1..3, camera on a sphere of 2.5x the scene
@@ -557,10 +557,11 @@ def bench_c5(ctx, rates, max_requests: int, workers: int = 64, arena_bytes: int 
                        "(sphere sets and heightfields), Poisson arrivals, Zipf(1.1) scenes, resolution "
                        "U{256^2, 512^2, 1280x720}, depth U{1..3}, camera on a sphere of 2.5x the scene radius",
            "triangles": [int(sd.T) for sd in sds], "requests_in_flight_max": workers, "arena_bytes": arena_bytes,
+           "groups": groups,
            "rates": list(rates), "server": [], "plain": []}
     for mode in ("server", "plain"):
         srv = rtx.Server(ctx, classes, workers=workers, max_bounces=3, plain=(mode == "plain"),
-                         arena_bytes=arena_bytes)
+                         arena_bytes=arena_bytes, groups=groups)
         ids = [srv.add_scene(sc) for sc in scs]
         wb, sb = srv.memory()
         res["request_memory_bytes" if mode == "server" else "plain_renderer_bytes"] = wb

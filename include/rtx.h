@@ -180,6 +180,10 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    for the wave's slowest walk of each root.
  *   "wf_grain":      wavefront renderer: consecutive 64-ray chunks a
  *                    traversal wave takes per queue atomic (1..64).
+ *   "wf_sched":      wavefront closest-hit and shadow launches: 0 one 64-ray
+ *                    chunk per wave in launch order (the faster for one large
+ *                    frame); 1 (default) persistent waves over per-XCD queues
+ *                    (the faster for many frames).
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */
@@ -551,6 +555,10 @@ typedef struct rtx_server_params {
     int32_t plain;            /* 1: baseline, one request at a time with the plain frame */
     int32_t queue_cap;        /* submissions held at most (0 = 65536) */
     uint64_t arena_bytes;     /* request arena (part of the budget); 0 = half the budget */
+    int32_t groups;           /* wavefront renderers on streams of their own, sharing the arena and the
+                                 requests in flight (1..8); 0 = 2 */
+    int32_t steps_ahead;      /* wavefront steps a group keeps enqueued (1..4): fewer, the sooner a new
+                                 request joins; more, the fewer idle gaps between steps; 0 = 2 */
 } rtx_server_params;
 
 typedef struct rtx_request {

@@ -279,6 +279,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "inst_lane_roots") {
         RTX_REQUIRE(value >= 0 && value <= 2, RTX_EINVAL, "rtx_tune_set: inst_lane_roots must be 0, 1 or 2");
         ctx->tune.inst_lane_roots = value;
+    } else if (k == "wf_sched") {
+        RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: wf_sched must be 0 or 1");
+        ctx->tune.wf_sched = value;
     } else if (k == "wf_grain") {
         RTX_REQUIRE(value >= 1 && value <= 64, RTX_EINVAL, "rtx_tune_set: wf_grain must be in [1, 64]");
         ctx->tune.wf_grain = value;
@@ -300,6 +303,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "side_stream") return ctx->tune.side_stream;
     if (k == "inst_lane_roots") return ctx->tune.inst_lane_roots;
     if (k == "wf_grain") return ctx->tune.wf_grain;
+    if (k == "wf_sched") return ctx->tune.wf_sched;
     return -1;
 }
 

@@ -74,6 +74,7 @@ struct rtx_tuning {
     int64_t inst_lane_roots = 2;  // instances, single-node forms: 1 each lane walks its own roots in one loop,
                                   // 2 the same over the roots whose world box its ray meets, 0 wave-uniform
     int64_t wf_grain = 1;         // wavefront traversal: 64-ray chunks a wave takes per queue atomic
+    int64_t wf_sched = 1;         // wavefront traversal: 0 one chunk per wave in launch order, 1 persistent waves
 };
 
 struct rtx_ctx {
@@ -413,19 +414,32 @@ struct WfSeg {
     uint32_t resolve;  // last bounce: the framebuffer is resolved to RGBA8
     uint32_t fin;      // resolving frames: slot of its counters in the step's output
 };
-// Counters of one step in flight (zeroed by the step's first kernel): the
-// trace and shadow launches' queues (8 counters each, 64 bytes apart) and the
-// secondary generation's ticket.
-enum { WF_Q_TRACE = 0, WF_Q_SHADOW = 128, WF_Q_SEC = 256, WF_Q_COUNT = 272 };
+// Counters of one step in flight, set by the step's first kernel: the trace
+// and shadow launches' queues (8 counters each, 64 bytes apart) and the
+// secondary generation's ticket (zeroed), and the trace launch's table of
+// chunk ends per segment with its total (in-order trace form).
+enum {
+    WF_Q_TRACE = 0,
+    WF_Q_SHADOW = 128,
+    WF_Q_SEC = 256,
+    WF_Q_ZERO = 272,  // words [0, WF_Q_ZERO) are zeroed
+    WF_TAB = 272,
+    WF_TOTAL = WF_TAB + RTX_WF_MAX_SEGS,
+    WF_STAB = WF_TOTAL + 16,  // the shadow launch's table (set by the plan kernel after the trace)
+    WF_STOTAL = WF_STAB + RTX_WF_MAX_SEGS,
+    WF_Q_COUNT = WF_STOTAL + 16
+};
 
 // Launchers of the segmented stage kernels (trace.hip, shade.hip,
 // secondary.hip).  `any_*` says whether some segment needs the stage.
 void rtx_wf_init(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, hipStream_t st);
 void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, hipStream_t st);
+// sched 1: persistent waves over per-XCD queues; 0: one chunk per wave in
+// launch order over a grid of `max_chunks` (an upper bound of the chunks).
 void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                  int cus, hipStream_t st);
+                  int sched, uint64_t max_chunks, int cus, hipStream_t st);
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                   int cus, hipStream_t st);
+                   int sched, uint64_t max_chunks, int cus, hipStream_t st);
 void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st);
 void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st);
 // Also copies the counters of each resolving frame to fin_out[S.fin] (4 x u64).

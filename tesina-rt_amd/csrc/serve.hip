@@ -9,10 +9,13 @@
 // framebuffer (rtx_wavefront_frame_bytes) fit the request arena and fewer
 // than `workers` requests are in flight; otherwise it waits at the head, and
 // one larger than the whole arena is refused.  While requests are in flight
-// the dispatcher keeps up to two wavefront steps enqueued: each step
-// advances every admitted request by one bounce in shared launches, so
-// requests that arrive while others run join at the next step, whatever
-// their scene, size or depth.  A request's latency is done_ns - arrival_ns,
+// the dispatcher keeps up to `steps_ahead` wavefront steps enqueued per group: each
+// step advances every request of the group by one bounce in shared
+// launches, so requests that arrive while others run join at the next step,
+// whatever their scene, size or depth.  Requests are spread over `groups`
+// wavefront renderers, each stepping on a stream of its own with its share
+// of the arena and of the requests in flight, so that the launch tails of
+// one group overlap the work of the others.  A request's latency is done_ns - arrival_ns,
 // where done_ns is when the dispatcher observed the event of its last step
 // (accumulators copied out included).
 //
@@ -43,12 +46,11 @@ struct Later {
 // A request between its arrival and its completion.
 struct Live {
     rtx_request req{};
+    int32_t group = -1;
     int32_t frame = -1;
     int64_t start_ns = 0;
     void* staging = nullptr;  // pinned copy of the accumulators (requests with out_accum)
 };
-
-constexpr int64_t STEPS_AHEAD = 2;  // wavefront steps kept enqueued
 
 }  // namespace
 
@@ -57,8 +59,10 @@ struct rtx_server {
     rtx_server_params p{};
     // plain mode
     rtx_renderer* plain_r[RTX_SERVER_MAX_CLASSES] = {};
-    // wavefront mode
-    rtx_wavefront* wf = nullptr;
+    // wavefront mode: groups of requests, each a wavefront renderer stepping
+    // on a stream of its own (one group's launch tails overlap another's work)
+    std::vector<rtx_wavefront*> wf;
+    std::vector<hipStream_t> gstream;
     hipStream_t stream = nullptr;
     std::vector<rtx_scene*> scenes;
     uint64_t workers_bytes = 0, scenes_bytes = 0, budget = 0;
@@ -151,13 +155,16 @@ void rtx_server::run_plain() {
 }
 
 void rtx_server::run_wavefront() {
+    const int G = (int)wf.size();
     rtxwf::Admission adm;
     std::unordered_map<uint64_t, Live> live;  // by the server's request sequence number
     uint64_t seq = 0;
     size_t inflight = 0;
-    int32_t active = 0;  // admitted requests with steps still to enqueue
+    std::vector<size_t> g_inflight(G, 0);
+    std::vector<int32_t> g_active(G, 0);  // admitted requests with steps still to enqueue, per group
     std::vector<rtx_completion> done;
-    const uint64_t arena = rtx_wf_arena_size(wf);
+    const uint64_t arena = rtx_wf_arena_size(wf[0]);
+    const int64_t steps_ahead = p.steps_ahead > 0 ? p.steps_ahead : 2;
     auto finish = [&](const rtx_completion& c) { done.push_back(c); };
     for (;;) {
         const int64_t now = rtx_now_ns();
@@ -183,7 +190,8 @@ void rtx_server::run_wavefront() {
             }
             stopping = stop && pending.empty();
         }
-        // admission: first come first served, by arena bytes
+        // admission: first come first served, by arena bytes, into the group
+        // with the fewest requests in flight that has room
         std::vector<uint64_t> admitted, rejected;
         auto try_add = [&](const rtxwf::Admission::Item& it) {
             Live& L = live.at(it.id);
@@ -192,30 +200,38 @@ void rtx_server::run_wavefront() {
                 std::lock_guard<std::mutex> lk(mu);
                 sc = scenes[L.req.scene];
             }
-            int32_t fr = -1;
-            const int32_t e = rtx_wavefront_add(wf, sc, &L.req.camera, L.req.width, L.req.height, L.req.bounces, 0,
-                                                it.id, &fr);
-            if (e == RTX_ENOMEM) return false;  // no room now: wait at the head
-            if (e) {
-                rtx_completion c = completion_of(L.req, e);
-                c.start_ns = c.done_ns = rtx_now_ns();
-                finish(c);
-                live.erase(it.id);
-                return true;  // leaves the queue (not in flight)
+            std::vector<int> order(G);
+            for (int g = 0; g < G; ++g) order[g] = g;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g_inflight[a] < g_inflight[b]; });
+            for (int g : order) {
+                if (g_inflight[g] >= (size_t)p.workers / G) continue;
+                int32_t fr = -1;
+                const int32_t e = rtx_wavefront_add(wf[g], sc, &L.req.camera, L.req.width, L.req.height,
+                                                    L.req.bounces, 0, it.id, &fr);
+                if (e == RTX_ENOMEM) continue;  // this group has no room now
+                if (e) {
+                    rtx_completion c = completion_of(L.req, e);
+                    c.start_ns = c.done_ns = rtx_now_ns();
+                    finish(c);
+                    live.erase(it.id);
+                    return true;  // leaves the queue (not in flight)
+                }
+                L.group = g;
+                L.frame = fr;
+                L.start_ns = rtx_now_ns();
+                ++g_inflight[g];
+                ++g_active[g];
+                if (L.req.out_accum &&
+                    hipHostMalloc(&L.staging, (size_t)L.req.width * L.req.height * 12, hipHostMallocDefault) !=
+                        hipSuccess)
+                    L.staging = nullptr;
+                return true;
             }
-            L.frame = fr;
-            L.start_ns = rtx_now_ns();
-            if (L.req.out_accum &&
-                hipHostMalloc(&L.staging, (size_t)L.req.width * L.req.height * 12, hipHostMallocDefault) != hipSuccess)
-                L.staging = nullptr;
-            return true;
+            return false;  // no group has room now: wait at the head
         };
         adm.admit(arena, inflight, (size_t)p.workers, try_add, admitted, rejected);
         for (uint64_t id : admitted)
-            if (live.count(id)) {
-                ++inflight;
-                ++active;
-            }
+            if (live.count(id)) ++inflight;
         for (uint64_t id : rejected) {
             Live& L = live.at(id);
             rtx_completion c = completion_of(L.req, RTX_ENOMEM);
@@ -223,39 +239,45 @@ void rtx_server::run_wavefront() {
             finish(c);
             live.erase(id);
         }
-        // steps: keep up to STEPS_AHEAD enqueued while requests need them
-        if (active > 0 && rtx_wf_steps_in_flight(wf) < STEPS_AHEAD) {
+        // steps: each group keeps up to STEPS_AHEAD enqueued while its requests need them
+        for (int g = 0; g < G; ++g) {
+            if (g_active[g] <= 0 || rtx_wf_steps_in_flight(wf[g]) >= steps_ahead) continue;
             auto copy_out = [&](hipStream_t st, const std::vector<int>& fin) -> int32_t {
                 for (auto& kv : live) {
                     Live& L = kv.second;
-                    if (!L.staging) continue;
+                    if (!L.staging || L.group != g) continue;
                     if (std::find(fin.begin(), fin.end(), L.frame) == fin.end()) continue;
-                    RTX_CHECK_HIP(hipMemcpyAsync(L.staging, rtx_wf_accum_dev(wf, L.frame),
+                    RTX_CHECK_HIP(hipMemcpyAsync(L.staging, rtx_wf_accum_dev(wf[g], L.frame),
                                                  (size_t)L.req.width * L.req.height * 12, hipMemcpyDeviceToHost, st));
                 }
                 return RTX_OK;
             };
             int32_t pend = 0;
-            const int32_t e = rtx_wavefront_step_impl(wf, stream, &pend, nullptr, copy_out);
-            if (e) {  // a failed step fails every request in flight
-                for (auto& kv : live)
-                    if (kv.second.frame >= 0) {
-                        rtx_completion c = completion_of(kv.second.req, e);
-                        c.start_ns = kv.second.start_ns;
+            const int32_t e = rtx_wavefront_step_impl(wf[g], gstream[g], &pend, nullptr, copy_out);
+            if (e) {  // a failed step fails every request of the group
+                for (auto it = live.begin(); it != live.end();) {
+                    if (it->second.group == g) {
+                        rtx_completion c = completion_of(it->second.req, e);
+                        c.start_ns = it->second.start_ns;
                         c.done_ns = rtx_now_ns();
                         finish(c);
+                        --inflight;
+                        it = live.erase(it);
+                    } else {
+                        ++it;
                     }
-                live.clear();
+                }
+                g_inflight[g] = 0;
                 pend = 0;
-                inflight = 0;
             }
-            active = pend;
+            g_active[g] = pend;
         }
         // completions
-        int32_t fr[64];
-        uint64_t tags[64];
-        uint32_t nf = 0;
-        if (rtx_wavefront_poll(wf, fr, tags, 64, &nf) == RTX_OK) {
+        for (int g = 0; g < G; ++g) {
+            int32_t fr[64];
+            uint64_t tags[64];
+            uint32_t nf = 0;
+            if (rtx_wavefront_poll(wf[g], fr, tags, 64, &nf) != RTX_OK) continue;
             for (uint32_t i = 0; i < nf; ++i) {
                 auto it = live.find(tags[i]);
                 if (it == live.end()) continue;
@@ -264,10 +286,10 @@ void rtx_server::run_wavefront() {
                 c.start_ns = L.start_ns;
                 c.done_ns = rtx_now_ns();
                 rtx_frame_stats fs{};
-                c.status = rtx_wavefront_frame_stats(wf, L.frame, &fs);
+                c.status = rtx_wavefront_frame_stats(wf[g], L.frame, &fs);
                 c.rays = fs.primary_rays + fs.secondary_rays;
                 c.shadow_rays = fs.shadow_rays;
-                c.worker = L.frame;
+                c.worker = g * RTX_WF_MAX_FRAMES + L.frame;
                 if (L.req.out_accum) {
                     if (L.staging)
                         memcpy(L.req.out_accum, L.staging, (size_t)L.req.width * L.req.height * 12);
@@ -275,8 +297,9 @@ void rtx_server::run_wavefront() {
                         c.status = RTX_ENOMEM;
                 }
                 if (L.staging) (void)hipHostFree(L.staging);
-                (void)rtx_wavefront_release(wf, L.frame);
+                (void)rtx_wavefront_release(wf[g], L.frame);
                 --inflight;
+                --g_inflight[g];
                 finish(c);
                 live.erase(it);
             }
@@ -323,7 +346,9 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
     srv->budget = p->hbm_budget ? p->hbm_budget : (uint64_t)(0.9 * (double)free_b);
     const uint64_t before = ctx->bytes_in_use;
     int32_t e = RTX_OK;
-    if (hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking) != hipSuccess) {
+    // (each stream takes one of the process's hardware queues: the plain
+    // server renders on one, the wavefront groups on one each)
+    if (srv->p.plain && hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking) != hipSuccess) {
         srv->stream = nullptr;
         rtx_set_error("rtx_server_create: stream creation failed");
         e = RTX_EDEVICE;
@@ -337,14 +362,29 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
             e = rtx_renderer_create(ctx, &rp, &srv->plain_r[c]);
         }
     } else if (!e) {
-        rtx_wavefront_params wp{};
-        wp.arena_bytes = p->arena_bytes ? p->arena_bytes : srv->budget / 2;
-        wp.max_frames = p->workers;
-        if (wp.arena_bytes >= srv->budget) {
+        const int G = p->groups > 0 ? p->groups : 2;
+        const uint64_t arena = p->arena_bytes ? p->arena_bytes : srv->budget / 2;
+        if (G > 8 || p->workers < G || p->steps_ahead < 0 || p->steps_ahead > 4) {
+            rtx_set_error("rtx_server_create: groups must be in [1, 8] and at most workers, steps_ahead in [0, 4]");
+            e = RTX_EINVAL;
+        } else if (arena >= srv->budget) {
             rtx_set_error("rtx_server_create: the request arena exceeds the HBM budget");
             e = RTX_ENOMEM;
-        } else {
-            e = rtx_wavefront_create(ctx, &wp, &srv->wf);
+        }
+        for (int g = 0; g < G && !e; ++g) {
+            rtx_wavefront_params wp{};
+            wp.arena_bytes = arena / G;
+            wp.max_frames = (p->workers + G - 1) / G;
+            rtx_wavefront* w = nullptr;
+            hipStream_t gs = nullptr;
+            if ((e = rtx_wavefront_create(ctx, &wp, &w))) break;
+            srv->wf.push_back(w);
+            if (hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) != hipSuccess) {
+                rtx_set_error("rtx_server_create: stream creation failed");
+                e = RTX_EDEVICE;
+                break;
+            }
+            srv->gstream.push_back(gs);
         }
     }
     if (!e && hipDeviceSynchronize() != hipSuccess) e = RTX_EDEVICE;
@@ -380,7 +420,11 @@ int32_t rtx_server_destroy(rtx_server* srv) {
     if (srv->stream) (void)hipStreamSynchronize(srv->stream);
     for (auto* r : srv->plain_r)
         if (r) rtx_renderer_destroy(r);
-    if (srv->wf) rtx_wavefront_destroy(srv->wf);
+    for (auto* w : srv->wf) rtx_wavefront_destroy(w);
+    for (auto gs : srv->gstream) {
+        (void)hipStreamSynchronize(gs);
+        (void)hipStreamDestroy(gs);
+    }
     if (srv->stream) (void)hipStreamDestroy(srv->stream);
     delete srv;
     return RTX_OK;

@@ -1842,10 +1842,16 @@ uint32_t trace_grid(rtx_ctx* c, uint64_t count, uint64_t cap, int TB) {
 // added to the frame's secondary count as k_sec_finalize does), an empty hit
 // list and, when this bounce generates, the next bounce's scan total and
 // drop offset.
-__global__ __launch_bounds__(256) void k_wf_init(const WfSeg* __restrict__ segs, uint32_t nseg,
-                                                 uint32_t* __restrict__ ctr) {
-    for (uint32_t i = threadIdx.x; i < WF_Q_COUNT; i += blockDim.x) ctr[i] = 0u;
-    for (uint32_t i = threadIdx.x; i < nseg; i += blockDim.x) {
+// Also the trace launch's chunk table (64-ray chunks per segment, inclusive
+// prefix, and the total) for the in-order trace form.  One block of
+// RTX_WF_MAX_SEGS threads, thread i for segment i.
+__global__ __launch_bounds__(RTX_WF_MAX_SEGS) void k_wf_init(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                             uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t s_w[RTX_WF_MAX_SEGS / RTX_WAVE];
+    for (uint32_t i = threadIdx.x; i < WF_Q_ZERO; i += blockDim.x) ctr[i] = 0u;
+    const uint32_t i = threadIdx.x;
+    uint32_t chunks = 0;
+    if (i < nseg) {
         const WfSeg& S = segs[i];
         if (S.primary) {
             S.count[0] = (uint32_t)S.w * (uint32_t)S.h;
@@ -1864,7 +1870,26 @@ __global__ __launch_bounds__(256) void k_wf_init(const WfSeg* __restrict__ segs,
             S.o_count[1] = 0u;
             S.o_count[2] = 0xffffffffu;
         }
+        chunks = (S.count[0] + RTX_WAVE - 1) / RTX_WAVE;
     }
+    // inclusive scan of the chunk counts over the block
+    const uint32_t lane = threadIdx.x & (RTX_WAVE - 1), w = threadIdx.x / RTX_WAVE;
+    uint32_t v = chunks;
+#pragma unroll
+    for (int off = 1; off < RTX_WAVE; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, RTX_WAVE);
+        if (lane >= (uint32_t)off) v += u;
+    }
+    if (lane == RTX_WAVE - 1) s_w[w] = v;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < RTX_WF_MAX_SEGS / RTX_WAVE; ++k) {
+        before += k < (int)w ? s_w[k] : 0u;
+        total += s_w[k];
+    }
+    if (i < nseg) ctr[WF_TAB + i] = before + v;
+    if (i == 0) ctr[WF_TOTAL] = total;
 }
 
 // Primary rays of the frames that start this step (one sample per pixel, the
@@ -1895,6 +1920,43 @@ __global__ __launch_bounds__(256) void k_wf_raygen(const WfSeg* __restrict__ seg
 // Closest hit over every frame's rays: persistent waves take 64-ray chunks
 // (ChunkSched).  A chunk that starts a secondary-generation tile also clears
 // that tile's look-back word for this step's generation.
+// One 64-ray chunk (chunk cl of segment S) of the closest-hit launch; the
+// whole wave calls it.  A chunk that starts a secondary-generation tile also
+// clears that tile's look-back word for this step's generation.
+template <int VAR, class Stack>
+__device__ __forceinline__ void wf_trace_chunk(const WfSeg& S, uint32_t cl, Stack& stk, uint32_t coop,
+                                               uint32_t lane) {
+    constexpr uint32_t PER_TILE = WF_GEN_TILE / RTX_WAVE;
+    if (S.gen && cl % PER_TILE == 0u && lane == 0) S.status[cl / PER_TILE] = 0u;
+    const uint64_t rr = (uint64_t)cl * RTX_WAVE + lane;
+    const bool active = rr < (uint64_t)S.count[0];
+    Closest best{0.f, 0.f, 0.f, -1};
+    RayC rc{};
+    if (active) {
+        const float4 o = S.ori4[rr], dd = S.dir4[rr];
+        rc = make_ray(xyz(o), xyz(dd), o.w);
+        best.t = dd.w;
+    }
+    closest_var<VAR>(S.node, S.delta, S.tri, rc, best, stk, coop, active);  // whole wave
+    const bool hit = active && best.id >= 0;
+    const uint64_t hb = ballot(hit);
+    uint32_t wbase = 0;
+    if (hb) {
+        if (lane == (uint32_t)__builtin_ctzll(hb))
+            wbase = atomicAdd(S.list_count, (uint32_t)__builtin_popcountll(hb));
+        wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
+    }
+    if (!active) return;
+    if (!hit) {
+        write_miss_record(rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+        return;
+    }
+    S.list[wbase + lanes_below(hb)] = (uint32_t)rr;
+    write_hit_record(S.tri, S.nrm, S.uv, rc, best, rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+}
+
+// Closest hit over every frame's rays, persistent form: waves take 64-ray
+// chunks (ChunkSched).
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                  uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
@@ -1905,43 +1967,79 @@ __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs,
     ChunkSched sch;
     sch.init(wc.total, queue, grain);
     const uint32_t lane = threadIdx.x & (RTX_WAVE - 1);
-    constexpr uint32_t PER_TILE = WF_GEN_TILE / RTX_WAVE;
     for (uint32_t c = 0; sch.next(c);) {
         uint32_t c0 = 0;
         const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
-        const WfSeg& S = segs[sg];
-        const uint32_t cl = c - c0;
-        if (S.gen && cl % PER_TILE == 0u && lane == 0) S.status[cl / PER_TILE] = 0u;
-        const uint64_t rr = (uint64_t)cl * RTX_WAVE + lane;
-        const bool active = rr < (uint64_t)S.count[0];
-        Closest best{0.f, 0.f, 0.f, -1};
-        RayC rc{};
-        if (active) {
-            const float4 o = S.ori4[rr], dd = S.dir4[rr];
-            rc = make_ray(xyz(o), xyz(dd), o.w);
-            best.t = dd.w;
-        }
-        closest_var<VAR>(S.node, S.delta, S.tri, rc, best, stk, coop, active);  // whole wave
-        const bool hit = active && best.id >= 0;
-        const uint64_t hb = ballot(hit);
-        uint32_t wbase = 0;
-        if (hb) {
-            if (lane == (uint32_t)__builtin_ctzll(hb))
-                wbase = atomicAdd(S.list_count, (uint32_t)__builtin_popcountll(hb));
-            wbase = __shfl(wbase, __builtin_ctzll(hb), RTX_WAVE);
-        }
-        if (!active) continue;
-        if (!hit) {
-            write_miss_record(rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
-            continue;
-        }
-        S.list[wbase + lanes_below(hb)] = (uint32_t)rr;
-        write_hit_record(S.tri, S.nrm, S.uv, rc, best, rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+        wf_trace_chunk<VAR>(segs[sg], c - c0, stk, coop, lane);
     }
 }
 
-// Shadow rays of every frame's (hit, light) pairs, as k_shadow, in 64-pair
-// chunks (ChunkSched).
+// The same, one chunk per wave in launch order (chunk = global wave index)
+// over a grid sized by the frames' ray capacities: the dispatcher hands out
+// chunks in order, so the waves in flight always cover neighbouring rays,
+// with no queue atomics; waves past the step's chunk total leave at once.
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_wf_trace_ord(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                     const uint32_t* __restrict__ ctr, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    const uint32_t total = ctr[WF_TOTAL];
+    const uint32_t c = blockIdx.x * (TB / RTX_WAVE) + threadIdx.x / RTX_WAVE;
+    if (c >= total) return;  // whole waves (no block barrier follows)
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    WaveChunks<WF_RAYS> wc;
+    wc.load(ctr + WF_TAB, nseg, total);
+    uint32_t c0 = 0;
+    const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
+    wf_trace_chunk<VAR>(segs[sg], c - c0, stk, coop, threadIdx.x & (RTX_WAVE - 1));
+}
+
+// One 64-pair chunk (chunk cl of segment S) of the shadow launch: the
+// (hit, light) pairs of k_shadow, light-major (a wave traces one light's
+// rays from neighbouring hits).  The whole wave calls it.
+template <int VAR, class Stack>
+__device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Stack& stk, uint32_t coop,
+                                                uint32_t lane) {
+    const int nl = S.lights->count;
+    const uint64_t nh = S.list_count[0];
+    const uint64_t it = (uint64_t)cl * RTX_WAVE + lane;
+    const bool valid = it < nh * (uint64_t)nl;
+    uint32_t rr = 0;
+    int l = 0;
+    RayC rc{};
+    float tmax = 1e37f;
+    bool shadowed = false, trace = false;
+    if (valid) {
+        rr = S.list[it % nh];
+        l = (int)(it / nh);
+        const rtx_light& L = S.lights->l[l];
+        const v3 p = xyz(S.p4[rr]);
+        v3 dir = mk3(0.f, 0.f, 0.f);
+        trace = true;
+        if (L.type == RTX_LIGHT_DIR) {
+            dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
+        } else if (L.type == RTX_LIGHT_SPOT) {
+            dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
+            if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
+                shadowed = true;
+                trace = false;
+            }
+        } else if (L.type == RTX_LIGHT_POINT) {
+            const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
+            tmax = __builtin_sqrtf(dot(toL, toL));
+            dir = normalize(toL);
+        } else {
+            shadowed = true;
+            trace = false;
+        }
+        if (trace) rc = make_ray(p, dir, 0.01f);
+    }
+    if (any_var<VAR>(S.node, S.delta, S.tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
+    if (valid) set_shadow_bit(&S.flags[rr], l, shadowed);
+    const uint64_t tb = ballot(trace);
+    if (lane == 0 && tb) atomicAdd(&S.stats[1], (unsigned long long)__builtin_popcountll(tb));
+}
+
+// Shadow rays of every frame's pairs, persistent form (ChunkSched).
 template <int VAR, int TB>
 __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                   uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
@@ -1955,54 +2053,59 @@ __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs
     for (uint32_t c = 0; sch.next(c);) {
         uint32_t c0 = 0;
         const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
-        const WfSeg& S = segs[sg];
-        const int nl = S.lights->count;
-        const uint64_t nh = S.list_count[0];
-        // light-major: a wave traces one light's rays from neighbouring hits
-        const uint64_t it = (uint64_t)(c - c0) * RTX_WAVE + lane;
-        const bool valid = it < nh * (uint64_t)nl;
-        uint32_t rr = 0;
-        int l = 0;
-        RayC rc{};
-        float tmax = 1e37f;
-        bool shadowed = false, trace = false;
-        if (valid) {
-            rr = S.list[it % nh];
-            l = (int)(it / nh);
-            const rtx_light& L = S.lights->l[l];
-            const v3 p = xyz(S.p4[rr]);
-            v3 dir = mk3(0.f, 0.f, 0.f);
-            trace = true;
-            if (L.type == RTX_LIGHT_DIR) {
-                dir = -mk3(L.dir[0], L.dir[1], L.dir[2]);
-            } else if (L.type == RTX_LIGHT_SPOT) {
-                dir = normalize(mk3(L.pos[0], L.pos[1], L.pos[2]) - p);
-                if (dot(-dir, mk3(L.dir[0], L.dir[1], L.dir[2])) < L.angle) {
-                    shadowed = true;
-                    trace = false;
-                }
-            } else if (L.type == RTX_LIGHT_POINT) {
-                const v3 toL = mk3(L.pos[0], L.pos[1], L.pos[2]) - p;
-                tmax = __builtin_sqrtf(dot(toL, toL));
-                dir = normalize(toL);
-            } else {
-                shadowed = true;
-                trace = false;
-            }
-            if (trace) rc = make_ray(p, dir, 0.01f);
-        }
-        if (any_var<VAR>(S.node, S.delta, S.tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
-        if (valid) set_shadow_bit(&S.flags[rr], l, shadowed);
-        const uint64_t tb = ballot(trace);
-        if (lane == 0 && tb) atomicAdd(&S.stats[1], (unsigned long long)__builtin_popcountll(tb));
+        wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, lane);
     }
+}
+
+// Between the trace and the in-order shadow launch: the shadow chunk table
+// (64-pair chunks per segment, inclusive prefix, total) from the hit lists
+// the trace left.  One block of RTX_WF_MAX_SEGS threads.
+__global__ __launch_bounds__(RTX_WF_MAX_SEGS) void k_wf_plan_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                                    uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t s_w[RTX_WF_MAX_SEGS / RTX_WAVE];
+    const uint32_t i = threadIdx.x;
+    uint32_t chunks = 0;
+    if (i < nseg) chunks = (uint32_t)((wf_items<WF_PAIRS>(segs[i]) + RTX_WAVE - 1) / RTX_WAVE);
+    const uint32_t lane = threadIdx.x & (RTX_WAVE - 1), w = threadIdx.x / RTX_WAVE;
+    uint32_t v = chunks;
+#pragma unroll
+    for (int off = 1; off < RTX_WAVE; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, RTX_WAVE);
+        if (lane >= (uint32_t)off) v += u;
+    }
+    if (lane == RTX_WAVE - 1) s_w[w] = v;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < RTX_WF_MAX_SEGS / RTX_WAVE; ++k) {
+        before += k < (int)w ? s_w[k] : 0u;
+        total += s_w[k];
+    }
+    if (i < nseg) ctr[WF_STAB + i] = before + v;
+    if (i == 0) ctr[WF_STOTAL] = total;
+}
+
+// Shadow rays, one chunk per wave in launch order (see k_wf_trace_ord).
+template <int VAR, int TB>
+__global__ __launch_bounds__(TB) void k_wf_shadow_ord(const WfSeg* __restrict__ segs, uint32_t nseg,
+                                                      const uint32_t* __restrict__ ctr, uint32_t coop) {
+    __shared__ uint32_t s_id[STACK * TB];
+    const uint32_t total = ctr[WF_STOTAL];
+    const uint32_t c = blockIdx.x * (TB / RTX_WAVE) + threadIdx.x / RTX_WAVE;
+    if (c >= total) return;  // whole waves (no block barrier follows)
+    LdsStack<TB> stk{s_id, threadIdx.x};
+    WaveChunks<WF_PAIRS> wc;
+    wc.load(ctr + WF_STAB, nseg, total);
+    uint32_t c0 = 0;
+    const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
+    wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, threadIdx.x & (RTX_WAVE - 1));
 }
 
 }  // namespace
 
 // ------------------------------------------------- wavefront launchers --
 void rtx_wf_init(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, hipStream_t st) {
-    hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(256), 0, st, segs, nseg, ctr);
+    hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(RTX_WF_MAX_SEGS), 0, st, segs, nseg, ctr);
 }
 
 void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, hipStream_t st) {
@@ -2013,7 +2116,16 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
 
 // Persistent grids: 20 waves per CU, the occupancy the LDS stack allows.
 void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                  int cus, hipStream_t st) {
+                  int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+    if (sched == 0) {
+        const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
+#define RTX_WF_TRACE_ORD(V, B)                                                                                \
+    hipLaunchKernelGGL((k_wf_trace_ord<V, B>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
+                       coop)
+        RTX_DISPATCH_TRAVERSAL(RTX_WF_TRACE_ORD, var, tb);
+#undef RTX_WF_TRACE_ORD
+        return;
+    }
 #define RTX_WF_TRACE(V, B)                                                                                  \
     hipLaunchKernelGGL((k_wf_trace<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
                        ctr + WF_Q_TRACE, coop, grain)
@@ -2022,7 +2134,17 @@ void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int 
 }
 
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                   int cus, hipStream_t st) {
+                   int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+    if (sched == 0) {
+        hipLaunchKernelGGL(k_wf_plan_shadow, dim3(1), dim3(RTX_WF_MAX_SEGS), 0, st, segs, nseg, ctr);
+        const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
+#define RTX_WF_SHADOW_ORD(V, B)                                                                                \
+    hipLaunchKernelGGL((k_wf_shadow_ord<V, B>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
+                       coop)
+        RTX_DISPATCH_TRAVERSAL(RTX_WF_SHADOW_ORD, var, tb);
+#undef RTX_WF_SHADOW_ORD
+        return;
+    }
 #define RTX_WF_SHADOW(V, B)                                                                                  \
     hipLaunchKernelGGL((k_wf_shadow<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
                        ctr + WF_Q_SHADOW, coop, grain)

@@ -280,7 +280,7 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
         if (e) return e;
     }
     WfSeg* T = wf->htab + (size_t)k * RTX_WF_MAX_SEGS;
-    uint64_t px_new = 0, px_done = 0;
+    uint64_t px_new = 0, px_done = 0, max_chunks = 0, max_pair_chunks = 0;
     bool gen = false, deep = false;
     std::vector<int> fin;
     const uint32_t n = (uint32_t)act.size();
@@ -289,6 +289,9 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
         const bool last = fr.next == fr.bounces;
         T[i] = make_seg(wf, fr, last ? (uint32_t)fin.size() : 0u);
         if (fr.next == 0) px_new += fr.L.P;
+        const uint64_t cap_in = rtxwf::bounce_capacity(fr.L.P, fr.next, fr.user_cap);
+        max_chunks += (cap_in + RTX_WAVE - 1) / RTX_WAVE;
+        max_pair_chunks += (cap_in * (uint64_t)fr.scene->host_lights.count + RTX_WAVE - 1) / RTX_WAVE;
         if (last) {
             px_done += fr.L.P;
             fin.push_back(act[i]);
@@ -319,11 +322,13 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     }
     {
         ProfScope ps(c, st, "wf_trace");
-        rtx_wf_trace(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, c->cus, st);
+        rtx_wf_trace(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, (int)c->tune.wf_sched, max_chunks,
+                     c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shadow");
-        rtx_wf_shadow(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, c->cus, st);
+        rtx_wf_shadow(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, (int)c->tune.wf_sched, max_pair_chunks,
+                      c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shade");

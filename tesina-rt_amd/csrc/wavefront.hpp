@@ -95,6 +95,16 @@ struct WaveChunks {
         }
         total = __builtin_amdgcn_readfirstlane(carry);
     }
+    // The table as the step's first kernel left it: ends[i] for i < nseg.
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ ends, uint32_t nseg, uint32_t total_) {
+        const uint32_t lane = lane_id();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t i = (uint32_t)r * RTX_WAVE + lane;
+            end[r] = i < nseg ? ends[i] : total_;
+        }
+        total = total_;
+    }
     // Segment of chunk c < total (wave-uniform) and its first chunk.
     __device__ __forceinline__ uint32_t find(uint32_t c, uint32_t& first) const {
         uint32_t prev = 0;

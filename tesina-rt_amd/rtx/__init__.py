@@ -58,7 +58,7 @@ class ServerParams(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_bounces", C.c_int32), ("classes", C.c_int32),
                 ("class_w", C.c_int32 * SERVER_MAX_CLASSES), ("class_h", C.c_int32 * SERVER_MAX_CLASSES),
                 ("hbm_budget", C.c_uint64), ("plain", C.c_int32), ("queue_cap", C.c_int32),
-                ("arena_bytes", C.c_uint64)]
+                ("arena_bytes", C.c_uint64), ("groups", C.c_int32), ("steps_ahead", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -986,14 +986,15 @@ class Server:
     on a renderer per resolution class."""
 
     def __init__(self, ctx: Context, classes=(), workers: int = 64, max_bounces: int = 3, hbm_budget: int = 0,
-                 plain: bool = False, queue_cap: int = 0, arena_bytes: int = 0):
+                 plain: bool = False, queue_cap: int = 0, arena_bytes: int = 0, groups: int = 0,
+                 steps_ahead: int = 0):
         assert len(classes) <= SERVER_MAX_CLASSES
         p = ServerParams()
         p.workers, p.max_bounces, p.classes = int(workers), int(max_bounces), len(classes)
         for i, (w, h) in enumerate(classes):
             p.class_w[i], p.class_h[i] = int(w), int(h)
         p.hbm_budget, p.plain, p.queue_cap = int(hbm_budget), int(bool(plain)), int(queue_cap)
-        p.arena_bytes = int(arena_bytes)
+        p.arena_bytes, p.groups, p.steps_ahead = int(arena_bytes), int(groups), int(steps_ahead)
         self.h = C.c_void_p()
         _check(lib().rtx_server_create(ctx.h, C.byref(p), C.byref(self.h)), "rtx_server_create")
 

@@ -62,10 +62,10 @@ def _alone(ctx, rtx, s, r):
     return acc, st
 
 
-@pytest.mark.parametrize("plain", [False, True])
-def test_served_images_equal_stand_alone(ctx, scenes2, plain):
+@pytest.mark.parametrize("plain,groups", [(False, 1), (False, 3), (True, 0)])
+def test_served_images_equal_stand_alone(ctx, scenes2, plain, groups):
     import rtx
-    srv = rtx.Server(ctx, CLASSES, workers=16, max_bounces=3, plain=plain, arena_bytes=1 << 30)
+    srv = rtx.Server(ctx, CLASSES, workers=16, max_bounces=3, plain=plain, arena_bytes=1 << 30, groups=groups)
     ids = [srv.add_scene(s) for _, s in scenes2]
     assert ids == [0, 1]
     reqs = _requests(rtx, 24, 7, rtx.now_ns() + 2_000_000)
@@ -132,7 +132,7 @@ def test_tiny_arena_queues_requests(ctx, scenes2):
     # memory, and still equal their stand-alone frames
     import rtx
     one = rtx.Wavefront.frame_bytes(128, 96, 3)
-    srv = rtx.Server(ctx, workers=8, max_bounces=3, arena_bytes=one + one // 2)
+    srv = rtx.Server(ctx, workers=8, max_bounces=3, arena_bytes=one + one // 2, groups=1)
     sid = [srv.add_scene(s) for _, s in scenes2]
     reqs = _requests(rtx, 6, 11, rtx.now_ns() + 1_000_000)
     for i, r in enumerate(reqs):

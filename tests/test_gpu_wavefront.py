@@ -129,14 +129,17 @@ def test_frames_together_equal_stand_alone(ctx, wf_scenes, expected):
         wf.destroy()
 
 
-@pytest.mark.parametrize("variant,block", [(0, 64), (1, 128), (2, 256), (2, 128), (3, 128), (4, 64)])
-def test_every_traversal_form(ctx, wf_scenes, expected, variant, block):
+@pytest.mark.parametrize("variant,block,sched", [(0, 64, 0), (1, 128, 0), (1, 128, 1), (2, 256, 0), (2, 128, 1),
+                                                 (3, 128, 0), (4, 64, 1)])
+def test_every_traversal_form(ctx, wf_scenes, expected, variant, block, sched):
     # the sibling-pair form falls back to the single-node form when a scene
-    # has 32 levels (the chain); the persistent and stackless forms map to it
+    # has 32 levels (the chain); the persistent and stackless forms map to it;
+    # sched: closest hit in launch order (0) or by persistent waves (1)
     rtx = _rtx()
-    old = (ctx.tuned("trace_variant"), ctx.tuned("trace_block"))
+    old = (ctx.tuned("trace_variant"), ctx.tuned("trace_block"), ctx.tuned("wf_sched"))
     ctx.tune("trace_variant", variant)
     ctx.tune("trace_block", block)
+    ctx.tune("wf_sched", sched)
     wf = rtx.Wavefront(ctx, arena_bytes=2 << 30, max_frames=32)
     try:
         ids = []
@@ -150,6 +153,7 @@ def test_every_traversal_form(ctx, wf_scenes, expected, variant, block):
         wf.destroy()
         ctx.tune("trace_variant", old[0])
         ctx.tune("trace_block", old[1])
+        ctx.tune("wf_sched", old[2])
 
 
 def test_frames_joining_at_different_steps(ctx, wf_scenes, expected):
