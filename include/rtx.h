@@ -184,6 +184,11 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    chunk per wave in launch order (the faster for one large
  *                    frame); 1 (default) persistent waves over per-XCD queues
  *                    (the faster for many frames).
+ *   "stack_lds":     traversal stack entries per lane kept in LDS by the
+ *                    single-node and sibling-pair forms (scene and wavefront
+ *                    launches): 32 (the whole MAX_LEVELS stack), 16 or 8 (a
+ *                    ring of the top levels; deeper levels spill to the
+ *                    lane's private memory), trading LDS for resident waves.
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */

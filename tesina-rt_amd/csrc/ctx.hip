@@ -282,6 +282,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "wf_sched") {
         RTX_REQUIRE(value == 0 || value == 1, RTX_EINVAL, "rtx_tune_set: wf_sched must be 0 or 1");
         ctx->tune.wf_sched = value;
+    } else if (k == "stack_lds") {
+        RTX_REQUIRE(value == 8 || value == 16 || value == 32, RTX_EINVAL, "rtx_tune_set: stack_lds must be 8, 16 or 32");
+        ctx->tune.stack_lds = value;
     } else if (k == "wf_grain") {
         RTX_REQUIRE(value >= 1 && value <= 64, RTX_EINVAL, "rtx_tune_set: wf_grain must be in [1, 64]");
         ctx->tune.wf_grain = value;
@@ -304,6 +307,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "inst_lane_roots") return ctx->tune.inst_lane_roots;
     if (k == "wf_grain") return ctx->tune.wf_grain;
     if (k == "wf_sched") return ctx->tune.wf_sched;
+    if (k == "stack_lds") return ctx->tune.stack_lds;
     return -1;
 }
 

@@ -51,6 +51,19 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// p, known to point into global memory, as a pointer the compiler can see is
+// global: loads and stores through it become global_* instead of flat_*
+// instructions (a flat access also counts against the LDS wait counter, so
+// an LDS stack read would wait for outstanding node loads).  For pointers
+// read from device tables, whose address space the compiler cannot infer.
+// (Through an integer: the compiler folds a plain generic->global->generic
+// cast pair back to the generic pointer.)
+template <class T>
+__device__ __forceinline__ T* gptr(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
+
 // Device-coherent atomics for the look-back protocol.  A status word carries
 // its value, so relaxed agent-scope accesses suffice: no release/acquire
 // fence (on gfx950 those write back / invalidate the XCD's whole L2).

@@ -68,6 +68,7 @@ struct rtx_tuning {
                                 // 4 stackless walk by parent pointers
     int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
+    int64_t stack_lds = 32;     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
     int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
@@ -436,10 +437,10 @@ void rtx_wf_init(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, hipStream_t st
 void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, hipStream_t st);
 // sched 1: persistent waves over per-XCD queues; 0: one chunk per wave in
 // launch order over a grid of `max_chunks` (an upper bound of the chunks).
-void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                  int sched, uint64_t max_chunks, int cus, hipStream_t st);
-void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                   int sched, uint64_t max_chunks, int cus, hipStream_t st);
+void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
+                  uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st);
+void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st);
 void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st);
 void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st);
 // Also copies the counters of each resolving frame to fin_out[S.fin] (4 x u64).

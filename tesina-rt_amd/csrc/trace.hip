@@ -13,15 +13,17 @@
 // internal nodes, triangles stored contiguously in leaf order, the stack in
 // LDS ([level][thread], conflict-free), SoA rays and hits.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <type_traits>
 
 #include "devmath.hpp"
 #include "rtx_impl.hpp"
+#include "stack.hpp"
 #include "wavefront.hpp"
 
 namespace {
 
-constexpr int STACK = 32;      // MAX_LEVELS of src/kernel/trace-bvh.cl:330
 
 __device__ __forceinline__ uint64_t ray_count(const uint32_t* dcount, uint64_t hcount) {
     return hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
@@ -142,14 +144,6 @@ __device__ __forceinline__ bool slab(float4 lo4, float4 hi4, const RayC& r, floa
     return t0 <= t1;
 }
 
-// Traversal stack of node ids, [entry][lane] in LDS (bank-conflict free).
-template <int BLOCK>
-struct LdsStack {
-    uint32_t* id;
-    uint32_t lane;
-    __device__ __forceinline__ void put(uint32_t lv, uint32_t i) { id[lv * BLOCK + lane] = i; }
-    __device__ __forceinline__ uint32_t get(uint32_t lv) const { return id[lv * BLOCK + lane]; }
-};
 
 struct Closest {
     float t, u, v;
@@ -253,6 +247,7 @@ __device__ __forceinline__ void traverse_closest(const float4* __restrict__ node
     }
     uint32_t cur = __float_as_uint(node[0].w);
     uint32_t level = 0;
+    stk.reset();
     for (;;) {
         const float4 L0 = node[2 * (size_t)cur], L1 = node[2 * (size_t)cur + 1];
         const float4 R0 = node[2 * (size_t)cur + 2], R1 = node[2 * (size_t)cur + 3];
@@ -300,6 +295,7 @@ __device__ __forceinline__ bool traverse_any(const float4* __restrict__ node, co
     }
     uint32_t cur = __float_as_uint(node[0].w);
     uint32_t level = 0;
+    stk.reset();
     for (;;) {
         const float4 L0 = node[2 * (size_t)cur], L1 = node[2 * (size_t)cur + 1];
         const float4 R0 = node[2 * (size_t)cur + 2], R1 = node[2 * (size_t)cur + 3];
@@ -460,6 +456,7 @@ __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__
                                                       Stack& stk, uint32_t coop, bool start) {
     const uint64_t live = ballot(true);
     uint32_t level = 0, curr = 0;
+    stk.reset();
     bool going = start;
     while (ballot(going)) {
         uint32_t la = 0, lcnt = 0;
@@ -484,7 +481,7 @@ __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__
                 } else {
                     if (!SPEC) dl4 = delta[curr];
                     const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-                    stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                    stk.put(level, nl ? a + 1 : a);
                     level++;
                     curr = nl ? a : a + 1;
                     pop = false;
@@ -509,6 +506,7 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
                                                   Stack& stk, uint32_t coop, bool start) {
     const uint64_t live = ballot(true);
     uint32_t level = 0, curr = 0;
+    stk.reset();
     bool going = start, hit = false;
     while (ballot(going)) {
         uint32_t la = 0, lcnt = 0;
@@ -534,7 +532,7 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
                 } else {
                     if (!SPEC) dl4 = delta[curr];
                     const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-                    stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                    stk.put(level, nl ? a + 1 : a);
                     level++;
                     curr = nl ? a : a + 1;
                     pop = false;
@@ -734,7 +732,7 @@ __device__ __forceinline__ void write_hit_record(const float4* __restrict__ tri,
 // Tracer::trace -> trace_single + complete_trace_info.  Rays with a hit are
 // also appended (wave-aggregated) to the bundle's hit list, which the shadow
 // stage walks densely.
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, const float4* __restrict__ delta,
                                               const float4* __restrict__ tri,
                                               const float4* __restrict__ nrm, const float2* __restrict__ uvs,
@@ -746,8 +744,9 @@ __global__ __launch_bounds__(TB) void k_trace(const float4* __restrict__ node, c
                                               float2* __restrict__ uv2, uint32_t* __restrict__ hit_list,
                                               uint32_t* __restrict__ hit_count, uint32_t xcds, uint32_t coop,
                                               const uint32_t* __restrict__ parent, uint32_t bound) {
-    __shared__ uint32_t s_id[VAR == 4 ? 1 : STACK * TB];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    __shared__ uint32_t s_id[VAR == 4 ? 1 : SR * TB];
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     const uint64_t n = ray_count(dcount, hcount);
     const uint32_t lane = threadIdx.x & 63;
     const XcdSpan sp = xcd_span(n, xcds, TB);
@@ -835,15 +834,16 @@ __device__ __forceinline__ void set_shadow_bit(uint32_t* f, int l, bool shadowed
 // (hit, light) pair of the hit lists: DIR toward -dir, SPOT toward pos
 // (outside the cone = shadowed), POINT toward pos with tMax = distance; any
 // hit with tMin 0.01 < t < tMax sets bit 8+l of the ray's flags.
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
                                                ShadowSegs segs, unsigned long long* shadow_count, uint32_t xcds,
                                                uint32_t coop, const uint32_t* __restrict__ parent, uint32_t bound) {
-    __shared__ uint32_t s_id[VAR == 4 ? 1 : STACK * TB];
+    __shared__ uint32_t s_id[VAR == 4 ? 1 : SR * TB];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_end[RTX_SEG_MAX];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     const int nl = lights->count;
     if (threadIdx.x == 0) {
         s_cnt = 0;
@@ -942,7 +942,7 @@ __device__ __forceinline__ bool step_closest(const float4* __restrict__ node, co
             leaf_closest(tri, a, cnt, r, best);
         } else {
             const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+            stk.put(level, nl ? a + 1 : a);
             level++;
             curr = nl ? a : a + 1;
             return false;
@@ -972,7 +972,7 @@ __device__ __forceinline__ int step_any(const float4* __restrict__ node, const f
             if (leaf_any(tri, a, cnt, r, tmax)) return 1;
         } else {
             const bool nl = dot(r.d, xyz(dl4)) > 0.f;
-            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+            stk.put(level, nl ? a + 1 : a);
             level++;
             curr = nl ? a : a + 1;
             return 0;
@@ -1039,7 +1039,7 @@ __global__ __launch_bounds__(TB) void k_trace_p(const float4* __restrict__ node,
                                                 uint32_t coop, uint32_t refill) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_hits[TB / RTX_WAVE][HITBUF];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     uint32_t* hbuf = s_hits[threadIdx.x / RTX_WAVE];
     uint32_t hcnt = 0;  // wave-uniform fill of hbuf
     const uint64_t n = ray_count(dcount, hcount);
@@ -1073,6 +1073,7 @@ __global__ __launch_bounds__(TB) void k_trace_p(const float4* __restrict__ node,
                 rc = make_ray(xyz(o), xyz(dd), o.w);
                 best = Closest{dd.w, 0.f, 0.f, -1};
                 level = 0;
+                stk.reset();
                 curr = 0;
                 have = true;
             }
@@ -1097,7 +1098,7 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_end[RTX_SEG_MAX];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     const int nl = lights->count;
     if (threadIdx.x == 0) {
         s_cnt = 0;
@@ -1157,6 +1158,7 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
                         ++traced;
                         rc = make_ray(p, dir, 0.01f);
                         level = 0;
+                        stk.reset();
                         curr = 0;
                         have = true;
                     }
@@ -1185,18 +1187,6 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
 }
 
 // Launch macro M(variant, block) with the tuned variant and block size.
-// Scene kernels: also the stackless form (4).
-#define RTX_DISPATCH_TRAVERSAL_NS(M, var, tb)                 \
-    do {                                                      \
-        if (var == 4) {                                       \
-            if (tb == 64) M(4, 64);                           \
-            else if (tb == 256) M(4, 256);                    \
-            else M(4, 128);                                   \
-        } else {                                              \
-            RTX_DISPATCH_TRAVERSAL(M, var, tb);               \
-        }                                                     \
-    } while (0)
-
 #define RTX_DISPATCH_TRAVERSAL(M, var, tb)     \
     do {                                       \
         if (tb == 64) {                        \
@@ -1212,6 +1202,43 @@ __global__ __launch_bounds__(TB) void k_shadow_p(const float4* __restrict__ node
             else if (var == 1) M(1, 128);      \
             else M(2, 128);                    \
         }                                      \
+    } while (0)
+
+// Launch macro M(variant, block, LDS stack entries) for the kernels with a
+// ring-stack form: the tuned variant, block size and stack_lds (8, 16, or
+// 32 = the whole stack in LDS).
+#define RTX_DISPATCH_VB(M, var, tb, S)            \
+    do {                                          \
+        if (tb == 64) {                           \
+            if (var == 0) M(0, 64, S);            \
+            else if (var == 1) M(1, 64, S);       \
+            else M(2, 64, S);                     \
+        } else if (tb == 256) {                   \
+            if (var == 0) M(0, 256, S);           \
+            else if (var == 1) M(1, 256, S);      \
+            else M(2, 256, S);                    \
+        } else {                                  \
+            if (var == 0) M(0, 128, S);           \
+            else if (var == 1) M(1, 128, S);      \
+            else M(2, 128, S);                    \
+        }                                         \
+    } while (0)
+#define RTX_DISPATCH_STACK(M, var, tb, sr)                  \
+    do {                                                    \
+        if (sr == 8) RTX_DISPATCH_VB(M, var, tb, 8);        \
+        else if (sr == 16) RTX_DISPATCH_VB(M, var, tb, 16); \
+        else RTX_DISPATCH_VB(M, var, tb, 32);               \
+    } while (0)
+// Scene kernels: also the stackless form (4), which has no stack.
+#define RTX_DISPATCH_STACK_NS(M, var, tb, sr)   \
+    do {                                        \
+        if (var == 4) {                         \
+            if (tb == 64) M(4, 64, 32);         \
+            else if (tb == 256) M(4, 256, 32);  \
+            else M(4, 128, 32);                 \
+        } else {                                \
+            RTX_DISPATCH_STACK(M, var, tb, sr); \
+        }                                       \
     } while (0)
 
 // Persistent traversal grid: enough groups to fill every CU at 20 waves
@@ -1316,7 +1343,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst(const InstRoot* __restrict__ 
                                                     uint32_t* __restrict__ hit_count, uint32_t* __restrict__ hit_root,
                                                     uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     const uint64_t n = ray_count(dcount, hcount);
     for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
         const uint64_t rr = base + threadIdx.x;
@@ -1374,7 +1401,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
                                                      unsigned long long* shadow_count, uint32_t coop) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const int nl = lights.count;
@@ -1534,7 +1561,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
                                                        uint32_t* __restrict__ hit_root, uint32_t coop,
                                                        const RootBox* __restrict__ wbox) {
     __shared__ uint32_t s_id[STACK * TB];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     const uint64_t n = ray_count(dcount, hcount);
     const bool cull = wbox != nullptr && (CHUNKS || nroots <= 64);
     for (uint64_t base = (uint64_t)blockIdx.x * TB; base < n; base += (uint64_t)gridDim.x * TB) {
@@ -1568,6 +1595,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
             rc = make_ray(mul_h(R.tr_inv, o, 1.f, 1e-26f), mul_h(R.tr_inv, d, 0.f, 1e-26f), tmin);
             c = Closest{tmax, 0.f, 0.f, -1};
             level = 0;
+            stk.reset();
             curr = 0;
         };
         const uint64_t live = ballot(true);
@@ -1606,7 +1634,7 @@ __global__ __launch_bounds__(TB) void k_trace_inst_lr(const InstRoot* __restrict
                         } else {
                             if (!SPEC) dl4 = delta[curr];
                             const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
-                            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                            stk.put(level, nl ? a + 1 : a);
                             level++;
                             curr = nl ? a : a + 1;
                             pop = false;
@@ -1661,7 +1689,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                                                         const RootBox* __restrict__ wbox) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const bool cull = wbox != nullptr && (CHUNKS || nroots <= 64);
@@ -1712,6 +1740,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
             tri = R.tri;
             rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
             level = 0;
+            stk.reset();
             curr = 0;
         };
         bool shadowed = false;
@@ -1762,7 +1791,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                         } else {
                             if (!SPEC) dl4 = delta[curr];
                             const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
-                            stk.put(level & (STACK - 1), nl ? a + 1 : a);
+                            stk.put(level, nl ? a + 1 : a);
                             level++;
                             curr = nl ? a : a + 1;
                             pop = false;
@@ -1933,11 +1962,11 @@ __device__ __forceinline__ void wf_trace_chunk(const WfSeg& S, uint32_t cl, Stac
     Closest best{0.f, 0.f, 0.f, -1};
     RayC rc{};
     if (active) {
-        const float4 o = S.ori4[rr], dd = S.dir4[rr];
+        const float4 o = gptr(S.ori4)[rr], dd = gptr(S.dir4)[rr];
         rc = make_ray(xyz(o), xyz(dd), o.w);
         best.t = dd.w;
     }
-    closest_var<VAR>(S.node, S.delta, S.tri, rc, best, stk, coop, active);  // whole wave
+    closest_var<VAR>(gptr(S.node), gptr(S.delta), gptr(S.tri), rc, best, stk, coop, active);  // whole wave
     const bool hit = active && best.id >= 0;
     const uint64_t hb = ballot(hit);
     uint32_t wbase = 0;
@@ -1948,20 +1977,23 @@ __device__ __forceinline__ void wf_trace_chunk(const WfSeg& S, uint32_t cl, Stac
     }
     if (!active) return;
     if (!hit) {
-        write_miss_record(rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+        write_miss_record(rr, gptr(S.flags), gptr(S.t), gptr(S.id), gptr(S.bary), gptr(S.n4), gptr(S.p4),
+                          gptr(S.uv2));
         return;
     }
-    S.list[wbase + lanes_below(hb)] = (uint32_t)rr;
-    write_hit_record(S.tri, S.nrm, S.uv, rc, best, rr, S.flags, S.t, S.id, S.bary, S.n4, S.p4, S.uv2);
+    gptr(S.list)[wbase + lanes_below(hb)] = (uint32_t)rr;
+    write_hit_record(gptr(S.tri), gptr(S.nrm), gptr(S.uv), rc, best, rr, gptr(S.flags), gptr(S.t), gptr(S.id),
+                     gptr(S.bary), gptr(S.n4), gptr(S.p4), gptr(S.uv2));
 }
 
 // Closest hit over every frame's rays, persistent form: waves take 64-ray
 // chunks (ChunkSched).
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                  uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
-    __shared__ uint32_t s_id[STACK * TB];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    __shared__ uint32_t s_id[SR * TB];
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     WaveChunks<WF_RAYS> wc;
     wc.init(segs, nseg, RTX_WAVE);
     ChunkSched sch;
@@ -1978,14 +2010,15 @@ __global__ __launch_bounds__(TB) void k_wf_trace(const WfSeg* __restrict__ segs,
 // over a grid sized by the frames' ray capacities: the dispatcher hands out
 // chunks in order, so the waves in flight always cover neighbouring rays,
 // with no queue atomics; waves past the step's chunk total leave at once.
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_trace_ord(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                      const uint32_t* __restrict__ ctr, uint32_t coop) {
-    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_id[SR * TB];
     const uint32_t total = ctr[WF_TOTAL];
     const uint32_t c = blockIdx.x * (TB / RTX_WAVE) + threadIdx.x / RTX_WAVE;
     if (c >= total) return;  // whole waves (no block barrier follows)
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     WaveChunks<WF_RAYS> wc;
     wc.load(ctr + WF_TAB, nseg, total);
     uint32_t c0 = 0;
@@ -2009,10 +2042,10 @@ __device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Sta
     float tmax = 1e37f;
     bool shadowed = false, trace = false;
     if (valid) {
-        rr = S.list[it % nh];
+        rr = gptr(S.list)[it % nh];
         l = (int)(it / nh);
-        const rtx_light& L = S.lights->l[l];
-        const v3 p = xyz(S.p4[rr]);
+        const rtx_light& L = gptr(S.lights)->l[l];
+        const v3 p = xyz(gptr(S.p4)[rr]);
         v3 dir = mk3(0.f, 0.f, 0.f);
         trace = true;
         if (L.type == RTX_LIGHT_DIR) {
@@ -2033,18 +2066,20 @@ __device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Sta
         }
         if (trace) rc = make_ray(p, dir, 0.01f);
     }
-    if (any_var<VAR>(S.node, S.delta, S.tri, rc, tmax, stk, coop, trace)) shadowed = true;  // whole wave
-    if (valid) set_shadow_bit(&S.flags[rr], l, shadowed);
+    if (any_var<VAR>(gptr(S.node), gptr(S.delta), gptr(S.tri), rc, tmax, stk, coop, trace))  // whole wave
+        shadowed = true;
+    if (valid) set_shadow_bit(gptr(S.flags) + rr, l, shadowed);
     const uint64_t tb = ballot(trace);
     if (lane == 0 && tb) atomicAdd(&S.stats[1], (unsigned long long)__builtin_popcountll(tb));
 }
 
 // Shadow rays of every frame's pairs, persistent form (ChunkSched).
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                   uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
-    __shared__ uint32_t s_id[STACK * TB];
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    __shared__ uint32_t s_id[SR * TB];
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     WaveChunks<WF_PAIRS> wc;
     wc.init(segs, nseg, RTX_WAVE);
     ChunkSched sch;
@@ -2086,14 +2121,15 @@ __global__ __launch_bounds__(RTX_WF_MAX_SEGS) void k_wf_plan_shadow(const WfSeg*
 }
 
 // Shadow rays, one chunk per wave in launch order (see k_wf_trace_ord).
-template <int VAR, int TB>
+template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_shadow_ord(const WfSeg* __restrict__ segs, uint32_t nseg,
                                                       const uint32_t* __restrict__ ctr, uint32_t coop) {
-    __shared__ uint32_t s_id[STACK * TB];
+    __shared__ uint32_t s_id[SR * TB];
     const uint32_t total = ctr[WF_STOTAL];
     const uint32_t c = blockIdx.x * (TB / RTX_WAVE) + threadIdx.x / RTX_WAVE;
     if (c >= total) return;  // whole waves (no block barrier follows)
-    LdsStack<TB> stk{s_id, threadIdx.x};
+    uint32_t s_low[stack_private_words<SR>()];
+    TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
     WaveChunks<WF_PAIRS> wc;
     wc.load(ctr + WF_STAB, nseg, total);
     uint32_t c0 = 0;
@@ -2114,41 +2150,54 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
     hipLaunchKernelGGL(k_wf_raygen, dim3((uint32_t)(g ? g : 1)), dim3(256), 0, st, segs, nseg);
 }
 
-// Persistent grids: 20 waves per CU, the occupancy the LDS stack allows.
-void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                  int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+// Persistent grid of kernel K: as many blocks as fit on every CU at once
+// (the LDS stack and the registers set how many), queried once per kernel.
+template <auto K>
+static uint32_t resident_grid(int block, int cus) {
+    static std::atomic<int> per_cu{0};
+    int n = per_cu.load(std::memory_order_relaxed);
+    if (n == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, K, block, 0) != hipSuccess || n <= 0)
+            n = 1280 / block;  // 20 waves per CU
+        per_cu.store(n, std::memory_order_relaxed);
+    }
+    return (uint32_t)cus * (uint32_t)n;
+}
+
+void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
+                  uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st) {
     if (sched == 0) {
         const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
-#define RTX_WF_TRACE_ORD(V, B)                                                                                \
-    hipLaunchKernelGGL((k_wf_trace_ord<V, B>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
+#define RTX_WF_TRACE_ORD(V, B, S)                                                                                \
+    hipLaunchKernelGGL((k_wf_trace_ord<V, B, S>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
                        coop)
-        RTX_DISPATCH_TRAVERSAL(RTX_WF_TRACE_ORD, var, tb);
+        RTX_DISPATCH_STACK(RTX_WF_TRACE_ORD, var, tb, sr);
 #undef RTX_WF_TRACE_ORD
         return;
     }
-#define RTX_WF_TRACE(V, B)                                                                                  \
-    hipLaunchKernelGGL((k_wf_trace<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
-                       ctr + WF_Q_TRACE, coop, grain)
-    RTX_DISPATCH_TRAVERSAL(RTX_WF_TRACE, var, tb);
+#define RTX_WF_TRACE(V, B, S)                                                                                   \
+    hipLaunchKernelGGL((k_wf_trace<V, B, S>), dim3(resident_grid<k_wf_trace<V, B, S>>(B, cus)), dim3(B), 0, st, \
+                       segs, nseg, ctr + WF_Q_TRACE, coop, grain)
+    RTX_DISPATCH_STACK(RTX_WF_TRACE, var, tb, sr);
 #undef RTX_WF_TRACE
 }
 
-void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, uint32_t coop, uint32_t grain,
-                   int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st) {
     if (sched == 0) {
         hipLaunchKernelGGL(k_wf_plan_shadow, dim3(1), dim3(RTX_WF_MAX_SEGS), 0, st, segs, nseg, ctr);
         const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
-#define RTX_WF_SHADOW_ORD(V, B)                                                                                \
-    hipLaunchKernelGGL((k_wf_shadow_ord<V, B>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
+#define RTX_WF_SHADOW_ORD(V, B, S)                                                                                \
+    hipLaunchKernelGGL((k_wf_shadow_ord<V, B, S>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
                        coop)
-        RTX_DISPATCH_TRAVERSAL(RTX_WF_SHADOW_ORD, var, tb);
+        RTX_DISPATCH_STACK(RTX_WF_SHADOW_ORD, var, tb, sr);
 #undef RTX_WF_SHADOW_ORD
         return;
     }
-#define RTX_WF_SHADOW(V, B)                                                                                  \
-    hipLaunchKernelGGL((k_wf_shadow<V, B>), dim3((uint32_t)cus * (1280 / B)), dim3(B), 0, st, segs, nseg, \
-                       ctr + WF_Q_SHADOW, coop, grain)
-    RTX_DISPATCH_TRAVERSAL(RTX_WF_SHADOW, var, tb);
+#define RTX_WF_SHADOW(V, B, S)                                                                                    \
+    hipLaunchKernelGGL((k_wf_shadow<V, B, S>), dim3(resident_grid<k_wf_shadow<V, B, S>>(B, cus)), dim3(B), 0, st, \
+                       segs, nseg, ctr + WF_Q_SHADOW, coop, grain)
+    RTX_DISPATCH_STACK(RTX_WF_SHADOW, var, tb, sr);
 #undef RTX_WF_SHADOW
 }
 
@@ -2366,15 +2415,15 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
         hits->list_for = count;
         return RTX_OK;
     }
-#define RTX_TRACE_LAUNCH(V, B)                                                                                       \
-    hipLaunchKernelGGL((k_trace<V, B>), dim3(trace_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,              \
+#define RTX_TRACE_LAUNCH(V, B, S)                                                                                    \
+    hipLaunchKernelGGL((k_trace<V, B, S>), dim3(trace_grid(s->ctx, count, rays->cap, B)), dim3(B), 0, st,              \
                        sv.node, sv.delta, sv.tri, sv.nrm,                                                 \
                        s->uv.as<float2>(), rays->ori4.as<float4>(), rays->dir4.as<float4>(),                        \
                        rays->count.as<uint32_t>(), count, hits->flags.as<uint32_t>(), hits->t.as<float>(),          \
                        hits->id.as<int32_t>(), hits->bary.as<float2>(), hits->n4.as<float4>(), hits->p4.as<float4>(), \
                        hits->uv2.as<float2>(), hits->list.as<uint32_t>(), hits->list_count.as<uint32_t>(),           \
                        (uint32_t)s->ctx->tune.xcds, coop_threshold(s->ctx), sv.parent, stackless_bound(s))
-    RTX_DISPATCH_TRAVERSAL_NS(RTX_TRACE_LAUNCH, var, tb);
+    RTX_DISPATCH_STACK_NS(RTX_TRACE_LAUNCH, var, tb, (int)s->ctx->tune.stack_lds);
 #undef RTX_TRACE_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
     hits->list_valid = true;
@@ -2438,12 +2487,12 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
         RTX_CHECK_HIP(hipGetLastError());
         return RTX_OK;
     }
-#define RTX_SHADOW_LAUNCH(V, B)                                                                                 \
-    hipLaunchKernelGGL((k_shadow<V, B>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,    \
+#define RTX_SHADOW_LAUNCH(V, B, S)                                                                              \
+    hipLaunchKernelGGL((k_shadow<V, B, S>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,    \
                        sv.delta, sv.tri, s->lights.as<rtx_lights>(), segs,                                     \
                        (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
                        coop_threshold(s->ctx), sv.parent, stackless_bound(s))
-    RTX_DISPATCH_TRAVERSAL_NS(RTX_SHADOW_LAUNCH, var, tb);
+    RTX_DISPATCH_STACK_NS(RTX_SHADOW_LAUNCH, var, tb, (int)s->ctx->tune.stack_lds);
 #undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;

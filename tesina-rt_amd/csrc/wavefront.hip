@@ -322,13 +322,13 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     }
     {
         ProfScope ps(c, st, "wf_trace");
-        rtx_wf_trace(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, (int)c->tune.wf_sched, max_chunks,
-                     c->cus, st);
+        rtx_wf_trace(D, n, ctr, var, tb, (int)c->tune.stack_lds, coop, (uint32_t)c->tune.wf_grain,
+                     (int)c->tune.wf_sched, max_chunks, c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shadow");
-        rtx_wf_shadow(D, n, ctr, var, tb, coop, (uint32_t)c->tune.wf_grain, (int)c->tune.wf_sched, max_pair_chunks,
-                      c->cus, st);
+        rtx_wf_shadow(D, n, ctr, var, tb, (int)c->tune.stack_lds, coop, (uint32_t)c->tune.wf_grain,
+                      (int)c->tune.wf_sched, max_pair_chunks, c->cus, st);
     }
     {
         ProfScope ps(c, st, "wf_shade");

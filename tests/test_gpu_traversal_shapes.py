@@ -5,7 +5,9 @@ of 1, 2, 9, 12, 30 and 32 levels with leaves at odd and even depths, larger
 leaves (min_leaf 4 and 8), duplicated triangles (runs of equal codes: fat
 leaves, tested by the whole wave) and a flat axis.  Random rays from inside
 and around the scene give long, incoherent traversals.  Hits and shadow
-flags of each form must equal the oracle's bit for bit.
+flags of each form must equal the oracle's bit for bit, with the whole
+traversal stack in LDS and with ring stacks of 16 and 8 LDS entries (the
+deeper levels in private memory).
 """
 import numpy as np
 import pytest
@@ -68,40 +70,48 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
     o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
     if T >= 300:
         assert (o["id"] >= 0).sum() > n // 8, "the rays should hit the scene"
-    old = ctx.tuned("trace_variant")
+    old = (ctx.tuned("trace_variant"), ctx.tuned("stack_lds"))
     try:
         for variant in (0, 1, 2):
-            ctx.tune("trace_variant", variant)
-            rtx.trace(s, r, hb, n)
-            rtx.trace_shadow(s, r, hb, n)
-            g = hb.read(n)
-            for k in ("flags", "id", "t", "bary", "n4", "p4"):
-                assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant}, T {T}, depth {depth})"
+            for sl in (32, 16, 8):
+                ctx.tune("trace_variant", variant)
+                ctx.tune("stack_lds", sl)
+                rtx.trace(s, r, hb, n)
+                rtx.trace_shadow(s, r, hb, n)
+                g = hb.read(n)
+                for k in ("flags", "id", "t", "bary", "n4", "p4"):
+                    assert np.array_equal(g[k], o[k]), \
+                        f"{k} differs (variant {variant}, stack_lds {sl}, T {T}, depth {depth})"
         # the stackless walk (parent pointers of the LBVH) against the oracle's
         with O.stackless():
             ons = O.trace(b, nrm, uv, ori4, dir4)
             ons["flags"] = O.trace_shadow(b, O.lights_from(lights), ons["flags"], ons["p4"])
         ctx.tune("trace_variant", 4)
+        ctx.tune("stack_lds", 32)
         rtx.trace(s, r, hb, n)
         rtx.trace_shadow(s, r, hb, n)
         g = hb.read(n)
         for k in ("flags", "id", "t", "bary", "n4", "p4"):
             assert np.array_equal(g[k], ons[k]), f"{k} differs (stackless, T {T}, depth {depth})"
     finally:
-        ctx.tune("trace_variant", old)
+        ctx.tune("trace_variant", old[0])
+        ctx.tune("stack_lds", old[1])
     r.destroy()
     hb.destroy()
     s.destroy()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
-def test_max_levels_rule(ctx, variant):
+@pytest.mark.parametrize("variant,stack_lds", [(0, 32), (1, 32), (2, 32), (3, 32), (4, 32), (0, 16), (1, 16),
+                                               (1, 8), (2, 8)])
+def test_max_levels_rule(ctx, variant, stack_lds):
     # A ray whose near children all lie on the chain has 32 entries on its
     # stack at the bottom leaf; the reference then ends the ray instead of
     # popping (src/kernel/trace-bvh.cl:361-367, shadow-trace-bvh.cl alike).
     # Rays aimed at the slivers with positive directions take that path, so
     # for some of them the reference (and the oracle) report no hit where a
-    # brute-force search finds one.  Every traversal form must agree.
+    # brute-force search finds one.  Every traversal form must agree, with
+    # every stack size (the ring stacks hold levels 16..31 or 8..31 of these
+    # paths in private memory).
     import rtx
     from rtx import make_lights, LIGHT_POINT
     pos, idx = chain_scene()
@@ -137,16 +147,18 @@ def test_max_levels_rule(ctx, variant):
     r = rtx.Rays(ctx, n)
     r.write(ori4, dir4)
     hb = rtx.Hits(ctx, n)
-    old = ctx.tuned("trace_variant")
+    old = (ctx.tuned("trace_variant"), ctx.tuned("stack_lds"))
     try:
         ctx.tune("trace_variant", variant)
+        ctx.tune("stack_lds", stack_lds)
         rtx.trace(s, r, hb, n)
         rtx.trace_shadow(s, r, hb, n)
         g = hb.read(n)
     finally:
-        ctx.tune("trace_variant", old)
+        ctx.tune("trace_variant", old[0])
+        ctx.tune("stack_lds", old[1])
     for k in ("flags", "id", "t", "bary", "n4", "p4"):
-        assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant})"
+        assert np.array_equal(g[k], o[k]), f"{k} differs (variant {variant}, stack_lds {stack_lds})"
     r.destroy()
     hb.destroy()
     s.destroy()
