@@ -22,6 +22,7 @@ host on the same frame (one core, median of 3 frames).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --copy-calibration 256   # device copy for counter scaling
+    python bench.py --c3-only --c3-frames 20 # the dynamic-scene leg alone
 """
 from __future__ import annotations
 
@@ -59,6 +60,66 @@ def cpu_baseline(sd, cam_struct):
     return rays / dt / 1e6, dt, rays, tb
 
 
+def fp64_hit_error(ctx, s, sd, cam):
+    """Hit distances of the frame's rays -- primary and every bounce, traced
+    by the library on the built scene -- against the same hits recomputed in
+    float64: Moller-Trumbore in float64 on the float32 ray (origin,
+    direction) and the hit triangle's float32 vertices.  The library's
+    distances are the float32 reference's bit for bit; this is their
+    rounding error.  Returns max absolute and relative error and counts."""
+    import numpy as np
+    import rtx
+    P = sd.width * sd.height
+    pos = s.vertices()[0].reshape(-1, 3).astype(np.float64)
+    idx = s.indices().astype(np.int64)
+    worst_abs = worst_rel = 0.0
+    hits_total = 0
+    per_bounce = []
+    bundles = []
+    rays, cap = rtx.Rays(ctx, P), P
+    rays.primary(cam, sd.width, sd.height)
+    n = P
+    for b in range(sd.max_bounces + 1):
+        hb = rtx.Hits(ctx, cap)
+        bundles += [rays, hb]
+        rtx.trace(s, rays, hb, n)
+        h = hb.read(n)
+        o, d, _, _ = rays.read(n)
+        m = h["id"] >= 0
+        tri = idx[h["id"][m]]
+        v0, v1, v2 = pos[tri[:, 0]], pos[tri[:, 1]], pos[tri[:, 2]]
+        oo, dd = o[m, :3].astype(np.float64), d[m, :3].astype(np.float64)
+        e1, e2 = v1 - v0, v2 - v0
+        hv = np.cross(dd, e2)
+        a = np.einsum("ij,ij->i", e1, hv)
+        q = np.cross(oo - v0, e1)
+        t64 = np.einsum("ij,ij->i", e2, q) / a
+        t32 = h["t"][m].astype(np.float64)
+        err = np.abs(t32 - t64)
+        rel = err / np.maximum(np.abs(t64), 1e-30)
+        if err.size:
+            worst_abs = max(worst_abs, float(err.max()))
+            worst_rel = max(worst_rel, float(rel.max()))
+        hits_total += int(m.sum())
+        per_bounce.append({"rays": int(n), "hits": int(m.sum()),
+                           "max_abs": float(err.max()) if err.size else 0.0,
+                           "max_rel": float(rel.max()) if err.size else 0.0})
+        if b == sd.max_bounces:
+            break
+        cap = (P << (b + 1)) + 2  # the renderer's bundle of this bounce
+        nxt = rtx.Rays(ctx, cap)
+        n = rtx.raygen_secondary(s, rays, hb, nxt, n)
+        rays = nxt
+        if n == 0:
+            break
+    for x in bundles + [rays]:
+        x.destroy()
+    return {"max_abs_error": worst_abs, "max_rel_error": worst_rel, "hits_checked": hits_total,
+            "per_bounce": per_bounce,
+            "what": "float32 hit distance of every hit of the frame (primary + bounces) minus the float64 "
+                    "Moller-Trumbore distance on the same float32 ray and vertices"}
+
+
 def cpu_visit_counts(sd, cam_struct, stride: int = 16):
     """Node visits and triangle tests per primary ray, counted by the oracle's
     traversal (deterministic) over every `stride`-th primary ray of the frame:
@@ -70,6 +131,39 @@ def cpu_visit_counts(sd, cam_struct, stride: int = 16):
     o4, d4 = o4[::stride].copy(), d4[::stride].copy()
     h = O.trace(b, sd.nrm, sd.uv, o4, d4, count_visits=True)
     return float(h["visits"]) / len(o4), float(h["tri_tests"]) / len(o4), len(o4)
+
+
+def visit_count_table(stride: int = 8):
+    """The oracle's traversal work per ray (node visits, triangle tests, hit
+    rate) of the C2 frame and of C3's first animated frame, for the primary
+    rays and the first bounce's secondary rays (every `stride`-th ray):
+    what a ray of each frame costs independent of the hardware."""
+    sys.path.insert(0, str(ROOT))
+    sys.path.insert(0, str(ROOT / "tesina-rt_amd"))
+    import oracle as O
+    from rtx import scenes
+    out = {}
+    for name in ("C2", "C3"):
+        sd = scenes.config_c2() if name == "C2" else scenes.config_c3()
+        pos = sd.pos
+        if name == "C3":  # the animation's frame at t = 0
+            pos = sd.pos.copy()
+            pos[:, 1] = scenes.displace_host(sd, 0.0)[:, 1]
+        b = O.Build(pos, sd.idx, sd.matmap, 30, 1)
+        oo, od, op, oc = O.raygen_primary(O.camera_from(scenes.camera_of(sd)), sd.width, sd.height)
+        o = O.trace(b, sd.nrm, sd.uv, oo, od)
+        so, sdir, _, _, _ = O.raygen_secondary(sd.materials, b.matmap, oo, od, op, oc, o, 4 * len(oo))
+        rows = {}
+        for what, (a, d) in (("primary", (oo, od)), ("bounce_1", (so, sdir))):
+            a, d = a[::stride].copy(), d[::stride].copy()
+            h = O.trace(b, sd.nrm, sd.uv, a, d, count_visits=True)
+            k = max(1, len(a))
+            rows[what] = {"rays": len(oo) if what == "primary" else len(so), "sampled": len(a),
+                          "node_visits_per_ray": round(float(h["visits"]) / k, 2),
+                          "triangle_tests_per_ray": round(float(h["tri_tests"]) / k, 2),
+                          "hit_rate": round(float((h["id"] >= 0).mean()) if len(a) else 0.0, 4)}
+        out[name] = {"triangles": sd.T, "nodes": int(b.node_count), **rows}
+    return out
 
 
 def traffic_from_profiles(kernel: str):
@@ -121,6 +215,11 @@ def main():
                     help="frames of the textured-frame leg; also times OBJ ingest (0 = skip)")
     ap.add_argument("--sort-sizes", type=str, default="16777216,67108864",
                     help="element counts of the sort-scaling leg ('' = skip)")
+    ap.add_argument("--visit-counts", action="store_true",
+                    help="only print the oracle's node visits and triangle tests per ray of the C2 and C3 frames "
+                         "(primary rays and the first bounce; CPU only) and exit")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="only run the dynamic-scene leg (--c3-frames frames) and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -131,6 +230,13 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(image-space sharding of a frame across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.visit_counts:
+        print(json.dumps(visit_count_table()), flush=True)
+        return
+    if args.c3_only:
+        import rtx
+        print(json.dumps({"c3_dynamic": bench_c3(rtx.Context(0), max(1, args.c3_frames))}), flush=True)
+        return
 
     import numpy as np
     import rtx
@@ -329,6 +435,9 @@ def main():
     primary_trace_ms = rep_p["trace"][1] / rep_p["trace"][0]
     prim_rays.destroy()
     prim_hits.destroy()
+    fp64 = fp64_hit_error(ctx, s, sd, cam)
+    log(f"float64 check of the frame's hit distances: max abs {fp64['max_abs_error']:.3e}, "
+        f"max rel {fp64['max_rel_error']:.3e} over {fp64['hits_checked']} hits")
 
     out = {
         "metric": "Mrays/s @1M-tri 1080p Whitted",
@@ -348,6 +457,7 @@ def main():
                                "(reflect+refract, IOR 1.5 on 30%), full LBVH rebuild each frame",
                    "triangles": T, "width": sd.width, "height": sd.height, "max_bounces": sd.max_bounces,
                    "bvh_depth": 30, "min_leaf": 1},
+        "env": {"GPU_MAX_HW_QUEUES": os.environ.get("GPU_MAX_HW_QUEUES")},
         "rays_per_frame": int(rays_per_frame),
         "shadow_rays_per_frame": int(shadow_per_frame),
         "mrays_incl_shadow": round((rays_per_frame + shadow_per_frame) / ms_per_step / 1e3, 3),
@@ -370,6 +480,7 @@ def main():
             "trace_mrays": round(rays_per_frame / trace_ms / 1e3, 2) if trace_ms else None,
             "primary_trace_ms": round(primary_trace_ms, 4),
             "primary_trace_mrays": round(sd.width * sd.height / primary_trace_ms / 1e3, 1),
+            "fp64_hit_distance": fp64,
         },
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
     }

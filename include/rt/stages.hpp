@@ -10,6 +10,7 @@
 //   Scene (geometry, materials, lights, accelerator state)  include/rt/scene.hpp
 //   RayBundle, HitBundle   include/rt/ray.hpp:63-102
 //   FrameBuffer            include/rt/framebuffer.hpp:14-47
+//   Log                    include/misc/log.hpp:9-58
 //   BVHBuilder             include/rt/bvh-builder.hpp:16-28
 //   PrimaryRayGenerator    include/rt/primary-ray-generator.hpp:19-54
 //   Tracer                 include/rt/tracer.hpp:12-70
@@ -18,7 +19,8 @@
 //   Renderer               include/rt/renderer.hpp, src/rt/renderer.cpp:12-272
 //
 // Differences by design: device memory lives behind opaque rtx handles
-// instead of DeviceMemory ids; FrameBuffer::copy resolves into a device RGBA8
+// instead of DeviceMemory ids (the accelerator copy the reference traces
+// from a DeviceMemory is a BVHMemory here); FrameBuffer::copy resolves into a device RGBA8
 // image (read with read_rgba8) instead of a GL texture; no OpenGL or OpenCL;
 // TextureAtlas and Cubemap take decoded RGBA8 images instead of file names.
 #pragma once
@@ -28,6 +30,8 @@
 #include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <fstream>
+#include <iostream>
 #include <string>
 #include <vector>
 
@@ -36,6 +40,34 @@
 struct vec3 {
     float x, y, z;
 };
+
+// Log (include/misc/log.hpp:9-58): text to stdout unless silent, and to a
+// file once initialize() opened one.
+class Log {
+public:
+    // Opens the file; returns true when it could not be opened (as the
+    // reference's does).
+    bool initialize(const std::string& file = "log") {
+        enabled = true;
+        silent = false;
+        m_file.open(file.c_str(), std::fstream::out);
+        return !m_file.is_open();
+    }
+    void out(const std::string& str) { m_file << str << std::endl; }
+    std::ostream& o() { return m_file; }
+    bool enabled = false;
+    bool silent = false;
+
+private:
+    std::fstream m_file;
+};
+
+template <typename T>
+Log& operator<<(Log& l, const T& t) {
+    if (!l.silent) std::cout << t;
+    if (l.enabled) l.o() << t;
+    return l;
+}
 
 // Process-wide device context (the reference's DeviceInterface/CLInfo
 // singletons, src/gpu/interface.cpp:27-38).
@@ -318,15 +350,41 @@ public:
     int32_t initialize();
     int32_t build_lbvh(Scene& scene, size_t cq_i = 0);
     int32_t refit_lbvh(Scene& scene, size_t cq_i = 0);
+    // With logging on and a log set, build_lbvh and refit_lbvh write their
+    // stage times to the log (src/rt/bvh-builder.cpp:266-270, 351-354,
+    // 456-459, 791-795); the build then waits for the device.
+    void set_log(Log* log) { m_log = log; }
     void timing(bool b) { m_timing = b; }
     void logging(bool l) { m_logging = l; }
     double get_exec_time() const { return m_time_ms; }
     void update_configuration(const RendererConfig& conf);
 
 private:
+    int32_t log_stages(const char* title);
+    Log* m_log = nullptr;
     bool m_timing = true, m_logging = false;
     double m_time_ms = 0;
     int m_depth = 32, m_min_leaf = 1;
+};
+
+// A copy of a built scene's accelerator (the bvh_mem DeviceMemory the
+// reference's threaded renderer copies from the scene and traces,
+// src/rt/scene.cpp:99-128, src/rt/tracer.cpp:90-135): single-root scenes.
+class BVHMemory {
+public:
+    BVHMemory() = default;
+    ~BVHMemory();
+    BVHMemory(const BVHMemory&) = delete;
+    BVHMemory& operator=(const BVHMemory&) = delete;
+    // Copies the scene's current accelerator (a device copy on the context's
+    // stream).  0 = ok, -1 = error.
+    int32_t copy_from(Scene& scene);
+    bool valid() const { return m_h != nullptr && m_of != nullptr; }
+    rtx_bvh* handle() const { return m_h; }
+
+private:
+    rtx_bvh* m_h = nullptr;
+    const Scene* m_of = nullptr;
 };
 
 class PrimaryRayGenerator {
@@ -351,7 +409,12 @@ class Tracer {
 public:
     int32_t initialize();
     int32_t trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary = false);
+    // Against an accelerator copy of this scene (include/rt/tracer.hpp:21, 26).
+    int32_t trace(Scene& scene, BVHMemory& bvh_mem, int32_t ray_count, RayBundle& rays, HitBundle& hits,
+                  bool secondary = false);
     int32_t shadow_trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary = false);
+    int32_t shadow_trace(Scene& scene, BVHMemory& bvh_mem, int32_t ray_count, RayBundle& rays, HitBundle& hits,
+                         bool secondary = false);
     void timing(bool b) { m_timing = b; }
     double get_trace_exec_time() const { return m_trace_ms; }
     double get_shadow_exec_time() const { return m_shadow_ms; }

@@ -46,6 +46,7 @@ typedef struct rtx_batch rtx_batch;
 typedef struct rtx_pipeline rtx_pipeline;
 typedef struct rtx_instances rtx_instances;
 typedef struct rtx_obj rtx_obj;
+typedef struct rtx_bvh rtx_bvh;
 
 /* ---------------------------------------------------- plain data types -- */
 
@@ -146,6 +147,8 @@ int32_t rtx_sync(rtx_ctx* ctx, void* stream);
 /* Per-kernel timing with hipEvents around every launch (FrameStats timing,
  * src/rt/renderer.cpp:441-446, without forced per-stage finishes). */
 int32_t rtx_profile_enable(rtx_ctx* ctx, int32_t on);
+/* 1 when per-kernel timing is on, 0 when off or ctx is null. */
+int32_t rtx_profile_enabled(rtx_ctx* ctx);
 /* Synchronizes; writes "name count total_ms\n" lines; returns bytes needed. */
 int64_t rtx_profile_report(rtx_ctx* ctx, char* buf, uint64_t cap);
 int32_t rtx_profile_reset(rtx_ctx* ctx);
@@ -335,6 +338,27 @@ int32_t rtx_trace(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, 
 /* Tracer::shadow_trace (src/rt/tracer.cpp:273-352;
  * src/kernel/shadow-trace-bvh.cl:160-418): one shadow ray per hit and light. */
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+
+/* A copy of a built scene's accelerator: the reference's bvh_mem
+ * DeviceMemory, a copy of the scene's node array that its threaded renderer
+ * keeps (SceneGPUCopy, src/rt/scene.cpp:99-128) so that frames trace it
+ * while the scene's own BVH is rebuilt.  Here it holds everything a trace
+ * reads beside the rays: the node array, near-child deltas and triangle
+ * records (in BVH order), the vertex normals and the material map. */
+int32_t rtx_bvh_create(rtx_ctx* ctx, rtx_bvh** out);
+int32_t rtx_bvh_destroy(rtx_bvh* b);
+/* Copies scene s's current accelerator into b on the stream (a device copy;
+ * b keeps it until the next copy).  s must be built. */
+int32_t rtx_bvh_copy(rtx_bvh* b, rtx_scene* s, void* stream);
+/* Tracer::trace(Scene&, DeviceMemory& bvh_mem, ...) (include/rt/tracer.hpp:21,
+ * src/rt/tracer.cpp:90-135): rtx_trace of scene s against the copy b instead
+ * of the scene's own accelerator (b must be a copy of s: same triangle and
+ * vertex counts).  Not thread-safe for one scene (as every call on it). */
+int32_t rtx_trace_with(rtx_scene* s, rtx_bvh* b, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream);
+/* Tracer::shadow_trace(Scene&, DeviceMemory& bvh_mem, ...)
+ * (include/rt/tracer.hpp:26): rtx_trace_shadow against the copy b. */
+int32_t rtx_trace_shadow_with(rtx_scene* s, rtx_bvh* b, rtx_rays* rays, rtx_hits* hits, uint64_t count,
+                              void* stream);
 
 /* ------------------------------------------------------------ OBJ ingest -- */
 /* ModelOBJ::import (src/rt/obj-loader.cpp:163-227) with its MTL reader

@@ -79,7 +79,8 @@ def build(verbose: bool = False) -> Path:
 
 TOOLS = {"stage_render": ROOT / "tests" / "cpp" / "stage_render.cpp",
          "stage_objects": ROOT / "tests" / "cpp" / "stage_objects.cpp",
-         "stage_boat": ROOT / "tests" / "cpp" / "stage_boat.cpp"}
+         "stage_boat": ROOT / "tests" / "cpp" / "stage_boat.cpp",
+         "stage_tracer": ROOT / "tests" / "cpp" / "stage_tracer.cpp"}
 BIN = PKG / "bin"
 
 

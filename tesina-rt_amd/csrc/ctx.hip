@@ -184,6 +184,8 @@ int32_t rtx_profile_enable(rtx_ctx* ctx, int32_t on) {
     return RTX_OK;
 }
 
+int32_t rtx_profile_enabled(rtx_ctx* ctx) { return ctx && ctx->profiling ? 1 : 0; }
+
 int32_t rtx_profile_reset(rtx_ctx* ctx) {
     RTX_REQUIRE(ctx, RTX_EINVAL, "rtx_profile_reset: null context");
     drain_profile(ctx);

@@ -56,9 +56,14 @@ int32_t rtx_reserve_render_copy(rtx_scene* s, int k, bool shading_only) {
 // Makes render copy k of what the render stages read, on `st` (shading only:
 // the normals and material ids).
 int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st, bool shading_only) {
-    int32_t e;
     RTX_REQUIRE(k >= 0 && k < rtx_scene::COPIES, RTX_EINVAL, "render copy: bad copy index");
-    rtx_scene::RenderCopy& rc = s->rc[k];
+    return rtx_copy_render_state(s, s->rc[k], st, shading_only);
+}
+
+// Copies what the render stages read (shading only: the normals and
+// material ids) into rc, on `st`.
+int32_t rtx_copy_render_state(rtx_scene* s, rtx_scene::RenderCopy& rc, hipStream_t st, bool shading_only) {
+    int32_t e;
     DevBuf* src[5] = {&s->nrm, &s->matmap, &s->b().node, &s->b().delta, &s->b().tri};
     DevBuf* dst[5] = {&rc.nrm, &rc.matmap, &rc.node, &rc.delta, &rc.tri};
     const uint64_t bytes[5] = {(uint64_t)s->V * 16, (uint64_t)s->T * 4, s->b().node.bytes, s->b().delta.bytes,

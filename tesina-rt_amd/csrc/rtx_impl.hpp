@@ -177,6 +177,20 @@ struct rtx_scene {
     BuildSet& b() { return sets[cur]; }
     const BuildSet& b() const { return sets[cur]; }
     float4* nodes() const { return sets[cur].nodes(); }
+    // An accelerator copy the launches being enqueued read instead of all of
+    // the above (rtx_trace_with); null: none.
+    const struct rtx_bvh* ext = nullptr;
+};
+
+// A copy of a scene's accelerator and the per-vertex data a trace reads
+// (rtx_bvh_copy), with the scene state it was taken from.
+struct rtx_bvh {
+    rtx_ctx* ctx = nullptr;
+    rtx_scene::RenderCopy copy;
+    const rtx_scene* of = nullptr;
+    uint32_t V = 0, T = 0;
+    int depth = 30;
+    bool valid = false;
 };
 
 // One image of the shading stage's atlas: first texel, width, height.
@@ -196,6 +210,12 @@ struct SceneView {
     const uint32_t* parent;  // node parents (the stackless walk); null for a render copy
 };
 inline SceneView scene_view(const rtx_scene* s) {
+    if (s->ext) {
+        const rtx_scene::RenderCopy& c = s->ext->copy;
+        return SceneView{reinterpret_cast<const float4*>(static_cast<const char*>(c.node.p) + 32),
+                         c.delta.as<float4>(), c.tri.as<float4>(), c.nrm.as<float4>(), c.matmap.as<int32_t>(),
+                         nullptr};
+    }
     if (s->use_set >= 0) {
         const BuildSet& b = s->sets[s->use_set];
         const bool c = s->use_copy >= 0;
@@ -339,6 +359,7 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
 // only) the normals and material ids, whose buffers the next frame's
 // animation and permutation overwrite in place.
 int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st, bool shading_only = false);
+int32_t rtx_copy_render_state(rtx_scene* s, rtx_scene::RenderCopy& rc, hipStream_t st, bool shading_only);
 int32_t rtx_reserve_render_copy(rtx_scene* s, int k, bool shading_only = false);
 // Build into set s->cur with the sort phase on sort_stream and the tree phase
 // on tree_stream (which waits for `sorted`, recorded after the sort phase).

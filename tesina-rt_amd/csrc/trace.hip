@@ -2374,7 +2374,8 @@ static int traversal_variant(const rtx_scene* s, const SceneView& sv) {
     // the stackless form needs the parents of the nodes it reads (a render
     // copy of the pipelined frame holds none): the single-node form instead
     if (var == 4 && !sv.parent) return 1;
-    return (var == 2 && s->depth >= STACK) ? 1 : var;
+    const int depth = s->ext ? s->ext->depth : s->depth;
+    return (var == 2 && depth >= STACK) ? 1 : var;
 }
 
 // Iteration bound of the stackless walk: a tree of N <= 2T - 1 nodes allows
@@ -2508,6 +2509,72 @@ int32_t rtx_trace_shadow_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint
 int32_t rtx_trace_shadow(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
     RTX_REQUIRE(s, RTX_EINVAL, "rtx_trace_shadow: null scene");
     return rtx_trace_shadow_impl(s, rays, hits, count, nullptr, s->ctx->pick(stream));
+}
+
+// ---------------------------------------------------- accelerator copies --
+int32_t rtx_bvh_create(rtx_ctx* ctx, rtx_bvh** out) {
+    RTX_REQUIRE(ctx && out, RTX_EINVAL, "rtx_bvh_create: null argument");
+    rtx_bvh* b = new rtx_bvh();
+    b->ctx = ctx;
+    *out = b;
+    return RTX_OK;
+}
+
+int32_t rtx_bvh_destroy(rtx_bvh* b) {
+    if (!b) return RTX_OK;
+    rtx_scene::RenderCopy& c = b->copy;
+    for (DevBuf* d : {&c.node, &c.delta, &c.tri, &c.nrm, &c.matmap}) d->release();
+    delete b;
+    return RTX_OK;
+}
+
+int32_t rtx_bvh_copy(rtx_bvh* b, rtx_scene* s, void* stream) {
+    RTX_REQUIRE(b && s, RTX_EINVAL, "rtx_bvh_copy: null argument");
+    RTX_REQUIRE(b->ctx == s->ctx, RTX_EINVAL, "rtx_bvh_copy: the copy and the scene belong to different contexts");
+    RTX_REQUIRE(s->built && !s->sah, RTX_ESTATE, "rtx_bvh_copy: scene has no LBVH (call rtx_lbvh_build)");
+    RTX_REQUIRE(s->use_set < 0 && s->use_copy < 0 && !s->ext, RTX_ESTATE,
+                "rtx_bvh_copy: the scene is being rendered from another buffer set");
+    int32_t e = rtx_copy_render_state(s, b->copy, s->ctx->pick(stream), false);
+    if (e) {
+        b->valid = false;
+        return e;
+    }
+    b->of = s;
+    b->V = s->V;
+    b->T = s->T;
+    b->depth = s->depth;
+    b->valid = true;
+    return RTX_OK;
+}
+
+// Tracer::trace / shadow_trace against an explicit accelerator
+// (src/rt/tracer.cpp:90-135): the same launches with the copy's buffers.
+static int32_t check_copy(rtx_scene* s, rtx_bvh* b, const char* who) {
+    RTX_REQUIRE(s && b, RTX_EINVAL, std::string(who) + ": null argument");
+    RTX_REQUIRE(b->valid && b->of == s && b->T == s->T && b->V == s->V, RTX_EINVAL,
+                std::string(who) + ": the accelerator copy was not taken from this scene");
+    RTX_REQUIRE(!s->ext && s->use_set < 0 && s->use_copy < 0, RTX_ESTATE,
+                std::string(who) + ": the scene is being rendered from another buffer set");
+    return RTX_OK;
+}
+
+int32_t rtx_trace_with(rtx_scene* s, rtx_bvh* b, rtx_rays* rays, rtx_hits* hits, uint64_t count, void* stream) {
+    int32_t e = check_copy(s, b, "rtx_trace_with");
+    if (e) return e;
+    s->ext = b;
+    e = rtx_trace(s, rays, hits, count, stream);
+    s->ext = nullptr;
+    return e;
+}
+
+int32_t rtx_trace_shadow_with(rtx_scene* s, rtx_bvh* b, rtx_rays* rays, rtx_hits* hits, uint64_t count,
+                              void* stream) {
+    int32_t e = check_copy(s, b, "rtx_trace_shadow_with");
+    if (e) return e;
+    s->ext = b;
+    e = rtx_trace_shadow(s, rays, hits, count, stream);
+    s->ext = nullptr;
+    return e;
 }
 
 // ---------------------------------------------------------- instances --

@@ -7,12 +7,31 @@
 #include <cmath>
 #include <cstring>
 #include <iostream>
+#include <map>
+#include <sstream>
 
 namespace {
 using clk = std::chrono::steady_clock;
 double ms_since(clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); }
 rtx_ctx* dev() { return DeviceInterface::instance()->ctx(); }
 void report(const char* what) { std::cerr << what << ": " << rtx_last_error() << "\n"; }
+
+// Per-kernel totals of the context's profile (launches, ms); waits for the
+// device.
+using ProfileTotals = std::map<std::string, std::pair<unsigned long long, double>>;
+ProfileTotals profile_totals(rtx_ctx* ctx) {
+    ProfileTotals t;
+    const int64_t n = rtx_profile_report(ctx, nullptr, 0);
+    if (n <= 1) return t;
+    std::string buf((size_t)n, '\0');
+    rtx_profile_report(ctx, &buf[0], (uint64_t)n);
+    std::istringstream in(buf.c_str());
+    std::string name;
+    unsigned long long c = 0;
+    double ms = 0;
+    while (in >> name >> c >> ms) t[name] = {c, ms};
+    return t;
+}
 }  // namespace
 
 // ------------------------------------------------------ DeviceInterface --
@@ -586,13 +605,60 @@ int32_t BVHBuilder::build_lbvh(Scene& scene, size_t cq_i) {
     if (!scene.valid()) return -1;
     auto t = clk::now();
     rtx_lbvh_params p{m_depth, m_min_leaf};
+    if (log_stages(nullptr)) return -1;
     if (rtx_lbvh_build(scene.handle(), &p, nullptr) != RTX_OK) {
         report("BVHBuilder::build_lbvh");
+        log_stages("-----BVH construction failed; kernels run: ");
         return -1;
     }
+    if (log_stages("-----BVH construction times: ")) return -1;
     if (rtx_lbvh_info(scene.handle(), &scene.bvh_node_count, scene.bvh_level_node_count) != RTX_OK) return -1;
     if (m_timing) m_time_ms = ms_since(t);
     scene.m_built = true;
+    return 0;
+}
+
+// With logging on and a log set: called with no title before the build
+// (turns the context's kernel timing on and takes its totals) and with the
+// title after it (waits for the device and writes each build kernel's time,
+// the reference's stage lines first, then the total; puts the timing back).
+int32_t BVHBuilder::log_stages(const char* title) {
+    static thread_local ProfileTotals before;
+    static thread_local int was_on = 0;
+    if (!m_logging || !m_log) return 0;
+    rtx_ctx* ctx = DeviceInterface::instance()->ctx();
+    if (!title) {
+        was_on = rtx_profile_enabled(ctx);
+        if (rtx_profile_enable(ctx, 1) != RTX_OK) return -1;
+        before = profile_totals(ctx);
+        return 0;
+    }
+    const ProfileTotals after = profile_totals(ctx);
+    if (!was_on) rtx_profile_enable(ctx, 0);
+    // the reference's stage names where a kernel is that stage
+    static const std::pair<const char*, const char*> named[] = {
+        {"lbvh_prim_bounds", "BBox builder time"}, {"lbvh_morton", "Morton encoder time"},
+        {"lbvh_sort_pass", "Morton sorter"}, {"lbvh_refit_level", "Node bbox compute time"}};
+    Log& log = *m_log;
+    log << title << "\n";
+    double total = 0;
+    std::map<std::string, double> ms;
+    for (const auto& kv : after) {
+        if (kv.first.compare(0, 5, "lbvh_") != 0) continue;
+        auto b = before.find(kv.first);
+        const double d = kv.second.second - (b == before.end() ? 0.0 : b->second.second);
+        if (kv.second.first == (b == before.end() ? 0ull : b->second.first)) continue;  // not launched
+        ms[kv.first] = d;
+        total += d;
+    }
+    for (const auto& n : named) {
+        auto it = ms.find(n.first);
+        if (it == ms.end()) continue;
+        log << "\t" << n.second << ": " << it->second << " ms\n";
+        ms.erase(it);
+    }
+    for (const auto& kv : ms) log << "\t" << kv.first << ": " << kv.second << " ms\n";
+    log << "\tTotal: " << total << " ms\n";
     return 0;
 }
 
@@ -601,10 +667,13 @@ int32_t BVHBuilder::refit_lbvh(Scene& scene, size_t cq_i) {
     (void)cq_i;
     if (!scene.valid() || !scene.ready()) return -1;
     auto t = clk::now();
+    if (log_stages(nullptr)) return -1;
     if (rtx_lbvh_refit(scene.handle(), nullptr) != RTX_OK) {
         report("BVHBuilder::refit_lbvh");
+        log_stages("-----BVH refit failed; kernels run: ");
         return -1;
     }
+    if (log_stages("-----BVH refit times: ")) return -1;
     if (m_timing) {
         DeviceInterface::instance()->finish_commands();
         m_time_ms = ms_since(t);
@@ -668,6 +737,26 @@ static rtx_scene* single_root(const Scene& scene) {
     return scene.aggregate_built() ? scene.handle() : scene.mesh_handle(scene.object_mesh(0));
 }
 
+// ------------------------------------------------------------ BVHMemory --
+BVHMemory::~BVHMemory() {
+    if (m_h) rtx_bvh_destroy(m_h);
+}
+
+int32_t BVHMemory::copy_from(Scene& scene) {
+    if (!scene.ready() || scene.root_count() > 1) return -1;  // single-root scenes
+    if (!m_h && rtx_bvh_create(DeviceInterface::instance()->ctx(), &m_h) != RTX_OK) {
+        report("BVHMemory::copy_from");
+        return -1;
+    }
+    m_of = nullptr;
+    if (rtx_bvh_copy(m_h, single_root(scene), nullptr) != RTX_OK) {
+        report("BVHMemory::copy_from");
+        return -1;
+    }
+    m_of = &scene;
+    return 0;
+}
+
 // Tracer::trace (src/rt/tracer.cpp:140-269).
 int32_t Tracer::trace(Scene& scene, int32_t ray_count, RayBundle& rays, HitBundle& hits, bool secondary) {
     (void)secondary;
@@ -699,6 +788,42 @@ int32_t Tracer::shadow_trace(Scene& scene, int32_t ray_count, RayBundle& rays, H
                            : rtx_trace_shadow(single_root(scene), rays.handle(), hits.handle(), (uint64_t)ray_count,
                                               nullptr);
     if (rc != RTX_OK) {
+        report("Tracer::shadow_trace");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_shadow_ms = ms_since(t);
+    }
+    return 0;
+}
+
+// Tracer::trace / shadow_trace against an accelerator copy
+// (src/rt/tracer.cpp:90-135; the shadow overload alike).
+int32_t Tracer::trace(Scene& scene, BVHMemory& bvh_mem, int32_t ray_count, RayBundle& rays, HitBundle& hits,
+                      bool secondary) {
+    (void)secondary;
+    if (!m_initialized || ray_count < 0 || !scene.ready() || !bvh_mem.valid() || scene.root_count() > 1) return -1;
+    auto t = clk::now();
+    if (rtx_trace_with(single_root(scene), bvh_mem.handle(), rays.handle(), hits.handle(), (uint64_t)ray_count,
+                       nullptr) != RTX_OK) {
+        report("Tracer::trace");
+        return -1;
+    }
+    if (m_timing) {
+        DeviceInterface::instance()->finish_commands();
+        m_trace_ms = ms_since(t);
+    }
+    return 0;
+}
+
+int32_t Tracer::shadow_trace(Scene& scene, BVHMemory& bvh_mem, int32_t ray_count, RayBundle& rays, HitBundle& hits,
+                             bool secondary) {
+    (void)secondary;
+    if (!m_initialized || ray_count < 0 || !scene.ready() || !bvh_mem.valid() || scene.root_count() > 1) return -1;
+    auto t = clk::now();
+    if (rtx_trace_shadow_with(single_root(scene), bvh_mem.handle(), rays.handle(), hits.handle(),
+                              (uint64_t)ray_count, nullptr) != RTX_OK) {
         report("Tracer::shadow_trace");
         return -1;
     }

@@ -111,8 +111,8 @@ class RenderParams(C.Structure):
 # Every symbol include/rtx.h declares (checked by tests/test_abi.py).
 SYMBOLS = [
     "rtx_init", "rtx_shutdown", "rtx_last_error", "rtx_abi_version", "rtx_stream", "rtx_sync",
-    "rtx_profile_enable", "rtx_profile_report", "rtx_profile_reset", "rtx_mem_in_use", "rtx_tune_set",
-    "rtx_tune_get",
+    "rtx_profile_enable", "rtx_profile_enabled", "rtx_profile_report", "rtx_profile_reset", "rtx_mem_in_use",
+    "rtx_tune_set", "rtx_tune_get",
     "rtx_scene_create", "rtx_scene_destroy", "rtx_scene_set_vertices", "rtx_scene_set_vertices_ref",
     "rtx_scene_update_positions", "rtx_scene_set_triangles", "rtx_scene_set_materials", "rtx_scene_set_lights",
     "rtx_scene_set_textures", "rtx_scene_set_cubemap",
@@ -124,6 +124,7 @@ SYMBOLS = [
     "rtx_rays_create", "rtx_rays_destroy", "rtx_rays_capacity", "rtx_rays_count", "rtx_rays_read", "rtx_rays_write",
     "rtx_hits_create", "rtx_hits_destroy", "rtx_hits_read", "rtx_hits_write",
     "rtx_raygen_primary", "rtx_trace", "rtx_trace_shadow", "rtx_raygen_secondary",
+    "rtx_bvh_create", "rtx_bvh_destroy", "rtx_bvh_copy", "rtx_trace_with", "rtx_trace_shadow_with",
     "rtx_fb_create", "rtx_fb_destroy", "rtx_fb_clear", "rtx_fb_resolve", "rtx_fb_read_accum", "rtx_fb_read_rgba8",
     "rtx_shade",
     "rtx_renderer_create", "rtx_renderer_destroy", "rtx_renderer_fb", "rtx_render_frame", "rtx_render_frame_async",
@@ -166,6 +167,7 @@ def lib() -> C.CDLL:
         "rtx_stream": (P, [P]),
         "rtx_sync": (I32, [P, P]),
         "rtx_profile_enable": (I32, [P, I32]),
+        "rtx_profile_enabled": (I32, [P]),
         "rtx_profile_report": (C.c_int64, [P, C.c_char_p, U64]),
         "rtx_profile_reset": (I32, [P]),
         "rtx_mem_in_use": (U64, [P]),
@@ -234,6 +236,11 @@ def lib() -> C.CDLL:
         "rtx_raygen_primary": (I32, [P, P, I32, I32, I32, I32, P, U64, U64, P]),
         "rtx_trace": (I32, [P, P, P, U64, P]),
         "rtx_trace_shadow": (I32, [P, P, P, U64, P]),
+        "rtx_bvh_create": (I32, [P, C.POINTER(P)]),
+        "rtx_bvh_destroy": (I32, [P]),
+        "rtx_bvh_copy": (I32, [P, P, P]),
+        "rtx_trace_with": (I32, [P, P, P, P, U64, P]),
+        "rtx_trace_shadow_with": (I32, [P, P, P, P, U64, P]),
         "rtx_raygen_secondary": (I32, [P, P, P, P, U64, C.POINTER(U64), P]),
         "rtx_fb_create": (I32, [P, I32, I32, C.POINTER(P)]),
         "rtx_fb_destroy": (I32, [P]),
@@ -693,11 +700,38 @@ class FrameBuffer:
         return out
 
 
-def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+class BVHCopy:
+    """A copy of a built scene's accelerator (rtx_bvh): the reference's
+    bvh_mem that Tracer::trace(Scene&, DeviceMemory& bvh_mem, ...) traces."""
+
+    def __init__(self, ctx: "Context"):
+        h = C.c_void_p()
+        _check(lib().rtx_bvh_create(ctx.h, C.byref(h)), "rtx_bvh_create")
+        self.h = h
+
+    def copy(self, scene: "Scene", stream=None):
+        _check(lib().rtx_bvh_copy(self.h, scene.h, stream), "rtx_bvh_copy")
+        return self
+
+    def destroy(self):
+        if self.h:
+            lib().rtx_bvh_destroy(self.h)
+            self.h = None
+
+
+def trace(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None, bvh: BVHCopy = None):
+    """Tracer::trace; with bvh, the overload that traces an accelerator copy."""
+    if bvh is not None:
+        _check(lib().rtx_trace_with(scene.h, bvh.h, rays.h, hits.h, count, stream), "rtx_trace_with")
+        return
     _check(lib().rtx_trace(scene.h, rays.h, hits.h, count, stream), "rtx_trace")
 
 
-def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None):
+def trace_shadow(scene: Scene, rays: Rays, hits: Hits, count=DEVICE_COUNT, stream=None, bvh: BVHCopy = None):
+    """Tracer::shadow_trace; with bvh, the overload that traces an accelerator copy."""
+    if bvh is not None:
+        _check(lib().rtx_trace_shadow_with(scene.h, bvh.h, rays.h, hits.h, count, stream), "rtx_trace_shadow_with")
+        return
     _check(lib().rtx_trace_shadow(scene.h, rays.h, hits.h, count, stream), "rtx_trace_shadow")
 
 
