@@ -19,7 +19,7 @@
 namespace {
 
 constexpr int BS = 256;
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS = 16;               // the smallest tile (sizes the look-back words)
 constexpr int SORT_TILE = BS * SORT_ITEMS;
 constexpr int BOUNDS_GRID = 512;
 constexpr uint32_t FAT_LEAF = 16;
@@ -29,8 +29,8 @@ constexpr int GRID_PER_CU = 2; // persistent grids: blocks per CU
 
 // Counter slots in scene->tickets.
 constexpr int TK_BOUNDS = 0;
-constexpr int TK_SORT = 1;    // 4 passes
-constexpr int TK_LEVEL = 8;   // up to 64 levels
+constexpr int TK_SORT = 1;    // up to 8 passes (4-bit digits)
+constexpr int TK_LEVEL = 16;  // up to 64 levels
 constexpr int TK_COUNT = 80;
 
 // Layout of scene->levels (uint32): [d] = first node of level d,
@@ -177,10 +177,14 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
 
 __global__ __launch_bounds__(BS) void k_morton(const float4* __restrict__ cen, const float* __restrict__ gbox6,
                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                               uint32_t* __restrict__ hist, uint32_t T, int depth, int passes) {
-    __shared__ uint32_t s_h[4][256];
-    for (int p = 0; p < 4; ++p) s_h[p][threadIdx.x] = 0;
+                                               uint32_t* __restrict__ hist, uint32_t T, int depth, int passes,
+                                               int rbits) {
+    // digit histograms of every pass: pass p, digit d at p * 2^rbits + d
+    // (4 passes of 8 bits or 8 of 4 bits: at most 1024 words)
+    __shared__ uint32_t s_h[1024];
+    for (int i = threadIdx.x; i < 1024; i += BS) s_h[i] = 0;
     __syncthreads();
+    const uint32_t rmask = (1u << rbits) - 1u;
     v3 glo = mk3(gbox6[0], gbox6[1], gbox6[2]);
     v3 ghi = mk3(gbox6[3], gbox6[4], gbox6[5]);
     v3 inv = recip(ghi - glo);
@@ -209,65 +213,77 @@ __global__ __launch_bounds__(BS) void k_morton(const float4* __restrict__ cen, c
         const uint64_t act = ballot(active);
         const uint32_t lead = (uint32_t)__builtin_ctzll(act);
         for (int p = 0; p < passes; ++p) {
-            const uint32_t d = (code >> (8 * p)) & 0xffu;
+            const uint32_t d = (code >> (rbits * p)) & rmask;
             const uint32_t d0 = __shfl(d, (int)lead, RTX_WAVE);
             // input in the last build's order is nearly sorted: whole waves share a digit
             if (ballot(active && d == d0) == act) {
-                if (lane == lead) atomicAdd(&s_h[p][d0], (uint32_t)__builtin_popcountll(act));
+                if (lane == lead) atomicAdd(&s_h[(p << rbits) + d0], (uint32_t)__builtin_popcountll(act));
             } else if (active) {
-                atomicAdd(&s_h[p][d], 1u);
+                atomicAdd(&s_h[(p << rbits) + d], 1u);
             }
         }
     }
     __syncthreads();
-    for (int p = 0; p < passes; ++p) {
-        uint32_t c = s_h[p][threadIdx.x];
-        if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+    for (int i = threadIdx.x; i < (passes << rbits); i += BS) {
+        const uint32_t c = s_h[i];
+        if (c) atomicAdd(&hist[i], c);
     }
 }
 
-// One stable LSD pass over `bits` bits at `shift` (onesweep: in-tile rank by
-// wave64 ballot matching, per-digit decoupled look-back across tiles,
-// LDS-staged scatter).  Replaces the bitonic network of
-// src/kernel/bvh-builder-sort.cl:26-232.
+// One stable LSD pass over `bits` <= RB bits at `shift` (onesweep: in-tile
+// rank by wave64 ballot matching, per-digit decoupled look-back across tiles,
+// LDS-staged scatter) over tiles of BS * ITEMS pairs.  Replaces the bitonic
+// network of src/kernel/bvh-builder-sort.cl:26-232.  RB = 8: 256 digits a
+// pass; RB = 4: 16 digits, twice the passes, longer runs per digit in the
+// scatter (the reference's radix prototype, src/kernel/sort.cl:491-628, uses
+// 4-bit digits).  The look-back word of tile t, digit d is status[t * 2^RB + d].
+template <int RB, int ITEMS>
 __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                   uint32_t n, int shift, int bits, const uint32_t* __restrict__ hist,
                                                   uint32_t* status, uint32_t* ticket) {
-    __shared__ uint32_t s_keys[SORT_TILE];
-    __shared__ uint32_t s_vals[SORT_TILE];
-    __shared__ uint32_t s_wcnt[BS / RTX_WAVE][256];
-    __shared__ uint32_t s_tofs[256];
-    __shared__ uint32_t s_gofs[256];
+    constexpr int TILE = BS * ITEMS;
+    constexpr uint32_t DIG = 1u << RB;
+    static_assert(DIG <= (uint32_t)BS, "one thread per digit");
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
+    __shared__ uint32_t s_wcnt[BS / RTX_WAVE][DIG];
+    __shared__ uint32_t s_tofs[DIG];
+    __shared__ uint32_t s_gofs[DIG];
     __shared__ uint32_t s_w[BS / RTX_WAVE];
     __shared__ uint32_t s_tile;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    if (tid < DIG) {
 #pragma unroll
-    for (int i = 0; i < BS / RTX_WAVE; ++i) s_wcnt[i][tid] = 0;
+        for (int i = 0; i < BS / RTX_WAVE; ++i) s_wcnt[i][tid] = 0;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t base = tile * SORT_TILE;
+    const uint32_t base = tile * TILE;
     const uint32_t dmask = (1u << bits) - 1u;
 
-    uint32_t k[SORT_ITEMS], v[SORT_ITEMS], r[SORT_ITEMS];
+    uint32_t k[ITEMS], v[ITEMS], r[ITEMS];
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
-        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        uint32_t e = base + (w * ITEMS + j) * RTX_WAVE + lane;
         bool ok = e < n;
         k[j] = ok ? kin[e] : 0xffffffffu;
         v[j] = ok ? vin[e] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
-        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        uint32_t e = base + (w * ITEMS + j) * RTX_WAVE + lane;
         bool ok = e < n;
         uint32_t d = (k[j] >> shift) & dmask;
         uint64_t peers = ballot(ok);
-        for (int b = 0; b < bits; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = ballot(bit);
-            peers &= bit ? bal : ~bal;
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if (b < bits) {
+                bool bit = (d >> b) & 1u;
+                uint64_t bal = ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
         }
         uint32_t below = lanes_below(peers);
         uint32_t cnt = (uint32_t)__builtin_popcountll(peers);
@@ -278,7 +294,7 @@ __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ k
     __syncthreads();
     // per-digit tile totals, wave offsets (exclusive, in wave order)
     uint32_t tot = 0;
-    {
+    if (tid < DIG) {
         uint32_t c[BS / RTX_WAVE];
 #pragma unroll
         for (int i = 0; i < BS / RTX_WAVE; ++i) c[i] = s_wcnt[i][tid];
@@ -290,19 +306,19 @@ __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ k
     }
     uint32_t dummy;
     uint32_t tofs = block_excl_scan(tot, s_w, &dummy);
-    s_tofs[tid] = tofs;
+    if (tid < DIG) s_tofs[tid] = tofs;
     // global start of each digit = exclusive scan of the pass histogram
     uint32_t hv = tid <= dmask ? hist[tid] : 0u;
     uint32_t dbase = block_excl_scan(hv, s_w, &dummy);
     if (tid <= dmask) {
-        uint32_t prefix = lookback_thread(status + tid, tile, tot, 256u);
+        uint32_t prefix = lookback_thread(status + tid, tile, tot, DIG);
         s_gofs[tid] = dbase + prefix - tofs;  // dest = s_gofs[d] + local position
     }
     __syncthreads();
     // stage the tile in LDS in sorted order
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
-        uint32_t e = base + (w * SORT_ITEMS + j) * RTX_WAVE + lane;
+    for (int j = 0; j < ITEMS; ++j) {
+        uint32_t e = base + (w * ITEMS + j) * RTX_WAVE + lane;
         if (e < n) {
             uint32_t d = (k[j] >> shift) & dmask;
             uint32_t lp = s_tofs[d] + s_wcnt[w][d] + r[j];
@@ -311,7 +327,7 @@ __global__ __launch_bounds__(BS) void k_sort_pass(const uint32_t* __restrict__ k
         }
     }
     __syncthreads();
-    uint32_t nvalid = n - base < (uint32_t)SORT_TILE ? n - base : (uint32_t)SORT_TILE;
+    uint32_t nvalid = n - base < (uint32_t)TILE ? n - base : (uint32_t)TILE;
     for (uint32_t i = tid; i < nvalid; i += BS) {
         uint32_t key = s_keys[i];
         uint32_t dst = s_gofs[(key >> shift) & dmask] + i;
@@ -781,14 +797,15 @@ static int32_t build_params(rtx_scene* s, const rtx_lbvh_params* p, int* depth_o
 // W.keys_alt / W.vals_alt (an odd pass count).
 static bool build_sort_phase(rtx_scene* s, int depth, hipStream_t st, const BuildScratch& W, rtx_ctx* pc) {
     const uint32_t T = s->T;
-    const int passes = (depth + 7) / 8;
-    const uint32_t sort_tiles = rtx_div_up(T, SORT_TILE);
+    const int rb = (int)s->ctx->tune.sort_bits, items = (int)s->ctx->tune.sort_items;
+    const int passes = (depth + rb - 1) / rb;
+    const uint32_t sort_tiles = rtx_div_up(T, (uint64_t)BS * items);
     const uint32_t hier_tiles = rtx_div_up(T, BS);
     uint32_t* tickets = W.tickets;
     uint32_t* levels = s->b().levels.as<uint32_t>();
     {
         ProfScope ps(pc, st, "lbvh_init");
-        uint32_t nb = (uint32_t)(passes * (size_t)sort_tiles * 256);
+        uint32_t nb = (uint32_t)(passes * (size_t)sort_tiles << rb);
         uint32_t nc = (uint32_t)((size_t)(depth + 1) * hier_tiles);
         uint32_t grid = rtx_div_up((uint64_t)(nb > nc ? nb : nc), BS);
         grid = grid < 1 ? 1 : (grid > 2048 ? 2048 : grid);
@@ -813,17 +830,29 @@ static bool build_sort_phase(rtx_scene* s, int depth, hipStream_t st, const Buil
         uint32_t grid = rtx_div_up(T, BS);
         grid = grid > 1024 ? 1024 : grid;
         hipLaunchKernelGGL(k_morton, dim3(grid), dim3(BS), 0, st, W.centroid, s->b().gbox.as<float>(),
-                           s->b().codes.as<uint32_t>(), s->b().perm.as<uint32_t>(), W.sort_hist, T, depth, passes);
+                           s->b().codes.as<uint32_t>(), s->b().perm.as<uint32_t>(), W.sort_hist, T, depth, passes,
+                           rb);
     }
     uint32_t *kin = s->b().codes.as<uint32_t>(), *vin = s->b().perm.as<uint32_t>();
     uint32_t *kout = W.keys_alt, *vout = W.vals_alt;
     for (int pass = 0; pass < passes; ++pass) {
         ProfScope ps(pc, st, "lbvh_sort_pass");
-        int shift = 8 * pass;
-        int bits = depth - shift < 8 ? depth - shift : 8;
-        hipLaunchKernelGGL(k_sort_pass, dim3(sort_tiles), dim3(BS), 0, st, kin, vin, kout, vout, T, shift, bits,
-                           W.sort_hist + 256 * pass, W.sort_status + (size_t)pass * sort_tiles * 256,
-                           tickets + TK_SORT + pass);
+        const int shift = rb * pass;
+        const int bits = depth - shift < rb ? depth - shift : rb;
+        uint32_t* const hist = W.sort_hist + ((size_t)pass << rb);
+        uint32_t* const status = W.sort_status + ((size_t)pass * sort_tiles << rb);
+#define RTX_SORT_PASS(R, I)                                                                                    \
+    hipLaunchKernelGGL((k_sort_pass<R, I>), dim3(sort_tiles), dim3(BS), 0, st, kin, vin, kout, vout, T, shift, \
+                       bits, hist, status, tickets + TK_SORT + pass)
+        if (rb == 4 && items == 32)
+            RTX_SORT_PASS(4, 32);
+        else if (rb == 4)
+            RTX_SORT_PASS(4, 16);
+        else if (items == 32)
+            RTX_SORT_PASS(8, 32);
+        else
+            RTX_SORT_PASS(8, 16);
+#undef RTX_SORT_PASS
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -935,7 +964,8 @@ int32_t rtx_lbvh_reserve(rtx_scene* s, int depth) { return lbvh_alloc(s, s->T, d
 // rtx_lbvh_build_noalloc runs the same kernels as rtx_lbvh_build with the
 // build scratch carved from `scratch`, a fixed array that every GPU holds for
 // the life of the process, so that it calls no allocator.  Sized for
-// RTX_LBVH_NOALLOC_MAX_TRIS triangles at any depth <= 32 (4 sort passes, 33
+// RTX_LBVH_NOALLOC_MAX_TRIS triangles at any depth <= 32 (4 sort passes of
+// 8-bit digits or 8 of 4-bit digits, whose look-back words fit the same space; 33
 // levels).
 namespace {
 constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }

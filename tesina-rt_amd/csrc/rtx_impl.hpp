@@ -68,6 +68,8 @@ struct rtx_tuning {
                                 // 4 stackless walk by parent pointers
     int64_t trace_block = 128;  // 64, 128 or 256 threads per traversal block
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
+    int64_t sort_bits = 8;      // LBVH radix sort digit width: 8 or 4 bits a pass
+    int64_t sort_items = 16;    // LBVH radix sort: pairs per thread of a tile (16 or 32)
     int64_t stack_lds = 32;     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill

@@ -94,6 +94,31 @@ def test_lbvh_params(ctx, depth, min_leaf):
     s.destroy()
 
 
+@pytest.mark.parametrize("bits,items", [(4, 16), (4, 32), (8, 32), (8, 16)])
+@pytest.mark.parametrize("T,depth,seed", [(50_000, 30, 31), (20_000, 32, 32), (9000, 9, 33), (5, 30, 34),
+                                          (70_000, 7, 35)])
+def test_lbvh_sort_settings(ctx, bits, items, T, depth, seed):
+    # every digit width and tile size of the radix sort gives the same stable
+    # (code, index) order, hence the same tree; odd and even pass counts
+    pos, idx, nrm, uv, mm = random_scene(seed, T)
+    b = O.Build(pos, idx, mm, depth, 1)
+    old = (ctx.tuned("sort_bits"), ctx.tuned("sort_items"))
+    try:
+        ctx.tune("sort_bits", bits)
+        ctx.tune("sort_items", items)
+        s = load_scene(ctx, pos, idx, nrm, uv, mm)
+        s.build(depth, 1)
+        check_build(s, b, T)
+        pos2 = (pos * np.float32(0.9) + np.float32(0.5)).astype(np.float32)
+        s.update_positions(pos2)  # a rebuild from the permuted buffers
+        s.build(depth, 1)
+        check_build(s, O.Build(pos2, b.idx.reshape(-1, 3), b.matmap, depth, 1), T)
+        s.destroy()
+    finally:
+        ctx.tune("sort_bits", old[0])
+        ctx.tune("sort_items", old[1])
+
+
 def test_lbvh_duplicates_and_flat_axis(ctx):
     # many identical triangles (equal codes never split) and a flat global axis
     pos, idx, nrm, uv, mm = random_scene(42, 3000, V=50, flat_axis=1)
