@@ -227,7 +227,7 @@ int32_t rtx_render_frame(rtx_renderer* r, rtx_scene* s, const rtx_camera* cam, i
         return e;
     if ((e = rtx_trace(s, r->rays[0], r->hits[0], P, st))) return e;
     if ((e = rtx_trace_shadow_impl(s, r->rays[0], r->hits[0], P, (uint64_t*)(cnt + 1), st))) return e;
-    if ((e = rtx_shade(s, r->rays[0], r->hits[0], r->fb, P, 1, st))) return e;
+    if ((e = rtx_shade_impl(s, r->rays[0], r->hits[0], r->fb, P, 1, true, st))) return e;  // one ray a pixel
     uint64_t n = P;
     for (int b = 0; b < r->p.max_bounces; ++b) {
         rtx_rays *in = r->rays[b], *out = r->rays[b + 1];
@@ -283,7 +283,7 @@ int32_t rtx_render_instances(rtx_renderer* r, rtx_instances* in, const rtx_camer
         return e;
     if ((e = rtx_trace_instances(in, r->rays[0], r->hits[0], P, st))) return e;
     if ((e = rtx_trace_shadow_instances_impl(in, r->rays[0], r->hits[0], P, (uint64_t*)(cnt + 1), st))) return e;
-    if ((e = rtx_shade(sh, r->rays[0], r->hits[0], r->fb, P, 1, st))) return e;
+    if ((e = rtx_shade_impl(sh, r->rays[0], r->hits[0], r->fb, P, 1, true, st))) return e;  // one ray a pixel
     uint64_t n = P;
     for (int b = 0; b < r->p.max_bounces; ++b) {
         rtx_rays *rin = r->rays[b], *rout = r->rays[b + 1];
@@ -368,7 +368,8 @@ static int32_t render_stages(rtx_renderer* r, rtx_scene* s, const rtx_camera* ca
             RTX_CHECK_HIP(hipStreamWaitEvent(sd, r->ev[b], 0));
         }
         if ((e = rtx_trace_shadow_impl(s, r->rays[b], r->hits[b], nb, (uint64_t*)(cnt + 1), sd))) return e;
-        if ((e = rtx_shade(s, r->rays[b], r->hits[b], r->fb, nb, b == 0, sd))) return e;
+        // the primary rays have one sample a pixel: no two add to one pixel
+        if ((e = rtx_shade_impl(s, r->rays[b], r->hits[b], r->fb, nb, b == 0, b == 0, sd))) return e;
     }
     if (two) {
         RTX_CHECK_HIP(hipEventRecord(r->ev_join, sd));

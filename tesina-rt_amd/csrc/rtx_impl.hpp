@@ -360,6 +360,8 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
 // Render copy k of what the render stages read: all of it, or (shading
 // only) the normals and material ids, whose buffers the next frame's
 // animation and permutation overwrite in place.
+int32_t rtx_shade_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint64_t count, int32_t primary,
+                       bool unique_pixels, void* stream);
 int32_t rtx_refresh_render_copy(rtx_scene* s, int k, hipStream_t st, bool shading_only = false);
 int32_t rtx_copy_render_state(rtx_scene* s, rtx_scene::RenderCopy& rc, hipStream_t st, bool shading_only);
 int32_t rtx_reserve_render_copy(rtx_scene* s, int k, bool shading_only = false);

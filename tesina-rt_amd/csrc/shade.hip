@@ -93,11 +93,20 @@ __device__ v3 cubemap_rgb(const Images& im, v3 d) {
     return mk3(0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ void add_pixel(uint32_t* __restrict__ accum, uint32_t px, const float ch[3]) {
+// unique: no other ray of this launch adds to this pixel (the primary rays
+// of a frame, one per pixel), so a plain read-add-write gives what the
+// atomic gives (the reference's shade_primary adds with +=,
+// src/kernel/ray-shader.cl:318-330) without a device-scope atomic, which
+// on this chip goes past the XCD's L2 to memory.
+__device__ __forceinline__ void add_pixel(uint32_t* __restrict__ accum, uint32_t px, const float ch[3], bool unique) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         uint32_t u = (uint32_t)floorf(clamp01(ch[q]) * 1e6f);
-        if (u) atomicAdd(&accum[3 * (size_t)px + q], u);
+        if (!u) continue;
+        if (unique)
+            accum[3 * (size_t)px + q] += u;
+        else
+            atomicAdd(&accum[3 * (size_t)px + q], u);
     }
 }
 
@@ -109,14 +118,14 @@ __device__ __forceinline__ void shade_ray(uint64_t r, const float4* __restrict__
                                           const float4* __restrict__ hp, const float2* __restrict__ huv,
                                           const rtx_material* __restrict__ mats, const int32_t* __restrict__ matmap,
                                           const rtx_lights* __restrict__ lights, int nl, v3 amb, const Images& im,
-                                          uint32_t* __restrict__ accum) {
+                                          uint32_t* __restrict__ accum, bool unique) {
     uint32_t fl = flags[r];
     if (!(fl & 1u)) {  // a miss adds the cubemap's colour, or nothing without one
         if (!im.cube) return;
         const v3 rgb = cubemap_rgb(im, xyz(dir4[r]));
         const float c = contrib[r];
         const float ch[3] = {c * rgb.x, c * rgb.y, c * rgb.z};
-        add_pixel(accum, (uint32_t)pixel[r], ch);
+        add_pixel(accum, (uint32_t)pixel[r], ch, unique);
         return;
     }
     const rtx_material m = mats[matmap[hid[r]]];
@@ -160,7 +169,7 @@ __device__ __forceinline__ void shade_ray(uint64_t r, const float4* __restrict__
     val = val * (1.f - m.reflectiveness);
     const float c = contrib[r];
     const float ch[3] = {val.x * c, val.y * c, val.z * c};
-    add_pixel(accum, (uint32_t)pixel[r], ch);
+    add_pixel(accum, (uint32_t)pixel[r], ch, unique);
 }
 
 __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, const int32_t* __restrict__ pixel,
@@ -171,12 +180,13 @@ __global__ __launch_bounds__(256) void k_shade(const float4* __restrict__ dir4, 
                                                const int32_t* __restrict__ matmap,
                                                const rtx_lights* __restrict__ lights, const Images im,
                                                const uint32_t* dcount, uint64_t hcount,
-                                               uint32_t* __restrict__ accum) {
+                                               uint32_t* __restrict__ accum, int unique) {
     const uint64_t n = hcount == RTX_DEVICE_COUNT ? (uint64_t)*dcount : hcount;
     const int nl = lights->count;
     const v3 amb = mk3(lights->ambient[0], lights->ambient[1], lights->ambient[2]);
     for (uint64_t r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256)
-        shade_ray(r, dir4, pixel, contrib, flags, hid, hn, hp, huv, mats, matmap, lights, nl, amb, im, accum);
+        shade_ray(r, dir4, pixel, contrib, flags, hid, hn, hp, huv, mats, matmap, lights, nl, amb, im, accum,
+                  unique != 0);
 }
 
 __global__ void k_resolve(const uint32_t* __restrict__ accum, uchar4* __restrict__ rgba, uint32_t P) {
@@ -205,8 +215,10 @@ __global__ __launch_bounds__(256) void k_wf_shade(const WfSeg* __restrict__ segs
         if (r >= (uint64_t)S.count[0]) continue;
         const Images im{S.tex, S.tex_tab, S.tex ? S.ntex : 0, S.tex && S.cube ? 1 : 0};
         const v3 amb = mk3(S.lights->ambient[0], S.lights->ambient[1], S.lights->ambient[2]);
+        // a frame's primary rays (one per pixel) are shaded in the step that
+        // generated them
         shade_ray(r, S.dir4, S.pixel, S.contrib, S.flags, S.id, S.n4, S.p4, S.uv2, S.mats, S.matmap, S.lights,
-                  S.lights->count, amb, im, S.accum);
+                  S.lights->count, amb, im, S.accum, S.primary != 0);
     }
 }
 
@@ -295,6 +307,14 @@ int32_t rtx_fb_read_rgba8(rtx_fb* fb, uint8_t* rgba) {
 
 int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint64_t count, int32_t primary,
                   void* stream) {
+    // the caller's primary rays may hold several samples per pixel
+    return rtx_shade_impl(s, rays, hits, fb, count, primary, false, stream);
+}
+
+// unique_pixels: every ray of the bundle has its own pixel (a renderer's
+// primary rays, one sample per pixel).
+int32_t rtx_shade_impl(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint64_t count, int32_t primary,
+                       bool unique_pixels, void* stream) {
     (void)primary;
     RTX_REQUIRE(s && rays && hits && fb, RTX_EINVAL, "rtx_shade: null argument");
     RTX_REQUIRE(s->mats.p && s->lights.p, RTX_ESTATE, "rtx_shade: scene has no materials or lights");
@@ -312,7 +332,7 @@ int32_t rtx_shade(rtx_scene* s, rtx_rays* rays, rtx_hits* hits, rtx_fb* fb, uint
                        s->mats.as<rtx_material>(), scene_view(s).matmap, s->lights.as<rtx_lights>(),
                        Images{s->tex.as<uchar4>(), s->tex_tab.as<TexEnt>(), s->tex.p ? s->ntex : 0,
                               s->tex.p && s->cube ? 1 : 0},
-                       rays->count.as<uint32_t>(), count, fb->accum.as<uint32_t>());
+                       rays->count.as<uint32_t>(), count, fb->accum.as<uint32_t>(), unique_pixels ? 1 : 0);
     RTX_CHECK_HIP(hipGetLastError());
     return RTX_OK;
 }
