@@ -338,15 +338,19 @@ def main():
         return dt / steps * 1e3
 
     tune_runs = {}
-    for tb, coop in ((128, 8), (64, 8), (64, 4), (128, 4)):
+    for tb, coop, sl in ((128, 8, 32), (64, 8, 32), (64, 4, 32), (128, 4, 32), (128, 8, 16)):
         ctx.tune("trace_block", tb)
         ctx.tune("coop_leaf", coop)
-        tune_runs[(tb, coop)] = flight_ms(3, 10)
-    tb, coop = min(tune_runs, key=tune_runs.get)
+        ctx.tune("stack_lds", sl)
+        tune_runs[(tb, coop, sl)] = flight_ms(3, 10)
+    tb, coop, sl = min(tune_runs, key=tune_runs.get)
     ctx.tune("trace_block", tb)
     ctx.tune("coop_leaf", coop)
-    log(f"frame pipeline tuning: {', '.join(f'block {k[0]} coop {k[1]}: {v:.3f} ms' for k, v in tune_runs.items())}"
-        f"; chosen block {tb}, coop {coop}")
+    ctx.tune("stack_lds", sl)
+    log(f"frame pipeline tuning: {', '.join(f'block {k[0]} coop {k[1]} stack {k[2]}: {v:.3f} ms' for k, v in tune_runs.items())}"
+        f"; chosen block {tb}, coop {coop}, stack_lds {sl}")
+    flight_tune = [{"trace_block": k[0], "coop_leaf": k[1], "stack_lds": k[2], "ms_per_frame": round(v, 4)}
+                   for k, v in tune_runs.items()]
     flights = {}
     for depth in (2, 3, 4):
         flights[depth] = flight_ms(depth, args.steps)
@@ -363,7 +367,8 @@ def main():
         "frames_in_flight": (flights[best_depth], f"frame pipeline with {best_depth} frames in flight (stream "
                                                   "launches): the next frames' LBVH rebuilds and the renders of "
                                                   f"consecutive frames overlap; traversal work-group {tb}, "
-                                                  f"cooperative leaves from {coop} triangles"),
+                                                  f"cooperative leaves from {coop} triangles, {sl} LDS stack "
+                                                  "entries"),
     }
     best_mode = min(modes, key=lambda k: modes[k][0])
     ms_per_step = modes[best_mode][0]
@@ -476,6 +481,7 @@ def main():
         "lbvh_build_wall_ms": round(build_wall_ms, 4),
         "frame_mode": frame_mode,
         "frames_in_flight_ms_per_frame": {str(k): round(v, 4) for k, v in flights.items()},
+        "frames_in_flight_autotune": flight_tune,
         "pipelined_graph_ms_per_frame": round(pip_graph_ms, 4),
         "pipelined_stage_launch_ms_per_frame": round(pip_eager_ms, 4),
         "sequential_graph_ms_per_frame": round(seq_ms, 4),
@@ -827,17 +833,16 @@ def bench_c4_wavefront(ctx, rs, scs, cams, rays_expected, reps):
         return (time.perf_counter() - t) / n
 
     tune = []
-    for var, tb, coop, grain in [(1, 128, 8, 1), (1, 128, 8, 2), (1, 128, 8, 4), (1, 64, 8, 1), (1, 256, 8, 1),
-                                 (0, 128, 8, 1), (2, 128, 8, 1), (1, 128, 4, 1), (1, 128, 16, 1)]:
-        ctx.tune("trace_variant", var)
-        ctx.tune("trace_block", tb)
-        ctx.tune("coop_leaf", coop)
-        ctx.tune("wf_grain", grain)
+    keys = ("trace_variant", "trace_block", "coop_leaf", "wf_grain", "stack_lds")
+    for cand in [(1, 128, 8, 1, 32), (1, 128, 8, 2, 32), (1, 128, 8, 4, 32), (1, 64, 8, 1, 32), (1, 256, 8, 1, 32),
+                 (0, 128, 8, 1, 32), (2, 128, 8, 1, 32), (1, 128, 4, 1, 32), (1, 128, 16, 1, 32),
+                 (1, 128, 8, 1, 16), (1, 256, 8, 1, 16), (1, 128, 8, 1, 8)]:
+        for key, v in zip(keys, cand):
+            ctx.tune(key, v)
         batch()
-        tune.append({"trace_variant": var, "trace_block": tb, "coop_leaf": coop, "wf_grain": grain,
-                     "ms": round(timed(2) * 1e3, 3)})
+        tune.append({**dict(zip(keys, cand)), "ms": round(timed(2) * 1e3, 3)})
     best = min(tune, key=lambda x: x["ms"])
-    for key in ("trace_variant", "trace_block", "coop_leaf", "wf_grain"):
+    for key in keys:
         ctx.tune(key, best[key])
     batch()
     assert last["rays"] == rays_expected, "the wavefront batch must trace the one-by-one frames' rays"
@@ -852,6 +857,7 @@ def bench_c4_wavefront(ctx, rs, scs, cams, rays_expected, reps):
     ctx.tune("trace_block", 128)
     ctx.tune("coop_leaf", 8)
     ctx.tune("wf_grain", 1)
+    ctx.tune("stack_lds", 32)
     return {"what": "the 64 frames admitted together into the wavefront renderer and stepped to completion "
                     "(admission, steps, device completion and release timed)",
             "ms_per_batch": round(dt * 1e3, 3), "mrays": round(rays_expected / dt / 1e6, 2),
