@@ -220,6 +220,10 @@ def main():
                          "(primary rays and the first bounce; CPU only) and exit")
     ap.add_argument("--c3-only", action="store_true",
                     help="only run the dynamic-scene leg (--c3-frames frames) and print its JSON (profiling runs)")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="only run the batched leg (--c4-scenes scenes) and print its JSON (profiling runs)")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="only run the serving leg (--c5-rates) and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -233,9 +237,17 @@ def main():
     if args.visit_counts:
         print(json.dumps(visit_count_table()), flush=True)
         return
-    if args.c3_only:
+    if args.c3_only or args.c4_only or args.c5_only:
         import rtx
-        print(json.dumps({"c3_dynamic": bench_c3(rtx.Context(0), max(1, args.c3_frames))}), flush=True)
+        ctx = rtx.Context(0)
+        out = {}
+        if args.c3_only:
+            out["c3_dynamic"] = bench_c3(ctx, max(1, args.c3_frames))
+        if args.c4_only:
+            out["c4_batched"] = bench_c4(ctx, max(1, args.c4_scenes))
+        if args.c5_only:
+            out["c5_serving"] = bench_c5(ctx, [int(x) for x in args.c5_rates.split(",") if x], args.c5_max_requests)
+        print(json.dumps(out), flush=True)
         return
 
     import numpy as np
