@@ -1,15 +1,10 @@
 """Shared pytest setup: markers, import paths, GPU gating."""
-import os
 import sys
 from pathlib import Path
 
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
-# pipelines, batches and the server run several streams at once; each needs a
-# hardware queue of its own (HIP's default is 4 per process).  Set before any
-# HIP initialisation.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, str(ROOT / "tesina-rt_amd"))
 sys.path.insert(0, str(ROOT))
 
