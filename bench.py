@@ -263,11 +263,11 @@ def main():
     R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
 
     # warm-up (also first-touch of every buffer), then capture the frame
-    # (LBVH rebuild + render) into a hipGraph.  Graphs are captured from one
-    # stream: with a hardware queue per stream, HIP's graph executor makes
-    # every fork and join a cross-queue wait, which costs more than the
-    # overlap of a side stream gains; stage-by-stage launches keep the side
-    # stream.
+    # (LBVH rebuild + render) into a hipGraph.  Both graph shapes are timed:
+    # captured from one stream (linear), or with each bounce's shadow rays and
+    # shading on a side stream (forked).  Which is faster depends on the
+    # hardware queues: with a queue per stream HIP's graph executor turns every
+    # fork and join into a cross-queue wait.
     for _ in range(max(1, args.warmup)):
         R.frame_async(s, cam, rebuild=True)
     ctx.device_sync()
@@ -275,46 +275,40 @@ def main():
     rays_per_frame = st.primary_rays + st.secondary_rays
     shadow_per_frame = st.shadow_rays
     log(f"rays/frame: primary {st.primary_rays} secondary {st.secondary_rays} shadow {st.shadow_rays}")
-    ctx.tune("side_stream", 0)
-    R.capture(s, cam, rebuild=True)
-    for _ in range(args.warmup):
-        R.replay()
-    ctx.device_sync()
 
-    # timed region: K graph replays, no host work inside
-    ctx.device_sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        R.replay()
-    ctx.device_sync()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt / args.steps * 1e3
-    value = rays_per_frame * args.steps / dt / 1e6
-    st2 = R.last_stats()
-    assert st2.primary_rays + st2.secondary_rays == rays_per_frame, "replayed frame differs in ray count"
-    log(f"timed {args.steps} graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
-    seq_ms, seq_value = ms_per_step, value
+    def graph_ms(pipelined: bool, side: int):
+        ctx.tune("side_stream", side)
+        R.set_pipelined(pipelined)
+        R.frame_async(s, cam, rebuild=True)
+        ctx.device_sync()
+        R.capture(s, cam, rebuild=True)
+        for _ in range(args.warmup):
+            R.replay()
+        # timed region: K graph replays, no host work inside
+        ctx.device_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            R.replay()
+        ctx.device_sync()
+        dt = time.perf_counter() - t0
+        stg = R.last_stats()
+        assert stg.primary_rays + stg.secondary_rays == rays_per_frame, "replayed frame differs in ray count"
+        return dt / args.steps * 1e3
 
-    # pipelined frames (the reference's RendererT): the next frame's LBVH
-    # build on a second stream while this frame renders; captured likewise
+    graph_shapes = {}
+    for pipelined in (False, True):
+        for side in (0, 1):
+            graph_shapes[("pipelined" if pipelined else "sequential", "forked" if side else "linear")] = \
+                graph_ms(pipelined, side)
+    for k, v in graph_shapes.items():
+        log(f"timed {args.steps} {k[0]} graph frames, {k[1]}: {v:.3f} ms/frame, "
+            f"{rays_per_frame / v / 1e3:.1f} Mrays/s")
+    seq_shape = min(("linear", "forked"), key=lambda k: graph_shapes[("sequential", k)])
+    pip_shape = min(("linear", "forked"), key=lambda k: graph_shapes[("pipelined", k)])
+    seq_ms = graph_shapes[("sequential", seq_shape)]
+    seq_value = rays_per_frame / seq_ms / 1e3
+    pip_graph_ms = graph_shapes[("pipelined", pip_shape)]
     R.set_pipelined(True)
-    R.frame_async(s, cam, rebuild=True)
-    ctx.device_sync()
-    R.capture(s, cam, rebuild=True)
-    for _ in range(args.warmup):
-        R.replay()
-    ctx.device_sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        R.replay()
-    ctx.device_sync()
-    dt = time.perf_counter() - t0
-    st3 = R.last_stats()
-    assert st3.primary_rays + st3.secondary_rays == rays_per_frame, "pipelined frame differs in ray count"
-    ms_per_step = dt / args.steps * 1e3
-    value = rays_per_frame * args.steps / dt / 1e6
-    log(f"timed {args.steps} pipelined graph frames: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s")
-    pip_graph_ms = ms_per_step
     # the same pipelined frames launched stage by stage (no graph)
     ctx.tune("side_stream", 1)
     for _ in range(args.warmup):
@@ -372,8 +366,9 @@ def main():
     # frame does one full LBVH rebuild and one full render in every mode)
     best_depth = min(flights, key=flights.get)
     modes = {
-        "pipelined_graph": (pip_graph_ms, "hipGraph replay of the pipelined frame: LBVH rebuild of the next frame "
-                                          "on a second stream while this frame renders (the reference's RendererT)"),
+        "pipelined_graph": (pip_graph_ms, f"hipGraph replay ({pip_shape} capture) of the pipelined frame: LBVH "
+                                          "rebuild of the next frame on a second stream while this frame renders "
+                                          "(the reference's RendererT)"),
         "pipelined_stage_launch": (pip_eager_ms, "the pipelined frame (the reference's RendererT) launched stage by "
                                                  "stage, shadows and shading of each bounce on a side stream"),
         "frames_in_flight": (flights[best_depth], f"frame pipeline with {best_depth} frames in flight (stream "
@@ -495,6 +490,7 @@ def main():
         "frames_in_flight_ms_per_frame": {str(k): round(v, 4) for k, v in flights.items()},
         "frames_in_flight_autotune": flight_tune,
         "pipelined_graph_ms_per_frame": round(pip_graph_ms, 4),
+        "graph_ms_per_frame_by_shape": {f"{k[0]}_{k[1]}": round(v, 4) for k, v in graph_shapes.items()},
         "pipelined_stage_launch_ms_per_frame": round(pip_eager_ms, 4),
         "sequential_graph_ms_per_frame": round(seq_ms, 4),
         "sequential_graph_mrays": round(seq_value, 3),
