@@ -1312,13 +1312,22 @@ def bench_sort_scaling(ctx, sizes):
 
 
 def _cpu_model() -> str:
+    """The host CPU's model name, with its family / model / stepping numbers
+    (a virtualized host may report only the product line)."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping") and k not in fields:
+                    fields[k] = v
+            if not line.strip() and fields:
+                break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = ", ".join(f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields)
+    return f"{name} ({ids})" if ids else name
 
 
 if __name__ == "__main__":
