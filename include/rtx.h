@@ -187,6 +187,12 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    chunk per wave in launch order (the faster for one large
  *                    frame); 1 (default) persistent waves over per-XCD queues
  *                    (the faster for many frames).
+ *   "shadow_order":  0 (default): shadow rays visit children in index order
+ *                    with no near-child delta fetched wherever the order cannot
+ *                    change the result (trees below 32 levels: the any-hit
+ *                    result is whether some leaf whose boxes pass at the fixed
+ *                    tMax holds a hit); 1: the reference's near-child order
+ *                    everywhere;
  *   "stack_lds":     traversal stack entries per lane kept in LDS by the
  *                    single-node and sibling-pair forms (scene and wavefront
  *                    launches): 32 (the whole MAX_LEVELS stack), 16 or 8 (a

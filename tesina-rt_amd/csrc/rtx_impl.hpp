@@ -70,7 +70,9 @@ struct rtx_tuning {
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t sort_bits = 8;      // LBVH radix sort digit width: 8 or 4 bits a pass
     int64_t sort_items = 16;    // LBVH radix sort: pairs per thread of a tile (16 or 32)
-    int64_t stack_lds = 32;     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
+    int64_t stack_lds = 32;
+    int64_t shadow_order = 0;   // 1: shadow rays visit children in the reference's near-child order everywhere;
+                                // 0: in index order (no delta fetch) wherever the order cannot change the result     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
     int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
@@ -465,7 +467,7 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
 void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
                   uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st);
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                   uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st);
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, bool ordered, hipStream_t st);
 void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st);
 void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st);
 // Also copies the counters of each resolving frame to fin_out[S.fin] (4 x u64).

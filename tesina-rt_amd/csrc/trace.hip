@@ -500,7 +500,12 @@ __device__ __forceinline__ void traverse_closest_node(const float4* __restrict__
     }
 }
 
-template <bool SPEC, class Stack>
+// ORDERED = false: children in index order (left first) with no near-child
+// delta fetched.  An any-hit result is the existence of a hit in a leaf whose
+// box and ancestors' boxes pass at the fixed tMax, which does not depend on
+// the order, unless a ray's stack can reach MAX_LEVELS (trees of 32 levels);
+// the host keeps the reference's order for those.
+template <bool SPEC, bool ORDERED, class Stack>
 __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                   const float4* __restrict__ tri, const RayC& r, float tmax,
                                                   Stack& stk, uint32_t coop, bool start) {
@@ -513,7 +518,7 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
         if (going) {
             const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
             float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (SPEC) {
+            if (SPEC && ORDERED) {
                 dl4 = delta[curr];
                 __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
             }
@@ -530,8 +535,8 @@ __device__ __forceinline__ bool traverse_any_node(const float4* __restrict__ nod
                         pop = false;
                     }
                 } else {
-                    if (!SPEC) dl4 = delta[curr];
-                    const bool nl = dot(r.d, xyz(dl4)) > 0.f;
+                    if (!SPEC && ORDERED) dl4 = delta[curr];
+                    const bool nl = !ORDERED || dot(r.d, xyz(dl4)) > 0.f;
                     stk.put(level, nl ? a + 1 : a);
                     level++;
                     curr = nl ? a : a + 1;
@@ -668,11 +673,13 @@ __device__ __forceinline__ void closest_var(const float4* __restrict__ node, con
     }
 }
 
+// ordered: visit children in the reference's near-child order (else any order,
+// see traverse_any_node; wave-uniform).
 template <int VAR, class Stack>
 __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const float4* __restrict__ delta,
                                         const float4* __restrict__ tri, const RayC& r, float tmax, Stack& stk,
                                         uint32_t coop, bool start, const uint32_t* __restrict__ parent = nullptr,
-                                        uint32_t bound = 0) {
+                                        uint32_t bound = 0, bool ordered = true) {
     if (VAR == 4) {
         bool h = false;
         if (start) h = traverse_any_ns(node, delta, parent, tri, r, tmax, bound);
@@ -683,7 +690,8 @@ __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const f
         if (start) h = traverse_any(node, tri, r, tmax, stk);
         return h;
     }
-    return traverse_any_node<VAR == 1>(node, delta, tri, r, tmax, stk, coop, start);
+    if (!ordered) return traverse_any_node<false, false>(node, delta, tri, r, tmax, stk, coop, start);
+    return traverse_any_node<VAR == 1, true>(node, delta, tri, r, tmax, stk, coop, start);
 }
 
 // -------------------------------------------------------------- trace ----
@@ -838,7 +846,8 @@ template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, const float4* __restrict__ delta,
                                                const float4* __restrict__ tri, const rtx_lights* __restrict__ lights,
                                                ShadowSegs segs, unsigned long long* shadow_count, uint32_t xcds,
-                                               uint32_t coop, const uint32_t* __restrict__ parent, uint32_t bound) {
+                                               uint32_t coop, const uint32_t* __restrict__ parent, uint32_t bound,
+                                               uint32_t ordered) {
     __shared__ uint32_t s_id[VAR == 4 ? 1 : SR * TB];
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_end[RTX_SEG_MAX];
@@ -902,7 +911,8 @@ __global__ __launch_bounds__(TB) void k_shadow(const float4* __restrict__ node, 
                 rc = make_ray(p, dir, 0.01f);
             }
         }
-        if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace, parent, bound)) shadowed = true;  // whole wave
+        if (any_var<VAR>(node, delta, tri, rc, tmax, stk, coop, trace, parent, bound, ordered != 0))  // whole wave
+            shadowed = true;
         if (valid) set_shadow_bit(&fl[rr], l, shadowed);
     }
     if (shadow_count) {
@@ -2031,7 +2041,7 @@ __global__ __launch_bounds__(TB) void k_wf_trace_ord(const WfSeg* __restrict__ s
 // rays from neighbouring hits).  The whole wave calls it.
 template <int VAR, class Stack>
 __device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Stack& stk, uint32_t coop,
-                                                uint32_t lane) {
+                                                uint32_t lane, bool ordered) {
     const int nl = S.lights->count;
     const uint64_t nh = S.list_count[0];
     const uint64_t it = (uint64_t)cl * RTX_WAVE + lane;
@@ -2066,7 +2076,8 @@ __device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Sta
         }
         if (trace) rc = make_ray(p, dir, 0.01f);
     }
-    if (any_var<VAR>(gptr(S.node), gptr(S.delta), gptr(S.tri), rc, tmax, stk, coop, trace))  // whole wave
+    if (any_var<VAR>(gptr(S.node), gptr(S.delta), gptr(S.tri), rc, tmax, stk, coop, trace, nullptr, 0,
+                     ordered))  // whole wave
         shadowed = true;
     if (valid) set_shadow_bit(gptr(S.flags) + rr, l, shadowed);
     const uint64_t tb = ballot(trace);
@@ -2076,7 +2087,8 @@ __device__ __forceinline__ void wf_shadow_chunk(const WfSeg& S, uint32_t cl, Sta
 // Shadow rays of every frame's pairs, persistent form (ChunkSched).
 template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs, uint32_t nseg,
-                                                  uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain) {
+                                                  uint32_t* __restrict__ queue, uint32_t coop, uint32_t grain,
+                                                  uint32_t ordered) {
     __shared__ uint32_t s_id[SR * TB];
     uint32_t s_low[stack_private_words<SR>()];
     TraversalStack<TB, SR> stk{s_id, threadIdx.x, s_low};
@@ -2088,7 +2100,7 @@ __global__ __launch_bounds__(TB) void k_wf_shadow(const WfSeg* __restrict__ segs
     for (uint32_t c = 0; sch.next(c);) {
         uint32_t c0 = 0;
         const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
-        wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, lane);
+        wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, lane, ordered != 0);
     }
 }
 
@@ -2123,7 +2135,8 @@ __global__ __launch_bounds__(RTX_WF_MAX_SEGS) void k_wf_plan_shadow(const WfSeg*
 // Shadow rays, one chunk per wave in launch order (see k_wf_trace_ord).
 template <int VAR, int TB, int SR>
 __global__ __launch_bounds__(TB) void k_wf_shadow_ord(const WfSeg* __restrict__ segs, uint32_t nseg,
-                                                      const uint32_t* __restrict__ ctr, uint32_t coop) {
+                                                      const uint32_t* __restrict__ ctr, uint32_t coop,
+                                                      uint32_t ordered) {
     __shared__ uint32_t s_id[SR * TB];
     const uint32_t total = ctr[WF_STOTAL];
     const uint32_t c = blockIdx.x * (TB / RTX_WAVE) + threadIdx.x / RTX_WAVE;
@@ -2134,7 +2147,7 @@ __global__ __launch_bounds__(TB) void k_wf_shadow_ord(const WfSeg* __restrict__ 
     wc.load(ctr + WF_STAB, nseg, total);
     uint32_t c0 = 0;
     const uint32_t sg = __builtin_amdgcn_readfirstlane(wc.find(c, c0));
-    wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, threadIdx.x & (RTX_WAVE - 1));
+    wf_shadow_chunk<VAR>(segs[sg], c - c0, stk, coop, threadIdx.x & (RTX_WAVE - 1), ordered != 0);
 }
 
 }  // namespace
@@ -2183,20 +2196,20 @@ void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int 
 }
 
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                   uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, bool ordered, hipStream_t st) {
     if (sched == 0) {
         hipLaunchKernelGGL(k_wf_plan_shadow, dim3(1), dim3(RTX_WF_MAX_SEGS), 0, st, segs, nseg, ctr);
         const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
 #define RTX_WF_SHADOW_ORD(V, B, S)                                                                                \
     hipLaunchKernelGGL((k_wf_shadow_ord<V, B, S>), dim3((uint32_t)(g ? g : 1)), dim3(B), 0, st, segs, nseg, ctr, \
-                       coop)
+                       coop, ordered ? 1u : 0u)
         RTX_DISPATCH_STACK(RTX_WF_SHADOW_ORD, var, tb, sr);
 #undef RTX_WF_SHADOW_ORD
         return;
     }
 #define RTX_WF_SHADOW(V, B, S)                                                                                    \
     hipLaunchKernelGGL((k_wf_shadow<V, B, S>), dim3(resident_grid<k_wf_shadow<V, B, S>>(B, cus)), dim3(B), 0, st, \
-                       segs, nseg, ctr + WF_Q_SHADOW, coop, grain)
+                       segs, nseg, ctr + WF_Q_SHADOW, coop, grain, ordered ? 1u : 0u)
     RTX_DISPATCH_STACK(RTX_WF_SHADOW, var, tb, sr);
 #undef RTX_WF_SHADOW
 }
@@ -2492,7 +2505,10 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
     hipLaunchKernelGGL((k_shadow<V, B, S>), dim3(trace_grid(s->ctx, n, n, B)), dim3(B), 0, st, sv.node,    \
                        sv.delta, sv.tri, s->lights.as<rtx_lights>(), segs,                                     \
                        (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
-                       coop_threshold(s->ctx), sv.parent, stackless_bound(s))
+                       coop_threshold(s->ctx), sv.parent, stackless_bound(s), ordered)
+    // the reference's child order only where a ray's stack can reach MAX_LEVELS
+    const int depth = s->ext ? s->ext->depth : s->depth;
+    const uint32_t ordered = (s->ctx->tune.shadow_order != 0 || depth >= STACK) ? 1u : 0u;
     RTX_DISPATCH_STACK_NS(RTX_SHADOW_LAUNCH, var, tb, (int)s->ctx->tune.stack_lds);
 #undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());

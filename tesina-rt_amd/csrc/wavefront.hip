@@ -327,8 +327,9 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     }
     {
         ProfScope ps(c, st, "wf_shadow");
+        // shadow rays in any child order unless a scene's tree has 32 levels
         rtx_wf_shadow(D, n, ctr, var, tb, (int)c->tune.stack_lds, coop, (uint32_t)c->tune.wf_grain,
-                      (int)c->tune.wf_sched, max_pair_chunks, c->cus, st);
+                      (int)c->tune.wf_sched, max_pair_chunks, c->cus, c->tune.shadow_order != 0 || deep, st);
     }
     {
         ProfScope ps(c, st, "wf_shade");

@@ -7,7 +7,8 @@ leaves, tested by the whole wave) and a flat axis.  Random rays from inside
 and around the scene give long, incoherent traversals.  Hits and shadow
 flags of each form must equal the oracle's bit for bit, with the whole
 traversal stack in LDS and with ring stacks of 16 and 8 LDS entries (the
-deeper levels in private memory).
+deeper levels in private memory), with shadow rays in the reference's child
+order and in index order.
 """
 import numpy as np
 import pytest
@@ -70,18 +71,20 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
     o["flags"] = O.trace_shadow(b, O.lights_from(lights), o["flags"], o["p4"])
     if T >= 300:
         assert (o["id"] >= 0).sum() > n // 8, "the rays should hit the scene"
-    old = (ctx.tuned("trace_variant"), ctx.tuned("stack_lds"))
+    old = (ctx.tuned("trace_variant"), ctx.tuned("stack_lds"), ctx.tuned("shadow_order"))
     try:
-        for variant in (0, 1, 2):
-            for sl in (32, 16, 8):
-                ctx.tune("trace_variant", variant)
-                ctx.tune("stack_lds", sl)
-                rtx.trace(s, r, hb, n)
-                rtx.trace_shadow(s, r, hb, n)
-                g = hb.read(n)
-                for k in ("flags", "id", "t", "bary", "n4", "p4"):
-                    assert np.array_equal(g[k], o[k]), \
-                        f"{k} differs (variant {variant}, stack_lds {sl}, T {T}, depth {depth})"
+        for variant, sl, so in ((0, 32, 0), (0, 16, 0), (0, 8, 0), (1, 32, 0), (1, 16, 0), (1, 8, 0), (2, 32, 0),
+                                (2, 16, 0), (2, 8, 0), (0, 32, 1), (1, 32, 1), (1, 16, 1)):
+            # so: shadow rays in the reference's child order (1) or in index order (0)
+            ctx.tune("trace_variant", variant)
+            ctx.tune("stack_lds", sl)
+            ctx.tune("shadow_order", so)
+            rtx.trace(s, r, hb, n)
+            rtx.trace_shadow(s, r, hb, n)
+            g = hb.read(n)
+            for k in ("flags", "id", "t", "bary", "n4", "p4"):
+                assert np.array_equal(g[k], o[k]), \
+                    f"{k} differs (variant {variant}, stack_lds {sl}, shadow_order {so}, T {T}, depth {depth})"
         # the stackless walk (parent pointers of the LBVH) against the oracle's
         with O.stackless():
             ons = O.trace(b, nrm, uv, ori4, dir4)
@@ -96,6 +99,7 @@ def test_traversal_forms_match_oracle(ctx, T, depth, min_leaf, seed, kind):
     finally:
         ctx.tune("trace_variant", old[0])
         ctx.tune("stack_lds", old[1])
+        ctx.tune("shadow_order", old[2])
     r.destroy()
     hb.destroy()
     s.destroy()
