@@ -187,6 +187,12 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    chunk per wave in launch order (the faster for one large
  *                    frame); 1 (default) persistent waves over per-XCD queues
  *                    (the faster for many frames).
+ *   "wf_share":      wavefront persistent launches (wf_sched 1): percent
+ *                    (1..100) of the blocks that fit on a CU that they take;
+ *                    0 (default) = 100 for a lone renderer and 65 for each of
+ *                    a server's groups when it has several, so that one
+ *                    group's short launches find room beside another group's
+ *                    traversal.
  *   "shadow_order":  0 (default): shadow rays visit children in index order
  *                    with no near-child delta fetched wherever the order cannot
  *                    change the result (trees below 32 levels: the any-hit

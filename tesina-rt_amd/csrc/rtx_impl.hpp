@@ -70,15 +70,18 @@ struct rtx_tuning {
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t sort_bits = 8;      // LBVH radix sort digit width: 8 or 4 bits a pass
     int64_t sort_items = 16;    // LBVH radix sort: pairs per thread of a tile (16 or 32)
-    int64_t stack_lds = 32;
+    int64_t stack_lds = 32;     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
     int64_t shadow_order = 0;   // 1: shadow rays visit children in the reference's near-child order everywhere;
-                                // 0: in index order (no delta fetch) wherever the order cannot change the result     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
+                                // 0: in index order (no delta fetch) wherever the order cannot change the result
     int64_t coop_leaf = 8;      // leaves of >= this many triangles are tested by the whole wave (0 = off)
     int64_t refill = 16;        // variant 3: idle lanes of a wave that trigger a queue refill
     int64_t side_stream = 1;    // renderers trace shadows and shade each bounce on a second stream
     int64_t inst_lane_roots = 2;  // instances, single-node forms: 1 each lane walks its own roots in one loop,
                                   // 2 the same over the roots whose world box its ray meets, 0 wave-uniform
     int64_t wf_grain = 1;         // wavefront traversal: 64-ray chunks a wave takes per queue atomic
+    int64_t wf_share = 0;         // wavefront persistent traversal: percent of the blocks that fit on a CU
+                                  // (1..100); 0 = automatic: 100 for a lone renderer, 65 for each of a
+                                  // server's groups when there are several
     int64_t wf_sched = 1;         // wavefront traversal: 0 one chunk per wave in launch order, 1 persistent waves
 };
 
@@ -465,9 +468,10 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
 // sched 1: persistent waves over per-XCD queues; 0: one chunk per wave in
 // launch order over a grid of `max_chunks` (an upper bound of the chunks).
 void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                  uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st);
+                  uint32_t grain, int sched, uint64_t max_chunks, int cus, int share, hipStream_t st);
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                   uint32_t grain, int sched, uint64_t max_chunks, int cus, bool ordered, hipStream_t st);
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, int share, bool ordered,
+                   hipStream_t st);
 void rtx_wf_shade(const WfSeg* segs, uint32_t nseg, int cus, hipStream_t st);
 void rtx_wf_secondary(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int cus, hipStream_t st);
 // Also copies the counters of each resolving frame to fin_out[S.fin] (4 x u64).
@@ -481,4 +485,7 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
                                 const std::function<int32_t(hipStream_t, const std::vector<int>&)>& before_done);
 uint64_t rtx_wf_arena_size(const rtx_wavefront* wf);
 int64_t rtx_wf_steps_in_flight(const rtx_wavefront* wf);
+// Percent of the resident blocks per CU that this renderer's persistent
+// traversal launches take when the wf_share tune is automatic (0).
+void rtx_wf_set_auto_share(rtx_wavefront* wf, int percent);
 const void* rtx_wf_accum_dev(rtx_wavefront* wf, int32_t frame);

@@ -378,6 +378,10 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
             rtx_wavefront* w = nullptr;
             hipStream_t gs = nullptr;
             if ((e = rtx_wavefront_create(ctx, &wp, &w))) break;
+            // several groups: each persistent traversal launch leaves room on
+            // every CU for the other groups' short launches (shading,
+            // secondary generation), which otherwise wait for it to drain
+            if (G > 1) rtx_wf_set_auto_share(w, 65);
             srv->wf.push_back(w);
             if (hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) != hipSuccess) {
                 rtx_set_error("rtx_server_create: stream creation failed");

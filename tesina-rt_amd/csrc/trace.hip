@@ -2163,10 +2163,13 @@ void rtx_wf_raygen(const WfSeg* segs, uint32_t nseg, uint64_t pixels, int cus, h
     hipLaunchKernelGGL(k_wf_raygen, dim3((uint32_t)(g ? g : 1)), dim3(256), 0, st, segs, nseg);
 }
 
-// Persistent grid of kernel K: as many blocks as fit on every CU at once
-// (the LDS stack and the registers set how many), queried once per kernel.
+// Persistent grid of kernel K: `share` percent (at least one) of the blocks
+// that fit on every CU at once (the LDS stack and the registers set how
+// many; queried once per kernel).  A share below 100 leaves room on every
+// CU for the short launches of another stream, which would otherwise wait
+// until the persistent waves drain.
 template <auto K>
-static uint32_t resident_grid(int block, int cus) {
+static uint32_t resident_grid(int block, int cus, int share) {
     static std::atomic<int> per_cu{0};
     int n = per_cu.load(std::memory_order_relaxed);
     if (n == 0) {
@@ -2174,11 +2177,12 @@ static uint32_t resident_grid(int block, int cus) {
             n = 1280 / block;  // 20 waves per CU
         per_cu.store(n, std::memory_order_relaxed);
     }
-    return (uint32_t)cus * (uint32_t)n;
+    const int m = share >= 100 ? n : std::max(1, n * share / 100);
+    return (uint32_t)cus * (uint32_t)m;
 }
 
 void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                  uint32_t grain, int sched, uint64_t max_chunks, int cus, hipStream_t st) {
+                  uint32_t grain, int sched, uint64_t max_chunks, int cus, int share, hipStream_t st) {
     if (sched == 0) {
         const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
 #define RTX_WF_TRACE_ORD(V, B, S)                                                                                \
@@ -2189,14 +2193,15 @@ void rtx_wf_trace(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int 
         return;
     }
 #define RTX_WF_TRACE(V, B, S)                                                                                   \
-    hipLaunchKernelGGL((k_wf_trace<V, B, S>), dim3(resident_grid<k_wf_trace<V, B, S>>(B, cus)), dim3(B), 0, st, \
+    hipLaunchKernelGGL((k_wf_trace<V, B, S>), dim3(resident_grid<k_wf_trace<V, B, S>>(B, cus, share)), dim3(B), 0, st, \
                        segs, nseg, ctr + WF_Q_TRACE, coop, grain)
     RTX_DISPATCH_STACK(RTX_WF_TRACE, var, tb, sr);
 #undef RTX_WF_TRACE
 }
 
 void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int tb, int sr, uint32_t coop,
-                   uint32_t grain, int sched, uint64_t max_chunks, int cus, bool ordered, hipStream_t st) {
+                   uint32_t grain, int sched, uint64_t max_chunks, int cus, int share, bool ordered,
+                   hipStream_t st) {
     if (sched == 0) {
         hipLaunchKernelGGL(k_wf_plan_shadow, dim3(1), dim3(RTX_WF_MAX_SEGS), 0, st, segs, nseg, ctr);
         const uint64_t g = (max_chunks + tb / RTX_WAVE - 1) / (tb / RTX_WAVE);
@@ -2208,7 +2213,7 @@ void rtx_wf_shadow(const WfSeg* segs, uint32_t nseg, uint32_t* ctr, int var, int
         return;
     }
 #define RTX_WF_SHADOW(V, B, S)                                                                                    \
-    hipLaunchKernelGGL((k_wf_shadow<V, B, S>), dim3(resident_grid<k_wf_shadow<V, B, S>>(B, cus)), dim3(B), 0, st, \
+    hipLaunchKernelGGL((k_wf_shadow<V, B, S>), dim3(resident_grid<k_wf_shadow<V, B, S>>(B, cus, share)), dim3(B), 0, st, \
                        segs, nseg, ctr + WF_Q_SHADOW, coop, grain, ordered ? 1u : 0u)
     RTX_DISPATCH_STACK(RTX_WF_SHADOW, var, tb, sr);
 #undef RTX_WF_SHADOW

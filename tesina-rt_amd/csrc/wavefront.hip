@@ -73,6 +73,7 @@ struct rtx_wavefront {
     StepSlot slot[RING];
     int64_t steps = 0;      // steps enqueued
     int64_t observed = -1;  // newest step seen complete
+    int auto_share = 100;   // persistent traversal share when the wf_share tune is 0
     std::deque<int> done_q; // frames complete and not yet returned by poll
 };
 
@@ -311,6 +312,7 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     if (var > 2 || (var == 2 && deep)) var = 1;
     const int tb = (int)c->tune.trace_block;
     const uint32_t coop = c->tune.coop_leaf > 0 ? (uint32_t)c->tune.coop_leaf : 0xffffffffu;
+    const int share = c->tune.wf_share > 0 ? (int)c->tune.wf_share : wf->auto_share;
     RTX_CHECK_HIP(hipMemcpyAsync(D, T, sizeof(WfSeg) * n, hipMemcpyHostToDevice, st));
     {
         ProfScope ps(c, st, "wf_init");
@@ -323,13 +325,14 @@ int32_t rtx_wavefront_step_impl(rtx_wavefront* wf, hipStream_t st, int32_t* pend
     {
         ProfScope ps(c, st, "wf_trace");
         rtx_wf_trace(D, n, ctr, var, tb, (int)c->tune.stack_lds, coop, (uint32_t)c->tune.wf_grain,
-                     (int)c->tune.wf_sched, max_chunks, c->cus, st);
+                     (int)c->tune.wf_sched, max_chunks, c->cus, share, st);
     }
     {
         ProfScope ps(c, st, "wf_shadow");
         // shadow rays in any child order unless a scene's tree has 32 levels
         rtx_wf_shadow(D, n, ctr, var, tb, (int)c->tune.stack_lds, coop, (uint32_t)c->tune.wf_grain,
-                      (int)c->tune.wf_sched, max_pair_chunks, c->cus, c->tune.shadow_order != 0 || deep, st);
+                      (int)c->tune.wf_sched, max_pair_chunks, c->cus, share, c->tune.shadow_order != 0 || deep,
+                      st);
     }
     {
         ProfScope ps(c, st, "wf_shade");
@@ -475,6 +478,8 @@ int32_t rtx_wavefront_memory(rtx_wavefront* wf, uint64_t* used, uint64_t* capaci
 // ------------------------------------------------- for the request server --
 uint64_t rtx_wf_arena_size(const rtx_wavefront* wf) { return wf->arena.size(); }
 int64_t rtx_wf_steps_in_flight(const rtx_wavefront* wf) { return wf->steps - 1 - wf->observed; }
+
+void rtx_wf_set_auto_share(rtx_wavefront* wf, int percent) { wf->auto_share = std::min(100, std::max(1, percent)); }
 const void* rtx_wf_accum_dev(rtx_wavefront* wf, int32_t frame) {
     return frame_base(wf, wf->f[frame]) + wf->f[frame].L.accum;
 }

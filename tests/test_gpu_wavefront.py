@@ -129,25 +129,29 @@ def test_frames_together_equal_stand_alone(ctx, wf_scenes, expected):
         wf.destroy()
 
 
-@pytest.mark.parametrize("variant,block,sched,stack_lds,shadow_order", [
-    (0, 64, 0, 32, 0), (1, 128, 0, 32, 0), (1, 128, 1, 32, 0), (2, 256, 0, 32, 0), (2, 128, 1, 32, 0),
-    (3, 128, 0, 32, 0), (4, 64, 1, 32, 0), (1, 128, 1, 16, 0), (1, 128, 0, 16, 0), (2, 256, 1, 16, 0),
-    (0, 64, 1, 8, 0), (1, 256, 0, 8, 0), (1, 128, 1, 32, 1), (0, 128, 0, 32, 1)])
-def test_every_traversal_form(ctx, wf_scenes, expected, variant, block, sched, stack_lds, shadow_order):
+@pytest.mark.parametrize("variant,block,sched,stack_lds,shadow_order,share", [
+    (0, 64, 0, 32, 0, 0), (1, 128, 0, 32, 0, 0), (1, 128, 1, 32, 0, 0), (2, 256, 0, 32, 0, 0), (2, 128, 1, 32, 0, 0),
+    (3, 128, 0, 32, 0, 0), (4, 64, 1, 32, 0, 0), (1, 128, 1, 16, 0, 0), (1, 128, 0, 16, 0, 0), (2, 256, 1, 16, 0, 0),
+    (0, 64, 1, 8, 0, 0), (1, 256, 0, 8, 0, 0), (1, 128, 1, 32, 1, 0), (0, 128, 0, 32, 1, 0), (1, 128, 1, 32, 0, 65),
+    (1, 64, 1, 16, 0, 1)])
+def test_every_traversal_form(ctx, wf_scenes, expected, variant, block, sched, stack_lds, shadow_order, share):
     # the sibling-pair form falls back to the single-node form when a scene
     # has 32 levels (the chain); the persistent and stackless forms map to it;
     # sched: closest hit in launch order (0) or by persistent waves (1);
     # stack_lds: traversal stack entries in LDS (the rest in private memory);
     # shadow_order: shadow rays in the reference's child order (1) or in index
-    # order (0; the chain scene's 32 levels keep the reference's order)
+    # order (0; the chain scene's 32 levels keep the reference's order);
+    # share: percent of the resident blocks a persistent launch takes (1: one
+    # block per CU)
     rtx = _rtx()
     old = (ctx.tuned("trace_variant"), ctx.tuned("trace_block"), ctx.tuned("wf_sched"), ctx.tuned("stack_lds"),
-           ctx.tuned("shadow_order"))
+           ctx.tuned("shadow_order"), ctx.tuned("wf_share"))
     ctx.tune("trace_variant", variant)
     ctx.tune("trace_block", block)
     ctx.tune("wf_sched", sched)
     ctx.tune("stack_lds", stack_lds)
     ctx.tune("shadow_order", shadow_order)
+    ctx.tune("wf_share", share)
     wf = rtx.Wavefront(ctx, arena_bytes=2 << 30, max_frames=32)
     try:
         ids = []
@@ -164,6 +168,7 @@ def test_every_traversal_form(ctx, wf_scenes, expected, variant, block, sched, s
         ctx.tune("wf_sched", old[2])
         ctx.tune("stack_lds", old[3])
         ctx.tune("shadow_order", old[4])
+        ctx.tune("wf_share", old[5])
 
 
 def test_frames_joining_at_different_steps(ctx, wf_scenes, expected):
