@@ -9,8 +9,14 @@
 #include "devmath.hpp"
 #include "rtx_impl.hpp"
 
-// One thread walks back one predecessor at a time (used where every thread
-// of a block runs its own look-back, e.g. one per radix digit).
+// One thread walks back on its own (used where every thread of a block runs
+// its own look-back, e.g. one per radix digit).  It reads the words of W
+// predecessors (k, k-1, ..., k-W+1) in one round trip and consumes them in
+// order up to the first that is not ready or holds an inclusive prefix, so a
+// run of aggregate-only predecessors costs one round trip per W tiles rather
+// than per tile.  Words below tile 0 are never consumed: tile 0's word becomes
+// inclusive, and the walk stops there.
+template <int W = 8>
 __device__ __forceinline__ uint32_t lookback_thread(uint32_t* status, uint32_t tile, uint32_t agg,
                                                     uint32_t stride) {
     if (tile == 0) {
@@ -21,17 +27,23 @@ __device__ __forceinline__ uint32_t lookback_thread(uint32_t* status, uint32_t t
     uint32_t acc = 0;
     int k = (int)tile - 1;
     for (;;) {
-        uint32_t s = ld_relaxed(&status[(size_t)k * stride]);
-        if (!(s & (RTX_LB_AGG | RTX_LB_INC))) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
+        uint32_t s[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) s[j] = k - j >= 0 ? ld_relaxed(&status[(size_t)(k - j) * stride]) : 0u;
+        int used = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (!(s[j] & (RTX_LB_AGG | RTX_LB_INC))) break;  // not ready: wait for it
+            acc += s[j] & RTX_LB_VAL;
+            if (s[j] & RTX_LB_INC) {
+                st_relaxed(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
+                return acc;
+            }
+            ++used;
         }
-        acc += s & RTX_LB_VAL;
-        if (s & RTX_LB_INC) break;
-        --k;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+        k -= used;
     }
-    st_relaxed(&status[(size_t)tile * stride], RTX_LB_INC | (acc + agg));
-    return acc;
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
