@@ -204,6 +204,12 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    launches): 32 (the whole MAX_LEVELS stack), 16 or 8 (a
  *                    ring of the top levels; deeper levels spill to the
  *                    lane's private memory), trading LDS for resident waves.
+ *   "sort_bits":     LBVH radix sort digit width: 8 (default, 4 passes of
+ *                    30-bit codes) or 4 (8 passes, longer scatter runs).
+ *   "sort_items":    LBVH radix sort pairs per thread of a tile: 16 (default)
+ *                    or 32.
+ *   "build_grid":    LBVH hierarchy and refit level launches: persistent
+ *                    blocks per CU (1..16, default 8).
  * All settings give identical results. */
 int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value);
 /* Current value, or -1 for an unknown key. */

@@ -25,7 +25,6 @@ constexpr int BOUNDS_GRID = 512;
 constexpr uint32_t FAT_LEAF = 16;
 constexpr int HIER_TOP = 8;    // levels 0..8 (<= 256 nodes) emitted by one block
 constexpr int REFIT_TOP = 10;  // levels 10..0 (<= 1024 nodes) refit by one block
-constexpr int GRID_PER_CU = 2; // persistent grids: blocks per CU
 
 // Counter slots in scene->tickets.
 constexpr int TK_BOUNDS = 0;
@@ -757,7 +756,7 @@ static void fat_leaf_bounds(rtx_scene* s, hipStream_t st, rtx_ctx* pc) {
 // Internal-node AABBs, deepest level first (levels depth-1 .. 0; level
 // `depth` holds only leaves).
 static void refit_internal(rtx_scene* s, hipStream_t st, int depth, rtx_ctx* pc) {
-    const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
+    const uint32_t gmax = (uint32_t)s->ctx->cus * (uint32_t)s->ctx->tune.build_grid;
     for (int d = depth - 1; d > REFIT_TOP; --d) {
         ProfScope ps(pc, st, "lbvh_refit_level");
         uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)s->T;
@@ -898,7 +897,7 @@ static void build_tree_phase(rtx_scene* s, int depth, int min_leaf, hipStream_t 
             ProfScope ps(pc, st, "lbvh_hier_top");
             hipLaunchKernelGGL(k_hier_top, dim3(1), dim3(BS), 0, st, A, W.hier_status, hier_tiles, dtop, depth);
         }
-        const uint32_t gmax = (uint32_t)s->ctx->cus * GRID_PER_CU;
+        const uint32_t gmax = (uint32_t)s->ctx->cus * (uint32_t)s->ctx->tune.build_grid;
         for (int d = dtop + 1; d <= depth; ++d) {
             ProfScope ps(pc, st, "lbvh_hier_level");
             uint64_t maxn = d < 32 ? (1ull << d) : (uint64_t)T;

@@ -70,6 +70,7 @@ struct rtx_tuning {
     int64_t xcds = 1;           // ray chunks = L2 domains (MI355X: 8 XCDs); 1 = plain grid order
     int64_t sort_bits = 8;      // LBVH radix sort digit width: 8 or 4 bits a pass
     int64_t sort_items = 16;    // LBVH radix sort: pairs per thread of a tile (16 or 32)
+    int64_t build_grid = 8;     // LBVH hierarchy and refit levels: persistent blocks per CU (1..16)
     int64_t stack_lds = 32;     // traversal stack entries per lane kept in LDS: 8, 16 or 32 (all)
     int64_t shadow_order = 0;   // 1: shadow rays visit children in the reference's near-child order everywhere;
                                 // 0: in index order (no delta fetch) wherever the order cannot change the result

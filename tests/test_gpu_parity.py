@@ -119,6 +119,23 @@ def test_lbvh_sort_settings(ctx, bits, items, T, depth, seed):
         ctx.tune("sort_items", old[1])
 
 
+@pytest.mark.parametrize("grid", [1, 2, 16])
+def test_lbvh_build_grid(ctx, grid):
+    # the hierarchy and refit levels' persistent blocks per CU change only
+    # how many tiles are in flight, never the tree
+    pos, idx, nrm, uv, mm = random_scene(50 + grid, 120_000)
+    b = O.Build(pos, idx, mm, 30, 1)
+    old = ctx.tuned("build_grid")
+    try:
+        ctx.tune("build_grid", grid)
+        s = load_scene(ctx, pos, idx, nrm, uv, mm)
+        s.build(30, 1)
+        check_build(s, b, idx.shape[0])
+        s.destroy()
+    finally:
+        ctx.tune("build_grid", old)
+
+
 def test_lbvh_duplicates_and_flat_axis(ctx):
     # many identical triangles (equal codes never split) and a flat global axis
     pos, idx, nrm, uv, mm = random_scene(42, 3000, V=50, flat_axis=1)
