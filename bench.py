@@ -267,6 +267,14 @@ def main():
     cam = scenes.camera_of(sd)
     R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
 
+    # the LBVH build's knobs, picked by back-to-back builds of this scene:
+    # radix digit width and pairs per thread of the sort, persistent blocks
+    # per CU of the level launches (every setting builds the same tree)
+    build_tune = lbvh_autotune(ctx, s)
+    log("LBVH build tuning: " + ", ".join(
+        f"bits {t['sort_bits']} items {t['sort_items']} grid {t['build_grid']}: {t['ms_per_build']:.3f} ms"
+        for t in build_tune["candidates"]) + f"; chosen {build_tune['chosen']}")
+
     # warm-up (also first-touch of every buffer), then capture the frame
     # (LBVH rebuild + render) into a hipGraph.  Both graph shapes are timed:
     # captured from one stream (linear), or with each bounce's shadow rays and
@@ -491,6 +499,7 @@ def main():
         "mrays_incl_shadow": round((rays_per_frame + shadow_per_frame) / ms_per_step / 1e3, 3),
         "lbvh_build_ms": round(build_ms, 4) if build_ms else None,
         "lbvh_build_wall_ms": round(build_wall_ms, 4),
+        "lbvh_autotune": build_tune,
         "frame_mode": frame_mode,
         "frames_in_flight_ms_per_frame": {str(k): round(v, 4) for k, v in flights.items()},
         "frames_in_flight_autotune": flight_tune,
@@ -671,6 +680,30 @@ def bench_c3(ctx, frames: int):
            "rays_per_frame_mean": round(rays / frames, 1), "triangles": sd.T}
     s.destroy()
     return out
+
+
+def lbvh_autotune(ctx, s, reps: int = 10):
+    """Times `reps` back-to-back LBVH builds of scene `s` for each setting of
+    the build's knobs and leaves the fastest set on the context."""
+    cands = [(8, 16, 8), (8, 32, 8), (4, 16, 8), (4, 32, 8), (8, 16, 2), (8, 16, 4)]
+    res = []
+    for bits, items, grid in cands:
+        ctx.tune("sort_bits", bits)
+        ctx.tune("sort_items", items)
+        ctx.tune("build_grid", grid)
+        for _ in range(2):
+            s.build(30, 1)
+        ctx.device_sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.build(30, 1)
+        ctx.device_sync()
+        res.append({"sort_bits": bits, "sort_items": items, "build_grid": grid,
+                    "ms_per_build": round((time.perf_counter() - t0) / reps * 1e3, 4)})
+    best = min(res, key=lambda r: r["ms_per_build"])
+    for k in ("sort_bits", "sort_items", "build_grid"):
+        ctx.tune(k, best[k])
+    return {"candidates": res, "chosen": {k: best[k] for k in ("sort_bits", "sort_items", "build_grid")}}
 
 
 def bench_c5(ctx, rates, max_requests: int, workers: int = 128, arena_bytes: int = 32 << 30, groups: int = 2,
