@@ -193,6 +193,10 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *                    a server's groups when it has several, so that one
  *                    group's short launches find room beside another group's
  *                    traversal.
+ *   "srv_step_pixels": request server: primary pixels of new requests one
+ *                    group's step takes at most (default 2500000; 0 = no
+ *                    cap); a step always takes at least one request, and the
+ *                    rest wait at the head of the queue for the next step.
  *   "shadow_order":  0 (default): shadow rays visit children in index order
  *                    with no near-child delta fetched wherever the order cannot
  *                    change the result (trees below 32 levels: the any-hit
@@ -605,7 +609,7 @@ typedef struct rtx_server_params {
     int32_t groups;           /* wavefront renderers on streams of their own, sharing the arena and the
                                  requests in flight (1..8); 0 = 2 */
     int32_t steps_ahead;      /* wavefront steps a group keeps enqueued (1..4): fewer, the sooner a new
-                                 request joins; more, the fewer idle gaps between steps; 0 = 2 */
+                                 request joins; more, the fewer idle gaps between steps; 0 = 1 */
 } rtx_server_params;
 
 typedef struct rtx_request {

@@ -296,6 +296,9 @@ int32_t rtx_tune_set(rtx_ctx* ctx, const char* key, int64_t value) {
     } else if (k == "build_grid") {
         RTX_REQUIRE(value >= 1 && value <= 16, RTX_EINVAL, "rtx_tune_set: build_grid must be in [1, 16]");
         ctx->tune.build_grid = value;
+    } else if (k == "srv_step_pixels") {
+        RTX_REQUIRE(value >= 0, RTX_EINVAL, "rtx_tune_set: srv_step_pixels must be >= 0");
+        ctx->tune.srv_step_pixels = value;
     } else if (k == "wf_share") {
         RTX_REQUIRE(value >= 0 && value <= 100, RTX_EINVAL, "rtx_tune_set: wf_share must be in [0, 100]");
         ctx->tune.wf_share = value;
@@ -327,6 +330,7 @@ int64_t rtx_tune_get(rtx_ctx* ctx, const char* key) {
     if (k == "stack_lds") return ctx->tune.stack_lds;
     if (k == "shadow_order") return ctx->tune.shadow_order;
     if (k == "wf_share") return ctx->tune.wf_share;
+    if (k == "srv_step_pixels") return ctx->tune.srv_step_pixels;
     if (k == "build_grid") return ctx->tune.build_grid;
     if (k == "sort_bits") return ctx->tune.sort_bits;
     if (k == "sort_items") return ctx->tune.sort_items;

@@ -80,6 +80,8 @@ struct rtx_tuning {
     int64_t inst_lane_roots = 2;  // instances, single-node forms: 1 each lane walks its own roots in one loop,
                                   // 2 the same over the roots whose world box its ray meets, 0 wave-uniform
     int64_t wf_grain = 1;         // wavefront traversal: 64-ray chunks a wave takes per queue atomic
+    int64_t srv_step_pixels = 2500000;  // server: new primary pixels a group's step takes at most (0: no
+                                        // cap; a step always takes at least one request)
     int64_t wf_share = 0;         // wavefront persistent traversal: percent of the blocks that fit on a CU
                                   // (1..100); 0 = automatic: 100 for a lone renderer, 65 for each of a
                                   // server's groups when there are several

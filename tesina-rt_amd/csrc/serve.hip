@@ -12,7 +12,10 @@
 // the dispatcher keeps up to `steps_ahead` wavefront steps enqueued per group: each
 // step advances every request of the group by one bounce in shared
 // launches, so requests that arrive while others run join at the next step,
-// whatever their scene, size or depth.  Requests are spread over `groups`
+// whatever their scene, size or depth, up to `srv_step_pixels` new primary
+// pixels a group's step (the others wait at the head for the next step, so
+// that a burst of arrivals does not lengthen one step for every request in
+// it).  Requests are spread over `groups`
 // wavefront renderers, each stepping on a stream of its own with its share
 // of the arena and of the requests in flight, so that the launch tails of
 // one group overlap the work of the others.  A request's latency is done_ns - arrival_ns,
@@ -164,7 +167,9 @@ void rtx_server::run_wavefront() {
     std::vector<int32_t> g_active(G, 0);  // admitted requests with steps still to enqueue, per group
     std::vector<rtx_completion> done;
     const uint64_t arena = rtx_wf_arena_size(wf[0]);
-    const int64_t steps_ahead = p.steps_ahead > 0 ? p.steps_ahead : 2;
+    const int64_t steps_ahead = p.steps_ahead > 0 ? p.steps_ahead : 1;
+    const uint64_t step_px_cap = (uint64_t)ctx->tune.srv_step_pixels;
+    std::vector<uint64_t> g_newpx(G, 0);  // primary pixels admitted into each group's next step
     auto finish = [&](const rtx_completion& c) { done.push_back(c); };
     for (;;) {
         const int64_t now = rtx_now_ns();
@@ -205,6 +210,8 @@ void rtx_server::run_wavefront() {
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g_inflight[a] < g_inflight[b]; });
             for (int g : order) {
                 if (g_inflight[g] >= (size_t)p.workers / G) continue;
+                const uint64_t px = (uint64_t)L.req.width * (uint64_t)L.req.height;
+                if (step_px_cap && g_newpx[g] > 0 && g_newpx[g] + px > step_px_cap) continue;
                 int32_t fr = -1;
                 const int32_t e = rtx_wavefront_add(wf[g], sc, &L.req.camera, L.req.width, L.req.height,
                                                     L.req.bounces, 0, it.id, &fr);
@@ -220,6 +227,7 @@ void rtx_server::run_wavefront() {
                 L.frame = fr;
                 L.start_ns = rtx_now_ns();
                 ++g_inflight[g];
+                g_newpx[g] += px;
                 ++g_active[g];
                 if (L.req.out_accum &&
                     hipHostMalloc(&L.staging, (size_t)L.req.width * L.req.height * 12, hipHostMallocDefault) !=
@@ -253,6 +261,7 @@ void rtx_server::run_wavefront() {
                 return RTX_OK;
             };
             int32_t pend = 0;
+            g_newpx[g] = 0;
             const int32_t e = rtx_wavefront_step_impl(wf[g], gstream[g], &pend, nullptr, copy_out);
             if (e) {  // a failed step fails every request of the group
                 for (auto it = live.begin(); it != live.end();) {
