@@ -196,7 +196,8 @@ uint64_t rtx_mem_in_use(rtx_ctx* ctx);
  *   "srv_step_pixels": request server: primary pixels of new requests one
  *                    group's step takes at most (default 2500000; 0 = no
  *                    cap); a step always takes at least one request, and the
- *                    rest wait at the head of the queue for the next step.
+ *                    rest wait at the head of the queue for the next step;
+ *                    read when a server is created.
  *   "shadow_order":  0 (default): shadow rays visit children in index order
  *                    with no near-child delta fetched wherever the order cannot
  *                    change the result (trees below 32 levels: the any-hit

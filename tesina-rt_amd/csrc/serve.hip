@@ -69,6 +69,7 @@ struct rtx_server {
     hipStream_t stream = nullptr;
     std::vector<rtx_scene*> scenes;
     uint64_t workers_bytes = 0, scenes_bytes = 0, budget = 0;
+    uint64_t step_px_cap = 0;  // the srv_step_pixels tune when the server was created
     std::mutex mu;
     std::condition_variable cv;
     std::priority_queue<rtx_request, std::vector<rtx_request>, Later> pending;  // by arrival
@@ -168,7 +169,6 @@ void rtx_server::run_wavefront() {
     std::vector<rtx_completion> done;
     const uint64_t arena = rtx_wf_arena_size(wf[0]);
     const int64_t steps_ahead = p.steps_ahead > 0 ? p.steps_ahead : 1;
-    const uint64_t step_px_cap = (uint64_t)ctx->tune.srv_step_pixels;
     std::vector<uint64_t> g_newpx(G, 0);  // primary pixels admitted into each group's next step
     auto finish = [&](const rtx_completion& c) { done.push_back(c); };
     for (;;) {
@@ -347,6 +347,7 @@ int32_t rtx_server_create(rtx_ctx* ctx, const rtx_server_params* p, rtx_server**
         RTX_REQUIRE(p->class_w[c] > 0 && p->class_h[c] > 0, RTX_EINVAL, "rtx_server_create: bad class size");
     rtx_server* srv = new rtx_server();
     srv->ctx = ctx;
+    srv->step_px_cap = (uint64_t)ctx->tune.srv_step_pixels;
     srv->p = *p;
     if (srv->p.plain) srv->p.workers = 1;
     if (srv->p.queue_cap <= 0) srv->p.queue_cap = 65536;

@@ -62,10 +62,19 @@ def _alone(ctx, rtx, s, r):
     return acc, st
 
 
-@pytest.mark.parametrize("plain,groups", [(False, 1), (False, 3), (True, 0)])
-def test_served_images_equal_stand_alone(ctx, scenes2, plain, groups):
+@pytest.mark.parametrize("plain,groups,step_pixels", [(False, 1, None), (False, 3, None), (True, 0, None),
+                                                      (False, 2, 1), (False, 2, 20_000)])
+def test_served_images_equal_stand_alone(ctx, scenes2, plain, groups, step_pixels):
+    # step_pixels: new primary pixels a group's step takes at most (1: one new
+    # request a step; the others wait at the head for the next step)
     import rtx
-    srv = rtx.Server(ctx, CLASSES, workers=16, max_bounces=3, plain=plain, arena_bytes=1 << 30, groups=groups)
+    old_cap = ctx.tuned("srv_step_pixels")
+    if step_pixels is not None:
+        ctx.tune("srv_step_pixels", step_pixels)
+    try:
+        srv = rtx.Server(ctx, CLASSES, workers=16, max_bounces=3, plain=plain, arena_bytes=1 << 30, groups=groups)
+    finally:
+        ctx.tune("srv_step_pixels", old_cap)
     ids = [srv.add_scene(s) for _, s in scenes2]
     assert ids == [0, 1]
     reqs = _requests(rtx, 24, 7, rtx.now_ns() + 2_000_000)
