@@ -228,6 +228,9 @@ def main():
                     help="only run the batched leg (--c4-scenes scenes) and print its JSON (profiling runs)")
     ap.add_argument("--c5-only", action="store_true",
                     help="only run the serving leg (--c5-rates) and print its JSON (profiling runs)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only run --profile-frames C2 frames with every launch on one stream (event-timed) and "
+                         "print the two roofline objects: a run whose kernel trace holds only those launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -266,6 +269,10 @@ def main():
     s = scenes.load(rtx.Scene(ctx), sd)
     cam = scenes.camera_of(sd)
     R = rtx.Renderer(ctx, sd.width, sd.height, sd.max_bounces, depth=30, min_leaf=1)
+
+    if args.roofline_only:
+        print(json.dumps(roofline_only(ctx, s, R, cam, sd, args.profile_frames, args.warmup)), flush=True)
+        return
 
     # the LBVH build's knobs, picked by back-to-back builds of this scene:
     # radix digit width and pairs per thread of the sort, persistent blocks
@@ -679,6 +686,37 @@ def bench_c3(ctx, frames: int):
            "sequential_mrays": round(timing["sequential"][1] / sdt / 1e6, 3),
            "rays_per_frame_mean": round(rays / frames, 1), "triangles": sd.T}
     s.destroy()
+    return out
+
+
+def roofline_only(ctx, s, R, cam, sd, frames: int, warmup: int):
+    """The roofline objects of the default bench line over `frames` C2 frames
+    (rebuild + render) launched on one stream with per-kernel HIP events, so
+    that a kernel trace of this run averages the same launches."""
+    ctx.tune("side_stream", 0)
+    for _ in range(max(1, warmup)):
+        R.frame_async(s, cam, rebuild=True)
+    ctx.device_sync()
+    st = R.last_stats()
+    rays = st.primary_rays + st.secondary_rays
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(frames):
+        R.frame_async(s, cam, rebuild=True)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    ctx.tune("side_stream", 1)
+    out = {"frames": frames, "rays_per_frame": rays}
+    for key, kernel, algo in (("trace", "k_trace", None), ("lbvh_sort_pass", "k_sort_pass", 16.0 * sd.T)):
+        c, ms = rep[key]
+        per_frame = c / frames
+        if algo is None:
+            algo = 48.0 * rays / per_frame  # ray in + hit out per ray, over the frame's launches
+        avg_s = ms / c / 1e3
+        ach = algo / avg_s / 1e9
+        out[kernel] = {"launches": c, "avg_launch_ms": round(avg_s * 1e3, 5), "algorithmic_bytes_per_launch": algo,
+                       "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic_from_profiles(kernel)}
     return out
 
 
