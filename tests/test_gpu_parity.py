@@ -136,6 +136,52 @@ def test_lbvh_build_grid(ctx, grid):
         ctx.tune("build_grid", old)
 
 
+def test_lbvh_16m_triangles(ctx):
+    # the sort-scaling size (2^24 triangles, a random soup over 2^23 vertices):
+    # codes, permutation, topology and AABBs bit-exact against the oracle
+    T = 1 << 24
+    pos, idx, nrm, uv, mm = random_scene(24, T, V=T // 2)
+    s = load_scene(ctx, pos, idx, nrm, uv, mm)
+    s.build(30, 1)
+    b = O.Build(pos, idx, mm, 30, 1)
+    check_build(s, b, T)
+    s.destroy()
+
+
+def test_lbvh_64m_triangles_properties(ctx):
+    # the largest sort-scaling size (2^26 triangles) through properties that do
+    # not depend on the size: codes nondecreasing, the permutation a bijection
+    # that keeps equal codes in index order (the stable order), the permuted
+    # index buffer and material map equal to the inputs gathered through it,
+    # and the root's box the global box of the vertices used
+    T = 1 << 26
+    rng = np.random.default_rng(26)
+    V = T // 2
+    pos = rng.uniform(-10.0, 10.0, (V, 3)).astype(np.float32)
+    idx = rng.integers(0, V, (T, 3)).astype(np.uint32)
+    mm = rng.integers(0, 3, T).astype(np.int32)
+    s = load_scene(ctx, pos, idx, None, None, mm)
+    s.build(30, 1)
+    codes = s.codes().astype(np.int64)
+    perm = s.perm().astype(np.int64)
+    assert codes.size == T and perm.size == T
+    assert np.all(codes[1:] >= codes[:-1]), "codes not sorted"
+    seen = np.zeros(T, bool)
+    seen[perm] = True
+    assert seen.all(), "the permutation is not a bijection"
+    tie = codes[1:] == codes[:-1]
+    assert np.all(perm[1:][tie] > perm[:-1][tie]), "equal codes out of index order"
+    assert np.array_equal(s.indices().reshape(-1, 3), idx[perm]), "permuted index buffer"
+    assert np.array_equal(s.matmap(), mm[perm]), "permuted material map"
+    used = np.zeros(V, bool)
+    used[idx.reshape(-1)] = True
+    lo, hi = s.global_bbox()
+    assert np.array_equal(lo, pos[used].min(0)) and np.array_equal(hi, pos[used].max(0)), "global AABB"
+    n, _ = s.info()
+    assert 0 < n < 2 * T, f"node count {n}"
+    s.destroy()
+
+
 def test_lbvh_duplicates_and_flat_axis(ctx):
     # many identical triangles (equal codes never split) and a flat global axis
     pos, idx, nrm, uv, mm = random_scene(42, 3000, V=50, flat_axis=1)
