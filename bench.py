@@ -378,7 +378,7 @@ def main():
     flight_tune = [{"trace_block": k[0], "coop_leaf": k[1], "stack_lds": k[2], "ms_per_frame": round(v, 4)}
                    for k, v in tune_runs.items()]
     flights = {}
-    for depth in (2, 3, 4):
+    for depth in (2, 3, 4, 5, 6):
         flights[depth] = flight_ms(depth, args.steps)
         log(f"frame pipeline, {depth} frames in flight: {flights[depth]:.3f} ms/frame, "
             f"{rays_per_frame / flights[depth] / 1e3:.1f} Mrays/s")
@@ -656,19 +656,22 @@ def bench_c3(ctx, frames: int):
     # frames in flight: the frame pipeline animates and rebuilds on its build
     # stream; frame k is the sequential frame k (same clock), so the ray
     # counts are the sequential pass's
-    depth = 3
-    pl = rtx.Pipeline(ctx, s, sd.width, sd.height, sd.max_bounces, depth=depth)
-    pl.set_animation(True, 0.0, 1.0 / 60.0)
-    s.set_clock(0)
-    ctx.device_sync()
-    t0 = time.perf_counter()
-    last = 0
-    for _ in range(frames):
-        last = pl.frame(cam, rebuild=True)
-    pl.wait(last)
-    ctx.device_sync()
-    timing["in_flight"] = (time.perf_counter() - t0,) + timing["sequential"][1:]
-    pl.destroy()
+    flight = {}
+    for depth in (2, 3, 4, 5, 6):
+        pl = rtx.Pipeline(ctx, s, sd.width, sd.height, sd.max_bounces, depth=depth)
+        pl.set_animation(True, 0.0, 1.0 / 60.0)
+        s.set_clock(0)
+        ctx.device_sync()
+        t0 = time.perf_counter()
+        last = 0
+        for _ in range(frames):
+            last = pl.frame(cam, rebuild=True)
+        pl.wait(last)
+        ctx.device_sync()
+        flight[depth] = time.perf_counter() - t0
+        pl.destroy()
+    depth = min(flight, key=flight.get)
+    timing["in_flight"] = (flight[depth],) + timing["sequential"][1:]
     best = min(timing, key=lambda k: timing[k][0])
     dt, rays, shadow = timing[best]
     sdt = timing["sequential"][0]
@@ -682,6 +685,7 @@ def bench_c3(ctx, frames: int):
            "frames": frames, "ms_per_frame": round(dt / frames * 1e3, 4),
            "mrays": round(rays / dt / 1e6, 3), "mrays_incl_shadow": round((rays + shadow) / dt / 1e6, 3),
            "ms_per_frame_by_mode": {k: round(v[0] / frames * 1e3, 4) for k, v in timing.items()},
+           "in_flight_ms_per_frame_by_depth": {str(k): round(v / frames * 1e3, 4) for k, v in flight.items()},
            "sequential_ms_per_frame": round(sdt / frames * 1e3, 4),
            "sequential_mrays": round(timing["sequential"][1] / sdt / 1e6, 3),
            "rays_per_frame_mean": round(rays / frames, 1), "triangles": sd.T}

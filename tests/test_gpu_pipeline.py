@@ -85,7 +85,7 @@ def test_pipelined_dynamic_frames_match_sequential(ctx, graph):
     assert not np.array_equal(seq[0][0], seq[1][0])
 
 
-@pytest.mark.parametrize("depth", [1, 2, 3])
+@pytest.mark.parametrize("depth", [1, 2, 3, 6])
 def test_frames_in_flight_match_sequential_dynamic(ctx, depth):
     # the frame pipeline keeps `depth` frames in flight; every frame must be
     # the sequential run's (dynamic scene: each frame has its own geometry)
