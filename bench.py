@@ -378,7 +378,7 @@ def main():
     flight_tune = [{"trace_block": k[0], "coop_leaf": k[1], "stack_lds": k[2], "ms_per_frame": round(v, 4)}
                    for k, v in tune_runs.items()]
     flights = {}
-    for depth in (2, 3, 4, 5, 6):
+    for depth in (2, 3, 4, 5, 6, 7, 8):
         flights[depth] = flight_ms(depth, args.steps)
         log(f"frame pipeline, {depth} frames in flight: {flights[depth]:.3f} ms/frame, "
             f"{rays_per_frame / flights[depth] / 1e3:.1f} Mrays/s")
@@ -657,7 +657,7 @@ def bench_c3(ctx, frames: int):
     # stream; frame k is the sequential frame k (same clock), so the ray
     # counts are the sequential pass's
     flight = {}
-    for depth in (2, 3, 4, 5, 6):
+    for depth in (2, 3, 4, 5, 6, 8):
         pl = rtx.Pipeline(ctx, s, sd.width, sd.height, sd.max_bounces, depth=depth)
         pl.set_animation(True, 0.0, 1.0 / 60.0)
         s.set_clock(0)

@@ -651,7 +651,7 @@ int32_t rtx_server_poll(rtx_server* srv, rtx_completion* out, uint32_t cap, uint
 int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
 
 /* ------------------------------------------------------- frame pipeline -- */
-/* Frames of one scene with `depth` (1..6) frames in flight: the reference's
+/* Frames of one scene with `depth` (1..8) frames in flight: the reference's
  * RendererT (src/rt/renderer-threaded.cpp:452-506) overlaps the next build
  * with the last render; this goes further.  Frame N uses slot N mod depth:
  * its own renderer, stream, LBVH buffer set and copy of the normals and

@@ -21,7 +21,7 @@ struct rtx_pipeline {
 int32_t rtx_pipeline_create(rtx_ctx* ctx, rtx_scene* s, const rtx_render_params* p, int32_t depth,
                             rtx_pipeline** out) {
     RTX_REQUIRE(ctx && s && p && out, RTX_EINVAL, "rtx_pipeline_create: null argument");
-    RTX_REQUIRE(depth >= 1 && depth <= rtx_scene::COPIES, RTX_EINVAL, "rtx_pipeline_create: depth must be in [1, 6]");
+    RTX_REQUIRE(depth >= 1 && depth <= rtx_scene::COPIES, RTX_EINVAL, "rtx_pipeline_create: depth must be in [1, 8]");
     rtx_pipeline* pl = new rtx_pipeline();
     pl->ctx = ctx;
     pl->s = s;

@@ -171,7 +171,7 @@ struct rtx_scene {
     struct RenderCopy {
         DevBuf node, delta, tri, nrm, matmap;
     };
-    static constexpr int COPIES = 6;
+    static constexpr int COPIES = 8;
     RenderCopy rc[COPIES];
     int use_copy = -1;
     // Frames in flight: the launches being enqueued read the nodes, deltas

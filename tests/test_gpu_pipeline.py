@@ -85,7 +85,7 @@ def test_pipelined_dynamic_frames_match_sequential(ctx, graph):
     assert not np.array_equal(seq[0][0], seq[1][0])
 
 
-@pytest.mark.parametrize("depth", [1, 2, 3, 6])
+@pytest.mark.parametrize("depth", [1, 2, 3, 8])
 def test_frames_in_flight_match_sequential_dynamic(ctx, depth):
     # the frame pipeline keeps `depth` frames in flight; every frame must be
     # the sequential run's (dynamic scene: each frame has its own geometry)
@@ -95,7 +95,8 @@ def test_frames_in_flight_match_sequential_dynamic(ctx, depth):
     cam = scenes.camera_of(sd)
     s = scenes.load(rtx.Scene(ctx), sd)       # both runs start from the uploaded order
     s.set_waves(sd.waves)
-    seq = _frames(ctx, rtx, s, cam, sd.width, sd.height, sd.max_bounces, 6, pipelined=False, graph=False,
+    nf = max(6, depth + 2)                    # every slot is reused at least once
+    seq = _frames(ctx, rtx, s, cam, sd.width, sd.height, sd.max_bounces, nf, pipelined=False, graph=False,
                   animate=(0.0, 1.0 / 30.0), reset=lambda: s.set_clock(0))
     s.destroy()
     s = scenes.load(rtx.Scene(ctx), sd)
@@ -103,19 +104,19 @@ def test_frames_in_flight_match_sequential_dynamic(ctx, depth):
     pl = rtx.Pipeline(ctx, s, sd.width, sd.height, sd.max_bounces, depth=depth)
     pl.set_animation(True, 0.0, 1.0 / 30.0)
     got = {}
-    for k in range(6):
+    for k in range(nf):
         f = pl.frame(cam, rebuild=True)
         assert f == k
         if k >= depth - 1:          # the oldest frame in flight, before its slot is reused
             old = k - (depth - 1)
             pl.wait(old)
             got[old] = (pl.fb(old).accum(), pl.stats(old))
-    for old in range(6 - (depth - 1), 6):
+    for old in range(nf - (depth - 1), nf):
         pl.wait(old)
         got[old] = (pl.fb(old).accum(), pl.stats(old))
     pl.destroy()
     s.destroy()
-    for k in range(6):
+    for k in range(nf):
         a, sa = seq[k]
         b, sb = got[k]
         assert np.array_equal(a, b), f"frame {k} (depth {depth}) differs from the sequential frame"
