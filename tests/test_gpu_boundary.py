@@ -10,7 +10,9 @@
     copy still gives the first tree's hits and shadow flags (the oracle's on
     the unmoved geometry), the scene itself the moved geometry's;
   * the same through the C++ stage classes (tests/cpp/stage_tracer.cpp), with
-    BVHBuilder::set_log writing the build's stage times.
+    BVHBuilder::set_log writing the build's stage times;
+  * the autotuner's keys (rtx_tune_set / rtx_tune_get): documented ranges
+    accepted and read back, values outside them refused.
 """
 import subprocess
 from pathlib import Path
@@ -156,3 +158,45 @@ def test_stage_tracer_overloads_and_builder_log(tmp_path):
     want2 = O.trace(ob2, sd.nrm, sd.uv, oo, od)
     want2["flags"] = O.trace_shadow(ob2, lights, want2["flags"], want2["p4"])
     assert np.array_equal(rd["moved_id"], want2["id"]) and np.array_equal(rd["moved_flags"], want2["flags"])
+
+
+TUNE_KEYS = {  # key: (values accepted, values refused)
+    "trace_variant": ([0, 4], [-1, 5]),
+    "trace_block": ([64, 128, 256], [32, 512]),
+    "xcds": ([1, 64], [0, 65]),
+    "refill": ([1, 64], [0, 65]),
+    "coop_leaf": ([0, 1 << 20], [-1, (1 << 20) + 1]),
+    "side_stream": ([0, 1], [2]),
+    "inst_lane_roots": ([0, 2], [3]),
+    "wf_sched": ([0, 1], [2]),
+    "wf_grain": ([1, 64], [0, 65]),
+    "wf_share": ([0, 1, 100], [-1, 101]),
+    "sort_bits": ([4, 8], [6, 16]),
+    "sort_items": ([16, 32], [8, 64]),
+    "build_grid": ([1, 16], [0, 17]),
+    "shadow_order": ([0, 1], [2]),
+    "stack_lds": ([8, 16, 32], [4, 24, 64]),
+    "srv_step_pixels": ([0, 2_500_000], [-1]),
+}
+
+
+def test_tune_keys_round_trip_and_refuse_bad_values(ctx):
+    # every autotuner key takes the values of its documented range, reads them
+    # back, and refuses values outside it without changing the setting
+    import rtx
+    for key, (good, bad) in TUNE_KEYS.items():
+        old = ctx.tuned(key)
+        assert old >= 0, f"{key}: unknown to rtx_tune_get"
+        try:
+            for v in good:
+                ctx.tune(key, v)
+                assert ctx.tuned(key) == v, key
+            for v in bad:
+                with pytest.raises(rtx.RtxError, match="rtx_tune_set"):
+                    ctx.tune(key, v)
+                assert ctx.tuned(key) == good[-1], f"{key}: a refused value changed the setting"
+        finally:
+            ctx.tune(key, old)
+    with pytest.raises(rtx.RtxError, match="unknown key"):
+        ctx.tune("no_such_key", 1)
+    assert ctx.tuned("no_such_key") == -1
