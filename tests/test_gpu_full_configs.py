@@ -1,10 +1,10 @@
-"""GPU parity at the configurations' real sizes (BASELINE's C2-C5), beside
+"""GPU parity at the benchmark workloads' real sizes (C2-C5 below), beside
 the smaller cases of the other GPU tests.
 
   C2: the headline frame -- 200 spheres (1,000,000 triangles), 1920x1080,
       2 point lights, Whitted depth 3, LBVH rebuilt every frame -- through
-      the plain path and through frames in flight (3 at once, the bench's
-      headline mode), every frame's accumulators bit-exact against the
+      the plain path and through frames in flight (3 at once, one of the
+      depths the bench sweeps), every frame's accumulators bit-exact against the
       oracle's render of the same frame.
   C3: two animated frames of the 724x724 grid (1,045,458 triangles),
       1280x720, depth 2, rebuilt every frame, against the oracle on the
