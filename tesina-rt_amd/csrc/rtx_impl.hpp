@@ -294,6 +294,7 @@ struct RootBox {
 // `table` is the host image of the device root table; `shipped` is what the
 // device holds (rewritten chunk by chunk, in stream order, when they differ).
 struct rtx_instances {
+    bool deep = false;  // some instance's tree has 32 levels (set by each root refresh)
     rtx_ctx* ctx = nullptr;
     int n = 0;
     std::vector<rtx_scene*> scene;

@@ -1408,7 +1408,8 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
                                                      const uint32_t* __restrict__ list_count,
                                                      uint32_t* __restrict__ flags, const float4* __restrict__ p4,
                                                      const uint32_t* __restrict__ hit_root,
-                                                     unsigned long long* shadow_count, uint32_t coop) {
+                                                     unsigned long long* shadow_count, uint32_t coop,
+                                                     uint32_t ordered) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
     LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
@@ -1455,7 +1456,8 @@ __global__ __launch_bounds__(TB) void k_shadow_inst(const InstRoot* __restrict__
             if (!ballot(go)) break;
             RayC rc{};
             if (go) rc = make_ray(mul_h(R.tr_inv, p, 1.f, 0.00001f), mul_h(R.tr_inv, dir, 0.f, 0.00001f), 0.01f);
-            if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go)) shadowed = true;  // whole wave
+            if (any_var<VAR>(R.node, R.delta, R.tri, rc, tmax, stk, coop, go, nullptr, 0, ordered != 0))
+                shadowed = true;  // whole wave
         }
         if (valid) set_shadow_bit(&flags[rr], l, shadowed);
     }
@@ -1696,7 +1698,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                                                         uint32_t* __restrict__ flags, const float4* __restrict__ p4,
                                                         const uint32_t* __restrict__ hit_root,
                                                         unsigned long long* shadow_count, uint32_t coop,
-                                                        const RootBox* __restrict__ wbox) {
+                                                        const RootBox* __restrict__ wbox, uint32_t ordered) {
     __shared__ uint32_t s_id[STACK * TB];
     __shared__ uint32_t s_cnt;
     LdsStack<TB> stk{s_id, threadIdx.x, nullptr};
@@ -1782,7 +1784,7 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                 if (going) {
                     const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
                     float4 dl4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (SPEC) {
+                    if (SPEC && ordered) {
                         dl4 = delta[curr];
                         __asm__ volatile("" : "+v"(dl4.x), "+v"(dl4.y), "+v"(dl4.z));  // see traverse_closest_node
                     }
@@ -1799,8 +1801,13 @@ __global__ __launch_bounds__(TB) void k_shadow_inst_lr(const InstRoot* __restric
                                 pop = false;
                             }
                         } else {
-                            if (!SPEC) dl4 = delta[curr];
-                            const bool nl = dot(rc.d, xyz(dl4)) > 0.f;
+                            // index order (left child first, no delta fetched)
+                            // unless a root's tree can reach MAX_LEVELS
+                            bool nl = true;
+                            if (ordered) {
+                                if (!SPEC) dl4 = delta[curr];
+                                nl = dot(rc.d, xyz(dl4)) > 0.f;
+                            }
                             stk.put(level, nl ? a + 1 : a);
                             level++;
                             curr = nl ? a : a + 1;
@@ -2753,6 +2760,7 @@ static int32_t refresh_roots(rtx_instances* in, hipStream_t st, int* var) {
         std::copy(&in->tr_inv[16 * (size_t)i], &in->tr_inv[16 * (size_t)i] + 16, R.tr_inv);
         deep = deep || s->depth >= STACK;
     }
+    in->deep = deep;
     // the sibling-pair form cannot follow the MAX_LEVELS rule (32-level
     // trees) and the persistent form has no multi-root kernel: both use the
     // single-node form with its early delta load
@@ -2859,10 +2867,11 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
         if (lane_roots) {                                                                                           \
             hipLaunchKernelGGL((chunks ? (V == 1 ? k_shadow_inst_lr<true, true, B> : k_shadow_inst_lr<false, true, B>)\
                                       : (V == 1 ? k_shadow_inst_lr<true, false, B> : k_shadow_inst_lr<false, false, B>)),\
-                               dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st, SHADOW_INST_ARGS, boxes);\
+                               dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st, SHADOW_INST_ARGS, boxes,  \
+                               ordered);                                                                            \
         } else {                                                                                                    \
             hipLaunchKernelGGL((k_shadow_inst<V, B>), dim3(trace_grid(in->ctx, items, items, B)), dim3(B), 0, st,   \
-                               SHADOW_INST_ARGS);                                                                   \
+                               SHADOW_INST_ARGS, ordered);                                                          \
         }                                                                                                           \
     } while (0)
 #define SHADOW_INST_ARGS                                                                                            \
@@ -2871,6 +2880,9 @@ int32_t rtx_trace_shadow_instances_impl(rtx_instances* in, rtx_rays* rays, rtx_h
         coop_threshold(in->ctx)
     const RootBox* boxes = root_boxes(in, lane_roots, st);
     const bool chunks = boxes != nullptr && in->n > 64;  // culling over chunks of 64 roots
+    // shadow rays in index order unless a root's tree has 32 levels (an OR
+    // over the roots of order-free any-hit walks)
+    const uint32_t ordered = (in->ctx->tune.shadow_order != 0 || in->deep) ? 1u : 0u;
     RTX_DISPATCH_TRAVERSAL(RTX_SHADOW_INST, var, tb);
 #undef SHADOW_INST_ARGS
 #undef RTX_SHADOW_INST

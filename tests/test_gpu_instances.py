@@ -81,27 +81,33 @@ def _check_all_forms(ctx, inst, r, hb, n, o, what):
     """Every traversal form; the single-node forms (0, 1 and 3, which maps to
     1) under the three root schedules: each lane on its own roots in one loop,
     over the roots whose world box its ray meets (2) or over all of them (1),
-    and the wave-uniform root loop (0)."""
+    and the wave-uniform root loop (0); shadow rays of the single-node forms
+    in index order (0) and in the reference's child order (1)."""
     old, old_lr, old_coop = ctx.tuned("trace_variant"), ctx.tuned("inst_lane_roots"), ctx.tuned("coop_leaf")
+    old_so = ctx.tuned("shadow_order")
     try:
         for variant in (0, 1, 2, 3):
             for lane_roots in ((2, 1, 0) if variant != 2 else (2,)):
                 # coop_leaf 1: every leaf tested by the whole wave (with the
                 # owner lane's triangle pointer in the lane-roots kernels)
                 for coop in ((old_coop, 1) if variant == 1 else (old_coop,)):
-                    ctx.tune("trace_variant", variant)
-                    ctx.tune("inst_lane_roots", lane_roots)
-                    ctx.tune("coop_leaf", coop)
-                    inst.trace(r, hb, n)
-                    inst.trace_shadow(r, hb, n)
-                    g = hb.read(n)
-                    for k in FIELDS:
-                        assert np.array_equal(g[k], o[k]), \
-                            f"{what}: {k} differs (variant {variant}, lane roots {lane_roots}, coop leaf {coop})"
+                    for so in ((0, 1) if variant in (0, 1) else (0,)):
+                        ctx.tune("trace_variant", variant)
+                        ctx.tune("inst_lane_roots", lane_roots)
+                        ctx.tune("coop_leaf", coop)
+                        ctx.tune("shadow_order", so)
+                        inst.trace(r, hb, n)
+                        inst.trace_shadow(r, hb, n)
+                        g = hb.read(n)
+                        for k in FIELDS:
+                            assert np.array_equal(g[k], o[k]), \
+                                f"{what}: {k} differs (variant {variant}, lane roots {lane_roots}, coop leaf " \
+                                f"{coop}, shadow order {so})"
     finally:
         ctx.tune("trace_variant", old)
         ctx.tune("inst_lane_roots", old_lr)
         ctx.tune("coop_leaf", old_coop)
+        ctx.tune("shadow_order", old_so)
 
 
 @pytest.mark.parametrize("case", list(CASES))
