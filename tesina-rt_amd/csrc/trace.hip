@@ -613,16 +613,19 @@ __device__ void traverse_closest_ns(const float4* __restrict__ node, const float
     }
 }
 
+// ordered = false: children in index order with no delta fetched.  The walk
+// has no stack, hence no MAX_LEVELS: its any-hit result is whether a leaf
+// whose box and ancestors' boxes pass holds a hit, whatever the order.
 __device__ bool traverse_any_ns(const float4* __restrict__ node, const float4* __restrict__ delta,
                                 const uint32_t* __restrict__ parent, const float4* __restrict__ tri, const RayC& r,
-                                float tmax, uint32_t bound) {
+                                float tmax, uint32_t bound, bool ordered = true) {
     bool up = false;
     uint32_t last = 0, curr = 0, first = 0, second = 0;
     for (uint32_t it = 0; it < bound; ++it) {
         const float4 f0 = node[2 * (size_t)curr], f1 = node[2 * (size_t)curr + 1];
         const uint32_t a = __float_as_uint(f0.w), cnt = __float_as_uint(f1.w);
         if (!cnt) {
-            const bool nl = dot(r.d, xyz(delta[curr])) > 0.f;
+            const bool nl = !ordered || dot(r.d, xyz(delta[curr])) > 0.f;
             first = nl ? a : a + 1;
             second = nl ? a + 1 : a;
         }
@@ -682,7 +685,7 @@ __device__ __forceinline__ bool any_var(const float4* __restrict__ node, const f
                                         uint32_t bound = 0, bool ordered = true) {
     if (VAR == 4) {
         bool h = false;
-        if (start) h = traverse_any_ns(node, delta, parent, tri, r, tmax, bound);
+        if (start) h = traverse_any_ns(node, delta, parent, tri, r, tmax, bound, ordered);
         return h;
     }
     if (VAR == 2) {
@@ -2519,8 +2522,9 @@ int32_t rtx_trace_shadow_multi(rtx_scene* s, rtx_rays* const* rays, rtx_hits* co
                        (unsigned long long*)dev_shadow_count, (uint32_t)s->ctx->tune.xcds, \
                        coop_threshold(s->ctx), sv.parent, stackless_bound(s), ordered)
     // the reference's child order only where a ray's stack can reach MAX_LEVELS
+    // (the stackless walk has none)
     const int depth = s->ext ? s->ext->depth : s->depth;
-    const uint32_t ordered = (s->ctx->tune.shadow_order != 0 || depth >= STACK) ? 1u : 0u;
+    const uint32_t ordered = (s->ctx->tune.shadow_order != 0 || (depth >= STACK && var != 4)) ? 1u : 0u;
     RTX_DISPATCH_STACK_NS(RTX_SHADOW_LAUNCH, var, tb, (int)s->ctx->tune.stack_lds);
 #undef RTX_SHADOW_LAUNCH
     RTX_CHECK_HIP(hipGetLastError());
