@@ -664,8 +664,9 @@ int32_t rtx_server_drain(rtx_server* srv, int64_t timeout_ms);
  * consecutive frames overlap.  Each frame's image equals the sequential
  * run's.  The scene must not be driven by another renderer while the
  * pipeline runs; afterwards its current buffer set holds the last frame's
- * build.  Each slot renders on one stream; concurrency needs a hardware queue
- * per stream (GPU_MAX_HW_QUEUES >= depth + 2; 16 is a good setting). */
+ * build.  Each slot renders on one stream (depth + 1 streams with the build
+ * stream); which depth is fastest depends on how those streams share the
+ * process's hardware queues (GPU_MAX_HW_QUEUES), so time a few depths. */
 int32_t rtx_pipeline_create(rtx_ctx* ctx, rtx_scene* s, const rtx_render_params* p, int32_t depth,
                             rtx_pipeline** out);
 int32_t rtx_pipeline_destroy(rtx_pipeline* pl);
